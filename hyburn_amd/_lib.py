@@ -1,0 +1,163 @@
+"""ctypes bindings for libhyburn_hip.so (include/hyburn_hip.h).
+
+The library is loaded from this package directory only; importing the
+bindings when it has not been built raises immediately. There is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhyburn_hip.so")
+
+# ---- constants (mirror include/hyburn_hip.h) ----
+HB_OK, HB_INVALID_CONFIG, HB_INVALID_STATE, HB_BACKEND_ERROR, HB_IO_ERROR = 0, 1, 2, 3, 4
+G2DO, MPR, JANSEN_RIT, WILSON_COWAN, KURAMOTO, RWW = 0, 1, 2, 3, 4, 5
+EULER, HEUN, RK4, EULER_STOCH, HEUN_STOCH, RK4_STOCH = 0, 1, 2, 3, 4, 5
+CF_LINEAR, CF_DIFFERENCE, CF_KURAMOTO, CF_SIGMOIDAL, CF_SCALED_LINEAR, CF_TANH, CF_SJR, CF_PRESIGMOIDAL = range(8)
+W_NONE, W_SCALAR_MEAN, W_SCALAR_DIAG, W_DENSE, W_CSR = 0, 1, 2, 3, 4
+DELAY_PER_EDGE, DELAY_SINGLE, DELAY_BATCH_COMPAT = 0, 1, 2
+ENGINE_BATCH, ENGINE_PLAIN = 0, 1
+FLAG_TRAJECTORY, FLAG_EXACT, FLAG_NO_GRAPH = 0x1, 0x2, 0x4
+
+MODEL_NAMES = {
+    "Generic2dOscillator": G2DO, "MontbrioPazoRoxin": MPR, "JansenRit": JANSEN_RIT,
+    "WilsonCowan": WILSON_COWAN, "Kuramoto": KURAMOTO, "ReducedWongWang": RWW,
+}
+# (nvar, ncvar, nparams), config.rs:43-50
+MODEL_INFO = {G2DO: (2, 1, 12), MPR: (2, 2, 7), JANSEN_RIT: (6, 1, 13),
+              WILSON_COWAN: (2, 1, 22), KURAMOTO: (1, 1, 1), RWW: (1, 1, 8)}
+# the batch-native sweep index per model, dfun.rs:49-72
+SWEEP_INDEX = {G2DO: 1, MPR: 4, JANSEN_RIT: 12, WILSON_COWAN: 18, KURAMOTO: 0, RWW: 7}
+COUPLING_NAMES = {
+    "Linear": CF_LINEAR, "Difference": CF_DIFFERENCE, "Kuramoto": CF_KURAMOTO,
+    "Sigmoidal": CF_SIGMOIDAL, "ScaledLinear": CF_SCALED_LINEAR, "HyperbolicTangent": CF_TANH,
+    "SigmoidalJansenRit": CF_SJR, "PreSigmoidal": CF_PRESIGMOIDAL,
+}
+INTEGRATOR_NAMES = {
+    "euler": EULER, "heun": HEUN, "rk4": RK4, "euler_stochastic": EULER_STOCH,
+    "heun_stochastic": HEUN_STOCH, "rk4_stochastic": RK4_STOCH,
+}
+
+fp = C.POINTER(C.c_float)
+ip = C.POINTER(C.c_int32)
+
+
+class HbInstance(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32), ("integrator", C.c_int32), ("engine", C.c_int32),
+        ("n_nodes", C.c_int32), ("n_steps", C.c_int32),
+        ("dt", C.c_float), ("dt_ms", C.c_double),
+        ("params", fp), ("sweep_form", C.c_int32), ("sweep_val", C.c_float),
+        ("node_param_idx", C.c_int32), ("node_param", fp),
+        ("x0", fp), ("nsig", fp), ("noise", fp),
+        ("noise_seed", C.c_uint64), ("noise_stream", C.c_uint64),
+        ("wkind", C.c_int32), ("cfn", C.c_int32), ("n_cparams", C.c_int32),
+        ("cparams", C.c_float * 8), ("scalar_w", C.c_float),
+        ("dense", fp), ("indptr", ip), ("indices", ip), ("weights", fp), ("edge_delays", ip),
+        ("nnz", C.c_int32), ("delay_mode", C.c_int32), ("single_delay", C.c_int32),
+        ("max_delay_hint", C.c_int32),
+        ("bold_period", C.c_int32), ("bold_tr_ms", C.c_double),
+        ("tavg", fp), ("final_state", fp), ("traj", fp), ("bold", fp),
+        ("bold_cap", C.c_int32), ("bold_n", ip),
+    ]
+
+
+class HbRunOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("flags", C.c_uint32), ("block_size", C.c_int32),
+                ("grid_waves", C.c_int32)]
+
+
+class HbRunStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double), ("node_steps", C.c_double),
+                ("n_launches", C.c_int32), ("block_size", C.c_int32), ("lds_bytes", C.c_int32)]
+
+
+class HbSubnetwork(C.Structure):
+    _fields_ = [("model", C.c_int32), ("n_nodes", C.c_int32), ("n_params", C.c_int32),
+                ("params", fp), ("x0", fp)]
+
+
+class HbProjection(C.Structure):
+    _fields_ = [("wkind", C.c_int32), ("scalar_w", C.c_float), ("dense", fp),
+                ("indptr", ip), ("indices", ip), ("data", fp), ("nnz", C.c_int32),
+                ("delays", ip), ("n_delays", C.c_int32), ("cfn", C.c_int32),
+                ("n_cparams", C.c_int32), ("cparams", C.c_float * 8)]
+
+
+class HbSimConfig(C.Structure):
+    _fields_ = [("sim_length", C.c_double), ("dt", C.c_double), ("integrator", C.c_int32),
+                ("n_nsig", C.c_int32), ("nsig", fp),
+                ("subnetworks", C.POINTER(HbSubnetwork)), ("n_subnetworks", C.c_int32),
+                ("projections", C.POINTER(HbProjection)), ("n_projections", C.c_int32),
+                ("bold_period", C.c_int32), ("bold_tr_s", C.c_double),
+                ("engine", C.c_int32), ("delay_mode", C.c_int32),
+                ("noise_seed", C.c_uint64), ("noise_shared", C.c_int32)]
+
+
+class HbSweepPoint(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("a", C.c_int32), ("b", C.c_int32),
+                ("values", fp), ("n_values", C.c_int32)]
+
+
+class HbResult(C.Structure):
+    _fields_ = [("n_sweep", C.c_int32), ("n_nodes", C.c_int32), ("nvar", C.c_int32),
+                ("n_steps", C.c_int32), ("tavg", fp), ("final_states", fp),
+                ("trajectories", fp), ("bold", fp), ("n_bold", C.c_int32),
+                ("elapsed_ms", C.c_double), ("kernel_ms", C.c_double)]
+
+
+# Every exported function of include/hyburn_hip.h with its prototype.
+PROTOTYPES = {
+    "hb_engine_create": (C.c_int, [C.POINTER(HbSimConfig), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "hb_engine_destroy": (None, [C.c_void_p]),
+    "hb_run_sweep": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, fp, C.c_int32, C.c_int64, C.c_uint32,
+                               C.POINTER(HbResult)]),
+    "hb_run_sweep_multi": (C.c_int, [C.c_void_p, C.POINTER(HbSweepPoint), C.c_int32, C.c_int64, C.c_uint32,
+                                     C.POINTER(HbResult)]),
+    "hb_result_free": (None, [C.POINTER(HbResult)]),
+    "hb_plan_create": (C.c_int, [C.POINTER(HbInstance), C.c_int32, C.POINTER(HbRunOpts), C.POINTER(C.c_void_p)]),
+    "hb_plan_run": (C.c_int, [C.c_void_p, C.POINTER(HbRunStats)]),
+    "hb_plan_download": (C.c_int, [C.c_void_p]),
+    "hb_plan_destroy": (None, [C.c_void_p]),
+    "hb_run_instances": (C.c_int, [C.POINTER(HbInstance), C.c_int32, C.POINTER(HbRunOpts), C.POINTER(HbRunStats)]),
+    "hb_philox_normal": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, fp, C.c_int32]),
+    "hb_last_error": (C.c_char_p, []),
+    "hb_version": (C.c_char_p, []),
+    "hb_device_count": (C.c_int, []),
+    "hb_select_backend": (C.c_char_p, [C.c_char_p]),
+}
+
+_lib = None
+
+
+class HyburnError(RuntimeError):
+    """A non-zero status from the C ABI (codes mirror src/error.rs:7-23)."""
+
+    NAMES = {1: "InvalidConfig", 2: "InvalidState", 3: "BackendError", 4: "IOError"}
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{self.NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib() -> C.CDLL:
+    """Load libhyburn_hip.so from this package (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run hyburn_amd.build.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in PROTOTYPES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != HB_OK:
+        raise HyburnError(rc, lib().hb_last_error().decode(errors="replace"))

@@ -1,0 +1,77 @@
+"""In-tree build of libhyburn_hip.so for gfx950 (hipcc, no JIT cache).
+
+The forward kernel translation unit is compiled twice: with FMA contraction
+(the default product path) and with ``-ffp-contract=off`` (the exact path,
+selected per run with HB_FLAG_EXACT).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libhyburn_hip.so")
+
+ARCH = os.environ.get("HB_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+
+# Packed-FP32 instructions are disabled for all device code.
+DEVICE_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
+
+UNITS = [
+    # (source, object, extra flags, device?)
+    ("hb_forward.hip", "hb_forward_fast.o", ["-DHB_EXACT=0"], True),
+    ("hb_forward.hip", "hb_forward_exact.o", ["-DHB_EXACT=1", "-ffp-contract=off"], True),
+    ("hb_noise.hip", "hb_noise.o", [], True),
+    ("hb_adjoint.hip", "hb_adjoint.o", [], True),
+    ("hb_host.cpp", "hb_host.o", [], False),
+    ("hb_engine.cpp", "hb_engine.o", [], False),
+    ("hb_adjoint_host.cpp", "hb_adjoint_host.o", [], False),
+]
+
+
+def _compile(src: str, obj: str, extra: list[str], device: bool) -> tuple[str, int, str]:
+    cmd = [HIPCC] + (DEVICE_FLAGS if device else HOST_FLAGS) + extra + [
+        "-c", os.path.join(CSRC, src), "-o", os.path.join(BUILD, obj)]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    # the host pass does not know the packed-fp32 feature; that warning is harmless
+    err = "\n".join(l for l in (p.stdout + p.stderr).splitlines() if "packed-fp32-ops" not in l)
+    return obj, p.returncode, err
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile every unit for gfx950 and link libhyburn_hip.so in-tree."""
+    os.makedirs(BUILD, exist_ok=True)
+    units = [u for u in UNITS if os.path.exists(os.path.join(CSRC, u[0]))]
+    jobs = jobs or min(8, len(units))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(lambda u: _compile(*u), units))
+    failed = [r for r in results if r[1] != 0]
+    for obj, rc, err in results:
+        if err.strip() and (verbose or rc != 0):
+            print(f"[build] {obj}:\n{err}", file=sys.stderr)
+    if failed:
+        raise RuntimeError("hipcc failed for: " + ", ".join(r[0] for r in failed))
+    objs = [os.path.join(BUILD, u[1]) for u in units]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

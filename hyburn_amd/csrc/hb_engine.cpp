@@ -1,0 +1,366 @@
+// hb_engine.cpp -- BatchHybridEngine / HybridEngine mirror over the instance
+// planner: config copy + validation, sweep expansion (single, Cartesian),
+// batched result layout.
+//
+// Reference (maedoc/hyburn):
+//   BatchHybridEngine::from_config    src/engine/batch_engine/engine.rs:145-471
+//   run_sweep / _with_trajectory      engine.rs:477-494, 496-842
+//   run_sweep_multi, Cartesian order  engine.rs:56-91, 845-1348
+//   SimConfig::validate               src/config.rs:286-428
+//   HybridEngine weights/delays       src/engine/construction.rs:543-790
+#include "hb_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace hb;
+
+struct hb_engine {
+    int device = 0;
+    int n_sweep = 0;
+    double sim_length = 0.0, dt = 0.0;
+    int integrator = HB_HEUN;
+    int model = 0, n_nodes = 0, nv = 0;
+    std::vector<float> params, x0, nsig;
+    bool has_proj = false;
+    int wkind_cfg = HB_W_NONE;   // HB_W_SCALAR_MEAN for "scalar", DENSE, CSR
+    float scalar_w = 0.0f;
+    std::vector<float> dense, data;
+    std::vector<int32_t> indptr, indices, delays;
+    int cfn = HB_CF_LINEAR;
+    std::vector<float> cparams;
+    int bold_period = 0;
+    double bold_tr_s = 2.0;
+    int engine = HB_ENGINE_BATCH;
+    int delay_mode = -1;
+    uint64_t noise_seed = 0;
+    int noise_shared = 1;
+};
+
+namespace {
+
+int efail(int code, const std::string& m) { return hb::set_error(code, m); }
+
+// Build the per-point instances of a sweep. Arrays referenced by the instances
+// live in `keep` (per-point parameter / coupling copies) or in the engine.
+struct Built {
+    std::vector<hb_instance> inst;
+    std::vector<std::vector<float>> params;
+};
+
+hb_instance base_instance(const hb_engine* E, int64_t n_steps) {
+    hb_instance I;
+    memset(&I, 0, sizeof(I));
+    I.model = E->model;
+    I.integrator = E->integrator;
+    I.engine = E->engine;
+    I.n_nodes = E->n_nodes;
+    I.n_steps = (int32_t)n_steps;
+    I.dt = (float)E->dt;           // engine.rs:456 (dt as f32)
+    I.dt_ms = E->dt;
+    I.params = E->params.data();
+    I.node_param_idx = -1;
+    I.x0 = E->x0.data();
+    I.nsig = E->nsig.data();
+    I.noise_seed = E->noise_seed;
+    I.cfn = E->cfn;
+    I.n_cparams = (int32_t)std::min<size_t>(E->cparams.size(), 8);
+    for (int k = 0; k < I.n_cparams; ++k) I.cparams[k] = E->cparams[k];
+    if (E->has_proj) {
+        const int ndl = (int)E->delays.size();
+        const int d0 = ndl > 0 ? E->delays[0] : 0;
+        switch (E->wkind_cfg) {
+        case HB_W_SCALAR_MEAN:
+            // scalar weights: mean field in the batch engine (engine.rs:602-608),
+            // s*I in the plain engine (construction.rs:639-655)
+            I.wkind = E->engine == HB_ENGINE_PLAIN ? HB_W_SCALAR_DIAG : HB_W_SCALAR_MEAN;
+            I.scalar_w = E->scalar_w;
+            break;
+        case HB_W_DENSE:
+            I.wkind = HB_W_DENSE;
+            I.dense = E->dense.data();
+            break;
+        case HB_W_CSR:
+            I.wkind = HB_W_CSR;
+            I.indptr = E->indptr.data();
+            I.indices = E->indices.data();
+            I.weights = E->data.data();
+            I.nnz = (int32_t)E->data.size();
+            break;
+        }
+        int mode = E->delay_mode;
+        if (mode < 0) {
+            if (E->engine == HB_ENGINE_PLAIN)
+                mode = (I.wkind == HB_W_CSR && ndl == I.nnz && ndl > 0) ? HB_DELAY_PER_EDGE : HB_DELAY_SINGLE;
+            else
+                mode = HB_DELAY_BATCH_COMPAT;   // engine.rs:410 + 541-589
+        }
+        I.delay_mode = mode;
+        I.single_delay = d0;
+        if (ndl > 0 && ndl == I.nnz && I.wkind == HB_W_CSR) I.edge_delays = E->delays.data();
+        // window horizon = max(delays) + 1 although only delays[0] is read
+        // (engine.rs:253-270, 410)
+        for (int32_t d : E->delays) I.max_delay_hint = std::max(I.max_delay_hint, d);
+        if (I.wkind != HB_W_CSR && mode == HB_DELAY_PER_EDGE) I.delay_mode = HB_DELAY_SINGLE;
+    }
+    I.bold_period = E->bold_period;
+    I.bold_tr_ms = E->bold_tr_s * 1000.0;   // bold_monitor.rs:131 (tr * 1000)
+    return I;
+}
+
+void fill_result(const hb_engine* E, int n_sweep, int64_t n_steps, bool traj, hb_result* out) {
+    memset(out, 0, sizeof(*out));
+    out->n_sweep = n_sweep;
+    out->n_nodes = E->n_nodes;
+    out->nvar = E->nv;
+    out->n_steps = (int32_t)n_steps;
+    const size_t st = (size_t)n_sweep * E->n_nodes * E->nv;
+    out->tavg = (float*)calloc(st ? st : 1, 4);
+    out->final_states = (float*)calloc(st ? st : 1, 4);
+    if (traj) out->trajectories = (float*)calloc(st * (size_t)n_steps + 1, 4);
+}
+
+int run_built(hb_engine* E, Built& B, int64_t n_steps, uint32_t flags, bool with_bold, hb_result* out) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const int n = (int)B.inst.size();
+    const bool traj = (flags & HB_FLAG_TRAJECTORY) != 0;
+    fill_result(E, n, n_steps, traj, out);
+    const size_t per = (size_t)E->n_nodes * E->nv;
+    std::vector<float> traj_tmp;
+    if (traj) traj_tmp.resize(per * (size_t)n_steps * n + 1);
+    int bold_rows = 0;
+    std::vector<int32_t> bold_n(n, 0);
+    if (with_bold && E->bold_period > 0) {
+        // rows per instance, same count for every instance of a sweep
+        const double dtb = E->dt * (double)E->bold_period;
+        const double tr_ms = E->bold_tr_s * 1000.0;
+        const int P = E->bold_period;
+        const int n_full = (int)(n_steps / P), rem = (int)(n_steps % P);
+        const int n_native = n_full + (rem > 0);
+        if (tr_ms <= 0.0 || std::fabs(tr_ms - dtb) < 1e-6) bold_rows = n_native;
+        else if (n_native > 0) {
+            const double max_t = rem > 0 ? (double)n_full * dtb + E->dt * (double)rem : (double)(n_full - 1) * dtb + dtb;
+            bold_rows = max_t >= tr_ms ? (int)std::floor(max_t / tr_ms) : 0;
+        }
+        out->n_bold = bold_rows;
+        out->bold = (float*)calloc((size_t)n * bold_rows * E->n_nodes + 1, 4);
+    }
+    for (int i = 0; i < n; ++i) {
+        hb_instance& I = B.inst[i];
+        I.tavg = out->tavg + (size_t)i * per;
+        I.final_state = out->final_states + (size_t)i * per;
+        I.traj = traj ? traj_tmp.data() + (size_t)i * per * (size_t)n_steps : nullptr;
+        if (out->bold) {
+            I.bold = out->bold + (size_t)i * bold_rows * E->n_nodes;
+            I.bold_cap = bold_rows;
+            I.bold_n = &bold_n[i];
+        } else {
+            I.bold_period = 0;
+        }
+    }
+    hb_run_opts o;
+    memset(&o, 0, sizeof(o));
+    o.device = E->device;
+    o.flags = flags;
+    hb_run_stats st;
+    memset(&st, 0, sizeof(st));
+    hb_plan* P = nullptr;
+    int rc = hb_plan_create(B.inst.data(), n, &o, &P);
+    if (!rc) rc = hb_plan_run(P, &st);
+    if (!rc) rc = hb_plan_download(P);
+    hb_plan_destroy(P);
+    if (rc) {
+        hb_result_free(out);
+        return rc;
+    }
+    if (traj) {   // [B][T][N][V] -> [T][B][N][V] (engine.rs:778-782 appends per step)
+        for (int i = 0; i < n; ++i)
+            for (int64_t t = 0; t < n_steps; ++t)
+                memcpy(out->trajectories + ((size_t)t * n + i) * per, traj_tmp.data() + ((size_t)i * n_steps + t) * per, per * 4);
+    }
+    out->kernel_ms = st.kernel_ms;
+    out->elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return HB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hb_engine_create(const hb_sim_config* cfg, int32_t n_sweep, int32_t device, hb_engine** out) {
+    if (!out) return HB_INVALID_CONFIG;
+    *out = nullptr;
+    if (!cfg) return HB_INVALID_CONFIG;
+    // SimConfig::validate subset, config.rs:286-428
+    if (!(cfg->dt > 0.0)) return efail(HB_INVALID_CONFIG, "dt must be positive");
+    if (!(cfg->sim_length > 0.0)) return efail(HB_INVALID_CONFIG, "sim_length must be positive");
+    if (cfg->n_subnetworks < 1 || !cfg->subnetworks) return efail(HB_INVALID_CONFIG, "At least one subnetwork required");
+    if (cfg->n_subnetworks > 1) return efail(HB_INVALID_CONFIG, "multiple subnetworks are not supported by this backend");
+    if (n_sweep < 1) return efail(HB_INVALID_CONFIG, "n_sweep must be >= 1");
+    const hb_subnetwork& S = cfg->subnetworks[0];
+    if (S.model < 0 || S.model > HB_RWW) return efail(HB_INVALID_CONFIG, "Unknown model");
+    if (S.n_params != model_nparams(S.model) || !S.params)
+        return efail(HB_INVALID_CONFIG, "model parameter count mismatch");
+    if (S.n_nodes < 1 || !S.x0) return efail(HB_INVALID_CONFIG, "subnetwork needs nodes and an initial state");
+    if (cfg->integrator < HB_EULER || cfg->integrator > HB_RK4_STOCH) return efail(HB_INVALID_CONFIG, "unknown integrator");
+    hb_engine* E = new hb_engine();
+    E->device = device;
+    E->n_sweep = n_sweep;
+    E->sim_length = cfg->sim_length;
+    E->dt = cfg->dt;
+    E->integrator = cfg->integrator;
+    E->model = S.model;
+    E->n_nodes = S.n_nodes;
+    E->nv = model_nvar(S.model);
+    E->params.assign(S.params, S.params + S.n_params);
+    E->x0.assign(S.x0, S.x0 + (size_t)S.n_nodes * E->nv);
+    E->nsig.assign(E->nv, 0.0f);
+    if (cfg->n_nsig == 1 && cfg->nsig) E->nsig.assign(E->nv, cfg->nsig[0]);
+    else if (cfg->n_nsig == E->nv && cfg->nsig) E->nsig.assign(cfg->nsig, cfg->nsig + E->nv);
+    else if (cfg->n_nsig != 0) { delete E; return efail(HB_INVALID_CONFIG, "nsig per-variable length does not match nvar"); }
+    E->bold_period = cfg->bold_period;
+    E->bold_tr_s = cfg->bold_tr_s;
+    E->engine = cfg->engine;
+    E->delay_mode = cfg->delay_mode;
+    E->noise_seed = cfg->noise_seed;
+    E->noise_shared = cfg->noise_shared;
+    if (cfg->n_projections > 1) { delete E; return efail(HB_INVALID_CONFIG, "at most one (self) projection is supported"); }
+    if (cfg->n_projections == 1) {
+        const hb_projection& Pj = cfg->projections[0];
+        const int N = S.n_nodes;
+        E->has_proj = true;
+        E->cfn = Pj.cfn;
+        E->cparams.assign(Pj.cparams, Pj.cparams + std::max(0, std::min(8, (int)Pj.n_cparams)));
+        if (Pj.wkind == HB_W_SCALAR_MEAN || Pj.wkind == HB_W_SCALAR_DIAG) {
+            E->wkind_cfg = HB_W_SCALAR_MEAN;
+            E->scalar_w = Pj.scalar_w;
+        } else if (Pj.wkind == HB_W_DENSE) {
+            if (!Pj.dense) { delete E; return efail(HB_INVALID_CONFIG, "dense weights missing"); }
+            E->wkind_cfg = HB_W_DENSE;
+            E->dense.assign(Pj.dense, Pj.dense + (size_t)N * N);
+        } else if (Pj.wkind == HB_W_CSR) {
+            if (!Pj.indptr || Pj.nnz < 0 || (Pj.nnz > 0 && (!Pj.indices || !Pj.data))) { delete E; return efail(HB_INVALID_CONFIG, "CSR arrays missing"); }
+            if (Pj.indptr[N] != Pj.nnz) { delete E; return efail(HB_INVALID_CONFIG, "CSR indptr length != tgt_nnodes + 1"); }
+            E->wkind_cfg = HB_W_CSR;
+            E->indptr.assign(Pj.indptr, Pj.indptr + N + 1);
+            E->indices.assign(Pj.indices, Pj.indices + Pj.nnz);
+            E->data.assign(Pj.data, Pj.data + Pj.nnz);
+            if (Pj.cfn == HB_CF_DIFFERENCE) {
+                // the batch engine densifies CSR before the Difference rewrite
+                // (engine.rs:298-350); keep that arithmetic by running it dense
+                E->dense.assign((size_t)N * N, 0.0f);
+                for (int r = 0; r < N; ++r)
+                    for (int e = Pj.indptr[r]; e < Pj.indptr[r + 1]; ++e) E->dense[(size_t)r * N + Pj.indices[e]] = Pj.data[e];
+                E->wkind_cfg = HB_W_DENSE;
+            }
+        } else {
+            E->has_proj = false;
+        }
+        if (Pj.delays && Pj.n_delays > 0) {
+            E->delays.assign(Pj.delays, Pj.delays + Pj.n_delays);
+            for (int32_t d : E->delays)
+                if (d < 0 || d > 65535) { delete E; return efail(HB_INVALID_CONFIG, "delay out of range"); }
+        }
+    }
+    *out = E;
+    return HB_OK;
+}
+
+void hb_engine_destroy(hb_engine* e) { delete e; }
+
+int hb_run_sweep(hb_engine* E, int32_t sub_idx, int32_t param_idx, const float* values,
+                 int32_t n_values, int64_t n_steps, uint32_t flags, hb_result* out) {
+    if (!E || !out) return efail(HB_INVALID_STATE, "null engine or result");
+    if (n_values != E->n_sweep || !values)
+        return efail(HB_INVALID_CONFIG, "number of sweep values must equal n_sweep");
+    if (sub_idx != 0) return efail(HB_INVALID_CONFIG, "subnetwork index out of range");
+    if (n_steps < 0 || n_steps > INT32_MAX) return efail(HB_INVALID_CONFIG, "n_steps out of range");
+    Built B;
+    B.inst.reserve(n_values);
+    // Only the model's batch-native index is applied; any other index is
+    // ignored, as in dfun_batch (dfun.rs:49-72).
+    const bool native = param_idx == model_sweep_index(E->model);
+    for (int i = 0; i < n_values; ++i) {
+        hb_instance I = base_instance(E, n_steps);
+        if (native) { I.sweep_form = 1; I.sweep_val = values[i]; }
+        I.noise_stream = E->noise_shared ? 0u : (uint64_t)i;
+        B.inst.push_back(I);
+    }
+    return run_built(E, B, n_steps, flags, true, out);
+}
+
+int hb_run_sweep_multi(hb_engine* E, const hb_sweep_point* pts, int32_t n_points, int64_t n_steps,
+                       uint32_t flags, hb_result* out) {
+    if (!E || !out) return efail(HB_INVALID_STATE, "null engine or result");
+    if (n_points < 1 || !pts) return efail(HB_INVALID_CONFIG, "empty sweep spec");
+    if (n_steps < 0 || n_steps > INT32_MAX) return efail(HB_INVALID_CONFIG, "n_steps out of range");
+    int64_t total = 1;
+    int n_model = 0;
+    for (int d = 0; d < n_points; ++d) {
+        if (pts[d].n_values < 1 || !pts[d].values) return efail(HB_INVALID_CONFIG, "sweep dimension without values");
+        if (pts[d].kind == 0) {
+            if (pts[d].a != 0 || pts[d].b < 0 || pts[d].b >= model_nparams(E->model))
+                return efail(HB_INVALID_CONFIG, "sweep parameter index out of range");
+            ++n_model;
+        } else if (pts[d].kind == 1) {
+            if (!E->has_proj || pts[d].a != 0 || pts[d].b < 0 || pts[d].b >= 8)
+                return efail(HB_INVALID_CONFIG, "coupling parameter index out of range");
+        } else {
+            return efail(HB_INVALID_CONFIG, "unknown sweep point kind");
+        }
+        total *= pts[d].n_values;
+        if (total > (int64_t)1 << 24) return efail(HB_INVALID_CONFIG, "sweep too large");
+    }
+    Built B;
+    B.inst.reserve((size_t)total);
+    B.params.resize((size_t)total);
+    std::vector<int> idx(n_points, 0);
+    for (int64_t s = 0; s < total; ++s) {
+        hb_instance I = base_instance(E, n_steps);
+        std::vector<float>& p = B.params[(size_t)s];
+        p = E->params;
+        for (int d = 0; d < n_points; ++d) {
+            const float v = pts[d].values[idx[d]];
+            if (pts[d].kind == 0) {
+                if (n_model == 1) {
+                    // dfun_batch_dispatch with one swept parameter: native index only
+                    if (pts[d].b == model_sweep_index(E->model)) { I.sweep_form = 1; I.sweep_val = v; }
+                } else {
+                    p[pts[d].b] = v;   // dfun_batch_multi_fallback: per-point params (dfun.rs:99-133)
+                }
+            } else {
+                const int k = pts[d].b;
+                if (k >= I.n_cparams) {
+                    for (int z = I.n_cparams; z <= k; ++z) I.cparams[z] = 0.0f;
+                    I.n_cparams = k + 1;
+                }
+                I.cparams[k] = v;
+            }
+        }
+        I.noise_stream = E->noise_shared ? 0u : (uint64_t)s;
+        B.inst.push_back(I);
+        for (int d = n_points - 1; d >= 0; --d) {   // last dimension fastest (engine.rs:69-90)
+            if (++idx[d] < pts[d].n_values) break;
+            idx[d] = 0;
+        }
+    }
+    for (int64_t s = 0; s < total; ++s) B.inst[(size_t)s].params = B.params[(size_t)s].data();
+    // run_sweep_multi rebuilds the engine without monitors (engine.rs:897-909)
+    return run_built(E, B, n_steps, flags, false, out);
+}
+
+void hb_result_free(hb_result* r) {
+    if (!r) return;
+    free(r->tavg);
+    free(r->final_states);
+    free(r->trajectories);
+    free(r->bold);
+    r->tavg = r->final_states = r->trajectories = r->bold = nullptr;
+}
+
+}  // extern "C"

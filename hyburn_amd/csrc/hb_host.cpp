@@ -1,0 +1,693 @@
+// hb_host.cpp -- host side of the C ABI: validation, HBM layout, cost-ordered
+// work lists, launch shapes, hipGraph capture, timing and the
+// BatchHybridEngine-style sweep API.
+//
+// Reference interfaces mirrored (maedoc/hyburn):
+//   BatchHybridEngine::from_config / run_sweep / run_sweep_multi
+//       src/engine/batch_engine/engine.rs:145-471, 477-494, 845-1348
+//   SimulationError codes            src/error.rs:7-23
+//   select_backend                   src/cli.rs:259-271
+#include "hb_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+using namespace hb;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+}  // namespace
+
+int hb::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
+
+#define HB_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(HB_BACKEND_ERROR, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Arena layout builder: records (offset, size) requests, one allocation later.
+struct Arena {
+    size_t top = 0;
+    size_t take(size_t bytes) {
+        size_t off = top;
+        top = align_up(top + (bytes ? bytes : 1), 256);
+        return off;
+    }
+};
+
+bool is_stoch(int integ) { return integ >= HB_EULER_STOCH; }
+
+// pre(0) computed with the host libm, as the reference computes pre() on a zero
+// tensor for single-delay reads of not-yet-simulated time (engine.rs:580-583).
+void pre_zero(int cfn, const float* cc, int classic, int np, float* out) {
+    out[0] = out[1] = 0.0f;
+    switch (cfn) {
+    case HB_CF_KURAMOTO:
+        out[0] = sinf(0.0f);
+        out[1] = cosf(0.0f);
+        break;
+    case HB_CF_TANH:
+        for (int c = 0; c < np; ++c) out[c] = (tanhf((0.0f * cc[1] + (-cc[2])) / cc[3]) + 1.0f) * cc[0];
+        break;
+    case HB_CF_SJR: {
+        const float den = expf((0.0f + (-cc[4])) * (-cc[3])) + 1.0f;
+        out[0] = classic ? (1.0f / den) * cc[2] + cc[1] : (1.0f / den) * cc[1];
+        break; }
+    case HB_CF_PRESIGMOIDAL:
+        for (int c = 0; c < np; ++c) out[c] = (tanhf((0.0f * cc[3] + (-cc[4])) * cc[2]) + cc[1]) * cc[0];
+        break;
+    default:
+        break;
+    }
+}
+
+struct BoldShape {
+    int native = 1;
+    int nout = 0;
+    double dtb = 0.0;
+};
+
+// Number of BOLD rows a run produces, bold_monitor.rs:72-227 (window times in
+// f64, TR resampling count floor(max_t / TR)).
+BoldShape bold_shape(int T, int P, double dt_ms, double tr_ms) {
+    BoldShape b;
+    if (P <= 0) return b;
+    b.dtb = dt_ms * (double)P;
+    const int n_full = T / P, rem = T % P;
+    const int n_native = n_full + (rem > 0 ? 1 : 0);
+    b.native = (tr_ms <= 0.0 || std::fabs(tr_ms - b.dtb) < 1e-6) ? 1 : 0;
+    if (b.native) { b.nout = n_native; return b; }
+    if (n_native == 0) { b.nout = 0; return b; }
+    double max_t;
+    if (rem > 0) max_t = (double)n_full * b.dtb + dt_ms * (double)rem;
+    else max_t = (double)(n_full - 1) * b.dtb + b.dtb;
+    b.nout = max_t >= tr_ms ? (int)std::floor(max_t / tr_ms) : 0;
+    return b;
+}
+
+}  // namespace
+
+// ============================================================== plan ========
+
+struct hb_plan {
+    int device = 0;
+    uint32_t flags = 0;
+    std::vector<hb_instance> inst;
+    std::vector<DevInst> dev;
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
+    DevInst* d_inst = nullptr;
+    int32_t* d_order = nullptr;
+    int32_t* d_counters = nullptr;
+    NoiseJob* d_jobs = nullptr;
+    int n_jobs = 0;
+    int64_t max_noise = 0;
+    std::vector<LaunchDesc> launches;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t g_noise = nullptr, g_fwd = nullptr;
+    hipGraph_t gr_noise = nullptr, gr_fwd = nullptr;
+    struct Out {
+        size_t tavg, fin, traj, bold;
+        size_t n_state, n_traj, n_bold;
+    };
+    std::vector<Out> outs;
+    double node_steps = 0.0;
+    int block_main = 0, lds_main = 0;
+};
+
+namespace {
+
+int validate_instance(const hb_instance& I, int idx) {
+    char buf[256];
+    auto bad = [&](const char* what) {
+        snprintf(buf, sizeof(buf), "instance %d: %s", idx, what);
+        return fail(HB_INVALID_CONFIG, buf);
+    };
+    if (I.model < 0 || I.model > HB_RWW) return bad("unknown model");
+    if (I.integrator < HB_EULER || I.integrator > HB_RK4_STOCH) return bad("unknown integrator");
+    if (I.n_nodes < 1 || I.n_nodes > 65535) return bad("n_nodes must be in [1, 65535]");
+    if (I.n_steps < 0) return bad("n_steps must be >= 0");
+    if (!(I.dt > 0.0f)) return bad("dt must be positive");
+    if (!I.params || !I.x0) return bad("params and x0 are required");
+    if (I.wkind < HB_W_NONE || I.wkind > HB_W_CSR) return bad("unknown weight kind");
+    if (I.cfn < HB_CF_LINEAR || I.cfn > HB_CF_PRESIGMOIDAL) return bad("unknown coupling function");
+    const int nc = model_ncvar(I.model);
+    const int np = coupling_npre(I.cfn, nc);
+    if (np > 2 || (I.cfn == HB_CF_SJR && nc != 1)) return bad("unsupported model/coupling-function combination");
+    if (I.cfn == HB_CF_KURAMOTO && I.model != HB_KURAMOTO && nc != 1) return bad("unsupported model/coupling-function combination");
+    if (I.cfn == HB_CF_KURAMOTO && I.model != HB_KURAMOTO) return bad("Kuramoto coupling is only instantiated for the Kuramoto model");
+    if (I.wkind == HB_W_DENSE && !I.dense) return bad("dense weights missing");
+    if (I.wkind == HB_W_CSR) {
+        if (!I.indptr || (I.nnz > 0 && (!I.indices || !I.weights))) return bad("CSR arrays missing");
+        if (I.indptr[0] != 0 || I.indptr[I.n_nodes] != I.nnz) return bad("CSR indptr inconsistent with nnz");
+        for (int i = 0; i < I.n_nodes; ++i)
+            if (I.indptr[i + 1] < I.indptr[i]) return bad("CSR indptr not monotone");
+        for (int e = 0; e < I.nnz; ++e) {
+            if (I.indices[e] < 0 || I.indices[e] >= I.n_nodes) return bad("CSR index out of range");
+            if (I.edge_delays && (I.edge_delays[e] < 0 || I.edge_delays[e] > 65535)) return bad("edge delay out of range [0, 65535]");
+        }
+    }
+    if (I.cfn == HB_CF_DIFFERENCE && I.wkind == HB_W_CSR)
+        return bad("Difference coupling with CSR weights is not supported");
+    if (I.delay_mode < HB_DELAY_PER_EDGE || I.delay_mode > HB_DELAY_BATCH_COMPAT) return bad("unknown delay mode");
+    if (I.single_delay < 0 || I.single_delay > 65535) return bad("delay out of range");
+    if (I.max_delay_hint < 0 || I.max_delay_hint > 65535) return bad("delay hint out of range");
+    if (I.node_param_idx >= model_nparams(I.model)) return bad("node parameter index out of range");
+    if (I.node_param_idx >= 0 && !I.node_param) return bad("node parameter values missing");
+    if (is_stoch(I.integrator) && !I.nsig) return bad("stochastic integrator needs nsig");
+    if (I.bold_period < 0) return bad("bold_period must be >= 0");
+    return HB_OK;
+}
+
+// Largest delay the ring must hold. For BATCH_COMPAT the reference's window
+// horizon is max(all delays) + 1 even though only delays[0] is read
+// (engine.rs:253-270, 410).
+int max_delay_of(const hb_instance& I) {
+    if (I.wkind == HB_W_NONE) return 0;
+    if (I.delay_mode == HB_DELAY_BATCH_COMPAT) {
+        int m = std::max(I.single_delay, I.max_delay_hint);
+        if (I.edge_delays)
+            for (int e = 0; e < I.nnz; ++e) m = std::max(m, (int)I.edge_delays[e]);
+        return m;
+    }
+    if (I.wkind == HB_W_CSR && I.delay_mode == HB_DELAY_PER_EDGE) {
+        int m = 0;
+        if (I.edge_delays)
+            for (int e = 0; e < I.nnz; ++e) m = std::max(m, (int)I.edge_delays[e]);
+        return m;
+    }
+    return I.single_delay;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hb_last_error(void) { return g_err.c_str(); }
+const char* hb_version(void) { return "hyburn-hip 0.1 (gfx950)"; }
+
+int hb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char* hb_select_backend(const char* name) {
+    if (name && (!strcmp(name, "hip") || !strcmp(name, "rocm"))) return "hip";
+    g_err = std::string("backend '") + (name ? name : "(null)") + "' is not provided by this library (HIP only)";
+    return nullptr;
+}
+
+void hb_plan_destroy(hb_plan* p) {
+    if (!p) return;
+    if (p->g_noise) (void)hipGraphExecDestroy(p->g_noise);
+    if (p->g_fwd) (void)hipGraphExecDestroy(p->g_fwd);
+    if (p->gr_noise) (void)hipGraphDestroy(p->gr_noise);
+    if (p->gr_fwd) (void)hipGraphDestroy(p->gr_fwd);
+    for (auto& e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->arena) (void)hipFree(p->arena);
+    delete p;
+}
+
+int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, hb_plan** out) {
+    if (!out) return fail(HB_INVALID_CONFIG, "null output pointer");
+    *out = nullptr;
+    if (n < 1 || !inst) return fail(HB_INVALID_CONFIG, "need at least one instance");
+    for (int i = 0; i < n; ++i) {
+        int rc = validate_instance(inst[i], i);
+        if (rc) return rc;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(HB_BACKEND_ERROR, "no HIP device available");
+    const int device = opts ? opts->device : 0;
+    if (device < 0 || device >= ndev) return fail(HB_INVALID_CONFIG, "device ordinal out of range");
+    HB_HIP(hipSetDevice(device));
+    int lds_max = 0, n_cu = 0;
+    HB_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    HB_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device));
+    const int lds_budget = lds_max - 512;   // static LDS of the kernels
+    {
+        static bool attrs_done[64] = {false};
+        if (device < 64 && !attrs_done[device]) {
+            HB_HIP(forward_set_attributes_fast(lds_budget));
+            HB_HIP(forward_set_attributes_exact(lds_budget));
+            attrs_done[device] = true;
+        }
+    }
+
+    hb_plan* P = new hb_plan();
+    P->device = device;
+    P->flags = opts ? opts->flags : 0u;
+    P->inst.assign(inst, inst + n);
+    P->dev.resize(n);
+    P->outs.resize(n);
+    const int req_block = opts ? opts->block_size : 0;
+    const bool exact = (P->flags & HB_FLAG_EXACT) != 0;
+
+    // ---- per-instance shapes, LDS-vs-HBM ring, block size
+    struct Shape {
+        int nv, nc, np, depth, block;
+        bool multi, lds_ring;
+        int lds_bytes;
+        double cost;
+    };
+    std::vector<Shape> sh(n);
+    for (int i = 0; i < n; ++i) {
+        const hb_instance& I = inst[i];
+        Shape& s = sh[i];
+        s.nv = model_nvar(I.model);
+        s.nc = model_ncvar(I.model);
+        s.np = coupling_npre(I.cfn, s.nc);
+        s.depth = max_delay_of(I) + 2;
+        int b = req_block > 0 ? req_block : (int)align_up((size_t)I.n_nodes, 64);
+        b = std::max(64, std::min(512, b));
+        s.block = b;
+        s.multi = I.n_nodes > b;
+        const int ring_b = s.depth * I.n_nodes * s.np * 4;
+        const int tab_b = s.multi ? I.n_nodes * (2 * s.nv + 6) * 4 : 0;
+        s.lds_ring = (int64_t)ring_b + tab_b <= lds_budget;
+        s.lds_bytes = (s.lds_ring ? ring_b : 0) + tab_b;
+        if (s.lds_bytes > lds_budget) {
+            hb_plan_destroy(P);
+            return fail(HB_INVALID_CONFIG, "instance state table exceeds LDS");
+        }
+        const int stages = (I.integrator == HB_RK4 || I.integrator == HB_RK4_STOCH) ? 4
+                           : (I.integrator == HB_HEUN || I.integrator == HB_HEUN_STOCH) ? 2 : 1;
+        const double nnz_r = I.wkind == HB_W_CSR ? (double)I.nnz / I.n_nodes
+                             : I.wkind == HB_W_DENSE ? (double)I.n_nodes : 1.0;
+        s.cost = (double)I.n_steps * ((double)((I.n_nodes + b - 1) / b) * (stages * s.nv * 8.0 + nnz_r * s.np * 2.0) + 64.0);
+        P->node_steps += (double)I.n_nodes * (double)I.n_steps;
+    }
+
+    // ---- HBM layout (inputs deduplicated by host pointer)
+    Arena A;
+    std::map<std::tuple<const void*, const void*, const void*, size_t>, size_t> dedup;
+    auto shared_off = [&](const void* a, const void* b, const void* c, size_t bytes, bool* fresh) {
+        auto key = std::make_tuple(a, b, c, bytes);
+        auto it = dedup.find(key);
+        if (it != dedup.end()) { *fresh = false; return it->second; }
+        size_t off = A.take(bytes);
+        dedup[key] = off;
+        *fresh = true;
+        return off;
+    };
+    struct Upload { size_t off; std::vector<char> bytes; };
+    std::vector<Upload> uploads;
+    auto stage = [&](size_t off, const void* src, size_t bytes) {
+        Upload u;
+        u.off = off;
+        u.bytes.assign((const char*)src, (const char*)src + bytes);
+        uploads.push_back(std::move(u));
+    };
+    struct Ptrs {
+        size_t x0, node_mc, indptr, edges, dense_t, noise, ring, tavg, fin, traj, bold;
+        bool has_node_mc, has_noise, has_ring, has_traj, has_bold, has_csr, has_dense;
+        int64_t noise_n;
+        uint64_t seed, stream;
+        bool gen_noise;
+    };
+    std::vector<Ptrs> pp(n);
+    std::map<std::tuple<uint64_t, uint64_t, int64_t>, size_t> noise_dedup;
+    std::vector<NoiseJob> jobs_host;  // out pointers filled after allocation (offsets for now)
+    std::vector<size_t> job_off;
+
+    for (int i = 0; i < n; ++i) {
+        const hb_instance& I = inst[i];
+        const Shape& s = sh[i];
+        Ptrs& q = pp[i];
+        memset(&q, 0, sizeof(q));
+        const size_t N = (size_t)I.n_nodes, NV = (size_t)s.nv;
+        bool fresh;
+        q.x0 = shared_off(I.x0, nullptr, nullptr, N * NV * 4, &fresh);
+        if (fresh) stage(q.x0, I.x0, N * NV * 4);
+        if (I.node_param_idx >= 0) {
+            std::vector<float> mcs(N * kMC);
+            std::vector<float> p(I.params, I.params + model_nparams(I.model));
+            for (size_t k = 0; k < N; ++k) {
+                p[I.node_param_idx] = I.node_param[k];
+                model_consts(I.model, p.data(), &mcs[k * kMC]);
+            }
+            q.has_node_mc = true;
+            q.node_mc = A.take(mcs.size() * 4);
+            stage(q.node_mc, mcs.data(), mcs.size() * 4);
+        }
+        if (I.wkind == HB_W_CSR) {
+            q.has_csr = true;
+            const bool per_edge = I.delay_mode == HB_DELAY_PER_EDGE;
+            q.indptr = shared_off(I.indptr, nullptr, nullptr, (N + 1) * 4, &fresh);
+            if (fresh) stage(q.indptr, I.indptr, (N + 1) * 4);
+            q.edges = shared_off(I.indices, I.weights, per_edge ? (const void*)I.edge_delays : nullptr,
+                                 (size_t)std::max(I.nnz, 1) * 8, &fresh);
+            if (fresh) {
+                std::vector<uint2> ed((size_t)std::max(I.nnz, 1));
+                for (int e = 0; e < I.nnz; ++e) {
+                    uint32_t d = (per_edge && I.edge_delays) ? (uint32_t)I.edge_delays[e] : 0u;
+                    uint32_t wb;
+                    memcpy(&wb, &I.weights[e], 4);
+                    ed[e] = make_uint2((uint32_t)I.indices[e] | (d << 16), wb);
+                }
+                stage(q.edges, ed.data(), ed.size() * 8);
+            }
+        }
+        if (I.wkind == HB_W_DENSE) {
+            q.has_dense = true;
+            q.dense_t = shared_off(I.dense, nullptr, (const void*)(intptr_t)I.cfn, N * N * 4, &fresh);
+            if (fresh) {
+                std::vector<float> wt(N * N);
+                std::vector<float> w(I.dense, I.dense + N * N);
+                if (I.cfn == HB_CF_DIFFERENCE) {
+                    // construction.rs:694-707 / engine.rs:327-350: W'[i,i] -= sum_j W[i,j]
+                    for (size_t r = 0; r < N; ++r) {
+                        float rs = 0.0f;
+                        for (size_t c = 0; c < N; ++c) rs += w[r * N + c];
+                        w[r * N + r] -= rs;
+                    }
+                }
+                for (size_t r = 0; r < N; ++r)
+                    for (size_t c = 0; c < N; ++c) wt[c * N + r] = w[r * N + c];
+                stage(q.dense_t, wt.data(), wt.size() * 4);
+            }
+        }
+        if (is_stoch(I.integrator)) {
+            q.has_noise = true;
+            q.noise_n = (int64_t)I.n_steps * (int64_t)N * (int64_t)NV;
+            if (I.noise) {
+                q.noise = shared_off(I.noise, nullptr, nullptr, (size_t)q.noise_n * 4, &fresh);
+                if (fresh) stage(q.noise, I.noise, (size_t)q.noise_n * 4);
+            } else {
+                auto key = std::make_tuple(I.noise_seed, I.noise_stream, q.noise_n);
+                auto it = noise_dedup.find(key);
+                if (it != noise_dedup.end()) q.noise = it->second;
+                else {
+                    q.noise = A.take((size_t)q.noise_n * 4);
+                    noise_dedup[key] = q.noise;
+                    NoiseJob j;
+                    j.out = nullptr;
+                    j.n = q.noise_n;
+                    j.seed = I.noise_seed;
+                    j.stream = I.noise_stream;
+                    jobs_host.push_back(j);
+                    job_off.push_back(q.noise);
+                    P->max_noise = std::max(P->max_noise, q.noise_n);
+                }
+            }
+        }
+        if (!s.lds_ring) {
+            q.has_ring = true;
+            q.ring = A.take((size_t)s.depth * N * s.np * 4);
+        }
+        P->outs[i].n_state = N * NV;
+        q.tavg = A.take(N * NV * 4);
+        q.fin = A.take(N * NV * 4);
+        P->outs[i].tavg = q.tavg;
+        P->outs[i].fin = q.fin;
+        if (I.traj) {
+            q.has_traj = true;
+            P->outs[i].n_traj = (size_t)I.n_steps * N * NV;
+            q.traj = A.take(P->outs[i].n_traj * 4);
+            P->outs[i].traj = q.traj;
+        }
+        if (I.bold && I.bold_period > 0) {
+            BoldShape b = bold_shape(I.n_steps, I.bold_period, I.dt_ms, I.bold_tr_ms);
+            q.has_bold = true;
+            P->outs[i].n_bold = (size_t)std::max(b.nout, 0) * N;
+            q.bold = A.take(std::max(P->outs[i].n_bold, (size_t)1) * 4);
+            P->outs[i].bold = q.bold;
+        }
+    }
+    // instance descriptors, order, counters, noise jobs
+    const size_t off_inst = A.take(sizeof(DevInst) * n);
+    const size_t off_order = A.take(sizeof(int32_t) * n);
+    const size_t off_cnt = A.take(sizeof(int32_t) * 64);
+    const size_t off_jobs = A.take(sizeof(NoiseJob) * std::max<size_t>(jobs_host.size(), 1));
+
+    P->arena_bytes = A.top;
+    hipError_t he = hipMalloc((void**)&P->arena, P->arena_bytes);
+    if (he != hipSuccess) {
+        hb_plan_destroy(P);
+        return fail(HB_BACKEND_ERROR, std::string("hipMalloc of plan arena failed: ") + hipGetErrorString(he));
+    }
+    char* base = P->arena;
+    for (auto& u : uploads) {
+        he = hipMemcpy(base + u.off, u.bytes.data(), u.bytes.size(), hipMemcpyHostToDevice);
+        if (he != hipSuccess) { hb_plan_destroy(P); return fail(HB_BACKEND_ERROR, "upload failed"); }
+    }
+    uploads.clear();
+
+    // ---- descriptors
+    for (int i = 0; i < n; ++i) {
+        const hb_instance& I = inst[i];
+        const Shape& s = sh[i];
+        const Ptrs& q = pp[i];
+        DevInst& D = P->dev[i];
+        memset(&D, 0, sizeof(D));
+        D.model = I.model; D.integrator = I.integrator; D.n_nodes = I.n_nodes; D.n_steps = I.n_steps;
+        D.np = s.np; D.wkind = I.wkind;
+        D.cfn = I.cfn == HB_CF_DIFFERENCE ? HB_CF_LINEAR : I.cfn;
+        D.delay_mode = I.wkind == HB_W_CSR ? I.delay_mode
+                       : (I.delay_mode == HB_DELAY_PER_EDGE ? HB_DELAY_SINGLE : I.delay_mode);
+        D.single_delay = I.single_delay;
+        D.ring_depth = s.depth;
+        D.sweep_form = I.sweep_form; D.sweep_val = I.sweep_val;
+        D.node_param_idx = q.has_node_mc ? I.node_param_idx : -1;
+        D.engine = I.engine;
+        D.cpu_form = (I.wkind == HB_W_CSR && I.delay_mode == HB_DELAY_PER_EDGE && I.edge_delays) ? 1 : 0;
+        D.dt = I.dt;
+        D.dt_half = I.dt * 0.5f;
+        D.dt_sixth = I.dt / 6.0f;
+        D.inv_nsteps = 1.0f / (float)I.n_steps;
+        model_consts(I.model, I.params, D.mc);
+        int classic = 0;
+        float cp[8];
+        memcpy(cp, I.cparams, sizeof(cp));
+        int ncp = I.n_cparams;
+        coupling_consts(D.cfn, cp, ncp, I.n_nodes, D.cc, &classic);
+        if (I.cfn == HB_CF_DIFFERENCE) {   // Difference -> Linear(a, 0) after weight preprocessing
+            D.cc[0] = ncp > 0 ? cp[0] : 1.0f;
+            D.cc[1] = 0.0f;
+        }
+        D.sjr_classic = classic;
+        D.scalar_w = I.scalar_w;
+        if (I.cfn == HB_CF_DIFFERENCE && (I.wkind == HB_W_SCALAR_MEAN || I.wkind == HB_W_SCALAR_DIAG))
+            D.scalar_w = 0.0f;   // engine.rs:330-333
+        float f0[2] = {0.0f, 0.0f};
+        const bool per_edge = I.wkind == HB_W_CSR && D.delay_mode == HB_DELAY_PER_EDGE;
+        if (!per_edge) pre_zero(D.cfn, D.cc, classic, s.np, f0);
+        D.fill0 = f0[0]; D.fill1 = f0[1];
+        for (int v = 0; v < s.nv; ++v) {
+            const float ns = I.nsig ? I.nsig[v] : 0.0f;
+            D.nscale[v] = sqrtf(2.0f * ns) * sqrtf(I.dt);   // integrator.rs:179-191
+        }
+        D.dt_ms = I.dt_ms;
+        if (I.bold_period > 0 && q.has_bold) {
+            BoldShape b = bold_shape(I.n_steps, I.bold_period, I.dt_ms, I.bold_tr_ms);
+            D.bold_period = I.bold_period;
+            D.bold_nout = b.nout;
+            D.bold_native = b.native;
+            D.tr_ms = I.bold_tr_ms;
+            D.dtb_ms = b.dtb;
+            D.bold = (float*)(base + q.bold);
+        }
+        D.x0 = (const float*)(base + q.x0);
+        D.node_mc = q.has_node_mc ? (const float*)(base + q.node_mc) : nullptr;
+        D.indptr = q.has_csr ? (const int32_t*)(base + q.indptr) : nullptr;
+        D.edges = q.has_csr ? (const uint2*)(base + q.edges) : nullptr;
+        D.dense_t = q.has_dense ? (const float*)(base + q.dense_t) : nullptr;
+        D.noise = q.has_noise ? (const float*)(base + q.noise) : nullptr;
+        D.ring = q.has_ring ? (float*)(base + q.ring) : nullptr;
+        D.tavg = (float*)(base + q.tavg);
+        D.fin = (float*)(base + q.fin);
+        D.traj = q.has_traj ? (float*)(base + q.traj) : nullptr;
+        D.traj_tstride = (int64_t)I.n_nodes * s.nv;
+    }
+    P->d_inst = (DevInst*)(base + off_inst);
+    P->d_order = (int32_t*)(base + off_order);
+    P->d_counters = (int32_t*)(base + off_cnt);
+    P->d_jobs = (NoiseJob*)(base + off_jobs);
+    for (size_t j = 0; j < jobs_host.size(); ++j) jobs_host[j].out = (float*)(base + job_off[j]);
+    P->n_jobs = (int)jobs_host.size();
+    HB_HIP(hipMemcpy(P->d_inst, P->dev.data(), sizeof(DevInst) * n, hipMemcpyHostToDevice));
+    if (!jobs_host.empty())
+        HB_HIP(hipMemcpy(P->d_jobs, jobs_host.data(), sizeof(NoiseJob) * jobs_host.size(), hipMemcpyHostToDevice));
+
+    // ---- launch groups: (block, lds_ring, multi); cost-descending order inside
+    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+    for (int i = 0; i < n; ++i) groups[std::make_tuple(sh[i].block, (int)sh[i].lds_ring, (int)sh[i].multi)].push_back(i);
+    std::vector<int32_t> order;
+    order.reserve(n);
+    int gi = 0;
+    int max_lds_all = 0;
+    for (auto& kv : groups) {
+        auto& v = kv.second;
+        std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return sh[a].cost > sh[b].cost; });
+        LaunchDesc L;
+        memset(&L, 0, sizeof(L));
+        L.insts = P->d_inst;
+        L.order = P->d_order + order.size();
+        L.n_work = (int)v.size();
+        L.block = std::get<0>(kv.first);
+        L.lds_ring = std::get<1>(kv.first) != 0;
+        L.multi = std::get<2>(kv.first) != 0;
+        int lds = 0, model = inst[v[0]].model, np = sh[v[0]].np;
+        for (int i : v) {
+            lds = std::max(lds, sh[i].lds_bytes);
+            if (inst[i].model != model || sh[i].np != np) model = -1;
+        }
+        L.lds_bytes = lds;
+        L.model = model;
+        L.np = model >= 0 ? np : 0;
+        max_lds_all = std::max(max_lds_all, lds);
+        for (int i : v) order.push_back(i);
+        // persistent when the work exceeds one wave of resident workgroups
+        int occ = 0;
+        L.grid = L.n_work;
+        L.counter = nullptr;
+        if (forward_occupancy_fast(L, &occ) == hipSuccess && occ > 0) {
+            const int cap = occ * n_cu;
+            if (L.n_work > cap && gi < 64) {
+                L.grid = cap;
+                L.counter = P->d_counters + gi;
+            }
+        }
+        if (opts && opts->grid_waves > 0 && L.counter) L.grid = std::min(L.n_work, opts->grid_waves);
+        P->launches.push_back(L);
+        ++gi;
+        P->block_main = std::max(P->block_main, L.block);
+        P->lds_main = std::max(P->lds_main, L.lds_bytes);
+    }
+    HB_HIP(hipMemcpy(P->d_order, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+
+    HB_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    for (auto& e : P->ev) HB_HIP(hipEventCreate(&e));
+
+    // ---- capture the launch sequence
+    if (!(P->flags & HB_FLAG_NO_GRAPH)) {
+        HB_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e1 = launch_noise(P->d_jobs, P->n_jobs, P->max_noise, P->stream);
+        hipError_t e2 = hipStreamEndCapture(P->stream, &P->gr_noise);
+        if (e1 != hipSuccess || e2 != hipSuccess) { hb_plan_destroy(P); return fail(HB_BACKEND_ERROR, "noise graph capture failed"); }
+        HB_HIP(hipGraphInstantiate(&P->g_noise, P->gr_noise, nullptr, nullptr, 0));
+        HB_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t el = hipSuccess;
+        if (gi > 0) el = launch_zero_i32(P->d_counters, std::min(gi, 64), P->stream);
+        for (auto& L : P->launches) {
+            if (el != hipSuccess) break;
+            el = exact ? launch_forward_exact(L, P->stream) : launch_forward_fast(L, P->stream);
+        }
+        e2 = hipStreamEndCapture(P->stream, &P->gr_fwd);
+        if (el != hipSuccess || e2 != hipSuccess) {
+            hb_plan_destroy(P);
+            return fail(HB_BACKEND_ERROR, std::string("forward graph capture failed: ") + hipGetErrorString(el != hipSuccess ? el : e2));
+        }
+        HB_HIP(hipGraphInstantiate(&P->g_fwd, P->gr_fwd, nullptr, nullptr, 0));
+    }
+    *out = P;
+    return HB_OK;
+}
+
+int hb_plan_run(hb_plan* P, hb_run_stats* stats) {
+    if (!P) return fail(HB_INVALID_STATE, "null plan");
+    HB_HIP(hipSetDevice(P->device));
+    const bool exact = (P->flags & HB_FLAG_EXACT) != 0;
+    HB_HIP(hipEventRecord(P->ev[0], P->stream));
+    if (P->g_noise) HB_HIP(hipGraphLaunch(P->g_noise, P->stream));
+    else HB_HIP(launch_noise(P->d_jobs, P->n_jobs, P->max_noise, P->stream));
+    HB_HIP(hipEventRecord(P->ev[1], P->stream));
+    if (P->g_fwd) HB_HIP(hipGraphLaunch(P->g_fwd, P->stream));
+    else {
+        HB_HIP(launch_zero_i32(P->d_counters, std::min((int)P->launches.size(), 64), P->stream));
+        for (auto& L : P->launches)
+            HB_HIP(exact ? launch_forward_exact(L, P->stream) : launch_forward_fast(L, P->stream));
+    }
+    HB_HIP(hipEventRecord(P->ev[2], P->stream));
+    HB_HIP(hipEventSynchronize(P->ev[2]));
+    if (stats) {
+        float t01 = 0.0f, t12 = 0.0f;
+        HB_HIP(hipEventElapsedTime(&t01, P->ev[0], P->ev[1]));
+        HB_HIP(hipEventElapsedTime(&t12, P->ev[1], P->ev[2]));
+        stats->kernel_ms = t12;
+        stats->total_ms = t01 + t12;
+        stats->node_steps = P->node_steps;
+        stats->n_launches = (int)P->launches.size();
+        stats->block_size = P->block_main;
+        stats->lds_bytes = P->lds_main;
+    }
+    return HB_OK;
+}
+
+int hb_plan_download(hb_plan* P) {
+    if (!P) return fail(HB_INVALID_STATE, "null plan");
+    HB_HIP(hipSetDevice(P->device));
+    for (size_t i = 0; i < P->inst.size(); ++i) {
+        const hb_instance& I = P->inst[i];
+        const auto& o = P->outs[i];
+        if (I.tavg) HB_HIP(hipMemcpy(I.tavg, P->arena + o.tavg, o.n_state * 4, hipMemcpyDeviceToHost));
+        if (I.final_state) HB_HIP(hipMemcpy(I.final_state, P->arena + o.fin, o.n_state * 4, hipMemcpyDeviceToHost));
+        if (I.traj && o.n_traj) HB_HIP(hipMemcpy(I.traj, P->arena + o.traj, o.n_traj * 4, hipMemcpyDeviceToHost));
+        if (I.bold && I.bold_period > 0) {
+            const int N = I.n_nodes;
+            const size_t rows = N ? o.n_bold / N : 0;
+            const size_t keep = std::min(rows, (size_t)std::max(I.bold_cap, 0));
+            if (keep) HB_HIP(hipMemcpy(I.bold, P->arena + o.bold, keep * N * 4, hipMemcpyDeviceToHost));
+            if (I.bold_n) *I.bold_n = (int32_t)rows;
+        }
+    }
+    return HB_OK;
+}
+
+int hb_run_instances(const hb_instance* inst, int32_t n, const hb_run_opts* opts, hb_run_stats* stats) {
+    hb_plan* P = nullptr;
+    int rc = hb_plan_create(inst, n, opts, &P);
+    if (rc) return rc;
+    rc = hb_plan_run(P, stats);
+    if (!rc) rc = hb_plan_download(P);
+    hb_plan_destroy(P);
+    return rc;
+}
+
+int hb_philox_normal(uint64_t seed, uint64_t stream, int64_t n, float* out_host, int32_t device) {
+    if (n < 0 || (n > 0 && !out_host)) return fail(HB_INVALID_CONFIG, "bad arguments");
+    if (n == 0) return HB_OK;
+    HB_HIP(hipSetDevice(device));
+    float* d = nullptr;
+    NoiseJob* dj = nullptr;
+    HB_HIP(hipMalloc((void**)&d, (size_t)n * 4));
+    if (hipMalloc((void**)&dj, sizeof(NoiseJob)) != hipSuccess) { (void)hipFree(d); return fail(HB_BACKEND_ERROR, "hipMalloc failed"); }
+    NoiseJob j;
+    j.out = d; j.n = n; j.seed = seed; j.stream = stream;
+    hipError_t e = hipMemcpy(dj, &j, sizeof(j), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_noise(dj, 1, n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out_host, d, (size_t)n * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    (void)hipFree(dj);
+    if (e != hipSuccess) return fail(HB_BACKEND_ERROR, std::string("philox: ") + hipGetErrorString(e));
+    return HB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+// hb_internal.h -- types shared by the host planner and the gfx950 kernels.
+//
+// The per-instance device descriptor (hb_dev_inst) carries everything one
+// workgroup needs to integrate one simulation: sizes, precomputed model and
+// coupling constants (f32, evaluated in the reference's host-side op order),
+// and device pointers into the plan's HBM arena.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/hyburn_hip.h"
+
+#define HB_HD __host__ __device__ __forceinline__
+
+namespace hb {
+
+constexpr int kMaxNV = 6;
+constexpr int kMC = 28;   // model constant slots
+constexpr int kCC = 12;   // coupling constant slots
+
+HB_HD int model_nvar(int m) { return m == HB_JANSEN_RIT ? 6 : (m == HB_KURAMOTO || m == HB_RWW) ? 1 : 2; }
+HB_HD int model_ncvar(int m) { return m == HB_MPR ? 2 : 1; }
+HB_HD int model_nparams(int m) {
+    switch (m) {
+    case HB_G2DO: return 12; case HB_MPR: return 7; case HB_JANSEN_RIT: return 13;
+    case HB_WILSON_COWAN: return 22; case HB_KURAMOTO: return 1; default: return 8;
+    }
+}
+// The reference's single batch-sweepable parameter per model (dfun.rs:49-72).
+HB_HD int model_sweep_index(int m) {
+    switch (m) {
+    case HB_G2DO: return 1; case HB_MPR: return 4; case HB_JANSEN_RIT: return 12;
+    case HB_WILSON_COWAN: return 18; case HB_KURAMOTO: return 0; default: return 7;
+    }
+}
+// Number of pre() channels the ring stores per node (coupling.rs:204-258).
+HB_HD int coupling_npre(int cfn, int nc) {
+    if (cfn == HB_CF_KURAMOTO) return 2 * nc;
+    if (cfn == HB_CF_SJR) return 1;
+    return nc;
+}
+
+// f32 epsilon guard used by the reference for time constants (dfun.rs:279).
+constexpr float kEps = 1.1920929e-07f;
+constexpr float kPi = 3.14159265358979323846f;
+
+// Model constants, f32, evaluated with the reference's host-side op order.
+// G2DO  dfun.rs:277-293 | MPR 445-456 | JR 327-366 | WC 392-403 | KUR 486 | RWW 500-502
+HB_HD void model_consts(int m, const float* p, float* mc) {
+    for (int i = 0; i < kMC; ++i) mc[i] = 0.0f;
+    switch (m) {
+    case HB_G2DO: {
+        float tau = p[0] > kEps ? p[0] : kEps;
+        mc[0] = p[2]; mc[1] = p[3]; mc[2] = p[4]; mc[3] = p[6]; mc[4] = p[7]; mc[5] = p[8];
+        mc[6] = p[9]; mc[7] = p[10]; mc[8] = p[11];
+        mc[9] = p[11] * p[1];          // gamma * I  (add_scalar(gamma * params[1]))
+        mc[10] = p[5] * tau;           // d * tau
+        mc[11] = p[5] / tau;           // d / tau
+        break; }
+    case HB_MPR: {
+        float tau = p[0] > kEps ? p[0] : kEps;
+        mc[0] = 1.0f / tau;
+        mc[1] = p[1] / (kPi * tau);
+        mc[2] = kPi * kPi * tau * tau;
+        mc[3] = p[3] * tau;
+        mc[4] = p[2] + p[4];           // eta + i_ext
+        mc[5] = p[2];                  // eta (swept form adds I separately)
+        mc[6] = p[5]; mc[7] = p[6];
+        break; }
+    case HB_JANSEN_RIT: {
+        float Ap = p[0], Bp = p[1], a = p[2], b = p[3], J = p[7];
+        mc[0] = p[4]; mc[1] = p[6]; mc[2] = 2.0f * p[5];
+        mc[3] = Ap * a; mc[4] = 2.0f * a; mc[5] = a * a;
+        mc[6] = p[8] * J; mc[7] = p[9] * J; mc[8] = p[10] * J; mc[9] = p[11] * J;
+        mc[10] = Bp * b; mc[11] = 2.0f * b; mc[12] = b * b; mc[13] = p[12];
+        break; }
+    case HB_WILSON_COWAN: {
+        float te = p[4] > kEps ? p[4] : kEps, ti = p[5] > kEps ? p[5] : kEps;
+        mc[0] = p[0]; mc[1] = p[1]; mc[2] = p[2]; mc[3] = p[3];
+        mc[4] = 1.0f / te; mc[5] = 1.0f / ti;
+        mc[6] = p[6]; mc[7] = p[7]; mc[8] = p[8]; mc[9] = p[9];
+        mc[10] = p[10]; mc[11] = p[11]; mc[12] = p[12]; mc[13] = p[13];
+        mc[14] = p[14]; mc[15] = p[15]; mc[16] = p[16]; mc[17] = p[17];
+        mc[18] = p[18]; mc[19] = p[19]; mc[20] = p[20]; mc[21] = p[21];
+        mc[22] = 1.0f / (1.0f + expf(p[6] * p[7]));
+        mc[23] = 1.0f / (1.0f + expf(p[10] * p[11]));
+        break; }
+    case HB_KURAMOTO:
+        mc[0] = p[0];
+        break;
+    default: // RWW
+        mc[0] = p[0]; mc[1] = p[1]; mc[2] = p[2]; mc[3] = p[3];
+        mc[4] = 1.0f / p[4];
+        mc[5] = p[5] * p[6]; mc[6] = p[6]; mc[7] = p[7];
+        break;
+    }
+}
+
+// Coupling constants (coupling.rs:114-202 parsing, host-side products as the
+// reference forms them). Slot meaning depends on cfn; see pre()/post().
+inline void coupling_consts(int cfn, const float* P, int n, int n_src, float* cc, int* classic) {
+    for (int i = 0; i < kCC; ++i) cc[i] = 0.0f;
+    *classic = 0;
+    switch (cfn) {
+    case HB_CF_LINEAR: case HB_CF_SCALED_LINEAR:
+        cc[0] = n > 0 ? P[0] : 1.0f; cc[1] = n > 1 ? P[1] : 0.0f; break;
+    case HB_CF_DIFFERENCE:
+        cc[0] = n > 0 ? P[0] : 1.0f; break;
+    case HB_CF_KURAMOTO: {
+        float a = n > 0 ? P[0] : 1.0f;
+        cc[0] = a / (float)n_src;     // coupling.rs:370: a / n_src as f32
+        cc[1] = a; break; }
+    case HB_CF_SIGMOIDAL: {
+        float cmin, cmax, mid, a, sig;
+        if (n >= 5) { cmin = P[0]; cmax = P[1]; mid = P[2]; a = P[3]; sig = P[4]; }
+        else { cmin = -1.0f; cmax = n > 0 ? P[0] : 1.0f; mid = n > 1 ? P[1] : 0.0f; sig = n > 2 ? P[2] : 1.0f; a = 1.0f; }
+        cc[0] = cmin; cc[1] = cmax - cmin; cc[2] = mid; cc[3] = a; cc[4] = sig; break; }
+    case HB_CF_TANH:
+        cc[0] = n > 0 ? P[0] : 1.0f; cc[1] = n > 1 ? P[1] : 1.0f;
+        cc[2] = n > 2 ? P[2] : 0.0f; cc[3] = n > 3 ? P[3] : 1.0f; break;
+    case HB_CF_SJR:
+        if (n >= 5) { // classic: a, cmin, cmax, r, midpoint
+            *classic = 1;
+            cc[0] = P[0]; cc[1] = P[1]; cc[2] = P[2] - P[1]; cc[3] = P[3]; cc[4] = P[4];
+        } else {      // legacy: a, e0, r, v0
+            float a = n > 0 ? P[0] : 1.0f, e0 = n > 1 ? P[1] : 0.005f;
+            cc[0] = a; cc[1] = (a * 2.0f) * e0; cc[3] = n > 2 ? P[2] : 0.56f; cc[4] = n > 3 ? P[3] : 6.0f;
+        }
+        break;
+    case HB_CF_PRESIGMOIDAL:
+        cc[0] = n > 0 ? P[0] : 1.0f; cc[1] = n > 1 ? P[1] : 1.0f; cc[2] = n > 2 ? P[2] : 1.0f;
+        cc[3] = n > 3 ? P[3] : 1.0f; cc[4] = n > 4 ? P[4] : 0.5f; break;
+    }
+}
+
+// Per-instance device descriptor (one per simulation in a plan).
+struct alignas(16) DevInst {
+    int32_t model, integrator, n_nodes, n_steps;
+    int32_t np, wkind, cfn, delay_mode;
+    int32_t single_delay, ring_depth, sweep_form, node_param_idx;
+    int32_t bold_period, bold_nout, bold_native, cpu_form;
+    int32_t sjr_classic, engine, pad0, pad1;
+    float dt, dt_half, dt_sixth, inv_nsteps;
+    float sweep_val, scalar_w, fill0, fill1;
+    float mc[kMC];
+    float cc[kCC];
+    float nscale[8];
+    double dt_ms, tr_ms, dtb_ms;
+    const float* x0;
+    const float* node_mc;      // [N][kMC] per-node model constants, or null
+    const int32_t* indptr;
+    const uint2* edges;        // per edge: .x = src | delay << 16, .y = weight bits
+    const float* dense_t;      // [src][tgt]
+    const float* noise;        // raw N(0,1), [T][N][NV]
+    float* ring;               // global ring [depth][N][np] when not in LDS
+    float* tavg;
+    float* fin;
+    float* traj;
+    int64_t traj_tstride;      // floats between consecutive steps of the trajectory
+    float* bold;               // [bold rows][N]
+};
+
+// Launch description handed from the planner to a kernel translation unit.
+struct LaunchDesc {
+    const DevInst* insts;      // device array
+    const int32_t* order;      // device array of instance indices, cost-descending
+    int32_t n_work;
+    int32_t* counter;          // device work counter for persistent launches (or null)
+    int32_t grid;              // workgroups
+    int32_t block;             // threads per workgroup
+    int32_t lds_bytes;         // dynamic LDS per workgroup
+    int32_t model;             // single-model launch: model id; -1 mixed
+    int32_t np;                // single-model launch: pre channels
+    bool lds_ring;             // rings live in LDS
+    bool multi;                // nodes per thread > 1 (state table in LDS)
+};
+
+// Record the thread-local last-error message (hb_last_error) and return code.
+int set_error(int code, const std::string& msg);
+
+// Forward launchers (hb_forward.hip compiled twice: contracted and exact).
+hipError_t launch_forward_fast(const LaunchDesc& d, hipStream_t s);
+hipError_t launch_forward_exact(const LaunchDesc& d, hipStream_t s);
+hipError_t forward_set_attributes_fast(int max_lds);
+hipError_t forward_set_attributes_exact(int max_lds);
+hipError_t forward_occupancy_fast(const LaunchDesc& d, int* blocks_per_cu);
+hipError_t launch_zero_i32(int32_t* p, int n, hipStream_t s);
+// Philox normals: for each instance, fill its raw noise buffer.
+struct NoiseJob {
+    float* out;
+    int64_t n;
+    uint64_t seed;
+    uint64_t stream;
+};
+hipError_t launch_noise(const NoiseJob* jobs_dev, int n_jobs, int64_t max_n, hipStream_t s);
+
+}  // namespace hb

@@ -1,0 +1,204 @@
+"""Seeded synthetic workloads (connectomes, delays, parameters, noise).
+
+Noise: Philox4x32-10 (Salmon et al., SC'11) keyed by the 64-bit seed with
+counter (pair index, stream id); two 53-bit uniforms per block, Box-Muller in
+f64, cast to f32. `philox_normal` is the numpy statement of the stream the HIP
+noise kernel (csrc/hb_noise.hip) generates.
+
+Structure (connectomes, tract lengths, parameters, initial states) comes from
+numpy's PCG64 `default_rng(seed)`; the same image gives the same arrays.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _lib as L
+from .instances import Instance
+
+_M0, _M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+_W0, _W1 = 0x9E3779B9, 0xBB67AE85
+_MASK = np.uint64(0xFFFFFFFF)
+
+
+def _philox4x32_10(c0, c1, c2, c3, k0: int, k1: int):
+    for _ in range(10):
+        p0 = c0.astype(np.uint64) * _M0
+        p1 = c2.astype(np.uint64) * _M1
+        hi0, lo0 = (p0 >> np.uint64(32)).astype(np.uint32), (p0 & _MASK).astype(np.uint32)
+        hi1, lo1 = (p1 >> np.uint64(32)).astype(np.uint32), (p1 & _MASK).astype(np.uint32)
+        c0, c1, c2, c3 = hi1 ^ c1 ^ np.uint32(k0), lo1, hi0 ^ c3 ^ np.uint32(k1), lo0
+        k0 = (k0 + _W0) & 0xFFFFFFFF
+        k1 = (k1 + _W1) & 0xFFFFFFFF
+    return c0, c1, c2, c3
+
+
+def philox_normal(seed: int, stream: int, n: int) -> np.ndarray:
+    """Element k of stream `stream` under key `seed`, as f32 (see module doc)."""
+    npairs = (n + 1) // 2
+    p = np.arange(npairs, dtype=np.uint64)
+    c0 = (p & _MASK).astype(np.uint32)
+    c1 = (p >> np.uint64(32)).astype(np.uint32)
+    c2 = np.full(npairs, stream & 0xFFFFFFFF, dtype=np.uint32)
+    c3 = np.full(npairs, (stream >> 32) & 0xFFFFFFFF, dtype=np.uint32)
+    c0, c1, c2, c3 = _philox4x32_10(c0, c1, c2, c3, seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)
+    a = ((c0 >> np.uint32(5)).astype(np.uint64) << np.uint64(26)) | (c1 >> np.uint32(6)).astype(np.uint64)
+    b = ((c2 >> np.uint32(5)).astype(np.uint64) << np.uint64(26)) | (c3 >> np.uint32(6)).astype(np.uint64)
+    u1 = (a + np.uint64(1)).astype(np.float64) * 2.0 ** -53
+    u2 = b.astype(np.float64) * 2.0 ** -53
+    r = np.sqrt(-2.0 * np.log(u1))
+    th = 6.283185307179586 * u2
+    z = np.empty(2 * npairs, dtype=np.float64)
+    z[0::2] = r * np.cos(th)
+    z[1::2] = r * np.sin(th)
+    return z[:n].astype(np.float32)
+
+
+# ---------------------------------------------------------------- structure --
+
+G2DO_DEFAULT = np.array([1, 0, -2, -10, 0, 0.02, 3, 1, 0, 1, 1, 1], np.float32)     # g2do.rs:76-78
+MPR_DEFAULT = np.array([1, 1, -5, 15, 0, 1, 0], np.float32)                        # mpr.rs:61-63
+JR_DEFAULT = np.array([3.25, 22, 0.1, 0.05, 5.52, 0.0025, 0.56, 135, 1, 0.8, 0.25, 0.25, 0.22],
+                      np.float32)                                                  # jansen_rit.rs:77-82
+WC_DEFAULT = np.array([12, 4, 13, 11, 10, 10, 1.2, 2.8, 1, 0, 1, 4, 1, 0, 1, 1, 1, 1, 0, 0, 1, 1],
+                      np.float32)                                                  # wilson_cowan.rs:85-92
+KUR_DEFAULT = np.array([1.0], np.float32)                                          # kuramoto_model.rs:53-55
+RWW_DEFAULT = np.array([0.270, 0.108, 154, 0.641, 100, 0.6, 0.2609, 0.33], np.float32)  # rww.rs:61-63
+DEFAULTS = {L.G2DO: G2DO_DEFAULT, L.MPR: MPR_DEFAULT, L.JANSEN_RIT: JR_DEFAULT,
+            L.WILSON_COWAN: WC_DEFAULT, L.KURAMOTO: KUR_DEFAULT, L.RWW: RWW_DEFAULT}
+
+
+def dense_to_csr(W: np.ndarray):
+    """CSR by target row (row = target), column order within rows."""
+    W = np.asarray(W, np.float32)
+    rows, cols = np.nonzero(W)
+    indptr = np.zeros(W.shape[0] + 1, np.int32)
+    np.add.at(indptr, rows + 1, 1)
+    indptr = np.cumsum(indptr).astype(np.int32)
+    return indptr, cols.astype(np.int32), W[rows, cols].astype(np.float32)
+
+
+def lognormal_connectome(rng, n: int, density: float, symmetric: bool = True):
+    """Lognormal(0,1) weights on a random `density` fraction of the off-diagonal
+    pairs (upper triangle mirrored when symmetric), zero diagonal, max = 1."""
+    W = np.zeros((n, n), np.float64)
+    if symmetric:
+        iu = np.triu_indices(n, 1)
+        m = len(iu[0])
+        k = int(round(density * m))
+        sel = rng.choice(m, size=k, replace=False)
+        w = rng.lognormal(0.0, 1.0, size=k)
+        W[iu[0][sel], iu[1][sel]] = w
+        W = W + W.T
+    else:
+        off = np.flatnonzero(~np.eye(n, dtype=bool))
+        k = int(round(density * len(off)))
+        sel = rng.choice(off, size=k, replace=False)
+        W.flat[sel] = rng.lognormal(0.0, 1.0, size=k)
+    if W.max() > 0:
+        W /= W.max()
+    return W.astype(np.float32)
+
+
+def symmetric_uniform(rng, n: int, lo: float, hi: float) -> np.ndarray:
+    L_ = rng.uniform(lo, hi, size=(n, n))
+    L_ = np.triu(L_, 1)
+    return (L_ + L_.T).astype(np.float32)
+
+
+def delays_from_lengths(lengths: np.ndarray, speed: float, dt: float) -> np.ndarray:
+    """Integer delays round(L / speed / dt): the stated rule for these workloads
+    (the reference's resolve_delays uses ceil and is never called, config.rs:436-452)."""
+    return np.rint(np.asarray(lengths, np.float64) / speed / dt).astype(np.int32)
+
+
+# ---------------------------------------------------------------- workloads --
+
+def config1(seed: int = 1) -> Instance:
+    """Plain forward run: G2DO, 74 nodes, Heun dt=0.1, 10k steps, nsig=0, scalar
+    self-coupling 0.01 with Linear a=0.01 (HybridEngine semantics: w * I)."""
+    rng = np.random.default_rng(seed)
+    N = 74
+    x0 = rng.normal(0.0, 0.1, size=(N, 2)).astype(np.float32)
+    return Instance(model=L.G2DO, n_nodes=N, n_steps=10_000, dt=0.1, params=G2DO_DEFAULT.copy(),
+                    x0=x0, integrator=L.HEUN, engine=L.ENGINE_PLAIN, wkind=L.W_SCALAR_DIAG,
+                    scalar_w=0.01, cfn=L.CF_LINEAR, cparams=(0.01,), record_traj=True)
+
+
+def config2(n_sweep: int = 128, n_steps: int = 10_000, seed: int = 2, noise_seed: int = 7,
+            explicit_noise: bool = False) -> list[Instance]:
+    """Batched sweep: G2DO, 74 nodes, Linear G log-uniform [1e-3, 1e-1] x param a
+    U[-2, 2]; symmetric 20% lognormal connectome; per-edge delays
+    round(U[10,150] mm / 3 mm/ms / 0.1 ms) <= 500; HeunStochastic nsig=0.01."""
+    rng = np.random.default_rng(seed)
+    N, dt = 74, 0.1
+    W = lognormal_connectome(np.random.default_rng(seed + 1), N, 0.20)
+    Lm = symmetric_uniform(np.random.default_rng(seed + 2), N, 10.0, 150.0)
+    D = delays_from_lengths(Lm, 3.0, dt)
+    indptr, indices, weights = dense_to_csr(W)
+    delays = D[np.repeat(np.arange(N), np.diff(indptr)), indices].astype(np.int32)
+    G = (10.0 ** rng.uniform(-3.0, -1.0, size=n_sweep)).astype(np.float32)
+    a = rng.uniform(-2.0, 2.0, size=n_sweep).astype(np.float32)
+    x0 = np.random.default_rng(seed + 3).normal(0.0, 0.1, size=(N, 2)).astype(np.float32)
+    nsig = np.full(2, 0.01, np.float32)
+    out = []
+    for i in range(n_sweep):
+        p = G2DO_DEFAULT.copy()
+        p[2] = a[i]
+        noise = philox_normal(noise_seed, i, n_steps * N * 2).reshape(n_steps, N, 2) if explicit_noise else None
+        out.append(Instance(model=L.G2DO, n_nodes=N, n_steps=n_steps, dt=dt, params=p, x0=x0,
+                            integrator=L.HEUN_STOCH, wkind=L.W_CSR, cfn=L.CF_LINEAR,
+                            cparams=(float(G[i]), 0.0), indptr=indptr, indices=indices,
+                            weights=weights, edge_delays=delays, delay_mode=L.DELAY_PER_EDGE,
+                            nsig=nsig, noise=noise, noise_seed=noise_seed, noise_stream=i))
+    return out
+
+
+def config3(n_inst: int = 4096, seed: int = 4, t_scale: float = 1.0, bold: bool = True) -> list[Instance]:
+    """Ragged batch: MPR (Linear, BOLD TR=720 ms) or JR (legacy SigmoidalJansenRit
+    on y0), N in {68,74,84,164,400}, lognormal CSR at U[5%,20%], delays <= 40 ms
+    at dt=0.05 (<= 800 steps), sim length U[500,5000] ms (integer steps), RK4."""
+    rng = np.random.default_rng(seed)
+    dt = 0.05
+    sizes = np.array([68, 74, 84, 164, 400])
+    nets = {}
+    out = []
+    for i in range(n_inst):
+        model = L.MPR if rng.random() < 0.5 else L.JANSEN_RIT
+        N = int(rng.choice(sizes))
+        dens = float(rng.uniform(0.05, 0.20))
+        key = (N, round(dens, 2))
+        if key not in nets:   # connectomes shared by instances of one (N, density) class
+            crng = np.random.default_rng([seed, N, int(round(dens * 100))])
+            W = lognormal_connectome(crng, N, round(dens, 2), symmetric=False)
+            Lm = crng.uniform(10.0, 120.0, size=(N, N)).astype(np.float32)
+            ip, ix, w = dense_to_csr(W)
+            d = delays_from_lengths(Lm[np.repeat(np.arange(N), np.diff(ip)), ix], 3.0, dt)
+            nets[key] = (ip, ix, w, d.astype(np.int32))
+        ip, ix, w, d = nets[key]
+        T = int(round(rng.uniform(500.0, 5000.0) / dt * t_scale))
+        if model == L.MPR:
+            p = MPR_DEFAULT.copy()
+            p[2] = float(rng.uniform(-6.0, -4.0))
+            x0 = np.stack([np.full(N, 0.1), np.full(N, -2.0)], 1) + rng.normal(0, 0.01, (N, 2))
+            G = float(10 ** rng.uniform(-3, -1.3))
+            out.append(Instance(model=L.MPR, n_nodes=N, n_steps=T, dt=dt, params=p,
+                                x0=x0.astype(np.float32), integrator=L.RK4, wkind=L.W_CSR,
+                                cfn=L.CF_LINEAR, cparams=(G, 0.0), indptr=ip, indices=ix, weights=w,
+                                edge_delays=d, delay_mode=L.DELAY_PER_EDGE,
+                                bold_period=10, bold_tr_ms=720.0, record_bold=bold))
+        else:
+            p = JR_DEFAULT.copy()
+            x0 = rng.normal(0.0, 0.01, (N, 6)).astype(np.float32)
+            a = float(rng.uniform(0.5, 2.0))
+            out.append(Instance(model=L.JANSEN_RIT, n_nodes=N, n_steps=T, dt=dt, params=p, x0=x0,
+                                integrator=L.RK4, wkind=L.W_CSR, cfn=L.CF_SJR, cparams=(a,),
+                                indptr=ip, indices=ix, weights=w, edge_delays=d,
+                                delay_mode=L.DELAY_PER_EDGE))
+    return out
+
+
+def node_steps(insts) -> float:
+    return float(sum(I.n_nodes * I.n_steps for I in insts))
+

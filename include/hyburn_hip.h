@@ -1,0 +1,217 @@
+/*
+ * hyburn_hip.h -- C ABI of the MI355X (gfx950) batched neural-mass integrator.
+ *
+ * This is the drop-in boundary for hyburn's batched integration path: the
+ * `--backend hip` counterpart of BatchHybridEngine<B: Backend>
+ * (reference: src/engine/batch_engine/engine.rs:145-860, src/cli.rs:259-271).
+ * Plain C types only: pointers, sizes, enums, integer error codes.
+ *
+ * Units follow the reference: dt and sim_length in ms, BOLD TR in seconds
+ * in hb_sim_config (config.rs:219-250) and in ms in hb_instance.
+ * State layout is node-major [n_nodes][nvar] per instance (engine.rs:161-170);
+ * batched outputs are [n_sweep][n_nodes][nvar] (engine.rs:15-31).
+ */
+#ifndef HYBURN_HIP_H
+#define HYBURN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes: mirror SimulationError (src/error.rs:7-23). */
+#define HB_OK 0
+#define HB_INVALID_CONFIG 1
+#define HB_INVALID_STATE 2
+#define HB_BACKEND_ERROR 3
+#define HB_IO_ERROR 4
+
+/* Models (config.rs:43-50 registry order of the six batch-native models). */
+enum { HB_G2DO = 0, HB_MPR = 1, HB_JANSEN_RIT = 2, HB_WILSON_COWAN = 3, HB_KURAMOTO = 4, HB_RWW = 5 };
+/* IntegratorKind (integrator.rs:17-30). */
+enum { HB_EULER = 0, HB_HEUN = 1, HB_RK4 = 2, HB_EULER_STOCH = 3, HB_HEUN_STOCH = 4, HB_RK4_STOCH = 5 };
+/* CouplingFnConfig (coupling.rs:53-71). */
+enum { HB_CF_LINEAR = 0, HB_CF_DIFFERENCE = 1, HB_CF_KURAMOTO = 2, HB_CF_SIGMOIDAL = 3,
+       HB_CF_SCALED_LINEAR = 4, HB_CF_TANH = 5, HB_CF_SJR = 6, HB_CF_PRESIGMOIDAL = 7 };
+/* Weight kinds. SCALAR_MEAN: w * mean_j pre(x_j) (batch engine, engine.rs:602-608);
+ * SCALAR_DIAG: w * I (plain engine, construction.rs:639-655). */
+enum { HB_W_NONE = 0, HB_W_SCALAR_MEAN = 1, HB_W_SCALAR_DIAG = 2, HB_W_DENSE = 3, HB_W_CSR = 4 };
+/* Delay semantics. PER_EDGE: runtime.rs:162-204; SINGLE: one delay per projection
+ * with the ring semantics of runtime.rs:214-234; BATCH_COMPAT: the sliding-window
+ * read of engine.rs:541-589. */
+enum { HB_DELAY_PER_EDGE = 0, HB_DELAY_SINGLE = 1, HB_DELAY_BATCH_COMPAT = 2 };
+/* Engine semantics where the two reference engines differ. */
+enum { HB_ENGINE_BATCH = 0, HB_ENGINE_PLAIN = 1 };
+
+/* Run options (bit flags). */
+#define HB_FLAG_TRAJECTORY 0x1u   /* record [n_steps][n_sweep][n_nodes][nvar] (engine.rs:487-494) */
+#define HB_FLAG_EXACT 0x2u        /* no FMA contraction: same f32 rounding as the written expressions */
+#define HB_FLAG_NO_GRAPH 0x4u     /* launch directly instead of replaying a captured hipGraph */
+
+/* ---------------- config-level API (BatchHybridEngine mirror) ---------------- */
+
+typedef struct {
+    int32_t model;        /* HB_G2DO ... */
+    int32_t n_nodes;
+    int32_t n_params;
+    const float* params;  /* model parameters, reference order (Appendix of model files) */
+    const float* x0;      /* [n_nodes][nvar] node-major initial state */
+} hb_subnetwork;
+
+typedef struct {
+    int32_t wkind;        /* HB_W_* */
+    float scalar_w;
+    const float* dense;   /* [n_nodes][n_nodes], row = target (config.rs:208-216) */
+    const int32_t* indptr;   /* CSR by target row, length n_nodes + 1 */
+    const int32_t* indices;
+    const float* data;
+    int32_t nnz;
+    const int32_t* delays;   /* per-edge (n_delays == nnz, CSR) or one per projection */
+    int32_t n_delays;
+    int32_t cfn;          /* HB_CF_* */
+    int32_t n_cparams;
+    float cparams[8];
+} hb_projection;
+
+typedef struct {
+    double sim_length;    /* ms */
+    double dt;            /* ms */
+    int32_t integrator;
+    int32_t n_nsig;       /* 0, 1 (scalar) or nvar */
+    const float* nsig;
+    const hb_subnetwork* subnetworks;
+    int32_t n_subnetworks;   /* 1 */
+    const hb_projection* projections;
+    int32_t n_projections;   /* 0 or 1 (a self-projection 0 -> 0) */
+    int32_t bold_period;     /* 0: no BOLD monitor */
+    double bold_tr_s;
+    int32_t engine;          /* HB_ENGINE_BATCH (BatchHybridEngine) or HB_ENGINE_PLAIN (HybridEngine) */
+    int32_t delay_mode;      /* HB_DELAY_* for a CSR projection with per-edge delays */
+    uint64_t noise_seed;     /* Philox4x32-10 key for the pre-generated noise */
+    int32_t noise_shared;    /* 1: one noise stream broadcast over the batch (integrator.rs:170-198) */
+} hb_sim_config;
+
+/* One dimension of a sweep (engine.rs:34-53). kind 0: subnetworks[a].params[b];
+ * kind 1: projections[a].coupling_params[b] (an extension: the reference cannot
+ * sweep coupling parameters). */
+typedef struct {
+    int32_t kind;
+    int32_t a;
+    int32_t b;
+    const float* values;
+    int32_t n_values;
+} hb_sweep_point;
+
+typedef struct {
+    int32_t n_sweep, n_nodes, nvar, n_steps;
+    float* tavg;             /* [n_sweep][n_nodes][nvar] */
+    float* final_states;     /* [n_sweep][n_nodes][nvar] */
+    float* trajectories;     /* [n_steps][n_sweep][n_nodes][nvar] or NULL */
+    float* bold;             /* [n_sweep][n_bold][n_nodes] or NULL (per instance) */
+    int32_t n_bold;
+    double elapsed_ms;       /* wall time of the run call (engine.rs:503,825) */
+    double kernel_ms;        /* integration kernel time measured with HIP events */
+} hb_result;
+
+typedef struct hb_engine hb_engine;
+
+/* BatchHybridEngine::from_config (engine.rs:145-149). device = HIP ordinal. */
+int hb_engine_create(const hb_sim_config* cfg, int32_t n_sweep, int32_t device, hb_engine** out);
+void hb_engine_destroy(hb_engine* e);
+/* BatchHybridEngine::run_sweep / run_sweep_with_trajectory (engine.rs:477-494).
+ * n_values must equal n_sweep (checked; the reference broadcasts unchecked). */
+int hb_run_sweep(hb_engine* e, int32_t sub_idx, int32_t param_idx, const float* values,
+                 int32_t n_values, int64_t n_steps, uint32_t flags, hb_result* out);
+/* BatchHybridEngine::run_sweep_multi (engine.rs:845-860): Cartesian product,
+ * last dimension fastest. */
+int hb_run_sweep_multi(hb_engine* e, const hb_sweep_point* points, int32_t n_points,
+                       int64_t n_steps, uint32_t flags, hb_result* out);
+void hb_result_free(hb_result* r);
+
+/* ---------------- instance-level API (ragged batches) ---------------- */
+
+/* One fully specified simulation. Arrays are host memory, read at plan time. */
+typedef struct {
+    int32_t model, integrator, engine, n_nodes, n_steps;
+    float dt;                 /* ms, f32 as the engine holds it (engine.rs:456) */
+    double dt_ms;             /* ms, f64 config value (BOLD window times) */
+    const float* params;
+    int32_t sweep_form;       /* 1: batch-native swept form of the model's sweep index (dfun.rs:49-72) */
+    float sweep_val;
+    int32_t node_param_idx;   /* -1, or a parameter given per node */
+    const float* node_param;  /* [n_nodes] */
+    const float* x0;          /* [n_nodes][nvar] */
+    const float* nsig;        /* [nvar] or NULL */
+    const float* noise;       /* optional explicit raw N(0,1) noise [n_steps][n_nodes][nvar] */
+    uint64_t noise_seed;      /* else Philox4x32-10(seed, noise_stream) */
+    uint64_t noise_stream;
+    int32_t wkind, cfn, n_cparams;
+    float cparams[8];
+    float scalar_w;
+    const float* dense;       /* [n_nodes][n_nodes], row = target */
+    const int32_t* indptr;
+    const int32_t* indices;
+    const float* weights;
+    const int32_t* edge_delays;
+    int32_t nnz;
+    int32_t delay_mode;
+    int32_t single_delay;
+    int32_t max_delay_hint;   /* ring horizon - 1 if larger than the delays read (BATCH_COMPAT) */
+    int32_t bold_period;      /* 0: off */
+    double bold_tr_ms;
+    /* outputs (host memory, caller-owned) */
+    float* tavg;              /* [n_nodes][nvar] */
+    float* final_state;       /* [n_nodes][nvar] */
+    float* traj;              /* [n_steps][n_nodes][nvar] or NULL */
+    float* bold;              /* [bold_cap][n_nodes] or NULL */
+    int32_t bold_cap;
+    int32_t* bold_n;
+} hb_instance;
+
+typedef struct {
+    int32_t device;
+    uint32_t flags;          /* HB_FLAG_EXACT, HB_FLAG_NO_GRAPH */
+    int32_t block_size;      /* 0: autotuned */
+    int32_t grid_waves;      /* persistent grid size in workgroups, 0: auto */
+} hb_run_opts;
+
+typedef struct {
+    double kernel_ms;        /* integration kernels, HIP events on the launch stream */
+    double total_ms;         /* plan run incl. noise generation, excl. uploads/downloads */
+    double node_steps;       /* sum over instances of n_nodes * n_steps */
+    int32_t n_launches;
+    int32_t block_size;
+    int32_t lds_bytes;       /* dynamic LDS per workgroup of the main launch */
+} hb_run_stats;
+
+typedef struct hb_plan hb_plan;
+
+/* Upload a ragged set of instances, pack them by cost, pick launch shapes and
+ * capture the launch sequence (noise fill + integration) in a hipGraph. */
+int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, hb_plan** out);
+/* Run the plan on device (inputs resident in HBM). Outputs stay on device. */
+int hb_plan_run(hb_plan* p, hb_run_stats* stats);
+/* Copy outputs to the host pointers given in the instances. */
+int hb_plan_download(hb_plan* p);
+void hb_plan_destroy(hb_plan* p);
+/* One-call convenience: create, run, download, destroy. */
+int hb_run_instances(const hb_instance* inst, int32_t n, const hb_run_opts* opts, hb_run_stats* stats);
+
+/* Philox4x32-10 standard normals (Box-Muller in f64), written as f32 to a host
+ * buffer: element k of stream s under key seed. Device-generated. */
+int hb_philox_normal(uint64_t seed, uint64_t stream, int64_t n, float* out_host, int32_t device);
+
+/* ---------------- misc ---------------- */
+const char* hb_last_error(void);   /* thread-local message of the last failure */
+const char* hb_version(void);
+int hb_device_count(void);
+/* Backend selection, the `--backend hip` value of select_backend (cli.rs:259-271).
+ * Returns "hip" for "hip"/"rocm"; anything else returns NULL and sets the last
+ * error: this library has no CPU fallback. */
+const char* hb_select_backend(const char* name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYBURN_HIP_H */

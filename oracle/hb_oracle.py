@@ -1,0 +1,295 @@
+"""ctypes wrapper of the CPU parity oracle (oracle/hb_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, where it is the checker or the timed CPU
+baseline; the product path never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libhb_oracle.so")
+
+fp = C.POINTER(C.c_float)
+ip = C.POINTER(C.c_int32)
+
+
+class OrcCoupling(C.Structure):
+    _fields_ = [("cfn", C.c_int32), ("n_cparams", C.c_int32), ("cparams", C.c_float * 8),
+                ("n_src", C.c_int32), ("cpu_form", C.c_int32), ("wkind", C.c_int32),
+                ("scalar_w", C.c_float), ("dense_w", fp), ("indptr", ip), ("indices", ip),
+                ("weights", fp), ("edge_delays", ip), ("delay_mode", C.c_int32),
+                ("single_delay", C.c_int32), ("max_delay", C.c_int32)]
+
+
+class OrcProblem(C.Structure):
+    _fields_ = [("model", C.c_int32), ("integrator", C.c_int32), ("engine", C.c_int32),
+                ("n_nodes", C.c_int32), ("n_steps", C.c_int32), ("dt", C.c_float),
+                ("dt_ms", C.c_double), ("params", fp), ("sweep_form", C.c_int32),
+                ("sweep_val", C.c_float), ("node_param_idx", C.c_int32), ("node_param", fp),
+                ("cpl", OrcCoupling), ("nsig", fp), ("noise", fp), ("x0", fp),
+                ("bold_period", C.c_int32), ("bold_cap", C.c_int32),
+                ("out_tavg", C.c_void_p), ("out_final", C.c_void_p), ("out_traj", C.c_void_p),
+                ("out_bold", C.c_void_p), ("out_bold_n", ip)]
+
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.hb_orc_run_f32.argtypes = [C.POINTER(OrcProblem)]
+        L.hb_orc_run_f64.argtypes = [C.POINTER(OrcProblem)]
+        L.hb_orc_run_many_f32.argtypes = [C.POINTER(OrcProblem), C.c_int32, C.c_int32]
+        L.hb_orc_dfun_f32.argtypes = [C.c_int, fp, C.c_int, C.c_float, fp, fp, fp]
+        L.hb_orc_dfun_f64.argtypes = [C.c_int, fp, C.c_int, C.c_float, C.POINTER(C.c_double),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.hb_orc_clamp_f32.argtypes = [C.c_int, fp]
+        L.hb_orc_coupling_f32.argtypes = [C.POINTER(OrcCoupling), C.c_int, C.c_int, C.c_int, fp, fp, fp]
+        L.hb_orc_csr_matvec_f32.argtypes = [fp, ip, ip, C.c_int, fp, fp]
+        L.hb_orc_bold_step_f32.argtypes = [C.c_int, fp, C.c_float, fp, fp, fp, fp]
+        L.hb_orc_bold_step_f64.argtypes = [C.c_int] + [C.POINTER(C.c_double)] + [C.c_double] + [C.POINTER(C.c_double)] * 4
+        L.hb_orc_bold_signal_f32.argtypes = [C.c_int, fp, fp, fp]
+        L.hb_orc_bold_downsample.argtypes = [C.POINTER(C.c_double), fp, C.c_int, C.c_int, C.c_double,
+                                             C.c_double, fp, C.c_int]
+        L.hb_orc_fc.argtypes = [fp, C.c_int, C.c_int, fp]
+        L.hb_orc_source_time.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64]
+        L.hb_orc_source_time.restype = C.c_int64
+        L.hb_orc_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return None if a is None else np.ascontiguousarray(a, np.float32)
+
+
+def _p(a, t=fp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def make_problem(inst, keep: list, noise=None, dtype=np.float32):
+    """Translate a hyburn_amd.Instance into the oracle's problem struct. The
+    coupling preprocessing (Difference rewrite) mirrors the reference's
+    construction (construction.rs:694-707, engine.rs:327-350)."""
+    from hyburn_amd import _lib as HL
+    P = OrcProblem()
+    nv = inst.nvar
+    k = lambda a: (keep.append(a), a)[1]
+    P.model, P.integrator, P.engine = inst.model, inst.integrator, inst.engine
+    P.n_nodes, P.n_steps = inst.n_nodes, inst.n_steps
+    P.dt = float(np.float32(inst.dt))
+    P.dt_ms = float(inst.dt if inst.dt_ms is None else inst.dt_ms)
+    P.params = _p(k(_f(inst.params)))
+    P.sweep_form, P.sweep_val = inst.sweep_form, float(inst.sweep_val)
+    P.node_param_idx = inst.node_param_idx
+    P.node_param = _p(k(_f(inst.node_param)))
+    P.nsig = _p(k(_f(inst.nsig)))
+    if inst.integrator >= HL.EULER_STOCH:
+        if noise is None:
+            noise = inst.noise
+        if noise is None:
+            from hyburn_amd.synth import philox_normal
+            noise = philox_normal(inst.noise_seed, inst.noise_stream, inst.n_steps * inst.n_nodes * nv)
+        P.noise = _p(k(_f(noise)))
+    P.x0 = _p(k(_f(np.asarray(inst.x0).reshape(inst.n_nodes, nv))))
+    c = P.cpl
+    cfn = inst.cfn
+    cp = list(inst.cparams)[:8]
+    dense = inst.dense
+    wkind = inst.wkind
+    scalar_w = inst.scalar_w
+    if cfn == HL.CF_DIFFERENCE:
+        a = cp[0] if cp else 1.0
+        cfn, cp = HL.CF_LINEAR, [a, 0.0]
+        if wkind in (HL.W_SCALAR_MEAN, HL.W_SCALAR_DIAG):
+            scalar_w = 0.0
+        elif wkind == HL.W_DENSE:
+            W = np.array(dense, np.float32, copy=True)
+            for i in range(W.shape[0]):
+                rs = np.float32(0.0)
+                for j in range(W.shape[1]):
+                    rs = np.float32(rs + W[i, j])
+                W[i, i] = np.float32(W[i, i] - rs)
+            dense = W
+    c.cfn = cfn
+    c.n_cparams = len(cp)
+    for i, v in enumerate(cp):
+        c.cparams[i] = float(v)
+    c.n_src = inst.n_nodes
+    c.wkind = wkind
+    c.scalar_w = float(scalar_w)
+    c.dense_w = _p(k(_f(dense)))
+    c.indptr = _p(k(None if inst.indptr is None else np.ascontiguousarray(inst.indptr, np.int32)), ip)
+    c.indices = _p(k(None if inst.indices is None else np.ascontiguousarray(inst.indices, np.int32)), ip)
+    c.weights = _p(k(_f(inst.weights)))
+    per_edge = wkind == HL.W_CSR and inst.delay_mode == HL.DELAY_PER_EDGE
+    if per_edge and inst.edge_delays is not None:
+        c.edge_delays = _p(k(np.ascontiguousarray(inst.edge_delays, np.int32)), ip)
+    dm = inst.delay_mode
+    if wkind != HL.W_CSR and dm == HL.DELAY_PER_EDGE:
+        dm = HL.DELAY_SINGLE
+    c.cpu_form = 1 if (per_edge and inst.edge_delays is not None) else 0
+    c.delay_mode = dm
+    c.single_delay = inst.single_delay
+    c.max_delay = inst.max_delay()
+    P.bold_period = inst.bold_period if inst.record_bold else 0
+    return P
+
+
+def run(inst, dtype=np.float32, noise=None, native_bold: bool = False):
+    """Run one instance on the oracle (FP32 twin or FP64 twin).
+    Returns dict(tavg, final_state, traj, bold) with BOLD resampled to TR as
+    bold_monitor.rs:123-203 does (native samples when native_bold)."""
+    keep: list = []
+    P = make_problem(inst, keep, noise=noise)
+    N, nv, T = inst.n_nodes, inst.nvar, inst.n_steps
+    tavg = np.zeros((N, nv), dtype)
+    fin = np.zeros((N, nv), dtype)
+    traj = np.zeros((T, N, nv), dtype) if inst.record_traj else None
+    nb_cap = (T // inst.bold_period + 1) if (inst.record_bold and inst.bold_period > 0) else 0
+    bold = np.zeros((max(nb_cap, 1), N), dtype) if nb_cap else None
+    nbn = np.zeros(1, np.int32)
+    P.out_tavg = tavg.ctypes.data
+    P.out_final = fin.ctypes.data
+    P.out_traj = traj.ctypes.data if traj is not None else None
+    P.out_bold = bold.ctypes.data if bold is not None else None
+    P.bold_cap = nb_cap
+    P.out_bold_n = nbn.ctypes.data_as(ip)
+    f = lib().hb_orc_run_f32 if dtype == np.float32 else lib().hb_orc_run_f64
+    rc = f(C.byref(P))
+    if rc != 0:
+        raise RuntimeError("oracle run failed")
+    out = {"tavg": tavg, "final_state": fin, "traj": traj, "bold": None}
+    if bold is not None:
+        nb = int(nbn[0])
+        native = bold[:nb]
+        out["bold_native"] = native
+        out["bold"] = native if native_bold else bold_resample(native, inst)
+    return out
+
+
+def bold_times(inst, n: int) -> np.ndarray:
+    """Native BOLD sample times (ms) as bold_monitor.rs:100,214 form them in f64."""
+    dt_ms = inst.dt if inst.dt_ms is None else inst.dt_ms
+    P, T = inst.bold_period, inst.n_steps
+    dtb = dt_ms * P
+    t = []
+    for k in range(n):
+        if k < T // P:
+            t.append(float(k) * dtb + dtb)
+        else:
+            t.append(float(k) * dtb + dt_ms * (T % P))
+    return np.array(t, np.float64)
+
+
+def bold_resample(native: np.ndarray, inst) -> np.ndarray:
+    n, N = native.shape
+    if n == 0:
+        return native
+    dt_ms = inst.dt if inst.dt_ms is None else inst.dt_ms
+    times = bold_times(inst, n)
+    cap = n * 64 + 1
+    out = np.zeros((cap, N), np.float32)
+    nat = np.ascontiguousarray(native, np.float32)
+    m = lib().hb_orc_bold_downsample(times.ctypes.data_as(C.POINTER(C.c_double)), _p(nat), n, N,
+                                     dt_ms * inst.bold_period, inst.bold_tr_ms, _p(out), cap)
+    return out[:m]
+
+
+def run_many_f32(insts, n_threads: int = 0) -> float:
+    """Run instances on host threads (OpenMP); returns wall seconds."""
+    import time
+    keep: list = []
+    arr = (OrcProblem * len(insts))()
+    outs = []
+    for i, I in enumerate(insts):
+        P = make_problem(I, keep)
+        N, nv = I.n_nodes, I.nvar
+        t = np.zeros((N, nv), np.float32)
+        f = np.zeros((N, nv), np.float32)
+        outs.append((t, f))
+        P.out_tavg = t.ctypes.data
+        P.out_final = f.ctypes.data
+        P.bold_period = 0
+        arr[i] = P
+    t0 = time.perf_counter()
+    rc = lib().hb_orc_run_many_f32(arr, len(insts), n_threads)
+    dt = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError("oracle batch failed")
+    return dt
+
+
+def dfun(model, params, x, c, sweep_form=0, sweep_val=0.0, dtype=np.float32):
+    x = np.ascontiguousarray(x, dtype)
+    c = np.ascontiguousarray(c, dtype)
+    p = np.ascontiguousarray(params, np.float32)
+    dx = np.zeros_like(x)
+    if dtype == np.float32:
+        lib().hb_orc_dfun_f32(model, _p(p), sweep_form, sweep_val, _p(x), _p(c), _p(dx))
+    else:
+        d = C.POINTER(C.c_double)
+        lib().hb_orc_dfun_f64(model, _p(p), sweep_form, sweep_val, x.ctypes.data_as(d),
+                              c.ctypes.data_as(d), dx.ctypes.data_as(d))
+    return dx
+
+
+def clamp(model, x):
+    x = np.array(x, np.float32, copy=True)
+    lib().hb_orc_clamp_f32(model, _p(x))
+    return x
+
+
+def coupling(cfn, cparams, x_src, x_tgt=None, dense=None, csr=None, n_src=None):
+    """One delay-free coupling evaluation post(W @ pre(x_src), x_tgt)."""
+    from hyburn_amd import _lib as HL
+    x_src = np.ascontiguousarray(x_src, np.float32)
+    ns, nc = x_src.shape
+    nt = dense.shape[0] if dense is not None else len(csr[0]) - 1
+    x_tgt = np.zeros((nt, nc), np.float32) if x_tgt is None else np.ascontiguousarray(x_tgt, np.float32)
+    cpl = OrcCoupling()
+    cpl.cfn = cfn
+    cpl.n_cparams = len(cparams)
+    for i, v in enumerate(cparams):
+        cpl.cparams[i] = v
+    cpl.n_src = ns if n_src is None else n_src
+    keep = []
+    if dense is not None:
+        d = np.ascontiguousarray(dense, np.float32)
+        keep.append(d)
+        cpl.wkind, cpl.dense_w = HL.W_DENSE, _p(d)
+    else:
+        ipt, ind, dat = (np.ascontiguousarray(csr[0], np.int32), np.ascontiguousarray(csr[1], np.int32),
+                         np.ascontiguousarray(csr[2], np.float32))
+        keep += [ipt, ind, dat]
+        cpl.wkind, cpl.indptr, cpl.indices, cpl.weights = HL.W_CSR, _p(ipt, ip), _p(ind, ip), _p(dat)
+    out = np.zeros((nt, nc), np.float32)
+    lib().hb_orc_coupling_f32(C.byref(cpl), nt, ns, nc, _p(x_src), _p(x_tgt), _p(out))
+    return out
+
+
+def fc(traj):
+    t = np.ascontiguousarray(traj, np.float32)
+    T, N = t.shape
+    out = np.zeros((N, N), np.float32)
+    lib().hb_orc_fc(_p(t), T, N, _p(out))
+    return out
+
+
+def source_time(mode, s, d, H):
+    return int(lib().hb_orc_source_time(mode, s, d, H))

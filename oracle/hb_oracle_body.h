@@ -1,0 +1,612 @@
+/*
+ * CPU restatement of hyburn's batched neural-mass integration path.
+ * TEST INFRASTRUCTURE ONLY: this is the parity oracle for the HIP library.
+ * It is included twice by hb_oracle.c, once with REAL=float (the FP32 twin,
+ * same op order as the reference's NdArray<f32> path) and once with
+ * REAL=double (the FP64 twin used to state each FP32 result's error).
+ *
+ * Every function cites the reference file:line it restates.
+ */
+
+/* ---- model constants, precomputed exactly as the reference does on host ---- */
+
+typedef struct {
+    int model;
+    /* G2DO  (src/engine/batch_engine/dfun.rs:271-319) */
+    REAL g_a, g_b, g_c, g_e, g_f, g_g, g_alpha, g_beta, g_gamma, g_i, g_dtau, g_dovt;
+    /* MPR   (dfun.rs:439-478) */
+    REAL m_delta, m_eta, m_j, m_iext, m_cr, m_cv, m_inv_tau, m_dr_c, m_pp, m_jtau, m_eta_i;
+    /* JR    (dfun.rs:321-384) */
+    REAL j_v0, j_r, j_2nu, j_Aa, j_2a, j_aa, j_a1J, j_a2J, j_a3J, j_a4J, j_Bb, j_2b, j_bb, j_mu;
+    /* WC    (dfun.rs:386-437) */
+    REAL w_cee, w_cei, w_cie, w_cii, w_inv_te, w_inv_ti, w_ae, w_be, w_ce, w_the, w_ai, w_bi,
+         w_ci, w_thi, w_re, w_ri, w_ke, w_ki, w_p, w_q, w_ale, w_ali, w_off_e, w_off_i;
+    /* Kuramoto (dfun.rs:480-492) */
+    REAL k_omega;
+    /* RWW   (dfun.rs:494-516) */
+    REAL r_a, r_b, r_d, r_gamma, r_inv_taus, r_wjn, r_jn, r_io;
+    int sweep_form;   /* 1: reference's batch-native swept form (dfun.rs:49-72) */
+    REAL sweep_val;
+} FN(mconst);
+
+/* Build the constants from a parameter vector (f32 values, as in the config).
+ * The FP32 twin evaluates each host-side constant in f32 with the reference's
+ * operation order; the FP64 twin uses the same expressions in double. */
+static void FN(mconst_init)(FN(mconst)* k, int model, const float* pf, int sweep_form, float sweep_val)
+{
+    REAL p[32];
+    int i;
+    for (i = 0; i < 32; ++i) p[i] = 0;
+    for (i = 0; i < hb_orc_nparams(model); ++i) p[i] = (REAL)pf[i];
+    memset(k, 0, sizeof(*k));
+    k->model = model;
+    k->sweep_form = sweep_form;
+    k->sweep_val = (REAL)sweep_val;
+    switch (model) {
+    case ORC_G2DO: { /* dfun.rs:277-293 */
+        REAL tau = p[0] > EPSR ? p[0] : EPSR;
+        k->g_i = p[1]; k->g_a = p[2]; k->g_b = p[3]; k->g_c = p[4];
+        k->g_e = p[6]; k->g_f = p[7]; k->g_g = p[8];
+        k->g_alpha = p[9]; k->g_beta = p[10]; k->g_gamma = p[11];
+        k->g_dtau = p[5] * tau;
+        k->g_dovt = p[5] / tau;
+        break; }
+    case ORC_MPR: { /* dfun.rs:445-456 */
+        REAL tau = p[0] > EPSR ? p[0] : EPSR;
+        REAL pi = PIR;
+        k->m_delta = p[1]; k->m_eta = p[2]; k->m_j = p[3]; k->m_iext = p[4];
+        k->m_cr = p[5]; k->m_cv = p[6];
+        k->m_inv_tau = (REAL)1 / tau;
+        k->m_dr_c = k->m_delta / (pi * tau);
+        k->m_pp = pi * pi * tau * tau;
+        k->m_jtau = k->m_j * tau;
+        k->m_eta_i = k->m_eta + k->m_iext;
+        break; }
+    case ORC_JR: { /* dfun.rs:327-366 */
+        REAL Ap = p[0], Bp = p[1], a = p[2], b = p[3], J = p[7];
+        k->j_v0 = p[4]; k->j_r = p[6];
+        k->j_2nu = (REAL)2 * p[5];
+        k->j_Aa = Ap * a; k->j_2a = (REAL)2 * a; k->j_aa = a * a;
+        k->j_a1J = p[8] * J; k->j_a2J = p[9] * J; k->j_a3J = p[10] * J; k->j_a4J = p[11] * J;
+        k->j_Bb = Bp * b; k->j_2b = (REAL)2 * b; k->j_bb = b * b;
+        k->j_mu = p[12];
+        break; }
+    case ORC_WC: { /* dfun.rs:392-403 */
+        REAL te = p[4] > EPSR ? p[4] : EPSR;
+        REAL ti = p[5] > EPSR ? p[5] : EPSR;
+        k->w_cee = p[0]; k->w_cei = p[1]; k->w_cie = p[2]; k->w_cii = p[3];
+        k->w_inv_te = (REAL)1 / te; k->w_inv_ti = (REAL)1 / ti;
+        k->w_ae = p[6]; k->w_be = p[7]; k->w_ce = p[8]; k->w_the = p[9];
+        k->w_ai = p[10]; k->w_bi = p[11]; k->w_ci = p[12]; k->w_thi = p[13];
+        k->w_re = p[14]; k->w_ri = p[15]; k->w_ke = p[16]; k->w_ki = p[17];
+        k->w_p = p[18]; k->w_q = p[19]; k->w_ale = p[20]; k->w_ali = p[21];
+        k->w_off_e = (REAL)1 / ((REAL)1 + EXPR(k->w_ae * k->w_be));
+        k->w_off_i = (REAL)1 / ((REAL)1 + EXPR(k->w_ai * k->w_bi));
+        break; }
+    case ORC_KUR:
+        k->k_omega = p[0];
+        break;
+    case ORC_RWW: /* dfun.rs:500-502 */
+        k->r_a = p[0]; k->r_b = p[1]; k->r_d = p[2]; k->r_gamma = p[3];
+        k->r_inv_taus = (REAL)1 / p[4];
+        k->r_wjn = p[5] * p[6]; k->r_jn = p[6]; k->r_io = p[7];
+        break;
+    }
+}
+
+/* JR sigmoid, dfun.rs:350-354: 2*nu_max / (exp((v0 - v)*r) + 1). */
+static REAL FN(jr_sigm)(const FN(mconst)* k, REAL v)
+{
+    REAL arg = (-v + k->j_v0) * k->j_r;
+    REAL den = EXPR(arg) + (REAL)1;
+    return k->j_2nu / den;
+}
+
+/* Derivative of one node: x[NV], c[NC] -> dx[NV].
+ * Op order follows the tensor expressions in dfun.rs exactly. */
+static void FN(dfun)(const FN(mconst)* k, const REAL* x, const REAL* c, REAL* dx)
+{
+    switch (k->model) {
+    case ORC_G2DO: { /* dfun.rs:295-318 */
+        REAL v = x[0], w = x[1];
+        REAL iext = k->sweep_form ? (c[0] + k->sweep_val) * k->g_gamma
+                                  : c[0] * k->g_gamma + k->g_gamma * k->g_i;
+        REAL v2 = v * v;
+        REAL v3 = v * v2;
+        dx[0] = ((((w * k->g_alpha + iext) - v3 * k->g_f) + v2 * k->g_e) + v * k->g_g) * k->g_dtau;
+        dx[1] = (((v * k->g_b + v2 * k->g_c) - w * k->g_beta) + k->g_a) * k->g_dovt;
+        break; }
+    case ORC_MPR: { /* dfun.rs:455-477 */
+        REAL r = x[0], v = x[1];
+        REAL dv, v2, r2;
+        dx[0] = ((r * v) * (REAL)2 + k->m_dr_c) * k->m_inv_tau;
+        v2 = v * v;
+        r2 = r * r;
+        if (k->sweep_form)
+            dv = (((((v2 - r2 * k->m_pp) + k->m_eta) + k->sweep_val) + r * k->m_jtau) + c[0] * k->m_cr) + c[1] * k->m_cv;
+        else
+            dv = ((((v2 - r2 * k->m_pp) + k->m_eta_i) + r * k->m_jtau) + c[0] * k->m_cr) + c[1] * k->m_cv;
+        dx[1] = dv * k->m_inv_tau;
+        break; }
+    case ORC_JR: { /* dfun.rs:356-383 */
+        REAL s12 = FN(jr_sigm)(k, x[1] - x[2]);
+        REAL s01 = FN(jr_sigm)(k, x[0] * k->j_a1J);
+        REAL s03 = FN(jr_sigm)(k, x[0] * k->j_a3J);
+        REAL mu = k->sweep_form ? k->sweep_val : k->j_mu;
+        REAL mut = (s01 * k->j_a2J + mu) + c[0];
+        dx[0] = x[3]; dx[1] = x[4]; dx[2] = x[5];
+        dx[3] = (s12 * k->j_Aa - x[3] * k->j_2a) - x[0] * k->j_aa;
+        dx[4] = (mut * k->j_Aa - x[4] * k->j_2a) - x[1] * k->j_aa;
+        dx[5] = ((s03 * k->j_a4J) * k->j_Bb - x[5] * k->j_2b) - x[2] * k->j_bb;
+        break; }
+    case ORC_WC: { /* dfun.rs:405-436 */
+        REAL e = x[0], i = x[1];
+        REAL xe = ((e * k->w_cee - i * k->w_cei) - k->w_the) + c[0];
+        REAL xi, ee, ei, se, si;
+        xe = (xe + (k->sweep_form ? k->sweep_val : k->w_p)) * k->w_ale;
+        xi = (((e * k->w_cie - i * k->w_cii) + k->w_q) - k->w_thi) * k->w_ali;
+        ee = EXPR((xe + (-k->w_be)) * (-k->w_ae));
+        se = ((REAL)1 / (ee + (REAL)1) - k->w_off_e) * k->w_ce;
+        ei = EXPR((xi + (-k->w_bi)) * (-k->w_ai));
+        si = ((REAL)1 / (ei + (REAL)1) - k->w_off_i) * k->w_ci;
+        dx[0] = (-e + (e * (-k->w_re) + k->w_ke) * se) * k->w_inv_te;
+        dx[1] = (-i + (i * (-k->w_ri) + k->w_ki) * si) * k->w_inv_ti;
+        break; }
+    case ORC_KUR: /* dfun.rs:486-491 */
+        dx[0] = c[0] + (k->sweep_form ? k->sweep_val : k->k_omega);
+        break;
+    case ORC_RWW: { /* dfun.rs:503-515 */
+        REAL s = x[0];
+        REAL xx = k->sweep_form ? (s * k->r_wjn + k->sweep_val) + c[0] * k->r_jn
+                                : (s * k->r_wjn + k->r_io) + c[0] * k->r_jn;
+        REAL axb = xx * k->r_a + (-k->r_b);
+        REAL et = EXPR(axb * (-k->r_d));
+        REAL den = -et + (REAL)1;
+        REAL h = axb / den;
+        dx[0] = (-s) * k->r_inv_taus + ((-s + (REAL)1) * h) * k->r_gamma;
+        break; }
+    }
+}
+
+/* clamp_batch, dfun.rs:170-239 (six hot models). */
+static void FN(clamp)(int model, REAL* x)
+{
+    switch (model) {
+    case ORC_WC:
+        x[0] = x[0] < 0 ? 0 : (x[0] > 1 ? 1 : x[0]);
+        x[1] = x[1] < 0 ? 0 : (x[1] > 1 ? 1 : x[1]);
+        break;
+    case ORC_MPR:
+        x[0] = x[0] < 0 ? 0 : x[0];
+        break;
+    case ORC_RWW:
+        x[0] = x[0] < 0 ? 0 : (x[0] > 1 ? 1 : x[0]);
+        break;
+    case ORC_KUR: { /* dfun.rs:188-191 */
+        REAL tp = (REAL)2 * PIR;
+        x[0] = x[0] - FLOORR(x[0] / tp) * tp;
+        break; }
+    default:
+        break;
+    }
+}
+
+/* ---- coupling pipeline: coupling.rs:204-345, 360-399 ---- */
+
+typedef struct {
+    int cfn;
+    REAL a, b, cmin, cmax, mid, sigma, r, e0, v0, h, q, g, p, theta;
+    REAL kur_scale;   /* a / n_src, coupling.rs:370 */
+    int classic;      /* SigmoidalJansenRit use_classic */
+    int cpu_form;     /* pre_cpu op order (runtime.rs per-edge path) */
+} FN(cconst);
+
+static void FN(cconst_init)(FN(cconst)* k, const hb_orc_coupling* c)
+{
+    const float* P = c->cparams;
+    int n = c->n_cparams;
+    memset(k, 0, sizeof(*k));
+    k->cfn = c->cfn;
+    k->cpu_form = c->cpu_form;
+    /* parameter parsing: coupling.rs:114-202 */
+    switch (c->cfn) {
+    case ORC_CF_LINEAR: case ORC_CF_SCALED_LINEAR:
+        k->a = n > 0 ? P[0] : 1; k->b = n > 1 ? P[1] : 0; break;
+    case ORC_CF_DIFFERENCE:
+        k->a = n > 0 ? P[0] : 1; break;
+    case ORC_CF_KURAMOTO:
+        k->a = n > 0 ? P[0] : 1;
+        k->kur_scale = (REAL)((float)k->a / (float)c->n_src);
+        if (sizeof(REAL) == 8) k->kur_scale = k->a / (REAL)c->n_src;
+        break;
+    case ORC_CF_SIGMOIDAL:
+        if (n >= 5) { k->cmin = P[0]; k->cmax = P[1]; k->mid = P[2]; k->a = P[3]; k->sigma = P[4]; }
+        else { k->cmin = -1; k->cmax = n > 0 ? P[0] : 1; k->mid = n > 1 ? P[1] : 0; k->sigma = n > 2 ? P[2] : 1; k->a = 1; }
+        break;
+    case ORC_CF_TANH:
+        k->a = n > 0 ? P[0] : 1; k->b = n > 1 ? P[1] : 1; k->mid = n > 2 ? P[2] : 0; k->sigma = n > 3 ? P[3] : 1;
+        break;
+    case ORC_CF_SJR:
+        if (n >= 5) { k->classic = 1; k->a = P[0]; k->cmin = P[1]; k->cmax = P[2]; k->r = P[3]; k->mid = P[4]; k->e0 = (REAL)0.005f; k->v0 = 6; }
+        else { k->classic = 0; k->a = n > 0 ? P[0] : 1; k->e0 = n > 1 ? P[1] : (REAL)0.005f; k->r = n > 2 ? P[2] : (REAL)0.56f; k->v0 = n > 3 ? P[3] : 6; k->cmin = 0; k->cmax = (REAL)0.005f; k->mid = 6; }
+        break;
+    case ORC_CF_PRESIGMOIDAL:
+        k->h = n > 0 ? P[0] : 1; k->q = n > 1 ? P[1] : 1; k->g = n > 2 ? P[2] : 1; k->p = n > 3 ? P[3] : 1; k->theta = n > 4 ? P[4] : (REAL)0.5f;
+        break;
+    }
+}
+
+/* pre(): one source node, nc cvars in -> npre channels out.
+ * Tensor form coupling.rs:204-258; CPU form (per-edge path) 268-321. */
+static void FN(pre)(const FN(cconst)* k, const REAL* x, int nc, REAL* out)
+{
+    int c;
+    switch (k->cfn) {
+    case ORC_CF_KURAMOTO:
+        /* tensor form: cat([sin x, cos x], 1) (channel-major);
+         * per-edge CPU form interleaves (sin,cos) per cvar. Identical for nc=1. */
+        for (c = 0; c < nc; ++c) { out[c] = SINR(x[c]); out[nc + c] = COSR(x[c]); }
+        break;
+    case ORC_CF_TANH:
+        for (c = 0; c < nc; ++c) {
+            REAL inner = (x[c] * k->b + (-k->mid)) / k->sigma;
+            out[c] = (TANHR(inner) + (REAL)1) * k->a;
+        }
+        break;
+    case ORC_CF_SJR: {
+        REAL diff = x[0] - (nc >= 2 ? x[1] : (REAL)0);
+        if (k->classic) {
+            REAL den = EXPR((diff + (-k->mid)) * (-k->r)) + (REAL)1;
+            out[0] = k->cpu_form ? k->cmin + (k->cmax - k->cmin) / den
+                                 : ((REAL)1 / den) * (k->cmax - k->cmin) + k->cmin;
+        } else {
+            REAL den = EXPR((diff + (-k->v0)) * (-k->r)) + (REAL)1;
+            out[0] = k->cpu_form ? ((k->a * (REAL)2) * k->e0) / den
+                                 : ((REAL)1 / den) * ((k->a * (REAL)2) * k->e0);
+        }
+        break; }
+    case ORC_CF_PRESIGMOIDAL:
+        for (c = 0; c < nc; ++c) {
+            REAL inner = x[c] * k->p + (-k->theta);
+            out[c] = (TANHR(inner * k->g) + k->q) * k->h;
+        }
+        break;
+    default: /* Linear, Sigmoidal, ScaledLinear, Difference: identity */
+        for (c = 0; c < nc; ++c) out[c] = x[c];
+        break;
+    }
+}
+
+static int FN(npre)(const FN(cconst)* k, int nc)
+{
+    if (k->cfn == ORC_CF_KURAMOTO) return 2 * nc;
+    if (k->cfn == ORC_CF_SJR) return 1;
+    return nc;
+}
+
+/* post_with_target: coupling.rs:323-399. g[npre] -> out[nc]; xi = target cvars. */
+static void FN(post)(const FN(cconst)* k, const REAL* g, const REAL* xi, REAL rowsum, int nc, REAL* out)
+{
+    int c;
+    switch (k->cfn) {
+    case ORC_CF_LINEAR:
+        for (c = 0; c < nc; ++c) out[c] = g[c] * k->a + k->b;
+        break;
+    case ORC_CF_SCALED_LINEAR:
+        for (c = 0; c < nc; ++c) out[c] = (g[c] + (-k->b)) * k->a;
+        break;
+    case ORC_CF_DIFFERENCE: /* non-square rowsum form, coupling.rs:376-380 */
+        for (c = 0; c < nc; ++c) out[c] = (g[c] - xi[c] * rowsum) * k->a;
+        break;
+    case ORC_CF_KURAMOTO: /* coupling.rs:367-374 */
+        for (c = 0; c < nc; ++c)
+            out[c] = (COSR(xi[c]) * g[c] - SINR(xi[c]) * g[nc + c]) * k->kur_scale;
+        break;
+    case ORC_CF_SIGMOIDAL: /* coupling.rs:326-331 */
+        for (c = 0; c < nc; ++c) {
+            REAL sh = (g[c] + (-k->mid)) / k->sigma;
+            REAL den = EXPR(-(sh * k->a)) + (REAL)1;
+            out[c] = ((REAL)1 / den) * (k->cmax - k->cmin) + k->cmin;
+        }
+        break;
+    case ORC_CF_SJR:
+        if (k->classic) { for (c = 0; c < nc; ++c) out[c] = g[c < 1 ? c : 0] * k->a; }
+        else { for (c = 0; c < nc; ++c) out[c] = g[c < 1 ? c : 0]; }
+        break;
+    default: /* Tanh, PreSigmoidal: identity */
+        for (c = 0; c < nc; ++c) out[c] = g[c];
+        break;
+    }
+}
+
+
+/* ---- Balloon-Windkessel: bold.rs:106-166, bold_monitor.rs:72-107 ---- */
+
+/* One forward-Euler BW step of size dt_s seconds for nodes [0,N). bold.rs:106-150 */
+static void FN(bold_step)(int N, const REAL* z_in, REAL dt_s, REAL* S, REAL* F, REAL* V, REAL* Q)
+{
+    const REAL tau_s = (REAL)1.54f, tau_f = (REAL)0.50f, tau_0 = (REAL)1.02f;
+    const REAL alpha = (REAL)0.32f, e0 = (REAL)0.34f;
+    const REAL kappa = (REAL)1 / tau_s, gamma = (REAL)1 / tau_f, tau_inv = (REAL)1 / tau_0;
+    const REAL ialpha = (REAL)1 / alpha;
+    int i;
+    for (i = 0; i < N; ++i) {
+        REAL z = z_in[i] > 0 ? z_in[i] : 0;
+        REAL s = S[i], f = F[i], v = V[i], q = Q[i];
+        REAL fs = f > (REAL)1e-6f ? f : (REAL)1e-6f;
+        REAL vs = v > (REAL)1e-6f ? v : (REAL)1e-6f;
+        REAL ds = (z - kappa * s) - gamma * (fs - (REAL)1);
+        REAL df = s;
+        REAL dv = tau_inv * (fs - POWR(vs, ialpha));
+        REAL dq = tau_inv * ((fs * ((REAL)1 - POWR((REAL)1 - e0, (REAL)1 / fs))) / e0
+                             - POWR(vs, ialpha) * (q / vs));
+        REAL sn = s + dt_s * ds;
+        REAL fn = fs + dt_s * df;
+        REAL vn = vs + dt_s * dv;
+        REAL qn = q + dt_s * dq;
+        S[i] = sn > 0 ? sn : 0;
+        F[i] = fn > (REAL)1e-6f ? fn : (REAL)1e-6f;
+        V[i] = vn > (REAL)1e-6f ? vn : (REAL)1e-6f;
+        Q[i] = qn > 0 ? qn : 0;
+    }
+}
+
+/* BOLD signal, bold.rs:153-166 */
+static REAL FN(bold_signal1)(REAL v, REAL q)
+{
+    const REAL V0 = 4, k1 = 7, k2 = 2, k3 = 2;
+    REAL vv = v > (REAL)1e-6f ? v : (REAL)1e-6f;
+    return V0 * ((k1 * ((REAL)1 - q) + k2 * ((REAL)1 - q / vv)) + k3 * ((REAL)1 - vv));
+}
+
+/* Close one averaging window of `count` neural steps: average, one BW step of
+ * dt*count ms, then the signal (bold_monitor.rs:86-106). */
+static void FN(bold_window)(const hb_orc_problem* P, int N, const REAL* acc, int count,
+                            REAL* S, REAL* F, REAL* V, REAL* Q, REAL* out)
+{
+    REAL* avg = (REAL*)malloc(sizeof(REAL) * (size_t)N);
+    int i;
+    double dt_bold = P->dt_ms * (double)count;
+    REAL dt_s = (REAL)(dt_bold / 1000.0);
+    if (!avg) return;
+    for (i = 0; i < N; ++i) avg[i] = acc[i] / (REAL)count;
+    FN(bold_step)(N, avg, dt_s, S, F, V, Q);
+    for (i = 0; i < N; ++i) out[i] = FN(bold_signal1)(V[i], Q[i]);
+    free(avg);
+}
+
+/* ---- the time loop: engine.rs:540-804 (batch) / runtime.rs:46-501 (plain) ---- */
+
+/* Run one instance. Outputs are REAL arrays (float for the FP32 twin). */
+static int FN(run)(const hb_orc_problem* P, REAL* tavg, REAL* fin, REAL* traj,
+                   REAL* bold, int32_t* bold_n)
+{
+    const int model = P->model, N = P->n_nodes, T = P->n_steps;
+    const int NV = hb_orc_nvar(model), NC = hb_orc_ncvar(model);
+    const REAL dt = (REAL)P->dt;
+    FN(mconst) kbase;
+    FN(mconst)* kn = NULL;   /* per-node constants when a node parameter is given */
+    FN(cconst) cc;
+    int npre, H, t, n, v, e;
+    REAL *x = NULL, *hist = NULL, *pre0 = NULL, *cpl = NULL, *tmp = NULL;
+    REAL nscale[8];
+    REAL *bw_s = NULL, *bw_f = NULL, *bw_v = NULL, *bw_q = NULL, *bw_acc = NULL;
+    int bw_count = 0, nb = 0;
+    const int stoch = P->integrator >= ORC_EULER_STOCH;
+    int rc = 0;
+
+    FN(mconst_init)(&kbase, model, P->params, P->sweep_form, P->sweep_val);
+    if (P->node_param_idx >= 0 && P->node_param) {
+        float pp[32];
+        kn = (FN(mconst)*)malloc(sizeof(FN(mconst)) * (size_t)N);
+        for (n = 0; n < N; ++n) {
+            memcpy(pp, P->params, sizeof(float) * (size_t)hb_orc_nparams(model));
+            pp[P->node_param_idx] = P->node_param[n];
+            FN(mconst_init)(&kn[n], model, pp, P->sweep_form, P->sweep_val);
+        }
+    }
+    FN(cconst_init)(&cc, &P->cpl);
+    npre = FN(npre)(&cc, NC);
+
+    /* history of pre(x_t) for every t, [T+1][N][npre]: the delayed read for
+     * delay d at step s is pre(x_{s-d}), runtime.rs:55-85,176 */
+    H = P->cpl.max_delay;
+    x = (REAL*)calloc((size_t)N * NV, sizeof(REAL));
+    hist = (REAL*)calloc((size_t)(H + 1) * N * npre, sizeof(REAL));
+    pre0 = (REAL*)calloc((size_t)npre, sizeof(REAL));
+    cpl = (REAL*)calloc((size_t)N * NC, sizeof(REAL));
+    tmp = (REAL*)calloc((size_t)npre * 4 + 64, sizeof(REAL));
+    if (!x || !hist || !pre0 || !cpl || !tmp) { rc = -1; goto done; }
+    {   /* pre(0): the value a single-delay read of "not yet simulated" time yields */
+        REAL z[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        FN(pre)(&cc, z, NC, pre0);
+    }
+    for (n = 0; n < N * NV; ++n) x[n] = (REAL)P->x0[n];
+    for (n = 0; n < N * NV; ++n) tavg[n] = 0;
+    for (v = 0; v < NV; ++v) {
+        float nsf = P->nsig ? P->nsig[v] : 0.0f;
+        /* integrator.rs:179-191: sqrt(2*nsig) * sqrt(dt) */
+        nscale[v] = SQRTR((REAL)2 * (REAL)nsf) * SQRTR(dt);
+    }
+    if (P->bold_period > 0) {
+        bw_s = (REAL*)calloc((size_t)N, sizeof(REAL));
+        bw_f = (REAL*)calloc((size_t)N, sizeof(REAL));
+        bw_v = (REAL*)calloc((size_t)N, sizeof(REAL));
+        bw_q = (REAL*)calloc((size_t)N, sizeof(REAL));
+        bw_acc = (REAL*)calloc((size_t)N, sizeof(REAL));
+        for (n = 0; n < N; ++n) { bw_f[n] = 1; bw_v[n] = 1; bw_q[n] = 1; }
+    }
+
+    {
+    const int Hd = H + 1;   /* reference horizon: max_delay + 1 */
+    for (t = 0; t < T; ++t) {
+        const int ws = t % Hd;
+        REAL* cur = hist + (size_t)ws * N * npre;
+        /* 0. history write of the current state (pre applied) */
+        for (n = 0; n < N; ++n) {
+            REAL xc[8];
+            for (v = 0; v < NC; ++v) xc[v] = x[n * NV + v];
+            FN(pre)(&cc, xc, NC, cur + (size_t)n * npre);
+        }
+        /* 1. coupling */
+        for (n = 0; n < N * NC; ++n) cpl[n] = 0;
+        if (P->cpl.wkind != ORC_W_NONE) {
+            REAL* g = tmp;           /* [npre] */
+            REAL mf[8];
+            const REAL* src_single = cur;
+            int single_zero = 0;
+            if (P->cpl.delay_mode != ORC_DELAY_PER_EDGE) {
+                int d = P->cpl.single_delay;
+                int teff;
+                if (d == 0 || t == 0) teff = t;
+                else if (P->cpl.delay_mode == ORC_DELAY_BATCH_COMPAT) {
+                    /* engine.rs:541-589: sliding window indexed as a ring */
+                    if (d > t) teff = -1;
+                    else teff = t - Hd + 1 + ((t - d + Hd) % Hd);
+                    (void)0;
+                } else teff = d > t ? -1 : t - d;
+                if (teff < 0) single_zero = 1;
+                else src_single = hist + (size_t)(teff % Hd) * N * npre;
+            }
+            if (P->cpl.wkind == ORC_W_SCALAR_MEAN) {
+                /* engine.rs:602-608: w * mean_j pre(x_j), broadcast */
+                int c;
+                for (c = 0; c < npre; ++c) {
+                    REAL s = 0;
+                    for (n = 0; n < N; ++n) s += single_zero ? pre0[c] : src_single[(size_t)n * npre + c];
+                    mf[c] = (s / (REAL)N) * (REAL)P->cpl.scalar_w;
+                }
+            }
+            for (n = 0; n < N; ++n) {
+                int c;
+                REAL xi[8];
+                REAL rowsum = 0;
+                for (c = 0; c < npre; ++c) g[c] = 0;
+                for (c = 0; c < NC; ++c) xi[c] = x[n * NV + c];
+                switch (P->cpl.wkind) {
+                case ORC_W_SCALAR_MEAN:
+                    for (c = 0; c < npre; ++c) g[c] = mf[c];
+                    break;
+                case ORC_W_SCALAR_DIAG: /* construction.rs:639-655: s * I */
+                    for (c = 0; c < npre; ++c)
+                        g[c] = (REAL)P->cpl.scalar_w * (single_zero ? pre0[c] : src_single[(size_t)n * npre + c]);
+                    break;
+                case ORC_W_DENSE: { /* coupling.rs:465-480: W @ pre(x) */
+                    int j;
+                    const float* wr = P->cpl.dense_w + (size_t)n * N;
+                    for (j = 0; j < N; ++j) {
+                        for (c = 0; c < npre; ++c)
+                            g[c] += (REAL)wr[j] * (single_zero ? pre0[c] : src_single[(size_t)j * npre + c]);
+                        rowsum += (REAL)wr[j];
+                    }
+                    break; }
+                case ORC_W_CSR: {
+                    for (e = P->cpl.indptr[n]; e < P->cpl.indptr[n + 1]; ++e) {
+                        const int j = P->cpl.indices[e];
+                        const REAL w = (REAL)P->cpl.weights[e];
+                        const REAL* src;
+                        if (P->cpl.delay_mode == ORC_DELAY_PER_EDGE) {
+                            /* runtime.rs:162-204 */
+                            const int d = P->cpl.edge_delays ? P->cpl.edge_delays[e] : 0;
+                            if (d == 0 || t == 0) src = cur + (size_t)j * npre;
+                            else if (d > t) src = NULL;     /* exact zero contribution */
+                            else src = hist + (size_t)((t - d) % Hd) * N * npre + (size_t)j * npre;
+                            for (c = 0; c < npre; ++c) g[c] += w * (src ? src[c] : (REAL)0);
+                        } else {
+                            src = single_zero ? pre0 : src_single + (size_t)j * npre;
+                            for (c = 0; c < npre; ++c) g[c] += w * src[c];
+                        }
+                        rowsum += w;
+                    }
+                    break; }
+                }
+                FN(post)(&cc, g, xi, rowsum, NC, cpl + (size_t)n * NC);
+            }
+        }
+        /* 2. integrate every node: engine.rs:676-773 */
+        for (n = 0; n < N; ++n) {
+            const FN(mconst)* k = kn ? &kn[n] : &kbase;
+            REAL* xn = x + (size_t)n * NV;
+            const REAL* cn = cpl + (size_t)n * NC;
+            REAL k1[8], k2[8], k3[8], k4[8], s[8], z[8];
+            if (stoch) {
+                const float* raw = P->noise + ((size_t)t * N + n) * NV;
+                for (v = 0; v < NV; ++v) z[v] = (REAL)raw[v] * nscale[v];
+            }
+            FN(dfun)(k, xn, cn, k1);
+            switch (P->integrator) {
+            case ORC_EULER:
+                for (v = 0; v < NV; ++v) xn[v] = xn[v] + k1[v] * dt;
+                break;
+            case ORC_EULER_STOCH:
+                for (v = 0; v < NV; ++v) xn[v] = (xn[v] + k1[v] * dt) + z[v];
+                break;
+            case ORC_HEUN: case ORC_HEUN_STOCH:
+                for (v = 0; v < NV; ++v) s[v] = stoch ? (xn[v] + k1[v] * dt) + z[v] : xn[v] + k1[v] * dt;
+                FN(clamp)(model, s);
+                FN(dfun)(k, s, cn, k2);
+                for (v = 0; v < NV; ++v) {
+                    REAL y = xn[v] + (k1[v] + k2[v]) * (dt * (REAL)0.5);
+                    xn[v] = stoch ? y + z[v] : y;
+                }
+                break;
+            case ORC_RK4: case ORC_RK4_STOCH:
+                for (v = 0; v < NV; ++v) s[v] = xn[v] + k1[v] * (dt / (REAL)2);
+                FN(clamp)(model, s);
+                FN(dfun)(k, s, cn, k2);
+                for (v = 0; v < NV; ++v) s[v] = xn[v] + k2[v] * (dt / (REAL)2);
+                FN(clamp)(model, s);
+                FN(dfun)(k, s, cn, k3);
+                for (v = 0; v < NV; ++v) s[v] = xn[v] + k3[v] * dt;
+                FN(clamp)(model, s);
+                FN(dfun)(k, s, cn, k4);
+                for (v = 0; v < NV; ++v)
+                    xn[v] = xn[v] + (((k1[v] + k2[v] * (REAL)2) + k3[v] * (REAL)2) + k4[v]) * (dt / (REAL)6);
+                if (P->integrator == ORC_RK4_STOCH) {
+                    /* batch: clamp(det + Z) (engine.rs:747-765);
+                     * plain: clamp(det) + Z, unclamped (integrator.rs:296-315) */
+                    if (P->engine == ORC_ENGINE_PLAIN) {
+                        FN(clamp)(model, xn);
+                        for (v = 0; v < NV; ++v) xn[v] = xn[v] + z[v];
+                        goto no_final_clamp;
+                    }
+                    for (v = 0; v < NV; ++v) xn[v] = xn[v] + z[v];
+                }
+                break;
+            }
+            FN(clamp)(model, xn);
+        no_final_clamp:
+            for (v = 0; v < NV; ++v) tavg[n * NV + v] += xn[v];
+        }
+        if (traj) memcpy(traj + (size_t)t * N * NV, x, sizeof(REAL) * (size_t)N * NV);
+        /* 3. BOLD: engine.rs:785-801 -> bold_monitor.rs:72-107 (per instance) */
+        if (P->bold_period > 0) {
+            for (n = 0; n < N; ++n) bw_acc[n] += x[n * NV];
+            bw_count++;
+            if (bw_count >= P->bold_period) {
+                if (nb < P->bold_cap)
+                    FN(bold_window)(P, N, bw_acc, bw_count, bw_s, bw_f, bw_v, bw_q, bold + (size_t)nb * N);
+                nb++;
+                bw_count = 0;
+                for (n = 0; n < N; ++n) bw_acc[n] = 0;
+            }
+        }
+    }
+    }
+    if (P->bold_period > 0 && bw_count > 0) { /* flush, bold_monitor.rs:207-221 */
+        if (nb < P->bold_cap)
+            FN(bold_window)(P, N, bw_acc, bw_count, bw_s, bw_f, bw_v, bw_q, bold + (size_t)nb * N);
+        nb++;
+    }
+    if (bold_n) *bold_n = nb;
+    {   /* engine.rs:813-818: tavg * (1/n_steps) */
+        REAL inv = (REAL)1 / (REAL)T;
+        if (sizeof(REAL) == 4) inv = (REAL)(1.0f / (float)T);
+        for (n = 0; n < N * NV; ++n) tavg[n] = tavg[n] * inv;
+    }
+    memcpy(fin, x, sizeof(REAL) * (size_t)N * NV);
+done:
+    free(x); free(hist); free(pre0); free(cpl); free(tmp); free(kn);
+    free(bw_s); free(bw_f); free(bw_v); free(bw_q); free(bw_acc);
+    return rc;
+}

@@ -1,0 +1,119 @@
+"""CPU checks of the boundary: the C-ABI library loads and exports every entry
+point include/hyburn_hip.h declares; host-side validation returns the
+reference's error classes without touching a GPU; the Philox noise stream
+matches the published Random123 known-answer vectors."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from hyburn_amd import _lib as L
+from hyburn_amd.instances import Instance, _Marshal, _opts
+from hyburn_amd.synth import G2DO_DEFAULT, _philox4x32_10, philox_normal
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    src = open(os.path.join(ROOT, "include", "hyburn_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hb_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = L.lib()
+    names = _declared_functions()
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared but not exported"
+        assert n in L.PROTOTYPES, f"{n} has no ctypes prototype"
+
+
+def test_struct_sizes_match_header():
+    # natural-alignment layout of the C structs, computed independently
+    assert C.sizeof(L.HbRunOpts) == 16
+    assert C.sizeof(L.HbRunStats) == 40
+    assert C.sizeof(L.HbSweepPoint) == 32
+
+
+def test_version_and_backend_selection():
+    lib = L.lib()
+    assert b"gfx950" in lib.hb_version()
+    assert lib.hb_select_backend(b"hip") == b"hip"
+    assert lib.hb_select_backend(b"rocm") == b"hip"
+    assert lib.hb_select_backend(b"ndarray") is None
+    assert b"HIP only" in lib.hb_last_error()
+
+
+def _plan_error(inst):
+    m = _Marshal([inst])
+    h = C.c_void_p()
+    o = _opts(0, False, True, 0, 0)
+    rc = L.lib().hb_plan_create(m.arr, 1, C.byref(o), C.byref(h))
+    return rc, L.lib().hb_last_error().decode()
+
+
+def _base(**kw):
+    d = dict(model=L.G2DO, n_nodes=4, n_steps=10, dt=0.1, params=G2DO_DEFAULT, x0=np.zeros((4, 2), np.float32))
+    d.update(kw)
+    return Instance(**d)
+
+
+@pytest.mark.parametrize("kw,needle", [
+    (dict(dt=0.0), "dt must be positive"),
+    (dict(n_steps=-1), "n_steps"),
+    (dict(wkind=L.W_CSR, indptr=np.array([0, 1, 1, 1, 1], np.int32), indices=np.array([9], np.int32),
+          weights=np.array([1.0], np.float32)), "CSR index out of range"),
+    (dict(wkind=L.W_CSR, indptr=np.array([0, 1, 1, 1, 2], np.int32), indices=np.array([1], np.int32),
+          weights=np.array([1.0], np.float32)), "indptr inconsistent"),
+    (dict(integrator=L.HEUN_STOCH), "nsig"),
+    (dict(cfn=L.CF_KURAMOTO, wkind=L.W_DENSE, dense=np.zeros((4, 4), np.float32)), "Kuramoto coupling"),
+    (dict(model=L.MPR, params=np.ones(7, np.float32), cfn=L.CF_SJR), "combination"),
+])
+def test_plan_validation_errors(kw, needle):
+    rc, msg = _plan_error(_base(**kw))
+    assert rc == L.HB_INVALID_CONFIG and needle in msg
+
+
+def test_engine_create_validation():
+    lib = L.lib()
+    cfg = L.HbSimConfig()
+    cfg.dt, cfg.sim_length = 0.0, 10.0
+    h = C.c_void_p()
+    assert lib.hb_engine_create(C.byref(cfg), 1, 0, C.byref(h)) == L.HB_INVALID_CONFIG
+    assert b"dt must be positive" in lib.hb_last_error()
+    cfg.dt = 0.1
+    assert lib.hb_engine_create(C.byref(cfg), 1, 0, C.byref(h)) == L.HB_INVALID_CONFIG
+    assert b"subnetwork" in lib.hb_last_error()
+    p = np.asarray(G2DO_DEFAULT[:5], np.float32)
+    x0 = np.zeros(8, np.float32)
+    sub = L.HbSubnetwork(L.G2DO, 4, 5, p.ctypes.data_as(L.fp), x0.ctypes.data_as(L.fp))
+    cfg.subnetworks, cfg.n_subnetworks = C.pointer(sub), 1
+    assert lib.hb_engine_create(C.byref(cfg), 1, 0, C.byref(h)) == L.HB_INVALID_CONFIG
+    assert b"parameter count" in lib.hb_last_error()
+
+
+# Random123 known-answer vectors for philox4x32-10 (kat_vectors).
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,want", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, want):
+    c = [np.array([v], np.uint32) for v in ctr]
+    out = _philox4x32_10(*c, key[0], key[1])
+    assert tuple(int(o[0]) for o in out) == want
+
+
+def test_philox_normal_moments_and_determinism():
+    z = philox_normal(123, 5, 400_001)
+    assert z.dtype == np.float32 and z.shape == (400_001,)
+    assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+    np.testing.assert_array_equal(z[:1000], philox_normal(123, 5, 1000))
+    assert not np.array_equal(z[:1000], philox_normal(123, 6, 1000))

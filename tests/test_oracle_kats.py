@@ -1,0 +1,301 @@
+"""Pins the CPU oracle against the reference's own known-answer unit tests
+(the literals in src/model/*.rs, src/engine/{coupling,sparse,bold,bold_monitor}.rs)
+and against closed forms the mathematics fixes."""
+import math
+
+import numpy as np
+import pytest
+
+from hyburn_amd import _lib as L
+from hyburn_amd.synth import G2DO_DEFAULT, JR_DEFAULT, MPR_DEFAULT, RWW_DEFAULT, WC_DEFAULT
+
+pytestmark = []
+
+
+# ---------------------------------------------------------------- models ----
+
+def test_g2do_equilibrium_defaults(orc):
+    # g2do.rs:88-113: dV = 0, dW = -0.04 at (0,0)
+    d = orc.dfun(L.G2DO, G2DO_DEFAULT, [0.0, 0.0], [0.0])
+    assert abs(d[0]) < 1e-5 and abs(d[1] + 0.04) < 1e-4
+
+
+def test_g2do_simplified_form(orc):
+    # g2do.rs:115-142: V=0.5, W=0.3 -> dV ~ 0.7583, dW = 1.2
+    p = np.array([1, 0, 1, 1, 0, 1, 0, 1 / 3, 1, 1, 1, 1], np.float32)
+    d = orc.dfun(L.G2DO, p, [0.5, 0.3], [0.0])
+    assert abs(d[0] - 0.7583) < 1e-3 and abs(d[1] - 1.2) < 1e-3
+    # exact value of the simplified form: W - V^3/3 + V
+    assert abs(d[0] - (0.3 - 0.125 / 3 + 0.5)) < 1e-6
+
+
+def test_g2do_swept_current_enters_like_I(orc):
+    # the swept form (c0 + I_s)*gamma equals the plain form with params[1] = I_s
+    p = G2DO_DEFAULT.copy()
+    p[1] = 0.7
+    a = orc.dfun(L.G2DO, p, [0.3, -0.2], [0.05], dtype=np.float64)
+    b = orc.dfun(L.G2DO, G2DO_DEFAULT, [0.3, -0.2], [0.05], sweep_form=1, sweep_val=0.7, dtype=np.float64)
+    np.testing.assert_allclose(a, b, rtol=1e-7)
+
+
+def test_mpr_at_zero(orc):
+    # mpr.rs:73-95: dr = 1/pi, dV = -5
+    d = orc.dfun(L.MPR, MPR_DEFAULT, [0.0, 0.0], [0.0, 0.0])
+    assert abs(d[0] - 1 / math.pi) < 1e-4 and abs(d[1] + 5.0) < 1e-4
+
+
+def test_mpr_with_state(orc):
+    # mpr.rs:97-122
+    d = orc.dfun(L.MPR, MPR_DEFAULT, [0.5, 0.3], [0.0, 0.0])
+    assert abs(d[0] - (1 / math.pi + 0.3)) < 1e-3
+    assert abs(d[1] - (0.09 - math.pi ** 2 * 0.25 - 5.0 + 7.5)) < 1e-3
+
+
+def test_mpr_clamp(orc):
+    # mpr.rs:124-135
+    x = orc.clamp(L.MPR, [-0.5, 1.0])
+    assert x[0] == 0.0 and x[1] == 1.0
+
+
+def test_jr_at_zero(orc):
+    # jansen_rit.rs:92-133
+    d = orc.dfun(L.JANSEN_RIT, JR_DEFAULT, np.zeros(6), [0.0])
+    nu, r, v0 = 0.0025, 0.56, 5.52
+    s0 = 2 * nu / (1 + math.exp(r * v0))
+    assert np.all(np.abs(d[:3]) < 1e-6)
+    assert abs(d[3] - 3.25 * 0.1 * s0) < 1e-6
+    assert abs(d[4] - 3.25 * 0.1 * (0.22 + 0.8 * 135 * s0)) < 1e-5
+    assert abs(d[5] - 22 * 0.05 * 0.25 * 135 * s0) < 1e-5
+
+
+def test_jr_nonzero_state_and_coupling(orc):
+    # jansen_rit.rs:135-190
+    d = orc.dfun(L.JANSEN_RIT, JR_DEFAULT, [0.1, 0.2, 0.05, 0, 0, 0], [0.1])
+    assert np.all(np.abs(d[:3]) < 1e-6)
+    assert abs(d[3] + 9.24e-4) < 5e-4
+    assert abs(d[4] - 0.2755) < 0.01
+    assert abs(d[5] - 0.0428) < 0.005
+
+
+def test_kuramoto_dfun(orc):
+    # kuramoto_model.rs:65-72
+    d = orc.dfun(L.KURAMOTO, [1.0], [0.5], [0.3])
+    assert abs(d[0] - 1.3) < 1e-5
+
+
+def test_kuramoto_clamp_wraps(orc):
+    # dfun.rs:188-191: x - floor(x / 2pi) * 2pi
+    x = orc.clamp(L.KURAMOTO, [7.0])
+    assert abs(x[0] - (7.0 - 2 * math.pi)) < 1e-5
+    x = orc.clamp(L.KURAMOTO, [-1.0])
+    assert abs(x[0] - (2 * math.pi - 1.0)) < 1e-5
+
+
+def test_rww_at_zero(orc):
+    # rww.rs:73-96
+    d = orc.dfun(L.RWW, RWW_DEFAULT, [0.0], [0.0])
+    x = 0.33
+    axb = 0.270 * x - 0.108
+    h = axb / (1 - math.exp(-154 * axb))
+    assert abs(d[0] - h * 0.641) < 1e-5
+
+
+def test_rww_clamp(orc):
+    # rww.rs:98-118
+    assert orc.clamp(L.RWW, [-0.2])[0] == 0.0
+    assert orc.clamp(L.RWW, [1.5])[0] == 1.0
+
+
+def test_wc_at_zero_and_clamp(orc):
+    # wilson_cowan.rs:102-132
+    d = orc.dfun(L.WILSON_COWAN, WC_DEFAULT, [0.0, 0.0], [0.0])
+    assert np.all(np.abs(d) < 1e-5)
+    x = orc.clamp(L.WILSON_COWAN, [-0.5, 1.5])
+    assert x[0] == 0.0 and x[1] == 1.0
+
+
+def test_models_fp32_vs_fp64_twins_agree(orc):
+    rng = np.random.default_rng(0)
+    cases = [(L.G2DO, G2DO_DEFAULT, 2, 1), (L.MPR, MPR_DEFAULT, 2, 2), (L.JANSEN_RIT, JR_DEFAULT, 6, 1),
+             (L.WILSON_COWAN, WC_DEFAULT, 2, 1), (L.KURAMOTO, np.array([1.0], np.float32), 1, 1),
+             (L.RWW, RWW_DEFAULT, 1, 1)]
+    for m, p, nv, nc in cases:
+        for _ in range(20):
+            x = rng.uniform(0.05, 0.9, nv)
+            c = rng.uniform(-0.1, 0.1, nc)
+            a = orc.dfun(m, p, x.astype(np.float32), c.astype(np.float32))
+            b = orc.dfun(m, p, x.astype(np.float32).astype(np.float64), c.astype(np.float32).astype(np.float64),
+                         dtype=np.float64)
+            np.testing.assert_allclose(a, b, rtol=2e-5, atol=1e-6)
+
+
+# ---------------------------------------------------------------- coupling --
+
+def test_dense_linear_pipeline(orc):
+    # coupling.rs:701-725: W=[[1,0,0],[0,1,1]], x=[[1,2],[3,4],[5,6]], a=2, b=1 -> 3,5,17,21
+    W = np.array([[1, 0, 0], [0, 1, 1]], np.float32)
+    x = np.array([[1, 2], [3, 4], [5, 6]], np.float32)
+    out = orc.coupling(L.CF_LINEAR, [2.0, 1.0], x, dense=W)
+    np.testing.assert_array_equal(out.ravel(), [3, 5, 17, 21])
+
+
+def test_kuramoto_dense_classic(orc):
+    # coupling.rs:674-699: +-0.25
+    W = np.array([[0, 0.5], [0.5, 0]], np.float32)
+    th = np.array([[0.0], [math.pi / 2]], np.float32)
+    out = orc.coupling(L.CF_KURAMOTO, [1.0], th, x_tgt=th, dense=W, n_src=2)
+    assert abs(out[0, 0] - 0.25) < 1e-5 and abs(out[1, 0] + 0.25) < 1e-5
+
+
+def test_kuramoto_self_coupling_zero(orc):
+    # coupling.rs:622-644: identical phases -> 0
+    th = np.full((2, 1), 1.23, np.float32)
+    W = np.array([[0, 0.5], [0.5, 0]], np.float32)
+    out = orc.coupling(L.CF_KURAMOTO, [1.0], th, x_tgt=th, dense=W, n_src=2)
+    assert np.all(np.abs(out) < 1e-5)
+
+
+def test_sigmoidal_pipeline(orc):
+    # coupling.rs:727-748: sigmoid(0.5 + 1.5) with cmin=0,cmax=1
+    W = np.array([[1.0, 1.0]], np.float32)
+    x = np.array([[0.5], [1.5]], np.float32)
+    out = orc.coupling(L.CF_SIGMOIDAL, [0.0, 1.0, 0.0, 1.0, 1.0], x, dense=W)
+    assert abs(out[0, 0] - 1 / (1 + math.exp(-2.0))) < 1e-5
+
+
+def test_difference_post_rowsum(orc):
+    # coupling.rs:660-672: a * (gx - x_i * rowsum) = 0.1 * (5 - 2*3) = -0.1
+    W = np.array([[2.0, 1.0]], np.float32)        # rowsum 3
+    x = np.array([[2.0], [1.0]], np.float32)       # W @ x = 5
+    out = orc.coupling(L.CF_DIFFERENCE, [0.1], x, x_tgt=np.array([[2.0]], np.float32), dense=W)
+    assert abs(out[0, 0] + 0.1) < 1e-6
+
+
+def test_csr_matvec_basic(orc):
+    # sparse.rs:134-154: -> 4, 9, 1
+    import ctypes as C
+    data = np.array([2, 3, 1], np.float32)
+    ind = np.array([1, 2, 0], np.int32)
+    ipt = np.array([0, 1, 2, 3], np.int32)
+    x = np.array([1, 2, 3], np.float32)
+    y = np.zeros(3, np.float32)
+    orc.lib().hb_orc_csr_matvec_f32(orc._p(data), orc._p(ind, orc.ip), orc._p(ipt, orc.ip), 3, orc._p(x), orc._p(y))
+    np.testing.assert_array_equal(y, [4, 9, 1])
+
+
+def test_sparse_matches_dense_ring(orc):
+    # sparse.rs:156-200 / engine/mod.rs:183-245: 5-node directed ring, w = 0.1
+    n = 5
+    W = np.zeros((n, n), np.float32)
+    for i in range(n):
+        W[i, (i - 1) % n] = 0.1
+    from hyburn_amd.synth import dense_to_csr
+    x = np.arange(1, n + 1, dtype=np.float32).reshape(n, 1)
+    a = orc.coupling(L.CF_LINEAR, [1.0, 0.0], x, dense=W)
+    b = orc.coupling(L.CF_LINEAR, [1.0, 0.0], x, csr=dense_to_csr(W))
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_allclose(a[:, 0], 0.1 * np.roll(x[:, 0], 1), rtol=1e-6)
+
+
+# ---------------------------------------------------------------- BOLD ------
+
+def _bold_run(orc, z, dt_s, steps, n=2):
+    import ctypes as C
+    s = np.zeros(n, np.float32); f = np.ones(n, np.float32); v = np.ones(n, np.float32); q = np.ones(n, np.float32)
+    zz = np.full(n, z, np.float32)
+    sig = np.zeros(n, np.float32)
+    out = []
+    for _ in range(steps):
+        orc.lib().hb_orc_bold_step_f32(n, orc._p(zz), dt_s, orc._p(s), orc._p(f), orc._p(v), orc._p(q))
+        orc.lib().hb_orc_bold_signal_f32(n, orc._p(v), orc._p(q), orc._p(sig))
+        out.append(sig.copy())
+    return np.array(out), (s, f, v, q)
+
+
+def test_bold_zero_input_fixed_point(orc):
+    # bold.rs:174-188 and the fixed point s=0,f=v=q=1 -> signal exactly 0
+    sig, (s, f, v, q) = _bold_run(orc, 0.0, 0.001, 100)
+    assert np.all(np.isfinite(sig)) and np.all(np.abs(sig[-1]) < 1e-3)
+    assert np.all(s == 0) and np.all(f == 1) and np.all(v == 1) and np.all(q == 1)
+    assert np.all(sig == 0.0)
+
+
+def test_bold_ramps_up(orc):
+    # bold.rs:190-204: constant input 1 for 5 s at 1 ms -> > 0.1
+    sig, _ = _bold_run(orc, 1.0, 0.001, 5000)
+    assert sig[-1, 0] > 0.1
+
+
+def test_bold_negative_input_rectified(orc):
+    # z = max(u, 0): negative input is the zero-input fixed point exactly
+    sig, _ = _bold_run(orc, -3.0, 0.001, 50)
+    assert np.all(sig == 0.0)
+
+
+def test_bold_monitor_shapes(orc):
+    # bold_monitor.rs:263-279: 20 steps, period 5 -> 4 samples; TR = dt_bold -> unchanged
+    from hyburn_amd.instances import Instance
+    I = Instance(model=L.G2DO, n_nodes=3, n_steps=20, dt=2.0, params=G2DO_DEFAULT, x0=np.zeros((3, 2), np.float32),
+                 integrator=L.EULER, bold_period=5, bold_tr_ms=10.0, record_bold=True)
+    r = orc.run(I)
+    assert r["bold_native"].shape == (4, 3) and r["bold"].shape == (4, 3)
+    assert I.bold_rows() == 4
+
+
+def test_bold_tr_downsample_count(orc):
+    # bold_monitor.rs:281-294: dt=1 ms, period 10, TR 25 ms, 100 steps -> floor(100/25) = 4 rows
+    from hyburn_amd.instances import Instance
+    I = Instance(model=L.G2DO, n_nodes=2, n_steps=100, dt=1.0, params=G2DO_DEFAULT, x0=np.full((2, 2), 0.1, np.float32),
+                 integrator=L.EULER, bold_period=10, bold_tr_ms=25.0, record_bold=True)
+    r = orc.run(I)
+    assert r["bold_native"].shape[0] == 10
+    assert r["bold"].shape[0] == 4 == I.bold_rows()
+    # t=25 lies between native samples at 20 and 30 ms: halfway interpolation
+    nat = r["bold_native"]
+    np.testing.assert_allclose(r["bold"][0], nat[1] + (nat[2] - nat[1]) * np.float32(0.5), rtol=1e-6)
+    # t=100 is the last native sample exactly -> nearest
+    np.testing.assert_array_equal(r["bold"][3], nat[9])
+
+
+# ---------------------------------------------------------------- FC --------
+
+def test_fc_matches_pearson(orc):
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(500, 7)).astype(np.float32)
+    X[:, 3] = 2 * X[:, 1] + 0.5                        # perfectly correlated pair
+    X[:, 5] = 4.0                                      # constant: std -> 1, FC 0 off-diagonal
+    F = orc.fc(X)
+    ref = np.corrcoef(X[:, [0, 1, 2, 3, 4, 6]].astype(np.float64), rowvar=False)
+    idx = [0, 1, 2, 3, 4, 6]
+    np.testing.assert_allclose(F[np.ix_(idx, idx)], ref, atol=1e-6)
+    assert abs(F[1, 3] - 1.0) < 1e-6
+    assert F[5, 5] == 0.0 and np.all(F[5, np.arange(7) != 5] == 0.0)
+    np.testing.assert_array_equal(F, F.T)
+
+
+# ---------------------------------------------------------------- delays ----
+
+def test_slot_tables_ring(orc):
+    # runtime.rs:162-204: d==0 or s==0 -> current; d > s -> zero; else s - d
+    H = 4
+    table = [[orc.source_time(L.DELAY_PER_EDGE, s, d, H) for d in range(H)] for s in range(6)]
+    assert table == [[0, 0, 0, 0], [1, 0, -1, -1], [2, 1, 0, -1], [3, 2, 1, 0], [4, 3, 2, 1], [5, 4, 3, 2]]
+
+
+def test_slot_tables_batch_compat(orc):
+    # engine.rs:541-589: the window holds x_{s-H+1..s}; slot (s-d+H)%H is read.
+    # The reference's example: H=4, d=3, s=4 reads effective delay 2.
+    H = 4
+    assert orc.source_time(L.DELAY_BATCH_COMPAT, 4, 3, H) == 2
+    for s in range(1, 40):
+        for d in range(1, H):
+            t = orc.source_time(L.DELAY_BATCH_COMPAT, s, d, H)
+            if d > s:
+                assert t == -1
+                continue
+            slot = (s - d + H) % H
+            want = s - (H - 1) + slot
+            assert t == (want if want >= 0 else -1)
+            # the read equals the true delay only when s = H-1 (mod H)
+            if s % H == H - 1:
+                assert t == s - d
