@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched neural-mass integration path on one MI355X.
+
+One step = one full pass of the hot path over the batched-sweep workload:
+Generic2dOscillator, 74 nodes x 128 sweep points (Linear coupling G
+log-uniform in [1e-3, 1e-1] x parameter a in U[-2, 2]), symmetric 20%
+lognormal connectome with per-edge delays <= 500 steps, Heun-stochastic
+dt = 0.1 ms, 10,000 integration steps, nsig = 0.01 Philox noise generated on
+device. Inputs are resident in HBM before timing; the timed region covers the
+noise generation and the fused integration kernel.
+
+Prints one JSON line (metric, value, unit, roofline, cpu_baseline, ...).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FP32 work per node-step of this workload (Heun: 2 G2DO dfuns of
+# 20 FLOP, 2 x 7 FLOP of stochastic Heun update per state variable, 2 FLOP per
+# CSR edge at 14.6 edges/row, 2 FLOP Linear post): 85 FLOP.
+FLOP_PER_NODE_STEP = 2 * 20 + 2 * 7 + 2 * 14.6 + 2
+# Algorithmic HBM bytes per node-step: the f32 noise of 2 state variables is
+# written once by the noise kernel and read once by the integrator (the delay
+# ring lives in LDS; tavg/final outputs amortise to ~0).
+BYTES_PER_NODE_STEP = 2 * 4 * 2
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """HBM bytes per launch of the forward kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    p = os.path.join(ROOT, "profiles", "forward_sweep_g2do_74x128_traffic.txt")
+    if not os.path.exists(p):
+        return None
+    vals = {}
+    for line in open(p):
+        if ":" in line and not line.lstrip().startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(insts, threads: int):
+    """The oracle (a C restatement of the reference NdArray path) on host cores."""
+    from oracle import hb_oracle
+    secs = hb_oracle.run_many_f32(insts, n_threads=threads)
+    ns = sum(I.n_nodes * I.n_steps for I in insts)
+    return ns / secs, secs
+
+
+def ragged_leg(args):
+    """Ragged 4096-instance MPR/JR RK4 sweep: one persistent launch vs
+    per-instance plain launches of the same kernel (sampled)."""
+    from hyburn_amd.instances import Plan
+    from hyburn_amd.synth import config3
+    insts = config3(n_inst=args.ragged_n, t_scale=args.ragged_tscale)
+    ns = sum(I.n_nodes * I.n_steps for I in insts)
+    log(f"[bench] ragged: {len(insts)} instances, {ns:.3e} node-steps; planning")
+    with Plan(insts) as P:
+        P.run()
+        t0 = time.perf_counter()
+        st = P.run()
+        wall = time.perf_counter() - t0
+    res = {"n_instances": len(insts), "node_steps": ns, "value": ns / wall, "wall_s": wall,
+           "kernel_ms": st.kernel_ms}
+    # per-instance plain launches: one plan (one launch) per instance, sampled
+    k = min(args.ragged_plain_sample, len(insts))
+    idx = list(range(0, len(insts), max(1, len(insts) // k)))[:k]
+    sub_ns, sub_t = 0.0, 0.0
+    for i in idx:
+        with Plan([insts[i]]) as P:
+            P.run()
+            t0 = time.perf_counter()
+            P.run()
+            sub_t += time.perf_counter() - t0
+            sub_ns += insts[i].n_nodes * insts[i].n_steps
+    res["plain_launch_value"] = sub_ns / sub_t
+    res["plain_launch_sample"] = len(idx)
+    res["speedup_vs_plain_launches"] = res["value"] / res["plain_launch_value"]
+    return res
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n-sweep", type=int, default=128)
+    ap.add_argument("--n-steps", type=int, default=10_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--ragged", action="store_true", help="also time the 4096-instance ragged sweep")
+    ap.add_argument("--ragged-n", type=int, default=4096)
+    ap.add_argument("--ragged-tscale", type=float, default=1.0)
+    ap.add_argument("--ragged-plain-sample", type=int, default=16)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(instances shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from hyburn_amd import _lib as L
+    from hyburn_amd.instances import Plan
+    from hyburn_amd.synth import config2
+
+    lib = L.lib()
+    if lib.hb_device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        return 1
+    insts = config2(n_sweep=args.n_sweep, n_steps=args.n_steps)
+    node_steps = sum(I.n_nodes * I.n_steps for I in insts)
+    log(f"[bench] workload: {len(insts)} x {insts[0].n_nodes} nodes x {insts[0].n_steps} steps")
+    P = Plan(insts)
+    for _ in range(args.warmup):
+        P.run()
+    L.check(lib.hb_device_synchronize(0))
+    kernel_ms, total_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = P.run()
+        kernel_ms.append(st.kernel_ms)
+        total_ms.append(st.total_ms)
+    L.check(lib.hb_device_synchronize(0))
+    wall = time.perf_counter() - t0
+    outs = P.download()
+    P.close()
+    import numpy as np
+    assert all(np.all(np.isfinite(o.tavg)) for o in outs), "non-finite output"
+
+    ms_per_step = wall / args.steps * 1e3
+    value = node_steps * args.steps / wall
+    k_ms = sum(kernel_ms) / len(kernel_ms)
+    achieved = FLOP_PER_NODE_STEP * node_steps / (k_ms * 1e-3) / 1e12
+    traffic = read_traffic()
+    line = {
+        "metric": "node-steps/s (ragged sweep, B=128..4096) + adjoint grads/s vs plain & FD",
+        "value": value,
+        "unit": "node-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded connectome, delays, parameters; Philox noise)",
+        "config": {
+            "workload": "batched sweep: Generic2dOscillator 74 nodes x 128 points x 10k Heun-stochastic "
+                        "steps, 20% lognormal CSR with per-edge delays <= 500 steps, nsig 0.01",
+            "n_sweep": len(insts), "n_nodes": insts[0].n_nodes, "n_steps": insts[0].n_steps,
+            "parallelism": "1 workgroup per sweep point, delay ring in LDS",
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": traffic,
+            "kernel": "hb::fast::k_single<G2DO,1,lds_ring>",
+            "kernel_ms": k_ms,
+            "flop_per_node_step": FLOP_PER_NODE_STEP,
+            "note": "FP32 VALU work against the FP32 peak (vector and matrix FP32 peaks are both 157.3 "
+                    "TFLOP/s on MI355X); the kernel is latency bound by 10k sequential steps per point",
+            "hbm_achieved_GBs": BYTES_PER_NODE_STEP * node_steps / (k_ms * 1e-3) / 1e9,
+        },
+        "timing": {"kernel_ms": k_ms, "noise_plus_kernel_ms": sum(total_ms) / len(total_ms)},
+    }
+    if not args.no_cpu:
+        # bounded sample: the whole workload on all host threads (~1-3 s wall)
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        from oracle import hb_oracle
+        hb_oracle.lib()
+        sample = config2(n_sweep=args.n_sweep, n_steps=args.n_steps)
+        v, secs = cpu_baseline(sample, threads)
+        line["cpu_baseline"] = {"value": v, "unit": "node-steps/s", "cores": threads, "kind": "port",
+                                "sample": f"the full workload ({len(sample)} points x {args.n_steps} steps) "
+                                          f"on {threads} host threads, {secs:.2f} s wall "
+                                          f"({secs * threads:.1f} core-s)"}
+    if args.ragged:
+        line["ragged"] = ragged_leg(args)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -20,7 +20,7 @@ CF_LINEAR, CF_DIFFERENCE, CF_KURAMOTO, CF_SIGMOIDAL, CF_SCALED_LINEAR, CF_TANH, 
 W_NONE, W_SCALAR_MEAN, W_SCALAR_DIAG, W_DENSE, W_CSR = 0, 1, 2, 3, 4
 DELAY_PER_EDGE, DELAY_SINGLE, DELAY_BATCH_COMPAT = 0, 1, 2
 ENGINE_BATCH, ENGINE_PLAIN = 0, 1
-FLAG_TRAJECTORY, FLAG_EXACT, FLAG_NO_GRAPH = 0x1, 0x2, 0x4
+FLAG_TRAJECTORY, FLAG_EXACT, FLAG_NO_GRAPH, FLAG_NO_BLOCKING = 0x1, 0x2, 0x4, 0x8
 
 MODEL_NAMES = {
     "Generic2dOscillator": G2DO, "MontbrioPazoRoxin": MPR, "JansenRit": JANSEN_RIT,
@@ -127,6 +127,7 @@ PROTOTYPES = {
     "hb_last_error": (C.c_char_p, []),
     "hb_version": (C.c_char_p, []),
     "hb_device_count": (C.c_int, []),
+    "hb_device_synchronize": (C.c_int, [C.c_int32]),
     "hb_select_backend": (C.c_char_p, [C.c_char_p]),
 }
 

@@ -23,6 +23,7 @@
 #include "hb_internal.h"
 
 #include <cstring>
+#include <type_traits>
 
 #ifndef HB_EXACT
 #define HB_EXACT 0
@@ -36,6 +37,13 @@
 
 namespace hb {
 namespace HB_NS {
+
+// CSR edges a node keeps in registers for the whole run (longer rows spill
+// the rest to the memory path), and the noise prefetch distance in steps.
+constexpr int kMaxRegEdges = 32;
+constexpr int kNoiseAhead = 4;
+// Steps per delay block (needs min delay >= kBlockSteps).
+constexpr int kBlockSteps = 8;
 
 template <int M> struct Tr;
 template <> struct Tr<HB_G2DO> { static constexpr int NV = 2, NC = 1; };
@@ -202,6 +210,59 @@ __device__ __forceinline__ void post(int cfn, const float* cc, bool classic, con
     }
 }
 
+
+// ---------------------------------------------------------------- view -----
+//
+// Descriptor fields read once per instance into wave-uniform values (SGPRs),
+// with device pointers cast back to the global address space so the loops
+// use global (not flat) memory instructions and never re-read the descriptor.
+template <typename T> using gp = __attribute__((address_space(1))) T*;
+
+__device__ __forceinline__ int u32(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uf32(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+__device__ __forceinline__ uint64_t u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ double uf64(double x) { return __longlong_as_double((long long)u64((uint64_t)__double_as_longlong(x))); }
+template <typename T> __device__ __forceinline__ gp<T> ug(T* p) { return (gp<T>)((T*)u64((uint64_t)p)); }
+__device__ __forceinline__ uint2 edge_at(gp<const uint32_t> e, int i) {
+    return make_uint2(e[2 * i], e[2 * i + 1]);
+}
+
+struct View {
+    int model, integrator, n_nodes, n_steps, np, wkind, cfn, delay_mode, single_delay, ring_depth;
+    int sweep_form, node_param_idx, bold_period, bold_nout, bold_native, cpu_form, sjr_classic, engine;
+    int max_row, blocked_k;
+    float dt, dt_half, dt_sixth, inv_nsteps, sweep_val, scalar_w, fill0, fill1;
+    double dt_ms, tr_ms, dtb_ms;
+    int64_t traj_tstride;
+    gp<const float> x0, node_mc, dense_t, noise;
+    gp<const int32_t> indptr;
+    gp<const uint32_t> edges;   // 2 words per edge: src | delay << 16, weight bits
+    gp<float> ring, tavg, fin, traj, bold;
+};
+
+__device__ __forceinline__ View make_view(const DevInst* I) {
+    View V;
+    V.model = u32(I->model); V.integrator = u32(I->integrator); V.n_nodes = u32(I->n_nodes);
+    V.n_steps = u32(I->n_steps); V.np = u32(I->np); V.wkind = u32(I->wkind); V.cfn = u32(I->cfn);
+    V.delay_mode = u32(I->delay_mode); V.single_delay = u32(I->single_delay); V.ring_depth = u32(I->ring_depth);
+    V.sweep_form = u32(I->sweep_form); V.node_param_idx = u32(I->node_param_idx);
+    V.bold_period = u32(I->bold_period); V.bold_nout = u32(I->bold_nout); V.bold_native = u32(I->bold_native);
+    V.cpu_form = u32(I->cpu_form); V.sjr_classic = u32(I->sjr_classic); V.engine = u32(I->engine);
+    V.max_row = u32(I->max_row); V.blocked_k = u32(I->blocked_k);
+    V.dt = uf32(I->dt); V.dt_half = uf32(I->dt_half); V.dt_sixth = uf32(I->dt_sixth); V.inv_nsteps = uf32(I->inv_nsteps);
+    V.sweep_val = uf32(I->sweep_val); V.scalar_w = uf32(I->scalar_w); V.fill0 = uf32(I->fill0); V.fill1 = uf32(I->fill1);
+    V.dt_ms = uf64(I->dt_ms); V.tr_ms = uf64(I->tr_ms); V.dtb_ms = uf64(I->dtb_ms);
+    V.traj_tstride = (int64_t)u64((uint64_t)I->traj_tstride);
+    V.x0 = ug(I->x0); V.node_mc = ug(I->node_mc); V.dense_t = ug(I->dense_t); V.noise = ug(I->noise);
+    V.indptr = ug(I->indptr); V.edges = ug((const uint32_t*)I->edges);
+    V.ring = ug(I->ring); V.tavg = ug(I->tavg); V.fin = ug(I->fin); V.traj = ug(I->traj); V.bold = ug(I->bold);
+    return V;
+}
+
 // ---------------------------------------------------------------- BOLD -----
 
 struct Bold {
@@ -237,18 +298,18 @@ struct BoldClock {
     double tprev; // time of the previous native sample
 };
 
-__device__ __forceinline__ void bold_emit(const DevInst* I, const BoldClock& bc, double tk, float sig,
-                                          float prev, float* out_col, int N) {
-    if (I->bold_native) {
-        if (bc.k < I->bold_nout) out_col[(size_t)bc.k * N] = sig;
+__device__ __forceinline__ void bold_emit(const View& I, const BoldClock& bc, double tk, float sig,
+                                          float prev, gp<float> out_col, int N) {
+    if (I.bold_native) {
+        if (bc.k < I.bold_nout) out_col[(size_t)bc.k * N] = sig;
         return;
     }
-    const double tr = I->tr_ms;
+    const double tr = I.tr_ms;
     int j = bc.j;
     if (bc.k == 0) {
-        while (j < I->bold_nout && (double)(j + 1) * tr <= tk) { out_col[(size_t)j * N] = sig; ++j; }
+        while (j < I.bold_nout && (double)(j + 1) * tr <= tk) { out_col[(size_t)j * N] = sig; ++j; }
     } else {
-        while (j < I->bold_nout && (double)(j + 1) * tr < tk) {
+        while (j < I.bold_nout && (double)(j + 1) * tr < tk) {
             const double tt = (double)(j + 1) * tr;
             const double d = tk - bc.tprev;
             double w = fabs(d) < 1e-12 ? 0.0 : (tt - bc.tprev) / d;
@@ -260,14 +321,85 @@ __device__ __forceinline__ void bold_emit(const DevInst* I, const BoldClock& bc,
 }
 
 // Advance the shared (uniform) clock after native sample k at time tk.
-__device__ __forceinline__ void bold_advance(const DevInst* I, BoldClock& bc, double tk) {
-    if (!I->bold_native) {
-        const double tr = I->tr_ms;
-        if (bc.k == 0) { while (bc.j < I->bold_nout && (double)(bc.j + 1) * tr <= tk) bc.j++; }
-        else { while (bc.j < I->bold_nout && (double)(bc.j + 1) * tr < tk) bc.j++; }
+__device__ __forceinline__ void bold_advance(const View& I, BoldClock& bc, double tk) {
+    if (!I.bold_native) {
+        const double tr = I.tr_ms;
+        if (bc.k == 0) { while (bc.j < I.bold_nout && (double)(bc.j + 1) * tr <= tk) bc.j++; }
+        else { while (bc.j < I.bold_nout && (double)(bc.j + 1) * tr < tk) bc.j++; }
     }
     bc.k++;
     bc.tprev = tk;
+}
+
+
+// ---------------------------------------------------------------- stepping -
+
+struct StepCtx {
+    float dt, dth, dt6;
+    int integ;
+    bool stoch, plain, sw;
+    float sv;
+};
+
+// One integration step of one node with frozen coupling (engine.rs:698-773,
+// integrator.rs:296-315 for the plain engine's RK4-stochastic order).
+template <int M>
+__device__ __forceinline__ void integrate(const StepCtx& S, const float* mc, const float* cpl,
+                                          const float* z, float* xs) {
+    constexpr int NV = Tr<M>::NV;
+    float k[NV], acc[NV], s[NV];
+    dfun<M>(mc, S.sw, S.sv, xs, cpl, k);
+    if (S.integ == HB_EULER || S.integ == HB_EULER_STOCH) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const float y = xs[v] + k[v] * S.dt;
+            xs[v] = S.stoch ? y + z[v] : y;
+        }
+        clampm<M>(xs);
+    } else if (S.integ == HB_HEUN || S.integ == HB_HEUN_STOCH) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const float y = xs[v] + k[v] * S.dt;
+            s[v] = S.stoch ? y + z[v] : y;
+            acc[v] = k[v];
+        }
+        clampm<M>(s);
+        dfun<M>(mc, S.sw, S.sv, s, cpl, k);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const float y = xs[v] + (acc[v] + k[v]) * S.dth;
+            xs[v] = S.stoch ? y + z[v] : y;
+        }
+        clampm<M>(xs);
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { acc[v] = k[v]; s[v] = xs[v] + k[v] * S.dth; }
+        clampm<M>(s);
+        dfun<M>(mc, S.sw, S.sv, s, cpl, k);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { acc[v] = acc[v] + k[v] * 2.0f; s[v] = xs[v] + k[v] * S.dth; }
+        clampm<M>(s);
+        dfun<M>(mc, S.sw, S.sv, s, cpl, k);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { acc[v] = acc[v] + k[v] * 2.0f; s[v] = xs[v] + k[v] * S.dt; }
+        clampm<M>(s);
+        dfun<M>(mc, S.sw, S.sv, s, cpl, k);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xs[v] = xs[v] + (acc[v] + k[v]) * S.dt6;
+        if (S.stoch) {
+            if (S.plain) {
+                clampm<M>(xs);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) xs[v] = xs[v] + z[v];
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) xs[v] = xs[v] + z[v];
+                clampm<M>(xs);
+            }
+        } else {
+            clampm<M>(xs);
+        }
+    }
 }
 
 // ---------------------------------------------------------------- instance -
@@ -275,57 +407,94 @@ __device__ __forceinline__ void bold_advance(const DevInst* I, BoldClock& bc, do
 // Integrate instance I with the whole workgroup. lds: dynamic LDS base.
 template <int M, int NP, bool LRING, bool MULTI>
 __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, float* lds) {
+    const DevInst* __restrict__ Ia = I;
+    const View V = make_view(I);
     constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
-    const int N = I->n_nodes, T = I->n_steps, Hd = I->ring_depth;
+    const int N = V.n_nodes, T = V.n_steps, Hd = V.ring_depth;
     const int tid = threadIdx.x, BS = blockDim.x;
     const int NNP = N * NP;
-    float* __restrict__ ring = LRING ? lds : I->ring;
+    using RingPtr = std::conditional_t<LRING, float*, gp<float>>;
+    RingPtr ring;
+    if constexpr (LRING) ring = lds; else ring = V.ring;
     // MULTI state table after the ring: x[N][NV], tavg[N][NV], bold[N][6]
     float* __restrict__ tab = LRING ? lds + (size_t)Hd * NNP : lds;
     float* __restrict__ tab_x = tab;
     float* __restrict__ tab_t = tab + (size_t)N * NV;
     float* __restrict__ tab_b = tab + (size_t)2 * N * NV;
 
-    const int cfn = I->cfn, wkind = I->wkind, dmode = I->delay_mode, integ = I->integrator;
-    const bool cpu_form = I->cpu_form != 0, classic = I->sjr_classic != 0;
-    const bool sw = I->sweep_form != 0;
-    const float sv = I->sweep_val;
-    const float dt = I->dt, dth = I->dt_half, dt6 = I->dt_sixth;
+    const int cfn = V.cfn, wkind = V.wkind, dmode = V.delay_mode, integ = V.integrator;
+    const bool cpu_form = V.cpu_form != 0, classic = V.sjr_classic != 0;
+    const bool sw = V.sweep_form != 0;
+    const float sv = V.sweep_val;
+    const float dt = V.dt, dth = V.dt_half, dt6 = V.dt_sixth;
     const bool stoch = integ >= HB_EULER_STOCH;
-    const bool plain = I->engine == HB_ENGINE_PLAIN;
+    const bool plain = V.engine == HB_ENGINE_PLAIN;
+    StepCtx S;
+    S.dt = dt; S.dth = dth; S.dt6 = dt6; S.integ = integ; S.stoch = stoch; S.plain = plain; S.sw = sw; S.sv = sv;
     float cc[kCC];
 #pragma unroll
-    for (int i = 0; i < kCC; ++i) cc[i] = I->cc[i];
+    for (int i = 0; i < kCC; ++i) cc[i] = Ia->cc[i];
     float nsc[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) nsc[v] = I->nscale[v];
-    const float fill[2] = {I->fill0, I->fill1};
+    for (int v = 0; v < NV; ++v) nsc[v] = Ia->nscale[v];
+    const float fill[2] = {V.fill0, V.fill1};
 
     // ring initial fill: 0 (per-edge: exact-zero contributions) or pre(0)
     {
         const int total = Hd * NNP;
-        for (int i = tid; i < total; i += BS) ring[i] = fill[NP == 2 ? (i & 1) : 0];
+        for (int i = tid; i < total; i += BS) ring[i] = (NP == 2 && (i & 1)) ? fill[1] : fill[0];
     }
 
     const int nper = MULTI ? (N + BS - 1) / BS : 1;
     // model constants (uniform, or per node when a parameter is given per node)
     float mc_u[kMC];
 #pragma unroll
-    for (int i = 0; i < kMC; ++i) mc_u[i] = I->mc[i];
-    const int npi = I->node_param_idx;
+    for (int i = 0; i < kMC; ++i) mc_u[i] = Ia->mc[i];
+    const int npi = V.node_param_idx;
 
     // per-thread node state (MULTI=false keeps it in registers)
     float x[NV], tv[NV];
     Bold bw;
     bw.s = 0.0f; bw.f = 1.0f; bw.v = 1.0f; bw.q = 1.0f; bw.acc = 0.0f; bw.prev = 0.0f;
     const bool own = tid < N;
+    // ---- CSR row of this thread's node, folded into ring offsets and kept in
+    // registers: offset = delay * N*np - src * np (read address = ws*N*np - offset)
+    const bool csr_reg = !MULTI && wkind == HB_W_CSR;
+    const bool per_edge = dmode == HB_DELAY_PER_EDGE;
+    const int kreg = csr_reg ? min(V.max_row, kMaxRegEdges) : 0;   // uniform
+    int eo[kMaxRegEdges];
+    float ew[kMaxRegEdges];
+    int r_e0 = 0, r_e1 = 0;
+    if (csr_reg) {
+        if (own) { r_e0 = V.indptr[tid]; r_e1 = V.indptr[tid + 1]; }
+#pragma unroll
+        for (int k = 0; k < kMaxRegEdges; ++k) {
+            eo[k] = -tid * NP;   // padding: zero weight on the node's own slot
+            ew[k] = 0.0f;
+            if (k < kreg && own && r_e0 + k < r_e1) {
+                const uint2 ed = edge_at(V.edges, r_e0 + k);
+                const int j = ed.x & 0xFFFF, d = per_edge ? (int)(ed.x >> 16) : 0;
+                eo[k] = d * NNP - j * NP;
+                ew[k] = __uint_as_float(ed.y);
+            }
+        }
+    }
+    // ---- noise prefetch FIFO (kNoiseAhead steps in flight)
+    float zq[kNoiseAhead][NV];
+    if (!MULTI && stoch) {
+#pragma unroll
+        for (int q = 0; q < kNoiseAhead; ++q)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                zq[q][v] = (own && q < T) ? V.noise[((size_t)q * N + tid) * NV + v] : 0.0f;
+    }
     if (!MULTI) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) { x[v] = own ? I->x0[tid * NV + v] : 0.0f; tv[v] = 0.0f; }
+        for (int v = 0; v < NV; ++v) { x[v] = own ? V.x0[tid * NV + v] : 0.0f; tv[v] = 0.0f; }
     } else {
         for (int n = tid; n < N; n += BS) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) { tab_x[n * NV + v] = I->x0[n * NV + v]; tab_t[n * NV + v] = 0.0f; }
+            for (int v = 0; v < NV; ++v) { tab_x[n * NV + v] = V.x0[n * NV + v]; tab_t[n * NV + v] = 0.0f; }
             float* b = tab_b + n * 6;
             b[0] = 0.0f; b[1] = 1.0f; b[2] = 1.0f; b[3] = 1.0f; b[4] = 0.0f; b[5] = 0.0f;
         }
@@ -333,9 +502,9 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
     float mc_n[kMC];
     if (!MULTI) {
 #pragma unroll
-        for (int i = 0; i < kMC; ++i) mc_n[i] = (npi >= 0 && own) ? I->node_mc[tid * kMC + i] : mc_u[i];
+        for (int i = 0; i < kMC; ++i) mc_n[i] = (npi >= 0 && own) ? V.node_mc[tid * kMC + i] : mc_u[i];
     }
-    const int bper = I->bold_period;
+    const int bper = V.bold_period;
     BoldClock bc;
     bc.k = 0; bc.j = 0; bc.tprev = 0.0;
     int bcount = 0;
@@ -351,7 +520,7 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
 #pragma unroll
                 for (int c = 0; c < NC; ++c) xc[c] = MULTI ? tab_x[n * NV + c] : x[c];
                 pre<NC, NP>(cfn, cc, cpu_form, classic, xc, pr);
-                float* dst = ring + (size_t)ws * NNP + n * NP;
+                auto dst = ring + (size_t)ws * NNP + n * NP;
 #pragma unroll
                 for (int c = 0; c < NP; ++c) dst[c] = pr[c];
             }
@@ -361,7 +530,7 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
         // uniform read slot for single-delay projections
         int slot_u = ws;
         if (dmode != HB_DELAY_PER_EDGE && t > 0) {
-            const int d = I->single_delay;
+            const int d = V.single_delay;
             int teff_back = 0;   // how many steps back
             if (d == 0) teff_back = 0;
             else if (d > t) teff_back = -1;
@@ -381,12 +550,12 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
             for (int c = 0; c < NP; ++c) {
                 float s = 0.0f;
                 if (slot_u >= 0) {
-                    const float* src = ring + (size_t)slot_u * NNP + c;
+                    const auto src = ring + (size_t)slot_u * NNP + c;
                     for (int j = 0; j < N; ++j) s += src[j * NP];
                 } else {
                     for (int j = 0; j < N; ++j) s += fill[c];
                 }
-                mf[c] = (s / (float)N) * I->scalar_w;
+                mf[c] = (s / (float)N) * V.scalar_w;
             }
         }
 
@@ -396,40 +565,75 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
             if (n < N) {
                 float z[NV];
                 if (stoch) {
-                    const float* nz = I->noise + ((size_t)t * N + n) * NV;
+                    if (!MULTI) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) z[v] = nz[v] * nsc[v];
+                        for (int v = 0; v < NV; ++v) z[v] = zq[0][v] * nsc[v];
+#pragma unroll
+                        for (int q = 0; q + 1 < kNoiseAhead; ++q)
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) zq[q][v] = zq[q + 1][v];
+                        if (t + kNoiseAhead < T) {
+                            const gp<const float> nz = V.noise + ((size_t)(t + kNoiseAhead) * N + n) * NV;
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) zq[kNoiseAhead - 1][v] = nz[v];
+                        }
+                    } else {
+                        const gp<const float> nz = V.noise + ((size_t)t * N + n) * NV;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) z[v] = nz[v] * nsc[v];
+                    }
                 }
                 float g[NP];
 #pragma unroll
                 for (int c = 0; c < NP; ++c) g[c] = 0.0f;
                 if (wkind == HB_W_CSR) {
-                    const int e0 = I->indptr[n], e1 = I->indptr[n + 1];
-                    if (dmode == HB_DELAY_PER_EDGE) {
-                        const float* rbase = ring;
-                        for (int e = e0; e < e1; ++e) {
-                            const uint2 ed = I->edges[e];
-                            const int j = ed.x & 0xFFFF;
-                            const int d = (ed.x >> 16) & dmask;
-                            const float w = __uint_as_float(ed.y);
-                            int sl = ws - d;
-                            sl += sl < 0 ? Hd : 0;
-                            const float* src = rbase + (size_t)sl * NNP + j * NP;
+                    int e_mem0, e_mem1;
+                    // register-resident rows (steady state): same summation order as the row
+                    const bool reg_ok = csr_reg && (per_edge ? t > 0 : slot_u >= 0);
+                    if (reg_ok) {
+                        const int base = (per_edge ? ws : slot_u) * NNP;
+                        const int wrap = Hd * NNP;
 #pragma unroll
-                            for (int c = 0; c < NP; ++c) g[c] += w * src[c];
+                        for (int k = 0; k < kMaxRegEdges; ++k) {
+                            if (k < kreg) {
+                                int a = base - eo[k];
+                                a += a < 0 ? wrap : 0;
+#pragma unroll
+                                for (int c = 0; c < NP; ++c) g[c] += ew[k] * ring[a + c];
+                            }
                         }
+                        e_mem0 = r_e0 + kreg;
+                        e_mem1 = r_e1;
                     } else {
-                        for (int e = e0; e < e1; ++e) {
-                            const uint2 ed = I->edges[e];
-                            const int j = ed.x & 0xFFFF;
-                            const float w = __uint_as_float(ed.y);
+                        e_mem0 = csr_reg ? r_e0 : V.indptr[n];
+                        e_mem1 = csr_reg ? r_e1 : V.indptr[n + 1];
+                    }
+                    if (e_mem0 < e_mem1) {
+                        if (per_edge) {
+                            for (int e = e_mem0; e < e_mem1; ++e) {
+                                const uint2 ed = edge_at(V.edges, e);
+                                const int j = ed.x & 0xFFFF;
+                                const int d = (ed.x >> 16) & dmask;
+                                const float w = __uint_as_float(ed.y);
+                                int sl = ws - d;
+                                sl += sl < 0 ? Hd : 0;
+                                const auto src = ring + (size_t)sl * NNP + j * NP;
 #pragma unroll
-                            for (int c = 0; c < NP; ++c)
-                                g[c] += w * (slot_u >= 0 ? ring[(size_t)slot_u * NNP + j * NP + c] : fill[c]);
+                                for (int c = 0; c < NP; ++c) g[c] += w * src[c];
+                            }
+                        } else {
+                            for (int e = e_mem0; e < e_mem1; ++e) {
+                                const uint2 ed = edge_at(V.edges, e);
+                                const int j = ed.x & 0xFFFF;
+                                const float w = __uint_as_float(ed.y);
+#pragma unroll
+                                for (int c = 0; c < NP; ++c)
+                                    g[c] += w * (slot_u >= 0 ? ring[(size_t)slot_u * NNP + j * NP + c] : fill[c]);
+                            }
                         }
                     }
                 } else if (wkind == HB_W_DENSE) {
-                    const float* wt = I->dense_t;
+                    const gp<const float> wt = V.dense_t;
                     for (int j = 0; j < N; ++j) {
                         const float w = wt[(size_t)j * N + n];
 #pragma unroll
@@ -442,12 +646,12 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
                 } else if (wkind == HB_W_SCALAR_DIAG) {
 #pragma unroll
                     for (int c = 0; c < NP; ++c)
-                        g[c] = I->scalar_w * (slot_u >= 0 ? ring[(size_t)slot_u * NNP + n * NP + c] : fill[c]);
+                        g[c] = V.scalar_w * (slot_u >= 0 ? ring[(size_t)slot_u * NNP + n * NP + c] : fill[c]);
                 }
                 float cpl[NC];
                 if (wkind != HB_W_NONE) {
                     float own_pre[NP];
-                    const float* op = ring + (size_t)ws * NNP + n * NP;
+                    const auto op = ring + (size_t)ws * NNP + n * NP;
 #pragma unroll
                     for (int c = 0; c < NP; ++c) own_pre[c] = op[c];
                     post<NC, NP>(cfn, cc, classic, g, own_pre, cpl);
@@ -458,75 +662,21 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
 
                 float xs[NV];
                 float mcl[kMC];
-                const float* mc;
                 if (!MULTI) {
-                    mc = mc_n;
+#pragma unroll
+                    for (int i = 0; i < kMC; ++i) mcl[i] = mc_n[i];
 #pragma unroll
                     for (int v = 0; v < NV; ++v) xs[v] = x[v];
                 } else {
 #pragma unroll
                     for (int v = 0; v < NV; ++v) xs[v] = tab_x[n * NV + v];
-                    if (npi >= 0) {
+                    const float* msrc = npi >= 0 ? (const float*)(V.node_mc + (size_t)n * kMC) : Ia->mc;
 #pragma unroll
-                        for (int i = 0; i < kMC; ++i) mcl[i] = I->node_mc[n * kMC + i];
-                        mc = mcl;
-                    } else mc = mc_u;
+                    for (int i = 0; i < kMC; ++i) mcl[i] = msrc[i];
                 }
+                const float* mc = mcl;
 
-                // ---- integrator stages, engine.rs:698-773
-                float k[NV], acc[NV], s[NV];
-                dfun<M>(mc, sw, sv, xs, cpl, k);
-                if (integ == HB_EULER || integ == HB_EULER_STOCH) {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        const float y = xs[v] + k[v] * dt;
-                        xs[v] = stoch ? y + z[v] : y;
-                    }
-                    clampm<M>(xs);
-                } else if (integ == HB_HEUN || integ == HB_HEUN_STOCH) {
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        const float y = xs[v] + k[v] * dt;
-                        s[v] = stoch ? y + z[v] : y;
-                        acc[v] = k[v];
-                    }
-                    clampm<M>(s);
-                    dfun<M>(mc, sw, sv, s, cpl, k);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        const float y = xs[v] + (acc[v] + k[v]) * dth;
-                        xs[v] = stoch ? y + z[v] : y;
-                    }
-                    clampm<M>(xs);
-                } else {  // RK4 / RK4 stochastic
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) { acc[v] = k[v]; s[v] = xs[v] + k[v] * dth; }
-                    clampm<M>(s);
-                    dfun<M>(mc, sw, sv, s, cpl, k);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) { acc[v] = acc[v] + k[v] * 2.0f; s[v] = xs[v] + k[v] * dth; }
-                    clampm<M>(s);
-                    dfun<M>(mc, sw, sv, s, cpl, k);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) { acc[v] = acc[v] + k[v] * 2.0f; s[v] = xs[v] + k[v] * dt; }
-                    clampm<M>(s);
-                    dfun<M>(mc, sw, sv, s, cpl, k);
-#pragma unroll
-                    for (int v = 0; v < NV; ++v) xs[v] = xs[v] + (acc[v] + k[v]) * dt6;
-                    if (stoch) {
-                        if (plain) {   // integrator.rs:296-315: clamp(rk4) + Z, no final clamp
-                            clampm<M>(xs);
-#pragma unroll
-                            for (int v = 0; v < NV; ++v) xs[v] = xs[v] + z[v];
-                        } else {       // engine.rs:747-773: clamp(det + Z)
-#pragma unroll
-                            for (int v = 0; v < NV; ++v) xs[v] = xs[v] + z[v];
-                            clampm<M>(xs);
-                        }
-                    } else {
-                        clampm<M>(xs);
-                    }
-                }
+                integrate<M>(S, mc, cpl, z, xs);
 
                 // ---- outputs
                 if (!MULTI) {
@@ -536,8 +686,8 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
 #pragma unroll
                     for (int v = 0; v < NV; ++v) { tab_x[n * NV + v] = xs[v]; tab_t[n * NV + v] = tab_t[n * NV + v] + xs[v]; }
                 }
-                if (I->traj) {
-                    float* tr = I->traj + (size_t)t * I->traj_tstride + (size_t)n * NV;
+                if (V.traj) {
+                    gp<float> tr = V.traj + (size_t)t * V.traj_tstride + (size_t)n * NV;
 #pragma unroll
                     for (int v = 0; v < NV; ++v) tr[v] = xs[v];
                 }
@@ -551,9 +701,9 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
         if (bper > 0) {
             bcount++;
             if (bcount >= bper) {
-                const double dtb = I->dt_ms * (double)bcount;
+                const double dtb = V.dt_ms * (double)bcount;
                 const float dts = (float)(dtb / 1000.0);
-                const double tk = (double)bc.k * dtb + dtb;
+                const double tk = __dadd_rn(__dmul_rn((double)bc.k, dtb), dtb);
                 for (int kk = 0; kk < nper; ++kk) {
                     const int n = MULTI ? tid + kk * BS : tid;
                     if (n < N) {
@@ -561,14 +711,14 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
                         if (!MULTI) b = bw;
                         else { const float* q = tab_b + n * 6; b.s = q[0]; b.f = q[1]; b.v = q[2]; b.q = q[3]; b.acc = q[4]; b.prev = q[5]; }
                         const float sig = bold_step(b, b.acc / (float)bcount, dts);
-                        bold_emit(I, bc, tk, sig, b.prev, I->bold + n, N);
+                        bold_emit(V, bc, tk, sig, b.prev, V.bold + n, N);
                         b.prev = sig;
                         b.acc = 0.0f;
                         if (!MULTI) bw = b;
                         else { float* q = tab_b + n * 6; q[0] = b.s; q[1] = b.f; q[2] = b.v; q[3] = b.q; q[4] = b.acc; q[5] = b.prev; }
                     }
                 }
-                bold_advance(I, bc, tk);
+                bold_advance(V, bc, tk);
                 bcount = 0;
             }
         }
@@ -579,9 +729,9 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
     if (bper > 0) {
         double tlast = bc.tprev;
         if (bcount > 0) {
-            const double dtb = I->dt_ms * (double)bcount;
+            const double dtb = V.dt_ms * (double)bcount;
             const float dts = (float)(dtb / 1000.0);
-            const double tk = (double)bc.k * I->dtb_ms + dtb;
+            const double tk = __dadd_rn(__dmul_rn((double)bc.k, V.dtb_ms), dtb);
             for (int kk = 0; kk < nper; ++kk) {
                 const int n = MULTI ? tid + kk * BS : tid;
                 if (n < N) {
@@ -589,29 +739,29 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
                     if (!MULTI) b = bw;
                     else { const float* q = tab_b + n * 6; b.s = q[0]; b.f = q[1]; b.v = q[2]; b.q = q[3]; b.acc = q[4]; b.prev = q[5]; }
                     const float sig = bold_step(b, b.acc / (float)bcount, dts);
-                    bold_emit(I, bc, tk, sig, b.prev, I->bold + n, N);
+                    bold_emit(V, bc, tk, sig, b.prev, V.bold + n, N);
                     b.prev = sig;
                     if (!MULTI) bw = b;
                     else tab_b[n * 6 + 5] = sig;
                 }
             }
-            bold_advance(I, bc, tk);
+            bold_advance(V, bc, tk);
             tlast = tk;
         }
         (void)tlast;
-        if (!I->bold_native && bc.k > 0) {
+        if (!V.bold_native && bc.k > 0) {
             for (int kk = 0; kk < nper; ++kk) {
                 const int n = MULTI ? tid + kk * BS : tid;
                 if (n < N) {
                     const float last = MULTI ? tab_b[n * 6 + 5] : bw.prev;
-                    for (int jj = bc.j; jj < I->bold_nout; ++jj) I->bold[(size_t)jj * N + n] = last;
+                    for (int jj = bc.j; jj < V.bold_nout; ++jj) V.bold[(size_t)jj * N + n] = last;
                 }
             }
         }
     }
 
     // ---- tavg * (1/n_steps), final state (engine.rs:813-818)
-    const float inv = I->inv_nsteps;
+    const float inv = V.inv_nsteps;
     for (int kk = 0; kk < nper; ++kk) {
         const int n = MULTI ? tid + kk * BS : tid;
         if (n < N) {
@@ -619,12 +769,276 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
             for (int v = 0; v < NV; ++v) {
                 const float xv = MULTI ? tab_x[n * NV + v] : x[v];
                 const float tt = MULTI ? tab_t[n * NV + v] : tv[v];
-                I->fin[n * NV + v] = xv;
-                I->tavg[n * NV + v] = tt * inv;
+                V.fin[n * NV + v] = xv;
+                V.tavg[n * NV + v] = tt * inv;
             }
         }
     }
     __syncthreads();
+}
+
+
+// ------------------------------------------------------- delay-blocked path -
+//
+// For per-edge CSR delays whose minimum d_min is >= K, the coupling inputs of
+// K consecutive steps t0..t0+K-1 read only history times <= t0-1, already in
+// the ring when the block starts. One barrier per block suffices, and each
+// edge's K values are one contiguous run of the node-major ring
+// [np][N][Hd] (Hd >= max_delay + K + 1 so in-block writes never alias reads).
+// Per-step arithmetic and the per-row summation order are unchanged.
+template <int M, int NP, bool LRING, int K>
+__device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float* lds) {
+    const DevInst* __restrict__ Ia = I;
+    const View V = make_view(I);
+    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
+    const int N = V.n_nodes, T = V.n_steps, Hd = V.ring_depth;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const int Hs = Hd + K - 1;      // row stride: ring slots + mirrored copy of slots 0..K-2
+    const int NHd = N * Hs;
+    using RingPtr = std::conditional_t<LRING, float*, gp<float>>;
+    RingPtr ring;
+    if constexpr (LRING) ring = lds; else ring = V.ring;
+    const int cfn = V.cfn;
+    const bool cpu_form = V.cpu_form != 0, classic = V.sjr_classic != 0;
+    StepCtx S;
+    S.dt = V.dt; S.dth = V.dt_half; S.dt6 = V.dt_sixth; S.integ = V.integrator;
+    S.stoch = S.integ >= HB_EULER_STOCH; S.plain = V.engine == HB_ENGINE_PLAIN;
+    S.sw = V.sweep_form != 0; S.sv = V.sweep_val;
+    float cc[kCC];
+#pragma unroll
+    for (int i = 0; i < kCC; ++i) cc[i] = Ia->cc[i];
+    float nsc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) nsc[v] = Ia->nscale[v];
+    const gp<const float> noise = V.noise;
+    const gp<float> traj = V.traj;
+    const int64_t tstride = V.traj_tstride;
+    const int bper = V.bold_period;
+
+    for (int i = tid; i < NP * NHd; i += BS) ring[i] = 0.0f;   // per-edge: exact-zero fill
+
+    const bool own = tid < N;
+    const int n = tid;
+    float mc[kMC];
+    {
+        const float* msrc = (V.node_param_idx >= 0 && own) ? (const float*)(V.node_mc + (size_t)n * kMC) : Ia->mc;
+#pragma unroll
+        for (int i = 0; i < kMC; ++i) mc[i] = msrc[i];
+    }
+    float x[NV], tv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) { x[v] = own ? V.x0[n * NV + v] : 0.0f; tv[v] = 0.0f; }
+    Bold bw;
+    bw.s = 0.0f; bw.f = 1.0f; bw.v = 1.0f; bw.q = 1.0f; bw.acc = 0.0f; bw.prev = 0.0f;
+    BoldClock bc;
+    bc.k = 0; bc.j = 0; bc.tprev = 0.0;
+    int bcount = 0;
+
+    // CSR row in registers: source base j*Hd, delay, weight
+    const int kreg = min(V.max_row, kMaxRegEdges);
+    int e0 = 0, e1 = 0;
+    if (own) { e0 = V.indptr[n]; e1 = V.indptr[n + 1]; }
+    uint32_t epk[kMaxRegEdges];   // src | delay << 16
+    float ewt[kMaxRegEdges];
+#pragma unroll
+    for (int k = 0; k < kMaxRegEdges; ++k) {
+        epk[k] = own ? (uint32_t)n : 0u; ewt[k] = 0.0f;   // padding: zero weight, own row
+        if (k < kreg && own && e0 + k < e1) {
+            const uint2 ed = edge_at(V.edges, e0 + k);
+            epk[k] = ed.x;
+            ewt[k] = __uint_as_float(ed.y);
+        }
+    }
+    const int em0 = e0 + kreg;   // memory tail of long rows
+
+    // own pre(x_t), kept for Kuramoto's target term
+    float own_pre[NP];
+    auto write_pre = [&](int slot) {
+        float xc[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) xc[c] = x[c];
+        pre<NC, NP>(cfn, cc, cpu_form, classic, xc, own_pre);
+        if (own) {
+#pragma unroll
+            for (int c = 0; c < NP; ++c) {
+                ring[c * NHd + n * Hs + slot] = own_pre[c];
+                if (slot < K - 1) ring[c * NHd + n * Hs + Hd + slot] = own_pre[c];   // mirror
+            }
+        }
+    };
+    auto finish_step = [&](int t, const float* g, const float* zraw) {
+        float cpl[NC], z[NV];
+        post<NC, NP>(cfn, cc, classic, g, own_pre, cpl);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) z[v] = zraw[v] * nsc[v];
+        integrate<M>(S, mc, cpl, z, x);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) tv[v] = tv[v] + x[v];
+        if (traj && own) {
+            gp<float> tr = traj + (size_t)t * tstride + (size_t)n * NV;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) tr[v] = x[v];
+        }
+        if (bper > 0) {
+            bw.acc = bw.acc + x[0];
+            if (++bcount >= bper) {
+                const double dtb = V.dt_ms * (double)bcount;
+                const float dts = (float)(dtb / 1000.0);
+                const double tk = __dadd_rn(__dmul_rn((double)bc.k, dtb), dtb);
+                const float sig = bold_step(bw, bw.acc / (float)bcount, dts);
+                if (own) bold_emit(V, bc, tk, sig, bw.prev, V.bold + n, N);
+                bw.prev = sig;
+                bw.acc = 0.0f;
+                bold_advance(V, bc, tk);
+                bcount = 0;
+            }
+        }
+    };
+
+    // ---- step 0: every edge reads the current state (runtime.rs:162-164)
+    write_pre(0);
+    __syncthreads();
+    if (T > 0) {
+        float g[NP];
+#pragma unroll
+        for (int c = 0; c < NP; ++c) g[c] = 0.0f;
+        for (int e = e0; e < e1; ++e) {
+            const uint2 ed = edge_at(V.edges, e);
+            const int j = ed.x & 0xFFFF;
+            const float w = __uint_as_float(ed.y);
+#pragma unroll
+            for (int c = 0; c < NP; ++c) g[c] += w * ring[c * NHd + j * Hs];
+        }
+        float z0[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) z0[v] = (S.stoch && own) ? noise[(size_t)n * NV + v] : 0.0f;
+        finish_step(0, g, z0);
+        write_pre(1 % Hd);
+    }
+
+    // ---- blocks of K steps starting at t0 = 1
+    int ws0 = 1 % Hd;   // ring slot of time t0
+    for (int t0 = 1; t0 < T; t0 += K) {
+        __syncthreads();
+        const int kb = min(K, T - t0);
+        // the block's noise, issued before the gather so its latency overlaps it
+        float zb[K][NV];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                zb[k][v] = (S.stoch && own && k < kb) ? noise[((size_t)(t0 + k) * N + n) * NV + v] : 0.0f;
+        float acc[K][NP];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+#pragma unroll
+            for (int c = 0; c < NP; ++c) acc[k][c] = 0.0f;
+        // each edge's K window is contiguous (mirrored tail). Software pipeline:
+        // the loads of edge e+1 are issued before edge e is accumulated, so two
+        // edges' loads are in flight; accumulation stays in row order per step.
+        auto edge_base = [&](int e) {
+            int sl = ws0 - (int)(epk[e] >> 16);
+            sl += sl < 0 ? Hd : 0;
+            return (int)(epk[e] & 0xFFFF) * Hs + sl;
+        };
+        float vbuf[2][K][NP];
+        if (kreg > 0) {
+            const int b0 = edge_base(0);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int c = 0; c < NP; ++c) vbuf[0][k][c] = ring[c * NHd + b0 + k];
+        }
+#pragma unroll
+        for (int e = 0; e < kMaxRegEdges; ++e) {
+            if (e < kreg) {
+                if (e + 1 < kMaxRegEdges && e + 1 < kreg) {
+                    const int b1 = edge_base(e + 1);
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+#pragma unroll
+                        for (int c = 0; c < NP; ++c) vbuf[(e + 1) & 1][k][c] = ring[c * NHd + b1 + k];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                const float w = ewt[e];
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) acc[k][c] += w * vbuf[e & 1][k][c];
+            }
+        }
+        for (int e = em0; e < e1; ++e) {        // rows longer than the register budget
+            const uint2 ed = edge_at(V.edges, e);
+            int sl = ws0 - (int)(ed.x >> 16);
+            sl += sl < 0 ? Hd : 0;
+            const int base = (int)(ed.x & 0xFFFF) * Hs + sl;
+            const float w = __uint_as_float(ed.y);
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int c = 0; c < NP; ++c) acc[k][c] += w * ring[c * NHd + base + k];
+        }
+        // rolled step loop: slot 0 of the block registers is the current step,
+        // shifted down after each step (one integrator copy in the code)
+#pragma nounroll
+        for (int k = 0; k < kb; ++k) {
+            finish_step(t0 + k, acc[0], zb[0]);
+            int sl = ws0 + k + 1;
+            sl -= sl >= Hd ? Hd : 0;
+            write_pre(sl);
+#pragma unroll
+            for (int q = 0; q + 1 < K; ++q) {
+#pragma unroll
+                for (int c = 0; c < NP; ++c) acc[q][c] = acc[q + 1][c];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) zb[q][v] = zb[q + 1][v];
+            }
+        }
+        ws0 += K;
+        ws0 -= ws0 >= Hd ? Hd : 0;
+    }
+
+    // ---- BOLD flush and TR tail rows
+    if (bper > 0) {
+        if (bcount > 0) {
+            const double dtb = V.dt_ms * (double)bcount;
+            const float dts = (float)(dtb / 1000.0);
+            const double tk = __dadd_rn(__dmul_rn((double)bc.k, V.dtb_ms), dtb);
+            const float sig = bold_step(bw, bw.acc / (float)bcount, dts);
+            if (own) bold_emit(V, bc, tk, sig, bw.prev, V.bold + n, N);
+            bw.prev = sig;
+            bold_advance(V, bc, tk);
+        }
+        if (!V.bold_native && bc.k > 0 && own)
+            for (int jj = bc.j; jj < V.bold_nout; ++jj) V.bold[(size_t)jj * N + n] = bw.prev;
+    }
+    if (own) {
+        const float inv = V.inv_nsteps;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            V.fin[n * NV + v] = x[v];
+            V.tavg[n * NV + v] = tv[v] * inv;
+        }
+    }
+    __syncthreads();
+}
+
+template <int M, int NP, bool LRING, bool MULTI>
+__device__ __forceinline__ void run_any(const DevInst* __restrict__ I, float* lds) {
+    if constexpr (!MULTI) {
+        const int bk = __builtin_amdgcn_readfirstlane(I->blocked_k);
+        if (bk == kBlockSteps) { run_blocked<M, NP, LRING, kBlockSteps>(I, lds); return; }
+    }
+    run_instance<M, NP, LRING, MULTI>(I, lds);
+}
+
+// The descriptor address is the same in every lane; saying so lets the
+// compiler read descriptor fields with scalar loads kept in SGPRs.
+__device__ __forceinline__ const DevInst* uniform_inst(const DevInst* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (const DevInst*)(((uint64_t)hi << 32) | lo);
 }
 
 // Work distribution: non-persistent (one workgroup per work item, hardware
@@ -645,7 +1059,7 @@ __global__ __launch_bounds__(512) void k_single(LaunchDesc a) {
     for (int it = 0;; ++it) {
         const int w = next_work(a, it);
         if (w >= a.n_work) break;
-        run_instance<M, NP, LRING, MULTI>(a.insts + a.order[w], lds);
+        run_any<M, NP, LRING, MULTI>(uniform_inst(a.insts + a.order[w]), lds);
     }
 }
 
@@ -655,16 +1069,16 @@ __global__ __launch_bounds__(512) void k_mixed(LaunchDesc a) {
     for (int it = 0;; ++it) {
         const int w = next_work(a, it);
         if (w >= a.n_work) break;
-        const DevInst* I = a.insts + a.order[w];
+        const DevInst* I = uniform_inst(a.insts + a.order[w]);
         const int key = __builtin_amdgcn_readfirstlane(I->model * 4 + I->np);
         switch (key) {
-        case HB_G2DO * 4 + 1: run_instance<HB_G2DO, 1, LRING, MULTI>(I, lds); break;
-        case HB_MPR * 4 + 2: run_instance<HB_MPR, 2, LRING, MULTI>(I, lds); break;
-        case HB_JANSEN_RIT * 4 + 1: run_instance<HB_JANSEN_RIT, 1, LRING, MULTI>(I, lds); break;
-        case HB_WILSON_COWAN * 4 + 1: run_instance<HB_WILSON_COWAN, 1, LRING, MULTI>(I, lds); break;
-        case HB_KURAMOTO * 4 + 1: run_instance<HB_KURAMOTO, 1, LRING, MULTI>(I, lds); break;
-        case HB_KURAMOTO * 4 + 2: run_instance<HB_KURAMOTO, 2, LRING, MULTI>(I, lds); break;
-        case HB_RWW * 4 + 1: run_instance<HB_RWW, 1, LRING, MULTI>(I, lds); break;
+        case HB_G2DO * 4 + 1: run_any<HB_G2DO, 1, LRING, MULTI>(I, lds); break;
+        case HB_MPR * 4 + 2: run_any<HB_MPR, 2, LRING, MULTI>(I, lds); break;
+        case HB_JANSEN_RIT * 4 + 1: run_any<HB_JANSEN_RIT, 1, LRING, MULTI>(I, lds); break;
+        case HB_WILSON_COWAN * 4 + 1: run_any<HB_WILSON_COWAN, 1, LRING, MULTI>(I, lds); break;
+        case HB_KURAMOTO * 4 + 1: run_any<HB_KURAMOTO, 1, LRING, MULTI>(I, lds); break;
+        case HB_KURAMOTO * 4 + 2: run_any<HB_KURAMOTO, 2, LRING, MULTI>(I, lds); break;
+        case HB_RWW * 4 + 1: run_any<HB_RWW, 1, LRING, MULTI>(I, lds); break;
         default: break;
         }
     }

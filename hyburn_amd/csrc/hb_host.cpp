@@ -214,6 +214,12 @@ int hb_device_count(void) {
     return n;
 }
 
+int hb_device_synchronize(int32_t device) {
+    HB_HIP(hipSetDevice(device));
+    HB_HIP(hipDeviceSynchronize());
+    return HB_OK;
+}
+
 const char* hb_select_backend(const char* name) {
     if (name && (!strcmp(name, "hip") || !strcmp(name, "rocm"))) return "hip";
     g_err = std::string("backend '") + (name ? name : "(null)") + "' is not provided by this library (HIP only)";
@@ -271,7 +277,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
 
     // ---- per-instance shapes, LDS-vs-HBM ring, block size
     struct Shape {
-        int nv, nc, np, depth, block;
+        int nv, nc, np, depth, block, blocked;
         bool multi, lds_ring;
         int lds_bytes;
         double cost;
@@ -288,8 +294,26 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         b = std::max(64, std::min(512, b));
         s.block = b;
         s.multi = I.n_nodes > b;
-        const int ring_b = s.depth * I.n_nodes * s.np * 4;
         const int tab_b = s.multi ? I.n_nodes * (2 * s.nv + 6) * 4 : 0;
+        // delay-blocked schedule: per-edge delays with min delay >= K, one node per thread
+        s.blocked = 0;
+        if (!s.multi && I.wkind == HB_W_CSR && I.delay_mode == HB_DELAY_PER_EDGE && I.edge_delays &&
+            !(opts && (opts->flags & HB_FLAG_NO_BLOCKING))) {
+            int dmin = 1 << 30;
+            for (int e = 0; e < I.nnz; ++e) dmin = std::min(dmin, (int)I.edge_delays[e]);
+            const int dmax = max_delay_of(I);
+            for (int K : {8}) {   // the kernel's delay-block length
+                if (dmin < K) continue;
+                const int64_t ring_k = (int64_t)(dmax + K + 1 + K - 1) * I.n_nodes * s.np * 4;
+                if (ring_k <= lds_budget || K == 8) {
+                    s.blocked = K;
+                    s.depth = dmax + K + 1;
+                    break;
+                }
+            }
+        }
+        // ring floats: depth slots per node and channel (+ K-1 mirrored slots when blocked)
+        const int ring_b = (s.depth + (s.blocked ? s.blocked - 1 : 0)) * I.n_nodes * s.np * 4;
         s.lds_ring = (int64_t)ring_b + tab_b <= lds_budget;
         s.lds_bytes = (s.lds_ring ? ring_b : 0) + tab_b;
         if (s.lds_bytes > lds_budget) {
@@ -419,7 +443,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         }
         if (!s.lds_ring) {
             q.has_ring = true;
-            q.ring = A.take((size_t)s.depth * N * s.np * 4);
+            q.ring = A.take((size_t)(s.depth + (s.blocked ? s.blocked - 1 : 0)) * N * s.np * 4);
         }
         P->outs[i].n_state = N * NV;
         q.tavg = A.take(N * NV * 4);
@@ -473,6 +497,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
                        : (I.delay_mode == HB_DELAY_PER_EDGE ? HB_DELAY_SINGLE : I.delay_mode);
         D.single_delay = I.single_delay;
         D.ring_depth = s.depth;
+        D.blocked_k = s.blocked;
         D.sweep_form = I.sweep_form; D.sweep_val = I.sweep_val;
         D.node_param_idx = q.has_node_mc ? I.node_param_idx : -1;
         D.engine = I.engine;
@@ -492,6 +517,8 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
             D.cc[1] = 0.0f;
         }
         D.sjr_classic = classic;
+        if (I.wkind == HB_W_CSR)
+            for (int r = 0; r < I.n_nodes; ++r) D.max_row = std::max(D.max_row, I.indptr[r + 1] - I.indptr[r]);
         D.scalar_w = I.scalar_w;
         if (I.cfn == HB_CF_DIFFERENCE && (I.wkind == HB_W_SCALAR_MEAN || I.wkind == HB_W_SCALAR_DIAG))
             D.scalar_w = 0.0f;   // engine.rs:330-333

@@ -142,7 +142,7 @@ struct alignas(16) DevInst {
     int32_t np, wkind, cfn, delay_mode;
     int32_t single_delay, ring_depth, sweep_form, node_param_idx;
     int32_t bold_period, bold_nout, bold_native, cpu_form;
-    int32_t sjr_classic, engine, pad0, pad1;
+    int32_t sjr_classic, engine, max_row, blocked_k;   // longest CSR row; delay-block length (0: per step)
     float dt, dt_half, dt_sixth, inv_nsteps;
     float sweep_val, scalar_w, fill0, fill1;
     float mc[kMC];

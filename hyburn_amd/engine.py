@@ -1,0 +1,457 @@
+"""Host-side mirror of hyburn's engine API for the batched integration path.
+
+Same names, argument meaning and error classes as the reference, backed by
+the HIP library through the C ABI:
+
+  SimConfig / SweepConfig       src/config.rs:83-114, 457-503
+  SweepParam / SweepPoint / SweepSpec / BatchSweepResult
+                                src/engine/batch_engine/engine.rs:15-91
+  BatchHybridEngine.from_config / run_sweep / run_sweep_with_trajectory /
+  run_sweep_multi               engine.rs:145-149, 477-494, 845-860
+  write_npy_f32                 src/io.rs:88-125
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+
+try:  # Python 3.11+: tomllib; 3.10 image: tomli
+    import tomllib as _toml
+except ImportError:  # pragma: no cover
+    import tomli as _toml
+
+
+class SimulationError(L.HyburnError):
+    pass
+
+
+def _err(code: int, msg: str):
+    raise L.HyburnError(code, msg)
+
+
+# ---------------------------------------------------------------- config ----
+
+@dataclass
+class SubnetworkConfig:
+    model: str
+    nnodes: int
+    params: list
+    initial_state: object            # list of floats (var-major as in TOML) or a .npy path
+    nmodes: int = 1
+
+
+@dataclass
+class ProjectionConfig:
+    src: int
+    tgt: int
+    weights: object                   # float | [[...]] | {data, indices, indptr}
+    delays: list = field(default_factory=list)
+    tract_lengths: list = field(default_factory=list)
+    coupling_fn: str = "Linear"
+    coupling_params: list = field(default_factory=list)
+    cvar_map: str = "0:0"
+    conn_type: str = "all_to_all"
+
+
+@dataclass
+class MonitorConfig:
+    monitor_type: str
+    period: Optional[float] = None
+    tr: Optional[float] = 2.0
+    bold_period: Optional[int] = 10
+
+
+@dataclass
+class SimConfig:
+    sim_length: float
+    dt: float
+    subnetworks: list
+    projections: list = field(default_factory=list)
+    integrator: str = "heun"
+    monitors: list = field(default_factory=list)
+    nsig: object = 0.0
+    speed: float = 3.0
+    backend: str = "ndarray"
+    noise_mode: str = "reproducible"
+    base_dir: str = "."
+
+    @classmethod
+    def from_dict(cls, d: dict, base_dir: str = ".") -> "SimConfig":
+        try:
+            net = d["network"]
+            subs = [SubnetworkConfig(model=s["model"], nnodes=int(s["nnodes"]), params=list(s["params"]),
+                                     initial_state=s["initial_state"], nmodes=int(s.get("nmodes", 1)))
+                    for s in net["subnetworks"]]
+            projs = [ProjectionConfig(src=int(p["src"]), tgt=int(p["tgt"]), weights=p["weights"],
+                                      delays=list(p.get("delays", [])), tract_lengths=list(p.get("tract_lengths", [])),
+                                      coupling_fn=p.get("coupling_fn", "Linear"),
+                                      coupling_params=list(p.get("coupling_params", [])),
+                                      cvar_map=p.get("cvar_map", "0:0"), conn_type=p.get("conn_type", "all_to_all"))
+                     for p in net.get("projections", [])]
+            mons = [MonitorConfig(monitor_type=m["monitor_type"], period=m.get("period"), tr=m.get("tr", 2.0),
+                                  bold_period=m.get("bold_period", 10)) for m in d.get("monitors", [])]
+            return cls(sim_length=float(d["sim_length"]), dt=float(d["dt"]), subnetworks=subs, projections=projs,
+                       integrator=str(d.get("integrator", "heun")).lower(), monitors=mons, nsig=d.get("nsig", 0.0),
+                       speed=float(d.get("speed", 3.0)), backend=d.get("backend", "ndarray"),
+                       noise_mode=d.get("noise_mode", "reproducible"), base_dir=base_dir)
+        except (KeyError, TypeError, ValueError) as e:
+            _err(L.HB_INVALID_CONFIG, f"config parse error: {e}")
+
+    @classmethod
+    def from_toml_str(cls, s: str, base_dir: str = ".") -> "SimConfig":
+        try:
+            return cls.from_dict(_toml.loads(s), base_dir)
+        except _toml.TOMLDecodeError as e:
+            _err(L.HB_INVALID_CONFIG, f"TOML parse error: {e}")
+
+    @classmethod
+    def from_json_str(cls, s: str, base_dir: str = ".") -> "SimConfig":
+        try:
+            return cls.from_dict(json.loads(s), base_dir)
+        except json.JSONDecodeError as e:
+            _err(L.HB_INVALID_CONFIG, f"JSON parse error: {e}")
+
+    @classmethod
+    def from_file(cls, path: str) -> "SimConfig":
+        try:
+            txt = open(path).read()
+        except OSError as e:
+            _err(L.HB_IO_ERROR, str(e))
+        base = os.path.dirname(os.path.abspath(path))
+        return cls.from_json_str(txt, base) if path.endswith(".json") else cls.from_toml_str(txt, base)
+
+    def n_steps(self) -> int:
+        """(sim_length / dt) as usize: f64 truncation (cli.rs:901)."""
+        return int(self.sim_length / self.dt)
+
+    def validate(self) -> None:
+        """SimConfig::validate, config.rs:286-428."""
+        if self.dt <= 0:
+            _err(L.HB_INVALID_CONFIG, "dt must be positive")
+        if self.sim_length <= 0:
+            _err(L.HB_INVALID_CONFIG, "sim_length must be positive")
+        if not self.subnetworks:
+            _err(L.HB_INVALID_CONFIG, "At least one subnetwork required")
+        for i, s in enumerate(self.subnetworks):
+            if s.model not in L.MODEL_NAMES:
+                _err(L.HB_INVALID_CONFIG, f"Unknown model '{s.model}' in subnetwork {i}")
+            nv, _, npar = L.MODEL_INFO[L.MODEL_NAMES[s.model]]
+            if s.nmodes < 1:
+                _err(L.HB_INVALID_CONFIG, f"Subnetwork {i}: nmodes must be >= 1")
+            if len(s.params) != npar:
+                _err(L.HB_INVALID_CONFIG, f"Subnetwork {i}: model '{s.model}' expects {npar} params, got {len(s.params)}")
+            if isinstance(s.initial_state, list) and len(s.initial_state) != nv * s.nnodes * s.nmodes:
+                _err(L.HB_INVALID_CONFIG, f"Subnetwork {i}: initial_state has {len(s.initial_state)} values, "
+                                          f"expected {nv * s.nnodes * s.nmodes}")
+        n_subs = len(self.subnetworks)
+        for i, p in enumerate(self.projections):
+            if p.src >= n_subs or p.tgt >= n_subs:
+                _err(L.HB_INVALID_CONFIG, f"Projection {i}: src/tgt out of range")
+            ns, nt = self.subnetworks[p.src].nnodes, self.subnetworks[p.tgt].nnodes
+            w = p.weights
+            if isinstance(w, dict):
+                if len(w["indptr"]) != nt + 1:
+                    _err(L.HB_INVALID_CONFIG, f"Projection {i}: CSR indptr length {len(w['indptr'])} != tgt_nnodes + 1 = {nt + 1}")
+                if len(w["data"]) != len(w["indices"]):
+                    _err(L.HB_INVALID_CONFIG, f"Projection {i}: CSR data.len() != indices.len()")
+            elif isinstance(w, list):
+                if len(w) != nt or any(len(r) != ns for r in w):
+                    _err(L.HB_INVALID_CONFIG, f"Projection {i}: dense weights shape mismatch")
+            elif ns != nt:
+                _err(L.HB_INVALID_CONFIG, f"Projection {i}: scalar weights require src and tgt nnodes to match")
+            md = max(p.delays) if p.delays else 0
+            if md + 1 > self.n_steps() + 1:
+                _err(L.HB_INVALID_CONFIG, f"Projection {i}: max delay {md} requires horizon >= {md + 1}, "
+                                          f"but sim only runs {self.n_steps()} steps")
+
+    def resolve_delays(self) -> None:
+        """ceil(tract_length / speed / dt), config.rs:436-452."""
+        if self.speed <= 0 and any(p.tract_lengths for p in self.projections):
+            _err(L.HB_INVALID_CONFIG, "speed must be positive when tract_lengths are provided")
+        for p in self.projections:
+            if not p.delays and p.tract_lengths:
+                dt = np.float32(self.dt)
+                p.delays = [int(max(math.ceil(np.float32(l) / np.float32(self.speed) / dt), 0)) for l in p.tract_lengths]
+
+
+@dataclass
+class SweepRange:
+    start: float
+    step: float
+    end: float
+
+
+@dataclass
+class SweepConfig:
+    """config.rs:457-503."""
+    parameter_name: str
+    values: Optional[list] = None
+    range: Optional[SweepRange] = None
+
+    @classmethod
+    def from_file(cls, path: str) -> "SweepConfig":
+        d = _toml.loads(open(path).read())
+        r = d.get("range")
+        return cls(parameter_name=d["parameter_name"], values=d.get("values"),
+                   range=SweepRange(**r) if r else None)
+
+    def generate_values(self) -> list:
+        if self.values is not None:
+            return [float(np.float32(v)) for v in self.values]
+        if self.range is not None:
+            st, sp, en = (np.float32(self.range.start), np.float32(self.range.step), np.float32(self.range.end))
+            if sp <= 0:
+                return []
+            n = int(math.ceil(np.float32((en - st) / sp))) + 1
+            return [float(np.float32(st + np.float32(i) * sp)) for i in range(n)]
+        return []
+
+
+def parse_sweep_param(name: str, cfg: SimConfig):
+    """'subnetworks[i].params[j]' -> (i, j), cli.rs:976-999."""
+    if name in ("dt", "nsig"):
+        _err(L.HB_INVALID_CONFIG, f"Batch sweep does not support sweeping {name} (model parameter sweeps only)")
+    if name.startswith("subnetworks[") and "].params[" in name:
+        sub = int(name[name.index("[") + 1:name.index("]")])
+        par = int(name[name.rindex("[") + 1:name.rindex("]")])
+        if sub >= len(cfg.subnetworks):
+            _err(L.HB_INVALID_CONFIG, f"subnetwork index {sub} out of range")
+        return sub, par
+    _err(L.HB_INVALID_CONFIG, f"unsupported sweep parameter_name: {name}. Use format: subnetworks[N].params[M]")
+
+
+@dataclass
+class SweepParam:
+    sub_idx: int
+    param_idx: int
+
+
+@dataclass
+class SweepPoint:
+    sub_idx: int
+    param_idx: int
+    values: list
+    kind: int = 0          # 0: model parameter, 1: coupling parameter (extension)
+
+
+@dataclass
+class SweepSpec:
+    points: list
+
+    def n_sweep(self) -> int:
+        n = 1
+        for p in self.points:
+            n *= len(p.values)
+        return n
+
+    def cartesian_product(self) -> list:
+        """Last dimension fastest, engine.rs:69-90."""
+        out = []
+        idx = [0] * len(self.points)
+        for _ in range(self.n_sweep()):
+            out.append([(p.sub_idx, p.param_idx, p.values[idx[i]]) for i, p in enumerate(self.points)])
+            for i in range(len(self.points) - 1, -1, -1):
+                idx[i] += 1
+                if idx[i] < len(self.points[i].values):
+                    break
+                idx[i] = 0
+        return out
+
+
+@dataclass
+class BatchSweepResult:
+    """engine.rs:15-31 (tavg/final per subnetwork, flat [n_sweep, nnodes, nvar])."""
+    param_values: list
+    tavg: list
+    final_states: list
+    trajectories: Optional[list]
+    bold: Optional[list]
+    elapsed_ms: float
+    n_sweep: int
+    kernel_ms: float = 0.0
+
+
+# ---------------------------------------------------------------- engine ----
+
+def _load_initial_state(sub: SubnetworkConfig, nv: int, base_dir: str) -> np.ndarray:
+    """Node-major [nnodes, nvar] initial state. Inline values are node-major in
+    the batch engine (engine.rs:161-170); a 2-D .npy [nvar, nnodes] is
+    transposed (engine.rs:177-196)."""
+    s = sub.initial_state
+    if isinstance(s, str):
+        path = s if os.path.isabs(s) else os.path.join(base_dir, s)
+        try:
+            a = np.load(path, allow_pickle=False)
+        except OSError as e:
+            _err(L.HB_IO_ERROR, str(e))
+        if a.ndim == 2:
+            return np.ascontiguousarray(a.astype(np.float32).T)
+        if a.ndim == 3 and a.shape[2] == 1:
+            return np.ascontiguousarray(a[:, :, 0].astype(np.float32).T)
+        _err(L.HB_INVALID_CONFIG, f"Initial state NPY has {a.ndim} dims, expected 2 or 3")
+    return np.asarray(s, np.float32).reshape(sub.nnodes, nv)
+
+
+class BatchHybridEngine:
+    """BatchHybridEngine<B> with B = the HIP backend."""
+
+    def __init__(self):
+        self._h = C.c_void_p()
+        self._keep: list = []
+
+    @classmethod
+    def from_config(cls, base_config: SimConfig, n_sweep: int, device: int = 0, *,
+                    engine: int = L.ENGINE_BATCH, delay_mode: int = -1, noise_seed: int = 0,
+                    noise_shared: bool = True) -> "BatchHybridEngine":
+        cfg = base_config
+        if len(cfg.subnetworks) != 1:
+            _err(L.HB_INVALID_CONFIG, "multiple subnetworks are not supported by this backend")
+        sub = cfg.subnetworks[0]
+        if sub.model not in L.MODEL_NAMES:
+            _err(L.HB_INVALID_CONFIG, f"Unknown model '{sub.model}'")
+        if sub.nmodes != 1:
+            _err(L.HB_INVALID_CONFIG, "nmodes > 1 is not supported by this backend")
+        model = L.MODEL_NAMES[sub.model]
+        nv = L.MODEL_INFO[model][0]
+        self = cls()
+        k = self._keep
+        self.model, self.nnodes, self.nvar, self.n_sweep = model, sub.nnodes, nv, n_sweep
+        self.dt = cfg.dt
+        params = np.ascontiguousarray(sub.params, np.float32)
+        x0 = np.ascontiguousarray(_load_initial_state(sub, nv, cfg.base_dir))
+        k += [params, x0]
+        S = L.HbSubnetwork(model, sub.nnodes, len(params), params.ctypes.data_as(L.fp), x0.ctypes.data_as(L.fp))
+        k.append(S)
+        c = L.HbSimConfig()
+        c.sim_length, c.dt = cfg.sim_length, cfg.dt
+        if cfg.integrator not in L.INTEGRATOR_NAMES:
+            _err(L.HB_INVALID_CONFIG, f"unknown integrator {cfg.integrator}")
+        c.integrator = L.INTEGRATOR_NAMES[cfg.integrator]
+        nsig = np.atleast_1d(np.asarray(cfg.nsig, np.float32))
+        k.append(nsig)
+        c.n_nsig, c.nsig = len(nsig), nsig.ctypes.data_as(L.fp)
+        c.subnetworks, c.n_subnetworks = C.pointer(S), 1
+        if cfg.projections:
+            if len(cfg.projections) > 1:
+                _err(L.HB_INVALID_CONFIG, "at most one (self) projection is supported by this backend")
+            p = cfg.projections[0]
+            if p.cvar_map != "0:0":
+                _err(L.HB_INVALID_CONFIG, "cvar_map other than 0:0 is not supported by this backend")
+            if p.coupling_fn not in L.COUPLING_NAMES:
+                _err(L.HB_INVALID_CONFIG, f"Unknown coupling function: {p.coupling_fn}")
+            Pj = L.HbProjection()
+            w = p.weights
+            if isinstance(w, dict):
+                ip = np.ascontiguousarray(w["indptr"], np.int32)
+                ix = np.ascontiguousarray(w["indices"], np.int32)
+                dat = np.ascontiguousarray(w["data"], np.float32)
+                k += [ip, ix, dat]
+                Pj.wkind, Pj.indptr, Pj.indices, Pj.data, Pj.nnz = (L.W_CSR, ip.ctypes.data_as(L.ip),
+                                                                    ix.ctypes.data_as(L.ip), dat.ctypes.data_as(L.fp), len(dat))
+            elif isinstance(w, (list, np.ndarray)):
+                dn = np.ascontiguousarray(w, np.float32)
+                k.append(dn)
+                Pj.wkind, Pj.dense = L.W_DENSE, dn.ctypes.data_as(L.fp)
+            else:
+                Pj.wkind, Pj.scalar_w = L.W_SCALAR_MEAN, float(w)
+            dl = np.ascontiguousarray(p.delays, np.int32)
+            k.append(dl)
+            Pj.delays, Pj.n_delays = (dl.ctypes.data_as(L.ip) if len(dl) else None), len(dl)
+            Pj.cfn = L.COUPLING_NAMES[p.coupling_fn]
+            cp = list(p.coupling_params)[:8]
+            Pj.n_cparams = len(cp)
+            for i, v in enumerate(cp):
+                Pj.cparams[i] = v
+            k.append(Pj)
+            c.projections, c.n_projections = C.pointer(Pj), 1
+        bold = [m for m in cfg.monitors if m.monitor_type.lower() == "bold"]
+        if bold:
+            m = bold[0]
+            bp = m.bold_period if m.bold_period is not None else max(1, round((m.period or 2000.0) / cfg.dt))
+            c.bold_period, c.bold_tr_s = int(bp), float(m.tr if m.tr is not None else 2.0)
+        c.engine, c.delay_mode, c.noise_seed, c.noise_shared = engine, delay_mode, noise_seed, int(noise_shared)
+        self._cfg = c
+        L.check(L.lib().hb_engine_create(C.byref(c), n_sweep, device, C.byref(self._h)))
+        return self
+
+    def _result(self, r: L.HbResult, values, traj: bool) -> BatchSweepResult:
+        n = r.n_sweep * r.n_nodes * r.nvar
+        tavg = np.ctypeslib.as_array(r.tavg, (n,)).copy()
+        fin = np.ctypeslib.as_array(r.final_states, (n,)).copy()
+        trj = np.ctypeslib.as_array(r.trajectories, (n * r.n_steps,)).copy() if traj and r.n_steps else None
+        bold = None
+        if r.bold and r.n_bold > 0:
+            bold = [np.ctypeslib.as_array(r.bold, (r.n_sweep * r.n_bold * r.n_nodes,)).copy()
+                    .reshape(r.n_sweep, r.n_bold, r.n_nodes)]
+        out = BatchSweepResult(param_values=list(values), tavg=[tavg], final_states=[fin],
+                               trajectories=[trj] if trj is not None else None, bold=bold,
+                               elapsed_ms=r.elapsed_ms, n_sweep=r.n_sweep, kernel_ms=r.kernel_ms)
+        L.lib().hb_result_free(C.byref(r))
+        return out
+
+    def _run(self, param: SweepParam, values, n_steps: int, traj: bool, exact: bool) -> BatchSweepResult:
+        v = np.ascontiguousarray(values, np.float32)
+        r = L.HbResult()
+        flags = (L.FLAG_TRAJECTORY if traj else 0) | (L.FLAG_EXACT if exact else 0)
+        L.check(L.lib().hb_run_sweep(self._h, param.sub_idx, param.param_idx, v.ctypes.data_as(L.fp), len(v),
+                                     int(n_steps), flags, C.byref(r)))
+        return self._result(r, v.tolist(), traj)
+
+    def run_sweep(self, param: SweepParam, param_values, n_steps: int, exact: bool = False) -> BatchSweepResult:
+        return self._run(param, param_values, n_steps, False, exact)
+
+    def run_sweep_with_trajectory(self, param: SweepParam, param_values, n_steps: int,
+                                  exact: bool = False) -> BatchSweepResult:
+        return self._run(param, param_values, n_steps, True, exact)
+
+    def run_sweep_multi(self, spec: SweepSpec, n_steps: int, exact: bool = False,
+                        trajectory: bool = False) -> BatchSweepResult:
+        arr = (L.HbSweepPoint * len(spec.points))()
+        keep = []
+        for i, p in enumerate(spec.points):
+            v = np.ascontiguousarray(p.values, np.float32)
+            keep.append(v)
+            arr[i] = L.HbSweepPoint(p.kind, p.sub_idx, p.param_idx, v.ctypes.data_as(L.fp), len(v))
+        r = L.HbResult()
+        flags = (L.FLAG_TRAJECTORY if trajectory else 0) | (L.FLAG_EXACT if exact else 0)
+        L.check(L.lib().hb_run_sweep_multi(self._h, arr, len(spec.points), int(n_steps), flags, C.byref(r)))
+        return self._result(r, [pt[0][2] for pt in spec.cartesian_product()], trajectory)
+
+    def close(self):
+        if self._h:
+            L.lib().hb_engine_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- NPY -------
+
+def write_npy_f32(path: str, data, shape) -> None:
+    """Byte-compatible with io.rs:88-125: v1.0 header, '<f4', padded so that
+    magic + header + newline is a multiple of 64."""
+    shape = list(shape)
+    shape_str = f"({shape[0]},)" if len(shape) == 1 else "(" + ", ".join(str(s) for s in shape) + ")"
+    header = "{'descr': '<f4', 'fortran_order': False, 'shape': " + shape_str + " }"
+    rem = (len(header) + 10 + 1) % 64
+    header += " " * ((64 - rem) if rem else 0) + "\n"
+    arr = np.ascontiguousarray(data, dtype="<f4").ravel()
+    with open(path, "wb") as f:
+        f.write(b"\x93NUMPY")
+        f.write(bytes([1, 0]))
+        f.write(len(header).to_bytes(2, "little"))
+        f.write(header.encode("latin1"))
+        f.write(arr.tobytes())

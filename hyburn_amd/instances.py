@@ -178,7 +178,7 @@ class _Marshal:
             out.bold = np.zeros((max(rows, 1), I.n_nodes), np.float32)
             s.bold = _fptr(out.bold)
             s.bold_cap = rows
-            s.bold_n = C.cast(C.byref(self.bold_n, i * 4), L.ip)
+            s.bold_n = C.cast(self.bold_n.ctypes.data + i * 4, L.ip)
         self.outputs.append(out)
 
     def finish(self) -> list[Output]:

@@ -48,6 +48,7 @@ enum { HB_ENGINE_BATCH = 0, HB_ENGINE_PLAIN = 1 };
 #define HB_FLAG_TRAJECTORY 0x1u   /* record [n_steps][n_sweep][n_nodes][nvar] (engine.rs:487-494) */
 #define HB_FLAG_EXACT 0x2u        /* no FMA contraction: same f32 rounding as the written expressions */
 #define HB_FLAG_NO_GRAPH 0x4u     /* launch directly instead of replaying a captured hipGraph */
+#define HB_FLAG_NO_BLOCKING 0x8u  /* disable the delay-blocked schedule (one barrier per step) */
 
 /* ---------------- config-level API (BatchHybridEngine mirror) ---------------- */
 
@@ -206,6 +207,8 @@ int hb_philox_normal(uint64_t seed, uint64_t stream, int64_t n, float* out_host,
 const char* hb_last_error(void);   /* thread-local message of the last failure */
 const char* hb_version(void);
 int hb_device_count(void);
+/* Wait for all work on a device (hipDeviceSynchronize). */
+int hb_device_synchronize(int32_t device);
 /* Backend selection, the `--backend hip` value of select_backend (cli.rs:259-271).
  * Returns "hip" for "hip"/"rocm"; anything else returns NULL and sets the last
  * error: this library has no CPU fallback. */

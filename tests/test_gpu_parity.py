@@ -253,3 +253,68 @@ def test_config2_full_length_exact_subset(gpu, orc):
         o = orc.run(I)
         np.testing.assert_array_equal(g.tavg, o["tavg"])
         np.testing.assert_array_equal(g.final_state, o["final_state"])
+
+
+# ---- delay-blocked schedule (per-edge delays with min delay >= 8) ----
+
+def _blocked_net(model, N=24, T=400, integ=L.HEUN, dmin=8, dmax=60, dens=0.3, seed=0, **kw):
+    rng = np.random.default_rng(seed)
+    I = net(model, N=N, T=T, integ=integ, dens=dens, seed=seed, **kw)
+    I.edge_delays = rng.integers(dmin, dmax + 1, len(I.indices)).astype(np.int32)
+    return I
+
+
+@pytest.mark.parametrize("model", EXACT_MODELS)
+@pytest.mark.parametrize("integ", [L.EULER, L.HEUN, L.RK4, L.EULER_STOCH, L.HEUN_STOCH, L.RK4_STOCH])
+def test_blocked_exact_bitwise(gpu, orc, model, integ):
+    I = _blocked_net(model, integ=integ, seed=500 + 7 * model + integ)
+    (g,), _ = run_instances([I], exact=True)
+    _assert_exact(g, orc.run(I), f"blocked model={model} integ={integ}")
+
+
+def test_blocked_matches_per_step_schedule(gpu):
+    """The blocked and the one-barrier-per-step schedules give identical bits."""
+    insts = [_blocked_net(m, integ=L.RK4, seed=600 + m) for m in (L.G2DO, L.MPR, L.JANSEN_RIT, L.RWW)]
+    a, _ = run_instances(insts, exact=True)
+    m = __import__("hyburn_amd.instances", fromlist=["_Marshal"])
+    import ctypes as C
+    mm = m._Marshal(insts)
+    st = L.HbRunStats()
+    o = m._opts(0, True, True, 0, 0)
+    o.flags |= L.FLAG_NO_BLOCKING
+    L.check(L.lib().hb_run_instances(mm.arr, len(insts), C.byref(o), C.byref(st)))
+    for x, y in zip(a, mm.finish()):
+        np.testing.assert_array_equal(x.traj, y.traj)
+
+
+def test_blocked_global_ring_and_long_rows(gpu, orc):
+    """Ring in HBM (300 nodes x 700 slots) and rows longer than 32 register edges."""
+    I = _blocked_net(L.G2DO, N=300, T=300, dens=0.2, dmin=9, dmax=690, seed=601)
+    assert I.indptr is not None and np.max(np.diff(I.indptr)) > 32
+    (g,), st = run_instances([I], exact=True)
+    assert st.lds_bytes == 0
+    _assert_exact(g, orc.run(I), "blocked global ring")
+
+
+def test_blocked_jr_sjr_and_kuramoto_fn(gpu, orc):
+    for I in (_blocked_net(L.JANSEN_RIT, integ=L.RK4, T=200, seed=602),
+              _blocked_net(L.KURAMOTO, integ=L.RK4, T=300, seed=603, cfn=L.CF_KURAMOTO, cparams=(1.2,))):
+        (g,), _ = run_instances([I], exact=True)
+        np.testing.assert_allclose(g.traj, orc.run(I)["traj"], rtol=1e-5, atol=1e-5)
+
+
+def test_blocked_bold_and_node_params(gpu, orc):
+    rng = np.random.default_rng(604)
+    I = _blocked_net(L.MPR, N=20, T=1207, integ=L.HEUN, seed=604, bold_period=10, bold_tr_ms=2.5,
+                     record_bold=True, record_traj=False, node_param_idx=2,
+                     node_param=rng.uniform(-6, -4, 20).astype(np.float32))
+    (g,), _ = run_instances([I], exact=True)
+    o = orc.run(I)
+    np.testing.assert_array_equal(g.tavg, o["tavg"])
+    np.testing.assert_allclose(g.bold, o["bold"], rtol=1e-5, atol=1e-6)
+
+
+def test_blocked_fast_mode_error(gpu, orc):
+    I = _blocked_net(L.MPR, integ=L.RK4, T=300, seed=605)
+    (g,), _ = run_instances([I], exact=False)
+    _err_bound(g.traj, orc.run(I)["traj"], orc.run(I, dtype=np.float64)["traj"])
