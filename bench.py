@@ -26,10 +26,10 @@ sys.path.insert(0, ROOT)
 # 20 FLOP, 2 x 7 FLOP of stochastic Heun update per state variable, 2 FLOP per
 # CSR edge at 14.6 edges/row, 2 FLOP Linear post): 85 FLOP.
 FLOP_PER_NODE_STEP = 2 * 20 + 2 * 7 + 2 * 14.6 + 2
-# Algorithmic HBM bytes per node-step: the f32 noise of 2 state variables is
-# written once by the noise kernel and read once by the integrator (the delay
-# ring lives in LDS; tavg/final outputs amortise to ~0).
-BYTES_PER_NODE_STEP = 2 * 4 * 2
+# Algorithmic HBM bytes per node-step of the forward kernel: the f32 noise of
+# 2 state variables, read once (the delay ring lives in LDS; tavg/final
+# outputs, 2 x 4 B per node and variable, are added per launch).
+BYTES_PER_NODE_STEP = 2 * 4
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
@@ -61,6 +61,22 @@ def cpu_baseline(insts, threads: int):
     secs = hb_oracle.run_many_f32(insts, n_threads=threads)
     ns = sum(I.n_nodes * I.n_steps for I in insts)
     return ns / secs, secs
+
+
+def copy_calibration(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB: the reference dispatch for
+    scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes (read mib, write mib)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0        # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy calibration: {mib} MiB device-to-device")
 
 
 def ragged_leg(args):
@@ -108,6 +124,8 @@ def main() -> int:
     ap.add_argument("--ragged-n", type=int, default=4096)
     ap.add_argument("--ragged-tscale", type=float, default=1.0)
     ap.add_argument("--ragged-plain-sample", type=int, default=16)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -123,6 +141,8 @@ def main() -> int:
     if lib.hb_device_count() < 1:
         print("no HIP device", file=sys.stderr)
         return 1
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     insts = config2(n_sweep=args.n_sweep, n_steps=args.n_steps)
     node_steps = sum(I.n_nodes * I.n_steps for I in insts)
     log(f"[bench] workload: {len(insts)} x {insts[0].n_nodes} nodes x {insts[0].n_steps} steps")
@@ -146,7 +166,11 @@ def main() -> int:
     ms_per_step = wall / args.steps * 1e3
     value = node_steps * args.steps / wall
     k_ms = sum(kernel_ms) / len(kernel_ms)
-    achieved = FLOP_PER_NODE_STEP * node_steps / (k_ms * 1e-3) / 1e12
+    flops = FLOP_PER_NODE_STEP * node_steps / (k_ms * 1e-3) / 1e12
+    # The roofline that binds: bytes/BW (8 B / 8 TB/s) exceeds FLOP/peak
+    # (85 FLOP / 157.3 TFLOP/s) per node-step, so the kernel is priced in HBM bytes.
+    alg_bytes = BYTES_PER_NODE_STEP * node_steps + 2 * 4 * sum(I.n_nodes * I.nvar for I in insts)
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     traffic = read_traffic()
     line = {
         "metric": "node-steps/s (ragged sweep, B=128..4096) + adjoint grads/s vs plain & FD",
@@ -168,18 +192,20 @@ def main() -> int:
             "parallelism": "1 workgroup per sweep point, delay ring in LDS",
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "hbm",
             "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS,
             "traffic": traffic,
+            "algorithmic_bytes": alg_bytes,
             "kernel": "hb::fast::k_single<G2DO,1,lds_ring>",
             "kernel_ms": k_ms,
+            "fp32_tflops": flops,
+            "fp32_frac": flops / PEAK_FP32_TFLOPS,
             "flop_per_node_step": FLOP_PER_NODE_STEP,
-            "note": "FP32 VALU work against the FP32 peak (vector and matrix FP32 peaks are both 157.3 "
-                    "TFLOP/s on MI355X); the kernel is latency bound by 10k sequential steps per point",
-            "hbm_achieved_GBs": BYTES_PER_NODE_STEP * node_steps / (k_ms * 1e-3) / 1e9,
+            "note": "each sweep point is a chain of 10k dependent steps: the kernel is latency bound, "
+                    "far below both the HBM and the FP32 roofline",
         },
         "timing": {"kernel_ms": k_ms, "noise_plus_kernel_ms": sum(total_ms) / len(total_ms)},
     }

@@ -29,10 +29,16 @@ DEVICE_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 
+# hb_forward.hip is compiled once per kernel unit (7 single-model units, 2
+# mixed-model units) and mode, so the units build in parallel.
+FORWARD_UNITS = 9
 UNITS = [
     # (source, object, extra flags, device?)
-    ("hb_forward.hip", "hb_forward_fast.o", ["-DHB_EXACT=0"], True),
-    ("hb_forward.hip", "hb_forward_exact.o", ["-DHB_EXACT=1", "-ffp-contract=off"], True),
+    *[("hb_forward.hip", f"hb_forward_fast_u{u}.o", ["-DHB_EXACT=0", f"-DHB_UNIT={u}"], True)
+      for u in range(FORWARD_UNITS)],
+    *[("hb_forward.hip", f"hb_forward_exact_u{u}.o", ["-DHB_EXACT=1", f"-DHB_UNIT={u}", "-ffp-contract=off"], True)
+      for u in range(FORWARD_UNITS)],
+    ("hb_forward_dispatch.cpp", "hb_forward_dispatch.o", [], False),
     ("hb_noise.hip", "hb_noise.o", [], True),
     ("hb_adjoint.hip", "hb_adjoint.o", [], True),
     ("hb_host.cpp", "hb_host.o", [], False),
@@ -54,7 +60,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     """Compile every unit for gfx950 and link libhyburn_hip.so in-tree."""
     os.makedirs(BUILD, exist_ok=True)
     units = [u for u in UNITS if os.path.exists(os.path.join(CSRC, u[0]))]
-    jobs = jobs or min(8, len(units))
+    jobs = jobs or min(os.cpu_count() or 8, 16, len(units))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         results = list(ex.map(lambda u: _compile(*u), units))
     failed = [r for r in results if r[1] != 0]

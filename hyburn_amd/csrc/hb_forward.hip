@@ -234,7 +234,7 @@ __device__ __forceinline__ uint2 edge_at(gp<const uint32_t> e, int i) {
 struct View {
     int model, integrator, n_nodes, n_steps, np, wkind, cfn, delay_mode, single_delay, ring_depth;
     int sweep_form, node_param_idx, bold_period, bold_nout, bold_native, cpu_form, sjr_classic, engine;
-    int max_row, blocked_k;
+    int max_row, blocked_k, lpn, lpn_shift;
     float dt, dt_half, dt_sixth, inv_nsteps, sweep_val, scalar_w, fill0, fill1;
     double dt_ms, tr_ms, dtb_ms;
     int64_t traj_tstride;
@@ -253,6 +253,7 @@ __device__ __forceinline__ View make_view(const DevInst* I) {
     V.bold_period = u32(I->bold_period); V.bold_nout = u32(I->bold_nout); V.bold_native = u32(I->bold_native);
     V.cpu_form = u32(I->cpu_form); V.sjr_classic = u32(I->sjr_classic); V.engine = u32(I->engine);
     V.max_row = u32(I->max_row); V.blocked_k = u32(I->blocked_k);
+    V.lpn = u32(I->lpn); V.lpn_shift = u32(I->lpn_shift);
     V.dt = uf32(I->dt); V.dt_half = uf32(I->dt_half); V.dt_sixth = uf32(I->dt_sixth); V.inv_nsteps = uf32(I->inv_nsteps);
     V.sweep_val = uf32(I->sweep_val); V.scalar_w = uf32(I->scalar_w); V.fill0 = uf32(I->fill0); V.fill1 = uf32(I->fill1);
     V.dt_ms = uf64(I->dt_ms); V.tr_ms = uf64(I->tr_ms); V.dtb_ms = uf64(I->dtb_ms);
@@ -261,6 +262,15 @@ __device__ __forceinline__ View make_view(const DevInst* I) {
     V.indptr = ug(I->indptr); V.edges = ug((const uint32_t*)I->edges);
     V.ring = ug(I->ring); V.tavg = ug(I->tavg); V.fin = ug(I->fin); V.traj = ug(I->traj); V.bold = ug(I->bold);
     return V;
+}
+
+// Sum over the lanes that share a node (1, 2 or 4 adjacent lanes of a quad).
+// Both quad swizzles are DPP moves; float addition is commutative, so every
+// lane of the group ends with bitwise the same total.
+__device__ __forceinline__ float lane_group_sum(float v, int lpn) {
+    if (lpn >= 2) v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, false));
+    if (lpn >= 4) v = v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, false));
+    return v;
 }
 
 // ---------------------------------------------------------------- BOLD -----
@@ -398,6 +408,72 @@ __device__ __forceinline__ void integrate(const StepCtx& S, const float* mc, con
             }
         } else {
             clampm<M>(xs);
+        }
+    }
+}
+
+// Integrator class at compile time: 0 Euler(-Maruyama), 1 Heun, 2 RK4; the
+// stochastic and plain-engine variants are selects on the same code.
+constexpr int integ_class(int integ) {
+    return (integ == HB_EULER || integ == HB_EULER_STOCH) ? 0 : (integ == HB_HEUN || integ == HB_HEUN_STOCH) ? 1 : 2;
+}
+
+template <int M, int IC>
+__device__ __forceinline__ void integrate_c(const StepCtx& S, const float* mc, const float* cpl,
+                                            const float* z, float* xs) {
+    constexpr int NV = Tr<M>::NV;
+    float k[NV], acc[NV], s[NV];
+    dfun<M>(mc, S.sw, S.sv, xs, cpl, k);
+    if constexpr (IC == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const float y = xs[v] + k[v] * S.dt;
+            xs[v] = S.stoch ? y + z[v] : y;
+        }
+        clampm<M>(xs);
+    } else if constexpr (IC == 1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const float y = xs[v] + k[v] * S.dt;
+            s[v] = S.stoch ? y + z[v] : y;
+            acc[v] = k[v];
+        }
+        clampm<M>(s);
+        dfun<M>(mc, S.sw, S.sv, s, cpl, k);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const float y = xs[v] + (acc[v] + k[v]) * S.dth;
+            xs[v] = S.stoch ? y + z[v] : y;
+        }
+        clampm<M>(xs);
+    } else {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { acc[v] = k[v]; s[v] = xs[v] + k[v] * S.dth; }
+        clampm<M>(s);
+        dfun<M>(mc, S.sw, S.sv, s, cpl, k);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { acc[v] = acc[v] + k[v] * 2.0f; s[v] = xs[v] + k[v] * S.dth; }
+        clampm<M>(s);
+        dfun<M>(mc, S.sw, S.sv, s, cpl, k);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { acc[v] = acc[v] + k[v] * 2.0f; s[v] = xs[v] + k[v] * S.dt; }
+        clampm<M>(s);
+        dfun<M>(mc, S.sw, S.sv, s, cpl, k);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xs[v] = xs[v] + (acc[v] + k[v]) * S.dt6;
+        // clamp(det + Z) in the batch engine; clamp(det) + Z (no final clamp) in
+        // the plain engine (integrator.rs:296-315); deterministic: clamp(det)
+        float zz[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) zz[v] = xs[v] + z[v];
+        if (S.stoch && !S.plain) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xs[v] = zz[v];
+        }
+        clampm<M>(xs);
+        if (S.stoch && S.plain) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xs[v] = xs[v] + z[v];
         }
     }
 }
@@ -786,7 +862,7 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
 // edge's K values are one contiguous run of the node-major ring
 // [np][N][Hd] (Hd >= max_delay + K + 1 so in-block writes never alias reads).
 // Per-step arithmetic and the per-row summation order are unchanged.
-template <int M, int NP, bool LRING, int K>
+template <int M, int NP, bool LRING, int K, int IC>
 __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float* lds) {
     const DevInst* __restrict__ Ia = I;
     const View V = make_view(I);
@@ -817,8 +893,11 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
 
     for (int i = tid; i < NP * NHd; i += BS) ring[i] = 0.0f;   // per-edge: exact-zero fill
 
-    const bool own = tid < N;
-    const int n = tid;
+    // lanes per node: lane `sub` of node n gathers row edges sub, sub+lpn, ...
+    const int lpn = V.lpn, sub = tid & (lpn - 1);
+    const int n = tid >> V.lpn_shift;
+    const bool own = n < N;
+    const bool lead = own && sub == 0;   // writes the node's outputs and ring entries
     float mc[kMC];
     {
         const float* msrc = (V.node_param_idx >= 0 && own) ? (const float*)(V.node_mc + (size_t)n * kMC) : Ia->mc;
@@ -835,7 +914,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     int bcount = 0;
 
     // CSR row in registers: source base j*Hd, delay, weight
-    const int kreg = min(V.max_row, kMaxRegEdges);
+    const int kreg = min((V.max_row + lpn - 1) >> V.lpn_shift, kMaxRegEdges);
     int e0 = 0, e1 = 0;
     if (own) { e0 = V.indptr[n]; e1 = V.indptr[n + 1]; }
     uint32_t epk[kMaxRegEdges];   // src | delay << 16
@@ -843,13 +922,13 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
 #pragma unroll
     for (int k = 0; k < kMaxRegEdges; ++k) {
         epk[k] = own ? (uint32_t)n : 0u; ewt[k] = 0.0f;   // padding: zero weight, own row
-        if (k < kreg && own && e0 + k < e1) {
-            const uint2 ed = edge_at(V.edges, e0 + k);
+        if (k < kreg && own && e0 + sub + k * lpn < e1) {
+            const uint2 ed = edge_at(V.edges, e0 + sub + k * lpn);
             epk[k] = ed.x;
             ewt[k] = __uint_as_float(ed.y);
         }
     }
-    const int em0 = e0 + kreg;   // memory tail of long rows
+    const int em0 = e0 + sub + kreg * lpn;   // memory tail of long rows (strided)
 
     // own pre(x_t), kept for Kuramoto's target term
     float own_pre[NP];
@@ -858,7 +937,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
 #pragma unroll
         for (int c = 0; c < NC; ++c) xc[c] = x[c];
         pre<NC, NP>(cfn, cc, cpu_form, classic, xc, own_pre);
-        if (own) {
+        if (lead) {
 #pragma unroll
             for (int c = 0; c < NP; ++c) {
                 ring[c * NHd + n * Hs + slot] = own_pre[c];
@@ -871,10 +950,10 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         post<NC, NP>(cfn, cc, classic, g, own_pre, cpl);
 #pragma unroll
         for (int v = 0; v < NV; ++v) z[v] = zraw[v] * nsc[v];
-        integrate<M>(S, mc, cpl, z, x);
+        integrate_c<M, IC>(S, mc, cpl, z, x);
 #pragma unroll
         for (int v = 0; v < NV; ++v) tv[v] = tv[v] + x[v];
-        if (traj && own) {
+        if (traj && lead) {
             gp<float> tr = traj + (size_t)t * tstride + (size_t)n * NV;
 #pragma unroll
             for (int v = 0; v < NV; ++v) tr[v] = x[v];
@@ -886,7 +965,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
                 const float dts = (float)(dtb / 1000.0);
                 const double tk = __dadd_rn(__dmul_rn((double)bc.k, dtb), dtb);
                 const float sig = bold_step(bw, bw.acc / (float)bcount, dts);
-                if (own) bold_emit(V, bc, tk, sig, bw.prev, V.bold + n, N);
+                if (lead) bold_emit(V, bc, tk, sig, bw.prev, V.bold + n, N);
                 bw.prev = sig;
                 bw.acc = 0.0f;
                 bold_advance(V, bc, tk);
@@ -902,13 +981,15 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         float g[NP];
 #pragma unroll
         for (int c = 0; c < NP; ++c) g[c] = 0.0f;
-        for (int e = e0; e < e1; ++e) {
+        for (int e = e0 + sub; e < e1; e += lpn) {
             const uint2 ed = edge_at(V.edges, e);
             const int j = ed.x & 0xFFFF;
             const float w = __uint_as_float(ed.y);
 #pragma unroll
             for (int c = 0; c < NP; ++c) g[c] += w * ring[c * NHd + j * Hs];
         }
+#pragma unroll
+        for (int c = 0; c < NP; ++c) g[c] = lane_group_sum(g[c], lpn);
         float z0[NV];
 #pragma unroll
         for (int v = 0; v < NV; ++v) z0[v] = (S.stoch && own) ? noise[(size_t)n * NV + v] : 0.0f;
@@ -967,7 +1048,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
                     for (int c = 0; c < NP; ++c) acc[k][c] += w * vbuf[e & 1][k][c];
             }
         }
-        for (int e = em0; e < e1; ++e) {        // rows longer than the register budget
+        for (int e = em0; e < e1; e += lpn) {   // rows longer than the register budget
             const uint2 ed = edge_at(V.edges, e);
             int sl = ws0 - (int)(ed.x >> 16);
             sl += sl < 0 ? Hd : 0;
@@ -977,6 +1058,12 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
             for (int k = 0; k < K; ++k)
 #pragma unroll
                 for (int c = 0; c < NP; ++c) acc[k][c] += w * ring[c * NHd + base + k];
+        }
+        if (lpn > 1) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int c = 0; c < NP; ++c) acc[k][c] = lane_group_sum(acc[k][c], lpn);
         }
         // rolled step loop: slot 0 of the block registers is the current step,
         // shifted down after each step (one integrator copy in the code)
@@ -1005,14 +1092,14 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
             const float dts = (float)(dtb / 1000.0);
             const double tk = __dadd_rn(__dmul_rn((double)bc.k, V.dtb_ms), dtb);
             const float sig = bold_step(bw, bw.acc / (float)bcount, dts);
-            if (own) bold_emit(V, bc, tk, sig, bw.prev, V.bold + n, N);
+            if (lead) bold_emit(V, bc, tk, sig, bw.prev, V.bold + n, N);
             bw.prev = sig;
             bold_advance(V, bc, tk);
         }
-        if (!V.bold_native && bc.k > 0 && own)
+        if (!V.bold_native && bc.k > 0 && lead)
             for (int jj = bc.j; jj < V.bold_nout; ++jj) V.bold[(size_t)jj * N + n] = bw.prev;
     }
-    if (own) {
+    if (lead) {
         const float inv = V.inv_nsteps;
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -1027,7 +1114,13 @@ template <int M, int NP, bool LRING, bool MULTI>
 __device__ __forceinline__ void run_any(const DevInst* __restrict__ I, float* lds) {
     if constexpr (!MULTI) {
         const int bk = __builtin_amdgcn_readfirstlane(I->blocked_k);
-        if (bk == kBlockSteps) { run_blocked<M, NP, LRING, kBlockSteps>(I, lds); return; }
+        if (bk == kBlockSteps) {
+            const int ic = __builtin_amdgcn_readfirstlane(integ_class(I->integrator));
+            if (ic == 0) run_blocked<M, NP, LRING, kBlockSteps, 0>(I, lds);
+            else if (ic == 1) run_blocked<M, NP, LRING, kBlockSteps, 1>(I, lds);
+            else run_blocked<M, NP, LRING, kBlockSteps, 2>(I, lds);
+            return;
+        }
     }
     run_instance<M, NP, LRING, MULTI>(I, lds);
 }
@@ -1090,66 +1183,66 @@ static const void* kptr_single(bool lring, bool multi) {
     return multi ? (const void*)k_single<M, NP, false, true> : (const void*)k_single<M, NP, false, false>;
 }
 
-// The kernel a launch description selects (nullptr: unsupported combination).
-static const void* kernel_ptr(const LaunchDesc& d) {
-    if (d.model < 0) {
-        if (d.lds_ring) return d.multi ? (const void*)k_mixed<true, true> : (const void*)k_mixed<true, false>;
-        return d.multi ? (const void*)k_mixed<false, true> : (const void*)k_mixed<false, false>;
-    }
-    switch (d.model * 4 + d.np) {
-    case HB_G2DO * 4 + 1: return kptr_single<HB_G2DO, 1>(d.lds_ring, d.multi);
-    case HB_MPR * 4 + 2: return kptr_single<HB_MPR, 2>(d.lds_ring, d.multi);
-    case HB_JANSEN_RIT * 4 + 1: return kptr_single<HB_JANSEN_RIT, 1>(d.lds_ring, d.multi);
-    case HB_WILSON_COWAN * 4 + 1: return kptr_single<HB_WILSON_COWAN, 1>(d.lds_ring, d.multi);
-    case HB_KURAMOTO * 4 + 1: return kptr_single<HB_KURAMOTO, 1>(d.lds_ring, d.multi);
-    case HB_KURAMOTO * 4 + 2: return kptr_single<HB_KURAMOTO, 2>(d.lds_ring, d.multi);
-    case HB_RWW * 4 + 1: return kptr_single<HB_RWW, 1>(d.lds_ring, d.multi);
-    default: return nullptr;
-    }
+// This translation unit's kernels: HB_UNIT 0..6 = one (model, pre channels)
+// pair, 7 = mixed-model kernels with LDS rings, 8 = mixed with HBM rings.
+#ifndef HB_UNIT
+#define HB_UNIT 0
+#endif
+static const void* unit_kernel(bool lring, bool multi) {
+#if HB_UNIT == 0
+    return kptr_single<HB_G2DO, 1>(lring, multi);
+#elif HB_UNIT == 1
+    return kptr_single<HB_MPR, 2>(lring, multi);
+#elif HB_UNIT == 2
+    return kptr_single<HB_JANSEN_RIT, 1>(lring, multi);
+#elif HB_UNIT == 3
+    return kptr_single<HB_WILSON_COWAN, 1>(lring, multi);
+#elif HB_UNIT == 4
+    return kptr_single<HB_KURAMOTO, 1>(lring, multi);
+#elif HB_UNIT == 5
+    return kptr_single<HB_KURAMOTO, 2>(lring, multi);
+#elif HB_UNIT == 6
+    return kptr_single<HB_RWW, 1>(lring, multi);
+#elif HB_UNIT == 7
+    return lring ? (multi ? (const void*)k_mixed<true, true> : (const void*)k_mixed<true, false>) : nullptr;
+#else
+    return !lring ? (multi ? (const void*)k_mixed<false, true> : (const void*)k_mixed<false, false>) : nullptr;
+#endif
 }
 
-static hipError_t launch(const LaunchDesc& d, hipStream_t s) {
-    const void* k = kernel_ptr(d);
-    if (!k) return hipErrorInvalidValue;
-    LaunchDesc a = d;
-    void* args[] = {&a};
-    return hipLaunchKernel(k, dim3(d.grid), dim3(d.block), args, (size_t)d.lds_bytes, s);
-}
-
-static hipError_t occupancy(const LaunchDesc& d, int* blocks) {
-    const void* k = kernel_ptr(d);
-    if (!k) return hipErrorInvalidValue;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k, d.block, (size_t)d.lds_bytes);
-}
-
-static hipError_t attrs(int b) {
-    static const int models[7][2] = {{HB_G2DO, 1}, {HB_MPR, 2}, {HB_JANSEN_RIT, 1}, {HB_WILSON_COWAN, 1},
-                                     {HB_KURAMOTO, 1}, {HB_KURAMOTO, 2}, {HB_RWW, 1}};
-    for (int m = -1; m < 7; ++m)
+// op 0: launch; 1: occupancy (blocks per CU into *out); 2: set the maximum
+// dynamic LDS (arg bytes) on every kernel of the unit.
+static hipError_t unit_op(int op, const LaunchDesc* d, hipStream_t s, int arg, int* out) {
+    if (op == 2) {
         for (int lr = 0; lr < 2; ++lr)
             for (int mu = 0; mu < 2; ++mu) {
-                LaunchDesc d;
-                memset(&d, 0, sizeof(d));
-                d.model = m < 0 ? -1 : models[m][0];
-                d.np = m < 0 ? 0 : models[m][1];
-                d.lds_ring = lr != 0;
-                d.multi = mu != 0;
-                const void* k = kernel_ptr(d);
-                hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, b);
+                const void* k = unit_kernel(lr != 0, mu != 0);
+                if (!k) continue;
+                hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, arg);
                 if (e != hipSuccess) return e;
             }
-    return hipSuccess;
+        return hipSuccess;
+    }
+    const void* k = unit_kernel(d->lds_ring, d->multi);
+    if (!k) return hipErrorInvalidValue;
+    if (op == 1) return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k, d->block, (size_t)d->lds_bytes);
+    LaunchDesc a = *d;
+    void* args[] = {&a};
+    return hipLaunchKernel(k, dim3(d->grid), dim3(d->block), args, (size_t)d->lds_bytes, s);
 }
 
 }  // namespace HB_NS
 
+#define HB_CAT3(a, b, c) a##b##c
+#define HB_XCAT3(a, b, c) HB_CAT3(a, b, c)
 #if HB_EXACT
-hipError_t launch_forward_exact(const LaunchDesc& d, hipStream_t s) { return exact::launch(d, s); }
-hipError_t forward_set_attributes_exact(int max_lds) { return exact::attrs(max_lds); }
+hipError_t HB_XCAT3(fwd_unit_exact_, HB_UNIT, )(int op, const LaunchDesc* d, hipStream_t s, int arg, int* out) {
+    return exact::unit_op(op, d, s, arg, out);
+}
 #else
-hipError_t launch_forward_fast(const LaunchDesc& d, hipStream_t s) { return fast::launch(d, s); }
-hipError_t forward_set_attributes_fast(int max_lds) { return fast::attrs(max_lds); }
-hipError_t forward_occupancy_fast(const LaunchDesc& d, int* blocks) { return fast::occupancy(d, blocks); }
+hipError_t HB_XCAT3(fwd_unit_fast_, HB_UNIT, )(int op, const LaunchDesc* d, hipStream_t s, int arg, int* out) {
+    return fast::unit_op(op, d, s, arg, out);
+}
 #endif
 
 }  // namespace hb

@@ -277,7 +277,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
 
     // ---- per-instance shapes, LDS-vs-HBM ring, block size
     struct Shape {
-        int nv, nc, np, depth, block, blocked;
+        int nv, nc, np, depth, block, blocked, lpn;
         bool multi, lds_ring;
         int lds_bytes;
         double cost;
@@ -297,6 +297,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         const int tab_b = s.multi ? I.n_nodes * (2 * s.nv + 6) * 4 : 0;
         // delay-blocked schedule: per-edge delays with min delay >= K, one node per thread
         s.blocked = 0;
+        s.lpn = 1;
         if (!s.multi && I.wkind == HB_W_CSR && I.delay_mode == HB_DELAY_PER_EDGE && I.edge_delays &&
             !(opts && (opts->flags & HB_FLAG_NO_BLOCKING))) {
             int dmin = 1 << 30;
@@ -308,6 +309,11 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
                 if (ring_k <= lds_budget || K == 8) {
                     s.blocked = K;
                     s.depth = dmax + K + 1;
+                    // lanes per node (FMA mode only: exact mode keeps the row's sequential sum)
+                    if (!exact && req_block <= 0)
+                        for (int l : {4, 2})
+                            if ((int)align_up((size_t)I.n_nodes * l, 64) <= 512) { s.lpn = l; break; }
+                    s.block = std::max(64, (int)align_up((size_t)I.n_nodes * s.lpn, 64));
                     break;
                 }
             }
@@ -498,6 +504,8 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         D.single_delay = I.single_delay;
         D.ring_depth = s.depth;
         D.blocked_k = s.blocked;
+        D.lpn = s.lpn;
+        D.lpn_shift = s.lpn == 4 ? 2 : (s.lpn == 2 ? 1 : 0);
         D.sweep_form = I.sweep_form; D.sweep_val = I.sweep_val;
         D.node_param_idx = q.has_node_mc ? I.node_param_idx : -1;
         D.engine = I.engine;

@@ -143,6 +143,7 @@ struct alignas(16) DevInst {
     int32_t single_delay, ring_depth, sweep_form, node_param_idx;
     int32_t bold_period, bold_nout, bold_native, cpu_form;
     int32_t sjr_classic, engine, max_row, blocked_k;   // longest CSR row; delay-block length (0: per step)
+    int32_t lpn, lpn_shift, pad2, pad3;                // lanes per node in the blocked path (1, 2, 4)
     float dt, dt_half, dt_sixth, inv_nsteps;
     float sweep_val, scalar_w, fill0, fill1;
     float mc[kMC];

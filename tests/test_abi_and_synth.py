@@ -117,3 +117,76 @@ def test_philox_normal_moments_and_determinism():
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
     np.testing.assert_array_equal(z[:1000], philox_normal(123, 5, 1000))
     assert not np.array_equal(z[:1000], philox_normal(123, 6, 1000))
+
+
+def _toml_g2do(n=4):
+    x0 = ", ".join(["0.0"] * (2 * n))
+    return f"""
+sim_length = 10.0
+dt = 0.1
+integrator = "heun"
+[network]
+[[network.subnetworks]]
+model = "Generic2dOscillator"
+nnodes = {n}
+params = [1, 0, -2, -10, 0, 0.02, 3, 1, 0, 1, 1, 1]
+initial_state = [{x0}]
+[[network.projections]]
+src = 0
+tgt = 0
+weights = 0.01
+coupling_fn = "Linear"
+coupling_params = [0.01]
+"""
+
+
+def test_engine_rejects_value_count_mismatch():
+    """n_values != n_sweep is an InvalidConfig error (checked before any device
+    work); the reference broadcasts it unchecked."""
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig, SweepParam
+    cfg = SimConfig.from_toml_str(_toml_g2do())
+    assert cfg.n_steps() == 100
+    eng = BatchHybridEngine.from_config(cfg, n_sweep=3)
+    with pytest.raises(L.HyburnError) as e:
+        eng.run_sweep(SweepParam(0, 1), [0.0, 1.0], 10)
+    assert e.value.code == L.HB_INVALID_CONFIG
+    eng.close()
+
+
+def test_config_parse_errors():
+    from hyburn_amd.engine import SimConfig, parse_sweep_param
+    with pytest.raises(L.HyburnError) as e:
+        SimConfig.from_toml_str("sim_length = ")
+    assert e.value.code == L.HB_INVALID_CONFIG
+    with pytest.raises(L.HyburnError) as e:
+        SimConfig.from_file("/nonexistent/cfg.toml")
+    assert e.value.code == L.HB_IO_ERROR
+    cfg = SimConfig.from_toml_str(_toml_g2do())
+    assert parse_sweep_param("subnetworks[0].params[2]", cfg) == (0, 2)
+    with pytest.raises(L.HyburnError):
+        parse_sweep_param("dt", cfg)
+
+
+def test_n_steps_f64_truncation():
+    """(sim_length / dt) as usize truncates in f64 (cli.rs:901): 4999.95 ms at
+    dt = 0.05 gives 99998 steps, not 99999."""
+    from hyburn_amd.engine import SimConfig
+    cfg = SimConfig.from_toml_str(_toml_g2do())
+    cfg.sim_length, cfg.dt = 4999.95, 0.05
+    assert cfg.n_steps() == 99998
+
+
+def test_write_npy_f32_format(tmp_path):
+    """v1.0 header, '<f4', total header length a multiple of 64 (io.rs:88-125);
+    numpy reads the file back with the written shape."""
+    from hyburn_amd.engine import write_npy_f32
+    data = np.arange(24, dtype=np.float32)
+    p = tmp_path / "x.npy"
+    write_npy_f32(str(p), data, [2, 3, 4])
+    raw = p.read_bytes()
+    assert raw[:8] == b"\x93NUMPY\x01\x00"
+    hlen = int.from_bytes(raw[8:10], "little")
+    assert (10 + hlen) % 64 == 0 and raw[10 + hlen - 1:10 + hlen] == b"\n"
+    back = np.load(str(p), allow_pickle=False)
+    assert back.shape == (2, 3, 4) and back.dtype == np.dtype("<f4")
+    np.testing.assert_array_equal(back.ravel(), data)

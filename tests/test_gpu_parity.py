@@ -318,3 +318,14 @@ def test_blocked_fast_mode_error(gpu, orc):
     I = _blocked_net(L.MPR, integ=L.RK4, T=300, seed=605)
     (g,), _ = run_instances([I], exact=False)
     _err_bound(g.traj, orc.run(I)["traj"], orc.run(I, dtype=np.float64)["traj"])
+
+
+def test_blocked_lanes_per_node_fast_mode(gpu, orc):
+    """FMA mode shares a node over up to 4 lanes; results stay within the FP32
+    error bound and are reproducible run to run."""
+    insts = [_blocked_net(L.G2DO, N=N, integ=L.HEUN_STOCH, T=300, seed=700 + N) for N in (7, 74, 130, 200)]
+    a, _ = run_instances(insts, exact=False)
+    b, _ = run_instances(insts, exact=False)
+    for I, x, y in zip(insts, a, b):
+        np.testing.assert_array_equal(x.traj, y.traj)
+        _err_bound(x.traj, orc.run(I)["traj"], orc.run(I, dtype=np.float64)["traj"])
