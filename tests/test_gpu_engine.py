@@ -143,7 +143,13 @@ def test_run_sweep_multi_cartesian_order(gpu, orc):
 
 def test_run_sweep_bold_per_instance(gpu, orc):
     """A BOLD monitor on the engine gives one BOLD series per sweep point,
-    [n_sweep][rows][N], resampled to TR (bold_monitor.rs:72-227)."""
+    [n_sweep][rows][N], resampled to TR (bold_monitor.rs:72-227).
+
+    FP32 BOLD is coarse: v and q stay within a few ulps of 1 for the first
+    windows, so the FP32 oracle's own error against the FP64 twin reaches
+    ~1e-4 here and a one-ulp difference between device and host powf shows up
+    at ~1e-6. The bar is therefore the FP64-relative one: the GPU's error is at
+    most 4x the FP32 oracle's, + 1e-6; the neural states (tavg) are bit-exact."""
     N, T = 12, 4000
     ip, ix, w, d = _csr_net(N, 17)
     x0 = np.stack([np.full(N, 0.1), np.full(N, -2.0)], 1).astype(np.float32)
@@ -154,13 +160,17 @@ def test_run_sweep_bold_per_instance(gpu, orc):
     r = eng.run_sweep(SweepParam(0, 4), vals, T, exact=True)
     eng.close()
     bold = r.bold[0]
+    tavg = r.tavg[0].reshape(3, N, 2)
     for b, v in enumerate(vals):
         I = _oracle_inst("MontbrioPazoRoxin", N, x0, ip, ix, w, d, engine=L.ENGINE_BATCH,
                          delay_mode=L.DELAY_BATCH_COMPAT, T=T, cp=(0.1, 0.0), sweep_val=v,
                          record_traj=False, bold_period=10, bold_tr_ms=50.0, record_bold=True)
-        o = orc.run(I)
-        assert bold.shape[1] == o["bold"].shape[0] == I.bold_rows()
-        np.testing.assert_allclose(bold[b], o["bold"], rtol=1e-5, atol=1e-6)
+        o32, o64 = orc.run(I), orc.run(I, dtype=np.float64)
+        np.testing.assert_array_equal(tavg[b], o32["tavg"])
+        assert bold.shape[1] == o32["bold"].shape[0] == I.bold_rows()
+        e_gpu = np.max(np.abs(bold[b].astype(np.float64) - o64["bold"]))
+        e_orc = np.max(np.abs(o32["bold"].astype(np.float64) - o64["bold"]))
+        assert e_gpu <= 4 * e_orc + 1e-6, (b, e_gpu, e_orc)
 
 
 def test_run_sweep_stochastic_shared_noise(gpu, orc):
