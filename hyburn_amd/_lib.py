@@ -21,6 +21,7 @@ W_NONE, W_SCALAR_MEAN, W_SCALAR_DIAG, W_DENSE, W_CSR = 0, 1, 2, 3, 4
 DELAY_PER_EDGE, DELAY_SINGLE, DELAY_BATCH_COMPAT = 0, 1, 2
 ENGINE_BATCH, ENGINE_PLAIN = 0, 1
 FLAG_TRAJECTORY, FLAG_EXACT, FLAG_NO_GRAPH, FLAG_NO_BLOCKING = 0x1, 0x2, 0x4, 0x8
+LOSS_FC, LOSS_BOLD_MSE, LOSS_ORDER = 0, 1, 2
 
 MODEL_NAMES = {
     "Generic2dOscillator": G2DO, "MontbrioPazoRoxin": MPR, "JansenRit": JANSEN_RIT,
@@ -109,6 +110,21 @@ class HbResult(C.Structure):
                 ("elapsed_ms", C.c_double), ("kernel_ms", C.c_double)]
 
 
+class HbAdjointSpec(C.Structure):
+    _fields_ = [("loss_kind", C.c_int32), ("target", fp), ("grad_param", C.c_int32),
+                ("loss", fp), ("grad_coupling", fp), ("grad_param_out", fp), ("grad_x0", fp),
+                ("final_state", fp)]
+
+
+class HbAdjointOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("checkpoint_every", C.c_int32), ("flags", C.c_uint32)]
+
+
+class HbAdjointStats(C.Structure):
+    _fields_ = [("forward_ms", C.c_double), ("backward_ms", C.c_double), ("node_steps", C.c_double),
+                ("device_bytes", C.c_int64)]
+
+
 # Every exported function of include/hyburn_hip.h with its prototype.
 PROTOTYPES = {
     "hb_engine_create": (C.c_int, [C.POINTER(HbSimConfig), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
@@ -129,6 +145,12 @@ PROTOTYPES = {
     "hb_device_count": (C.c_int, []),
     "hb_device_synchronize": (C.c_int, [C.c_int32]),
     "hb_select_backend": (C.c_char_p, [C.c_char_p]),
+    "hb_adjoint_plan_create": (C.c_int, [C.POINTER(HbInstance), C.POINTER(HbAdjointSpec), C.c_int32,
+                                         C.POINTER(HbAdjointOpts), C.POINTER(C.c_void_p)]),
+    "hb_adjoint_forward": (C.c_int, [C.c_void_p, C.POINTER(HbAdjointStats)]),
+    "hb_adjoint_backward": (C.c_int, [C.c_void_p, C.POINTER(HbAdjointStats)]),
+    "hb_adjoint_download": (C.c_int, [C.c_void_p]),
+    "hb_adjoint_plan_destroy": (None, [C.c_void_p]),
 }
 
 _lib = None

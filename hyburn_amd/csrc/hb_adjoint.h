@@ -1,0 +1,62 @@
+// hb_adjoint.h -- descriptors shared by the adjoint planner and its kernels.
+#pragma once
+
+#include "hb_internal.h"
+
+namespace hb {
+
+// One instance of an adjoint plan. Device pointers into the plan's arena.
+struct alignas(16) AdjInst {
+    int32_t model, integrator, n_nodes, n_steps;
+    int32_t np, cfn, engine, has_coupling;
+    int32_t seg_len, n_seg, ring_depth, aring_depth;   // S, ceil(T/S), forward and adjoint ring depths
+    int32_t ring_lds, aring_lds, bold_period, n_win;
+    int32_t loss_kind, pad0, pad1, pad2;
+    float dt, dt_half, dt_sixth, pad3;
+    float mc[kMC];
+    float cc[kCC];
+    float nscale[8];
+    double dt_ms;
+    const float* x0;            // [N][NV]
+    const float* node_mc;       // [N][kMC] or null (uniform mc)
+    const float* node_jac;      // [N][kMC] d mc / d p[grad_param], or null
+    const int32_t* indptr;      // CSR by target
+    const uint2* edges;         // .x = src | delay << 16, .y = weight bits
+    const int32_t* indptr_t;    // CSC by source
+    const uint2* edges_t;       // .x = tgt | delay << 16, .y = weight bits
+    const float* noise;         // raw N(0,1) [T][N][NV] or null
+    float* ring;                // HBM ring (forward, then adjoint) when not in LDS
+    float* glog;                // [T][N][np] gathered coupling sums
+    float* ckpt;                // [n_seg][N][NV]
+    float* tape;                // [S + 1][N][NV]
+    float* zlog;                // [n_win][N] BOLD window inputs
+    double* bstate;             // [n_win][N][4] Balloon-Windkessel states (FP64)
+    double* bold;               // [n_win][N] native BOLD samples (FP64)
+    double* dsig;               // [n_win][N] dL/dBOLD
+    float* seedw;               // [n_win][N] dL/d(window input) / count
+    const float* target;        // FC [N][N], BOLD [n_win][N] or order parameter [T]
+    double* fc_work;            // [3N + 2N^2]
+    float* ord_work;            // [S + 1][4]
+    float* fin;                 // [N][NV] final state
+    float* out_loss;            // [1]
+    float* out_gcc;             // [1] dL / d cc[0]
+    float* out_gp;              // [N] or null
+    float* out_gx0;             // [N][NV] or null
+};
+
+// One launch group: instances of one model / coupling kind / nodes-per-thread.
+struct AdjLaunch {
+    const AdjInst* insts;       // device array (contiguous for the group)
+    int32_t n;
+    int32_t model;
+    bool kur;                   // Kuramoto coupling (two pre channels per cvar)
+    int32_t npt;                // nodes per thread (1 or 2)
+    int32_t block;
+    int32_t lds_fwd, lds_rev;   // dynamic LDS bytes
+};
+
+hipError_t adjoint_set_attributes(int max_lds);
+// stage 0: forward save; 1: BOLD forward; 2: loss; 3: BOLD reverse; 4: reverse
+hipError_t launch_adjoint_stage(int stage, const AdjLaunch& L, hipStream_t s);
+
+}  // namespace hb

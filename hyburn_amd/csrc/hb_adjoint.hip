@@ -1,0 +1,1000 @@
+// hb_adjoint.hip -- discrete adjoint of the batched integration path on gfx950.
+//
+// Gradients of a per-instance loss with respect to the coupling scale G
+// (coupling_params[0]), one model parameter per node and the initial state,
+// through the exact discrete recursion the forward kernels run: Euler / Heun /
+// RK4 (+ additive noise), clamps, per-edge delayed CSR coupling and, for the
+// BOLD losses, the Balloon-Windkessel monitor and Pearson FC.
+//
+// Kernels, all one workgroup per instance:
+//   k_adj_forward   forward integration that also logs, per step, the raw
+//                   gathered coupling sums g_t [T][N][np] and writes the state
+//                   every S steps (checkpoints), plus window inputs for BOLD.
+//   k_bold_forward  Balloon-Windkessel over the window inputs (states kept).
+//   k_loss          FC / BOLD-MSE loss and dL/dBOLD.
+//   k_bold_reverse  adjoint of the BOLD monitor -> dL/d(window input).
+//   k_adj_reverse   for each segment, last first: recompute its states from
+//                   the checkpoint and the coupling log (no delay ring and no
+//                   barrier needed), then step the adjoint backwards. The
+//                   adjoint of the delayed coupling is a gather over each
+//                   node's out-edges (CSC) from a ring of coupling adjoints of
+//                   later steps: edge (j -> i, delay d) read x_j(s) at step
+//                   s + d, so lambda_s[j] += w * lambda_g_{s+d}[i].
+//
+// Reference semantics differentiated (maedoc/hyburn file:line):
+//   time loop, stages, clamps   src/engine/batch_engine/engine.rs:676-773, dfun.rs:170-516
+//   per-edge delays             src/engine/runtime.rs:162-204 (d==0 || t==0: current; d>t: 0)
+//   coupling pre/post           src/engine/coupling.rs:204-399
+//   BOLD                        src/engine/bold.rs:106-166, bold_monitor.rs:72-120
+//   FC                          src/sbi/features/fc.rs:11-57
+#include "hb_internal.h"
+#include "hb_adjoint.h"
+
+#include <cmath>
+
+namespace hb {
+namespace adj {
+
+#include "hb_model.h"
+
+// ------------------------------------------------------------ clamp masks --
+// 1 where clampm<M> passes its argument through, 0 where it pins it.
+template <int M> __device__ __forceinline__ void clamp_pass(const float* u, float* m) {
+#pragma unroll
+    for (int v = 0; v < Tr<M>::NV; ++v) m[v] = 1.0f;
+    if constexpr (M == HB_WILSON_COWAN) {
+        m[0] = (u[0] < 0.0f || u[0] > 1.0f) ? 0.0f : 1.0f;
+        m[1] = (u[1] < 0.0f || u[1] > 1.0f) ? 0.0f : 1.0f;
+    } else if constexpr (M == HB_MPR) {
+        m[0] = u[0] < 0.0f ? 0.0f : 1.0f;
+    } else if constexpr (M == HB_RWW) {
+        m[0] = (u[0] < 0.0f || u[0] > 1.0f) ? 0.0f : 1.0f;
+    }
+}
+
+// ------------------------------------------------------------ dfun VJPs ----
+// Given lk = dL/d(dx), accumulate dL/dx into lx, dL/dc into lc, and
+// sum_m jac[m] * dL/dmc[m] into dp (jac = d mc / d parameter).
+template <int M>
+__device__ __forceinline__ void dfun_vjp(const float* mc, const float* x, const float* c, const float* lk,
+                                         float* lx, float* lc, const float* jac, float& dp);
+
+template <> __device__ __forceinline__ void dfun_vjp<HB_G2DO>(const float* mc, const float* x, const float* c,
+                                                              const float* lk, float* lx, float* lc,
+                                                              const float* jac, float& dp) {
+    const float v = x[0], w = x[1];
+    const float iext = c[0] * mc[8] + mc[9];
+    const float v2 = v * v, v3 = v * v2;
+    const float A = (((w * mc[6] + iext) - v3 * mc[4]) + v2 * mc[3]) + v * mc[5];
+    const float B = ((v * mc[1] + v2 * mc[2]) - w * mc[7]) + mc[0];
+    const float a0 = lk[0] * mc[10], a1 = lk[1] * mc[11];
+    lx[0] += a0 * ((mc[5] + 2.0f * v * mc[3]) - 3.0f * v2 * mc[4]) + a1 * (mc[1] + 2.0f * v * mc[2]);
+    lx[1] += a0 * mc[6] - a1 * mc[7];
+    lc[0] += a0 * mc[8];
+    dp += jac[0] * a1 + jac[1] * (a1 * v) + jac[2] * (a1 * v2) + jac[3] * (a0 * v2) - jac[4] * (a0 * v3) +
+          jac[5] * (a0 * v) + jac[6] * (a0 * w) - jac[7] * (a1 * w) + jac[8] * (a0 * c[0]) + jac[9] * a0 +
+          jac[10] * (lk[0] * A) + jac[11] * (lk[1] * B);
+}
+
+template <> __device__ __forceinline__ void dfun_vjp<HB_MPR>(const float* mc, const float* x, const float* c,
+                                                             const float* lk, float* lx, float* lc,
+                                                             const float* jac, float& dp) {
+    const float r = x[0], v = x[1];
+    const float r2 = r * r;
+    const float dv = ((((v * v - r2 * mc[2]) + mc[4]) + r * mc[3]) + c[0] * mc[6]) + c[1] * mc[7];
+    const float b0 = lk[0] * mc[0], b1 = lk[1] * mc[0];
+    lx[0] += b0 * (2.0f * v) + b1 * (mc[3] - 2.0f * r * mc[2]);
+    lx[1] += b0 * (2.0f * r) + b1 * (2.0f * v);
+    lc[0] += b1 * mc[6];
+    lc[1] += b1 * mc[7];
+    dp += jac[0] * (lk[0] * ((r * v) * 2.0f + mc[1]) + lk[1] * dv) + jac[1] * b0 - jac[2] * (b1 * r2) +
+          jac[3] * (b1 * r) + jac[4] * b1 + jac[6] * (b1 * c[0]) + jac[7] * (b1 * c[1]);
+}
+
+// JR sigmoid mc2 / (exp((mc0 - u) * mc1) + 1) and its partials.
+struct Sig { float val, du, dm0, dm1, dm2; };
+__device__ __forceinline__ Sig jr_sig(const float* mc, float u) {
+    const float e = expf((-u + mc[0]) * mc[1]);
+    const float sg = 1.0f / (e + 1.0f);
+    const float core = mc[2] * sg * (1.0f - sg);
+    Sig s;
+    s.val = mc[2] * sg;
+    s.du = core * mc[1];
+    s.dm0 = -core * mc[1];
+    s.dm1 = -core * (mc[0] - u);
+    s.dm2 = sg;
+    return s;
+}
+
+template <> __device__ __forceinline__ void dfun_vjp<HB_JANSEN_RIT>(const float* mc, const float* x, const float* c,
+                                                                    const float* lk, float* lx, float* lc,
+                                                                    const float* jac, float& dp) {
+    const Sig s12 = jr_sig(mc, x[1] - x[2]);
+    const Sig s01 = jr_sig(mc, x[0] * mc[6]);
+    const Sig s03 = jr_sig(mc, x[0] * mc[8]);
+    const float mut = (s01.val * mc[7] + mc[13]) + c[0];
+    float g[kMC];
+#pragma unroll
+    for (int i = 0; i < kMC; ++i) g[i] = 0.0f;
+    lx[3] += lk[0]; lx[4] += lk[1]; lx[5] += lk[2];
+    // dx3 = s12*mc3 - x3*mc4 - x0*mc5
+    const float ls12 = lk[3] * mc[3];
+    lx[3] -= lk[3] * mc[4]; lx[0] -= lk[3] * mc[5];
+    g[3] += lk[3] * s12.val; g[4] -= lk[3] * x[3]; g[5] -= lk[3] * x[0];
+    // dx4 = mut*mc3 - x4*mc4 - x1*mc5
+    const float lmut = lk[4] * mc[3];
+    lx[4] -= lk[4] * mc[4]; lx[1] -= lk[4] * mc[5];
+    g[3] += lk[4] * mut; g[4] -= lk[4] * x[4]; g[5] -= lk[4] * x[1];
+    const float ls01 = lmut * mc[7];
+    g[7] += lmut * s01.val; g[13] += lmut;
+    lc[0] += lmut;
+    // dx5 = (s03*mc9)*mc10 - x5*mc11 - x2*mc12
+    const float ls03 = lk[5] * mc[9] * mc[10];
+    g[9] += lk[5] * s03.val * mc[10]; g[10] += lk[5] * s03.val * mc[9];
+    lx[5] -= lk[5] * mc[11]; lx[2] -= lk[5] * mc[12];
+    g[11] -= lk[5] * x[5]; g[12] -= lk[5] * x[2];
+    // sigmoids
+    lx[1] += ls12 * s12.du; lx[2] -= ls12 * s12.du;
+    lx[0] += ls01 * s01.du * mc[6]; g[6] += ls01 * s01.du * x[0];
+    lx[0] += ls03 * s03.du * mc[8]; g[8] += ls03 * s03.du * x[0];
+    g[0] += ls12 * s12.dm0 + ls01 * s01.dm0 + ls03 * s03.dm0;
+    g[1] += ls12 * s12.dm1 + ls01 * s01.dm1 + ls03 * s03.dm1;
+    g[2] += ls12 * s12.dm2 + ls01 * s01.dm2 + ls03 * s03.dm2;
+#pragma unroll
+    for (int i = 0; i < 14; ++i) dp += jac[i] * g[i];
+}
+
+template <> __device__ __forceinline__ void dfun_vjp<HB_WILSON_COWAN>(const float* mc, const float* x, const float* c,
+                                                                      const float* lk, float* lx, float* lc,
+                                                                      const float* jac, float& dp) {
+    const float e = x[0], i = x[1];
+    const float xe0 = ((e * mc[0] - i * mc[1]) - mc[9]) + c[0];
+    const float xe = (xe0 + mc[18]) * mc[20];
+    const float xi0 = ((e * mc[2] - i * mc[3]) + mc[19]) - mc[13];
+    const float xi = xi0 * mc[21];
+    const float ee = expf((xe + (-mc[7])) * (-mc[6]));
+    const float sge = 1.0f / (ee + 1.0f);
+    const float se = (sge - mc[22]) * mc[8];
+    const float ei = expf((xi + (-mc[11])) * (-mc[10]));
+    const float sgi = 1.0f / (ei + 1.0f);
+    const float si = (sgi - mc[23]) * mc[12];
+    const float Ae = e * (-mc[14]) + mc[16], Ai = i * (-mc[15]) + mc[17];
+    float g[kMC];
+#pragma unroll
+    for (int k = 0; k < kMC; ++k) g[k] = 0.0f;
+    const float a0 = lk[0] * mc[4], a1 = lk[1] * mc[5];
+    g[4] += lk[0] * (-e + Ae * se);
+    g[5] += lk[1] * (-i + Ai * si);
+    lx[0] += -a0 + a0 * (-mc[14]) * se;
+    lx[1] += -a1 + a1 * (-mc[15]) * si;
+    g[14] -= a0 * e * se; g[16] += a0 * se;
+    g[15] -= a1 * i * si; g[17] += a1 * si;
+    const float lse = a0 * Ae, lsi = a1 * Ai;
+    // se = (sge - mc22) * mc8, sge = 1 / (1 + exp(-(xe - mc7) * mc6))
+    const float lsge = lse * mc[8];
+    g[8] += lse * (sge - mc[22]); g[22] -= lse * mc[8];
+    const float de = lsge * sge * (1.0f - sge);
+    const float lxe = de * mc[6];
+    g[6] += de * (xe - mc[7]); g[7] -= de * mc[6];
+    const float lsgi = lsi * mc[12];
+    g[12] += lsi * (sgi - mc[23]); g[23] -= lsi * mc[12];
+    const float di = lsgi * sgi * (1.0f - sgi);
+    const float lxi = di * mc[10];
+    g[10] += di * (xi - mc[11]); g[11] -= di * mc[10];
+    // xe = (xe0 + mc18) * mc20
+    const float lxe0 = lxe * mc[20];
+    g[20] += lxe * (xe0 + mc[18]); g[18] += lxe * mc[20];
+    lx[0] += lxe0 * mc[0]; lx[1] -= lxe0 * mc[1];
+    g[0] += lxe0 * e; g[1] -= lxe0 * i; g[9] -= lxe0;
+    lc[0] += lxe0;
+    // xi = xi0 * mc21
+    const float lxi0 = lxi * mc[21];
+    g[21] += lxi * xi0;
+    lx[0] += lxi0 * mc[2]; lx[1] -= lxi0 * mc[3];
+    g[2] += lxi0 * e; g[3] -= lxi0 * i; g[19] += lxi0; g[13] -= lxi0;
+#pragma unroll
+    for (int k = 0; k < 24; ++k) dp += jac[k] * g[k];
+}
+
+template <> __device__ __forceinline__ void dfun_vjp<HB_KURAMOTO>(const float* mc, const float* x, const float* c,
+                                                                  const float* lk, float* lx, float* lc,
+                                                                  const float* jac, float& dp) {
+    (void)mc; (void)x; (void)c; (void)lx;
+    lc[0] += lk[0];
+    dp += jac[0] * lk[0];
+}
+
+template <> __device__ __forceinline__ void dfun_vjp<HB_RWW>(const float* mc, const float* x, const float* c,
+                                                             const float* lk, float* lx, float* lc,
+                                                             const float* jac, float& dp) {
+    const float s = x[0];
+    const float xx = (s * mc[5] + mc[7]) + c[0] * mc[6];
+    const float axb = xx * mc[0] + (-mc[1]);
+    const float et = expf(axb * (-mc[2]));
+    const float den = -et + 1.0f;
+    const float h = axb / den;
+    const float dh_daxb = (den - axb * mc[2] * et) / (den * den);
+    const float dh_dm2 = -(axb * axb * et) / (den * den);
+    const float l = lk[0];
+    const float lh = l * (1.0f - s) * mc[3];
+    const float laxb = lh * dh_daxb;
+    const float lxx = laxb * mc[0];
+    lx[0] += l * (-mc[4]) - l * h * mc[3] + lxx * mc[5];
+    lc[0] += lxx * mc[6];
+    dp += jac[0] * (laxb * xx) - jac[1] * laxb + jac[2] * (lh * dh_dm2) + jac[3] * (l * (1.0f - s) * h) -
+          jac[4] * (l * s) + jac[5] * (lxx * s) + jac[6] * (lxx * c[0]) + jac[7] * lxx;
+}
+
+// ------------------------------------------------------- coupling VJPs -----
+// Coupling functions with an adjoint: Linear (a*g + b), ScaledLinear
+// ((g - b) * a) and Kuramoto ((a/n) * (cos x_i * S - sin x_i * C)).
+// post_vjp: given mu = dL/dc, return dL/dg (lg), the target's own-state term
+// (lx_own, Kuramoto) and dL/d cc[0] (dcc).
+template <int NC, int NP>
+__device__ __forceinline__ void post_vjp(int cfn, const float* cc, const float* g, const float* own_pre,
+                                         const float* mu, float* lg, float* lx_own, float& dcc) {
+#pragma unroll
+    for (int c = 0; c < NP; ++c) lg[c] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        lx_own[c] = 0.0f;
+        if constexpr (NP == 2 * NC) {
+            // Kuramoto: r = (cos_i * gs - sin_i * gc) * cc0
+            const float sn = own_pre[c], cs = own_pre[NC + c];
+            lg[c] = mu[c] * cc[0] * cs;
+            lg[NC + c] = -mu[c] * cc[0] * sn;
+            lx_own[c] = mu[c] * cc[0] * (-sn * g[c] - cs * g[NC + c]);
+            dcc += mu[c] * (cs * g[c] - sn * g[NC + c]);
+        } else {
+            lg[c] = mu[c] * cc[0];
+            dcc += cfn == HB_CF_SCALED_LINEAR ? mu[c] * (g[c] - cc[1]) : mu[c] * g[c];
+        }
+    }
+}
+
+// Source side: dL/dx_j from the gathered coupling adjoints acc[np].
+template <int NC, int NP>
+__device__ __forceinline__ void pre_vjp(const float* x, const float* acc, float* lx) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        if constexpr (NP == 2 * NC) lx[c] += cosf(x[c]) * acc[c] - sinf(x[c]) * acc[NC + c];
+        else lx[c] += acc[c];
+    }
+}
+
+// ------------------------------------------------------- step VJP ----------
+// Recompute the stages of one step from (x, c, z) and pull lam (dL/dx_{t+1})
+// back: lx = dL/dx_t (integrator part), mu = dL/dc_t, dp += parameter term.
+template <int M>
+__device__ __forceinline__ void step_vjp(const StepCtx& S, const float* mc, const float* c, const float* z,
+                                         const float* x, const float* lam, float* lx, float* mu,
+                                         const float* jac, float& dp) {
+    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
+    const int ic = integ_class(S.integ);
+    float k1[NV], k2[NV], k3[NV], k4[NV], s2[NV], s3[NV], s4[NV], u[NV], m[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lx[v] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) mu[q] = 0.0f;
+    dfun<M>(mc, false, 0.0f, x, c, k1);
+    if (ic == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { const float y = x[v] + k1[v] * S.dt; u[v] = S.stoch ? y + z[v] : y; }
+        clamp_pass<M>(u, m);
+        float lk[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { const float lu = lam[v] * m[v]; lx[v] += lu; lk[v] = lu * S.dt; }
+        dfun_vjp<M>(mc, x, c, lk, lx, mu, jac, dp);
+    } else if (ic == 1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { const float y = x[v] + k1[v] * S.dt; s2[v] = S.stoch ? y + z[v] : y; }
+        float mp[NV];
+        clamp_pass<M>(s2, mp);
+        float xp[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xp[v] = s2[v];
+        clampm<M>(xp);
+        dfun<M>(mc, false, 0.0f, xp, c, k2);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { const float y = x[v] + (k1[v] + k2[v]) * S.dth; u[v] = S.stoch ? y + z[v] : y; }
+        clamp_pass<M>(u, m);
+        float lk1[NV], lk2[NV], lxp[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const float lu = lam[v] * m[v];
+            lx[v] += lu;
+            lk1[v] = lu * S.dth;
+            lk2[v] = lu * S.dth;
+            lxp[v] = 0.0f;
+        }
+        dfun_vjp<M>(mc, xp, c, lk2, lxp, mu, jac, dp);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { const float l1 = lxp[v] * mp[v]; lx[v] += l1; lk1[v] += l1 * S.dt; }
+        dfun_vjp<M>(mc, x, c, lk1, lx, mu, jac, dp);
+    } else {
+        float m2[NV], m3[NV], m4[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s2[v] = x[v] + k1[v] * S.dth;
+        clamp_pass<M>(s2, m2);
+        clampm<M>(s2);
+        dfun<M>(mc, false, 0.0f, s2, c, k2);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s3[v] = x[v] + k2[v] * S.dth;
+        clamp_pass<M>(s3, m3);
+        clampm<M>(s3);
+        dfun<M>(mc, false, 0.0f, s3, c, k3);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s4[v] = x[v] + k3[v] * S.dt;
+        clamp_pass<M>(s4, m4);
+        clampm<M>(s4);
+        dfun<M>(mc, false, 0.0f, s4, c, k4);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            float acc = k1[v];
+            acc = acc + k2[v] * 2.0f;
+            acc = acc + k3[v] * 2.0f;
+            const float det = x[v] + (acc + k4[v]) * S.dt6;
+            u[v] = (S.stoch && !S.plain) ? det + z[v] : det;
+        }
+        clamp_pass<M>(u, m);
+        float lk[NV], ls[NV];
+        // k4 and s4
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { const float lu = lam[v] * m[v]; lx[v] += lu; u[v] = lu; lk[v] = lu * S.dt6; ls[v] = 0.0f; }
+        dfun_vjp<M>(mc, s4, c, lk, ls, mu, jac, dp);
+        float lk3[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { const float l = ls[v] * m4[v]; lx[v] += l; lk3[v] = u[v] * S.dt6 * 2.0f + l * S.dt; ls[v] = 0.0f; }
+        dfun_vjp<M>(mc, s3, c, lk3, ls, mu, jac, dp);
+        float lk2[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { const float l = ls[v] * m3[v]; lx[v] += l; lk2[v] = u[v] * S.dt6 * 2.0f + l * S.dth; ls[v] = 0.0f; }
+        dfun_vjp<M>(mc, s2, c, lk2, ls, mu, jac, dp);
+        float lk1[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { const float l = ls[v] * m2[v]; lx[v] += l; lk1[v] = u[v] * S.dt6 + l * S.dth; }
+        dfun_vjp<M>(mc, x, c, lk1, lx, mu, jac, dp);
+    }
+}
+
+// ------------------------------------------------------------ helpers ------
+
+__device__ __forceinline__ StepCtx make_ctx(const AdjInst& I) {
+    StepCtx S;
+    S.dt = I.dt; S.dth = I.dt_half; S.dt6 = I.dt_sixth;
+    S.integ = I.integrator;
+    S.stoch = I.integrator >= HB_EULER_STOCH;
+    S.plain = I.engine == HB_ENGINE_PLAIN;
+    S.sw = false; S.sv = 0.0f;
+    return S;
+}
+
+__device__ __forceinline__ float block_sum(float v, float* red) {
+    // sum over the workgroup, fixed order: wave reductions then wave 0
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    float r = 0.0f;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) r += red[i];
+    }
+    return r;
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) r += red[i];
+    }
+    return r;
+}
+
+// ------------------------------------------------------- forward save ------
+// Same step order as the forward kernels (per-step schedule): ring write of
+// pre(x_t), barrier, per-edge gather (slot t-d, t==0 reads current), post,
+// stages, clamp. Ring depth dmax + 2.
+template <int M, bool KUR, int NPT>
+__global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__ insts) {
+    extern __shared__ float lds[];
+    __shared__ float ordp[2][8][2];   // per-wave (sum cos, sum sin) of the last two steps
+    const AdjInst& I = insts[blockIdx.x];
+    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC, NP = KUR ? 2 * NC : NC;
+    const int N = I.n_nodes, T = I.n_steps, Hd = I.ring_depth, Sg = I.seg_len;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const int NNP = N * NP;
+    float* ring = I.ring_lds ? lds : I.ring;
+    const StepCtx S = make_ctx(I);
+    float cc[kCC];
+#pragma unroll
+    for (int i = 0; i < kCC; ++i) cc[i] = I.cc[i];
+    for (int i = tid; i < Hd * NNP; i += BS) ring[i] = 0.0f;
+
+    float x[NPT][NV], mcn[NPT][kMC], bacc[NPT];
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const int n = tid + q * BS;
+        bacc[q] = 0.0f;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) x[q][v] = n < N ? I.x0[n * NV + v] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < kMC; ++i) mcn[q][i] = (I.node_mc && n < N) ? I.node_mc[(size_t)n * kMC + i] : I.mc[i];
+        if (n < N) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) I.ckpt[n * NV + v] = x[q][v];
+        }
+    }
+    const int P = I.bold_period;
+    int bcount = 0, kwin = 0;
+    const bool ord = I.loss_kind == HB_LOSS_ORDER;
+    const int nwaves = (BS + 63) >> 6;
+    double lsum = 0.0;
+    // order parameter R(x_{t+1}) of step t from the wave partials (thread 0)
+    auto ord_close = [&](int t) {
+        float cs = 0.0f, sn = 0.0f;
+        for (int w = 0; w < nwaves; ++w) { cs += ordp[t & 1][w][0]; sn += ordp[t & 1][w][1]; }
+        const float C = cs / (float)N, Sn = sn / (float)N;
+        const float R = sqrtf(C * C + Sn * Sn);
+        const double dd = (double)R - (double)I.target[t];
+        lsum += dd * dd;
+    };
+    __syncthreads();
+    for (int t = 0; t < T; ++t) {
+        const int ws = t % Hd;
+#pragma unroll
+        for (int q = 0; q < NPT; ++q) {
+            const int n = tid + q * BS;
+            if (n < N) {
+                float pr[NP];
+                pre<NC, NP>(I.cfn, cc, false, false, x[q], pr);
+#pragma unroll
+                for (int c = 0; c < NP; ++c) ring[(size_t)ws * NNP + n * NP + c] = pr[c];
+            }
+        }
+        __syncthreads();
+        if (ord && tid == 0 && t > 0) ord_close(t - 1);
+        ++bcount;
+#pragma unroll
+        for (int q = 0; q < NPT; ++q) {
+            const int n = tid + q * BS;
+            if (n >= N) continue;
+            float g[NP];
+#pragma unroll
+            for (int c = 0; c < NP; ++c) g[c] = 0.0f;
+            const int e0 = I.indptr[n], e1 = I.indptr[n + 1];
+            for (int e = e0; e < e1; ++e) {
+                const uint2 ed = I.edges[e];
+                const int j = ed.x & 0xFFFF;
+                const int d = t == 0 ? 0 : (int)(ed.x >> 16);
+                const float w = __uint_as_float(ed.y);
+                int sl = ws - d;
+                sl += sl < 0 ? Hd : 0;
+                const float* src = ring + (size_t)sl * NNP + j * NP;
+#pragma unroll
+                for (int c = 0; c < NP; ++c) g[c] += w * src[c];
+            }
+            float* gl = I.glog + ((size_t)t * N + n) * NP;
+#pragma unroll
+            for (int c = 0; c < NP; ++c) gl[c] = g[c];
+            float own[NP], cpl[NC];
+#pragma unroll
+            for (int c = 0; c < NP; ++c) own[c] = ring[(size_t)ws * NNP + n * NP + c];
+            if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, g, own, cpl);
+            else {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+            }
+            float z[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
+            integrate<M>(S, mcn[q], cpl, z, x[q]);
+            bacc[q] += x[q][0];
+            if (Sg > 0 && (t + 1) % Sg == 0 && t + 1 < T) {
+                float* ck = I.ckpt + ((size_t)((t + 1) / Sg) * N + n) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ck[v] = x[q][v];
+            }
+        }
+        if (P > 0 && (bcount == P || t == T - 1)) {
+#pragma unroll
+            for (int q = 0; q < NPT; ++q) {
+                const int n = tid + q * BS;
+                if (n < N) I.zlog[(size_t)kwin * N + n] = bacc[q] / (float)bcount;
+                bacc[q] = 0.0f;
+            }
+            ++kwin;
+            bcount = 0;
+        }
+        if (ord) {
+            float cs = 0.0f, sn = 0.0f;
+#pragma unroll
+            for (int q = 0; q < NPT; ++q) {
+                if (tid + q * BS < N) { cs += cosf(x[q][0]); sn += sinf(x[q][0]); }
+            }
+            for (int o = 32; o > 0; o >>= 1) { cs += __shfl_down(cs, o, 64); sn += __shfl_down(sn, o, 64); }
+            if ((tid & 63) == 0) { ordp[t & 1][tid >> 6][0] = cs; ordp[t & 1][tid >> 6][1] = sn; }
+        }
+    }
+    if (ord) {
+        __syncthreads();
+        if (tid == 0) {
+            ord_close(T - 1);
+            I.out_loss[0] = (float)(lsum / (double)T);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const int n = tid + q * BS;
+        if (n < N) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) I.fin[n * NV + v] = x[q][v];
+        }
+    }
+}
+
+// ------------------------------------------------------- BOLD forward ------
+// The adjoint plan evaluates the Balloon-Windkessel monitor in FP64 (the same
+// expressions and f32-valued constants as bold_step / bold.rs:106-166): near
+// rest v and q sit within a few f32 ulps of 1, where one ulp moves the signal
+// by ~2e-6, too coarse to differentiate. Neural inputs stay FP32.
+struct BoldD { double s, f, v, q; };
+constexpr double kBwKappa = 1.0 / (double)1.54f, kBwGamma = 1.0 / (double)0.50f, kBwTauInv = 1.0 / (double)1.02f;
+constexpr double kBwIalpha = 1.0 / (double)0.32f, kBwE0 = (double)0.34f, kBwLo = (double)1e-6f;
+
+__device__ __forceinline__ double bold_step_d(BoldD& b, double z_raw, double dts) {
+    const double z = z_raw > 0.0 ? z_raw : 0.0;
+    const double fs = b.f > kBwLo ? b.f : kBwLo;
+    const double vs = b.v > kBwLo ? b.v : kBwLo;
+    const double ds = (z - kBwKappa * b.s) - kBwGamma * (fs - 1.0);
+    const double pv = pow(vs, kBwIalpha);
+    const double dv = kBwTauInv * (fs - pv);
+    const double dq = kBwTauInv * ((fs * (1.0 - pow(1.0 - kBwE0, 1.0 / fs))) / kBwE0 - pv * (b.q / vs));
+    const double sn = b.s + dts * ds, fn = fs + dts * b.s, vn = vs + dts * dv, qn = b.q + dts * dq;
+    b.s = sn > 0.0 ? sn : 0.0;
+    b.f = fn > kBwLo ? fn : kBwLo;
+    b.v = vn > kBwLo ? vn : kBwLo;
+    b.q = qn > 0.0 ? qn : 0.0;
+    const double vv = b.v > kBwLo ? b.v : kBwLo;
+    return 4.0 * ((7.0 * (1.0 - b.q) + 2.0 * (1.0 - b.q / vv)) + 2.0 * (1.0 - vv));
+}
+
+__global__ __launch_bounds__(256) void k_bold_forward(const AdjInst* __restrict__ insts) {
+    const AdjInst& I = insts[blockIdx.x];
+    if (I.bold_period <= 0) return;
+    const int N = I.n_nodes, nw = I.n_win, P = I.bold_period, T = I.n_steps;
+    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+        BoldD b;
+        b.s = 0.0; b.f = 1.0; b.v = 1.0; b.q = 1.0;
+        for (int k = 0; k < nw; ++k) {
+            const int cnt = (k == nw - 1) ? T - k * P : P;
+            const double dts = (I.dt_ms * (double)cnt) / 1000.0;
+            double* st = I.bstate + ((size_t)k * N + n) * 4;
+            st[0] = b.s; st[1] = b.f; st[2] = b.v; st[3] = b.q;
+            I.bold[(size_t)k * N + n] = bold_step_d(b, (double)I.zlog[(size_t)k * N + n], dts);
+        }
+    }
+}
+
+// ------------------------------------------------------------- loss --------
+// FC (fc.rs:11-57: f64 accumulators, population std, std < 1e-12 -> 1,
+// clamp [-1, 1]) of the native BOLD series [nw][N]; loss = sum_ij (FC_ij -
+// target_ij)^2 with FC rounded to f32 as the reference returns it. Gradient
+// with Z = centred columns / (sqrt(n) s):  dL/dX_ti = (sum_j Gs_ij Z_tj -
+// h_i Z_ti) / (sqrt(n) s_i), Gs = G + G^T, G_ij = 2 (r_ij - target_ij) (0 where
+// clamped), h_i = sum_j Gs_ij r_ij.
+// BOLD MSE: loss = mean (BOLD - target)^2.
+__global__ __launch_bounds__(256) void k_loss(const AdjInst* __restrict__ insts) {
+    __shared__ double red[8];
+    const AdjInst& I = insts[blockIdx.x];
+    if (I.bold_period <= 0 || I.loss_kind == HB_LOSS_ORDER) return;
+    const int N = I.n_nodes, nw = I.n_win;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const double* X = I.bold;
+    if (I.loss_kind == HB_LOSS_BOLD_MSE) {
+        const double scale = 1.0 / ((double)nw * (double)N);
+        double acc = 0.0;
+        for (size_t i = tid; i < (size_t)nw * N; i += BS) {
+            const double dd = X[i] - (double)I.target[i];
+            acc += dd * dd;
+            I.dsig[i] = 2.0 * dd * scale;
+        }
+        const double tot = block_sum_d(acc, red);
+        if (tid == 0) I.out_loss[0] = (float)(tot * scale);
+        return;
+    }
+    double* mean = I.fc_work;                  // [N]
+    double* sd = mean + N;                     // [N]
+    double* rr = sd + N;                       // [N][N] r (f64)
+    double* gs = rr + (size_t)N * N;           // [N][N] G + G^T
+    double* h = gs + (size_t)N * N;            // [N]
+    for (int n = tid; n < N; n += BS) {
+        double m = 0.0;
+        for (int k = 0; k < nw; ++k) m += (double)X[(size_t)k * N + n];
+        m /= (double)nw;
+        double s = 0.0;
+        for (int k = 0; k < nw; ++k) { const double d = (double)X[(size_t)k * N + n] - m; s += d * d; }
+        s = sqrt(s / (double)nw);
+        if (s < 1e-12) s = 1.0;
+        mean[n] = m;
+        sd[n] = s;
+    }
+    __syncthreads();
+    const int npairs = N * (N + 1) / 2;
+    double lacc = 0.0;
+    for (int p = tid; p < npairs; p += BS) {
+        // p -> (i, j), j <= i
+        int i = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+        while (i * (i + 1) / 2 > p) --i;
+        while ((i + 1) * (i + 2) / 2 <= p) ++i;
+        const int j = p - i * (i + 1) / 2;
+        double cov = 0.0;
+        const double mi = mean[i], mj = mean[j];
+        for (int k = 0; k < nw; ++k) cov += ((double)X[(size_t)k * N + i] - mi) * ((double)X[(size_t)k * N + j] - mj);
+        cov /= (double)nw;
+        const double raw = cov / (sd[i] * sd[j]);
+        const bool clamped = raw < -1.0 || raw > 1.0;
+        const double r = clamped ? (raw < -1.0 ? -1.0 : 1.0) : raw;
+        const double rf = (double)(float)r;
+        const double tij = I.target[(size_t)i * N + j], tji = I.target[(size_t)j * N + i];
+        const double gij = clamped ? 0.0 : 2.0 * (rf - tij);
+        const double gji = clamped ? 0.0 : 2.0 * (rf - tji);
+        rr[(size_t)i * N + j] = r;
+        rr[(size_t)j * N + i] = r;
+        gs[(size_t)i * N + j] = gij + gji;
+        gs[(size_t)j * N + i] = gij + gji;
+        lacc += (rf - tij) * (rf - tij);
+        if (i != j) lacc += (rf - tji) * (rf - tji);
+    }
+    const double tot = block_sum_d(lacc, red);
+    if (tid == 0) I.out_loss[0] = (float)tot;
+    __syncthreads();
+    for (int n = tid; n < N; n += BS) {
+        double a = 0.0;
+        for (int j = 0; j < N; ++j) a += gs[(size_t)n * N + j] * rr[(size_t)n * N + j];
+        h[n] = a;
+    }
+    __syncthreads();
+    const double sqn = sqrt((double)nw);
+    for (size_t idx = tid; idx < (size_t)nw * N; idx += BS) {
+        const int k = (int)(idx / N), i = (int)(idx % N);
+        const double* row = X + (size_t)k * N;
+        double a = 0.0;
+        for (int j = 0; j < N; ++j) a += gs[(size_t)i * N + j] * ((row[j] - mean[j]) / (sqn * sd[j]));
+        const double zi = (row[i] - mean[i]) / (sqn * sd[i]);
+        I.dsig[idx] = (a - h[i] * zi) / (sqn * sd[i]);
+    }
+}
+
+// ------------------------------------------------------- BOLD reverse ------
+// Adjoint of bold_step_d per node, windows last to first (FP64).
+__global__ __launch_bounds__(256) void k_bold_reverse(const AdjInst* __restrict__ insts) {
+    const AdjInst& I = insts[blockIdx.x];
+    if (I.bold_period <= 0 || I.loss_kind == HB_LOSS_ORDER) return;
+    const int N = I.n_nodes, nw = I.n_win, P = I.bold_period, T = I.n_steps;
+    const double ln1e = log(1.0 - kBwE0);
+    for (int n = threadIdx.x; n < N; n += blockDim.x) {
+        double ls = 0.0, lf = 0.0, lv = 0.0, lq = 0.0;
+        for (int k = nw - 1; k >= 0; --k) {
+            const int cnt = (k == nw - 1) ? T - k * P : P;
+            const double dts = (I.dt_ms * (double)cnt) / 1000.0;
+            const double* st = I.bstate + ((size_t)k * N + n) * 4;
+            const double s = st[0], f = st[1], v = st[2], q = st[3];
+            const double zr = (double)I.zlog[(size_t)k * N + n];
+            const double zz = zr > 0.0 ? zr : 0.0;
+            const double fs = f > kBwLo ? f : kBwLo;
+            const double vs = v > kBwLo ? v : kBwLo;
+            const double ds = (zz - kBwKappa * s) - kBwGamma * (fs - 1.0);
+            const double pv = pow(vs, kBwIalpha);
+            const double dv = kBwTauInv * (fs - pv);
+            const double ef = pow(1.0 - kBwE0, 1.0 / fs);
+            const double dq = kBwTauInv * ((fs * (1.0 - ef)) / kBwE0 - pv * (q / vs));
+            const double sn = s + dts * ds, fn = fs + dts * s, vn = vs + dts * dv, qn = q + dts * dq;
+            const double v1 = vn > kBwLo ? vn : kBwLo;
+            const double q1 = qn > 0.0 ? qn : 0.0;
+            // signal 4 * (7 (1 - q') + 2 (1 - q'/v') + 2 (1 - v'))
+            const double dsg = I.dsig[(size_t)k * N + n];
+            lq += dsg * 4.0 * (-7.0 - 2.0 / v1);
+            lv += (v1 > kBwLo) ? dsg * 4.0 * (2.0 * q1 / (v1 * v1) - 2.0) : 0.0;
+            const double lsn = sn > 0.0 ? ls : 0.0;
+            const double lfn = fn > kBwLo ? lf : 0.0;
+            const double lvn = vn > kBwLo ? lv : 0.0;
+            const double lqn = qn > 0.0 ? lq : 0.0;
+            const double dfe = (1.0 - ef) + ef * ln1e / fs;            // d(fs (1 - ef)) / dfs
+            const double lfs = lsn * dts * (-kBwGamma) + lfn + lvn * dts * kBwTauInv +
+                               lqn * dts * kBwTauInv * (dfe / kBwE0);
+            const double dpv = kBwIalpha * pv / vs;
+            const double dpvq = (kBwIalpha - 1.0) * pv / (vs * vs);    // d(pv / vs) / dvs
+            const double lvs = lvn * (1.0 - dts * kBwTauInv * dpv) - lqn * dts * kBwTauInv * q * dpvq;
+            const double nls = lsn * (1.0 - dts * kBwKappa) + lfn * dts;
+            const double nlq = lqn * (1.0 - dts * kBwTauInv * (pv / vs));
+            const double lz = zr > 0.0 ? lsn * dts : 0.0;
+            ls = nls;
+            lf = f > kBwLo ? lfs : 0.0;
+            lv = v > kBwLo ? lvs : 0.0;
+            lq = nlq;
+            I.seedw[(size_t)k * N + n] = (float)(lz / (double)cnt);
+        }
+    }
+}
+
+// ------------------------------------------------------- reverse ----------
+template <int M, bool KUR, int NPT>
+__global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__ insts) {
+    extern __shared__ float lds[];
+    __shared__ double redd[8];
+    const AdjInst& I = insts[blockIdx.x];
+    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC, NP = KUR ? 2 * NC : NC;
+    const int N = I.n_nodes, T = I.n_steps, Ha = I.aring_depth, Sg = I.seg_len, nseg = I.n_seg;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const int NNP = N * NP;
+    float* aring = I.aring_lds ? lds : I.ring;
+    const StepCtx S = make_ctx(I);
+    float cc[kCC];
+#pragma unroll
+    for (int i = 0; i < kCC; ++i) cc[i] = I.cc[i];
+    for (int i = tid; i < Ha * NNP; i += BS) aring[i] = 0.0f;
+
+    float lam[NPT][NV], mcn[NPT][kMC], jac[NPT][kMC];
+    // parameter gradients are sums over every step and stage: FP64 accumulators
+    double gp[NPT];
+    double gcc = 0.0;
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const int n = tid + q * BS;
+        gp[q] = 0.0;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) lam[q][v] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < kMC; ++i) {
+            mcn[q][i] = (I.node_mc && n < N) ? I.node_mc[(size_t)n * kMC + i] : I.mc[i];
+            jac[q][i] = (I.node_jac && n < N) ? I.node_jac[(size_t)n * kMC + i] : 0.0f;
+        }
+    }
+    const int P = I.bold_period;
+    const bool seed_win = P > 0 && (I.loss_kind == HB_LOSS_FC || I.loss_kind == HB_LOSS_BOLD_MSE);
+    const bool seed_ord = I.loss_kind == HB_LOSS_ORDER;
+    __syncthreads();
+
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+        const int t0 = seg * Sg, t1 = min(T, t0 + Sg);
+        // ---- recompute states t0..t1 from the checkpoint and the coupling log
+#pragma unroll
+        for (int q = 0; q < NPT; ++q) {
+            const int n = tid + q * BS;
+            if (n >= N) continue;
+            float x[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { x[v] = I.ckpt[((size_t)seg * N + n) * NV + v]; I.tape[(size_t)n * NV + v] = x[v]; }
+            for (int t = t0; t < t1; ++t) {
+                float g[NP], own[NP], cpl[NC], z[NV];
+                const float* gl = I.glog + ((size_t)t * N + n) * NP;
+#pragma unroll
+                for (int c = 0; c < NP; ++c) g[c] = gl[c];
+                pre<NC, NP>(I.cfn, cc, false, false, x, own);
+                if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, g, own, cpl);
+                else {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
+                integrate<M>(S, mcn[q], cpl, z, x);
+                float* tp = I.tape + ((size_t)(t - t0 + 1) * N + n) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) tp[v] = x[v];
+            }
+        }
+        __syncthreads();
+        // ---- order parameter of states t0..t1: per row (wave-strided)
+        if (seed_ord) {
+            const int w = tid >> 6, l = tid & 63, nwv = BS >> 6;
+            for (int r = w; r <= t1 - t0; r += nwv) {
+                float cs = 0.0f, sn = 0.0f;
+                for (int n = l; n < N; n += 64) {
+                    const float th = I.tape[((size_t)r * N + n) * NV];
+                    cs += cosf(th);
+                    sn += sinf(th);
+                }
+                for (int o = 32; o > 0; o >>= 1) { cs += __shfl_down(cs, o, 64); sn += __shfl_down(sn, o, 64); }
+                if (l == 0) {
+                    const int s = t0 + r;
+                    float a = 0.0f, b = 0.0f;
+                    if (s >= 1) {
+                        const float C = cs / (float)N, Sn = sn / (float)N;
+                        const float R = sqrtf(C * C + Sn * Sn);
+                        const float diff = R - I.target[s - 1];
+                        const float al = R > 0.0f ? 2.0f * diff / ((float)T * (float)N * R) : 0.0f;
+                        a = -al * C;
+                        b = al * Sn;
+                    }
+                    I.ord_work[(size_t)r * 4 + 0] = a;
+                    I.ord_work[(size_t)r * 4 + 1] = b;
+                }
+            }
+            __syncthreads();
+        }
+        // ---- seed of the final state x_T
+        if (seg == nseg - 1) {
+#pragma unroll
+            for (int q = 0; q < NPT; ++q) {
+                const int n = tid + q * BS;
+                if (n >= N) continue;
+                if (seed_win) lam[q][0] += I.seedw[(size_t)((T - 1) / P) * N + n];
+                if (seed_ord) {
+                    const float th = I.tape[((size_t)(t1 - t0) * N + n) * NV];
+                    const float* cf = I.ord_work + (size_t)(t1 - t0) * 4;
+                    lam[q][0] += cf[0] * sinf(th) + cf[1] * cosf(th);
+                }
+            }
+        }
+        // ---- reverse steps t1-1 .. t0
+        for (int t = t1 - 1; t >= t0; --t) {
+            const int slot = t % Ha;
+            float lxp[NPT][NV];
+#pragma unroll
+            for (int q = 0; q < NPT; ++q) {
+                const int n = tid + q * BS;
+                if (n >= N) continue;
+                float x[NV], g[NP], own[NP], cpl[NC], z[NV], mu[NC], lg[NP], lown[NC];
+                const float* tp = I.tape + ((size_t)(t - t0) * N + n) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) x[v] = tp[v];
+                const float* gl = I.glog + ((size_t)t * N + n) * NP;
+#pragma unroll
+                for (int c = 0; c < NP; ++c) g[c] = gl[c];
+                pre<NC, NP>(I.cfn, cc, false, false, x, own);
+                if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, g, own, cpl);
+                else {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
+                float dps = 0.0f, dcs = 0.0f;
+                step_vjp<M>(S, mcn[q], cpl, z, x, lam[q], lxp[q], mu, jac[q], dps);
+                gp[q] += (double)dps;
+                if (I.has_coupling) {
+                    post_vjp<NC, NP>(I.cfn, cc, g, own, mu, lg, lown, dcs);
+                    gcc += (double)dcs;
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) lxp[q][c] += lown[c];
+                } else {
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) lg[c] = 0.0f;
+                }
+                float* ar = aring + (size_t)slot * NNP + n * NP;
+#pragma unroll
+                for (int c = 0; c < NP; ++c) ar[c] = lg[c];
+            }
+            __syncthreads();
+            // gather over out-edges: d == 0 -> this step; d >= 1 -> step t + d;
+            // at t == 0 every edge also read the current state.
+#pragma unroll
+            for (int q = 0; q < NPT; ++q) {
+                const int n = tid + q * BS;
+                if (n >= N) continue;
+                float acc[NP];
+#pragma unroll
+                for (int c = 0; c < NP; ++c) acc[c] = 0.0f;
+                if (I.has_coupling) {
+                    const int e0 = I.indptr_t[n], e1 = I.indptr_t[n + 1];
+                    for (int e = e0; e < e1; ++e) {
+                        const uint2 ed = I.edges_t[e];
+                        const int i = ed.x & 0xFFFF;
+                        const int d = (int)(ed.x >> 16);
+                        const float w = __uint_as_float(ed.y);
+                        int sl = slot + d;
+                        sl -= sl >= Ha ? Ha : 0;
+                        const float* a = aring + (size_t)sl * NNP + i * NP;
+#pragma unroll
+                        for (int c = 0; c < NP; ++c) acc[c] += w * a[c];
+                        if (t == 0 && d > 0) {
+                            const float* a0 = aring + (size_t)slot * NNP + i * NP;
+#pragma unroll
+                            for (int c = 0; c < NP; ++c) acc[c] += w * a0[c];
+                        }
+                    }
+                }
+                float x[NV];
+                const float* tp = I.tape + ((size_t)(t - t0) * N + n) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) x[v] = tp[v];
+                pre_vjp<NC, NP>(x, acc, lxp[q]);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) lam[q][v] = lxp[q][v];
+                if (t >= 1) {
+                    if (seed_win) lam[q][0] += I.seedw[(size_t)((t - 1) / P) * N + n];
+                    if (seed_ord) {
+                        const float* cf = I.ord_work + (size_t)(t - t0) * 4;
+                        lam[q][0] += cf[0] * sinf(x[0]) + cf[1] * cosf(x[0]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---- outputs
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+        const int n = tid + q * BS;
+        if (n >= N) continue;
+        if (I.out_gx0) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) I.out_gx0[n * NV + v] = lam[q][v];
+        }
+        if (I.out_gp) I.out_gp[n] = (float)gp[q];
+    }
+    const double tot = block_sum_d(gcc, redd);
+    if (tid == 0) I.out_gcc[0] = (float)tot;
+}
+
+// ------------------------------------------------------------ dispatch -----
+
+template <int M, bool KUR, int NPT>
+static const void* fwd_ptr() { return (const void*)k_adj_forward<M, KUR, NPT>; }
+template <int M, bool KUR, int NPT>
+static const void* rev_ptr() { return (const void*)k_adj_reverse<M, KUR, NPT>; }
+
+template <int M>
+static const void* pick(bool rev, bool kur, int npt) {
+    if (rev) {
+        if (kur) return npt == 1 ? rev_ptr<M, true, 1>() : rev_ptr<M, true, 2>();
+        return npt == 1 ? rev_ptr<M, false, 1>() : rev_ptr<M, false, 2>();
+    }
+    if (kur) return npt == 1 ? fwd_ptr<M, true, 1>() : fwd_ptr<M, true, 2>();
+    return npt == 1 ? fwd_ptr<M, false, 1>() : fwd_ptr<M, false, 2>();
+}
+
+static const void* kernel_for(int model, bool rev, bool kur, int npt) {
+    switch (model) {
+    case HB_G2DO: return pick<HB_G2DO>(rev, kur, npt);
+    case HB_MPR: return pick<HB_MPR>(rev, kur, npt);
+    case HB_JANSEN_RIT: return pick<HB_JANSEN_RIT>(rev, kur, npt);
+    case HB_WILSON_COWAN: return pick<HB_WILSON_COWAN>(rev, kur, npt);
+    case HB_KURAMOTO: return pick<HB_KURAMOTO>(rev, kur, npt);
+    default: return pick<HB_RWW>(rev, kur, npt);
+    }
+}
+
+}  // namespace adj
+
+hipError_t adjoint_set_attributes(int max_lds) {
+    for (int m = 0; m <= HB_RWW; ++m)
+        for (int rev = 0; rev < 2; ++rev)
+            for (int kur = 0; kur < 2; ++kur)
+                for (int npt = 1; npt <= 2; ++npt) {
+                    hipError_t e = hipFuncSetAttribute(adj::kernel_for(m, rev, kur, npt),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+                    if (e != hipSuccess) return e;
+                }
+    return hipSuccess;
+}
+
+hipError_t launch_adjoint_stage(int stage, const AdjLaunch& L, hipStream_t s) {
+    const AdjInst* d = L.insts;
+    switch (stage) {
+    case 0: {
+        const void* k = adj::kernel_for(L.model, false, L.kur, L.npt);
+        void* args[] = {(void*)&d};
+        return hipLaunchKernel(k, dim3(L.n), dim3(L.block), args, L.lds_fwd, s);
+    }
+    case 1: hipLaunchKernelGGL(adj::k_bold_forward, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
+    case 2: hipLaunchKernelGGL(adj::k_loss, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
+    case 3: hipLaunchKernelGGL(adj::k_bold_reverse, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
+    default: {
+        const void* k = adj::kernel_for(L.model, true, L.kur, L.npt);
+        void* args[] = {(void*)&d};
+        return hipLaunchKernel(k, dim3(L.n), dim3(L.block), args, L.lds_rev, s);
+    }
+    }
+}
+
+}  // namespace hb

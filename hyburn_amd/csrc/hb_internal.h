@@ -49,11 +49,17 @@ constexpr float kPi = 3.14159265358979323846f;
 
 // Model constants, f32, evaluated with the reference's host-side op order.
 // G2DO  dfun.rs:277-293 | MPR 445-456 | JR 327-366 | WC 392-403 | KUR 486 | RWW 500-502
-HB_HD void model_consts(int m, const float* p, float* mc) {
-    for (int i = 0; i < kMC; ++i) mc[i] = 0.0f;
+// Templated on the scalar so the adjoint's host side can evaluate the same
+// expressions on dual numbers (d mc / d parameter); S = float is the forward.
+HB_HD float mc_exp(float x) { return expf(x); }
+
+template <class S>
+HB_HD void model_consts_t(int m, const S* p, S* mc) {
+    const S eps = S(kEps), one = S(1.0f), two = S(2.0f), pi = S(kPi);
+    for (int i = 0; i < kMC; ++i) mc[i] = S(0.0f);
     switch (m) {
     case HB_G2DO: {
-        float tau = p[0] > kEps ? p[0] : kEps;
+        S tau = p[0] > eps ? p[0] : eps;
         mc[0] = p[2]; mc[1] = p[3]; mc[2] = p[4]; mc[3] = p[6]; mc[4] = p[7]; mc[5] = p[8];
         mc[6] = p[9]; mc[7] = p[10]; mc[8] = p[11];
         mc[9] = p[11] * p[1];          // gamma * I  (add_scalar(gamma * params[1]))
@@ -61,43 +67,45 @@ HB_HD void model_consts(int m, const float* p, float* mc) {
         mc[11] = p[5] / tau;           // d / tau
         break; }
     case HB_MPR: {
-        float tau = p[0] > kEps ? p[0] : kEps;
-        mc[0] = 1.0f / tau;
-        mc[1] = p[1] / (kPi * tau);
-        mc[2] = kPi * kPi * tau * tau;
+        S tau = p[0] > eps ? p[0] : eps;
+        mc[0] = one / tau;
+        mc[1] = p[1] / (pi * tau);
+        mc[2] = pi * pi * tau * tau;
         mc[3] = p[3] * tau;
         mc[4] = p[2] + p[4];           // eta + i_ext
         mc[5] = p[2];                  // eta (swept form adds I separately)
         mc[6] = p[5]; mc[7] = p[6];
         break; }
     case HB_JANSEN_RIT: {
-        float Ap = p[0], Bp = p[1], a = p[2], b = p[3], J = p[7];
-        mc[0] = p[4]; mc[1] = p[6]; mc[2] = 2.0f * p[5];
-        mc[3] = Ap * a; mc[4] = 2.0f * a; mc[5] = a * a;
+        S Ap = p[0], Bp = p[1], a = p[2], b = p[3], J = p[7];
+        mc[0] = p[4]; mc[1] = p[6]; mc[2] = two * p[5];
+        mc[3] = Ap * a; mc[4] = two * a; mc[5] = a * a;
         mc[6] = p[8] * J; mc[7] = p[9] * J; mc[8] = p[10] * J; mc[9] = p[11] * J;
-        mc[10] = Bp * b; mc[11] = 2.0f * b; mc[12] = b * b; mc[13] = p[12];
+        mc[10] = Bp * b; mc[11] = two * b; mc[12] = b * b; mc[13] = p[12];
         break; }
     case HB_WILSON_COWAN: {
-        float te = p[4] > kEps ? p[4] : kEps, ti = p[5] > kEps ? p[5] : kEps;
+        S te = p[4] > eps ? p[4] : eps, ti = p[5] > eps ? p[5] : eps;
         mc[0] = p[0]; mc[1] = p[1]; mc[2] = p[2]; mc[3] = p[3];
-        mc[4] = 1.0f / te; mc[5] = 1.0f / ti;
+        mc[4] = one / te; mc[5] = one / ti;
         mc[6] = p[6]; mc[7] = p[7]; mc[8] = p[8]; mc[9] = p[9];
         mc[10] = p[10]; mc[11] = p[11]; mc[12] = p[12]; mc[13] = p[13];
         mc[14] = p[14]; mc[15] = p[15]; mc[16] = p[16]; mc[17] = p[17];
         mc[18] = p[18]; mc[19] = p[19]; mc[20] = p[20]; mc[21] = p[21];
-        mc[22] = 1.0f / (1.0f + expf(p[6] * p[7]));
-        mc[23] = 1.0f / (1.0f + expf(p[10] * p[11]));
+        mc[22] = one / (one + mc_exp(p[6] * p[7]));
+        mc[23] = one / (one + mc_exp(p[10] * p[11]));
         break; }
     case HB_KURAMOTO:
         mc[0] = p[0];
         break;
     default: // RWW
         mc[0] = p[0]; mc[1] = p[1]; mc[2] = p[2]; mc[3] = p[3];
-        mc[4] = 1.0f / p[4];
+        mc[4] = one / p[4];
         mc[5] = p[5] * p[6]; mc[6] = p[6]; mc[7] = p[7];
         break;
     }
 }
+
+HB_HD void model_consts(int m, const float* p, float* mc) { model_consts_t<float>(m, p, mc); }
 
 // Coupling constants (coupling.rs:114-202 parsing, host-side products as the
 // reference forms them). Slot meaning depends on cfn; see pre()/post().

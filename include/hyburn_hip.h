@@ -203,6 +203,57 @@ int hb_run_instances(const hb_instance* inst, int32_t n, const hb_run_opts* opts
  * buffer: element k of stream s under key seed. Device-generated. */
 int hb_philox_normal(uint64_t seed, uint64_t stream, int64_t n, float* out_host, int32_t device);
 
+/* ---------------- adjoint (gradients) ---------------- */
+
+/* Loss of one instance. FC: sum_ij (FC_ij - target_ij)^2 with FC the Pearson
+ * matrix of the native BOLD series (fc.rs:11-57), target [N][N]. BOLD_MSE:
+ * mean over rows and nodes of (BOLD - target)^2, target [n_bold][N] native
+ * rows. ORDER: (1/T) sum_t (R_t - target_t)^2, R_t = |mean_j exp(i x_{t+1}[j])|
+ * (Kuramoto phases after step t), target [n_steps]. */
+enum { HB_LOSS_FC = 0, HB_LOSS_BOLD_MSE = 1, HB_LOSS_ORDER = 2 };
+
+typedef struct {
+    int32_t loss_kind;
+    const float* target;
+    int32_t grad_param;       /* model parameter index differentiated per node, or -1 */
+    /* outputs (host memory, caller-owned), written by hb_adjoint_download */
+    float* loss;              /* [1] */
+    float* grad_coupling;     /* [1] dL / d cparams[0] (the global coupling G / a) */
+    float* grad_param_out;    /* [n_nodes] dL / d params[grad_param] of each node */
+    float* grad_x0;           /* [n_nodes][nvar] or NULL */
+    float* final_state;       /* [n_nodes][nvar] or NULL */
+} hb_adjoint_spec;
+
+typedef struct {
+    int32_t device;
+    int32_t checkpoint_every; /* steps per checkpoint segment; 0: ceil(sqrt(n_steps)) */
+    uint32_t flags;           /* reserved, 0 */
+} hb_adjoint_opts;
+
+typedef struct {
+    double forward_ms;        /* forward save + BOLD + loss, HIP events */
+    double backward_ms;       /* BOLD adjoint + reverse sweep, HIP events */
+    double node_steps;
+    int64_t device_bytes;     /* plan arena: coupling log, checkpoints, tapes, buffers */
+} hb_adjoint_stats;
+
+typedef struct hb_adjoint_plan hb_adjoint_plan;
+
+/* The differentiable graph of each instance is its forward recursion (the same
+ * step order, clamps and per-edge delayed coupling as hb_plan_*); supported:
+ * CSR weights with per-edge delays (or none), Linear / ScaledLinear / Kuramoto
+ * coupling, all six models and integrators, up to 1024 nodes. No reference
+ * counterpart: it is the gradient path added beside the plain forward run. */
+int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec, int32_t n,
+                           const hb_adjoint_opts* opts, hb_adjoint_plan** out);
+/* Forward with checkpoints every S steps and the per-step coupling log; loss. */
+int hb_adjoint_forward(hb_adjoint_plan* p, hb_adjoint_stats* stats);
+/* Reverse sweep: recompute each segment from its checkpoint, step the adjoint back. */
+int hb_adjoint_backward(hb_adjoint_plan* p, hb_adjoint_stats* stats);
+/* Copy losses and gradients to the spec's host pointers. */
+int hb_adjoint_download(hb_adjoint_plan* p);
+void hb_adjoint_plan_destroy(hb_adjoint_plan* p);
+
 /* ---------------- misc ---------------- */
 const char* hb_last_error(void);   /* thread-local message of the last failure */
 const char* hb_version(void);

@@ -36,6 +36,9 @@ def test_struct_sizes_match_header():
     assert C.sizeof(L.HbRunOpts) == 16
     assert C.sizeof(L.HbRunStats) == 40
     assert C.sizeof(L.HbSweepPoint) == 32
+    assert C.sizeof(L.HbAdjointSpec) == 64
+    assert C.sizeof(L.HbAdjointOpts) == 12
+    assert C.sizeof(L.HbAdjointStats) == 32
 
 
 def test_version_and_backend_selection():
