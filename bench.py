@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -111,6 +113,50 @@ def ragged_leg(args):
     return res
 
 
+def _fc64(B):
+    n = B.shape[0]
+    A = B - B.mean(0)
+    sd = np.sqrt((A * A).sum(0) / n)
+    sd = np.where(sd < 1e-12, 1.0, sd)
+    return np.clip((A.T @ A) / n / (sd[:, None] * sd[None, :]), -1.0, 1.0)
+
+
+def adjoint_leg(name, insts, losses, checkpoint_every, n_params):
+    """Gradient evaluations (forward with checkpoints + loss + reverse sweep)
+    per second over the whole batch, against the plain forward of the same
+    batch; central differences would need 2 * n_params plain forwards."""
+    from hyburn_amd.adjoint import AdjointPlan
+    from hyburn_amd.instances import Plan
+    B = len(insts)
+    log(f"[bench] adjoint {name}: {B} x {insts[0].n_nodes} nodes x {insts[0].n_steps} steps; planning")
+    t0 = time.perf_counter()
+    P = AdjointPlan(insts, losses, checkpoint_every=checkpoint_every)
+    plan_s = time.perf_counter() - t0
+    P.forward()
+    P.backward()
+    t0 = time.perf_counter()
+    P.forward()
+    P.backward()
+    wall = time.perf_counter() - t0
+    fwd_ms, bwd_ms, dev_bytes = P.stats.forward_ms, P.stats.backward_ms, P.stats.device_bytes
+    P.close()
+    with Plan(insts) as Q:
+        Q.run()
+        t0 = time.perf_counter()
+        st = Q.run()
+        plain_wall = time.perf_counter() - t0
+    plain_ms = st.total_ms
+    return {
+        "workload": name, "n_instances": B, "n_nodes": insts[0].n_nodes, "n_steps": insts[0].n_steps,
+        "grads_per_s": B / wall, "wall_ms": wall * 1e3, "adjoint_forward_ms": fwd_ms, "adjoint_backward_ms": bwd_ms,
+        "plain_forward_ms": plain_ms, "cost_vs_forward": (fwd_ms + bwd_ms) / plain_ms,
+        "checkpoint_every": checkpoint_every, "device_bytes": dev_bytes, "plan_s": plan_s,
+        "fd_grads_per_s_est": B / (2 * n_params * plain_wall),
+        "fd_note": f"central differences over {n_params} parameters per instance = {2 * n_params} plain forwards "
+                   "per gradient, estimated from the measured plain forward",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +170,10 @@ def main() -> int:
     ap.add_argument("--ragged-n", type=int, default=4096)
     ap.add_argument("--ragged-tscale", type=float, default=1.0)
     ap.add_argument("--ragged-plain-sample", type=int, default=16)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient legs")
+    ap.add_argument("--adj4-batch", type=int, default=256)
+    ap.add_argument("--adj5-batch", type=int, default=512)
+    ap.add_argument("--adj5-steps", type=int, default=10_000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -160,7 +210,6 @@ def main() -> int:
     wall = time.perf_counter() - t0
     outs = P.download()
     P.close()
-    import numpy as np
     assert all(np.all(np.isfinite(o.tavg)) for o in outs), "non-finite output"
 
     ms_per_step = wall / args.steps * 1e3
@@ -222,6 +271,24 @@ def main() -> int:
                                           f"({secs * threads:.1f} core-s)"}
     if args.ragged:
         line["ragged"] = ragged_leg(args)
+    if not args.no_adjoint:
+        import dataclasses
+        from hyburn_amd.adjoint import Loss
+        from hyburn_amd.instances import run_instances
+        from hyburn_amd.synth import config4, config5
+        c4, hidden = config4(n_inst=args.adj4_batch)
+        (h,), _ = run_instances([dataclasses.replace(hidden, record_bold=True)])
+        fc_t = _fc64(h.bold.astype(np.float64)).astype(np.float32)
+        line["adjoint"] = {
+            "fc_mpr_84": adjoint_leg("MPR 84 nodes, 20k Heun steps, FC of native BOLD vs hidden-parameter FC; "
+                                     "grads of G and 84 eta", c4, [Loss(L.LOSS_FC, fc_t, 2) for _ in c4], 500,
+                                     1 + c4[0].n_nodes),
+        }
+        del c4
+        c5, tgt = config5(n_inst=args.adj5_batch, n_steps=args.adj5_steps)
+        line["adjoint"]["order_kuramoto_1000"] = adjoint_leg(
+            "Kuramoto 1000 nodes, 5% CSR, delays <= 40 ms, RK4, order-parameter MSE; grads of G and 1000 omega",
+            c5, [Loss(L.LOSS_ORDER, tgt, 0) for _ in c5], 0, 1 + c5[0].n_nodes)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -49,8 +49,10 @@ struct AdjLaunch {
     const AdjInst* insts;       // device array (contiguous for the group)
     int32_t n;
     int32_t model;
-    bool kur;                   // Kuramoto coupling (two pre channels per cvar)
+    int32_t np;                 // pre channels stored per node
     int32_t npt;                // nodes per thread (1 or 2)
+    bool blocked;               // delay-blocked kernels (min delay >= 8, npt 1)
+    bool lring;                 // blocked kernels: rings in LDS
     int32_t block;
     int32_t lds_fwd, lds_rev;   // dynamic LDS bytes
 };

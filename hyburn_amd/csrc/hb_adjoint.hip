@@ -31,6 +31,7 @@
 #include "hb_adjoint.h"
 
 #include <cmath>
+#include <type_traits>
 
 namespace hb {
 namespace adj {
@@ -245,10 +246,11 @@ __device__ __forceinline__ void post_vjp(int cfn, const float* cc, const float* 
             lg[NC + c] = -mu[c] * cc[0] * sn;
             lx_own[c] = mu[c] * cc[0] * (-sn * g[c] - cs * g[NC + c]);
             dcc += mu[c] * (cs * g[c] - sn * g[NC + c]);
-        } else {
+        } else if (c < NP) {
             lg[c] = mu[c] * cc[0];
             dcc += cfn == HB_CF_SCALED_LINEAR ? mu[c] * (g[c] - cc[1]) : mu[c] * g[c];
         }
+        // c >= NP: a channel whose coefficient is zero (mu[c] == 0), not stored
     }
 }
 
@@ -258,7 +260,7 @@ __device__ __forceinline__ void pre_vjp(const float* x, const float* acc, float*
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         if constexpr (NP == 2 * NC) lx[c] += cosf(x[c]) * acc[c] - sinf(x[c]) * acc[NC + c];
-        else lx[c] += acc[c];
+        else if (c < NP) lx[c] += acc[c];
     }
 }
 
@@ -402,12 +404,12 @@ __device__ __forceinline__ double block_sum_d(double v, double* red) {
 // Same step order as the forward kernels (per-step schedule): ring write of
 // pre(x_t), barrier, per-edge gather (slot t-d, t==0 reads current), post,
 // stages, clamp. Ring depth dmax + 2.
-template <int M, bool KUR, int NPT>
+template <int M, int NP, int NPT>
 __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__ insts) {
     extern __shared__ float lds[];
     __shared__ float ordp[2][8][2];   // per-wave (sum cos, sum sin) of the last two steps
     const AdjInst& I = insts[blockIdx.x];
-    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC, NP = KUR ? 2 * NC : NC;
+    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
     const int N = I.n_nodes, T = I.n_steps, Hd = I.ring_depth, Sg = I.seg_len;
     const int tid = threadIdx.x, BS = blockDim.x;
     const int NNP = N * NP;
@@ -726,12 +728,12 @@ __global__ __launch_bounds__(256) void k_bold_reverse(const AdjInst* __restrict_
 }
 
 // ------------------------------------------------------- reverse ----------
-template <int M, bool KUR, int NPT>
+template <int M, int NP, int NPT>
 __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__ insts) {
     extern __shared__ float lds[];
     __shared__ double redd[8];
     const AdjInst& I = insts[blockIdx.x];
-    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC, NP = KUR ? 2 * NC : NC;
+    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
     const int N = I.n_nodes, T = I.n_steps, Ha = I.aring_depth, Sg = I.seg_len, nseg = I.n_seg;
     const int tid = threadIdx.x, BS = blockDim.x;
     const int NNP = N * NP;
@@ -936,31 +938,584 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
     if (tid == 0) I.out_gcc[0] = (float)tot;
 }
 
-// ------------------------------------------------------------ dispatch -----
+// ------------------------------------------------- delay-blocked kernels ----
+//
+// When every edge delay is >= kK, the coupling inputs of kK consecutive steps
+// (forward) and the delayed coupling adjoints of kK consecutive reverse steps
+// depend only on ring entries written by earlier blocks, so a workgroup needs
+// one barrier per block instead of one per step. Ring layout [np][N][Hs],
+// Hs = H + kK - 1: slots H..Hs-1 mirror slots 0..kK-2 so that every kK-step
+// window of one node and channel is contiguous. H = dmax + kK + 1 (host).
 
-template <int M, bool KUR, int NPT>
-static const void* fwd_ptr() { return (const void*)k_adj_forward<M, KUR, NPT>; }
-template <int M, bool KUR, int NPT>
-static const void* rev_ptr() { return (const void*)k_adj_reverse<M, KUR, NPT>; }
+constexpr int kK = 8;
 
-template <int M>
-static const void* pick(bool rev, bool kur, int npt) {
-    if (rev) {
-        if (kur) return npt == 1 ? rev_ptr<M, true, 1>() : rev_ptr<M, true, 2>();
-        return npt == 1 ? rev_ptr<M, false, 1>() : rev_ptr<M, false, 2>();
-    }
-    if (kur) return npt == 1 ? fwd_ptr<M, true, 1>() : fwd_ptr<M, true, 2>();
-    return npt == 1 ? fwd_ptr<M, false, 1>() : fwd_ptr<M, false, 2>();
+template <typename T> using gp = __attribute__((address_space(1))) T*;
+__device__ __forceinline__ int u32(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uf32(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uf64(double x) {
+    const long long v = __double_as_longlong(x);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <typename T> __device__ __forceinline__ gp<T> ug(T* p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (gp<T>)((T*)(((uint64_t)hi << 32) | lo));
 }
 
-static const void* kernel_for(int model, bool rev, bool kur, int npt) {
+// Uniform per-instance view (scalar registers), read once per kernel.
+struct BView {
+    int N, T, H, Hs, Sg, nseg, P, cfn, integ, loss, has_c;
+    float dt, dth, dt6;
+    bool stoch, plain;
+    double dt_ms;
+    gp<const int32_t> indptr;
+    gp<const uint32_t> edges;   // (src|tgt) | delay << 16, weight bits (pairs)
+    gp<const float> x0, node_mc, node_jac, noise, target;
+    gp<float> ring, glog, ckpt, tape, zlog, fin, loss_out, ord, gx0, gp_out, gcc_out;
+    gp<const float> seedw;
+};
+
+__device__ __forceinline__ BView bview(const AdjInst* I, bool csc) {
+    BView V;
+    V.N = u32(I->n_nodes); V.T = u32(I->n_steps); V.H = u32(I->ring_depth); V.Hs = V.H + kK - 1;
+    V.Sg = u32(I->seg_len); V.nseg = u32(I->n_seg); V.P = u32(I->bold_period); V.cfn = u32(I->cfn);
+    V.integ = u32(I->integrator); V.loss = u32(I->loss_kind); V.has_c = u32(I->has_coupling);
+    V.dt = uf32(I->dt); V.dth = uf32(I->dt_half); V.dt6 = uf32(I->dt_sixth);
+    V.stoch = V.integ >= HB_EULER_STOCH;
+    V.plain = u32(I->engine) == HB_ENGINE_PLAIN;
+    V.dt_ms = uf64(I->dt_ms);
+    V.indptr = ug(csc ? I->indptr_t : I->indptr);
+    V.edges = ug((const uint32_t*)(csc ? I->edges_t : I->edges));
+    V.x0 = ug(I->x0); V.node_mc = ug(I->node_mc); V.node_jac = ug(I->node_jac);
+    V.noise = ug(I->noise); V.target = ug(I->target);
+    V.ring = ug(I->ring); V.glog = ug(I->glog); V.ckpt = ug(I->ckpt); V.tape = ug(I->tape);
+    V.zlog = ug(I->zlog); V.fin = ug(I->fin); V.loss_out = ug(I->out_loss); V.ord = ug(I->ord_work);
+    V.gx0 = ug(I->out_gx0); V.gp_out = ug(I->out_gp); V.gcc_out = ug(I->out_gcc);
+    V.seedw = ug((const float*)I->seedw);
+    return V;
+}
+
+__device__ __forceinline__ int wrap_slot(int t, int H) { int s = t % H; return s < 0 ? s + H : s; }
+
+// Forward save, delay-blocked. One node per thread (N <= blockDim).
+template <int M, int NP, bool LRING>
+__global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict__ insts) {
+    extern __shared__ float lds[];
+    __shared__ float ordp[2][kK][8][2];   // per-wave order-parameter partials, by block parity
+    const AdjInst* Ip = insts + blockIdx.x;
+    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
+    const BView V = bview(Ip, false);
+    const int N = V.N, T = V.T, H = V.H, Hs = V.Hs;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const bool own = tid < N;
+    using RingPtr = std::conditional_t<LRING, float*, gp<float>>;
+    RingPtr ring;
+    if constexpr (LRING) ring = lds; else ring = V.ring;
+    StepCtx S;
+    S.dt = V.dt; S.dth = V.dth; S.dt6 = V.dt6; S.integ = V.integ; S.stoch = V.stoch; S.plain = V.plain;
+    S.sw = false; S.sv = 0.0f;
+    float cc[kCC], nsc[NV];
+#pragma unroll
+    for (int i = 0; i < kCC; ++i) cc[i] = uf32(Ip->cc[i]);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) nsc[v] = uf32(Ip->nscale[v]);
+    for (int i = tid; i < NP * N * Hs; i += BS) ring[i] = 0.0f;
+    float x[NV], mc[kMC];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) x[v] = own ? V.x0[tid * NV + v] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < kMC; ++i) mc[i] = (V.node_mc && own) ? V.node_mc[(size_t)tid * kMC + i] : uf32(Ip->mc[i]);
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) V.ckpt[tid * NV + v] = x[v];
+    }
+    // CSR row of this node in registers: source row base, delay, weight
+    constexpr int kReg = 32;
+    int en[kReg], ed[kReg];
+    float ew[kReg];
+    int e0 = 0, e1 = 0;
+    if (own) { e0 = V.indptr[tid]; e1 = V.indptr[tid + 1]; }
+    const int nreg = min(e1 - e0, kReg);
+#pragma unroll
+    for (int k = 0; k < kReg; ++k) {
+        en[k] = 0; ed[k] = 0; ew[k] = 0.0f;
+        if (k < nreg) {
+            const uint32_t ex = V.edges[2 * (e0 + k)], ey = V.edges[2 * (e0 + k) + 1];
+            en[k] = (int)(ex & 0xFFFF) * Hs;
+            ed[k] = (int)(ex >> 16);
+            ew[k] = __uint_as_float(ey);
+        }
+    }
+    const bool ord = V.loss == HB_LOSS_ORDER;
+    const int nwaves = (BS + 63) >> 6;
+    double lsum = 0.0;
+    int bcount = 0, kwin = 0;
+    float bacc = 0.0f;
+    const size_t chan = (size_t)N * Hs;
+
+    auto ring_put = [&](int t, const float* xs) {
+        float pr[NP];
+        pre<NC, NP>(V.cfn, cc, false, false, xs, pr);
+        const int sl = wrap_slot(t, H);
+#pragma unroll
+        for (int c = 0; c < NP; ++c) {
+            const size_t b = (size_t)c * chan + (size_t)tid * Hs;
+            ring[b + sl] = pr[c];
+            if (sl < kK - 1) ring[b + H + sl] = pr[c];
+        }
+    };
+    auto step = [&](int t, const float* g) {
+        if (own) {
+            gp<float> gl = V.glog + ((size_t)t * N + tid) * NP;
+#pragma unroll
+            for (int c = 0; c < NP; ++c) gl[c] = g[c];
+            float own_pre[NP], cpl[NC], z[NV];
+            pre<NC, NP>(V.cfn, cc, false, false, x, own_pre);
+            if (V.has_c) post<NC, NP>(V.cfn, cc, false, g, own_pre, cpl);
+            else {
+#pragma unroll
+                for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) z[v] = S.stoch ? V.noise[((size_t)t * N + tid) * NV + v] * nsc[v] : 0.0f;
+            integrate<M>(S, mc, cpl, z, x);
+            bacc += x[0];
+            if (V.Sg > 0 && (t + 1) % V.Sg == 0 && t + 1 < T) {
+                gp<float> ck = V.ckpt + ((size_t)((t + 1) / V.Sg) * N + tid) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ck[v] = x[v];
+            }
+        }
+        ++bcount;
+        if (V.P > 0 && (bcount == V.P || t == T - 1)) {
+            if (own) V.zlog[(size_t)kwin * N + tid] = bacc / (float)bcount;
+            bacc = 0.0f;
+            ++kwin;
+            bcount = 0;
+        }
+    };
+    auto ord_part = [&](int buf, int k) {
+        float cs = own ? cosf(x[0]) : 0.0f, sn = own ? sinf(x[0]) : 0.0f;
+        for (int o = 32; o > 0; o >>= 1) { cs += __shfl_down(cs, o, 64); sn += __shfl_down(sn, o, 64); }
+        if ((tid & 63) == 0) { ordp[buf][k][tid >> 6][0] = cs; ordp[buf][k][tid >> 6][1] = sn; }
+    };
+    auto ord_close = [&](int buf, int k, int t) {
+        float cs = 0.0f, sn = 0.0f;
+        for (int w = 0; w < nwaves; ++w) { cs += ordp[buf][k][w][0]; sn += ordp[buf][k][w][1]; }
+        const float C = cs / (float)N, Sn = sn / (float)N;
+        const float R = sqrtf(C * C + Sn * Sn);
+        const double dd = (double)R - (double)V.target[t];
+        lsum += dd * dd;
+    };
+
+    // ---- step 0: every edge reads the current state (block 0, one step)
+    __syncthreads();
+    if (own) ring_put(0, x);
+    __syncthreads();
+    {
+        float g[NP];
+#pragma unroll
+        for (int c = 0; c < NP; ++c) g[c] = 0.0f;
+        if (own) {
+            for (int e = e0; e < e1; ++e) {
+                const uint32_t ex = V.edges[2 * e], ey = V.edges[2 * e + 1];
+                const float w = __uint_as_float(ey);
+#pragma unroll
+                for (int c = 0; c < NP; ++c) g[c] += w * ring[(size_t)c * chan + (size_t)(ex & 0xFFFF) * Hs];
+            }
+        }
+        step(0, g);
+        if (ord) ord_part(0, 0);
+    }
+    __syncthreads();
+    int kprev = 1, tprev = 0, blk = 1;
+    // ---- blocks of kK steps from t = 1; reads times t0 - d <= t0 - kK
+    for (int t0 = 1; t0 < T; t0 += kK, ++blk) {
+        const int kk = min(kK, T - t0);
+        float acc[kK][NP];
+#pragma unroll
+        for (int k = 0; k < kK; ++k)
+#pragma unroll
+            for (int c = 0; c < NP; ++c) acc[k][c] = 0.0f;
+        if (own) {
+            const int s0 = wrap_slot(t0, H);
+#pragma unroll
+            for (int e = 0; e < kReg; ++e) {
+                if (e < nreg) {
+                    int sl = s0 - ed[e];
+                    sl += sl < 0 ? H : 0;
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) {
+                        const size_t base = (size_t)c * chan + (size_t)en[e] + sl;
+#pragma unroll
+                        for (int k = 0; k < kK; ++k) acc[k][c] += ew[e] * ring[base + k];
+                    }
+                }
+            }
+            for (int e = e0 + nreg; e < e1; ++e) {
+                const uint32_t ex = V.edges[2 * e], ey = V.edges[2 * e + 1];
+                const float w = __uint_as_float(ey);
+                int sl = s0 - (int)(ex >> 16);
+                sl += sl < 0 ? H : 0;
+#pragma unroll
+                for (int c = 0; c < NP; ++c) {
+                    const size_t base = (size_t)c * chan + (size_t)(ex & 0xFFFF) * Hs + sl;
+#pragma unroll
+                    for (int k = 0; k < kK; ++k) acc[k][c] += w * ring[base + k];
+                }
+            }
+        }
+        // the previous block's order partials are complete (barrier at its end)
+        if (ord && tid == 0)
+            for (int k = 0; k < kprev; ++k) ord_close((blk - 1) & 1, k, tprev + k);
+        float xh[kK][NV];
+#pragma unroll
+        for (int k = 0; k < kK; ++k) {
+            if (k < kk) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) xh[k][v] = x[v];   // state at the start of step t0 + k
+                step(t0 + k, acc[k]);
+                if (ord) ord_part(blk & 1, k);
+            }
+        }
+        if (own) {
+#pragma unroll
+            for (int k = 0; k < kK; ++k)
+                if (k < kk) ring_put(t0 + k, xh[k]);
+        }
+        kprev = kk;
+        tprev = t0;
+        __syncthreads();
+    }
+    if (ord && tid == 0) {
+        for (int k = 0; k < kprev; ++k) ord_close((blk - 1) & 1, k, tprev + k);
+        V.loss_out[0] = (float)(lsum / (double)T);
+    }
+    if (own) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) V.fin[tid * NV + v] = x[v];
+    }
+}
+
+// Reverse sweep, delay-blocked. One node per thread; CSC out-edges. The ring
+// holds dL/dg (coupling adjoints) by step: step s reads, over out-edge
+// (j -> i, delay d >= kK), the adjoint of step s + d -- always in a later
+// block. Step 0 (every edge read the current state) runs last, per step.
+template <int M, int NP, bool LRING>
+__global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict__ insts) {
+    extern __shared__ float lds[];
+    __shared__ double redd[8];
+    const AdjInst* Ip = insts + blockIdx.x;
+    constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
+    const BView V = bview(Ip, true);
+    const int N = V.N, T = V.T, H = V.H, Hs = V.Hs, Sg = V.Sg, P = V.P;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const bool own = tid < N;
+    const size_t chan = (size_t)N * Hs;
+    using RingPtr = std::conditional_t<LRING, float*, gp<float>>;
+    RingPtr ring;
+    if constexpr (LRING) ring = lds; else ring = V.ring;
+    StepCtx S;
+    S.dt = V.dt; S.dth = V.dth; S.dt6 = V.dt6; S.integ = V.integ; S.stoch = V.stoch; S.plain = V.plain;
+    S.sw = false; S.sv = 0.0f;
+    float cc[kCC], nsc[NV];
+#pragma unroll
+    for (int i = 0; i < kCC; ++i) cc[i] = uf32(Ip->cc[i]);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) nsc[v] = uf32(Ip->nscale[v]);
+    for (int i = tid; i < NP * N * Hs; i += BS) ring[i] = 0.0f;
+    float mc[kMC], jac[kMC], lam[NV];
+#pragma unroll
+    for (int i = 0; i < kMC; ++i) {
+        mc[i] = (V.node_mc && own) ? V.node_mc[(size_t)tid * kMC + i] : uf32(Ip->mc[i]);
+        jac[i] = (V.node_jac && own) ? V.node_jac[(size_t)tid * kMC + i] : 0.0f;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lam[v] = 0.0f;
+    double gpar = 0.0, gcc = 0.0;
+    constexpr int kReg = 32;
+    int en[kReg], ed[kReg];
+    float ew[kReg];
+    int e0 = 0, e1 = 0;
+    if (own) { e0 = V.indptr[tid]; e1 = V.indptr[tid + 1]; }
+    const int nreg = min(e1 - e0, kReg);
+#pragma unroll
+    for (int k = 0; k < kReg; ++k) {
+        en[k] = 0; ed[k] = 0; ew[k] = 0.0f;
+        if (k < nreg) {
+            const uint32_t ex = V.edges[2 * (e0 + k)], ey = V.edges[2 * (e0 + k) + 1];
+            en[k] = (int)(ex & 0xFFFF) * Hs;
+            ed[k] = (int)(ex >> 16);
+            ew[k] = __uint_as_float(ey);
+        }
+    }
+    const bool seed_win = P > 0 && (V.loss == HB_LOSS_FC || V.loss == HB_LOSS_BOLD_MSE);
+    const bool seed_ord = V.loss == HB_LOSS_ORDER;
+
+    // one reverse step of own node: lam (dL/dx_{s+1}) -> lx (partial dL/dx_s), lg
+    auto vjp = [&](int s, const float* xs, const float* g, const float* zs, float* lx, float* lg) {
+        float own_pre[NP], cpl[NC], mu[NC], lown[NC];
+        pre<NC, NP>(V.cfn, cc, false, false, xs, own_pre);
+        if (V.has_c) post<NC, NP>(V.cfn, cc, false, g, own_pre, cpl);
+        else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+        }
+        float dps = 0.0f, dcs = 0.0f;
+        step_vjp<M>(S, mc, cpl, zs, xs, lam, lx, mu, jac, dps);
+        gpar += (double)dps;
+        if (V.has_c) {
+            post_vjp<NC, NP>(V.cfn, cc, g, own_pre, mu, lg, lown, dcs);
+            gcc += (double)dcs;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) lx[c] += lown[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < NP; ++c) lg[c] = 0.0f;
+        }
+        (void)s;
+    };
+    auto seed = [&](int s, int t0s, const float* xs, float* l) {
+        if (s < 1) return;
+        if (seed_win) l[0] += V.seedw[(size_t)((s - 1) / P) * N + tid];
+        if (seed_ord) {
+            const gp<float> cf = V.ord + (size_t)(s - t0s) * 4;
+            l[0] += cf[0] * sinf(xs[0]) + cf[1] * cosf(xs[0]);
+        }
+    };
+    auto ring_put = [&](int t, const float* lg) {
+        const int sl = wrap_slot(t, H);
+#pragma unroll
+        for (int c = 0; c < NP; ++c) {
+            const size_t b = (size_t)c * chan + (size_t)tid * Hs;
+            ring[b + sl] = lg[c];
+            if (sl < kK - 1) ring[b + H + sl] = lg[c];
+        }
+    };
+    __syncthreads();
+
+    for (int seg = V.nseg - 1; seg >= 0; --seg) {
+        const int t0s = seg * Sg, t1s = min(T, t0s + Sg);
+        // ---- recompute states t0s..t1s from the checkpoint and the coupling log
+        if (own) {
+            float x[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { x[v] = V.ckpt[((size_t)seg * N + tid) * NV + v]; V.tape[(size_t)tid * NV + v] = x[v]; }
+            for (int t = t0s; t < t1s; ++t) {
+                float g[NP], own_pre[NP], cpl[NC], z[NV];
+                const gp<float> gl = V.glog + ((size_t)t * N + tid) * NP;
+#pragma unroll
+                for (int c = 0; c < NP; ++c) g[c] = gl[c];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) z[v] = S.stoch ? V.noise[((size_t)t * N + tid) * NV + v] * nsc[v] : 0.0f;
+                pre<NC, NP>(V.cfn, cc, false, false, x, own_pre);
+                if (V.has_c) post<NC, NP>(V.cfn, cc, false, g, own_pre, cpl);
+                else {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+                }
+                integrate<M>(S, mc, cpl, z, x);
+                gp<float> tp = V.tape + ((size_t)(t - t0s + 1) * N + tid) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) tp[v] = x[v];
+            }
+        }
+        __syncthreads();
+        if (seed_ord) {
+            const int w = tid >> 6, l = tid & 63, nwv = BS >> 6;
+            for (int r = w; r <= t1s - t0s; r += nwv) {
+                float cs = 0.0f, sn = 0.0f;
+                for (int n = l; n < N; n += 64) {
+                    const float th = V.tape[((size_t)r * N + n) * NV];
+                    cs += cosf(th);
+                    sn += sinf(th);
+                }
+                for (int o = 32; o > 0; o >>= 1) { cs += __shfl_down(cs, o, 64); sn += __shfl_down(sn, o, 64); }
+                if (l == 0) {
+                    const int st = t0s + r;
+                    float a = 0.0f, b = 0.0f;
+                    if (st >= 1) {
+                        const float C = cs / (float)N, Sn = sn / (float)N;
+                        const float R = sqrtf(C * C + Sn * Sn);
+                        const float al = R > 0.0f ? 2.0f * (R - V.target[st - 1]) / ((float)T * (float)N * R) : 0.0f;
+                        a = -al * C;
+                        b = al * Sn;
+                    }
+                    V.ord[(size_t)r * 4 + 0] = a;
+                    V.ord[(size_t)r * 4 + 1] = b;
+                }
+            }
+            __syncthreads();
+        }
+        if (seg == V.nseg - 1 && own) {
+            float xT[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xT[v] = V.tape[((size_t)(t1s - t0s) * N + tid) * NV + v];
+            seed(T, t0s, xT, lam);
+        }
+        // ---- blocks of reverse steps over [max(t0s, 1), t1s - 1], highest first
+        const int tlo = max(t0s, 1);
+        for (int ttop = t1s - 1; ttop >= tlo; ttop -= kK) {
+            const int kk = min(kK, ttop - tlo + 1), tbot = ttop - kk + 1;
+            float xb[kK][NV], gb[kK][NP], zb[kK][NV];
+            if (own) {
+#pragma unroll
+                for (int m = 0; m < kK; ++m) {
+                    if (m < kk) {
+                        const int st = tbot + m;
+                        const gp<float> tp = V.tape + ((size_t)(st - t0s) * N + tid) * NV;
+                        const gp<float> gl = V.glog + ((size_t)st * N + tid) * NP;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) {
+                            xb[m][v] = tp[v];
+                            zb[m][v] = S.stoch ? V.noise[((size_t)st * N + tid) * NV + v] * nsc[v] : 0.0f;
+                        }
+#pragma unroll
+                        for (int c = 0; c < NP; ++c) gb[m][c] = gl[c];
+                    }
+                }
+            }
+            float acc[kK][NP];
+#pragma unroll
+            for (int m = 0; m < kK; ++m)
+#pragma unroll
+                for (int c = 0; c < NP; ++c) acc[m][c] = 0.0f;
+            if (own && V.has_c) {
+                const int sb = wrap_slot(tbot, H);
+#pragma unroll
+                for (int e = 0; e < kReg; ++e) {
+                    if (e < nreg) {
+                        int sl = sb + ed[e];
+                        sl -= sl >= H ? H : 0;
+#pragma unroll
+                        for (int c = 0; c < NP; ++c) {
+                            const size_t base = (size_t)c * chan + (size_t)en[e] + sl;
+#pragma unroll
+                            for (int m = 0; m < kK; ++m) acc[m][c] += ew[e] * ring[base + m];
+                        }
+                    }
+                }
+                for (int e = e0 + nreg; e < e1; ++e) {
+                    const uint32_t ex = V.edges[2 * e], ey = V.edges[2 * e + 1];
+                    const float w = __uint_as_float(ey);
+                    int sl = sb + (int)(ex >> 16);
+                    sl -= sl >= H ? H : 0;
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) {
+                        const size_t base = (size_t)c * chan + (size_t)(ex & 0xFFFF) * Hs + sl;
+#pragma unroll
+                        for (int m = 0; m < kK; ++m) acc[m][c] += w * ring[base + m];
+                    }
+                }
+            }
+            float lgb[kK][NP];
+            if (own) {
+#pragma unroll
+                for (int m = kK - 1; m >= 0; --m) {
+                    if (m < kk) {
+                        const int st = tbot + m;
+                        float lx[NV];
+                        vjp(st, xb[m], gb[m], zb[m], lx, lgb[m]);
+                        pre_vjp<NC, NP>(xb[m], acc[m], lx);
+                        seed(st, t0s, xb[m], lx);
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) lam[v] = lx[v];
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < kK; ++m)
+                    if (m < kk) ring_put(tbot + m, lgb[m]);
+            }
+            __syncthreads();
+        }
+        if (seg > 0) __syncthreads();
+    }
+    // ---- step 0: the delayed terms (steps d) plus every edge's current-state read
+    {
+        float lx[NV], lg[NP], acc[NP];
+        float x0v[NV], g0[NP], z0[NV];
+        if (own) {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                x0v[v] = V.tape[(size_t)tid * NV + v];
+                z0[v] = S.stoch ? V.noise[(size_t)tid * NV + v] * nsc[v] : 0.0f;
+            }
+#pragma unroll
+            for (int c = 0; c < NP; ++c) g0[c] = V.glog[(size_t)tid * NP + c];
+            vjp(0, x0v, g0, z0, lx, lg);
+            ring_put(0, lg);
+        }
+        __syncthreads();
+        if (own) {
+#pragma unroll
+            for (int c = 0; c < NP; ++c) acc[c] = 0.0f;
+            if (V.has_c) {
+                for (int e = e0; e < e1; ++e) {
+                    const uint32_t ex = V.edges[2 * e], ey = V.edges[2 * e + 1];
+                    const float w = __uint_as_float(ey);
+                    const int d = (int)(ex >> 16), node = (int)(ex & 0xFFFF);
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) {
+                        const size_t b = (size_t)c * chan + (size_t)node * Hs;
+                        acc[c] += w * ring[b + wrap_slot(d, H)];
+                        if (d > 0) acc[c] += w * ring[b];
+                    }
+                }
+            }
+            pre_vjp<NC, NP>(x0v, acc, lx);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) lam[v] = lx[v];
+            if (V.gx0) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) V.gx0[tid * NV + v] = lam[v];
+            }
+            if (V.gp_out) V.gp_out[tid] = (float)gpar;
+        }
+    }
+    const double tot = block_sum_d(gcc, redd);
+    if (tid == 0) V.gcc_out[0] = (float)tot;
+}
+
+// ------------------------------------------------------------ dispatch -----
+
+template <int M, int NP>
+static const void* ptr(bool rev, int npt) {
+    if (rev) return npt == 1 ? (const void*)k_adj_reverse<M, NP, 1> : (const void*)k_adj_reverse<M, NP, 2>;
+    return npt == 1 ? (const void*)k_adj_forward<M, NP, 1> : (const void*)k_adj_forward<M, NP, 2>;
+}
+
+template <int M, int NP>
+static const void* ptr_b(bool rev, bool lring) {
+    if (rev) return lring ? (const void*)k_adj_reverse_b<M, NP, true> : (const void*)k_adj_reverse_b<M, NP, false>;
+    return lring ? (const void*)k_adj_forward_b<M, NP, true> : (const void*)k_adj_forward_b<M, NP, false>;
+}
+
+static const void* kernel_b(int model, bool rev, int np, bool lring) {
     switch (model) {
-    case HB_G2DO: return pick<HB_G2DO>(rev, kur, npt);
-    case HB_MPR: return pick<HB_MPR>(rev, kur, npt);
-    case HB_JANSEN_RIT: return pick<HB_JANSEN_RIT>(rev, kur, npt);
-    case HB_WILSON_COWAN: return pick<HB_WILSON_COWAN>(rev, kur, npt);
-    case HB_KURAMOTO: return pick<HB_KURAMOTO>(rev, kur, npt);
-    default: return pick<HB_RWW>(rev, kur, npt);
+    case HB_G2DO: return np == 1 ? ptr_b<HB_G2DO, 1>(rev, lring) : nullptr;
+    case HB_MPR: return np == 1 ? ptr_b<HB_MPR, 1>(rev, lring) : np == 2 ? ptr_b<HB_MPR, 2>(rev, lring) : nullptr;
+    case HB_JANSEN_RIT: return np == 1 ? ptr_b<HB_JANSEN_RIT, 1>(rev, lring) : nullptr;
+    case HB_WILSON_COWAN: return np == 1 ? ptr_b<HB_WILSON_COWAN, 1>(rev, lring) : nullptr;
+    case HB_KURAMOTO: return np == 1 ? ptr_b<HB_KURAMOTO, 1>(rev, lring) : np == 2 ? ptr_b<HB_KURAMOTO, 2>(rev, lring) : nullptr;
+    case HB_RWW: return np == 1 ? ptr_b<HB_RWW, 1>(rev, lring) : nullptr;
+    default: return nullptr;
+    }
+}
+
+// Instantiated (model, pre channels): Kuramoto with and without Kuramoto
+// coupling, MPR with two channels or with the zero-coefficient one dropped.
+static const void* kernel_for(int model, bool rev, int np, int npt) {
+    switch (model) {
+    case HB_G2DO: return np == 1 ? ptr<HB_G2DO, 1>(rev, npt) : nullptr;
+    case HB_MPR: return np == 1 ? ptr<HB_MPR, 1>(rev, npt) : np == 2 ? ptr<HB_MPR, 2>(rev, npt) : nullptr;
+    case HB_JANSEN_RIT: return np == 1 ? ptr<HB_JANSEN_RIT, 1>(rev, npt) : nullptr;
+    case HB_WILSON_COWAN: return np == 1 ? ptr<HB_WILSON_COWAN, 1>(rev, npt) : nullptr;
+    case HB_KURAMOTO: return np == 1 ? ptr<HB_KURAMOTO, 1>(rev, npt) : np == 2 ? ptr<HB_KURAMOTO, 2>(rev, npt) : nullptr;
+    case HB_RWW: return np == 1 ? ptr<HB_RWW, 1>(rev, npt) : nullptr;
+    default: return nullptr;
     }
 }
 
@@ -969,10 +1524,11 @@ static const void* kernel_for(int model, bool rev, bool kur, int npt) {
 hipError_t adjoint_set_attributes(int max_lds) {
     for (int m = 0; m <= HB_RWW; ++m)
         for (int rev = 0; rev < 2; ++rev)
-            for (int kur = 0; kur < 2; ++kur)
-                for (int npt = 1; npt <= 2; ++npt) {
-                    hipError_t e = hipFuncSetAttribute(adj::kernel_for(m, rev, kur, npt),
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+            for (int np = 1; np <= 2; ++np)
+                for (int v = 0; v < 4; ++v) {
+                    const void* k = v < 2 ? adj::kernel_for(m, rev, np, v + 1) : adj::kernel_b(m, rev, np, v == 3);
+                    if (!k) continue;
+                    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
                     if (e != hipSuccess) return e;
                 }
     return hipSuccess;
@@ -982,7 +1538,8 @@ hipError_t launch_adjoint_stage(int stage, const AdjLaunch& L, hipStream_t s) {
     const AdjInst* d = L.insts;
     switch (stage) {
     case 0: {
-        const void* k = adj::kernel_for(L.model, false, L.kur, L.npt);
+        const void* k = L.blocked ? adj::kernel_b(L.model, false, L.np, L.lring) : adj::kernel_for(L.model, false, L.np, L.npt);
+        if (!k) return hipErrorInvalidValue;
         void* args[] = {(void*)&d};
         return hipLaunchKernel(k, dim3(L.n), dim3(L.block), args, L.lds_fwd, s);
     }
@@ -990,7 +1547,8 @@ hipError_t launch_adjoint_stage(int stage, const AdjLaunch& L, hipStream_t s) {
     case 2: hipLaunchKernelGGL(adj::k_loss, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
     case 3: hipLaunchKernelGGL(adj::k_bold_reverse, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
     default: {
-        const void* k = adj::kernel_for(L.model, true, L.kur, L.npt);
+        const void* k = L.blocked ? adj::kernel_b(L.model, true, L.np, L.lring) : adj::kernel_for(L.model, true, L.np, L.npt);
+        if (!k) return hipErrorInvalidValue;
         void* args[] = {(void*)&d};
         return hipLaunchKernel(k, dim3(L.n), dim3(L.block), args, L.lds_rev, s);
     }

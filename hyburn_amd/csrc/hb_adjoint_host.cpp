@@ -146,7 +146,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
     HB_HIP(hipSetDevice(device));
     int lds_max = 0;
     HB_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-    const int lds_budget = lds_max - 512;
+    const int lds_budget = lds_max - 2048;   // the adjoint kernels' static LDS is at most 1 KB
     static bool attrs_done[64] = {false};
     if (device >= 0 && device < 64 && !attrs_done[device]) {
         HB_HIP(adjoint_set_attributes(lds_budget));
@@ -159,31 +159,38 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
     P->inst.assign(inst, inst + n);
 
     // ---- shapes and grouping
-    struct Shape { int nv, nc, np, npt, block, Hd, S, nseg, nwin, nnz; bool kur, ring_lds; size_t ring_b; };
+    struct Shape { int nv, nc, np, npt, block, Hd, S, nseg, nwin, nnz; bool kur, ring_lds, blocked; size_t ring_b; };
+    constexpr int kBlock = 8;   // steps per delay block of the blocked kernels
     std::vector<Shape> sh(n);
-    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+    std::map<std::tuple<int, int, int, int, int>, std::vector<int>> groups;
     for (int i = 0; i < n; ++i) {
         const hb_instance& I = inst[i];
         Shape& s = sh[i];
         s.nv = model_nvar(I.model);
         s.nc = model_ncvar(I.model);
         s.kur = I.cfn == HB_CF_KURAMOTO && I.wkind == HB_W_CSR;
-        s.np = s.kur ? 2 * s.nc : s.nc;
+        s.np = I.wkind == HB_W_CSR ? ring_channels(I.model, I.cfn, I.params, I.node_param_idx, spec[i].grad_param)
+                                   : s.nc;
         s.npt = I.n_nodes <= 512 ? 1 : 2;
         s.block = s.npt == 1 ? std::max(64, (I.n_nodes + 63) / 64 * 64) : 512;
-        int dmax = 0;
+        int dmax = 0, dmin = 1 << 30;
         s.nnz = I.wkind == HB_W_CSR ? I.nnz : 0;
-        if (I.edge_delays)
-            for (int e = 0; e < s.nnz; ++e) dmax = std::max(dmax, (int)I.edge_delays[e]);
-        s.Hd = dmax + 2;
+        for (int e = 0; e < s.nnz; ++e) {
+            const int d = I.edge_delays ? (int)I.edge_delays[e] : 0;
+            dmax = std::max(dmax, d);
+            dmin = std::min(dmin, d);
+        }
+        // delay-blocked kernels: one barrier per kBlock steps when every delay >= kBlock
+        s.blocked = s.nnz > 0 && dmin >= kBlock && I.n_nodes <= 512 && !(opts && (opts->flags & HB_FLAG_NO_BLOCKING));
+        s.Hd = s.blocked ? dmax + kBlock + 1 : dmax + 2;
         const int ce = opts ? opts->checkpoint_every : 0;
         s.S = ce > 0 ? ce : std::max(1, (int)std::ceil(std::sqrt((double)I.n_steps)));
         s.S = std::min(s.S, I.n_steps);
         s.nseg = (I.n_steps + s.S - 1) / s.S;
         s.nwin = I.bold_period > 0 ? (I.n_steps + I.bold_period - 1) / I.bold_period : 0;
-        s.ring_b = (size_t)s.Hd * I.n_nodes * s.np * 4;
+        s.ring_b = (size_t)(s.Hd + (s.blocked ? kBlock - 1 : 0)) * I.n_nodes * s.np * 4;
         s.ring_lds = s.ring_b <= (size_t)lds_budget;
-        groups[std::make_tuple(I.model, (int)s.kur, s.npt)].push_back(i);
+        groups[std::make_tuple(I.model, s.np, s.npt, (int)s.blocked, (int)s.ring_lds)].push_back(i);
         P->node_steps += (double)I.n_nodes * (double)I.n_steps;
     }
 
@@ -395,8 +402,10 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         L.insts = P->d_inst + pos;
         L.n = (int)kv.second.size();
         L.model = std::get<0>(kv.first);
-        L.kur = std::get<1>(kv.first) != 0;
+        L.np = std::get<1>(kv.first);
         L.npt = std::get<2>(kv.first);
+        L.blocked = std::get<3>(kv.first) != 0;
+        L.lring = std::get<4>(kv.first) != 0;
         int blk = 64, lds = 0;
         for (int i : kv.second) {
             blk = std::max(blk, sh[i].block);

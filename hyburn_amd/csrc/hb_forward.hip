@@ -842,6 +842,7 @@ __global__ __launch_bounds__(512) void k_mixed(LaunchDesc a) {
         switch (key) {
         case HB_G2DO * 4 + 1: run_any<HB_G2DO, 1, LRING, MULTI>(I, lds); break;
         case HB_MPR * 4 + 2: run_any<HB_MPR, 2, LRING, MULTI>(I, lds); break;
+        case HB_MPR * 4 + 1: run_any<HB_MPR, 1, LRING, MULTI>(I, lds); break;
         case HB_JANSEN_RIT * 4 + 1: run_any<HB_JANSEN_RIT, 1, LRING, MULTI>(I, lds); break;
         case HB_WILSON_COWAN * 4 + 1: run_any<HB_WILSON_COWAN, 1, LRING, MULTI>(I, lds); break;
         case HB_KURAMOTO * 4 + 1: run_any<HB_KURAMOTO, 1, LRING, MULTI>(I, lds); break;
@@ -863,11 +864,14 @@ static const void* kptr_single(bool lring, bool multi) {
 #ifndef HB_UNIT
 #define HB_UNIT 0
 #endif
-static const void* unit_kernel(bool lring, bool multi) {
+// MPR has a second instantiation with one pre channel: when its second
+// coupling coefficient (cv) is zero the V channel only ever adds c1 * 0.
+static const void* unit_kernel(bool lring, bool multi, int np) {
+    (void)np;
 #if HB_UNIT == 0
     return kptr_single<HB_G2DO, 1>(lring, multi);
 #elif HB_UNIT == 1
-    return kptr_single<HB_MPR, 2>(lring, multi);
+    return np == 1 ? kptr_single<HB_MPR, 1>(lring, multi) : kptr_single<HB_MPR, 2>(lring, multi);
 #elif HB_UNIT == 2
     return kptr_single<HB_JANSEN_RIT, 1>(lring, multi);
 #elif HB_UNIT == 3
@@ -890,15 +894,16 @@ static const void* unit_kernel(bool lring, bool multi) {
 static hipError_t unit_op(int op, const LaunchDesc* d, hipStream_t s, int arg, int* out) {
     if (op == 2) {
         for (int lr = 0; lr < 2; ++lr)
-            for (int mu = 0; mu < 2; ++mu) {
-                const void* k = unit_kernel(lr != 0, mu != 0);
-                if (!k) continue;
-                hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, arg);
-                if (e != hipSuccess) return e;
-            }
+            for (int mu = 0; mu < 2; ++mu)
+                for (int np = 1; np <= 2; ++np) {
+                    const void* k = unit_kernel(lr != 0, mu != 0, np);
+                    if (!k) continue;
+                    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, arg);
+                    if (e != hipSuccess) return e;
+                }
         return hipSuccess;
     }
-    const void* k = unit_kernel(d->lds_ring, d->multi);
+    const void* k = unit_kernel(d->lds_ring, d->multi, d->np);
     if (!k) return hipErrorInvalidValue;
     if (op == 1) return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k, d->block, (size_t)d->lds_bytes);
     LaunchDesc a = *d;

@@ -28,6 +28,7 @@ int unit_of(const LaunchDesc& d) {
     switch (d.model * 4 + d.np) {
     case HB_G2DO * 4 + 1: return 0;
     case HB_MPR * 4 + 2: return 1;
+    case HB_MPR * 4 + 1: return 1;
     case HB_JANSEN_RIT * 4 + 1: return 2;
     case HB_WILSON_COWAN * 4 + 1: return 3;
     case HB_KURAMOTO * 4 + 1: return 4;

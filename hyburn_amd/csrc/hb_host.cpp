@@ -288,7 +288,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         Shape& s = sh[i];
         s.nv = model_nvar(I.model);
         s.nc = model_ncvar(I.model);
-        s.np = coupling_npre(I.cfn, s.nc);
+        s.np = ring_channels(I.model, I.cfn, I.params, I.node_param_idx, -1);
         s.depth = max_delay_of(I) + 2;
         int b = req_block > 0 ? req_block : (int)align_up((size_t)I.n_nodes, 64);
         b = std::max(64, std::min(512, b));

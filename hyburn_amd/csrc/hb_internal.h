@@ -43,6 +43,18 @@ HB_HD int coupling_npre(int cfn, int nc) {
     return nc;
 }
 
+// Pre channels the ring must hold for this instance: MPR couples through
+// c0 * cr + c1 * cv (dfun.rs:455-477); with cv == 0 (the default) the second
+// channel only adds c1 * 0, so it is neither stored nor gathered. Per-node
+// values of cv, or a gradient with respect to cv, keep it.
+HB_HD int ring_channels(int model, int cfn, const float* params, int node_param_idx, int grad_param) {
+    const int np = coupling_npre(cfn, model_ncvar(model));
+    if (model == HB_MPR && np == 2 && cfn != HB_CF_KURAMOTO && params && params[6] == 0.0f &&
+        node_param_idx != 6 && grad_param != 6)
+        return 1;
+    return np;
+}
+
 // f32 epsilon guard used by the reference for time constants (dfun.rs:279).
 constexpr float kEps = 1.1920929e-07f;
 constexpr float kPi = 3.14159265358979323846f;

@@ -199,6 +199,59 @@ def config3(n_inst: int = 4096, seed: int = 4, t_scale: float = 1.0, bold: bool 
     return out
 
 
+def config4(n_inst: int = 256, n_steps: int = 20_000, seed: int = 5, target_seed: int = 6):
+    """Differentiation: MPR, 84 nodes, CSR 10% lognormal, per-edge delays
+    round(U[10,90] mm / 3 mm/ms / 0.1 ms) <= 300 steps, Heun dt = 0.1 ms,
+    per-instance BOLD on the native 1 ms grid (bold_period 10), per-instance
+    G and eta perturbations. Returns (instances, hidden instance for the FC
+    target): the target is the FC of the hidden-parameter run."""
+    rng = np.random.default_rng(seed)
+    N, dt = 84, 0.1
+    W = lognormal_connectome(np.random.default_rng(seed + 100), N, 0.10, symmetric=False)
+    ip, ix, w = dense_to_csr(W)
+    lengths = np.random.default_rng(seed + 200).uniform(10.0, 90.0, len(ix))
+    d = delays_from_lengths(lengths, 3.0, dt)
+    x0 = np.stack([np.full(N, 0.1), np.full(N, -2.0)], 1).astype(np.float32)
+
+    def make(G, eta):
+        return Instance(model=L.MPR, n_nodes=N, n_steps=n_steps, dt=dt, params=MPR_DEFAULT.copy(), x0=x0,
+                        integrator=L.HEUN, wkind=L.W_CSR, cfn=L.CF_LINEAR, cparams=(float(G), 0.0),
+                        indptr=ip, indices=ix, weights=w, edge_delays=d, delay_mode=L.DELAY_PER_EDGE,
+                        node_param_idx=2, node_param=eta.astype(np.float32), bold_period=10)
+    eta0 = np.full(N, MPR_DEFAULT[2], np.float32)
+    insts = [make(0.3 * (1.0 + 0.2 * rng.standard_normal()), eta0 + 0.1 * rng.standard_normal(N))
+             for _ in range(n_inst)]
+    trng = np.random.default_rng(target_seed)
+    hidden = make(0.4, eta0 + 0.2 * trng.standard_normal(N))
+    return insts, hidden
+
+
+def config5(n_inst: int = 512, n_steps: int = 10_000, seed: int = 7, max_delay: int = 400):
+    """Large-network adjoint: Kuramoto, 1000 nodes placed uniformly in a
+    150 mm cube, CSR 5% lognormal, delays round(dist / 3 mm/ms / 0.1 ms)
+    clipped to 400 steps (40 ms), natural frequencies N(10 Hz, 1 Hz) in rad/ms,
+    RK4 dt = 0.1 ms, G = linspace(0, 2) over the batch. Returns (instances,
+    order-parameter target trace [n_steps])."""
+    rng = np.random.default_rng(seed)
+    N, dt = 1000, 0.1
+    pos = rng.uniform(0.0, 150.0, (N, 3))
+    W = lognormal_connectome(np.random.default_rng(seed + 100), N, 0.05, symmetric=False)
+    ip, ix, w = dense_to_csr(W)
+    tgt = np.repeat(np.arange(N), np.diff(ip))
+    dist = np.linalg.norm(pos[tgt] - pos[ix], axis=1)
+    d = np.minimum(delays_from_lengths(dist, 3.0, dt), max_delay).astype(np.int32)
+    omega = (2.0 * np.pi * rng.normal(10.0, 1.0, N) / 1000.0).astype(np.float32)
+    x0 = rng.uniform(0.0, 2.0 * np.pi, (N, 1)).astype(np.float32)
+    Gs = np.linspace(0.0, 2.0, n_inst)
+    insts = [Instance(model=L.KURAMOTO, n_nodes=N, n_steps=n_steps, dt=dt, params=np.array([0.0], np.float32),
+                      x0=x0, integrator=L.RK4, wkind=L.W_CSR, cfn=L.CF_KURAMOTO, cparams=(float(G),),
+                      indptr=ip, indices=ix, weights=w, edge_delays=d, delay_mode=L.DELAY_PER_EDGE,
+                      node_param_idx=0, node_param=omega) for G in Gs]
+    t = np.arange(n_steps) * dt
+    target = (0.5 + 0.3 * np.sin(2.0 * np.pi * t / 400.0) * np.exp(-t / 2000.0)).astype(np.float32)
+    return insts, target
+
+
 def node_steps(insts) -> float:
     return float(sum(I.n_nodes * I.n_steps for I in insts))
 

@@ -227,7 +227,7 @@ typedef struct {
 typedef struct {
     int32_t device;
     int32_t checkpoint_every; /* steps per checkpoint segment; 0: ceil(sqrt(n_steps)) */
-    uint32_t flags;           /* reserved, 0 */
+    uint32_t flags;           /* HB_FLAG_NO_BLOCKING: one barrier per step even when min delay >= 8 */
 } hb_adjoint_opts;
 
 typedef struct {

@@ -41,9 +41,20 @@ CASES = {
 }
 
 
-def gpu_grads(I, kind, target, checkpoint_every=0):
+# every delay >= 8 steps: the delay-blocked kernels (one barrier per 8 steps)
+BLOCKED = {
+    "kuramoto_order_rk4_blocked": lambda orc: kuramoto_case(N=16, T=1000, dmin=8, dmax=40, seed=6, dens=0.3),
+    "mpr_bold_mse_heun_blocked": lambda orc: bold_mse_case(L.MPR, orc, G=0.2, node_param_idx=2, dmin=8, dmax=20),
+    "jr_bold_mse_rk4_blocked": lambda orc: bold_mse_case(L.JANSEN_RIT, orc, integ=L.RK4, G=0.5, node_param_idx=12,
+                                                         dmin=8, dmax=16),
+    "g2do_bold_mse_heun_stoch_blocked": lambda orc: bold_mse_case(L.G2DO, orc, integ=L.HEUN_STOCH, noise=True,
+                                                                  node_param_idx=1, dmin=9, dmax=30),
+}
+
+
+def gpu_grads(I, kind, target, checkpoint_every=0, blocking=True):
     gp = I.node_param_idx
-    with AdjointPlan([I], [Loss(kind, target, gp)], checkpoint_every=checkpoint_every) as P:
+    with AdjointPlan([I], [Loss(kind, target, gp)], checkpoint_every=checkpoint_every, blocking=blocking) as P:
         loss = P.forward()
         g = P.backward()[0]
     return float(loss[0]), g
@@ -55,9 +66,9 @@ def ad():
     return hb_autodiff
 
 
-@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("name", list(CASES) + list(BLOCKED))
 def test_adjoint_matches_fp64_autograd(gpu, orc, ad, name):
-    I, kind, target = CASES[name](orc)
+    I, kind, target = {**CASES, **BLOCKED}[name](orc)
     print(f"[adjoint] {name}: N={I.n_nodes} T={I.n_steps}", flush=True)
     loss, g = gpu_grads(I, kind, target)
     lt, gG, gP, gX = ad.gradients(I, kind, target)
@@ -95,8 +106,22 @@ def test_adjoint_fc_loss_long_run_vs_fd(gpu, orc):
     assert rel_err(g.grad_param, fP) <= 5e-3, (g.grad_param, fP)
 
 
-def test_checkpoint_interval_is_bitwise_invariant(gpu, orc):
-    I, kind, target = CASES["kuramoto_order_rk4_long"](orc)
+@pytest.mark.parametrize("name", list(BLOCKED))
+def test_blocked_schedule_is_bitwise_per_step(gpu, orc, name):
+    """The delay-blocked kernels compute the same arithmetic in the same order
+    as the one-barrier-per-step kernels."""
+    I, kind, target = BLOCKED[name](orc)
+    la, ga = gpu_grads(I, kind, target, blocking=True)
+    lb, gb = gpu_grads(I, kind, target, blocking=False)
+    assert la == lb
+    assert ga.grad_coupling == gb.grad_coupling
+    np.testing.assert_array_equal(ga.grad_param, gb.grad_param)
+    np.testing.assert_array_equal(ga.grad_x0, gb.grad_x0)
+
+
+@pytest.mark.parametrize("name", ["kuramoto_order_rk4_long", "kuramoto_order_rk4_blocked"])
+def test_checkpoint_interval_is_bitwise_invariant(gpu, orc, name):
+    I, kind, target = {**CASES, **BLOCKED}[name](orc)
     ref = None
     for S in (0, 1, 7, 250, I.n_steps):
         loss, g = gpu_grads(I, kind, target, checkpoint_every=S)
