@@ -116,6 +116,7 @@ struct hb_adjoint_plan {
     std::vector<hb_adjoint_spec> spec; // caller order
     std::vector<hb_instance> inst;     // caller order (host pointers only for outputs / sizes)
     std::vector<OutPtrs> outs;         // caller order
+    size_t outs_off = 0, outs_bytes = 0;   // contiguous output region of the arena
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     double node_steps = 0.0;
     bool forward_done = false;
@@ -235,11 +236,24 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         o.target = A.take(tgt_n * 4);
         o.fc = Sp.loss_kind == HB_LOSS_FC ? A.take((3 * N + 2 * N * N) * 8) : 0;
         o.ord = A.take((size_t)(s.S + 1) * 16);
-        o.fin = A.take(N * s.nv * 4);
-        o.loss = A.take(4);
-        o.gcc = A.take(4);
-        o.gp = A.take(N * 4);
-        o.gx0 = A.take(N * s.nv * 4);
+    }
+    // outputs of every instance in one region (one download): per instance
+    // loss, dL/dcc0, dL/dp[N], dL/dx0[N][NV], final state[N][NV]
+    size_t out_floats = 0;
+    std::vector<size_t> out_at(n);
+    for (int i = 0; i < n; ++i) {
+        out_at[i] = out_floats;
+        out_floats += 2 + (size_t)inst[i].n_nodes * (1 + 2 * sh[i].nv);
+    }
+    const size_t outs_off = A.take(out_floats * 4);
+    for (int i = 0; i < n; ++i) {
+        Off& o = off[i];
+        const size_t N = inst[i].n_nodes, b = outs_off + out_at[i] * 4;
+        o.loss = b;
+        o.gcc = b + 4;
+        o.gp = b + 8;
+        o.gx0 = o.gp + N * 4;
+        o.fin = o.gx0 + N * sh[i].nv * 4;
     }
     // noise jobs after the instances
     std::vector<NoiseJob> jobs;
@@ -377,9 +391,11 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         D.out_gp = o.has_jac ? (float*)(base + o.gp) : nullptr;
         D.out_gx0 = (float*)(base + o.gx0);
         OutPtrs& op = P->outs[i];
-        op.loss = o.loss; op.gcc = o.gcc; op.gp = o.has_jac ? o.gp : 0; op.gx0 = o.gx0; op.fin = o.fin;
+        op.loss = o.loss; op.gcc = o.gcc; op.gp = o.gp; op.gx0 = o.gx0; op.fin = o.fin;
         op.n_nodes = N; op.nv = s.nv;
     }
+    P->outs_off = outs_off;
+    P->outs_bytes = out_floats * 4;
     P->d_inst = (AdjInst*)(base + inst_off);
     he = hipMemcpy(P->d_inst, hdev.data(), sizeof(AdjInst) * (size_t)n, hipMemcpyHostToDevice);
     if (he == hipSuccess && !jobs.empty()) {
@@ -463,20 +479,21 @@ int hb_adjoint_download(hb_adjoint_plan* P) {
     if (!P) return set_error(HB_INVALID_STATE, "null adjoint plan");
     HB_HIP(hipSetDevice(P->device));
     HB_HIP(hipStreamSynchronize(P->stream));
-    const char* base = P->arena;
+    std::vector<char> host(P->outs_bytes);
+    HB_HIP(hipMemcpy(host.data(), P->arena + P->outs_off, P->outs_bytes, hipMemcpyDeviceToHost));
+    const char* base = host.data() - P->outs_off;   // arena offsets into the staged copy
     for (size_t i = 0; i < P->spec.size(); ++i) {
         const hb_adjoint_spec& S = P->spec[i];
         const hb_instance& I = P->inst[i];
         const OutPtrs& o = P->outs[i];
-        HB_HIP(hipMemcpy(S.loss, base + o.loss, 4, hipMemcpyDeviceToHost));
+        memcpy(S.loss, base + o.loss, 4);
         float gcc = 0.0f;
-        HB_HIP(hipMemcpy(&gcc, base + o.gcc, 4, hipMemcpyDeviceToHost));
+        memcpy(&gcc, base + o.gcc, 4);
         // Kuramoto holds a / n_src (coupling.rs:370); the gradient is reported for a
         S.grad_coupling[0] = I.cfn == HB_CF_KURAMOTO ? gcc / (float)I.n_nodes : gcc;
-        if (S.grad_param >= 0 && S.grad_param_out)
-            HB_HIP(hipMemcpy(S.grad_param_out, base + o.gp, (size_t)o.n_nodes * 4, hipMemcpyDeviceToHost));
-        if (S.grad_x0) HB_HIP(hipMemcpy(S.grad_x0, base + o.gx0, (size_t)o.n_nodes * o.nv * 4, hipMemcpyDeviceToHost));
-        if (S.final_state) HB_HIP(hipMemcpy(S.final_state, base + o.fin, (size_t)o.n_nodes * o.nv * 4, hipMemcpyDeviceToHost));
+        if (S.grad_param >= 0 && S.grad_param_out) memcpy(S.grad_param_out, base + o.gp, (size_t)o.n_nodes * 4);
+        if (S.grad_x0) memcpy(S.grad_x0, base + o.gx0, (size_t)o.n_nodes * o.nv * 4);
+        if (S.final_state) memcpy(S.final_state, base + o.fin, (size_t)o.n_nodes * o.nv * 4);
     }
     return HB_OK;
 }
