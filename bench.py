@@ -89,13 +89,16 @@ def ragged_leg(args):
     insts = config3(n_inst=args.ragged_n, t_scale=args.ragged_tscale)
     ns = sum(I.n_nodes * I.n_steps for I in insts)
     log(f"[bench] ragged: {len(insts)} instances, {ns:.3e} node-steps; planning")
-    with Plan(insts) as P:
+    from hyburn_amd.instances import autotune_entries
+    t0 = time.perf_counter()
+    with Plan(insts, autotune=True) as P:
+        plan_s = time.perf_counter() - t0
         P.run()
         t0 = time.perf_counter()
         st = P.run()
         wall = time.perf_counter() - t0
     res = {"n_instances": len(insts), "node_steps": ns, "value": ns / wall, "wall_s": wall,
-           "kernel_ms": st.kernel_ms}
+           "kernel_ms": st.kernel_ms, "plan_and_autotune_s": plan_s, "autotuned": autotune_entries()}
     # per-instance plain launches: one plan (one launch) per instance, sampled
     k = min(args.ragged_plain_sample, len(insts))
     idx = list(range(0, len(insts), max(1, len(insts) // k)))[:k]

@@ -20,7 +20,7 @@ CF_LINEAR, CF_DIFFERENCE, CF_KURAMOTO, CF_SIGMOIDAL, CF_SCALED_LINEAR, CF_TANH, 
 W_NONE, W_SCALAR_MEAN, W_SCALAR_DIAG, W_DENSE, W_CSR = 0, 1, 2, 3, 4
 DELAY_PER_EDGE, DELAY_SINGLE, DELAY_BATCH_COMPAT = 0, 1, 2
 ENGINE_BATCH, ENGINE_PLAIN = 0, 1
-FLAG_TRAJECTORY, FLAG_EXACT, FLAG_NO_GRAPH, FLAG_NO_BLOCKING = 0x1, 0x2, 0x4, 0x8
+FLAG_TRAJECTORY, FLAG_EXACT, FLAG_NO_GRAPH, FLAG_NO_BLOCKING, FLAG_AUTOTUNE = 0x1, 0x2, 0x4, 0x8, 0x10
 LOSS_FC, LOSS_BOLD_MSE, LOSS_ORDER = 0, 1, 2
 
 MODEL_NAMES = {
@@ -151,6 +151,8 @@ PROTOTYPES = {
     "hb_adjoint_backward": (C.c_int, [C.c_void_p, C.POINTER(HbAdjointStats)]),
     "hb_adjoint_download": (C.c_int, [C.c_void_p]),
     "hb_adjoint_plan_destroy": (None, [C.c_void_p]),
+    "hb_autotune_entries": (C.c_int, [ip, C.c_int32]),
+    "hb_autotune_clear": (None, []),
 }
 
 _lib = None

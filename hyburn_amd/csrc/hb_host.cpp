@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -139,6 +140,29 @@ struct hb_plan {
 
 namespace {
 
+// ---- launch-shape autotuner ------------------------------------------------
+// A shape class is (model, ring channels, nodes, delay-blocked, ring in LDS,
+// integrator class, stochastic). Its tunable is the lanes per node of the
+// delay-blocked kernel (1, 2, 4) or the block size of a network larger than
+// one block (256, 512). Choices are timed on a short sub-plan of sampled
+// instances and cached for the process.
+struct TuneKey {
+    int model, np, n_nodes, blocked, lring, integ, stoch;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(model, np, n_nodes, blocked, lring, integ, stoch) <
+               std::tie(o.model, o.np, o.n_nodes, o.blocked, o.lring, o.integ, o.stoch);
+    }
+};
+struct TuneResult {
+    int choice;        // lanes per node (blocked) or block size (multi)
+    double ms[3];      // measured kernel time of each candidate (0 if not run)
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneResult> g_tuned;
+// Forced choice for the sub-plans the tuner times (thread-local: one tuning
+// at a time per thread).
+thread_local int g_force_choice = 0;
+
 int validate_instance(const hb_instance& I, int idx) {
     char buf[256];
     auto bad = [&](const char* what) {
@@ -201,9 +225,118 @@ int max_delay_of(const hb_instance& I) {
     return I.single_delay;
 }
 
+TuneKey tune_key(const hb_instance& I, int np, int blocked, int lring) {
+    const int ic = (I.integrator == HB_RK4 || I.integrator == HB_RK4_STOCH) ? 2
+                   : (I.integrator == HB_HEUN || I.integrator == HB_HEUN_STOCH) ? 1 : 0;
+    return TuneKey{I.model, np, I.n_nodes, blocked, lring, ic, I.integrator >= HB_EULER_STOCH ? 1 : 0};
+}
+
+}  // namespace
+
+extern "C" int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, hb_plan** out);
+extern "C" int hb_plan_run(hb_plan* p, hb_run_stats* stats);
+extern "C" void hb_plan_destroy(hb_plan* p);
+
+namespace {
+
+// Time each candidate of every untuned shape class on a sub-plan of up to 32
+// sampled instances truncated to 512 steps (outputs into scratch), keep the
+// fastest. Two runs per candidate; the second is timed.
+int autotune_plan(const hb_instance* inst, int n, const hb_run_opts* opts, int lds_budget) {
+    struct Cls { std::vector<int> members; std::vector<int> cand; };
+    std::map<TuneKey, Cls> todo;
+    for (int i = 0; i < n; ++i) {
+        const hb_instance& I = inst[i];
+        const int np = ring_channels(I.model, I.cfn, I.params, I.node_param_idx, -1);
+        const bool per_edge = I.wkind == HB_W_CSR && I.delay_mode == HB_DELAY_PER_EDGE && I.edge_delays;
+        TuneKey key;
+        std::vector<int> cand;
+        if (I.n_nodes > 512) {
+            key = tune_key(I, np, 0, 0);
+            cand = {256, 512};
+        } else if (per_edge && !(opts->flags & HB_FLAG_EXACT) && !(opts->flags & HB_FLAG_NO_BLOCKING)) {
+            int dmin = 1 << 30;
+            for (int e = 0; e < I.nnz; ++e) dmin = std::min(dmin, (int)I.edge_delays[e]);
+            if (dmin < 8) continue;
+            const int64_t ring_k = (int64_t)(max_delay_of(I) + 16) * I.n_nodes * np * 4;
+            key = tune_key(I, np, 1, ring_k <= lds_budget);
+            for (int l : {1, 2, 4})
+                if ((int)align_up((size_t)I.n_nodes * l, 64) <= 512) cand.push_back(l);
+        } else {
+            continue;
+        }
+        if (cand.size() < 2) continue;
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            if (g_tuned.count(key)) continue;
+        }
+        Cls& c = todo[key];
+        c.cand = cand;
+        if (c.members.size() < 32) c.members.push_back(i);
+    }
+    for (auto& kv : todo) {
+        const Cls& c = kv.second;
+        std::vector<hb_instance> sub;
+        std::vector<std::vector<float>> scratch;
+        for (int i : c.members) {
+            hb_instance J = inst[i];
+            J.n_steps = std::min(J.n_steps, 512);
+            J.traj = nullptr;
+            J.bold = nullptr;
+            J.bold_period = 0;
+            J.bold_n = nullptr;
+            scratch.emplace_back((size_t)J.n_nodes * model_nvar(J.model) * 2);
+            J.tavg = scratch.back().data();
+            J.final_state = scratch.back().data() + (size_t)J.n_nodes * model_nvar(J.model);
+            sub.push_back(J);
+        }
+        hb_run_opts o = *opts;
+        o.flags = (opts->flags & ~(uint32_t)HB_FLAG_AUTOTUNE) | HB_FLAG_NO_GRAPH;
+        o.block_size = 0;
+        TuneResult best{0, {0.0, 0.0, 0.0}};
+        double best_ms = 1e300;
+        for (size_t k = 0; k < c.cand.size() && k < 3; ++k) {
+            g_force_choice = c.cand[k];
+            hb_plan* Q = nullptr;
+            int rc = hb_plan_create(sub.data(), (int)sub.size(), &o, &Q);
+            hb_run_stats st;
+            memset(&st, 0, sizeof(st));
+            if (!rc) rc = hb_plan_run(Q, &st);
+            if (!rc) rc = hb_plan_run(Q, &st);
+            hb_plan_destroy(Q);
+            g_force_choice = 0;
+            if (rc) return rc;
+            best.ms[k] = st.kernel_ms;
+            if (st.kernel_ms < best_ms) { best_ms = st.kernel_ms; best.choice = c.cand[k]; }
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[kv.first] = best;
+    }
+    return HB_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int hb_autotune_entries(int32_t* out, int32_t max_entries) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    int k = 0;
+    for (const auto& kv : g_tuned) {
+        if (out && k < max_entries) {
+            int32_t* r = out + (size_t)k * 8;
+            r[0] = kv.first.model; r[1] = kv.first.np; r[2] = kv.first.n_nodes; r[3] = kv.first.blocked;
+            r[4] = kv.first.lring; r[5] = kv.first.integ; r[6] = kv.first.stoch; r[7] = kv.second.choice;
+        }
+        ++k;
+    }
+    return k;
+}
+
+void hb_autotune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+}
 
 const char* hb_last_error(void) { return g_err.c_str(); }
 const char* hb_version(void) { return "hyburn-hip 0.1 (gfx950)"; }
@@ -274,6 +407,11 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
     P->outs.resize(n);
     const int req_block = opts ? opts->block_size : 0;
     const bool exact = (P->flags & HB_FLAG_EXACT) != 0;
+    const bool autotune = (P->flags & HB_FLAG_AUTOTUNE) != 0 && g_force_choice == 0;
+    if (autotune) {
+        const int rc = autotune_plan(inst, n, opts, lds_budget);
+        if (rc) { hb_plan_destroy(P); return rc; }
+    }
 
     // ---- per-instance shapes, LDS-vs-HBM ring, block size
     struct Shape {
@@ -309,13 +447,35 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
                 if (ring_k <= lds_budget || K == 8) {
                     s.blocked = K;
                     s.depth = dmax + K + 1;
-                    // lanes per node (FMA mode only: exact mode keeps the row's sequential sum)
-                    if (!exact && req_block <= 0)
+                    // lanes per node (FMA mode only: exact mode keeps the row's sequential sum):
+                    // forced by the tuner's trials, tuned, or the widest that fits 512 threads
+                    if (!exact && req_block <= 0) {
+                        int want = 0;
+                        if (g_force_choice > 0) want = g_force_choice;
+                        else if (autotune) {
+                            std::lock_guard<std::mutex> lk(g_tune_mu);
+                            const auto it = g_tuned.find(tune_key(I, s.np, 1, ring_k <= lds_budget));
+                            if (it != g_tuned.end()) want = it->second.choice;
+                        }
                         for (int l : {4, 2})
-                            if ((int)align_up((size_t)I.n_nodes * l, 64) <= 512) { s.lpn = l; break; }
+                            if ((want == 0 || l <= want) && (int)align_up((size_t)I.n_nodes * l, 64) <= 512) { s.lpn = l; break; }
+                    }
                     s.block = std::max(64, (int)align_up((size_t)I.n_nodes * s.lpn, 64));
                     break;
                 }
+            }
+        }
+        if (s.multi && req_block <= 0) {
+            int want = 0;
+            if (g_force_choice > 0) want = g_force_choice;
+            else if (autotune) {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                const auto it = g_tuned.find(tune_key(I, s.np, 0, 0));
+                if (it != g_tuned.end()) want = it->second.choice;
+            }
+            if (want == 256 || want == 512) {
+                s.block = want;
+                s.multi = I.n_nodes > s.block;
             }
         }
         // ring floats: depth slots per node and channel (+ K-1 mirrored slots when blocked)

@@ -188,10 +188,10 @@ class _Marshal:
         return self.outputs
 
 
-def _opts(device: int, exact: bool, graph: bool, block_size: int, grid: int) -> L.HbRunOpts:
+def _opts(device: int, exact: bool, graph: bool, block_size: int, grid: int, autotune: bool = False) -> L.HbRunOpts:
     o = L.HbRunOpts()
     o.device = device
-    o.flags = (L.FLAG_EXACT if exact else 0) | (0 if graph else L.FLAG_NO_GRAPH)
+    o.flags = (L.FLAG_EXACT if exact else 0) | (0 if graph else L.FLAG_NO_GRAPH) | (L.FLAG_AUTOTUNE if autotune else 0)
     o.block_size = block_size
     o.grid_waves = grid
     return o
@@ -202,10 +202,10 @@ class Plan:
     in HBM until download()."""
 
     def __init__(self, insts: Sequence[Instance], device: int = 0, exact: bool = False,
-                 graph: bool = True, block_size: int = 0, grid: int = 0):
+                 graph: bool = True, block_size: int = 0, grid: int = 0, autotune: bool = False):
         self._m = _Marshal(insts)
         self._h = C.c_void_p()
-        o = _opts(device, exact, graph, block_size, grid)
+        o = _opts(device, exact, graph, block_size, grid, autotune)
         L.check(L.lib().hb_plan_create(self._m.arr, len(insts), C.byref(o), C.byref(self._h)))
         self.node_steps = float(sum(I.n_nodes * I.n_steps for I in insts))
 
@@ -237,13 +237,22 @@ class Plan:
 
 
 def run_instances(insts: Sequence[Instance], device: int = 0, exact: bool = False,
-                  graph: bool = True, block_size: int = 0, grid: int = 0):
+                  graph: bool = True, block_size: int = 0, grid: int = 0, autotune: bool = False):
     """Run a ragged batch once; returns (outputs, stats)."""
     m = _Marshal(insts)
     st = L.HbRunStats()
-    o = _opts(device, exact, graph, block_size, grid)
+    o = _opts(device, exact, graph, block_size, grid, autotune)
     L.check(L.lib().hb_run_instances(m.arr, len(insts), C.byref(o), C.byref(st)))
     return m.finish(), st
+
+
+def autotune_entries() -> list:
+    """Cached launch-shape choices: dicts of the shape class and its choice."""
+    n = L.lib().hb_autotune_entries(None, 0)
+    buf = np.zeros((max(n, 1), 8), np.int32)
+    L.lib().hb_autotune_entries(buf.ctypes.data_as(L.ip), n)
+    keys = ("model", "np", "n_nodes", "blocked", "lring", "integ_class", "stoch", "choice")
+    return [dict(zip(keys, map(int, r))) for r in buf[:n]]
 
 
 def philox_normal(seed: int, stream: int, n: int, device: int = 0) -> np.ndarray:

@@ -49,6 +49,7 @@ enum { HB_ENGINE_BATCH = 0, HB_ENGINE_PLAIN = 1 };
 #define HB_FLAG_EXACT 0x2u        /* no FMA contraction: same f32 rounding as the written expressions */
 #define HB_FLAG_NO_GRAPH 0x4u     /* launch directly instead of replaying a captured hipGraph */
 #define HB_FLAG_NO_BLOCKING 0x8u  /* disable the delay-blocked schedule (one barrier per step) */
+#define HB_FLAG_AUTOTUNE 0x10u    /* time candidate launch shapes per shape class before planning */
 
 /* ---------------- config-level API (BatchHybridEngine mirror) ---------------- */
 
@@ -253,6 +254,18 @@ int hb_adjoint_backward(hb_adjoint_plan* p, hb_adjoint_stats* stats);
 /* Copy losses and gradients to the spec's host pointers. */
 int hb_adjoint_download(hb_adjoint_plan* p);
 void hb_adjoint_plan_destroy(hb_adjoint_plan* p);
+
+/* Launch-shape autotuner (HB_FLAG_AUTOTUNE on hb_plan_create). A shape class is
+ * (model, ring channels, nodes, delay-blocked, ring in LDS, integrator class,
+ * stochastic); its choice is the lanes per node of the delay-blocked kernel
+ * (1, 2, 4) or the block size of a network larger than one block (256, 512),
+ * timed on a short sub-plan of up to 32 sampled instances and cached for the
+ * process. Instances per workgroup stay 1: every class fills at least one wave.
+ * hb_autotune_entries writes up to max_entries rows of 8 int32 (model, np,
+ * n_nodes, blocked, lring, integrator class, stochastic, choice) and returns
+ * the number of cached classes. No reference counterpart. */
+int hb_autotune_entries(int32_t* out, int32_t max_entries);
+void hb_autotune_clear(void);
 
 /* ---------------- misc ---------------- */
 const char* hb_last_error(void);   /* thread-local message of the last failure */

@@ -329,3 +329,23 @@ def test_blocked_lanes_per_node_fast_mode(gpu, orc):
     for I, x, y in zip(insts, a, b):
         np.testing.assert_array_equal(x.traj, y.traj)
         _err_bound(x.traj, orc.run(I)["traj"], orc.run(I, dtype=np.float64)["traj"])
+
+
+def test_autotuned_plan_matches_default(gpu, orc):
+    """The launch-shape autotuner only changes lanes per node / block size:
+    results stay within the FP32 bound of the default shapes, and every tuned
+    class records one of its candidates."""
+    from hyburn_amd.instances import autotune_entries
+    from hyburn_amd.synth import config3
+    L.lib().hb_autotune_clear()
+    insts = config3(n_inst=24, t_scale=0.02, bold=False)
+    for I in insts:
+        I.record_traj = False
+    a, _ = run_instances(insts, autotune=False)
+    b, _ = run_instances(insts, autotune=True)
+    ent = autotune_entries()
+    assert len(ent) >= 1
+    for e in ent:
+        assert e["choice"] in ((1, 2, 4) if e["blocked"] else (256, 512)), e
+    for I, x, y in zip(insts, a, b):
+        np.testing.assert_allclose(y.final_state, x.final_state, rtol=1e-4, atol=1e-5)
