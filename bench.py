@@ -113,6 +113,15 @@ def ragged_leg(args):
     res["plain_launch_value"] = sub_ns / sub_t
     res["plain_launch_sample"] = len(idx)
     res["speedup_vs_plain_launches"] = res["value"] / res["plain_launch_value"]
+    if not args.no_cpu:
+        # bounded CPU sample: every 64th instance at full length, all host threads
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        sample = insts[:: max(1, len(insts) // 64)]
+        v, secs = cpu_baseline(sample, threads)
+        res["cpu_baseline"] = {"value": v, "unit": "node-steps/s", "cores": threads, "kind": "port",
+                               "sample": f"{len(sample)} of the {len(insts)} instances (every "
+                                         f"{max(1, len(insts) // 64)}th, full length), {secs:.1f} s wall"}
+        res["speedup_vs_cpu"] = res["value"] / v
     return res
 
 
@@ -169,7 +178,8 @@ def main() -> int:
     ap.add_argument("--n-steps", type=int, default=10_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--ragged", action="store_true", help="also time the 4096-instance ragged sweep")
+    ap.add_argument("--no-ragged", dest="ragged", action="store_false",
+                    help="skip the 4096-instance ragged sweep leg")
     ap.add_argument("--ragged-n", type=int, default=4096)
     ap.add_argument("--ragged-tscale", type=float, default=1.0)
     ap.add_argument("--ragged-plain-sample", type=int, default=16)

@@ -593,13 +593,147 @@ __global__ __launch_bounds__(256) void k_bold_forward(const AdjInst* __restrict_
 // h_i Z_ti) / (sqrt(n) s_i), Gs = G + G^T, G_ij = 2 (r_ij - target_ij) (0 where
 // clamped), h_i = sum_j Gs_ij r_ij.
 // BOLD MSE: loss = mean (BOLD - target)^2.
+// LDS-tiled FC path for N <= kFcTile: samples are staged in chunks of kFcChunk
+// rows ([chunk][N] doubles, coalesced), G + G^T lives in LDS.
+constexpr int kFcTile = 96, kFcChunk = 32, kFcPairs = 19;   // pairs per thread: ceil(96*97/2 / 256)
+constexpr int kFcLds = (kFcTile * kFcTile + kFcChunk * kFcTile + 3 * kFcTile) * 8;
+
+__device__ void fc_loss_tiled(const AdjInst& I, double* sh, double* red) {
+    const int N = I.n_nodes, nw = I.n_win;
+    const int tid = threadIdx.x, BS = blockDim.x;
+    const double* X = I.bold;
+    double* gsm = sh;                                  // [N][N] G + G^T
+    double* chunk = sh + kFcTile * kFcTile;            // [kFcChunk][N]
+    double* mean = chunk + kFcChunk * kFcTile;         // [N]
+    double* sd = mean + kFcTile;                       // [N]
+    double* hh = sd + kFcTile;                         // [N]
+    // ---- means and population standard deviations (two passes over chunks)
+    double acc = 0.0;
+    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
+        const int kc = min(kFcChunk, nw - k0);
+        __syncthreads();
+        for (int q = tid; q < kc * N; q += BS) chunk[q] = X[(size_t)k0 * N + q];
+        __syncthreads();
+        if (tid < N)
+            for (int k = 0; k < kc; ++k) acc += chunk[k * N + tid];
+    }
+    if (tid < N) mean[tid] = acc / (double)nw;
+    acc = 0.0;
+    __syncthreads();
+    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
+        const int kc = min(kFcChunk, nw - k0);
+        __syncthreads();
+        for (int q = tid; q < kc * N; q += BS) chunk[q] = X[(size_t)k0 * N + q] - mean[q % N];
+        __syncthreads();
+        if (tid < N)
+            for (int k = 0; k < kc; ++k) acc += chunk[k * N + tid] * chunk[k * N + tid];
+    }
+    if (tid < N) {
+        double sdev = sqrt(acc / (double)nw);
+        sd[tid] = sdev < 1e-12 ? 1.0 : sdev;
+    }
+    __syncthreads();
+    // ---- covariances of this thread's pairs (j <= i), accumulated over chunks
+    const int npairs = N * (N + 1) / 2;
+    int pij[kFcPairs];          // pair (i, j), j <= i, packed as i << 16 | j
+    double cv[kFcPairs];
+#pragma unroll
+    for (int q = 0; q < kFcPairs; ++q) {
+        const int p = tid + q * BS;
+        int i = 0, j = 0;
+        if (p < npairs) {
+            i = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+            while (i * (i + 1) / 2 > p) --i;
+            while ((i + 1) * (i + 2) / 2 <= p) ++i;
+            j = p - i * (i + 1) / 2;
+        }
+        pij[q] = (i << 16) | j;
+        cv[q] = 0.0;
+    }
+    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
+        const int kc = min(kFcChunk, nw - k0);
+        __syncthreads();
+        for (int q = tid; q < kc * N; q += BS) chunk[q] = X[(size_t)k0 * N + q] - mean[q % N];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kFcPairs; ++q) {
+            if (tid + q * BS < npairs) {
+                const int i = pij[q] >> 16, j = pij[q] & 0xFFFF;
+                double a = cv[q];
+                for (int k = 0; k < kc; ++k) a += chunk[k * N + i] * chunk[k * N + j];
+                cv[q] = a;
+            }
+        }
+    }
+    double lacc = 0.0;
+#pragma unroll
+    for (int q = 0; q < kFcPairs; ++q) {
+        if (tid + q * BS < npairs) {
+            const int i = pij[q] >> 16, j = pij[q] & 0xFFFF;
+            const double raw = (cv[q] / (double)nw) / (sd[i] * sd[j]);
+            const bool clamped = raw < -1.0 || raw > 1.0;
+            const double r = clamped ? (raw < -1.0 ? -1.0 : 1.0) : raw;
+            const double rf = (double)(float)r;
+            const double tij = I.target[(size_t)i * N + j], tji = I.target[(size_t)j * N + i];
+            const double g = clamped ? 0.0 : 2.0 * (rf - tij) + 2.0 * (rf - tji);
+            gsm[i * N + j] = g;
+            gsm[j * N + i] = g;
+            cv[q] = r;   // keep r for h
+            lacc += (rf - tij) * (rf - tij);
+            if (i != j) lacc += (rf - tji) * (rf - tji);
+        }
+    }
+    const double tot = block_sum_d(lacc, red);
+    if (tid == 0) I.out_loss[0] = (float)tot;
+    __syncthreads();
+    // ---- h_i = sum_j Gs_ij r_ij, with r staged through the instance's scratch
+    double* rr = I.fc_work + 2 * (size_t)N;
+#pragma unroll
+    for (int q = 0; q < kFcPairs; ++q) {
+        if (tid + q * BS < npairs) {
+            const int i = pij[q] >> 16, j = pij[q] & 0xFFFF;
+            rr[(size_t)i * N + j] = cv[q];
+            rr[(size_t)j * N + i] = cv[q];
+        }
+    }
+    __syncthreads();
+    if (tid < N) {
+        double a = 0.0;
+        for (int j = 0; j < N; ++j) a += gsm[tid * N + j] * rr[(size_t)tid * N + j];
+        hh[tid] = a;
+    }
+    __syncthreads();
+    // ---- dL/dX = (Z Gs - h .* Z) / (sqrt(n) s), Z = centred / (sqrt(n) s), per chunk
+    const double sqn = sqrt((double)nw);
+    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
+        const int kc = min(kFcChunk, nw - k0);
+        __syncthreads();
+        for (int q = tid; q < kc * N; q += BS) {
+            const int j = q % N;
+            chunk[q] = (X[(size_t)k0 * N + q] - mean[j]) / (sqn * sd[j]);
+        }
+        __syncthreads();
+        for (int q = tid; q < kc * N; q += BS) {
+            const int k = q / N, i = q % N;
+            double a = 0.0;
+            for (int j = 0; j < N; ++j) a += gsm[i * N + j] * chunk[k * N + j];
+            I.dsig[(size_t)k0 * N + q] = (a - hh[i] * chunk[q]) / (sqn * sd[i]);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_loss(const AdjInst* __restrict__ insts) {
+    extern __shared__ double shd[];
     __shared__ double red[8];
     const AdjInst& I = insts[blockIdx.x];
     if (I.bold_period <= 0 || I.loss_kind == HB_LOSS_ORDER) return;
     const int N = I.n_nodes, nw = I.n_win;
     const int tid = threadIdx.x, BS = blockDim.x;
     const double* X = I.bold;
+    if (I.loss_kind == HB_LOSS_FC && N <= kFcTile) {
+        fc_loss_tiled(I, shd, red);
+        return;
+    }
     if (I.loss_kind == HB_LOSS_BOLD_MSE) {
         const double scale = 1.0 / ((double)nw * (double)N);
         double acc = 0.0;
@@ -1236,7 +1370,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
 #pragma unroll
     for (int v = 0; v < NV; ++v) lam[v] = 0.0f;
     double gpar = 0.0, gcc = 0.0;
-    constexpr int kReg = 32;
+    constexpr int kReg = 16;   // register-resident out-edges (the rest read from memory)
     int en[kReg], ed[kReg];
     float ew[kReg];
     int e0 = 0, e1 = 0;
@@ -1299,28 +1433,53 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
 
     for (int seg = V.nseg - 1; seg >= 0; --seg) {
         const int t0s = seg * Sg, t1s = min(T, t0s + Sg);
-        // ---- recompute states t0s..t1s from the checkpoint and the coupling log
+        // ---- recompute states t0s..t1s from the checkpoint and the coupling log;
+        // the log and noise rows of the next kK steps are loaded while the
+        // current kK steps integrate (double buffer)
         if (own) {
             float x[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) { x[v] = V.ckpt[((size_t)seg * N + tid) * NV + v]; V.tape[(size_t)tid * NV + v] = x[v]; }
-            for (int t = t0s; t < t1s; ++t) {
-                float g[NP], own_pre[NP], cpl[NC], z[NV];
-                const gp<float> gl = V.glog + ((size_t)t * N + tid) * NP;
+            float gq0[kK][NP], gq1[kK][NP];
+            auto fetch = [&](float (&gq)[kK][NP], int tg) {
 #pragma unroll
-                for (int c = 0; c < NP; ++c) g[c] = gl[c];
+                for (int m = 0; m < kK; ++m) {
+                    const int t = tg + m;
+                    if (t < t1s) {
+                        const gp<float> gl = V.glog + ((size_t)t * N + tid) * NP;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) z[v] = S.stoch ? V.noise[((size_t)t * N + tid) * NV + v] * nsc[v] : 0.0f;
-                pre<NC, NP>(V.cfn, cc, false, false, x, own_pre);
-                if (V.has_c) post<NC, NP>(V.cfn, cc, false, g, own_pre, cpl);
-                else {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+                        for (int c = 0; c < NP; ++c) gq[m][c] = gl[c];
+                    }
                 }
-                integrate<M>(S, mc, cpl, z, x);
-                gp<float> tp = V.tape + ((size_t)(t - t0s + 1) * N + tid) * NV;
+            };
+            auto compute = [&](const float (&gq)[kK][NP], int tg) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) tp[v] = x[v];
+                for (int m = 0; m < kK; ++m) {
+                    const int t = tg + m;
+                    if (t < t1s) {
+                        float own_pre[NP], cpl[NC], z[NV];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) z[v] = S.stoch ? V.noise[((size_t)t * N + tid) * NV + v] * nsc[v] : 0.0f;
+                        pre<NC, NP>(V.cfn, cc, false, false, x, own_pre);
+                        if (V.has_c) post<NC, NP>(V.cfn, cc, false, gq[m], own_pre, cpl);
+                        else {
+#pragma unroll
+                            for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+                        }
+                        integrate<M>(S, mc, cpl, z, x);
+                        gp<float> tp = V.tape + ((size_t)(t - t0s + 1) * N + tid) * NV;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) tp[v] = x[v];
+                    }
+                }
+            };
+            fetch(gq0, t0s);
+            for (int tg = t0s; tg < t1s; tg += 2 * kK) {
+                if (tg + kK < t1s) fetch(gq1, tg + kK);
+                compute(gq0, tg);
+                if (tg + kK >= t1s) break;
+                if (tg + 2 * kK < t1s) fetch(gq0, tg + 2 * kK);
+                compute(gq1, tg + kK);
             }
         }
         __syncthreads();
@@ -1360,7 +1519,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
         const int tlo = max(t0s, 1);
         for (int ttop = t1s - 1; ttop >= tlo; ttop -= kK) {
             const int kk = min(kK, ttop - tlo + 1), tbot = ttop - kk + 1;
-            float xb[kK][NV], gb[kK][NP], zb[kK][NV];
+            float xb[kK][NV], gb[kK][NP];
             if (own) {
 #pragma unroll
                 for (int m = 0; m < kK; ++m) {
@@ -1369,10 +1528,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                         const gp<float> tp = V.tape + ((size_t)(st - t0s) * N + tid) * NV;
                         const gp<float> gl = V.glog + ((size_t)st * N + tid) * NP;
 #pragma unroll
-                        for (int v = 0; v < NV; ++v) {
-                            xb[m][v] = tp[v];
-                            zb[m][v] = S.stoch ? V.noise[((size_t)st * N + tid) * NV + v] * nsc[v] : 0.0f;
-                        }
+                        for (int v = 0; v < NV; ++v) xb[m][v] = tp[v];
 #pragma unroll
                         for (int c = 0; c < NP; ++c) gb[m][c] = gl[c];
                     }
@@ -1417,8 +1573,10 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                 for (int m = kK - 1; m >= 0; --m) {
                     if (m < kk) {
                         const int st = tbot + m;
-                        float lx[NV];
-                        vjp(st, xb[m], gb[m], zb[m], lx, lgb[m]);
+                        float lx[NV], zs[NV];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) zs[v] = S.stoch ? V.noise[((size_t)st * N + tid) * NV + v] * nsc[v] : 0.0f;
+                        vjp(st, xb[m], gb[m], zs, lx, lgb[m]);
                         pre_vjp<NC, NP>(xb[m], acc[m], lx);
                         seed(st, t0s, xb[m], lx);
 #pragma unroll
@@ -1522,6 +1680,10 @@ static const void* kernel_for(int model, bool rev, int np, int npt) {
 }  // namespace adj
 
 hipError_t adjoint_set_attributes(int max_lds) {
+    {
+        hipError_t e = hipFuncSetAttribute((const void*)adj::k_loss, hipFuncAttributeMaxDynamicSharedMemorySize, adj::kFcLds);
+        if (e != hipSuccess) return e;
+    }
     for (int m = 0; m <= HB_RWW; ++m)
         for (int rev = 0; rev < 2; ++rev)
             for (int np = 1; np <= 2; ++np)
@@ -1544,7 +1706,7 @@ hipError_t launch_adjoint_stage(int stage, const AdjLaunch& L, hipStream_t s) {
         return hipLaunchKernel(k, dim3(L.n), dim3(L.block), args, L.lds_fwd, s);
     }
     case 1: hipLaunchKernelGGL(adj::k_bold_forward, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
-    case 2: hipLaunchKernelGGL(adj::k_loss, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
+    case 2: hipLaunchKernelGGL(adj::k_loss, dim3(L.n), dim3(256), adj::kFcLds, s, d); return hipGetLastError();
     case 3: hipLaunchKernelGGL(adj::k_bold_reverse, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
     default: {
         const void* k = L.blocked ? adj::kernel_b(L.model, true, L.np, L.lring) : adj::kernel_for(L.model, true, L.np, L.npt);
