@@ -1,0 +1,103 @@
+"""`python -m hyburn_amd run`: the `hyburn run` command on the HIP backend.
+
+Mirrors the reference command line (src/cli.rs:16-60, 785-842, 894-974, 1040-1103):
+
+  run -c sim.toml -o output [-b hip] [--sweep sweep.toml] [--noise-seed S]
+
+* without --sweep: one plain-engine run (HybridEngine semantics), writing
+  state_final_sub0.npy [nvar, nnodes, nmodes] and state_traj.npy
+  [n_steps, nvar, nnodes, nmodes], var-major as the reference stores them;
+* with --sweep: a batched sweep (BatchHybridEngine semantics) over the sweep
+  file's parameter values, writing tavg_sub0.npy with the reference's header
+  [n_sweep, nvar, nnodes] over its node-major [n_sweep, nnodes, nvar] data.
+
+n_steps = (sim_length / dt) truncated in f64, as the reference computes it.
+Only the hip backend is provided: any other --backend value is an error
+(there is no CPU path in this library).
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+import numpy as np
+
+from . import _lib as L
+from .engine import (BatchHybridEngine, SimConfig, SweepConfig, SweepParam, parse_sweep_param,
+                     write_npy_f32)
+
+
+def _backend(name: str) -> None:
+    got = L.lib().hb_select_backend(name.encode())
+    if got is None:
+        raise SystemExit(f"backend '{name}' is not provided by this build (hip only): "
+                         + L.lib().hb_last_error().decode(errors="replace"))
+
+
+def run(config: str, output: str, backend: str = "hip", sweep: str | None = None, noise_seed: int = 0,
+        device: int = 0) -> dict:
+    """Run the command; returns the written file paths."""
+    _backend(backend)
+    cfg = SimConfig.from_file(config)
+    cfg.validate()
+    n_steps = cfg.n_steps()
+    os.makedirs(output, exist_ok=True)
+    written = {}
+    if sweep:
+        sc = SweepConfig.from_file(sweep)
+        values = sc.generate_values()
+        if not values:
+            raise SystemExit("Sweep config has no values")
+        sub_idx, param_idx = parse_sweep_param(sc.parameter_name, cfg)
+        eng = BatchHybridEngine.from_config(cfg, len(values), device, noise_seed=noise_seed)
+        res = eng.run_sweep(SweepParam(sub_idx, param_idx), values, n_steps)
+        eng.close()
+        nnodes, nvar = eng.nnodes, eng.nvar
+        path = os.path.join(output, "tavg_sub0.npy")
+        # cli.rs:958-970 labels the [n_sweep, nnodes, nvar] data as [n_sweep, nvar, nnodes]
+        write_npy_f32(path, res.tavg[0], [res.n_sweep, nvar, nnodes])
+        written["tavg_sub0"] = path
+    else:
+        # HybridEngine reads an inline initial state var-major [nvar][nnodes]
+        # (construction.rs:557-565); the batch engine node-major (engine.rs:161-170)
+        s0 = cfg.subnetworks[0]
+        if isinstance(s0.initial_state, list):
+            nv0 = L.MODEL_INFO[L.MODEL_NAMES[s0.model]][0]
+            s0.initial_state = np.asarray(s0.initial_state, np.float32).reshape(nv0, s0.nnodes).T.ravel().tolist()
+        eng = BatchHybridEngine.from_config(cfg, 1, device, engine=L.ENGINE_PLAIN, noise_seed=noise_seed)
+        # a plain run: no parameter is swept (an index the dfun ignores)
+        res = eng.run_sweep_with_trajectory(SweepParam(0, -1), [0.0], n_steps)
+        eng.close()
+        nnodes, nvar = eng.nnodes, eng.nvar
+        fin = np.asarray(res.final_states[0], np.float32).reshape(nnodes, nvar)
+        traj = np.asarray(res.trajectories[0], np.float32).reshape(n_steps, nnodes, nvar)
+        p_fin = os.path.join(output, "state_final_sub0.npy")
+        p_traj = os.path.join(output, "state_traj.npy")
+        # HybridEngine state layout is [nvar, nnodes, nmodes] (runtime.rs, cli.rs:1073-1095)
+        write_npy_f32(p_fin, fin.T.copy(), [nvar, nnodes, 1])
+        write_npy_f32(p_traj, traj.transpose(0, 2, 1).copy(), [n_steps, nvar, nnodes, 1])
+        written["state_final_sub0"] = p_fin
+        written["state_traj"] = p_traj
+    return written
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m hyburn_amd")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    r = sub.add_parser("run", help="run a simulation or a parameter sweep")
+    r.add_argument("-c", "--config", default="sim.toml")
+    r.add_argument("-o", "--output", default="output")
+    r.add_argument("-b", "--backend", default="hip")
+    r.add_argument("--sweep", default=None)
+    r.add_argument("--noise-seed", type=int, default=0)
+    r.add_argument("--device", type=int, default=0)
+    a = ap.parse_args(argv)
+    try:
+        out = run(a.config, a.output, a.backend, a.sweep, a.noise_seed, a.device)
+    except L.HyburnError as e:
+        print(f"error: {e}", file=sys.stderr)
+        return 1
+    for k, v in out.items():
+        print(f"{k}: {v}")
+    return 0

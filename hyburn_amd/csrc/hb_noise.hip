@@ -39,12 +39,14 @@ __global__ __launch_bounds__(256) void k_philox_normal(const NoiseJob* __restric
         const double u1 = (double)(a + 1) * 0x1.0p-53;   // (0, 1]
         const double u2 = (double)b * 0x1.0p-53;         // [0, 1)
         const double r = sqrt(-2.0 * log(u1));
-        const double th = 6.283185307179586 * u2;
-        const float z0 = (float)(r * cos(th));
-        const float z1 = (float)(r * sin(th));
+        // (cos, sin)(2 pi u2) with an exact argument reduction
+        double sn, cs;
+        sincospi(2.0 * u2, &sn, &cs);
+        const float z0 = (float)(r * cs);
+        const float z1 = (float)(r * sn);
         const int64_t k = p << 1;
-        J.out[k] = z0;
-        if (k + 1 < J.n) J.out[k + 1] = z1;
+        if (k + 1 < J.n) *reinterpret_cast<float2*>(J.out + k) = make_float2(z0, z1);   // k even: 8-byte aligned
+        else J.out[k] = z0;
     }
 }
 

@@ -1,0 +1,87 @@
+"""`python -m hyburn_amd run`: the hyburn run command on the HIP backend
+(src/cli.rs:785-842, 894-974, 1040-1103)."""
+import os
+
+import numpy as np
+import pytest
+
+from hyburn_amd import _lib as L
+from hyburn_amd.cli import main, run
+from hyburn_amd.instances import Instance
+
+SIM = """
+sim_length = 5.0
+dt = 0.1
+integrator = "heun"
+[network]
+[[network.subnetworks]]
+model = "Generic2dOscillator"
+nnodes = 6
+params = [1, 0, -2, -10, 0, 0.02, 3, 1, 0, 1, 1, 1]
+initial_state = [{ic}]
+[[network.projections]]
+src = 0
+tgt = 0
+weights = 0.01
+coupling_fn = "Linear"
+coupling_params = [0.01]
+"""
+
+SWEEP = """
+parameter_name = "subnetworks[0].params[1]"
+values = [0.0, 0.1, 0.2]
+"""
+
+
+def _files(tmp_path):
+    ic = np.round(np.linspace(-0.3, 0.3, 12), 3)      # var-major [2][6] for the plain run
+    cfg = tmp_path / "sim.toml"
+    cfg.write_text(SIM.format(ic=", ".join(str(v) for v in ic)))
+    sw = tmp_path / "sweep.toml"
+    sw.write_text(SWEEP)
+    return str(cfg), str(sw), ic.astype(np.float32)
+
+
+def test_non_hip_backend_is_rejected(tmp_path):
+    cfg, _, _ = _files(tmp_path)
+    with pytest.raises(SystemExit) as e:
+        run(cfg, str(tmp_path / "out"), backend="ndarray")
+    assert "hip only" in str(e.value)
+
+
+@pytest.mark.gpu
+def test_plain_run_writes_reference_layout(gpu, orc, tmp_path):
+    """state_traj.npy [T, nvar, nnodes, 1] and state_final_sub0.npy [nvar,
+    nnodes, 1], var-major; values equal the plain-engine oracle run (scalar
+    weights as w * I, construction.rs:639-655)."""
+    cfg, _, ic = _files(tmp_path)
+    out = tmp_path / "out"
+    assert main(["run", "-c", cfg, "-o", str(out), "-b", "hip"]) == 0
+    traj = np.load(out / "state_traj.npy")
+    fin = np.load(out / "state_final_sub0.npy")
+    assert traj.shape == (50, 2, 6, 1) and fin.shape == (2, 6, 1)
+    x0 = ic.reshape(2, 6).T.copy()
+    I = Instance(model=L.G2DO, n_nodes=6, n_steps=50, dt=0.1,
+                 params=np.array([1, 0, -2, -10, 0, 0.02, 3, 1, 0, 1, 1, 1], np.float32), x0=x0,
+                 integrator=L.HEUN, engine=L.ENGINE_PLAIN, wkind=L.W_SCALAR_DIAG, scalar_w=0.01,
+                 cfn=L.CF_LINEAR, cparams=(0.01,), record_traj=True)
+    o = orc.run(I)
+    np.testing.assert_allclose(traj[:, :, :, 0], o["traj"].transpose(0, 2, 1), rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(fin[:, :, 0], o["final_state"].T, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_sweep_run_writes_tavg_with_reference_header(gpu, tmp_path):
+    """tavg_sub0.npy carries the header [n_sweep, nvar, nnodes] over the
+    node-major [n_sweep, nnodes, nvar] data, as cli.rs:958-970 writes it."""
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig, SweepParam
+    cfg, sw, _ = _files(tmp_path)
+    out = tmp_path / "sw"
+    assert main(["run", "-c", cfg, "-o", str(out), "--sweep", sw]) == 0
+    raw = open(out / "tavg_sub0.npy", "rb").read()
+    assert b"'shape': (3, 2, 6)" in raw
+    t = np.load(out / "tavg_sub0.npy")
+    eng = BatchHybridEngine.from_config(SimConfig.from_file(cfg), 3)
+    r = eng.run_sweep(SweepParam(0, 1), [0.0, 0.1, 0.2], 50)
+    eng.close()
+    np.testing.assert_array_equal(t.ravel(), r.tavg[0])
