@@ -472,7 +472,30 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
 #pragma unroll
             for (int c = 0; c < NP; ++c) g[c] = 0.0f;
             const int e0 = I.indptr[n], e1 = I.indptr[n + 1];
-            for (int e = e0; e < e1; ++e) {
+            constexpr int kB = 8;   // edges per batch: records, ring values, then FMAs in row order
+            int e = e0;
+            for (; e + kB <= e1; e += kB) {
+                uint2 ed[kB];
+                float val[kB][NP];
+#pragma unroll
+                for (int k = 0; k < kB; ++k) ed[k] = I.edges[e + k];
+#pragma unroll
+                for (int k = 0; k < kB; ++k) {
+                    const int d = t == 0 ? 0 : (int)(ed[k].x >> 16);
+                    int sl = ws - d;
+                    sl += sl < 0 ? Hd : 0;
+                    const float* src = ring + (size_t)sl * NNP + (ed[k].x & 0xFFFF) * NP;
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) val[k][c] = src[c];
+                }
+#pragma unroll
+                for (int k = 0; k < kB; ++k) {
+                    const float w = __uint_as_float(ed[k].y);
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) g[c] += w * val[k][c];
+                }
+            }
+            for (; e < e1; ++e) {
                 const uint2 ed = I.edges[e];
                 const int j = ed.x & 0xFFFF;
                 const int d = t == 0 ? 0 : (int)(ed.x >> 16);
@@ -901,7 +924,8 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
 
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int t0 = seg * Sg, t1 = min(T, t0 + Sg);
-        // ---- recompute states t0..t1 from the checkpoint and the coupling log
+        // ---- recompute states t0..t1 from the checkpoint and the coupling log;
+        // log rows of the next 8 steps load while the current 8 integrate
 #pragma unroll
         for (int q = 0; q < NPT; ++q) {
             const int n = tid + q * BS;
@@ -909,23 +933,46 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
             float x[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) { x[v] = I.ckpt[((size_t)seg * N + n) * NV + v]; I.tape[(size_t)n * NV + v] = x[v]; }
-            for (int t = t0; t < t1; ++t) {
-                float g[NP], own[NP], cpl[NC], z[NV];
-                const float* gl = I.glog + ((size_t)t * N + n) * NP;
+            constexpr int kG = 8;
+            float gq0[kG][NP], gq1[kG][NP];
+            auto fetch = [&](float (&gq)[kG][NP], int tg) {
 #pragma unroll
-                for (int c = 0; c < NP; ++c) g[c] = gl[c];
-                pre<NC, NP>(I.cfn, cc, false, false, x, own);
-                if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, g, own, cpl);
-                else {
+                for (int m = 0; m < kG; ++m) {
+                    if (tg + m < t1) {
+                        const float* gl = I.glog + ((size_t)(tg + m) * N + n) * NP;
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+                        for (int c = 0; c < NP; ++c) gq[m][c] = gl[c];
+                    }
                 }
+            };
+            auto compute = [&](const float (&gq)[kG][NP], int tg) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
-                integrate<M>(S, mcn[q], cpl, z, x);
-                float* tp = I.tape + ((size_t)(t - t0 + 1) * N + n) * NV;
+                for (int m = 0; m < kG; ++m) {
+                    const int t = tg + m;
+                    if (t < t1) {
+                        float own[NP], cpl[NC], z[NV];
+                        pre<NC, NP>(I.cfn, cc, false, false, x, own);
+                        if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, gq[m], own, cpl);
+                        else {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) tp[v] = x[v];
+                            for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+                        }
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
+                        integrate<M>(S, mcn[q], cpl, z, x);
+                        float* tp = I.tape + ((size_t)(t - t0 + 1) * N + n) * NV;
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) tp[v] = x[v];
+                    }
+                }
+            };
+            fetch(gq0, t0);
+            for (int tg = t0; tg < t1; tg += 2 * kG) {
+                if (tg + kG < t1) fetch(gq1, tg + kG);
+                compute(gq0, tg);
+                if (tg + kG >= t1) break;
+                if (tg + 2 * kG < t1) fetch(gq0, tg + 2 * kG);
+                compute(gq1, tg + kG);
             }
         }
         __syncthreads();
@@ -1022,7 +1069,31 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
                 for (int c = 0; c < NP; ++c) acc[c] = 0.0f;
                 if (I.has_coupling) {
                     const int e0 = I.indptr_t[n], e1 = I.indptr_t[n + 1];
-                    for (int e = e0; e < e1; ++e) {
+                    int e = e0;
+                    if (t > 0) {
+                        constexpr int kB = 8;   // batched as in the forward gather
+                        for (; e + kB <= e1; e += kB) {
+                            uint2 ed[kB];
+                            float val[kB][NP];
+#pragma unroll
+                            for (int k = 0; k < kB; ++k) ed[k] = I.edges_t[e + k];
+#pragma unroll
+                            for (int k = 0; k < kB; ++k) {
+                                int sl = slot + (int)(ed[k].x >> 16);
+                                sl -= sl >= Ha ? Ha : 0;
+                                const float* a = aring + (size_t)sl * NNP + (ed[k].x & 0xFFFF) * NP;
+#pragma unroll
+                                for (int c = 0; c < NP; ++c) val[k][c] = a[c];
+                            }
+#pragma unroll
+                            for (int k = 0; k < kB; ++k) {
+                                const float w = __uint_as_float(ed[k].y);
+#pragma unroll
+                                for (int c = 0; c < NP; ++c) acc[c] += w * val[k][c];
+                            }
+                        }
+                    }
+                    for (; e < e1; ++e) {
                         const uint2 ed = I.edges_t[e];
                         const int i = ed.x & 0xFFFF;
                         const int d = (int)(ed.x >> 16);

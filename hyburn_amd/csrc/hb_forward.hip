@@ -361,7 +361,33 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
                     }
                     if (e_mem0 < e_mem1) {
                         if (per_edge) {
-                            for (int e = e_mem0; e < e_mem1; ++e) {
+                            // 8 edges per batch: edge records, then ring values, then
+                            // the FMAs in row order (same sums as one edge at a time)
+                            constexpr int kB = 8;
+                            int e = e_mem0;
+                            for (; e + kB <= e_mem1; e += kB) {
+                                uint2 ed[kB];
+#pragma unroll
+                                for (int k = 0; k < kB; ++k) ed[k] = edge_at(V.edges, e + k);
+                                float val[kB][NP];
+#pragma unroll
+                                for (int k = 0; k < kB; ++k) {
+                                    const int j = ed[k].x & 0xFFFF;
+                                    const int d = (ed[k].x >> 16) & dmask;
+                                    int sl = ws - d;
+                                    sl += sl < 0 ? Hd : 0;
+                                    const auto src = ring + (size_t)sl * NNP + j * NP;
+#pragma unroll
+                                    for (int c = 0; c < NP; ++c) val[k][c] = src[c];
+                                }
+#pragma unroll
+                                for (int k = 0; k < kB; ++k) {
+                                    const float w = __uint_as_float(ed[k].y);
+#pragma unroll
+                                    for (int c = 0; c < NP; ++c) g[c] += w * val[k][c];
+                                }
+                            }
+                            for (; e < e_mem1; ++e) {
                                 const uint2 ed = edge_at(V.edges, e);
                                 const int j = ed.x & 0xFFFF;
                                 const int d = (ed.x >> 16) & dmask;
