@@ -127,6 +127,10 @@ struct hb_plan {
     std::vector<LaunchDesc> launches;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // launch groups run as parallel branches: one side stream and join event each
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> join;
+    hipEvent_t fork = nullptr;
     hipGraphExec_t g_noise = nullptr, g_fwd = nullptr;
     hipGraph_t gr_noise = nullptr, gr_fwd = nullptr;
     struct Out {
@@ -238,6 +242,30 @@ extern "C" int hb_plan_run(hb_plan* p, hb_run_stats* stats);
 extern "C" void hb_plan_destroy(hb_plan* p);
 
 namespace {
+
+// Launch every group: one group on the plan stream, several as parallel
+// branches (fork from the plan stream, one side stream each, join back), so
+// groups of different launch shapes share the chip instead of queueing.
+hipError_t launch_groups(hb_plan* P, bool exact) {
+    auto go = [&](const LaunchDesc& L, hipStream_t st) {
+        return exact ? launch_forward_exact(L, st) : launch_forward_fast(L, st);
+    };
+    if (P->launches.size() <= 1 || P->side.empty()) {
+        for (auto& L : P->launches) {
+            const hipError_t e = go(L, P->stream);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    hipError_t e = hipEventRecord(P->fork, P->stream);
+    for (size_t k = 0; k < P->launches.size() && e == hipSuccess; ++k) {
+        e = hipStreamWaitEvent(P->side[k], P->fork, 0);
+        if (e == hipSuccess) e = go(P->launches[k], P->side[k]);
+        if (e == hipSuccess) e = hipEventRecord(P->join[k], P->side[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(P->stream, P->join[k], 0);
+    }
+    return e;
+}
 
 // Time each candidate of every untuned shape class on a sub-plan of up to 32
 // sampled instances truncated to 512 steps (outputs into scratch), keep the
@@ -367,6 +395,11 @@ void hb_plan_destroy(hb_plan* p) {
     if (p->gr_fwd) (void)hipGraphDestroy(p->gr_fwd);
     for (auto& e : p->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : p->join)
+        if (e) (void)hipEventDestroy(e);
+    if (p->fork) (void)hipEventDestroy(p->fork);
+    for (auto& st : p->side)
+        if (st) (void)hipStreamDestroy(st);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     if (p->arena) (void)hipFree(p->arena);
     delete p;
@@ -779,6 +812,15 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
 
     HB_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     for (auto& e : P->ev) HB_HIP(hipEventCreate(&e));
+    if (P->launches.size() > 1) {
+        HB_HIP(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
+        P->side.assign(P->launches.size(), nullptr);
+        P->join.assign(P->launches.size(), nullptr);
+        for (size_t k = 0; k < P->launches.size(); ++k) {
+            HB_HIP(hipStreamCreateWithFlags(&P->side[k], hipStreamNonBlocking));   // default priority
+            HB_HIP(hipEventCreateWithFlags(&P->join[k], hipEventDisableTiming));
+        }
+    }
 
     // ---- capture the launch sequence
     if (!(P->flags & HB_FLAG_NO_GRAPH)) {
@@ -790,10 +832,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         HB_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
         hipError_t el = hipSuccess;
         if (gi > 0) el = launch_zero_i32(P->d_counters, std::min(gi, 64), P->stream);
-        for (auto& L : P->launches) {
-            if (el != hipSuccess) break;
-            el = exact ? launch_forward_exact(L, P->stream) : launch_forward_fast(L, P->stream);
-        }
+        if (el == hipSuccess) el = launch_groups(P, exact);
         e2 = hipStreamEndCapture(P->stream, &P->gr_fwd);
         if (el != hipSuccess || e2 != hipSuccess) {
             hb_plan_destroy(P);
@@ -816,8 +855,7 @@ int hb_plan_run(hb_plan* P, hb_run_stats* stats) {
     if (P->g_fwd) HB_HIP(hipGraphLaunch(P->g_fwd, P->stream));
     else {
         HB_HIP(launch_zero_i32(P->d_counters, std::min((int)P->launches.size(), 64), P->stream));
-        for (auto& L : P->launches)
-            HB_HIP(exact ? launch_forward_exact(L, P->stream) : launch_forward_fast(L, P->stream));
+        HB_HIP(launch_groups(P, exact));
     }
     HB_HIP(hipEventRecord(P->ev[2], P->stream));
     HB_HIP(hipEventSynchronize(P->ev[2]));
