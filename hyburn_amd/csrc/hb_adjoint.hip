@@ -574,14 +574,19 @@ struct BoldD { double s, f, v, q; };
 constexpr double kBwKappa = 1.0 / (double)1.54f, kBwGamma = 1.0 / (double)0.50f, kBwTauInv = 1.0 / (double)1.02f;
 constexpr double kBwIalpha = 1.0 / (double)0.32f, kBwE0 = (double)0.34f, kBwLo = (double)1e-6f;
 
-__device__ __forceinline__ double bold_step_d(BoldD& b, double z_raw, double dts) {
+// pow(v, 1/alpha) and pow(1 - E0, 1/f) as exp(log(.) * .) (v, f >= 1e-6 > 0):
+// a shorter dependent chain than pow, within a few ulps of it.
+__device__ __forceinline__ double bw_pv(double vs) { return exp(kBwIalpha * log(vs)); }
+__device__ __forceinline__ double bw_ef(double fs, double ln1e) { return exp(ln1e / fs); }
+
+__device__ __forceinline__ double bold_step_d(BoldD& b, double z_raw, double dts, double ln1e) {
     const double z = z_raw > 0.0 ? z_raw : 0.0;
     const double fs = b.f > kBwLo ? b.f : kBwLo;
     const double vs = b.v > kBwLo ? b.v : kBwLo;
     const double ds = (z - kBwKappa * b.s) - kBwGamma * (fs - 1.0);
-    const double pv = pow(vs, kBwIalpha);
+    const double pv = bw_pv(vs);
     const double dv = kBwTauInv * (fs - pv);
-    const double dq = kBwTauInv * ((fs * (1.0 - pow(1.0 - kBwE0, 1.0 / fs))) / kBwE0 - pv * (b.q / vs));
+    const double dq = kBwTauInv * ((fs * (1.0 - bw_ef(fs, ln1e))) / kBwE0 - pv * (b.q / vs));
     const double sn = b.s + dts * ds, fn = fs + dts * b.s, vn = vs + dts * dv, qn = b.q + dts * dq;
     b.s = sn > 0.0 ? sn : 0.0;
     b.f = fn > kBwLo ? fn : kBwLo;
@@ -591,19 +596,42 @@ __device__ __forceinline__ double bold_step_d(BoldD& b, double z_raw, double dts
     return 4.0 * ((7.0 * (1.0 - b.q) + 2.0 * (1.0 - b.q / vv)) + 2.0 * (1.0 - vv));
 }
 
+// Each thread runs one node's recursion over the windows; the window inputs of
+// the next kBf windows load while the current ones step (the recursion is a
+// dependent FP64 chain, so their latency would otherwise add per window).
+constexpr int kBf = 8;
 __global__ __launch_bounds__(256) void k_bold_forward(const AdjInst* __restrict__ insts) {
     const AdjInst& I = insts[blockIdx.x];
     if (I.bold_period <= 0) return;
     const int N = I.n_nodes, nw = I.n_win, P = I.bold_period, T = I.n_steps;
+    const double ln1e = log(1.0 - kBwE0);
     for (int n = threadIdx.x; n < N; n += blockDim.x) {
         BoldD b;
         b.s = 0.0; b.f = 1.0; b.v = 1.0; b.q = 1.0;
-        for (int k = 0; k < nw; ++k) {
-            const int cnt = (k == nw - 1) ? T - k * P : P;
-            const double dts = (I.dt_ms * (double)cnt) / 1000.0;
-            double* st = I.bstate + ((size_t)k * N + n) * 4;
-            st[0] = b.s; st[1] = b.f; st[2] = b.v; st[3] = b.q;
-            I.bold[(size_t)k * N + n] = bold_step_d(b, (double)I.zlog[(size_t)k * N + n], dts);
+        float za[kBf], zb[kBf];
+        auto fetch = [&](float (&z)[kBf], int k0) {
+#pragma unroll
+            for (int m = 0; m < kBf; ++m)
+                z[m] = k0 + m < nw ? I.zlog[(size_t)(k0 + m) * N + n] : 0.0f;
+        };
+        auto run = [&](const float (&z)[kBf], int k0) {
+#pragma unroll
+            for (int m = 0; m < kBf; ++m) {
+                const int k = k0 + m;
+                if (k >= nw) break;
+                const int cnt = (k == nw - 1) ? T - k * P : P;
+                const double dts = (I.dt_ms * (double)cnt) / 1000.0;
+                double* st = I.bstate + ((size_t)k * N + n) * 4;
+                st[0] = b.s; st[1] = b.f; st[2] = b.v; st[3] = b.q;
+                I.bold[(size_t)k * N + n] = bold_step_d(b, (double)z[m], dts, ln1e);
+            }
+        };
+        fetch(za, 0);
+        for (int k0 = 0; k0 < nw; k0 += 2 * kBf) {
+            fetch(zb, k0 + kBf);
+            run(za, k0);
+            fetch(za, k0 + 2 * kBf);
+            run(zb, k0 + kBf);
         }
     }
 }
@@ -616,131 +644,203 @@ __global__ __launch_bounds__(256) void k_bold_forward(const AdjInst* __restrict_
 // h_i Z_ti) / (sqrt(n) s_i), Gs = G + G^T, G_ij = 2 (r_ij - target_ij) (0 where
 // clamped), h_i = sum_j Gs_ij r_ij.
 // BOLD MSE: loss = mean (BOLD - target)^2.
-// LDS-tiled FC path for N <= kFcTile: samples are staged in chunks of kFcChunk
-// rows ([chunk][N] doubles, coalesced), G + G^T lives in LDS.
-constexpr int kFcTile = 96, kFcChunk = 32, kFcPairs = 19;   // pairs per thread: ceil(96*97/2 / 256)
-constexpr int kFcLds = (kFcTile * kFcTile + kFcChunk * kFcTile + 3 * kFcTile) * 8;
+// FC path for N <= kFcTile on the FP64 matrix cores (v_mfma_f64_16x16x4f64),
+// one workgroup of four waves per instance:
+//   covariance C = Zc^T Zc over centred samples staged kFcChunk rows at a time
+//   in LDS (lower-triangle 16x16 tiles, dealt round-robin to the waves; the
+//   diagonal gives the variances), then r, the loss and Gs = G + G^T in
+//   LDS, then dL/dX = (Zs Gs - h .* Zs) / (sqrt(n) s) with Zs = Zc / (sqrt(n) s)
+//   as 16-sample x 16-node output tiles of Zs Gs.
+// Fragment layout of the 16x16x4 FP64 product (lane l, item r):
+//   A[i][k] = a(lane i + 16 k), B[k][j] = b(lane j + 16 k),
+//   D[l / 16 + 4 r][l % 16] = d(lane l)[r] (rows interleaved by item).
+constexpr int kFcTile = 96, kFcChunk = 32;
+constexpr int kFcCs = kFcTile + 1;            // padded chunk row stride (doubles)
+constexpr int kFcTiles = 6;                   // lower-triangle tiles per wave: ceil(21 / 4)
+constexpr int kFcLds = (kFcTile * kFcTile + kFcChunk * kFcCs + 4 * kFcTile + 4 * kFcTile) * 8;
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+constexpr int kFcStage = kFcChunk * kFcTile / 256;   // chunk elements per thread (256 threads)
 
 __device__ void fc_loss_tiled(const AdjInst& I, double* sh, double* red) {
     const int N = I.n_nodes, nw = I.n_win;
-    const int tid = threadIdx.x, BS = blockDim.x;
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+    constexpr int nwave = 4;
+    const int nt = (N + 15) >> 4, NP = nt * 16;   // node tiles, padded width
     const double* X = I.bold;
-    double* gsm = sh;                                  // [N][N] G + G^T
-    double* chunk = sh + kFcTile * kFcTile;            // [kFcChunk][N]
-    double* mean = chunk + kFcChunk * kFcTile;         // [N]
-    double* sd = mean + kFcTile;                       // [N]
-    double* hh = sd + kFcTile;                         // [N]
-    // ---- means and population standard deviations (two passes over chunks)
-    double acc = 0.0;
-    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
-        const int kc = min(kFcChunk, nw - k0);
-        __syncthreads();
-        for (int q = tid; q < kc * N; q += BS) chunk[q] = X[(size_t)k0 * N + q];
-        __syncthreads();
-        if (tid < N)
-            for (int k = 0; k < kc; ++k) acc += chunk[k * N + tid];
-    }
-    if (tid < N) mean[tid] = acc / (double)nw;
-    acc = 0.0;
-    __syncthreads();
-    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
-        const int kc = min(kFcChunk, nw - k0);
-        __syncthreads();
-        for (int q = tid; q < kc * N; q += BS) chunk[q] = X[(size_t)k0 * N + q] - mean[q % N];
-        __syncthreads();
-        if (tid < N)
-            for (int k = 0; k < kc; ++k) acc += chunk[k * N + tid] * chunk[k * N + tid];
-    }
-    if (tid < N) {
-        double sdev = sqrt(acc / (double)nw);
-        sd[tid] = sdev < 1e-12 ? 1.0 : sdev;
-    }
-    __syncthreads();
-    // ---- covariances of this thread's pairs (j <= i), accumulated over chunks
-    const int npairs = N * (N + 1) / 2;
-    int pij[kFcPairs];          // pair (i, j), j <= i, packed as i << 16 | j
-    double cv[kFcPairs];
+    double* gsm = sh;                                  // [NP][NP] (stride kFcTile)
+    double* chunk = sh + kFcTile * kFcTile;            // [kFcChunk][kFcCs]
+    double* mean = chunk + kFcChunk * kFcCs;           // [kFcTile]
+    double* zsc = mean + kFcTile;                      // [kFcTile] 1 / (sqrt(n) s)
+    double* hh = zsc + kFcTile;                        // [kFcTile]
+    double* rdg = hh + kFcTile;                        // [kFcTile] r_ii
+    double* sd = rdg + kFcTile;                        // [kFcTile]
+    // This thread's chunk elements: (row, column) packed as row << 8 | column.
+    // Each chunk's samples load into registers while the previous chunk is
+    // consumed, so the global-load latency overlaps the matrix products.
+    int el[kFcStage];
 #pragma unroll
-    for (int q = 0; q < kFcPairs; ++q) {
-        const int p = tid + q * BS;
-        int i = 0, j = 0;
-        if (p < npairs) {
-            i = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
-            while (i * (i + 1) / 2 > p) --i;
-            while ((i + 1) * (i + 2) / 2 <= p) ++i;
-            j = p - i * (i + 1) / 2;
+    for (int u = 0; u < kFcStage; ++u) {
+        const int q = tid + u * 256, k = q / NP, c = q - k * NP;
+        el[u] = q < kFcChunk * NP ? (k << 8) | c : -1;
+    }
+    auto fetch = [&](double (&v)[kFcStage], int k0) {
+        const int kc = min(kFcChunk, nw - k0);
+#pragma unroll
+        for (int u = 0; u < kFcStage; ++u) {
+            const int k = el[u] >> 8, c = el[u] & 255;
+            v[u] = (el[u] >= 0 && k < kc && c < N) ? X[(size_t)(k0 + k) * N + c] : 0.0;
         }
-        pij[q] = (i << 16) | j;
-        cv[q] = 0.0;
-    }
-    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
+    };
+    // store (v - mean) * scale into the LDS chunk; padding stays zero
+    auto stage = [&](const double (&v)[kFcStage], int k0, bool scaled) {
         const int kc = min(kFcChunk, nw - k0);
-        __syncthreads();
-        for (int q = tid; q < kc * N; q += BS) chunk[q] = X[(size_t)k0 * N + q] - mean[q % N];
-        __syncthreads();
 #pragma unroll
-        for (int q = 0; q < kFcPairs; ++q) {
-            if (tid + q * BS < npairs) {
-                const int i = pij[q] >> 16, j = pij[q] & 0xFFFF;
-                double a = cv[q];
-                for (int k = 0; k < kc; ++k) a += chunk[k * N + i] * chunk[k * N + j];
-                cv[q] = a;
+        for (int u = 0; u < kFcStage; ++u) {
+            if (el[u] < 0) continue;
+            const int k = el[u] >> 8, c = el[u] & 255;
+            double z = 0.0;
+            if (k < kc && c < N) z = scaled ? (v[u] - mean[c]) * zsc[c] : v[u] - mean[c];
+            chunk[k * kFcCs + c] = z;
+        }
+    };
+    double va[kFcStage], vb[kFcStage];
+    // ---- means: per-element partial sums, then per-column sums over the rows
+    {
+        double ps[kFcStage];
+#pragma unroll
+        for (int u = 0; u < kFcStage; ++u) ps[u] = 0.0;
+        fetch(va, 0);
+        for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
+            if (k0 + kFcChunk < nw) fetch(vb, k0 + kFcChunk);
+#pragma unroll
+            for (int u = 0; u < kFcStage; ++u) { ps[u] += va[u]; va[u] = vb[u]; }
+        }
+#pragma unroll
+        for (int u = 0; u < kFcStage; ++u)
+            if (el[u] >= 0) chunk[(el[u] >> 8) * kFcCs + (el[u] & 255)] = ps[u];
+        __syncthreads();
+        if (tid < N) {
+            double a = 0.0;
+            for (int k = 0; k < kFcChunk; ++k) a += chunk[k * kFcCs + tid];
+            mean[tid] = a / (double)nw;
+        }
+    }
+    // ---- covariance tiles (ti >= tj) of this wave
+    const int ntt = nt * (nt + 1) / 2;
+    int tti[kFcTiles], ttj[kFcTiles];
+    d4 acc[kFcTiles];
+#pragma unroll
+    for (int q = 0; q < kFcTiles; ++q) {
+        const int t = wave + q * nwave;
+        int ti = 0, tj = 0;
+        if (t < ntt) {
+            while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+            tj = t - ti * (ti + 1) / 2;
+        }
+        tti[q] = ti; ttj[q] = tj;
+        acc[q] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    fetch(va, 0);
+    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
+        __syncthreads();
+        stage(va, k0, false);
+        __syncthreads();
+        if (k0 + kFcChunk < nw) fetch(va, k0 + kFcChunk);
+#pragma unroll
+        for (int kk = 0; kk < kFcChunk; kk += 4) {
+            const double* row = chunk + (kk + lq) * kFcCs + lr;
+#pragma unroll
+            for (int q = 0; q < kFcTiles; ++q)
+                if (wave + q * nwave < ntt) acc[q] = mfma_f64(row[tti[q] * 16], row[ttj[q] * 16], acc[q]);
+        }
+    }
+    // variances from the diagonal tiles
+    const double sqn = sqrt((double)nw);
+#pragma unroll
+    for (int q = 0; q < kFcTiles; ++q) {
+        if (wave + q * nwave < ntt && tti[q] == ttj[q]) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = tti[q] * 16 + lq + 4 * r, j = ttj[q] * 16 + lr;
+                if (i == j && i < N) {
+                    double sdev = sqrt(acc[q][r] / (double)nw);
+                    sdev = sdev < 1e-12 ? 1.0 : sdev;
+                    sd[i] = sdev;
+                    zsc[i] = 1.0 / (sqn * sdev);
+                }
             }
         }
     }
+    __syncthreads();
+    // ---- r, loss, Gs: for j < i, gsm[j][i] = Gs_ij (upper) and gsm[i][j] = r_ij
+    // (lower); gsm[i][i] = Gs_ii, rdg[i] = r_ii; zero outside N x N
     double lacc = 0.0;
 #pragma unroll
-    for (int q = 0; q < kFcPairs; ++q) {
-        if (tid + q * BS < npairs) {
-            const int i = pij[q] >> 16, j = pij[q] & 0xFFFF;
-            const double raw = (cv[q] / (double)nw) / (sd[i] * sd[j]);
-            const bool clamped = raw < -1.0 || raw > 1.0;
-            const double r = clamped ? (raw < -1.0 ? -1.0 : 1.0) : raw;
-            const double rf = (double)(float)r;
-            const double tij = I.target[(size_t)i * N + j], tji = I.target[(size_t)j * N + i];
-            const double g = clamped ? 0.0 : 2.0 * (rf - tij) + 2.0 * (rf - tji);
-            gsm[i * N + j] = g;
-            gsm[j * N + i] = g;
-            cv[q] = r;   // keep r for h
-            lacc += (rf - tij) * (rf - tij);
-            if (i != j) lacc += (rf - tji) * (rf - tji);
+    for (int q = 0; q < kFcTiles; ++q) {
+        if (wave + q * nwave >= ntt) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = tti[q] * 16 + lq + 4 * r, j = ttj[q] * 16 + lr;
+            if (j > i) continue;
+            double g = 0.0, rv = 0.0;
+            if (i < N && j < N) {
+                const double raw = (acc[q][r] / (double)nw) / (sd[i] * sd[j]);
+                const bool clamped = raw < -1.0 || raw > 1.0;
+                rv = clamped ? (raw < -1.0 ? -1.0 : 1.0) : raw;
+                const double rf = (double)(float)rv;
+                const double tij = I.target[(size_t)i * N + j], tji = I.target[(size_t)j * N + i];
+                g = clamped ? 0.0 : 2.0 * (rf - tij) + 2.0 * (rf - tji);
+                lacc += (rf - tij) * (rf - tij);
+                if (i != j) lacc += (rf - tji) * (rf - tji);
+            }
+            if (i == j) { gsm[i * kFcTile + i] = g; rdg[i] = rv; }
+            else { gsm[j * kFcTile + i] = g; gsm[i * kFcTile + j] = rv; }
         }
     }
     const double tot = block_sum_d(lacc, red);
     if (tid == 0) I.out_loss[0] = (float)tot;
     __syncthreads();
-    // ---- h_i = sum_j Gs_ij r_ij, with r staged through the instance's scratch
-    double* rr = I.fc_work + 2 * (size_t)N;
-#pragma unroll
-    for (int q = 0; q < kFcPairs; ++q) {
-        if (tid + q * BS < npairs) {
-            const int i = pij[q] >> 16, j = pij[q] & 0xFFFF;
-            rr[(size_t)i * N + j] = cv[q];
-            rr[(size_t)j * N + i] = cv[q];
-        }
-    }
-    __syncthreads();
+    // ---- h_i = sum_j Gs_ij r_ij: each off-diagonal product is upper x lower
     if (tid < N) {
-        double a = 0.0;
-        for (int j = 0; j < N; ++j) a += gsm[tid * N + j] * rr[(size_t)tid * N + j];
+        double a = gsm[tid * kFcTile + tid] * rdg[tid];
+        for (int j = 0; j < N; ++j)
+            if (j != tid) a += gsm[tid * kFcTile + j] * gsm[j * kFcTile + tid];
         hh[tid] = a;
     }
     __syncthreads();
-    // ---- dL/dX = (Z Gs - h .* Z) / (sqrt(n) s), Z = centred / (sqrt(n) s), per chunk
-    const double sqn = sqrt((double)nw);
+    for (int q = tid; q < NP * NP; q += 256) {   // lower <- upper: Gs symmetric
+        const int i = q / NP, j = q - i * NP;
+        if (j < i) gsm[i * kFcTile + j] = gsm[j * kFcTile + i];
+    }
+    // ---- dL/dX per chunk: output tiles (16 samples) x (16 nodes)
+    const int ntile = (kFcChunk / 16) * nt;
+    fetch(va, 0);
     for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
         const int kc = min(kFcChunk, nw - k0);
         __syncthreads();
-        for (int q = tid; q < kc * N; q += BS) {
-            const int j = q % N;
-            chunk[q] = (X[(size_t)k0 * N + q] - mean[j]) / (sqn * sd[j]);
-        }
+        stage(va, k0, true);
         __syncthreads();
-        for (int q = tid; q < kc * N; q += BS) {
-            const int k = q / N, i = q % N;
-            double a = 0.0;
-            for (int j = 0; j < N; ++j) a += gsm[i * N + j] * chunk[k * N + j];
-            I.dsig[(size_t)k0 * N + q] = (a - hh[i] * chunk[q]) / (sqn * sd[i]);
+        if (k0 + kFcChunk < nw) fetch(va, k0 + kFcChunk);
+        for (int t = wave; t < ntile; t += nwave) {
+            const int ts = t / nt, tn = t - ts * nt;
+            d4 d = d4{0.0, 0.0, 0.0, 0.0};
+            const double* arow = chunk + (ts * 16 + lr) * kFcCs + lq;        // A[s][j] = Zs[s][j]
+            const double* bcol = gsm + lq * kFcTile + tn * 16 + lr;           // B[j][i] = Gs[j][i]
+            for (int j0 = 0; j0 < NP; j0 += 4)
+                d = mfma_f64(arow[j0], bcol[j0 * kFcTile], d);
+            const int i = tn * 16 + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int sm = ts * 16 + lq + 4 * r;
+                if (sm < kc && i < N)
+                    I.dsig[(size_t)(k0 + sm) * N + i] = (d[r] - hh[i] * chunk[sm * kFcCs + i]) * zsc[i];
+            }
         }
     }
 }
@@ -833,7 +933,11 @@ __global__ __launch_bounds__(256) void k_loss(const AdjInst* __restrict__ insts)
 }
 
 // ------------------------------------------------------- BOLD reverse ------
-// Adjoint of bold_step_d per node, windows last to first (FP64).
+// Adjoint of bold_step_d per node, windows last to first (FP64). The saved
+// states, window inputs and dL/dBOLD of the next kBr windows load while the
+// current ones are differentiated.
+constexpr int kBr = 4;
+struct BoldRevIn { double s, f, v, q, dsg; float z; };
 __global__ __launch_bounds__(256) void k_bold_reverse(const AdjInst* __restrict__ insts) {
     const AdjInst& I = insts[blockIdx.x];
     if (I.bold_period <= 0 || I.loss_kind == HB_LOSS_ORDER) return;
@@ -841,45 +945,71 @@ __global__ __launch_bounds__(256) void k_bold_reverse(const AdjInst* __restrict_
     const double ln1e = log(1.0 - kBwE0);
     for (int n = threadIdx.x; n < N; n += blockDim.x) {
         double ls = 0.0, lf = 0.0, lv = 0.0, lq = 0.0;
-        for (int k = nw - 1; k >= 0; --k) {
-            const int cnt = (k == nw - 1) ? T - k * P : P;
-            const double dts = (I.dt_ms * (double)cnt) / 1000.0;
-            const double* st = I.bstate + ((size_t)k * N + n) * 4;
-            const double s = st[0], f = st[1], v = st[2], q = st[3];
-            const double zr = (double)I.zlog[(size_t)k * N + n];
-            const double zz = zr > 0.0 ? zr : 0.0;
-            const double fs = f > kBwLo ? f : kBwLo;
-            const double vs = v > kBwLo ? v : kBwLo;
-            const double ds = (zz - kBwKappa * s) - kBwGamma * (fs - 1.0);
-            const double pv = pow(vs, kBwIalpha);
-            const double dv = kBwTauInv * (fs - pv);
-            const double ef = pow(1.0 - kBwE0, 1.0 / fs);
-            const double dq = kBwTauInv * ((fs * (1.0 - ef)) / kBwE0 - pv * (q / vs));
-            const double sn = s + dts * ds, fn = fs + dts * s, vn = vs + dts * dv, qn = q + dts * dq;
-            const double v1 = vn > kBwLo ? vn : kBwLo;
-            const double q1 = qn > 0.0 ? qn : 0.0;
-            // signal 4 * (7 (1 - q') + 2 (1 - q'/v') + 2 (1 - v'))
-            const double dsg = I.dsig[(size_t)k * N + n];
-            lq += dsg * 4.0 * (-7.0 - 2.0 / v1);
-            lv += (v1 > kBwLo) ? dsg * 4.0 * (2.0 * q1 / (v1 * v1) - 2.0) : 0.0;
-            const double lsn = sn > 0.0 ? ls : 0.0;
-            const double lfn = fn > kBwLo ? lf : 0.0;
-            const double lvn = vn > kBwLo ? lv : 0.0;
-            const double lqn = qn > 0.0 ? lq : 0.0;
-            const double dfe = (1.0 - ef) + ef * ln1e / fs;            // d(fs (1 - ef)) / dfs
-            const double lfs = lsn * dts * (-kBwGamma) + lfn + lvn * dts * kBwTauInv +
-                               lqn * dts * kBwTauInv * (dfe / kBwE0);
-            const double dpv = kBwIalpha * pv / vs;
-            const double dpvq = (kBwIalpha - 1.0) * pv / (vs * vs);    // d(pv / vs) / dvs
-            const double lvs = lvn * (1.0 - dts * kBwTauInv * dpv) - lqn * dts * kBwTauInv * q * dpvq;
-            const double nls = lsn * (1.0 - dts * kBwKappa) + lfn * dts;
-            const double nlq = lqn * (1.0 - dts * kBwTauInv * (pv / vs));
-            const double lz = zr > 0.0 ? lsn * dts : 0.0;
-            ls = nls;
-            lf = f > kBwLo ? lfs : 0.0;
-            lv = v > kBwLo ? lvs : 0.0;
-            lq = nlq;
-            I.seedw[(size_t)k * N + n] = (float)(lz / (double)cnt);
+        BoldRevIn ba[kBr], bb[kBr];
+        auto fetch = [&](BoldRevIn (&b)[kBr], int k0) {   // windows k0, k0 - 1, ...
+#pragma unroll
+            for (int m = 0; m < kBr; ++m) {
+                const int k = k0 - m;
+                if (k >= 0) {
+                    const double4 st = *(const double4*)(I.bstate + ((size_t)k * N + n) * 4);
+                    b[m].s = st.x; b[m].f = st.y; b[m].v = st.z; b[m].q = st.w;
+                    b[m].z = I.zlog[(size_t)k * N + n];
+                    b[m].dsg = I.dsig[(size_t)k * N + n];
+                } else {
+                    b[m].s = 0.0; b[m].f = 1.0; b[m].v = 1.0; b[m].q = 1.0; b[m].z = 0.0f; b[m].dsg = 0.0;
+                }
+            }
+        };
+        auto run = [&](const BoldRevIn (&b)[kBr], int k0) {
+#pragma unroll
+            for (int m = 0; m < kBr; ++m) {
+                const int k = k0 - m;
+                if (k < 0) break;
+                const int cnt = (k == nw - 1) ? T - k * P : P;
+                const double dts = (I.dt_ms * (double)cnt) / 1000.0;
+                const double s = b[m].s, f = b[m].f, v = b[m].v, q = b[m].q;
+                const double zr = (double)b[m].z;
+                const double zz = zr > 0.0 ? zr : 0.0;
+                const double fs = f > kBwLo ? f : kBwLo;
+                const double vs = v > kBwLo ? v : kBwLo;
+                const double ds = (zz - kBwKappa * s) - kBwGamma * (fs - 1.0);
+                const double pv = bw_pv(vs);
+                const double dv = kBwTauInv * (fs - pv);
+                const double ef = bw_ef(fs, ln1e);
+                const double dq = kBwTauInv * ((fs * (1.0 - ef)) / kBwE0 - pv * (q / vs));
+                const double sn = s + dts * ds, fn = fs + dts * s, vn = vs + dts * dv, qn = q + dts * dq;
+                const double v1 = vn > kBwLo ? vn : kBwLo;
+                const double q1 = qn > 0.0 ? qn : 0.0;
+                // signal 4 * (7 (1 - q') + 2 (1 - q'/v') + 2 (1 - v'))
+                const double dsg = b[m].dsg;
+                lq += dsg * 4.0 * (-7.0 - 2.0 / v1);
+                lv += (v1 > kBwLo) ? dsg * 4.0 * (2.0 * q1 / (v1 * v1) - 2.0) : 0.0;
+                const double lsn = sn > 0.0 ? ls : 0.0;
+                const double lfn = fn > kBwLo ? lf : 0.0;
+                const double lvn = vn > kBwLo ? lv : 0.0;
+                const double lqn = qn > 0.0 ? lq : 0.0;
+                const double dfe = (1.0 - ef) + ef * ln1e / fs;            // d(fs (1 - ef)) / dfs
+                const double lfs = lsn * dts * (-kBwGamma) + lfn + lvn * dts * kBwTauInv +
+                                   lqn * dts * kBwTauInv * (dfe / kBwE0);
+                const double dpv = kBwIalpha * pv / vs;
+                const double dpvq = (kBwIalpha - 1.0) * pv / (vs * vs);    // d(pv / vs) / dvs
+                const double lvs = lvn * (1.0 - dts * kBwTauInv * dpv) - lqn * dts * kBwTauInv * q * dpvq;
+                const double nls = lsn * (1.0 - dts * kBwKappa) + lfn * dts;
+                const double nlq = lqn * (1.0 - dts * kBwTauInv * (pv / vs));
+                const double lz = zr > 0.0 ? lsn * dts : 0.0;
+                ls = nls;
+                lf = f > kBwLo ? lfs : 0.0;
+                lv = v > kBwLo ? lvs : 0.0;
+                lq = nlq;
+                I.seedw[(size_t)k * N + n] = (float)(lz / (double)cnt);
+            }
+        };
+        fetch(ba, nw - 1);
+        for (int k0 = nw - 1; k0 >= 0; k0 -= 2 * kBr) {
+            fetch(bb, k0 - kBr);
+            run(ba, k0);
+            fetch(ba, k0 - 2 * kBr);
+            run(bb, k0 - kBr);
         }
     }
 }
