@@ -21,6 +21,7 @@ W_NONE, W_SCALAR_MEAN, W_SCALAR_DIAG, W_DENSE, W_CSR = 0, 1, 2, 3, 4
 DELAY_PER_EDGE, DELAY_SINGLE, DELAY_BATCH_COMPAT = 0, 1, 2
 ENGINE_BATCH, ENGINE_PLAIN = 0, 1
 FLAG_TRAJECTORY, FLAG_EXACT, FLAG_NO_GRAPH, FLAG_NO_BLOCKING, FLAG_AUTOTUNE = 0x1, 0x2, 0x4, 0x8, 0x10
+FLAG_SEPARATE_BOLD = 0x20
 LOSS_FC, LOSS_BOLD_MSE, LOSS_ORDER = 0, 1, 2
 
 MODEL_NAMES = {

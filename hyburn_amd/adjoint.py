@@ -40,7 +40,7 @@ class Gradients:
 
 class AdjointPlan:
     def __init__(self, insts: Sequence[Instance], losses: Sequence[Loss], device: int = 0,
-                 checkpoint_every: int = 0, blocking: bool = True):
+                 checkpoint_every: int = 0, blocking: bool = True, bold_helpers: bool = True):
         if len(insts) != len(losses):
             raise ValueError("one loss per instance")
         self.insts = list(insts)
@@ -75,7 +75,8 @@ class AdjointPlan:
             s.final_state = fs.ctypes.data_as(L.fp)
         o = L.HbAdjointOpts()
         o.device, o.checkpoint_every = device, int(checkpoint_every)
-        o.flags = 0 if blocking else L.FLAG_NO_BLOCKING
+        # bold_helpers=False: the BOLD monitor's forward as a kernel of its own, not helper waves
+        o.flags = (0 if blocking else L.FLAG_NO_BLOCKING) | (0 if bold_helpers else L.FLAG_SEPARATE_BOLD)
         self._h = C.c_void_p()
         L.check(L.lib().hb_adjoint_plan_create(self._m.arr, self._spec, n, C.byref(o), C.byref(self._h)))
         self.stats = L.HbAdjointStats()

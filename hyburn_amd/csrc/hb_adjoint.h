@@ -11,7 +11,7 @@ struct alignas(16) AdjInst {
     int32_t np, cfn, engine, has_coupling;
     int32_t seg_len, n_seg, ring_depth, aring_depth;   // S, ceil(T/S), forward and adjoint ring depths
     int32_t ring_lds, aring_lds, bold_period, n_win;
-    int32_t loss_kind, pad0, pad1, pad2;
+    int32_t loss_kind, bold_fused, pad1, pad2;   // bold_fused: BOLD forward in helper waves of the blocked forward
     float dt, dt_half, dt_sixth, pad3;
     float mc[kMC];
     float cc[kCC];
@@ -53,6 +53,7 @@ struct AdjLaunch {
     int32_t npt;                // nodes per thread (1 or 2)
     bool blocked;               // delay-blocked kernels (min delay >= 8, npt 1)
     bool lring;                 // blocked kernels: rings in LDS
+    bool bold_fused;            // blocked forward carries the BOLD monitor in helper waves (launched 2 x block)
     int32_t block;
     int32_t lds_fwd, lds_rev;   // dynamic LDS bytes
 };

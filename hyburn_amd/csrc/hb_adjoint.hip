@@ -596,14 +596,25 @@ __device__ __forceinline__ double bold_step_d(BoldD& b, double z_raw, double dts
     return 4.0 * ((7.0 * (1.0 - b.q) + 2.0 * (1.0 - b.q / vv)) + 2.0 * (1.0 - vv));
 }
 
+// One window of node n: saves the state entering the window, steps it with the
+// window's mean input z and stores the native BOLD sample.
+__device__ __forceinline__ void bold_fwd_window(const AdjInst& I, BoldD& b, int k, int n, float z, double ln1e) {
+    const int N = I.n_nodes, nw = I.n_win, P = I.bold_period, T = I.n_steps;
+    const int cnt = (k == nw - 1) ? T - k * P : P;
+    const double dts = (I.dt_ms * (double)cnt) / 1000.0;
+    double* st = I.bstate + ((size_t)k * N + n) * 4;
+    st[0] = b.s; st[1] = b.f; st[2] = b.v; st[3] = b.q;
+    I.bold[(size_t)k * N + n] = bold_step_d(b, (double)z, dts, ln1e);
+}
+
 // Each thread runs one node's recursion over the windows; the window inputs of
 // the next kBf windows load while the current ones step (the recursion is a
 // dependent FP64 chain, so their latency would otherwise add per window).
 constexpr int kBf = 8;
 __global__ __launch_bounds__(256) void k_bold_forward(const AdjInst* __restrict__ insts) {
     const AdjInst& I = insts[blockIdx.x];
-    if (I.bold_period <= 0) return;
-    const int N = I.n_nodes, nw = I.n_win, P = I.bold_period, T = I.n_steps;
+    if (I.bold_period <= 0 || I.bold_fused) return;
+    const int N = I.n_nodes, nw = I.n_win;
     const double ln1e = log(1.0 - kBwE0);
     for (int n = threadIdx.x; n < N; n += blockDim.x) {
         BoldD b;
@@ -619,11 +630,7 @@ __global__ __launch_bounds__(256) void k_bold_forward(const AdjInst* __restrict_
             for (int m = 0; m < kBf; ++m) {
                 const int k = k0 + m;
                 if (k >= nw) break;
-                const int cnt = (k == nw - 1) ? T - k * P : P;
-                const double dts = (I.dt_ms * (double)cnt) / 1000.0;
-                double* st = I.bstate + ((size_t)k * N + n) * 4;
-                st[0] = b.s; st[1] = b.f; st[2] = b.v; st[3] = b.q;
-                I.bold[(size_t)k * N + n] = bold_step_d(b, (double)z[m], dts, ln1e);
+                bold_fwd_window(I, b, k, n, z[m], ln1e);
             }
         };
         fetch(za, 0);
@@ -938,23 +945,73 @@ __global__ __launch_bounds__(256) void k_loss(const AdjInst* __restrict__ insts)
 // current ones are differentiated.
 constexpr int kBr = 4;
 struct BoldRevIn { double s, f, v, q, dsg; float z; };
+struct BoldAdj { double ls, lf, lv, lq; };
+__device__ __forceinline__ BoldRevIn bold_rev_load(const AdjInst& I, int k, int n) {
+    BoldRevIn b;
+    const int N = I.n_nodes;
+    const double4 st = *(const double4*)(I.bstate + ((size_t)k * N + n) * 4);
+    b.s = st.x; b.f = st.y; b.v = st.z; b.q = st.w;
+    b.z = I.zlog[(size_t)k * N + n];
+    b.dsg = I.dsig[(size_t)k * N + n];
+    return b;
+}
+// Adjoint of window k of node n: a = dL/d(state leaving the window) on entry,
+// dL/d(state entering it) on return; stores dL/d(window input) / count.
+__device__ __forceinline__ void bold_rev_window(const AdjInst& I, BoldAdj& a, int k, int n, const BoldRevIn& in,
+                                                double ln1e) {
+    const int N = I.n_nodes, nw = I.n_win, P = I.bold_period, T = I.n_steps;
+    const int cnt = (k == nw - 1) ? T - k * P : P;
+    const double dts = (I.dt_ms * (double)cnt) / 1000.0;
+    const double s = in.s, f = in.f, v = in.v, q = in.q;
+    const double zr = (double)in.z;
+    const double zz = zr > 0.0 ? zr : 0.0;
+    const double fs = f > kBwLo ? f : kBwLo;
+    const double vs = v > kBwLo ? v : kBwLo;
+    const double ds = (zz - kBwKappa * s) - kBwGamma * (fs - 1.0);
+    const double pv = bw_pv(vs);
+    const double dv = kBwTauInv * (fs - pv);
+    const double ef = bw_ef(fs, ln1e);
+    const double dq = kBwTauInv * ((fs * (1.0 - ef)) / kBwE0 - pv * (q / vs));
+    const double sn = s + dts * ds, fn = fs + dts * s, vn = vs + dts * dv, qn = q + dts * dq;
+    const double v1 = vn > kBwLo ? vn : kBwLo;
+    const double q1 = qn > 0.0 ? qn : 0.0;
+    // signal 4 * (7 (1 - q') + 2 (1 - q'/v') + 2 (1 - v'))
+    const double dsg = in.dsg;
+    a.lq += dsg * 4.0 * (-7.0 - 2.0 / v1);
+    a.lv += (v1 > kBwLo) ? dsg * 4.0 * (2.0 * q1 / (v1 * v1) - 2.0) : 0.0;
+    const double lsn = sn > 0.0 ? a.ls : 0.0;
+    const double lfn = fn > kBwLo ? a.lf : 0.0;
+    const double lvn = vn > kBwLo ? a.lv : 0.0;
+    const double lqn = qn > 0.0 ? a.lq : 0.0;
+    const double dfe = (1.0 - ef) + ef * ln1e / fs;            // d(fs (1 - ef)) / dfs
+    const double lfs = lsn * dts * (-kBwGamma) + lfn + lvn * dts * kBwTauInv +
+                       lqn * dts * kBwTauInv * (dfe / kBwE0);
+    const double dpv = kBwIalpha * pv / vs;
+    const double dpvq = (kBwIalpha - 1.0) * pv / (vs * vs);    // d(pv / vs) / dvs
+    const double lvs = lvn * (1.0 - dts * kBwTauInv * dpv) - lqn * dts * kBwTauInv * q * dpvq;
+    const double nls = lsn * (1.0 - dts * kBwKappa) + lfn * dts;
+    const double nlq = lqn * (1.0 - dts * kBwTauInv * (pv / vs));
+    const double lz = zr > 0.0 ? lsn * dts : 0.0;
+    a.ls = nls;
+    a.lf = f > kBwLo ? lfs : 0.0;
+    a.lv = v > kBwLo ? lvs : 0.0;
+    a.lq = nlq;
+    I.seedw[(size_t)k * N + n] = (float)(lz / (double)cnt);
+}
 __global__ __launch_bounds__(256) void k_bold_reverse(const AdjInst* __restrict__ insts) {
     const AdjInst& I = insts[blockIdx.x];
     if (I.bold_period <= 0 || I.loss_kind == HB_LOSS_ORDER) return;
-    const int N = I.n_nodes, nw = I.n_win, P = I.bold_period, T = I.n_steps;
+    const int N = I.n_nodes, nw = I.n_win;
     const double ln1e = log(1.0 - kBwE0);
     for (int n = threadIdx.x; n < N; n += blockDim.x) {
-        double ls = 0.0, lf = 0.0, lv = 0.0, lq = 0.0;
+        BoldAdj la{0.0, 0.0, 0.0, 0.0};
         BoldRevIn ba[kBr], bb[kBr];
         auto fetch = [&](BoldRevIn (&b)[kBr], int k0) {   // windows k0, k0 - 1, ...
 #pragma unroll
             for (int m = 0; m < kBr; ++m) {
                 const int k = k0 - m;
                 if (k >= 0) {
-                    const double4 st = *(const double4*)(I.bstate + ((size_t)k * N + n) * 4);
-                    b[m].s = st.x; b[m].f = st.y; b[m].v = st.z; b[m].q = st.w;
-                    b[m].z = I.zlog[(size_t)k * N + n];
-                    b[m].dsg = I.dsig[(size_t)k * N + n];
+                    b[m] = bold_rev_load(I, k, n);
                 } else {
                     b[m].s = 0.0; b[m].f = 1.0; b[m].v = 1.0; b[m].q = 1.0; b[m].z = 0.0f; b[m].dsg = 0.0;
                 }
@@ -965,43 +1022,7 @@ __global__ __launch_bounds__(256) void k_bold_reverse(const AdjInst* __restrict_
             for (int m = 0; m < kBr; ++m) {
                 const int k = k0 - m;
                 if (k < 0) break;
-                const int cnt = (k == nw - 1) ? T - k * P : P;
-                const double dts = (I.dt_ms * (double)cnt) / 1000.0;
-                const double s = b[m].s, f = b[m].f, v = b[m].v, q = b[m].q;
-                const double zr = (double)b[m].z;
-                const double zz = zr > 0.0 ? zr : 0.0;
-                const double fs = f > kBwLo ? f : kBwLo;
-                const double vs = v > kBwLo ? v : kBwLo;
-                const double ds = (zz - kBwKappa * s) - kBwGamma * (fs - 1.0);
-                const double pv = bw_pv(vs);
-                const double dv = kBwTauInv * (fs - pv);
-                const double ef = bw_ef(fs, ln1e);
-                const double dq = kBwTauInv * ((fs * (1.0 - ef)) / kBwE0 - pv * (q / vs));
-                const double sn = s + dts * ds, fn = fs + dts * s, vn = vs + dts * dv, qn = q + dts * dq;
-                const double v1 = vn > kBwLo ? vn : kBwLo;
-                const double q1 = qn > 0.0 ? qn : 0.0;
-                // signal 4 * (7 (1 - q') + 2 (1 - q'/v') + 2 (1 - v'))
-                const double dsg = b[m].dsg;
-                lq += dsg * 4.0 * (-7.0 - 2.0 / v1);
-                lv += (v1 > kBwLo) ? dsg * 4.0 * (2.0 * q1 / (v1 * v1) - 2.0) : 0.0;
-                const double lsn = sn > 0.0 ? ls : 0.0;
-                const double lfn = fn > kBwLo ? lf : 0.0;
-                const double lvn = vn > kBwLo ? lv : 0.0;
-                const double lqn = qn > 0.0 ? lq : 0.0;
-                const double dfe = (1.0 - ef) + ef * ln1e / fs;            // d(fs (1 - ef)) / dfs
-                const double lfs = lsn * dts * (-kBwGamma) + lfn + lvn * dts * kBwTauInv +
-                                   lqn * dts * kBwTauInv * (dfe / kBwE0);
-                const double dpv = kBwIalpha * pv / vs;
-                const double dpvq = (kBwIalpha - 1.0) * pv / (vs * vs);    // d(pv / vs) / dvs
-                const double lvs = lvn * (1.0 - dts * kBwTauInv * dpv) - lqn * dts * kBwTauInv * q * dpvq;
-                const double nls = lsn * (1.0 - dts * kBwKappa) + lfn * dts;
-                const double nlq = lqn * (1.0 - dts * kBwTauInv * (pv / vs));
-                const double lz = zr > 0.0 ? lsn * dts : 0.0;
-                ls = nls;
-                lf = f > kBwLo ? lfs : 0.0;
-                lv = v > kBwLo ? lvs : 0.0;
-                lq = nlq;
-                I.seedw[(size_t)k * N + n] = (float)(lz / (double)cnt);
+                bold_rev_window(I, la, k, n, b[m], ln1e);
             }
         };
         fetch(ba, nw - 1);
@@ -1054,8 +1075,10 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
 
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int t0 = seg * Sg, t1 = min(T, t0 + Sg);
-        // ---- recompute states t0..t1 from the checkpoint and the coupling log;
-        // log rows of the next 8 steps load while the current 8 integrate
+        // ---- recompute states t0..t1 from the checkpoint and the coupling log.
+        // Rolled step loop: slot 0 of `cur` is the current step's log row, the
+        // window shifts down each step and `nxt` (the following kG rows) loads
+        // kG steps ahead of use.
 #pragma unroll
         for (int q = 0; q < NPT; ++q) {
             const int n = tid + q * BS;
@@ -1064,7 +1087,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
 #pragma unroll
             for (int v = 0; v < NV; ++v) { x[v] = I.ckpt[((size_t)seg * N + n) * NV + v]; I.tape[(size_t)n * NV + v] = x[v]; }
             constexpr int kG = 8;
-            float gq0[kG][NP], gq1[kG][NP];
+            float cur[kG][NP], nxt[kG][NP];
             auto fetch = [&](float (&gq)[kG][NP], int tg) {
 #pragma unroll
                 for (int m = 0; m < kG; ++m) {
@@ -1075,34 +1098,34 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
                     }
                 }
             };
-            auto compute = [&](const float (&gq)[kG][NP], int tg) {
+            fetch(cur, t0);
+            fetch(nxt, t0 + kG);
+#pragma nounroll
+            for (int t = t0; t < t1; ++t) {
+                float own[NP], cpl[NC], z[NV];
+                pre<NC, NP>(I.cfn, cc, false, false, x, own);
+                if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, cur[0], own, cpl);
+                else {
 #pragma unroll
-                for (int m = 0; m < kG; ++m) {
-                    const int t = tg + m;
-                    if (t < t1) {
-                        float own[NP], cpl[NC], z[NV];
-                        pre<NC, NP>(I.cfn, cc, false, false, x, own);
-                        if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, gq[m], own, cpl);
-                        else {
-#pragma unroll
-                            for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
-                        }
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
-                        integrate<M>(S, mcn[q], cpl, z, x);
-                        float* tp = I.tape + ((size_t)(t - t0 + 1) * N + n) * NV;
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) tp[v] = x[v];
-                    }
+                    for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
                 }
-            };
-            fetch(gq0, t0);
-            for (int tg = t0; tg < t1; tg += 2 * kG) {
-                if (tg + kG < t1) fetch(gq1, tg + kG);
-                compute(gq0, tg);
-                if (tg + kG >= t1) break;
-                if (tg + 2 * kG < t1) fetch(gq0, tg + 2 * kG);
-                compute(gq1, tg + kG);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
+                integrate<M>(S, mcn[q], cpl, z, x);
+                float* tp = I.tape + ((size_t)(t - t0 + 1) * N + n) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) tp[v] = x[v];
+#pragma unroll
+                for (int m = 0; m + 1 < kG; ++m)
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) cur[m][c] = cur[m + 1][c];
+                if (((t - t0) & (kG - 1)) == kG - 1) {
+#pragma unroll
+                    for (int m = 0; m < kG; ++m)
+#pragma unroll
+                        for (int c = 0; c < NP; ++c) cur[m][c] = nxt[m][c];
+                    fetch(nxt, t + 1 + kG);
+                }
             }
         }
         __syncthreads();
@@ -1335,6 +1358,38 @@ __device__ __forceinline__ BView bview(const AdjInst* I, bool csc) {
 
 __device__ __forceinline__ int wrap_slot(int t, int H) { int s = t % H; return s < 0 ? s + H : s; }
 
+// BOLD helper waves of the blocked forward kernel. With AdjInst::bold_fused
+// the workgroup is launched twice as wide as the neural threads need; threads
+// NB + n (NB = blockDim / 2) run node n's FP64 Balloon-Windkessel recursion on
+// other SIMDs while the neural threads integrate, taking part in exactly the
+// barriers the neural threads execute. Window inputs cross from the neural
+// threads through zlog in global memory, ordered by those barriers: after the
+// barrier that closes the block ending at step t, the windows completed by t
+// are in zlog, and the helpers step them during the next block.
+__device__ __forceinline__ void bold_helper_forward(const AdjInst& I, int n) {
+    const int N = I.n_nodes, nw = I.n_win, P = I.bold_period, T = I.n_steps;
+    const bool hown = n < N;
+    const double ln1e = log(1.0 - kBwE0);
+    BoldD b;
+    b.s = 0.0; b.f = 1.0; b.v = 1.0; b.q = 1.0;
+    int kb = 0;
+    auto advance = [&](int kmax) {
+        for (; kb < kmax; ++kb)
+            if (hown) bold_fwd_window(I, b, kb, n, I.zlog[(size_t)kb * N + n], ln1e);
+    };
+    auto done_after = [&](int t) { return t >= T - 1 ? nw : (t + 1) / P; };
+    __syncthreads();   // before the step-0 ring write
+    __syncthreads();   // after it
+    __syncthreads();   // after step 0
+    int tlast = 0;
+    for (int t0 = 1; t0 < T; t0 += kK) {
+        advance(done_after(tlast));
+        tlast = min(T - 1, t0 + kK - 1);
+        __syncthreads();   // end of the block t0 .. tlast
+    }
+    advance(nw);
+}
+
 // Forward save, delay-blocked. One node per thread (N <= blockDim).
 template <int M, int NP, bool LRING>
 __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict__ insts) {
@@ -1358,6 +1413,10 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
 #pragma unroll
     for (int v = 0; v < NV; ++v) nsc[v] = uf32(Ip->nscale[v]);
     for (int i = tid; i < NP * N * Hs; i += BS) ring[i] = 0.0f;
+    if (u32(Ip->bold_fused) && tid >= (BS >> 1)) {
+        bold_helper_forward(*Ip, tid - (BS >> 1));
+        return;
+    }
     float x[NV], mc[kMC];
 #pragma unroll
     for (int v = 0; v < NV; ++v) x[v] = own ? V.x0[tid * NV + v] : 0.0f;
@@ -1506,20 +1565,20 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
         // the previous block's order partials are complete (barrier at its end)
         if (ord && tid == 0)
             for (int k = 0; k < kprev; ++k) ord_close((blk - 1) & 1, k, tprev + k);
-        float xh[kK][NV];
+        // rolled step loop (one integrator copy in the code): slot 0 of acc is
+        // the current step's coupling sum, shifted down after each step
+#pragma nounroll
+        for (int k = 0; k < kk; ++k) {
+            float xs[NV];
 #pragma unroll
-        for (int k = 0; k < kK; ++k) {
-            if (k < kk) {
+            for (int v = 0; v < NV; ++v) xs[v] = x[v];   // state at the start of step t0 + k
+            step(t0 + k, acc[0]);
+            if (ord) ord_part(blk & 1, k);
+            if (own) ring_put(t0 + k, xs);   // slot of t0 + k: read by later blocks only
 #pragma unroll
-                for (int v = 0; v < NV; ++v) xh[k][v] = x[v];   // state at the start of step t0 + k
-                step(t0 + k, acc[k]);
-                if (ord) ord_part(blk & 1, k);
-            }
-        }
-        if (own) {
+            for (int q = 0; q + 1 < kK; ++q)
 #pragma unroll
-            for (int k = 0; k < kK; ++k)
-                if (k < kk) ring_put(t0 + k, xh[k]);
+                for (int c = 0; c < NP; ++c) acc[q][c] = acc[q + 1][c];
         }
         kprev = kk;
         tprev = t0;
@@ -1904,9 +1963,13 @@ hipError_t launch_adjoint_stage(int stage, const AdjLaunch& L, hipStream_t s) {
         const void* k = L.blocked ? adj::kernel_b(L.model, false, L.np, L.lring) : adj::kernel_for(L.model, false, L.np, L.npt);
         if (!k) return hipErrorInvalidValue;
         void* args[] = {(void*)&d};
-        return hipLaunchKernel(k, dim3(L.n), dim3(L.block), args, L.lds_fwd, s);
+        const int block = L.bold_fused ? 2 * L.block : L.block;   // + BOLD helper waves
+        return hipLaunchKernel(k, dim3(L.n), dim3(block), args, L.lds_fwd, s);
     }
-    case 1: hipLaunchKernelGGL(adj::k_bold_forward, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
+    case 1:
+        if (L.bold_fused) return hipSuccess;   // in the forward kernel's helper waves
+        hipLaunchKernelGGL(adj::k_bold_forward, dim3(L.n), dim3(256), 0, s, d);
+        return hipGetLastError();
     case 2: hipLaunchKernelGGL(adj::k_loss, dim3(L.n), dim3(256), adj::kFcLds, s, d); return hipGetLastError();
     case 3: hipLaunchKernelGGL(adj::k_bold_reverse, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
     default: {

@@ -194,6 +194,22 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         groups[std::make_tuple(I.model, s.np, s.npt, (int)s.blocked, (int)s.ring_lds)].push_back(i);
         P->node_steps += (double)I.n_nodes * (double)I.n_steps;
     }
+    // The BOLD monitor's forward runs in helper waves of the blocked forward
+    // kernel (one helper thread per node, workgroup doubled) when every
+    // instance of a blocked group carries a BOLD loss and the doubled
+    // workgroup fits 512 threads.
+    std::vector<int> bold_fused(n, 0);
+    for (auto& kv : groups) {
+        bool f = std::get<3>(kv.first) != 0;
+        int blk = 64;
+        for (int i : kv.second) {
+            f = f && inst[i].bold_period > 0 &&
+                (spec[i].loss_kind == HB_LOSS_FC || spec[i].loss_kind == HB_LOSS_BOLD_MSE);
+            blk = std::max(blk, sh[i].block);
+        }
+        if (f && 2 * blk <= 512 && !(opts && (opts->flags & HB_FLAG_SEPARATE_BOLD)))
+            for (int i : kv.second) bold_fused[i] = 1;
+    }
 
     // ---- arena layout
     struct Off {
@@ -296,6 +312,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         D.bold_period = I.bold_period > 0 ? I.bold_period : 0;
         D.n_win = s.nwin;
         D.loss_kind = Sp.loss_kind;
+        D.bold_fused = bold_fused[i];
         D.dt = I.dt; D.dt_half = I.dt * 0.5f; D.dt_sixth = I.dt / 6.0f;
         model_consts(I.model, I.params, D.mc);
         int classic = 0;
@@ -427,6 +444,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
             blk = std::max(blk, sh[i].block);
             if (sh[i].ring_lds) lds = std::max(lds, (int)sh[i].ring_b);
         }
+        L.bold_fused = bold_fused[kv.second[0]] != 0;
         L.block = blk;
         L.lds_fwd = lds;
         L.lds_rev = lds;

@@ -50,6 +50,7 @@ enum { HB_ENGINE_BATCH = 0, HB_ENGINE_PLAIN = 1 };
 #define HB_FLAG_NO_GRAPH 0x4u     /* launch directly instead of replaying a captured hipGraph */
 #define HB_FLAG_NO_BLOCKING 0x8u  /* disable the delay-blocked schedule (one barrier per step) */
 #define HB_FLAG_AUTOTUNE 0x10u    /* time candidate launch shapes per shape class before planning */
+#define HB_FLAG_SEPARATE_BOLD 0x20u /* adjoint: BOLD monitor forward in its own kernel, not in helper waves */
 
 /* ---------------- config-level API (BatchHybridEngine mirror) ---------------- */
 
@@ -228,7 +229,8 @@ typedef struct {
 typedef struct {
     int32_t device;
     int32_t checkpoint_every; /* steps per checkpoint segment; 0: ceil(sqrt(n_steps)) */
-    uint32_t flags;           /* HB_FLAG_NO_BLOCKING: one barrier per step even when min delay >= 8 */
+    uint32_t flags;           /* HB_FLAG_NO_BLOCKING: one barrier per step even when min delay >= 8;
+                                 HB_FLAG_SEPARATE_BOLD: BOLD monitor kernels instead of helper waves */
 } hb_adjoint_opts;
 
 typedef struct {

@@ -52,9 +52,10 @@ BLOCKED = {
 }
 
 
-def gpu_grads(I, kind, target, checkpoint_every=0, blocking=True):
+def gpu_grads(I, kind, target, checkpoint_every=0, blocking=True, bold_helpers=True):
     gp = I.node_param_idx
-    with AdjointPlan([I], [Loss(kind, target, gp)], checkpoint_every=checkpoint_every, blocking=blocking) as P:
+    with AdjointPlan([I], [Loss(kind, target, gp)], checkpoint_every=checkpoint_every, blocking=blocking,
+                     bold_helpers=bold_helpers) as P:
         loss = P.forward()
         g = P.backward()[0]
     return float(loss[0]), g
@@ -113,6 +114,20 @@ def test_blocked_schedule_is_bitwise_per_step(gpu, orc, name):
     I, kind, target = BLOCKED[name](orc)
     la, ga = gpu_grads(I, kind, target, blocking=True)
     lb, gb = gpu_grads(I, kind, target, blocking=False)
+    assert la == lb
+    assert ga.grad_coupling == gb.grad_coupling
+    np.testing.assert_array_equal(ga.grad_param, gb.grad_param)
+    np.testing.assert_array_equal(ga.grad_x0, gb.grad_x0)
+
+
+@pytest.mark.parametrize("name", [n for n in BLOCKED if "bold" in n])
+@pytest.mark.parametrize("S", [0, 7])
+def test_bold_helper_waves_are_bitwise_separate_kernels(gpu, orc, name, S):
+    """The BOLD monitor's forward in helper waves of the blocked forward kernel
+    gives the same bits as its own kernel (loss and every gradient)."""
+    I, kind, target = BLOCKED[name](orc)
+    la, ga = gpu_grads(I, kind, target, checkpoint_every=S, bold_helpers=True)
+    lb, gb = gpu_grads(I, kind, target, checkpoint_every=S, bold_helpers=False)
     assert la == lb
     assert ga.grad_coupling == gb.grad_coupling
     np.testing.assert_array_equal(ga.grad_param, gb.grad_param)
