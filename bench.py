@@ -162,7 +162,8 @@ def adjoint_leg(name, insts, losses, checkpoint_every, n_params):
         "workload": name, "n_instances": B, "n_nodes": insts[0].n_nodes, "n_steps": insts[0].n_steps,
         "grads_per_s": B / wall, "wall_ms": wall * 1e3, "adjoint_forward_ms": fwd_ms, "adjoint_backward_ms": bwd_ms,
         "plain_forward_ms": plain_ms, "cost_vs_forward": (fwd_ms + bwd_ms) / plain_ms,
-        "checkpoint_every": checkpoint_every, "device_bytes": dev_bytes, "plan_s": plan_s,
+        "checkpoint_every": checkpoint_every or "auto: whole trajectory kept when it fits a quarter of free HBM",
+        "device_bytes": dev_bytes, "plan_s": plan_s,
         "fd_grads_per_s_est": B / (2 * n_params * plain_wall),
         "fd_note": f"central differences over {n_params} parameters per instance = {2 * n_params} plain forwards "
                    "per gradient, estimated from the measured plain forward",
@@ -294,7 +295,7 @@ def main() -> int:
         fc_t = _fc64(h.bold.astype(np.float64)).astype(np.float32)
         line["adjoint"] = {
             "fc_mpr_84": adjoint_leg("MPR 84 nodes, 20k Heun steps, FC of native BOLD vs hidden-parameter FC; "
-                                     "grads of G and 84 eta", c4, [Loss(L.LOSS_FC, fc_t, 2) for _ in c4], 500,
+                                     "grads of G and 84 eta", c4, [Loss(L.LOSS_FC, fc_t, 2) for _ in c4], 0,
                                      1 + c4[0].n_nodes),
         }
         del c4

@@ -33,6 +33,7 @@ struct alignas(16) AdjInst {
     double* bstate;             // [n_win][N][4] Balloon-Windkessel states (FP64)
     double* bold;               // [n_win][N] native BOLD samples (FP64)
     double* dsig;               // [n_win][N] dL/dBOLD
+    double* bscan;              // [16][N][20] chunk maps of the BOLD adjoint
     float* seedw;               // [n_win][N] dL/d(window input) / count
     const float* target;        // FC [N][N], BOLD [n_win][N] or order parameter [T]
     double* fc_work;            // [3N + 2N^2]

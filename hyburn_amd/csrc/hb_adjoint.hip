@@ -413,6 +413,9 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
     const int N = I.n_nodes, T = I.n_steps, Hd = I.ring_depth, Sg = I.seg_len;
     const int tid = threadIdx.x, BS = blockDim.x;
     const int NNP = N * NP;
+    // one segment: the forward keeps the whole trajectory and the reverse
+    // reads it instead of recomputing
+    const bool full_tape = I.n_seg == 1;
     float* ring = I.ring_lds ? lds : I.ring;
     const StepCtx S = make_ctx(I);
     float cc[kCC];
@@ -431,7 +434,10 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
         for (int i = 0; i < kMC; ++i) mcn[q][i] = (I.node_mc && n < N) ? I.node_mc[(size_t)n * kMC + i] : I.mc[i];
         if (n < N) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) I.ckpt[n * NV + v] = x[q][v];
+            for (int v = 0; v < NV; ++v) {
+                I.ckpt[n * NV + v] = x[q][v];
+                if (full_tape) I.tape[n * NV + v] = x[q][v];
+            }
         }
     }
     const int P = I.bold_period;
@@ -526,6 +532,11 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
                 float* ck = I.ckpt + ((size_t)((t + 1) / Sg) * N + n) * NV;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) ck[v] = x[q][v];
+            }
+            if (full_tape) {
+                float* tp = I.tape + ((size_t)(t + 1) * N + n) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) tp[v] = x[q][v];
             }
         }
         if (P > 0 && (bcount == P || t == T - 1)) {
@@ -940,12 +951,22 @@ __global__ __launch_bounds__(256) void k_loss(const AdjInst* __restrict__ insts)
 }
 
 // ------------------------------------------------------- BOLD reverse ------
-// Adjoint of bold_step_d per node, windows last to first (FP64). The saved
-// states, window inputs and dL/dBOLD of the next kBr windows load while the
-// current ones are differentiated.
-constexpr int kBr = 4;
+// Adjoint of bold_step_d per node, windows last to first (FP64). For window k
+// the adjoint state a = (ls, lf, lv, lq) maps affinely,
+//   a <- B_k D_k (a + g_k),  D_k = clamp masks, g_k = dL/dBOLD_k terms,
+// with B_k, D_k, g_k functions of the saved forward state only, and the seed
+// dL/d(window input) is linear in the incoming ls. The kernel splits each
+// node's windows into C chunks (one thread per node and chunk): every chunk
+// composes its affine map (4x4 matrix and offset), each thread combines the
+// maps of the chunks after its own to get its incoming state, then reruns its
+// chunk from that state and writes the seeds.
+constexpr int kBrChunks = 16, kBrMinChunk = 16;
 struct BoldRevIn { double s, f, v, q, dsg; float z; };
-struct BoldAdj { double ls, lf, lv, lq; };
+struct BoldCoef {
+    bool ms, mf, mv, mq, mfo, mvo;               // clamp masks
+    double gv, gq;                               // dL/dBOLD terms added to lv, lq
+    double css, csf, cfs, cfv, cfq, cvv, cvq, cqq, cz;
+};
 __device__ __forceinline__ BoldRevIn bold_rev_load(const AdjInst& I, int k, int n) {
     BoldRevIn b;
     const int N = I.n_nodes;
@@ -955,11 +976,8 @@ __device__ __forceinline__ BoldRevIn bold_rev_load(const AdjInst& I, int k, int 
     b.dsg = I.dsig[(size_t)k * N + n];
     return b;
 }
-// Adjoint of window k of node n: a = dL/d(state leaving the window) on entry,
-// dL/d(state entering it) on return; stores dL/d(window input) / count.
-__device__ __forceinline__ void bold_rev_window(const AdjInst& I, BoldAdj& a, int k, int n, const BoldRevIn& in,
-                                                double ln1e) {
-    const int N = I.n_nodes, nw = I.n_win, P = I.bold_period, T = I.n_steps;
+__device__ __forceinline__ BoldCoef bold_rev_coef(const AdjInst& I, int k, const BoldRevIn& in, double ln1e) {
+    const int nw = I.n_win, P = I.bold_period, T = I.n_steps;
     const int cnt = (k == nw - 1) ? T - k * P : P;
     const double dts = (I.dt_ms * (double)cnt) / 1000.0;
     const double s = in.s, f = in.f, v = in.v, q = in.q;
@@ -975,64 +993,112 @@ __device__ __forceinline__ void bold_rev_window(const AdjInst& I, BoldAdj& a, in
     const double sn = s + dts * ds, fn = fs + dts * s, vn = vs + dts * dv, qn = q + dts * dq;
     const double v1 = vn > kBwLo ? vn : kBwLo;
     const double q1 = qn > 0.0 ? qn : 0.0;
+    BoldCoef c;
     // signal 4 * (7 (1 - q') + 2 (1 - q'/v') + 2 (1 - v'))
-    const double dsg = in.dsg;
-    a.lq += dsg * 4.0 * (-7.0 - 2.0 / v1);
-    a.lv += (v1 > kBwLo) ? dsg * 4.0 * (2.0 * q1 / (v1 * v1) - 2.0) : 0.0;
-    const double lsn = sn > 0.0 ? a.ls : 0.0;
-    const double lfn = fn > kBwLo ? a.lf : 0.0;
-    const double lvn = vn > kBwLo ? a.lv : 0.0;
-    const double lqn = qn > 0.0 ? a.lq : 0.0;
+    c.gq = in.dsg * 4.0 * (-7.0 - 2.0 / v1);
+    c.gv = (v1 > kBwLo) ? in.dsg * 4.0 * (2.0 * q1 / (v1 * v1) - 2.0) : 0.0;
+    c.ms = sn > 0.0; c.mf = fn > kBwLo; c.mv = vn > kBwLo; c.mq = qn > 0.0;
+    c.mfo = f > kBwLo; c.mvo = v > kBwLo;
     const double dfe = (1.0 - ef) + ef * ln1e / fs;            // d(fs (1 - ef)) / dfs
-    const double lfs = lsn * dts * (-kBwGamma) + lfn + lvn * dts * kBwTauInv +
-                       lqn * dts * kBwTauInv * (dfe / kBwE0);
     const double dpv = kBwIalpha * pv / vs;
     const double dpvq = (kBwIalpha - 1.0) * pv / (vs * vs);    // d(pv / vs) / dvs
-    const double lvs = lvn * (1.0 - dts * kBwTauInv * dpv) - lqn * dts * kBwTauInv * q * dpvq;
-    const double nls = lsn * (1.0 - dts * kBwKappa) + lfn * dts;
-    const double nlq = lqn * (1.0 - dts * kBwTauInv * (pv / vs));
-    const double lz = zr > 0.0 ? lsn * dts : 0.0;
-    a.ls = nls;
-    a.lf = f > kBwLo ? lfs : 0.0;
-    a.lv = v > kBwLo ? lvs : 0.0;
-    a.lq = nlq;
-    I.seedw[(size_t)k * N + n] = (float)(lz / (double)cnt);
+    c.css = 1.0 - dts * kBwKappa;
+    c.csf = dts;
+    c.cfs = -kBwGamma * dts;
+    c.cfv = kBwTauInv * dts;
+    c.cfq = kBwTauInv * dts * (dfe / kBwE0);
+    c.cvv = 1.0 - dts * kBwTauInv * dpv;
+    c.cvq = -(dts * kBwTauInv * q * dpvq);
+    c.cqq = 1.0 - dts * kBwTauInv * (pv / vs);
+    c.cz = zr > 0.0 ? dts / (double)cnt : 0.0;
+    return c;
 }
-__global__ __launch_bounds__(256) void k_bold_reverse(const AdjInst* __restrict__ insts) {
+// a <- B D (a + g) (offset) or B D a (linear part)
+__device__ __forceinline__ void bold_rev_apply(const BoldCoef& c, double (&a)[4], bool offset) {
+    const double lsn = c.ms ? a[0] : 0.0;
+    const double lfn = c.mf ? a[1] : 0.0;
+    const double lvn = c.mv ? (offset ? a[2] + c.gv : a[2]) : 0.0;
+    const double lqn = c.mq ? (offset ? a[3] + c.gq : a[3]) : 0.0;
+    a[0] = c.css * lsn + c.csf * lfn;
+    a[1] = c.mfo ? c.cfs * lsn + lfn + c.cfv * lvn + c.cfq * lqn : 0.0;
+    a[2] = c.mvo ? c.cvv * lvn + c.cvq * lqn : 0.0;
+    a[3] = c.cqq * lqn;
+}
+// One window with its seed: dL/d(window input) / count = cz * (masked ls).
+__device__ __forceinline__ void bold_rev_window(const AdjInst& I, double (&a)[4], int k, int n, const BoldRevIn& in,
+                                                double ln1e) {
+    const BoldCoef c = bold_rev_coef(I, k, in, ln1e);
+    I.seedw[(size_t)k * I.n_nodes + n] = (float)(c.cz * (c.ms ? a[0] : 0.0));
+    bold_rev_apply(c, a, true);
+}
+
+__global__ __launch_bounds__(1024) void k_bold_reverse(const AdjInst* __restrict__ insts) {
     const AdjInst& I = insts[blockIdx.x];
     if (I.bold_period <= 0 || I.loss_kind == HB_LOSS_ORDER) return;
-    const int N = I.n_nodes, nw = I.n_win;
+    const int N = I.n_nodes, nw = I.n_win, BS = blockDim.x;
     const double ln1e = log(1.0 - kBwE0);
-    for (int n = threadIdx.x; n < N; n += blockDim.x) {
-        BoldAdj la{0.0, 0.0, 0.0, 0.0};
-        BoldRevIn ba[kBr], bb[kBr];
-        auto fetch = [&](BoldRevIn (&b)[kBr], int k0) {   // windows k0, k0 - 1, ...
-#pragma unroll
-            for (int m = 0; m < kBr; ++m) {
-                const int k = k0 - m;
-                if (k >= 0) {
-                    b[m] = bold_rev_load(I, k, n);
-                } else {
-                    b[m].s = 0.0; b[m].f = 1.0; b[m].v = 1.0; b[m].q = 1.0; b[m].z = 0.0f; b[m].dsg = 0.0;
-                }
-            }
-        };
-        auto run = [&](const BoldRevIn (&b)[kBr], int k0) {
-#pragma unroll
-            for (int m = 0; m < kBr; ++m) {
-                const int k = k0 - m;
-                if (k < 0) break;
-                bold_rev_window(I, la, k, n, b[m], ln1e);
-            }
-        };
-        fetch(ba, nw - 1);
-        for (int k0 = nw - 1; k0 >= 0; k0 -= 2 * kBr) {
-            fetch(bb, k0 - kBr);
-            run(ba, k0);
-            fetch(ba, k0 - 2 * kBr);
-            run(bb, k0 - kBr);
+    const int C = max(1, min(min(kBrChunks, BS / N), nw / kBrMinChunk));
+    // one window of look-ahead: the next window's inputs load while the
+    // current one is differentiated
+    auto sweep = [&](double (&a)[4], int n, int khi, int klo) {
+        BoldRevIn nx = bold_rev_load(I, khi, n);
+        for (int k = khi; k >= klo; --k) {
+            const BoldRevIn cu = nx;
+            if (k > klo) nx = bold_rev_load(I, k - 1, n);
+            bold_rev_window(I, a, k, n, cu, ln1e);
         }
+    };
+    if (C == 1) {   // whole nodes per thread
+        for (int n = threadIdx.x; n < N; n += BS) {
+            double a[4] = {0.0, 0.0, 0.0, 0.0};
+            sweep(a, n, nw - 1, 0);
+        }
+        return;
     }
+    const int Lc = (nw + C - 1) / C;
+    const int n = threadIdx.x % N, c = threadIdx.x / N;
+    const bool act = c < C;
+    const int klo = c * Lc, khi = min(nw, klo + Lc) - 1;
+    double* map = I.bscan;   // [C][N][20]: columns of the 4x4 matrix, then the offset
+    // ---- chunk maps (chunk 0's is never needed); one window of look-ahead
+    if (act && c > 0 && klo <= khi) {
+        double M[4][4], o[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) M[j][r] = j == r ? 1.0 : 0.0;
+        BoldRevIn nx = bold_rev_load(I, khi, n);
+        for (int k = khi; k >= klo; --k) {
+            const BoldRevIn cu = nx;
+            if (k > klo) nx = bold_rev_load(I, k - 1, n);
+            const BoldCoef cf = bold_rev_coef(I, k, cu, ln1e);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bold_rev_apply(cf, M[j], false);
+            bold_rev_apply(cf, o, true);
+        }
+        double* dst = map + ((size_t)c * N + n) * 20;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[j * 4 + r] = M[j][r];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dst[16 + r] = o[r];
+    }
+    __syncthreads();
+    if (!act || klo > khi) return;
+    // ---- incoming state: the maps of chunks C-1 .. c+1 applied to zero
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int cc = C - 1; cc > c; --cc) {
+        const double* src = map + ((size_t)cc * N + n) * 20;
+        double na[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            na[r] = src[16 + r] + src[r] * a[0] + src[4 + r] * a[1] + src[8 + r] * a[2] + src[12 + r] * a[3];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = na[r];
+    }
+    // ---- rerun the chunk from its incoming state, writing the seeds
+    sweep(a, n, khi, klo);
 }
 
 // ------------------------------------------------------- reverse ----------
@@ -1075,14 +1141,15 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
 
     for (int seg = nseg - 1; seg >= 0; --seg) {
         const int t0 = seg * Sg, t1 = min(T, t0 + Sg);
-        // ---- recompute states t0..t1 from the checkpoint and the coupling log.
-        // Rolled step loop: slot 0 of `cur` is the current step's log row, the
-        // window shifts down each step and `nxt` (the following kG rows) loads
-        // kG steps ahead of use.
+        // ---- recompute states t0..t1 from the checkpoint and the coupling log
+        // (with one segment the forward wrote the whole trajectory). Rolled
+        // step loop: slot 0 of `cur` is the current step's log row, the window
+        // shifts down each step and `nxt` (the following kG rows) loads kG
+        // steps ahead of use.
 #pragma unroll
         for (int q = 0; q < NPT; ++q) {
             const int n = tid + q * BS;
-            if (n >= N) continue;
+            if (n >= N || nseg == 1) continue;
             float x[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) { x[v] = I.ckpt[((size_t)seg * N + n) * NV + v]; I.tape[(size_t)n * NV + v] = x[v]; }
@@ -1417,6 +1484,7 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
         bold_helper_forward(*Ip, tid - (BS >> 1));
         return;
     }
+    const bool full_tape = V.nseg == 1;
     float x[NV], mc[kMC];
 #pragma unroll
     for (int v = 0; v < NV; ++v) x[v] = own ? V.x0[tid * NV + v] : 0.0f;
@@ -1424,7 +1492,10 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
     for (int i = 0; i < kMC; ++i) mc[i] = (V.node_mc && own) ? V.node_mc[(size_t)tid * kMC + i] : uf32(Ip->mc[i]);
     if (own) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) V.ckpt[tid * NV + v] = x[v];
+        for (int v = 0; v < NV; ++v) {
+            V.ckpt[tid * NV + v] = x[v];
+            if (full_tape) V.tape[tid * NV + v] = x[v];
+        }
     }
     // CSR row of this node in registers: source row base, delay, weight
     constexpr int kReg = 32;
@@ -1481,6 +1552,11 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
                 gp<float> ck = V.ckpt + ((size_t)((t + 1) / V.Sg) * N + tid) * NV;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) ck[v] = x[v];
+            }
+            if (full_tape) {   // one segment: the whole trajectory, read by the reverse
+                gp<float> tp = V.tape + ((size_t)(t + 1) * N + tid) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) tp[v] = x[v];
             }
         }
         ++bcount;
@@ -1693,10 +1769,11 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
 
     for (int seg = V.nseg - 1; seg >= 0; --seg) {
         const int t0s = seg * Sg, t1s = min(T, t0s + Sg);
-        // ---- recompute states t0s..t1s from the checkpoint and the coupling log;
-        // the log and noise rows of the next kK steps are loaded while the
-        // current kK steps integrate (double buffer)
-        if (own) {
+        // ---- recompute states t0s..t1s from the checkpoint and the coupling log
+        // (with one segment the forward wrote the whole trajectory); the log
+        // and noise rows of the next kK steps are loaded while the current kK
+        // steps integrate (double buffer)
+        if (own && V.nseg > 1) {
             float x[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) { x[v] = V.ckpt[((size_t)seg * N + tid) * NV + v]; V.tape[(size_t)tid * NV + v] = x[v]; }
@@ -1971,7 +2048,7 @@ hipError_t launch_adjoint_stage(int stage, const AdjLaunch& L, hipStream_t s) {
         hipLaunchKernelGGL(adj::k_bold_forward, dim3(L.n), dim3(256), 0, s, d);
         return hipGetLastError();
     case 2: hipLaunchKernelGGL(adj::k_loss, dim3(L.n), dim3(256), adj::kFcLds, s, d); return hipGetLastError();
-    case 3: hipLaunchKernelGGL(adj::k_bold_reverse, dim3(L.n), dim3(256), 0, s, d); return hipGetLastError();
+    case 3: hipLaunchKernelGGL(adj::k_bold_reverse, dim3(L.n), dim3(1024), 0, s, d); return hipGetLastError();
     default: {
         const void* k = L.blocked ? adj::kernel_b(L.model, true, L.np, L.lring) : adj::kernel_for(L.model, true, L.np, L.npt);
         if (!k) return hipErrorInvalidValue;

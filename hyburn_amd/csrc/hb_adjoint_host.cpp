@@ -159,6 +159,21 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
     P->spec.assign(spec, spec + n);
     P->inst.assign(inst, inst + n);
 
+    // ---- checkpointing: with checkpoint_every = 0 the plan keeps every
+    // instance's whole trajectory (one segment: the forward writes it and the
+    // reverse reads it, no recompute) when all of them fit a quarter of the
+    // free device memory; otherwise segments of ceil(sqrt(n_steps)) steps
+    const int ce = opts ? opts->checkpoint_every : 0;
+    bool keep_all = false;
+    if (ce == 0) {
+        size_t free_b = 0, total_b = 0;
+        HB_HIP(hipMemGetInfo(&free_b, &total_b));
+        double tape_b = 0.0;
+        for (int i = 0; i < n; ++i)
+            tape_b += ((double)inst[i].n_steps + 1.0) * inst[i].n_nodes * model_nvar(inst[i].model) * 4.0;
+        keep_all = tape_b <= 0.25 * (double)free_b;
+    }
+
     // ---- shapes and grouping
     struct Shape { int nv, nc, np, npt, block, Hd, S, nseg, nwin, nnz; bool kur, ring_lds, blocked; size_t ring_b; };
     constexpr int kBlock = 8;   // steps per delay block of the blocked kernels
@@ -184,8 +199,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         // delay-blocked kernels: one barrier per kBlock steps when every delay >= kBlock
         s.blocked = s.nnz > 0 && dmin >= kBlock && I.n_nodes <= 512 && !(opts && (opts->flags & HB_FLAG_NO_BLOCKING));
         s.Hd = s.blocked ? dmax + kBlock + 1 : dmax + 2;
-        const int ce = opts ? opts->checkpoint_every : 0;
-        s.S = ce > 0 ? ce : std::max(1, (int)std::ceil(std::sqrt((double)I.n_steps)));
+        s.S = ce > 0 ? ce : keep_all ? I.n_steps : std::max(1, (int)std::ceil(std::sqrt((double)I.n_steps)));
         s.S = std::min(s.S, I.n_steps);
         s.nseg = (I.n_steps + s.S - 1) / s.S;
         s.nwin = I.bold_period > 0 ? (I.n_steps + I.bold_period - 1) / I.bold_period : 0;
@@ -214,7 +228,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
     // ---- arena layout
     struct Off {
         size_t x0, node_mc, node_jac, indptr, edges, indptr_t, edges_t, noise, ring, glog, ckpt, tape,
-            zlog, bstate, bold, dsig, seedw, target, fc, ord, fin, loss, gcc, gp, gx0;
+            zlog, bstate, bold, dsig, bscan, seedw, target, fc, ord, fin, loss, gcc, gp, gx0;
         bool has_node_mc, has_jac, has_noise, has_ring;
     };
     std::vector<Off> off(n);
@@ -248,6 +262,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         o.bold = A.take(nw * N * 8);
         o.dsig = A.take(nw * N * 8);
         o.seedw = A.take(nw * N * 4);
+        o.bscan = nw > 0 ? A.take((size_t)16 * N * 20 * 8) : 0;
         const size_t tgt_n = Sp.loss_kind == HB_LOSS_FC ? N * N : Sp.loss_kind == HB_LOSS_BOLD_MSE ? nw * N : T;
         o.target = A.take(tgt_n * 4);
         o.fc = Sp.loss_kind == HB_LOSS_FC ? A.take((3 * N + 2 * N * N) * 8) : 0;
@@ -398,6 +413,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         D.bstate = (double*)(base + o.bstate);
         D.bold = (double*)(base + o.bold);
         D.dsig = (double*)(base + o.dsig);
+        D.bscan = s.nwin > 0 ? (double*)(base + o.bscan) : nullptr;
         D.seedw = (float*)(base + o.seedw);
         D.target = (const float*)(base + o.target);
         D.fc_work = Sp.loss_kind == HB_LOSS_FC ? (double*)(base + o.fc) : nullptr;

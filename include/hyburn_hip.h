@@ -228,7 +228,9 @@ typedef struct {
 
 typedef struct {
     int32_t device;
-    int32_t checkpoint_every; /* steps per checkpoint segment; 0: ceil(sqrt(n_steps)) */
+    int32_t checkpoint_every; /* steps per checkpoint segment; 0: the whole trajectory kept in HBM
+                                 (no recompute) when every instance's fits a quarter of the free
+                                 memory, else ceil(sqrt(n_steps)); >= n_steps: one segment, kept */
     uint32_t flags;           /* HB_FLAG_NO_BLOCKING: one barrier per step even when min delay >= 8;
                                  HB_FLAG_SEPARATE_BOLD: BOLD monitor kernels instead of helper waves */
 } hb_adjoint_opts;
