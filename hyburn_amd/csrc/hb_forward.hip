@@ -563,7 +563,10 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
 // edge's K values are one contiguous run of the node-major ring
 // [np][N][Hd] (Hd >= max_delay + K + 1 so in-block writes never alias reads).
 // Per-step arithmetic and the per-row summation order are unchanged.
-template <int M, int NP, bool LRING, int K, int IC>
+// CF >= 0 fixes the coupling function at compile time (-1: read per
+// instance); OUTS = false drops the trajectory and BOLD outputs. Both remove
+// per-step uniform branches from the step loop of the common sweep case.
+template <int M, int NP, bool LRING, int K, int IC, int CF, bool OUTS>
 __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float* lds) {
     const DevInst* __restrict__ Ia = I;
     const View V = make_view(I);
@@ -575,7 +578,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     using RingPtr = std::conditional_t<LRING, float*, gp<float>>;
     RingPtr ring;
     if constexpr (LRING) ring = lds; else ring = V.ring;
-    const int cfn = V.cfn;
+    const int cfn = CF >= 0 ? CF : V.cfn;
     const bool cpu_form = V.cpu_form != 0, classic = V.sjr_classic != 0;
     StepCtx S;
     S.dt = V.dt; S.dth = V.dt_half; S.dt6 = V.dt_sixth; S.integ = V.integrator;
@@ -588,9 +591,9 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
 #pragma unroll
     for (int v = 0; v < NV; ++v) nsc[v] = Ia->nscale[v];
     const gp<const float> noise = V.noise;
-    const gp<float> traj = V.traj;
+    const gp<float> traj = OUTS ? V.traj : gp<float>(nullptr);
     const int64_t tstride = V.traj_tstride;
-    const int bper = V.bold_period;
+    const int bper = OUTS ? V.bold_period : 0;
 
     for (int i = tid; i < NP * NHd; i += BS) ring[i] = 0.0f;   // per-edge: exact-zero fill
 
@@ -817,9 +820,17 @@ __device__ __forceinline__ void run_any(const DevInst* __restrict__ I, float* ld
         const int bk = __builtin_amdgcn_readfirstlane(I->blocked_k);
         if (bk == kBlockSteps) {
             const int ic = __builtin_amdgcn_readfirstlane(integ_class(I->integrator));
-            if (ic == 0) run_blocked<M, NP, LRING, kBlockSteps, 0>(I, lds);
-            else if (ic == 1) run_blocked<M, NP, LRING, kBlockSteps, 1>(I, lds);
-            else run_blocked<M, NP, LRING, kBlockSteps, 2>(I, lds);
+            const bool lin = __builtin_amdgcn_readfirstlane(I->cfn) == HB_CF_LINEAR && I->traj == nullptr &&
+                             __builtin_amdgcn_readfirstlane(I->bold_period) <= 0;
+            if (lin) {   // Linear coupling, tavg / final state only
+                if (ic == 0) run_blocked<M, NP, LRING, kBlockSteps, 0, HB_CF_LINEAR, false>(I, lds);
+                else if (ic == 1) run_blocked<M, NP, LRING, kBlockSteps, 1, HB_CF_LINEAR, false>(I, lds);
+                else run_blocked<M, NP, LRING, kBlockSteps, 2, HB_CF_LINEAR, false>(I, lds);
+            } else {
+                if (ic == 0) run_blocked<M, NP, LRING, kBlockSteps, 0, -1, true>(I, lds);
+                else if (ic == 1) run_blocked<M, NP, LRING, kBlockSteps, 1, -1, true>(I, lds);
+                else run_blocked<M, NP, LRING, kBlockSteps, 2, -1, true>(I, lds);
+            }
             return;
         }
     }
