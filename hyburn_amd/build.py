@@ -29,9 +29,9 @@ DEVICE_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 
-# hb_forward.hip is compiled once per kernel unit (7 single-model units, 2
-# mixed-model units) and mode, so the units build in parallel.
-FORWARD_UNITS = 9
+# hb_forward.hip is compiled once per kernel unit (one per model and pre
+# channel count) and mode, so the units build in parallel.
+FORWARD_UNITS = 7
 UNITS = [
     # (source, object, extra flags, device?)
     *[("hb_forward.hip", f"hb_forward_fast_u{u}.o", ["-DHB_EXACT=0", f"-DHB_UNIT={u}"], True)

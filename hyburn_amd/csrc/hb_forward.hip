@@ -814,23 +814,27 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     __syncthreads();
 }
 
+template <int M, int NP, bool LRING, int CF, bool OUTS>
+__device__ __forceinline__ void run_blocked_ic(const DevInst* __restrict__ I, float* lds, int ic) {
+    if (ic == 0) run_blocked<M, NP, LRING, kBlockSteps, 0, CF, OUTS>(I, lds);
+    else if (ic == 1) run_blocked<M, NP, LRING, kBlockSteps, 1, CF, OUTS>(I, lds);
+    else run_blocked<M, NP, LRING, kBlockSteps, 2, CF, OUTS>(I, lds);
+}
+
 template <int M, int NP, bool LRING, bool MULTI>
 __device__ __forceinline__ void run_any(const DevInst* __restrict__ I, float* lds) {
     if constexpr (!MULTI) {
         const int bk = __builtin_amdgcn_readfirstlane(I->blocked_k);
         if (bk == kBlockSteps) {
             const int ic = __builtin_amdgcn_readfirstlane(integ_class(I->integrator));
-            const bool lin = __builtin_amdgcn_readfirstlane(I->cfn) == HB_CF_LINEAR && I->traj == nullptr &&
-                             __builtin_amdgcn_readfirstlane(I->bold_period) <= 0;
-            if (lin) {   // Linear coupling, tavg / final state only
-                if (ic == 0) run_blocked<M, NP, LRING, kBlockSteps, 0, HB_CF_LINEAR, false>(I, lds);
-                else if (ic == 1) run_blocked<M, NP, LRING, kBlockSteps, 1, HB_CF_LINEAR, false>(I, lds);
-                else run_blocked<M, NP, LRING, kBlockSteps, 2, HB_CF_LINEAR, false>(I, lds);
-            } else {
-                if (ic == 0) run_blocked<M, NP, LRING, kBlockSteps, 0, -1, true>(I, lds);
-                else if (ic == 1) run_blocked<M, NP, LRING, kBlockSteps, 1, -1, true>(I, lds);
-                else run_blocked<M, NP, LRING, kBlockSteps, 2, -1, true>(I, lds);
-            }
+            const int cfn = __builtin_amdgcn_readfirstlane(I->cfn);
+            const bool outs = I->traj != nullptr || __builtin_amdgcn_readfirstlane(I->bold_period) > 0;
+            // compile-time variants: Linear with and without outputs, legacy
+            // sigmoidal (Jansen-Rit) without outputs; anything else generic
+            if (cfn == HB_CF_LINEAR && !outs) run_blocked_ic<M, NP, LRING, HB_CF_LINEAR, false>(I, lds, ic);
+            else if (cfn == HB_CF_LINEAR) run_blocked_ic<M, NP, LRING, HB_CF_LINEAR, true>(I, lds, ic);
+            else if (cfn == HB_CF_SJR && !outs) run_blocked_ic<M, NP, LRING, HB_CF_SJR, false>(I, lds, ic);
+            else run_blocked_ic<M, NP, LRING, -1, true>(I, lds, ic);
             return;
         }
     }
@@ -868,28 +872,6 @@ __global__ __launch_bounds__(512) void k_single(LaunchDesc a) {
     }
 }
 
-template <bool LRING, bool MULTI>
-__global__ __launch_bounds__(512) void k_mixed(LaunchDesc a) {
-    extern __shared__ float lds[];
-    for (int it = 0;; ++it) {
-        const int w = next_work(a, it);
-        if (w >= a.n_work) break;
-        const DevInst* I = uniform_inst(a.insts + a.order[w]);
-        const int key = __builtin_amdgcn_readfirstlane(I->model * 4 + I->np);
-        switch (key) {
-        case HB_G2DO * 4 + 1: run_any<HB_G2DO, 1, LRING, MULTI>(I, lds); break;
-        case HB_MPR * 4 + 2: run_any<HB_MPR, 2, LRING, MULTI>(I, lds); break;
-        case HB_MPR * 4 + 1: run_any<HB_MPR, 1, LRING, MULTI>(I, lds); break;
-        case HB_JANSEN_RIT * 4 + 1: run_any<HB_JANSEN_RIT, 1, LRING, MULTI>(I, lds); break;
-        case HB_WILSON_COWAN * 4 + 1: run_any<HB_WILSON_COWAN, 1, LRING, MULTI>(I, lds); break;
-        case HB_KURAMOTO * 4 + 1: run_any<HB_KURAMOTO, 1, LRING, MULTI>(I, lds); break;
-        case HB_KURAMOTO * 4 + 2: run_any<HB_KURAMOTO, 2, LRING, MULTI>(I, lds); break;
-        case HB_RWW * 4 + 1: run_any<HB_RWW, 1, LRING, MULTI>(I, lds); break;
-        default: break;
-        }
-    }
-}
-
 template <int M, int NP>
 static const void* kptr_single(bool lring, bool multi) {
     if (lring) return multi ? (const void*)k_single<M, NP, true, true> : (const void*)k_single<M, NP, true, false>;
@@ -897,7 +879,7 @@ static const void* kptr_single(bool lring, bool multi) {
 }
 
 // This translation unit's kernels: HB_UNIT 0..6 = one (model, pre channels)
-// pair, 7 = mixed-model kernels with LDS rings, 8 = mixed with HBM rings.
+// pair.
 #ifndef HB_UNIT
 #define HB_UNIT 0
 #endif
@@ -917,12 +899,8 @@ static const void* unit_kernel(bool lring, bool multi, int np) {
     return kptr_single<HB_KURAMOTO, 1>(lring, multi);
 #elif HB_UNIT == 5
     return kptr_single<HB_KURAMOTO, 2>(lring, multi);
-#elif HB_UNIT == 6
-    return kptr_single<HB_RWW, 1>(lring, multi);
-#elif HB_UNIT == 7
-    return lring ? (multi ? (const void*)k_mixed<true, true> : (const void*)k_mixed<true, false>) : nullptr;
 #else
-    return !lring ? (multi ? (const void*)k_mixed<false, true> : (const void*)k_mixed<false, false>) : nullptr;
+    return kptr_single<HB_RWW, 1>(lring, multi);
 #endif
 }
 

@@ -8,23 +8,22 @@ namespace hb {
     hipError_t fwd_unit_fast_##n(int op, const LaunchDesc* d, hipStream_t s, int arg, int* out); \
     hipError_t fwd_unit_exact_##n(int op, const LaunchDesc* d, hipStream_t s, int arg, int* out);
 HB_DECL_UNIT(0) HB_DECL_UNIT(1) HB_DECL_UNIT(2) HB_DECL_UNIT(3) HB_DECL_UNIT(4)
-HB_DECL_UNIT(5) HB_DECL_UNIT(6) HB_DECL_UNIT(7) HB_DECL_UNIT(8)
+HB_DECL_UNIT(5) HB_DECL_UNIT(6)
 #undef HB_DECL_UNIT
 
 namespace {
 
 using UnitFn = hipError_t (*)(int, const LaunchDesc*, hipStream_t, int, int*);
-constexpr int kUnits = 9;
+constexpr int kUnits = 7;
 const UnitFn kFast[kUnits] = {fwd_unit_fast_0, fwd_unit_fast_1, fwd_unit_fast_2, fwd_unit_fast_3, fwd_unit_fast_4,
-                              fwd_unit_fast_5, fwd_unit_fast_6, fwd_unit_fast_7, fwd_unit_fast_8};
+                              fwd_unit_fast_5, fwd_unit_fast_6};
 const UnitFn kExact[kUnits] = {fwd_unit_exact_0, fwd_unit_exact_1, fwd_unit_exact_2, fwd_unit_exact_3,
-                               fwd_unit_exact_4, fwd_unit_exact_5, fwd_unit_exact_6, fwd_unit_exact_7,
-                               fwd_unit_exact_8};
+                               fwd_unit_exact_4, fwd_unit_exact_5, fwd_unit_exact_6};
 
-// Unit of a launch: one per instantiated (model, pre channels) pair; mixed
-// launches by ring placement. -1: no kernel for this combination.
+// Unit of a launch: one per instantiated (model, pre channels) pair. -1: no
+// kernel for this combination (every launch group holds a single model).
 int unit_of(const LaunchDesc& d) {
-    if (d.model < 0) return d.lds_ring ? 7 : 8;
+    if (d.model < 0) return -1;
     switch (d.model * 4 + d.np) {
     case HB_G2DO * 4 + 1: return 0;
     case HB_MPR * 4 + 2: return 1;

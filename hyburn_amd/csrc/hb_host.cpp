@@ -246,23 +246,37 @@ namespace {
 // Launch every group: one group on the plan stream, several as parallel
 // branches (fork from the plan stream, one side stream each, join back), so
 // groups of different launch shapes share the chip instead of queueing.
+thread_local std::string g_launch_err;   // the step of launch_groups that failed
+
 hipError_t launch_groups(hb_plan* P, bool exact) {
     auto go = [&](const LaunchDesc& L, hipStream_t st) {
         return exact ? launch_forward_exact(L, st) : launch_forward_fast(L, st);
     };
+    g_launch_err.clear();
     if (P->launches.size() <= 1 || P->side.empty()) {
         for (auto& L : P->launches) {
             const hipError_t e = go(L, P->stream);
-            if (e != hipSuccess) return e;
+            if (e != hipSuccess) {
+                g_launch_err = "kernel launch (model " + std::to_string(L.model) + ", np " + std::to_string(L.np) +
+                               ", block " + std::to_string(L.block) + ", lds " + std::to_string(L.lds_bytes) + ")";
+                return e;
+            }
         }
         return hipSuccess;
     }
     hipError_t e = hipEventRecord(P->fork, P->stream);
+    if (e != hipSuccess) g_launch_err = "fork record";
     for (size_t k = 0; k < P->launches.size() && e == hipSuccess; ++k) {
+        const LaunchDesc& L = P->launches[k];
+        const char* step = "branch wait";
         e = hipStreamWaitEvent(P->side[k], P->fork, 0);
-        if (e == hipSuccess) e = go(P->launches[k], P->side[k]);
-        if (e == hipSuccess) e = hipEventRecord(P->join[k], P->side[k]);
-        if (e == hipSuccess) e = hipStreamWaitEvent(P->stream, P->join[k], 0);
+        if (e == hipSuccess) { step = "kernel launch"; e = go(L, P->side[k]); }
+        if (e == hipSuccess) { step = "join record"; e = hipEventRecord(P->join[k], P->side[k]); }
+        if (e == hipSuccess) { step = "join wait"; e = hipStreamWaitEvent(P->stream, P->join[k], 0); }
+        if (e != hipSuccess)
+            g_launch_err = std::string(step) + " of group " + std::to_string(k) + " (model " + std::to_string(L.model) +
+                           ", np " + std::to_string(L.np) + ", block " + std::to_string(L.block) + ", grid " +
+                           std::to_string(L.grid) + ", lds " + std::to_string(L.lds_bytes) + ")";
     }
     return e;
 }
@@ -763,9 +777,12 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
     if (!jobs_host.empty())
         HB_HIP(hipMemcpy(P->d_jobs, jobs_host.data(), sizeof(NoiseJob) * jobs_host.size(), hipMemcpyHostToDevice));
 
-    // ---- launch groups: (block, lds_ring, multi); cost-descending order inside
-    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
-    for (int i = 0; i < n; ++i) groups[std::make_tuple(sh[i].block, (int)sh[i].lds_ring, (int)sh[i].multi)].push_back(i);
+    // ---- launch groups: (model, pre channels, block, lds_ring, multi), each a
+    // single-model kernel; cost-descending order inside. Groups run as
+    // parallel graph branches, so groups of different models share the chip.
+    std::map<std::tuple<int, int, int, int, int>, std::vector<int>> groups;
+    for (int i = 0; i < n; ++i)
+        groups[std::make_tuple(inst[i].model, sh[i].np, sh[i].block, (int)sh[i].lds_ring, (int)sh[i].multi)].push_back(i);
     std::vector<int32_t> order;
     order.reserve(n);
     int gi = 0;
@@ -778,17 +795,14 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         L.insts = P->d_inst;
         L.order = P->d_order + order.size();
         L.n_work = (int)v.size();
-        L.block = std::get<0>(kv.first);
-        L.lds_ring = std::get<1>(kv.first) != 0;
-        L.multi = std::get<2>(kv.first) != 0;
-        int lds = 0, model = inst[v[0]].model, np = sh[v[0]].np;
-        for (int i : v) {
-            lds = std::max(lds, sh[i].lds_bytes);
-            if (inst[i].model != model || sh[i].np != np) model = -1;
-        }
+        L.block = std::get<2>(kv.first);
+        L.lds_ring = std::get<3>(kv.first) != 0;
+        L.multi = std::get<4>(kv.first) != 0;
+        int lds = 0;
+        for (int i : v) lds = std::max(lds, sh[i].lds_bytes);
         L.lds_bytes = lds;
-        L.model = model;
-        L.np = model >= 0 ? np : 0;
+        L.model = std::get<0>(kv.first);
+        L.np = std::get<1>(kv.first);
         max_lds_all = std::max(max_lds_all, lds);
         for (int i : v) order.push_back(i);
         // persistent when the work exceeds one wave of resident workgroups
@@ -836,7 +850,9 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         e2 = hipStreamEndCapture(P->stream, &P->gr_fwd);
         if (el != hipSuccess || e2 != hipSuccess) {
             hb_plan_destroy(P);
-            return fail(HB_BACKEND_ERROR, std::string("forward graph capture failed: ") + hipGetErrorString(el != hipSuccess ? el : e2));
+            return fail(HB_BACKEND_ERROR, std::string("forward graph capture failed: ") +
+                                              hipGetErrorString(el != hipSuccess ? el : e2) +
+                                              (g_launch_err.empty() ? "" : " at " + g_launch_err));
         }
         HB_HIP(hipGraphInstantiate(&P->g_fwd, P->gr_fwd, nullptr, nullptr, 0));
     }
@@ -855,7 +871,9 @@ int hb_plan_run(hb_plan* P, hb_run_stats* stats) {
     if (P->g_fwd) HB_HIP(hipGraphLaunch(P->g_fwd, P->stream));
     else {
         HB_HIP(launch_zero_i32(P->d_counters, std::min((int)P->launches.size(), 64), P->stream));
-        HB_HIP(launch_groups(P, exact));
+        const hipError_t el = launch_groups(P, exact);
+        if (el != hipSuccess)
+            return fail(HB_BACKEND_ERROR, std::string("forward launch failed: ") + hipGetErrorString(el) + " at " + g_launch_err);
     }
     HB_HIP(hipEventRecord(P->ev[2], P->stream));
     HB_HIP(hipEventSynchronize(P->ev[2]));

@@ -193,8 +193,8 @@ struct LaunchDesc {
     int32_t grid;              // workgroups
     int32_t block;             // threads per workgroup
     int32_t lds_bytes;         // dynamic LDS per workgroup
-    int32_t model;             // single-model launch: model id; -1 mixed
-    int32_t np;                // single-model launch: pre channels
+    int32_t model;             // model id (every launch holds a single model)
+    int32_t np;                // pre channels
     bool lds_ring;             // rings live in LDS
     bool multi;                // nodes per thread > 1 (state table in LDS)
 };
