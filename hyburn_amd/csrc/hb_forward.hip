@@ -42,8 +42,10 @@ namespace HB_NS {
 // the rest to the memory path), and the noise prefetch distance in steps.
 constexpr int kMaxRegEdges = 32;
 constexpr int kNoiseAhead = 4;
-// Steps per delay block (needs min delay >= kBlockSteps).
-constexpr int kBlockSteps = 8;
+// Steps per delay block (needs min delay >= kBlockSteps). Deterministic runs
+// whose ring lives in HBM use kLongBlockSteps: each edge's window is then 64
+// contiguous bytes (fewer DRAM segments per step) and barriers halve.
+constexpr int kBlockSteps = 8, kLongBlockSteps = 16;
 
 #include "hb_model.h"
 
@@ -707,12 +709,14 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         __syncthreads();
         const int kb = min(K, T - t0);
         // the block's noise, issued before the gather so its latency overlaps it
-        float zb[K][NV];
+        // (the long block serves deterministic runs only: no noise registers)
+        constexpr int KZ = K == kBlockSteps ? K : 1;
+        float zb[KZ][NV];
 #pragma unroll
-        for (int k = 0; k < K; ++k)
+        for (int k = 0; k < KZ; ++k)
 #pragma unroll
             for (int v = 0; v < NV; ++v)
-                zb[k][v] = (S.stoch && own && k < kb) ? noise[((size_t)(t0 + k) * N + n) * NV + v] : 0.0f;
+                zb[k][v] = (KZ == K && S.stoch && own && k < kb) ? noise[((size_t)(t0 + k) * N + n) * NV + v] : 0.0f;
         float acc[K][NP];
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -781,6 +785,9 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
             for (int q = 0; q + 1 < K; ++q) {
 #pragma unroll
                 for (int c = 0; c < NP; ++c) acc[q][c] = acc[q + 1][c];
+            }
+#pragma unroll
+            for (int q = 0; q + 1 < KZ; ++q) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) zb[q][v] = zb[q + 1][v];
             }
@@ -814,11 +821,24 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     __syncthreads();
 }
 
-template <int M, int NP, bool LRING, int CF, bool OUTS>
+template <int M, int NP, bool LRING, int K, int CF, bool OUTS>
 __device__ __forceinline__ void run_blocked_ic(const DevInst* __restrict__ I, float* lds, int ic) {
-    if (ic == 0) run_blocked<M, NP, LRING, kBlockSteps, 0, CF, OUTS>(I, lds);
-    else if (ic == 1) run_blocked<M, NP, LRING, kBlockSteps, 1, CF, OUTS>(I, lds);
-    else run_blocked<M, NP, LRING, kBlockSteps, 2, CF, OUTS>(I, lds);
+    if (ic == 0) run_blocked<M, NP, LRING, K, 0, CF, OUTS>(I, lds);
+    else if (ic == 1) run_blocked<M, NP, LRING, K, 1, CF, OUTS>(I, lds);
+    else run_blocked<M, NP, LRING, K, 2, CF, OUTS>(I, lds);
+}
+
+// compile-time variants: Linear with and without outputs, legacy sigmoidal
+// (Jansen-Rit) without outputs; anything else generic
+template <int M, int NP, bool LRING, int K>
+__device__ __forceinline__ void run_blocked_any(const DevInst* __restrict__ I, float* lds) {
+    const int ic = __builtin_amdgcn_readfirstlane(integ_class(I->integrator));
+    const int cfn = __builtin_amdgcn_readfirstlane(I->cfn);
+    const bool outs = I->traj != nullptr || __builtin_amdgcn_readfirstlane(I->bold_period) > 0;
+    if (cfn == HB_CF_LINEAR && !outs) run_blocked_ic<M, NP, LRING, K, HB_CF_LINEAR, false>(I, lds, ic);
+    else if (cfn == HB_CF_LINEAR) run_blocked_ic<M, NP, LRING, K, HB_CF_LINEAR, true>(I, lds, ic);
+    else if (cfn == HB_CF_SJR && !outs) run_blocked_ic<M, NP, LRING, K, HB_CF_SJR, false>(I, lds, ic);
+    else run_blocked_ic<M, NP, LRING, K, -1, true>(I, lds, ic);
 }
 
 template <int M, int NP, bool LRING, bool MULTI>
@@ -826,16 +846,14 @@ __device__ __forceinline__ void run_any(const DevInst* __restrict__ I, float* ld
     if constexpr (!MULTI) {
         const int bk = __builtin_amdgcn_readfirstlane(I->blocked_k);
         if (bk == kBlockSteps) {
-            const int ic = __builtin_amdgcn_readfirstlane(integ_class(I->integrator));
-            const int cfn = __builtin_amdgcn_readfirstlane(I->cfn);
-            const bool outs = I->traj != nullptr || __builtin_amdgcn_readfirstlane(I->bold_period) > 0;
-            // compile-time variants: Linear with and without outputs, legacy
-            // sigmoidal (Jansen-Rit) without outputs; anything else generic
-            if (cfn == HB_CF_LINEAR && !outs) run_blocked_ic<M, NP, LRING, HB_CF_LINEAR, false>(I, lds, ic);
-            else if (cfn == HB_CF_LINEAR) run_blocked_ic<M, NP, LRING, HB_CF_LINEAR, true>(I, lds, ic);
-            else if (cfn == HB_CF_SJR && !outs) run_blocked_ic<M, NP, LRING, HB_CF_SJR, false>(I, lds, ic);
-            else run_blocked_ic<M, NP, LRING, -1, true>(I, lds, ic);
+            run_blocked_any<M, NP, LRING, kBlockSteps>(I, lds);
             return;
+        }
+        if constexpr (!LRING) {
+            if (bk == kLongBlockSteps) {   // deterministic runs with the ring in HBM
+                run_blocked_any<M, NP, LRING, kLongBlockSteps>(I, lds);
+                return;
+            }
         }
     }
     run_instance<M, NP, LRING, MULTI>(I, lds);

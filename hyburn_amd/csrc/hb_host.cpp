@@ -488,10 +488,14 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
             int dmin = 1 << 30;
             for (int e = 0; e < I.nnz; ++e) dmin = std::min(dmin, (int)I.edge_delays[e]);
             const int dmax = max_delay_of(I);
-            for (int K : {8}) {   // the kernel's delay-block length
+            const bool stoch = I.integrator >= HB_EULER_STOCH;
+            for (int K : {8, 16}) {   // the kernel's delay-block lengths
                 if (dmin < K) continue;
                 const int64_t ring_k = (int64_t)(dmax + K + 1 + K - 1) * I.n_nodes * s.np * 4;
-                if (ring_k <= lds_budget || K == 8) {
+                // 8 when the ring fits LDS or the run is stochastic; 16 for
+                // deterministic runs whose ring lives in HBM (64-byte windows)
+                if (K == 8 && ring_k > lds_budget && dmin >= 16 && !stoch) continue;
+                {
                     s.blocked = K;
                     s.depth = dmax + K + 1;
                     // lanes per node (FMA mode only: exact mode keeps the row's sequential sum):

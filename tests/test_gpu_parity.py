@@ -296,6 +296,22 @@ def test_blocked_global_ring_and_long_rows(gpu, orc):
     _assert_exact(g, orc.run(I), "blocked global ring")
 
 
+@pytest.mark.parametrize("model", EXACT_MODELS)
+def test_long_block_hbm_ring(gpu, orc, model):
+    """Deterministic runs with the ring in HBM and every delay >= 16 use the
+    16-step block (64-byte edge windows): bitwise equal to the oracle in exact
+    mode, and within FP32 of it with four lanes per node."""
+    I = _blocked_net(model, N=300, T=260, integ=L.RK4, dens=0.1, dmin=16, dmax=400, seed=610 + model)
+    (g,), st = run_instances([I], exact=True)
+    assert st.lds_bytes == 0
+    o = orc.run(I)
+    _assert_exact(g, o, f"long block model={model}")
+    J = _blocked_net(model, N=120, T=260, integ=L.HEUN, dens=0.2, dmin=17, dmax=450, seed=620 + model)
+    (f,), st = run_instances([J], exact=False)
+    assert st.lds_bytes == 0
+    np.testing.assert_allclose(f.traj, orc.run(J)["traj"], rtol=1e-5, atol=1e-5)
+
+
 def test_blocked_jr_sjr_and_kuramoto_fn(gpu, orc):
     for I in (_blocked_net(L.JANSEN_RIT, integ=L.RK4, T=200, seed=602),
               _blocked_net(L.KURAMOTO, integ=L.RK4, T=300, seed=603, cfn=L.CF_KURAMOTO, cparams=(1.2,))):
