@@ -32,6 +32,8 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall"]
 # hb_forward.hip is compiled once per kernel unit (one per model and pre
 # channel count) and mode, so the units build in parallel.
 FORWARD_UNITS = 7
+# hb_adjoint.hip: one unit per model (0..5) and one for the shared kernels (6).
+ADJOINT_UNITS = 7
 UNITS = [
     # (source, object, extra flags, device?)
     *[("hb_forward.hip", f"hb_forward_fast_u{u}.o", ["-DHB_EXACT=0", f"-DHB_UNIT={u}"], True)
@@ -40,7 +42,7 @@ UNITS = [
       for u in range(FORWARD_UNITS)],
     ("hb_forward_dispatch.cpp", "hb_forward_dispatch.o", [], False),
     ("hb_noise.hip", "hb_noise.o", [], True),
-    ("hb_adjoint.hip", "hb_adjoint.o", [], True),
+    *[("hb_adjoint.hip", f"hb_adjoint_u{u}.o", [f"-DHB_ADJ_UNIT={u}"], True) for u in range(ADJOINT_UNITS)],
     ("hb_host.cpp", "hb_host.o", [], False),
     ("hb_engine.cpp", "hb_engine.o", [], False),
     ("hb_adjoint_host.cpp", "hb_adjoint_host.o", [], False),

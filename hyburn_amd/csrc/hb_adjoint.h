@@ -55,11 +55,20 @@ struct AdjLaunch {
     bool blocked;               // delay-blocked kernels (min delay >= 8, npt 1)
     bool lring;                 // blocked kernels: rings in LDS
     bool bold_fused;            // blocked forward carries the BOLD monitor in helper waves (launched 2 x block)
+    int32_t ic;                 // integrator class of every instance (0 Euler, 1 Heun, 2 RK4)
+    int32_t cfk;                // HB_CF_LINEAR when every instance couples linearly (blocked kernels), else -1
     int32_t block;
     int32_t lds_fwd, lds_rev;   // dynamic LDS bytes
 };
 
 hipError_t adjoint_set_attributes(int max_lds);
+// per-model kernel units (hb_adjoint.hip built with HB_ADJ_UNIT = model)
+hipError_t adj_unit_0(int op, const AdjLaunch* L, hipStream_t s, int arg);
+hipError_t adj_unit_1(int op, const AdjLaunch* L, hipStream_t s, int arg);
+hipError_t adj_unit_2(int op, const AdjLaunch* L, hipStream_t s, int arg);
+hipError_t adj_unit_3(int op, const AdjLaunch* L, hipStream_t s, int arg);
+hipError_t adj_unit_4(int op, const AdjLaunch* L, hipStream_t s, int arg);
+hipError_t adj_unit_5(int op, const AdjLaunch* L, hipStream_t s, int arg);
 // stage 0: forward save; 1: BOLD forward; 2: loss; 3: BOLD reverse; 4: reverse
 hipError_t launch_adjoint_stage(int stage, const AdjLaunch& L, hipStream_t s);
 

@@ -30,6 +30,12 @@
 #include "hb_internal.h"
 #include "hb_adjoint.h"
 
+// Translation unit: 0..5 = one model's kernels, 6 = shared kernels (see the
+// dispatch section at the end).
+#ifndef HB_ADJ_UNIT
+#define HB_ADJ_UNIT 6
+#endif
+
 #include <cmath>
 #include <type_traits>
 
@@ -267,12 +273,13 @@ __device__ __forceinline__ void pre_vjp(const float* x, const float* acc, float*
 // ------------------------------------------------------- step VJP ----------
 // Recompute the stages of one step from (x, c, z) and pull lam (dL/dx_{t+1})
 // back: lx = dL/dx_t (integrator part), mu = dL/dc_t, dp += parameter term.
-template <int M>
+// IC >= 0 fixes the integrator class at compile time (-1: from S.integ).
+template <int M, int IC = -1>
 __device__ __forceinline__ void step_vjp(const StepCtx& S, const float* mc, const float* c, const float* z,
                                          const float* x, const float* lam, float* lx, float* mu,
                                          const float* jac, float& dp) {
     constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
-    const int ic = integ_class(S.integ);
+    const int ic = IC >= 0 ? IC : integ_class(S.integ);
     float k1[NV], k2[NV], k3[NV], k4[NV], s2[NV], s3[NV], s4[NV], u[NV], m[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) lx[v] = 0.0f;
@@ -371,6 +378,14 @@ __device__ __forceinline__ StepCtx make_ctx(const AdjInst& I) {
     return S;
 }
 
+// One forward step; IC >= 0 fixes the integrator class at compile time.
+template <int M, int IC>
+__device__ __forceinline__ void integ_step(const StepCtx& S, const float* mc, const float* cpl, const float* z,
+                                          float* x) {
+    if constexpr (IC >= 0) integrate_c<M, IC>(S, mc, cpl, z, x);
+    else integ_step<M, IC>(S, mc, cpl, z, x);
+}
+
 __device__ __forceinline__ float block_sum(float v, float* red) {
     // sum over the workgroup, fixed order: wave reductions then wave 0
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -404,11 +419,12 @@ __device__ __forceinline__ double block_sum_d(double v, double* red) {
 // Same step order as the forward kernels (per-step schedule): ring write of
 // pre(x_t), barrier, per-edge gather (slot t-d, t==0 reads current), post,
 // stages, clamp. Ring depth dmax + 2.
-template <int M, int NP, int NPT>
+template <int M, int NP, int NPT, int IC, int CF>
 __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__ insts) {
     extern __shared__ float lds[];
     __shared__ float ordp[2][8][2];   // per-wave (sum cos, sum sin) of the last two steps
     const AdjInst& I = insts[blockIdx.x];
+    const int cfn = CF >= 0 ? CF : I.cfn;   // coupling function, fixed at compile time when CF >= 0
     constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
     const int N = I.n_nodes, T = I.n_steps, Hd = I.ring_depth, Sg = I.seg_len;
     const int tid = threadIdx.x, BS = blockDim.x;
@@ -462,7 +478,7 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
             const int n = tid + q * BS;
             if (n < N) {
                 float pr[NP];
-                pre<NC, NP>(I.cfn, cc, false, false, x[q], pr);
+                pre<NC, NP>(cfn, cc, false, false, x[q], pr);
 #pragma unroll
                 for (int c = 0; c < NP; ++c) ring[(size_t)ws * NNP + n * NP + c] = pr[c];
             }
@@ -518,7 +534,7 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
             float own[NP], cpl[NC];
 #pragma unroll
             for (int c = 0; c < NP; ++c) own[c] = ring[(size_t)ws * NNP + n * NP + c];
-            if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, g, own, cpl);
+            if (I.has_coupling) post<NC, NP>(cfn, cc, false, g, own, cpl);
             else {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
@@ -526,7 +542,7 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
             float z[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
-            integrate<M>(S, mcn[q], cpl, z, x[q]);
+            integ_step<M, IC>(S, mcn[q], cpl, z, x[q]);
             bacc[q] += x[q][0];
             if (Sg > 0 && (t + 1) % Sg == 0 && t + 1 < T) {
                 float* ck = I.ckpt + ((size_t)((t + 1) / Sg) * N + n) * NV;
@@ -622,6 +638,7 @@ __device__ __forceinline__ void bold_fwd_window(const AdjInst& I, BoldD& b, int 
 // the next kBf windows load while the current ones step (the recursion is a
 // dependent FP64 chain, so their latency would otherwise add per window).
 constexpr int kBf = 8;
+#if HB_ADJ_UNIT == 6
 __global__ __launch_bounds__(256) void k_bold_forward(const AdjInst* __restrict__ insts) {
     const AdjInst& I = insts[blockIdx.x];
     if (I.bold_period <= 0 || I.bold_fused) return;
@@ -653,6 +670,7 @@ __global__ __launch_bounds__(256) void k_bold_forward(const AdjInst* __restrict_
         }
     }
 }
+#endif
 
 // ------------------------------------------------------------- loss --------
 // FC (fc.rs:11-57: f64 accumulators, population std, std < 1e-12 -> 1,
@@ -863,6 +881,7 @@ __device__ void fc_loss_tiled(const AdjInst& I, double* sh, double* red) {
     }
 }
 
+#if HB_ADJ_UNIT == 6
 __global__ __launch_bounds__(256) void k_loss(const AdjInst* __restrict__ insts) {
     extern __shared__ double shd[];
     __shared__ double red[8];
@@ -949,6 +968,7 @@ __global__ __launch_bounds__(256) void k_loss(const AdjInst* __restrict__ insts)
         I.dsig[idx] = (a - h[i] * zi) / (sqn * sd[i]);
     }
 }
+#endif
 
 // ------------------------------------------------------- BOLD reverse ------
 // Adjoint of bold_step_d per node, windows last to first (FP64). For window k
@@ -1032,6 +1052,7 @@ __device__ __forceinline__ void bold_rev_window(const AdjInst& I, double (&a)[4]
     bold_rev_apply(c, a, true);
 }
 
+#if HB_ADJ_UNIT == 6
 __global__ __launch_bounds__(1024) void k_bold_reverse(const AdjInst* __restrict__ insts) {
     const AdjInst& I = insts[blockIdx.x];
     if (I.bold_period <= 0 || I.loss_kind == HB_LOSS_ORDER) return;
@@ -1100,13 +1121,15 @@ __global__ __launch_bounds__(1024) void k_bold_reverse(const AdjInst* __restrict
     // ---- rerun the chunk from its incoming state, writing the seeds
     sweep(a, n, khi, klo);
 }
+#endif
 
 // ------------------------------------------------------- reverse ----------
-template <int M, int NP, int NPT>
+template <int M, int NP, int NPT, int IC, int CF>
 __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__ insts) {
     extern __shared__ float lds[];
     __shared__ double redd[8];
     const AdjInst& I = insts[blockIdx.x];
+    const int cfn = CF >= 0 ? CF : I.cfn;   // coupling function, fixed at compile time when CF >= 0
     constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
     const int N = I.n_nodes, T = I.n_steps, Ha = I.aring_depth, Sg = I.seg_len, nseg = I.n_seg;
     const int tid = threadIdx.x, BS = blockDim.x;
@@ -1170,15 +1193,15 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
 #pragma nounroll
             for (int t = t0; t < t1; ++t) {
                 float own[NP], cpl[NC], z[NV];
-                pre<NC, NP>(I.cfn, cc, false, false, x, own);
-                if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, cur[0], own, cpl);
+                pre<NC, NP>(cfn, cc, false, false, x, own);
+                if (I.has_coupling) post<NC, NP>(cfn, cc, false, cur[0], own, cpl);
                 else {
 #pragma unroll
                     for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
                 }
 #pragma unroll
                 for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
-                integrate<M>(S, mcn[q], cpl, z, x);
+                integ_step<M, IC>(S, mcn[q], cpl, z, x);
                 float* tp = I.tape + ((size_t)(t - t0 + 1) * N + n) * NV;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) tp[v] = x[v];
@@ -1253,8 +1276,8 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
                 const float* gl = I.glog + ((size_t)t * N + n) * NP;
 #pragma unroll
                 for (int c = 0; c < NP; ++c) g[c] = gl[c];
-                pre<NC, NP>(I.cfn, cc, false, false, x, own);
-                if (I.has_coupling) post<NC, NP>(I.cfn, cc, false, g, own, cpl);
+                pre<NC, NP>(cfn, cc, false, false, x, own);
+                if (I.has_coupling) post<NC, NP>(cfn, cc, false, g, own, cpl);
                 else {
 #pragma unroll
                     for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
@@ -1262,10 +1285,10 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
 #pragma unroll
                 for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
                 float dps = 0.0f, dcs = 0.0f;
-                step_vjp<M>(S, mcn[q], cpl, z, x, lam[q], lxp[q], mu, jac[q], dps);
+                step_vjp<M, IC>(S, mcn[q], cpl, z, x, lam[q], lxp[q], mu, jac[q], dps);
                 gp[q] += (double)dps;
                 if (I.has_coupling) {
-                    post_vjp<NC, NP>(I.cfn, cc, g, own, mu, lg, lown, dcs);
+                    post_vjp<NC, NP>(cfn, cc, g, own, mu, lg, lown, dcs);
                     gcc += (double)dcs;
 #pragma unroll
                     for (int c = 0; c < NC; ++c) lxp[q][c] += lown[c];
@@ -1458,13 +1481,14 @@ __device__ __forceinline__ void bold_helper_forward(const AdjInst& I, int n) {
 }
 
 // Forward save, delay-blocked. One node per thread (N <= blockDim).
-template <int M, int NP, bool LRING>
+template <int M, int NP, bool LRING, int IC, int CF>
 __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict__ insts) {
     extern __shared__ float lds[];
     __shared__ float ordp[2][kK][8][2];   // per-wave order-parameter partials, by block parity
     const AdjInst* Ip = insts + blockIdx.x;
     constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
     const BView V = bview(Ip, false);
+    const int cfn = CF >= 0 ? CF : V.cfn;   // coupling function, fixed at compile time when CF >= 0
     const int N = V.N, T = V.T, H = V.H, Hs = V.Hs;
     const int tid = threadIdx.x, BS = blockDim.x;
     const bool own = tid < N;
@@ -1523,7 +1547,7 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
 
     auto ring_put = [&](int t, const float* xs) {
         float pr[NP];
-        pre<NC, NP>(V.cfn, cc, false, false, xs, pr);
+        pre<NC, NP>(cfn, cc, false, false, xs, pr);
         const int sl = wrap_slot(t, H);
 #pragma unroll
         for (int c = 0; c < NP; ++c) {
@@ -1538,15 +1562,15 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
 #pragma unroll
             for (int c = 0; c < NP; ++c) gl[c] = g[c];
             float own_pre[NP], cpl[NC], z[NV];
-            pre<NC, NP>(V.cfn, cc, false, false, x, own_pre);
-            if (V.has_c) post<NC, NP>(V.cfn, cc, false, g, own_pre, cpl);
+            pre<NC, NP>(cfn, cc, false, false, x, own_pre);
+            if (V.has_c) post<NC, NP>(cfn, cc, false, g, own_pre, cpl);
             else {
 #pragma unroll
                 for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
             }
 #pragma unroll
             for (int v = 0; v < NV; ++v) z[v] = S.stoch ? V.noise[((size_t)t * N + tid) * NV + v] * nsc[v] : 0.0f;
-            integrate<M>(S, mc, cpl, z, x);
+            integ_step<M, IC>(S, mc, cpl, z, x);
             bacc += x[0];
             if (V.Sg > 0 && (t + 1) % V.Sg == 0 && t + 1 < T) {
                 gp<float> ck = V.ckpt + ((size_t)((t + 1) / V.Sg) * N + tid) * NV;
@@ -1674,13 +1698,14 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
 // holds dL/dg (coupling adjoints) by step: step s reads, over out-edge
 // (j -> i, delay d >= kK), the adjoint of step s + d -- always in a later
 // block. Step 0 (every edge read the current state) runs last, per step.
-template <int M, int NP, bool LRING>
+template <int M, int NP, bool LRING, int IC, int CF>
 __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict__ insts) {
     extern __shared__ float lds[];
     __shared__ double redd[8];
     const AdjInst* Ip = insts + blockIdx.x;
     constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
     const BView V = bview(Ip, true);
+    const int cfn = CF >= 0 ? CF : V.cfn;   // coupling function, fixed at compile time when CF >= 0
     const int N = V.N, T = V.T, H = V.H, Hs = V.Hs, Sg = V.Sg, P = V.P;
     const int tid = threadIdx.x, BS = blockDim.x;
     const bool own = tid < N;
@@ -1728,17 +1753,17 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
     // one reverse step of own node: lam (dL/dx_{s+1}) -> lx (partial dL/dx_s), lg
     auto vjp = [&](int s, const float* xs, const float* g, const float* zs, float* lx, float* lg) {
         float own_pre[NP], cpl[NC], mu[NC], lown[NC];
-        pre<NC, NP>(V.cfn, cc, false, false, xs, own_pre);
-        if (V.has_c) post<NC, NP>(V.cfn, cc, false, g, own_pre, cpl);
+        pre<NC, NP>(cfn, cc, false, false, xs, own_pre);
+        if (V.has_c) post<NC, NP>(cfn, cc, false, g, own_pre, cpl);
         else {
 #pragma unroll
             for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
         }
         float dps = 0.0f, dcs = 0.0f;
-        step_vjp<M>(S, mc, cpl, zs, xs, lam, lx, mu, jac, dps);
+        step_vjp<M, IC>(S, mc, cpl, zs, xs, lam, lx, mu, jac, dps);
         gpar += (double)dps;
         if (V.has_c) {
-            post_vjp<NC, NP>(V.cfn, cc, g, own_pre, mu, lg, lown, dcs);
+            post_vjp<NC, NP>(cfn, cc, g, own_pre, mu, lg, lown, dcs);
             gcc += (double)dcs;
 #pragma unroll
             for (int c = 0; c < NC; ++c) lx[c] += lown[c];
@@ -1797,13 +1822,13 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                         float own_pre[NP], cpl[NC], z[NV];
 #pragma unroll
                         for (int v = 0; v < NV; ++v) z[v] = S.stoch ? V.noise[((size_t)t * N + tid) * NV + v] * nsc[v] : 0.0f;
-                        pre<NC, NP>(V.cfn, cc, false, false, x, own_pre);
-                        if (V.has_c) post<NC, NP>(V.cfn, cc, false, gq[m], own_pre, cpl);
+                        pre<NC, NP>(cfn, cc, false, false, x, own_pre);
+                        if (V.has_c) post<NC, NP>(cfn, cc, false, gq[m], own_pre, cpl);
                         else {
 #pragma unroll
                             for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
                         }
-                        integrate<M>(S, mc, cpl, z, x);
+                        integ_step<M, IC>(S, mc, cpl, z, x);
                         gp<float> tp = V.tape + ((size_t)(t - t0s + 1) * N + tid) * NV;
 #pragma unroll
                         for (int v = 0; v < NV; ++v) tp[v] = x[v];
@@ -1975,87 +2000,123 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
 }
 
 // ------------------------------------------------------------ dispatch -----
+//
+// hb_adjoint.hip is compiled once per model (HB_ADJ_UNIT 0..5 = the model
+// id: that model's forward-save and reverse kernels) plus once for the shared
+// kernels and the stage launcher (HB_ADJ_UNIT 6), so the units build in
+// parallel. Each model kernel is instantiated per integrator class (IC 0..2);
+// the delay-blocked ones also with the Linear coupling fixed (CF).
 
-template <int M, int NP>
+#if HB_ADJ_UNIT < 6
+template <int M, int NP, int IC>
 static const void* ptr(bool rev, int npt) {
-    if (rev) return npt == 1 ? (const void*)k_adj_reverse<M, NP, 1> : (const void*)k_adj_reverse<M, NP, 2>;
-    return npt == 1 ? (const void*)k_adj_forward<M, NP, 1> : (const void*)k_adj_forward<M, NP, 2>;
+    if (rev) return npt == 1 ? (const void*)k_adj_reverse<M, NP, 1, IC, -1> : (const void*)k_adj_reverse<M, NP, 2, IC, -1>;
+    return npt == 1 ? (const void*)k_adj_forward<M, NP, 1, IC, -1> : (const void*)k_adj_forward<M, NP, 2, IC, -1>;
+}
+
+template <int M, int NP, int IC, int CF>
+static const void* ptr_b(bool rev, bool lring) {
+    if (rev) return lring ? (const void*)k_adj_reverse_b<M, NP, true, IC, CF> : (const void*)k_adj_reverse_b<M, NP, false, IC, CF>;
+    return lring ? (const void*)k_adj_forward_b<M, NP, true, IC, CF> : (const void*)k_adj_forward_b<M, NP, false, IC, CF>;
 }
 
 template <int M, int NP>
-static const void* ptr_b(bool rev, bool lring) {
-    if (rev) return lring ? (const void*)k_adj_reverse_b<M, NP, true> : (const void*)k_adj_reverse_b<M, NP, false>;
-    return lring ? (const void*)k_adj_forward_b<M, NP, true> : (const void*)k_adj_forward_b<M, NP, false>;
-}
-
-static const void* kernel_b(int model, bool rev, int np, bool lring) {
-    switch (model) {
-    case HB_G2DO: return np == 1 ? ptr_b<HB_G2DO, 1>(rev, lring) : nullptr;
-    case HB_MPR: return np == 1 ? ptr_b<HB_MPR, 1>(rev, lring) : np == 2 ? ptr_b<HB_MPR, 2>(rev, lring) : nullptr;
-    case HB_JANSEN_RIT: return np == 1 ? ptr_b<HB_JANSEN_RIT, 1>(rev, lring) : nullptr;
-    case HB_WILSON_COWAN: return np == 1 ? ptr_b<HB_WILSON_COWAN, 1>(rev, lring) : nullptr;
-    case HB_KURAMOTO: return np == 1 ? ptr_b<HB_KURAMOTO, 1>(rev, lring) : np == 2 ? ptr_b<HB_KURAMOTO, 2>(rev, lring) : nullptr;
-    case HB_RWW: return np == 1 ? ptr_b<HB_RWW, 1>(rev, lring) : nullptr;
-    default: return nullptr;
+static const void* pick(const AdjLaunch& L, bool rev) {
+    if (L.blocked) {
+        const bool lin = L.cfk == HB_CF_LINEAR;
+        switch (L.ic) {
+        case 0: return lin ? ptr_b<M, NP, 0, HB_CF_LINEAR>(rev, L.lring) : ptr_b<M, NP, 0, -1>(rev, L.lring);
+        case 1: return lin ? ptr_b<M, NP, 1, HB_CF_LINEAR>(rev, L.lring) : ptr_b<M, NP, 1, -1>(rev, L.lring);
+        default: return lin ? ptr_b<M, NP, 2, HB_CF_LINEAR>(rev, L.lring) : ptr_b<M, NP, 2, -1>(rev, L.lring);
+        }
+    }
+    switch (L.ic) {
+    case 0: return ptr<M, NP, 0>(rev, L.npt);
+    case 1: return ptr<M, NP, 1>(rev, L.npt);
+    default: return ptr<M, NP, 2>(rev, L.npt);
     }
 }
 
 // Instantiated (model, pre channels): Kuramoto with and without Kuramoto
 // coupling, MPR with two channels or with the zero-coefficient one dropped.
-static const void* kernel_for(int model, bool rev, int np, int npt) {
-    switch (model) {
-    case HB_G2DO: return np == 1 ? ptr<HB_G2DO, 1>(rev, npt) : nullptr;
-    case HB_MPR: return np == 1 ? ptr<HB_MPR, 1>(rev, npt) : np == 2 ? ptr<HB_MPR, 2>(rev, npt) : nullptr;
-    case HB_JANSEN_RIT: return np == 1 ? ptr<HB_JANSEN_RIT, 1>(rev, npt) : nullptr;
-    case HB_WILSON_COWAN: return np == 1 ? ptr<HB_WILSON_COWAN, 1>(rev, npt) : nullptr;
-    case HB_KURAMOTO: return np == 1 ? ptr<HB_KURAMOTO, 1>(rev, npt) : np == 2 ? ptr<HB_KURAMOTO, 2>(rev, npt) : nullptr;
-    case HB_RWW: return np == 1 ? ptr<HB_RWW, 1>(rev, npt) : nullptr;
-    default: return nullptr;
+static const void* kernel_of(const AdjLaunch& L, bool rev) {
+    constexpr int M = HB_ADJ_UNIT;
+    if (L.model != M) return nullptr;
+    if constexpr (M == HB_MPR || M == HB_KURAMOTO) {
+        if (L.np == 2) return pick<M, 2>(L, rev);
     }
+    return L.np == 1 ? pick<M, 1>(L, rev) : nullptr;
 }
+#endif
 
 }  // namespace adj
+
+#if HB_ADJ_UNIT < 6
+#define HB_ADJ_CAT(a, b) a##b
+#define HB_ADJ_XCAT(a, b) HB_ADJ_CAT(a, b)
+// op 0: launch the forward save; 1: launch the reverse sweep; 2: set the
+// maximum dynamic LDS (arg bytes) on every kernel of the unit.
+hipError_t HB_ADJ_XCAT(adj_unit_, HB_ADJ_UNIT)(int op, const AdjLaunch* L, hipStream_t s, int arg) {
+    if (op == 2) {
+        AdjLaunch q{};
+        q.model = HB_ADJ_UNIT;
+        for (int np = 1; np <= 2; ++np)
+            for (int ic = 0; ic < 3; ++ic)
+                for (int v = 0; v < 6; ++v) {
+                    q.np = np; q.ic = ic;
+                    q.blocked = v >= 2;
+                    q.npt = v == 1 ? 2 : 1;
+                    q.lring = v >= 4;
+                    q.cfk = (v & 1) && v >= 2 ? HB_CF_LINEAR : -1;
+                    for (int rev = 0; rev < 2; ++rev) {
+                        const void* k = adj::kernel_of(q, rev != 0);
+                        if (!k) continue;
+                        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, arg);
+                        if (e != hipSuccess) return e;
+                    }
+                }
+        return hipSuccess;
+    }
+    const void* k = adj::kernel_of(*L, op == 1);
+    if (!k) return hipErrorInvalidValue;
+    const AdjInst* d = L->insts;
+    void* args[] = {(void*)&d};
+    // the forward save of a fused group carries the BOLD helper waves
+    const int block = (op == 0 && L->bold_fused) ? 2 * L->block : L->block;
+    return hipLaunchKernel(k, dim3(L->n), dim3(block), args, op == 0 ? L->lds_fwd : L->lds_rev, s);
+}
+#else
+namespace {
+using AdjUnitFn = hipError_t (*)(int, const AdjLaunch*, hipStream_t, int);
+const AdjUnitFn kAdjUnits[6] = {adj_unit_0, adj_unit_1, adj_unit_2, adj_unit_3, adj_unit_4, adj_unit_5};
+}  // namespace
 
 hipError_t adjoint_set_attributes(int max_lds) {
     {
         hipError_t e = hipFuncSetAttribute((const void*)adj::k_loss, hipFuncAttributeMaxDynamicSharedMemorySize, adj::kFcLds);
         if (e != hipSuccess) return e;
     }
-    for (int m = 0; m <= HB_RWW; ++m)
-        for (int rev = 0; rev < 2; ++rev)
-            for (int np = 1; np <= 2; ++np)
-                for (int v = 0; v < 4; ++v) {
-                    const void* k = v < 2 ? adj::kernel_for(m, rev, np, v + 1) : adj::kernel_b(m, rev, np, v == 3);
-                    if (!k) continue;
-                    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-                    if (e != hipSuccess) return e;
-                }
+    for (int m = 0; m < 6; ++m) {
+        const hipError_t e = kAdjUnits[m](2, nullptr, nullptr, max_lds);
+        if (e != hipSuccess) return e;
+    }
     return hipSuccess;
 }
 
 hipError_t launch_adjoint_stage(int stage, const AdjLaunch& L, hipStream_t s) {
     const AdjInst* d = L.insts;
+    if ((stage == 0 || stage == 4) && (L.model < 0 || L.model > 5)) return hipErrorInvalidValue;
     switch (stage) {
-    case 0: {
-        const void* k = L.blocked ? adj::kernel_b(L.model, false, L.np, L.lring) : adj::kernel_for(L.model, false, L.np, L.npt);
-        if (!k) return hipErrorInvalidValue;
-        void* args[] = {(void*)&d};
-        const int block = L.bold_fused ? 2 * L.block : L.block;   // + BOLD helper waves
-        return hipLaunchKernel(k, dim3(L.n), dim3(block), args, L.lds_fwd, s);
-    }
+    case 0: return kAdjUnits[L.model](0, &L, s, 0);
     case 1:
         if (L.bold_fused) return hipSuccess;   // in the forward kernel's helper waves
         hipLaunchKernelGGL(adj::k_bold_forward, dim3(L.n), dim3(256), 0, s, d);
         return hipGetLastError();
     case 2: hipLaunchKernelGGL(adj::k_loss, dim3(L.n), dim3(256), adj::kFcLds, s, d); return hipGetLastError();
     case 3: hipLaunchKernelGGL(adj::k_bold_reverse, dim3(L.n), dim3(1024), 0, s, d); return hipGetLastError();
-    default: {
-        const void* k = L.blocked ? adj::kernel_b(L.model, true, L.np, L.lring) : adj::kernel_for(L.model, true, L.np, L.npt);
-        if (!k) return hipErrorInvalidValue;
-        void* args[] = {(void*)&d};
-        return hipLaunchKernel(k, dim3(L.n), dim3(L.block), args, L.lds_rev, s);
-    }
+    default: return kAdjUnits[L.model](1, &L, s, 0);
     }
 }
+#endif
 
 }  // namespace hb

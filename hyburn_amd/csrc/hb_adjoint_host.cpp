@@ -178,7 +178,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
     struct Shape { int nv, nc, np, npt, block, Hd, S, nseg, nwin, nnz; bool kur, ring_lds, blocked; size_t ring_b; };
     constexpr int kBlock = 8;   // steps per delay block of the blocked kernels
     std::vector<Shape> sh(n);
-    std::map<std::tuple<int, int, int, int, int>, std::vector<int>> groups;
+    std::map<std::tuple<int, int, int, int, int, int, int>, std::vector<int>> groups;
     for (int i = 0; i < n; ++i) {
         const hb_instance& I = inst[i];
         Shape& s = sh[i];
@@ -205,7 +205,11 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         s.nwin = I.bold_period > 0 ? (I.n_steps + I.bold_period - 1) / I.bold_period : 0;
         s.ring_b = (size_t)(s.Hd + (s.blocked ? kBlock - 1 : 0)) * I.n_nodes * s.np * 4;
         s.ring_lds = s.ring_b <= (size_t)lds_budget;
-        groups[std::make_tuple(I.model, s.np, s.npt, (int)s.blocked, (int)s.ring_lds)].push_back(i);
+        // integrator class and (blocked kernels) Linear coupling are compile-time kernel variants
+        const int icl = (I.integrator == HB_EULER || I.integrator == HB_EULER_STOCH) ? 0
+                        : (I.integrator == HB_HEUN || I.integrator == HB_HEUN_STOCH) ? 1 : 2;
+        const int cfk = s.blocked && I.cfn == HB_CF_LINEAR ? (int)HB_CF_LINEAR : -1;
+        groups[std::make_tuple(I.model, s.np, s.npt, (int)s.blocked, (int)s.ring_lds, icl, cfk)].push_back(i);
         P->node_steps += (double)I.n_nodes * (double)I.n_steps;
     }
     // The BOLD monitor's forward runs in helper waves of the blocked forward
@@ -455,6 +459,8 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         L.npt = std::get<2>(kv.first);
         L.blocked = std::get<3>(kv.first) != 0;
         L.lring = std::get<4>(kv.first) != 0;
+        L.ic = std::get<5>(kv.first);
+        L.cfk = std::get<6>(kv.first);
         int blk = 64, lds = 0;
         for (int i : kv.second) {
             blk = std::max(blk, sh[i].block);
