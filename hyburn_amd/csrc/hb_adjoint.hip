@@ -458,6 +458,7 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
     }
     const int P = I.bold_period;
     int bcount = 0, kwin = 0;
+    int next_ck = Sg > 0 ? Sg : -1;   // the next step count that ends a checkpoint segment
     const bool ord = I.loss_kind == HB_LOSS_ORDER;
     const int nwaves = (BS + 63) >> 6;
     double lsum = 0.0;
@@ -544,7 +545,7 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
             for (int v = 0; v < NV; ++v) z[v] = S.stoch ? I.noise[((size_t)t * N + n) * NV + v] * I.nscale[v] : 0.0f;
             integ_step<M, IC>(S, mcn[q], cpl, z, x[q]);
             bacc[q] += x[q][0];
-            if (Sg > 0 && (t + 1) % Sg == 0 && t + 1 < T) {
+            if (t + 1 == next_ck && t + 1 < T) {   // (t + 1) is a multiple of the segment length
                 float* ck = I.ckpt + ((size_t)((t + 1) / Sg) * N + n) * NV;
 #pragma unroll
                 for (int v = 0; v < NV; ++v) ck[v] = x[q][v];
@@ -555,6 +556,7 @@ __global__ __launch_bounds__(512) void k_adj_forward(const AdjInst* __restrict__
                 for (int v = 0; v < NV; ++v) tp[v] = x[q][v];
             }
         }
+        if (t + 1 == next_ck) next_ck += Sg;
         if (P > 0 && (bcount == P || t == T - 1)) {
 #pragma unroll
             for (int q = 0; q < NPT; ++q) {
@@ -1261,9 +1263,13 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
                 }
             }
         }
-        // ---- reverse steps t1-1 .. t0
-        for (int t = t1 - 1; t >= t0; --t) {
-            const int slot = t % Ha;
+        // ---- reverse steps t1-1 .. t0; the adjoint ring slot of t and the BOLD
+        // window of t, (t - 1) / P, are stepped down with t
+        int slot = (t1 - 1) % Ha, win_t = 0, wr_t = 0;
+        if (seed_win && t1 >= 2) { win_t = (t1 - 2) / P; wr_t = (t1 - 2) - win_t * P; }
+        for (int t = t1 - 1; t >= t0;
+             --t, slot = slot == 0 ? Ha - 1 : slot - 1, wr_t = seed_win ? wr_t - 1 : 0) {
+            if (wr_t < 0) { wr_t = P - 1; --win_t; }
             float lxp[NPT][NV];
 #pragma unroll
             for (int q = 0; q < NPT; ++q) {
@@ -1361,7 +1367,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse(const AdjInst* __restrict__
 #pragma unroll
                 for (int v = 0; v < NV; ++v) lam[q][v] = lxp[q][v];
                 if (t >= 1) {
-                    if (seed_win) lam[q][0] += I.seedw[(size_t)((t - 1) / P) * N + n];
+                    if (seed_win) lam[q][0] += I.seedw[(size_t)win_t * N + n];
                     if (seed_ord) {
                         const float* cf = I.ord_work + (size_t)(t - t0) * 4;
                         lam[q][0] += cf[0] * sinf(x[0]) + cf[1] * cosf(x[0]);
@@ -1542,13 +1548,13 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
     const int nwaves = (BS + 63) >> 6;
     double lsum = 0.0;
     int bcount = 0, kwin = 0;
+    int ck_left = V.Sg > 0 ? V.Sg : -1, ck_idx = 0;   // steps to the next checkpoint, its index
     float bacc = 0.0f;
     const size_t chan = (size_t)N * Hs;
 
-    auto ring_put = [&](int t, const float* xs) {
+    auto ring_put = [&](int sl, const float* xs) {   // sl: ring slot of the state's time
         float pr[NP];
         pre<NC, NP>(cfn, cc, false, false, xs, pr);
-        const int sl = wrap_slot(t, H);
 #pragma unroll
         for (int c = 0; c < NP; ++c) {
             const size_t b = (size_t)c * chan + (size_t)tid * Hs;
@@ -1572,10 +1578,14 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
             for (int v = 0; v < NV; ++v) z[v] = S.stoch ? V.noise[((size_t)t * N + tid) * NV + v] * nsc[v] : 0.0f;
             integ_step<M, IC>(S, mc, cpl, z, x);
             bacc += x[0];
-            if (V.Sg > 0 && (t + 1) % V.Sg == 0 && t + 1 < T) {
-                gp<float> ck = V.ckpt + ((size_t)((t + 1) / V.Sg) * N + tid) * NV;
+            if (--ck_left == 0) {   // (t + 1) is a multiple of the segment length
+                ck_left = V.Sg;
+                ++ck_idx;
+                if (t + 1 < T) {
+                    gp<float> ck = V.ckpt + ((size_t)ck_idx * N + tid) * NV;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) ck[v] = x[v];
+                    for (int v = 0; v < NV; ++v) ck[v] = x[v];
+                }
             }
             if (full_tape) {   // one segment: the whole trajectory, read by the reverse
                 gp<float> tp = V.tape + ((size_t)(t + 1) * N + tid) * NV;
@@ -1607,7 +1617,7 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
 
     // ---- step 0: every edge reads the current state (block 0, one step)
     __syncthreads();
-    if (own) ring_put(0, x);
+    if (own) ring_put(0, x);   // slot of time 0
     __syncthreads();
     {
         float g[NP];
@@ -1629,13 +1639,14 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
     // ---- blocks of kK steps from t = 1; reads times t0 - d <= t0 - kK
     for (int t0 = 1; t0 < T; t0 += kK, ++blk) {
         const int kk = min(kK, T - t0);
+        const int s0b = wrap_slot(t0, H);   // ring slot of t0 (one division per block)
         float acc[kK][NP];
 #pragma unroll
         for (int k = 0; k < kK; ++k)
 #pragma unroll
             for (int c = 0; c < NP; ++c) acc[k][c] = 0.0f;
         if (own) {
-            const int s0 = wrap_slot(t0, H);
+            const int s0 = s0b;
 #pragma unroll
             for (int e = 0; e < kReg; ++e) {
                 if (e < nreg) {
@@ -1674,7 +1685,11 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
             for (int v = 0; v < NV; ++v) xs[v] = x[v];   // state at the start of step t0 + k
             step(t0 + k, acc[0]);
             if (ord) ord_part(blk & 1, k);
-            if (own) ring_put(t0 + k, xs);   // slot of t0 + k: read by later blocks only
+            if (own) {   // slot of t0 + k (read by later blocks only), from the block's first slot
+                int sl = s0b + k;
+                sl -= sl >= H ? H : 0;
+                ring_put(sl, xs);
+            }
 #pragma unroll
             for (int q = 0; q + 1 < kK; ++q)
 #pragma unroll
@@ -1773,16 +1788,16 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
         }
         (void)s;
     };
-    auto seed = [&](int s, int t0s, const float* xs, float* l) {
+    // win: BOLD window of step s, (s - 1) / P, tracked by the caller
+    auto seed = [&](int s, int t0s, const float* xs, float* l, int win) {
         if (s < 1) return;
-        if (seed_win) l[0] += V.seedw[(size_t)((s - 1) / P) * N + tid];
+        if (seed_win) l[0] += V.seedw[(size_t)win * N + tid];
         if (seed_ord) {
             const gp<float> cf = V.ord + (size_t)(s - t0s) * 4;
             l[0] += cf[0] * sinf(xs[0]) + cf[1] * cosf(xs[0]);
         }
     };
-    auto ring_put = [&](int t, const float* lg) {
-        const int sl = wrap_slot(t, H);
+    auto ring_put = [&](int sl, const float* lg) {   // sl: ring slot of the step
 #pragma unroll
         for (int c = 0; c < NP; ++c) {
             const size_t b = (size_t)c * chan + (size_t)tid * Hs;
@@ -1875,7 +1890,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
             float xT[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) xT[v] = V.tape[((size_t)(t1s - t0s) * N + tid) * NV + v];
-            seed(T, t0s, xT, lam);
+            seed(T, t0s, xT, lam, seed_win ? (T - 1) / P : 0);
         }
         // ---- blocks of reverse steps over [max(t0s, 1), t1s - 1], highest first
         const int tlo = max(t0s, 1);
@@ -1901,8 +1916,8 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
             for (int m = 0; m < kK; ++m)
 #pragma unroll
                 for (int c = 0; c < NP; ++c) acc[m][c] = 0.0f;
+            const int sb = wrap_slot(tbot, H);   // ring slot of tbot (one division per block)
             if (own && V.has_c) {
-                const int sb = wrap_slot(tbot, H);
 #pragma unroll
                 for (int e = 0; e < kReg; ++e) {
                     if (e < nreg) {
@@ -1931,6 +1946,9 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
             }
             float lgb[kK][NP];
             if (own) {
+                // BOLD window of the block's top step and its position, stepped down per step
+                int win = 0, wr = 0;
+                if (seed_win) { win = (ttop - 1) / P; wr = (ttop - 1) - win * P; }
 #pragma unroll
                 for (int m = kK - 1; m >= 0; --m) {
                     if (m < kk) {
@@ -1940,14 +1958,20 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                         for (int v = 0; v < NV; ++v) zs[v] = S.stoch ? V.noise[((size_t)st * N + tid) * NV + v] * nsc[v] : 0.0f;
                         vjp(st, xb[m], gb[m], zs, lx, lgb[m]);
                         pre_vjp<NC, NP>(xb[m], acc[m], lx);
-                        seed(st, t0s, xb[m], lx);
+                        seed(st, t0s, xb[m], lx, win);
+                        if (--wr < 0) { wr = P - 1; --win; }
 #pragma unroll
                         for (int v = 0; v < NV; ++v) lam[v] = lx[v];
                     }
                 }
 #pragma unroll
-                for (int m = 0; m < kK; ++m)
-                    if (m < kk) ring_put(tbot + m, lgb[m]);
+                for (int m = 0; m < kK; ++m) {
+                    if (m < kk) {
+                        int sl = sb + m;
+                        sl -= sl >= H ? H : 0;
+                        ring_put(sl, lgb[m]);
+                    }
+                }
             }
             __syncthreads();
         }
@@ -1966,7 +1990,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
 #pragma unroll
             for (int c = 0; c < NP; ++c) g0[c] = V.glog[(size_t)tid * NP + c];
             vjp(0, x0v, g0, z0, lx, lg);
-            ring_put(0, lg);
+            ring_put(0, lg);   // slot of step 0
         }
         __syncthreads();
         if (own) {
