@@ -1894,22 +1894,33 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
         }
         // ---- blocks of reverse steps over [max(t0s, 1), t1s - 1], highest first
         const int tlo = max(t0s, 1);
+        // the block's states and coupling log rows; the next block's load while
+        // this one is differentiated when they fit the register budget
+        constexpr bool kPrefetch = (NV + NP) * kK <= 24;
+        auto load_block = [&](float (&xx)[kK][NV], float (&gg)[kK][NP], int tb, int nk) {
+            if (!own) return;
+#pragma unroll
+            for (int m = 0; m < kK; ++m) {
+                if (m < nk) {
+                    const int st = tb + m;
+                    const gp<float> tp = V.tape + ((size_t)(st - t0s) * N + tid) * NV;
+                    const gp<float> gl = V.glog + ((size_t)st * N + tid) * NP;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) xx[m][v] = tp[v];
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) gg[m][c] = gl[c];
+                }
+            }
+        };
+        float xb[kK][NV], gb[kK][NP];
+        float xn[kPrefetch ? kK : 1][NV], gn[kPrefetch ? kK : 1][NP];
+        bool have = false;   // xb / gb already hold the block (prefetched)
         for (int ttop = t1s - 1; ttop >= tlo; ttop -= kK) {
             const int kk = min(kK, ttop - tlo + 1), tbot = ttop - kk + 1;
-            float xb[kK][NV], gb[kK][NP];
-            if (own) {
-#pragma unroll
-                for (int m = 0; m < kK; ++m) {
-                    if (m < kk) {
-                        const int st = tbot + m;
-                        const gp<float> tp = V.tape + ((size_t)(st - t0s) * N + tid) * NV;
-                        const gp<float> gl = V.glog + ((size_t)st * N + tid) * NP;
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) xb[m][v] = tp[v];
-#pragma unroll
-                        for (int c = 0; c < NP; ++c) gb[m][c] = gl[c];
-                    }
-                }
+            if (!have) load_block(xb, gb, tbot, kk);
+            const int ttop2 = ttop - kK, kk2 = min(kK, ttop2 - tlo + 1), tbot2 = ttop2 - kk2 + 1;
+            if constexpr (kPrefetch) {
+                if (ttop2 >= tlo) load_block(xn, gn, tbot2, kk2);
             }
             float acc[kK][NP];
 #pragma unroll
@@ -1974,6 +1985,19 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                 }
             }
             __syncthreads();
+            have = false;
+            if constexpr (kPrefetch) {
+                if (ttop2 >= tlo) {
+#pragma unroll
+                    for (int m = 0; m < kK; ++m) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) xb[m][v] = xn[m][v];
+#pragma unroll
+                        for (int c = 0; c < NP; ++c) gb[m][c] = gn[m][c];
+                    }
+                    have = true;
+                }
+            }
         }
         if (seg > 0) __syncthreads();
     }
