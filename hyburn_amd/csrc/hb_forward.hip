@@ -773,23 +773,31 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
 #pragma unroll
                 for (int c = 0; c < NP; ++c) acc[k][c] = lane_group_sum(acc[k][c], lpn);
         }
-        // rolled step loop: slot 0 of the block registers is the current step,
-        // shifted down after each step (one integrator copy in the code)
-#pragma nounroll
-        for (int k = 0; k < kb; ++k) {
-            finish_step(t0 + k, acc[0], zb[0]);
-            int sl = ws0 + k + 1;
-            sl -= sl >= Hd ? Hd : 0;
-            write_pre(sl);
+        if constexpr (K == kBlockSteps) {
+            // 8-step blocks: unrolled (static register slots, no loop or shifts)
 #pragma unroll
-            for (int q = 0; q + 1 < K; ++q) {
-#pragma unroll
-                for (int c = 0; c < NP; ++c) acc[q][c] = acc[q + 1][c];
+            for (int k = 0; k < K; ++k) {
+                if (k < kb) {
+                    finish_step(t0 + k, acc[k], zb[k]);
+                    int sl = ws0 + k + 1;
+                    sl -= sl >= Hd ? Hd : 0;
+                    write_pre(sl);
+                }
             }
+        } else {
+            // long blocks: rolled (one integrator copy in the code); slot 0 of
+            // the block registers is the current step, shifted down after it
+#pragma nounroll
+            for (int k = 0; k < kb; ++k) {
+                finish_step(t0 + k, acc[0], zb[0]);
+                int sl = ws0 + k + 1;
+                sl -= sl >= Hd ? Hd : 0;
+                write_pre(sl);
 #pragma unroll
-            for (int q = 0; q + 1 < KZ; ++q) {
+                for (int q = 0; q + 1 < K; ++q) {
 #pragma unroll
-                for (int v = 0; v < NV; ++v) zb[q][v] = zb[q + 1][v];
+                    for (int c = 0; c < NP; ++c) acc[q][c] = acc[q + 1][c];
+                }
             }
         }
         ws0 += K;
