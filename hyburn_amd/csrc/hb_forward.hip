@@ -620,13 +620,16 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     int bcount = 0;
 
     // CSR row in registers: source base j*Hd, delay, weight
-    const int kreg = min((V.max_row + lpn - 1) >> V.lpn_shift, kMaxRegEdges);
+    // register-resident row edges: fewer with an HBM ring, whose deeper gather
+    // pipeline needs the registers (the rest stream through the batched tail)
+    constexpr int RE = LRING ? kMaxRegEdges : kMaxRegEdges / 2;
+    const int kreg = min((V.max_row + lpn - 1) >> V.lpn_shift, RE);
     int e0 = 0, e1 = 0;
     if (own) { e0 = V.indptr[n]; e1 = V.indptr[n + 1]; }
-    uint32_t epk[kMaxRegEdges];   // src | delay << 16
-    float ewt[kMaxRegEdges];
+    uint32_t epk[RE];   // src | delay << 16
+    float ewt[RE];
 #pragma unroll
-    for (int k = 0; k < kMaxRegEdges; ++k) {
+    for (int k = 0; k < RE; ++k) {
         epk[k] = own ? (uint32_t)n : 0u; ewt[k] = 0.0f;   // padding: zero weight, own row
         if (k < kreg && own && e0 + sub + k * lpn < e1) {
             const uint2 ed = edge_at(V.edges, e0 + sub + k * lpn);
@@ -730,42 +733,58 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
             sl += sl < 0 ? Hd : 0;
             return (int)(epk[e] & 0xFFFF) * Hs + sl;
         };
-        float vbuf[2][K][NP];
-        if (kreg > 0) {
-            const int b0 = edge_base(0);
+        // D edge windows in flight: 2 from LDS; 4 from an HBM ring when they fit
+        // the register budget (HBM latency is what the gather waits on there)
+        constexpr int D = (!LRING && K * NP <= 16) ? 4 : 2;
+        float vbuf[D][K][NP];
+        auto load_win = [&](float (&vb)[K][NP], int base) {
 #pragma unroll
             for (int k = 0; k < K; ++k)
 #pragma unroll
-                for (int c = 0; c < NP; ++c) vbuf[0][k][c] = ring[c * NHd + b0 + k];
-        }
+                for (int c = 0; c < NP; ++c) vb[k][c] = ring[c * NHd + base + k];
+        };
 #pragma unroll
-        for (int e = 0; e < kMaxRegEdges; ++e) {
+        for (int p = 0; p + 1 < D; ++p)
+            if (p < kreg) load_win(vbuf[p], edge_base(p));
+#pragma unroll
+        for (int e = 0; e < RE; ++e) {
             if (e < kreg) {
-                if (e + 1 < kMaxRegEdges && e + 1 < kreg) {
-                    const int b1 = edge_base(e + 1);
-#pragma unroll
-                    for (int k = 0; k < K; ++k)
-#pragma unroll
-                        for (int c = 0; c < NP; ++c) vbuf[(e + 1) & 1][k][c] = ring[c * NHd + b1 + k];
-                }
+                const int en = e + D - 1;
+                if (en < RE && en < kreg) load_win(vbuf[en % D], edge_base(en));
                 __builtin_amdgcn_sched_barrier(0);
                 const float w = ewt[e];
 #pragma unroll
                 for (int k = 0; k < K; ++k)
 #pragma unroll
-                    for (int c = 0; c < NP; ++c) acc[k][c] += w * vbuf[e & 1][k][c];
+                    for (int c = 0; c < NP; ++c) acc[k][c] += w * vbuf[e % D][k][c];
             }
         }
-        for (int e = em0; e < e1; e += lpn) {   // rows longer than the register budget
-            const uint2 ed = edge_at(V.edges, e);
-            int sl = ws0 - (int)(ed.x >> 16);
-            sl += sl < 0 ? Hd : 0;
-            const int base = (int)(ed.x & 0xFFFF) * Hs + sl;
-            const float w = __uint_as_float(ed.y);
+        // rows longer than the register budget: D edges per batch (records, then
+        // windows, then the sums in row order)
+        for (int e = em0; e < e1; e += D * lpn) {
+            float w[D];
+            int base[D];
 #pragma unroll
-            for (int k = 0; k < K; ++k)
+            for (int b = 0; b < D; ++b) {
+                const int eb = e + b * lpn;
+                w[b] = 0.0f;
+                base[b] = n * Hs;   // padding: zero weight, own row
+                if (eb < e1) {
+                    const uint2 ed = edge_at(V.edges, eb);
+                    int sl = ws0 - (int)(ed.x >> 16);
+                    sl += sl < 0 ? Hd : 0;
+                    base[b] = (int)(ed.x & 0xFFFF) * Hs + sl;
+                    w[b] = __uint_as_float(ed.y);
+                }
+            }
 #pragma unroll
-                for (int c = 0; c < NP; ++c) acc[k][c] += w * ring[c * NHd + base + k];
+            for (int b = 0; b < D; ++b) load_win(vbuf[b], base[b]);
+#pragma unroll
+            for (int b = 0; b < D; ++b)
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+#pragma unroll
+                    for (int c = 0; c < NP; ++c) acc[k][c] += w[b] * vbuf[b][k][c];
         }
         if (lpn > 1) {
 #pragma unroll
