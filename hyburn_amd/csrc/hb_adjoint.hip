@@ -748,17 +748,29 @@ __device__ void fc_loss_tiled(const AdjInst& I, double* sh, double* red) {
         }
     };
     double va[kFcStage], vb[kFcStage];
+    // Streams the chunks through the two register buffers, two chunks in
+    // flight: body(v, k0, refill) consumes chunk k0 from v and calls refill()
+    // once v is no longer needed (v then loads the chunk two ahead).
+    auto stream = [&](auto&& body) {
+        fetch(va, 0);
+        if (kFcChunk < nw) fetch(vb, kFcChunk);
+        for (int k0 = 0; k0 < nw; k0 += 2 * kFcChunk) {
+            body(va, k0, [&]() { if (k0 + 2 * kFcChunk < nw) fetch(va, k0 + 2 * kFcChunk); });
+            const int k1 = k0 + kFcChunk;
+            if (k1 >= nw) break;
+            body(vb, k1, [&]() { if (k1 + 2 * kFcChunk < nw) fetch(vb, k1 + 2 * kFcChunk); });
+        }
+    };
     // ---- means: per-element partial sums, then per-column sums over the rows
     {
         double ps[kFcStage];
 #pragma unroll
         for (int u = 0; u < kFcStage; ++u) ps[u] = 0.0;
-        fetch(va, 0);
-        for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
-            if (k0 + kFcChunk < nw) fetch(vb, k0 + kFcChunk);
+        stream([&](double (&v)[kFcStage], int, auto&& refill) {
 #pragma unroll
-            for (int u = 0; u < kFcStage; ++u) { ps[u] += va[u]; va[u] = vb[u]; }
-        }
+            for (int u = 0; u < kFcStage; ++u) ps[u] += v[u];
+            refill();
+        });
 #pragma unroll
         for (int u = 0; u < kFcStage; ++u)
             if (el[u] >= 0) chunk[(el[u] >> 8) * kFcCs + (el[u] & 255)] = ps[u];
@@ -784,12 +796,11 @@ __device__ void fc_loss_tiled(const AdjInst& I, double* sh, double* red) {
         tti[q] = ti; ttj[q] = tj;
         acc[q] = d4{0.0, 0.0, 0.0, 0.0};
     }
-    fetch(va, 0);
-    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
+    stream([&](double (&v)[kFcStage], int k0, auto&& refill) {
         __syncthreads();
-        stage(va, k0, false);
+        stage(v, k0, false);
         __syncthreads();
-        if (k0 + kFcChunk < nw) fetch(va, k0 + kFcChunk);
+        refill();
 #pragma unroll
         for (int kk = 0; kk < kFcChunk; kk += 4) {
             const double* row = chunk + (kk + lq) * kFcCs + lr;
@@ -797,7 +808,7 @@ __device__ void fc_loss_tiled(const AdjInst& I, double* sh, double* red) {
             for (int q = 0; q < kFcTiles; ++q)
                 if (wave + q * nwave < ntt) acc[q] = mfma_f64(row[tti[q] * 16], row[ttj[q] * 16], acc[q]);
         }
-    }
+    });
     // variances from the diagonal tiles
     const double sqn = sqrt((double)nw);
 #pragma unroll
@@ -856,31 +867,48 @@ __device__ void fc_loss_tiled(const AdjInst& I, double* sh, double* red) {
         const int i = q / NP, j = q - i * NP;
         if (j < i) gsm[i * kFcTile + j] = gsm[j * kFcTile + i];
     }
-    // ---- dL/dX per chunk: output tiles (16 samples) x (16 nodes)
+    // ---- dL/dX per chunk: output tiles (16 samples) x (16 nodes); a wave's
+    // tiles advance together so their matrix products are independent
+    constexpr int kDt = (kFcChunk / 16) * (kFcTile / 16) / 4;   // output tiles per wave at most
     const int ntile = (kFcChunk / 16) * nt;
-    fetch(va, 0);
-    for (int k0 = 0; k0 < nw; k0 += kFcChunk) {
+    const double* arow[kDt];
+    const double* bcol[kDt];
+    int oti[kDt];
+#pragma unroll
+    for (int q = 0; q < kDt; ++q) {
+        const int t = wave + q * nwave;
+        const int ts = t < ntile ? t / nt : 0, tn = t < ntile ? t - ts * nt : 0;
+        oti[q] = t < ntile ? (ts << 8) | tn : -1;
+        arow[q] = chunk + (ts * 16 + lr) * kFcCs + lq;        // A[s][j] = Zs[s][j]
+        bcol[q] = gsm + lq * kFcTile + tn * 16 + lr;           // B[j][i] = Gs[j][i]
+    }
+    stream([&](double (&v)[kFcStage], int k0, auto&& refill) {
         const int kc = min(kFcChunk, nw - k0);
         __syncthreads();
-        stage(va, k0, true);
+        stage(v, k0, true);
         __syncthreads();
-        if (k0 + kFcChunk < nw) fetch(va, k0 + kFcChunk);
-        for (int t = wave; t < ntile; t += nwave) {
-            const int ts = t / nt, tn = t - ts * nt;
-            d4 d = d4{0.0, 0.0, 0.0, 0.0};
-            const double* arow = chunk + (ts * 16 + lr) * kFcCs + lq;        // A[s][j] = Zs[s][j]
-            const double* bcol = gsm + lq * kFcTile + tn * 16 + lr;           // B[j][i] = Gs[j][i]
-            for (int j0 = 0; j0 < NP; j0 += 4)
-                d = mfma_f64(arow[j0], bcol[j0 * kFcTile], d);
+        refill();
+        d4 d[kDt];
+#pragma unroll
+        for (int q = 0; q < kDt; ++q) d[q] = d4{0.0, 0.0, 0.0, 0.0};
+        for (int j0 = 0; j0 < NP; j0 += 4) {
+#pragma unroll
+            for (int q = 0; q < kDt; ++q)
+                if (oti[q] >= 0) d[q] = mfma_f64(arow[q][j0], bcol[q][j0 * kFcTile], d[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < kDt; ++q) {
+            if (oti[q] < 0) continue;
+            const int ts = oti[q] >> 8, tn = oti[q] & 255;
             const int i = tn * 16 + lr;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int sm = ts * 16 + lq + 4 * r;
                 if (sm < kc && i < N)
-                    I.dsig[(size_t)(k0 + sm) * N + i] = (d[r] - hh[i] * chunk[sm * kFcCs + i]) * zsc[i];
+                    I.dsig[(size_t)(k0 + sm) * N + i] = (d[q][r] - hh[i] * chunk[sm * kFcCs + i]) * zsc[i];
             }
         }
-    }
+    });
 }
 
 #if HB_ADJ_UNIT == 6
@@ -1676,24 +1704,21 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
         // the previous block's order partials are complete (barrier at its end)
         if (ord && tid == 0)
             for (int k = 0; k < kprev; ++k) ord_close((blk - 1) & 1, k, tprev + k);
-        // rolled step loop (one integrator copy in the code): slot 0 of acc is
-        // the current step's coupling sum, shifted down after each step
-#pragma nounroll
-        for (int k = 0; k < kk; ++k) {
-            float xs[NV];
+        // unrolled step loop: static register slots
 #pragma unroll
-            for (int v = 0; v < NV; ++v) xs[v] = x[v];   // state at the start of step t0 + k
-            step(t0 + k, acc[0]);
-            if (ord) ord_part(blk & 1, k);
-            if (own) {   // slot of t0 + k (read by later blocks only), from the block's first slot
-                int sl = s0b + k;
-                sl -= sl >= H ? H : 0;
-                ring_put(sl, xs);
+        for (int k = 0; k < kK; ++k) {
+            if (k < kk) {
+                float xs[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) xs[v] = x[v];   // state at the start of step t0 + k
+                step(t0 + k, acc[k]);
+                if (ord) ord_part(blk & 1, k);
+                if (own) {   // slot of t0 + k (read by later blocks only), from the block's first slot
+                    int sl = s0b + k;
+                    sl -= sl >= H ? H : 0;
+                    ring_put(sl, xs);
+                }
             }
-#pragma unroll
-            for (int q = 0; q + 1 < kK; ++q)
-#pragma unroll
-                for (int c = 0; c < NP; ++c) acc[q][c] = acc[q + 1][c];
         }
         kprev = kk;
         tprev = t0;
