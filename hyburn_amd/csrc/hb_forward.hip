@@ -45,7 +45,7 @@ constexpr int kNoiseAhead = 4;
 // Steps per delay block (needs min delay >= kBlockSteps). Deterministic runs
 // whose ring lives in HBM use kLongBlockSteps: each edge's window is then 64
 // contiguous bytes (fewer DRAM segments per step) and barriers halve.
-constexpr int kBlockSteps = 8, kLongBlockSteps = 16;
+constexpr int kBlockSteps = 8, kLongBlockSteps = 16, kLongerBlockSteps = 32;
 
 #include "hb_model.h"
 
@@ -880,6 +880,13 @@ __device__ __forceinline__ void run_any(const DevInst* __restrict__ I, float* ld
             if (bk == kLongBlockSteps) {   // deterministic runs with the ring in HBM
                 run_blocked_any<M, NP, LRING, kLongBlockSteps>(I, lds);
                 return;
+            }
+            // 128-byte windows for the models whose registers allow them
+            if constexpr (Tr<M>::NV <= 2 && NP == 1) {
+                if (bk == kLongerBlockSteps) {
+                    run_blocked_any<M, NP, LRING, kLongerBlockSteps>(I, lds);
+                    return;
+                }
             }
         }
     }

@@ -489,12 +489,16 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
             for (int e = 0; e < I.nnz; ++e) dmin = std::min(dmin, (int)I.edge_delays[e]);
             const int dmax = max_delay_of(I);
             const bool stoch = I.integrator >= HB_EULER_STOCH;
-            for (int K : {8, 16}) {   // the kernel's delay-block lengths
+            // 32-step blocks exist for two-variable, one-channel models
+            const bool k32 = s.nv <= 2 && s.np == 1;
+            for (int K : {8, 16, 32}) {   // the kernel's delay-block lengths
                 if (dmin < K) continue;
                 const int64_t ring_k = (int64_t)(dmax + K + 1 + K - 1) * I.n_nodes * s.np * 4;
-                // 8 when the ring fits LDS or the run is stochastic; 16 for
-                // deterministic runs whose ring lives in HBM (64-byte windows)
+                // 8 when the ring fits LDS or the run is stochastic; 16 or 32 for
+                // deterministic runs whose ring lives in HBM (64- or 128-byte windows)
                 if (K == 8 && ring_k > lds_budget && dmin >= 16 && !stoch) continue;
+                if (K == 16 && k32 && dmin >= 32) continue;
+                if (K == 32 && !k32) continue;
                 {
                     s.blocked = K;
                     s.depth = dmax + K + 1;

@@ -310,6 +310,11 @@ def test_long_block_hbm_ring(gpu, orc, model):
     (f,), st = run_instances([J], exact=False)
     assert st.lds_bytes == 0
     np.testing.assert_allclose(f.traj, orc.run(J)["traj"], rtol=1e-5, atol=1e-5)
+    # every delay >= 32: 32-step blocks (128-byte windows) for two-variable models
+    I = _blocked_net(model, N=300, T=260, integ=L.RK4, dens=0.1, dmin=32, dmax=400, seed=630 + model)
+    (g,), st = run_instances([I], exact=True)
+    assert st.lds_bytes == 0
+    _assert_exact(g, orc.run(I), f"32-step block model={model}")
 
 
 def test_blocked_jr_sjr_and_kuramoto_fn(gpu, orc):
