@@ -735,7 +735,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         };
         // D edge windows in flight: 2 from LDS; 4 from an HBM ring when they fit
         // the register budget (HBM latency is what the gather waits on there)
-        constexpr int D = (!LRING && K * NP <= 16) ? 4 : 2;
+        constexpr int D = (!LRING && K * NP <= 16) ? 4 : (K * NP * NV <= 64 ? 2 : 1);
         float vbuf[D][K][NP];
         auto load_win = [&](float (&vb)[K][NP], int base) {
 #pragma unroll
@@ -881,8 +881,8 @@ __device__ __forceinline__ void run_any(const DevInst* __restrict__ I, float* ld
                 run_blocked_any<M, NP, LRING, kLongBlockSteps>(I, lds);
                 return;
             }
-            // 128-byte windows for the models whose registers allow them
-            if constexpr (Tr<M>::NV <= 2 && NP == 1) {
+            // 128-byte windows (one ring channel)
+            if constexpr (NP == 1) {
                 if (bk == kLongerBlockSteps) {
                     run_blocked_any<M, NP, LRING, kLongerBlockSteps>(I, lds);
                     return;

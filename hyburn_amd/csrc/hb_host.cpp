@@ -489,8 +489,8 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
             for (int e = 0; e < I.nnz; ++e) dmin = std::min(dmin, (int)I.edge_delays[e]);
             const int dmax = max_delay_of(I);
             const bool stoch = I.integrator >= HB_EULER_STOCH;
-            // 32-step blocks exist for two-variable, one-channel models
-            const bool k32 = s.nv <= 2 && s.np == 1;
+            // 32-step blocks exist for one-channel rings
+            const bool k32 = s.np == 1;
             for (int K : {8, 16, 32}) {   // the kernel's delay-block lengths
                 if (dmin < K) continue;
                 const int64_t ring_k = (int64_t)(dmax + K + 1 + K - 1) * I.n_nodes * s.np * 4;

@@ -310,7 +310,7 @@ def test_long_block_hbm_ring(gpu, orc, model):
     (f,), st = run_instances([J], exact=False)
     assert st.lds_bytes == 0
     np.testing.assert_allclose(f.traj, orc.run(J)["traj"], rtol=1e-5, atol=1e-5)
-    # every delay >= 32: 32-step blocks (128-byte windows) for two-variable models
+    # every delay >= 32: 32-step blocks (128-byte windows) for one-channel rings
     I = _blocked_net(model, N=300, T=260, integ=L.RK4, dens=0.1, dmin=32, dmax=400, seed=630 + model)
     (g,), st = run_instances([I], exact=True)
     assert st.lds_bytes == 0
@@ -321,6 +321,16 @@ def test_blocked_jr_sjr_and_kuramoto_fn(gpu, orc):
     for I in (_blocked_net(L.JANSEN_RIT, integ=L.RK4, T=200, seed=602),
               _blocked_net(L.KURAMOTO, integ=L.RK4, T=300, seed=603, cfn=L.CF_KURAMOTO, cparams=(1.2,))):
         (g,), _ = run_instances([I], exact=True)
+        np.testing.assert_allclose(g.traj, orc.run(I)["traj"], rtol=1e-5, atol=1e-5)
+
+
+def test_long_block_jansen_rit(gpu, orc):
+    """Jansen-Rit (6 variables, legacy sigmoidal coupling) with the ring in
+    HBM on 16- and 32-step blocks, against the oracle."""
+    for dmin in (16, 32):
+        I = _blocked_net(L.JANSEN_RIT, N=300, T=200, integ=L.RK4, dens=0.1, dmin=dmin, dmax=400, seed=640 + dmin)
+        (g,), st = run_instances([I], exact=True)
+        assert st.lds_bytes == 0
         np.testing.assert_allclose(g.traj, orc.run(I)["traj"], rtol=1e-5, atol=1e-5)
 
 
