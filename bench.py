@@ -85,8 +85,8 @@ def ragged_leg(args):
     """Ragged 4096-instance MPR/JR RK4 sweep: one persistent launch vs
     per-instance plain launches of the same kernel (sampled)."""
     from hyburn_amd.instances import Plan
-    from hyburn_amd.synth import config3
-    insts = config3(n_inst=args.ragged_n, t_scale=args.ragged_tscale)
+    from hyburn_amd.synth import ragged_mpr_jr
+    insts = ragged_mpr_jr(n_inst=args.ragged_n, t_scale=args.ragged_tscale)
     ns = sum(I.n_nodes * I.n_steps for I in insts)
     log(f"[bench] ragged: {len(insts)} instances, {ns:.3e} node-steps; planning")
     from hyburn_amd.instances import autotune_entries
@@ -193,13 +193,13 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(instances shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path "
+              "(instances would shard across GPUs without communication)", file=sys.stderr)
         return 2
 
     from hyburn_amd import _lib as L
     from hyburn_amd.instances import Plan
-    from hyburn_amd.synth import config2
+    from hyburn_amd.synth import sweep_g2do_74
 
     lib = L.lib()
     if lib.hb_device_count() < 1:
@@ -207,7 +207,7 @@ def main() -> int:
         return 1
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
-    insts = config2(n_sweep=args.n_sweep, n_steps=args.n_steps)
+    insts = sweep_g2do_74(n_sweep=args.n_sweep, n_steps=args.n_steps)
     node_steps = sum(I.n_nodes * I.n_steps for I in insts)
     log(f"[bench] workload: {len(insts)} x {insts[0].n_nodes} nodes x {insts[0].n_steps} steps")
     P = Plan(insts)
@@ -277,7 +277,7 @@ def main() -> int:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         from oracle import hb_oracle
         hb_oracle.lib()
-        sample = config2(n_sweep=args.n_sweep, n_steps=args.n_steps)
+        sample = sweep_g2do_74(n_sweep=args.n_sweep, n_steps=args.n_steps)
         v, secs = cpu_baseline(sample, threads)
         line["cpu_baseline"] = {"value": v, "unit": "node-steps/s", "cores": threads, "kind": "port",
                                 "sample": f"the full workload ({len(sample)} points x {args.n_steps} steps) "
@@ -289,20 +289,20 @@ def main() -> int:
         import dataclasses
         from hyburn_amd.adjoint import Loss
         from hyburn_amd.instances import run_instances
-        from hyburn_amd.synth import config4, config5
-        c4, hidden = config4(n_inst=args.adj4_batch)
+        from hyburn_amd.synth import fc_fit_mpr_84, kuramoto_order_1000
+        fc_insts, hidden = fc_fit_mpr_84(n_inst=args.adj4_batch)
         (h,), _ = run_instances([dataclasses.replace(hidden, record_bold=True)])
         fc_t = _fc64(h.bold.astype(np.float64)).astype(np.float32)
         line["adjoint"] = {
             "fc_mpr_84": adjoint_leg("MPR 84 nodes, 20k Heun steps, FC of native BOLD vs hidden-parameter FC; "
-                                     "grads of G and 84 eta", c4, [Loss(L.LOSS_FC, fc_t, 2) for _ in c4], 0,
-                                     1 + c4[0].n_nodes),
+                                     "grads of G and 84 eta", fc_insts, [Loss(L.LOSS_FC, fc_t, 2) for _ in fc_insts], 0,
+                                     1 + fc_insts[0].n_nodes),
         }
-        del c4
-        c5, tgt = config5(n_inst=args.adj5_batch, n_steps=args.adj5_steps)
+        del fc_insts
+        kur_insts, tgt = kuramoto_order_1000(n_inst=args.adj5_batch, n_steps=args.adj5_steps)
         line["adjoint"]["order_kuramoto_1000"] = adjoint_leg(
             "Kuramoto 1000 nodes, 5% CSR, delays <= 40 ms, RK4, order-parameter MSE; grads of G and 1000 omega",
-            c5, [Loss(L.LOSS_ORDER, tgt, 0) for _ in c5], 0, 1 + c5[0].n_nodes)
+            kur_insts, [Loss(L.LOSS_ORDER, tgt, 0) for _ in kur_insts], 0, 1 + kur_insts[0].n_nodes)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -31,7 +31,7 @@ from .engine import (BatchHybridEngine, SimConfig, SweepConfig, SweepParam, pars
 def _backend(name: str) -> None:
     got = L.lib().hb_select_backend(name.encode())
     if got is None:
-        raise SystemExit(f"backend '{name}' is not provided by this build (hip only): "
+        raise SystemExit(f"backend '{name}' is not provided by this library (hip only): "
                          + L.lib().hb_last_error().decode(errors="replace"))
 
 

@@ -115,7 +115,7 @@ def delays_from_lengths(lengths: np.ndarray, speed: float, dt: float) -> np.ndar
 
 # ---------------------------------------------------------------- workloads --
 
-def config1(seed: int = 1) -> Instance:
+def demo_g2do_74(seed: int = 1) -> Instance:
     """Plain forward run: G2DO, 74 nodes, Heun dt=0.1, 10k steps, nsig=0, scalar
     self-coupling 0.01 with Linear a=0.01 (HybridEngine semantics: w * I)."""
     rng = np.random.default_rng(seed)
@@ -126,7 +126,7 @@ def config1(seed: int = 1) -> Instance:
                     scalar_w=0.01, cfn=L.CF_LINEAR, cparams=(0.01,), record_traj=True)
 
 
-def config2(n_sweep: int = 128, n_steps: int = 10_000, seed: int = 2, noise_seed: int = 7,
+def sweep_g2do_74(n_sweep: int = 128, n_steps: int = 10_000, seed: int = 2, noise_seed: int = 7,
             explicit_noise: bool = False) -> list[Instance]:
     """Batched sweep: G2DO, 74 nodes, Linear G log-uniform [1e-3, 1e-1] x param a
     U[-2, 2]; symmetric 20% lognormal connectome; per-edge delays
@@ -155,7 +155,7 @@ def config2(n_sweep: int = 128, n_steps: int = 10_000, seed: int = 2, noise_seed
     return out
 
 
-def config3(n_inst: int = 4096, seed: int = 4, t_scale: float = 1.0, bold: bool = True) -> list[Instance]:
+def ragged_mpr_jr(n_inst: int = 4096, seed: int = 4, t_scale: float = 1.0, bold: bool = True) -> list[Instance]:
     """Ragged batch: MPR (Linear, BOLD TR=720 ms) or JR (legacy SigmoidalJansenRit
     on y0), N in {68,74,84,164,400}, lognormal CSR at U[5%,20%], delays <= 40 ms
     at dt=0.05 (<= 800 steps), sim length U[500,5000] ms (integer steps), RK4."""
@@ -199,7 +199,7 @@ def config3(n_inst: int = 4096, seed: int = 4, t_scale: float = 1.0, bold: bool 
     return out
 
 
-def config4(n_inst: int = 256, n_steps: int = 20_000, seed: int = 5, target_seed: int = 6):
+def fc_fit_mpr_84(n_inst: int = 256, n_steps: int = 20_000, seed: int = 5, target_seed: int = 6):
     """Differentiation: MPR, 84 nodes, CSR 10% lognormal, per-edge delays
     round(U[10,90] mm / 3 mm/ms / 0.1 ms) <= 300 steps, Heun dt = 0.1 ms,
     per-instance BOLD on the native 1 ms grid (bold_period 10), per-instance
@@ -226,7 +226,7 @@ def config4(n_inst: int = 256, n_steps: int = 20_000, seed: int = 5, target_seed
     return insts, hidden
 
 
-def config5(n_inst: int = 512, n_steps: int = 10_000, seed: int = 7, max_delay: int = 400):
+def kuramoto_order_1000(n_inst: int = 512, n_steps: int = 10_000, seed: int = 7, max_delay: int = 400):
     """Large-network adjoint: Kuramoto, 1000 nodes placed uniformly in a
     150 mm cube, CSR 5% lognormal, delays round(dist / 3 mm/ms / 0.1 ms)
     clipped to 400 steps (40 ms), natural frequencies N(10 Hz, 1 Hz) in rad/ms,

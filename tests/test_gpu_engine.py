@@ -14,7 +14,7 @@ from hyburn_amd import _lib as L
 from hyburn_amd.engine import (BatchHybridEngine, MonitorConfig, ProjectionConfig, SimConfig,
                                SubnetworkConfig, SweepParam, SweepPoint, SweepSpec)
 from hyburn_amd.instances import Instance, run_instances
-from hyburn_amd.synth import DEFAULTS, config1, dense_to_csr, lognormal_connectome
+from hyburn_amd.synth import DEFAULTS, demo_g2do_74, dense_to_csr, lognormal_connectome
 
 pytestmark = pytest.mark.gpu
 
@@ -193,12 +193,12 @@ def test_run_sweep_stochastic_shared_noise(gpu, orc):
         np.testing.assert_allclose(traj[:, b], orc.run(I)["traj"], rtol=1e-6, atol=1e-7)
 
 
-def test_config1_plain_run_full_length(gpu, orc):
+def test_demo_g2do_74_plain_run_full_length(gpu, orc):
     """The plain forward workload at full length (G2DO, 74 nodes, 10k Heun steps,
     scalar self-coupling w*I): trajectory [10000][74][2] bit-identical to the
     oracle in exact mode; fast mode within 4x the FP32 oracle's own error
     against the FP64 twin."""
-    I = config1()
+    I = demo_g2do_74()
     (g,), _ = run_instances([I], exact=True)
     o32 = orc.run(I)
     np.testing.assert_array_equal(g.traj, o32["traj"])

@@ -16,7 +16,7 @@ import pytest
 
 from hyburn_amd import _lib as L
 from hyburn_amd.instances import Instance, Plan, philox_normal as gpu_philox, run_instances
-from hyburn_amd.synth import DEFAULTS, config2, dense_to_csr, lognormal_connectome, philox_normal
+from hyburn_amd.synth import DEFAULTS, sweep_g2do_74, dense_to_csr, lognormal_connectome, philox_normal
 
 pytestmark = pytest.mark.gpu
 
@@ -241,10 +241,10 @@ def test_generated_noise_path(gpu, orc):
     np.testing.assert_allclose(g.traj, o["traj"], rtol=1e-6, atol=1e-7)
 
 
-def test_config2_full_length_exact_subset(gpu, orc):
+def test_sweep_g2do_74_full_length_exact_subset(gpu, orc):
     """The batched-sweep workload at full length (10k steps) for 8 of its 128
     points: bit-identical to the oracle in exact mode."""
-    insts = config2(n_sweep=128, explicit_noise=False)
+    insts = sweep_g2do_74(n_sweep=128, explicit_noise=False)
     sub = [insts[i] for i in (0, 17, 40, 63, 64, 90, 111, 127)]
     for I in sub:
         I.noise = philox_normal(I.noise_seed, I.noise_stream, I.n_steps * I.n_nodes * 2).reshape(I.n_steps, I.n_nodes, 2)
@@ -367,9 +367,9 @@ def test_autotuned_plan_matches_default(gpu, orc):
     results stay within the FP32 bound of the default shapes, and every tuned
     class records one of its candidates."""
     from hyburn_amd.instances import autotune_entries
-    from hyburn_amd.synth import config3
+    from hyburn_amd.synth import ragged_mpr_jr
     L.lib().hb_autotune_clear()
-    insts = config3(n_inst=24, t_scale=0.02, bold=False)
+    insts = ragged_mpr_jr(n_inst=24, t_scale=0.02, bold=False)
     for I in insts:
         I.record_traj = False
     a, _ = run_instances(insts, autotune=False)
