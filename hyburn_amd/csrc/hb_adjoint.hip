@@ -1813,10 +1813,10 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
         }
         (void)s;
     };
-    // win: BOLD window of step s, (s - 1) / P, tracked by the caller
-    auto seed = [&](int s, int t0s, const float* xs, float* l, int win) {
+    // sw: dL/d(window input) / count of step s's BOLD window (prefetched by the caller)
+    auto seed = [&](int s, int t0s, const float* xs, float* l, float sw) {
         if (s < 1) return;
-        if (seed_win) l[0] += V.seedw[(size_t)win * N + tid];
+        if (seed_win) l[0] += sw;
         if (seed_ord) {
             const gp<float> cf = V.ord + (size_t)(s - t0s) * 4;
             l[0] += cf[0] * sinf(xs[0]) + cf[1] * cosf(xs[0]);
@@ -1915,15 +1915,19 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
             float xT[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) xT[v] = V.tape[((size_t)(t1s - t0s) * N + tid) * NV + v];
-            seed(T, t0s, xT, lam, seed_win ? (T - 1) / P : 0);
+            seed(T, t0s, xT, lam, seed_win ? V.seedw[(size_t)((T - 1) / P) * N + tid] : 0.0f);
         }
         // ---- blocks of reverse steps over [max(t0s, 1), t1s - 1], highest first
         const int tlo = max(t0s, 1);
         // the block's states and coupling log rows; the next block's load while
         // this one is differentiated when they fit the register budget
-        constexpr bool kPrefetch = (NV + NP) * kK <= 24;
-        auto load_block = [&](float (&xx)[kK][NV], float (&gg)[kK][NP], int tb, int nk) {
+        // (Wilson-Cowan's sigmoid VJPs leave no registers for a second block)
+        constexpr bool kPrefetch = (NV + NP) * kK <= 24 && M != HB_WILSON_COWAN;
+        // ... and the BOLD seeds of its steps (window (st - 1) / P, stepped up)
+        auto load_block = [&](float (&xx)[kK][NV], float (&gg)[kK][NP], float (&ss)[kK], int tb, int nk) {
             if (!own) return;
+            int wl = 0, rl = 0;
+            if (seed_win) { wl = (tb - 1) / P; rl = (tb - 1) - wl * P; }
 #pragma unroll
             for (int m = 0; m < kK; ++m) {
                 if (m < nk) {
@@ -1934,18 +1938,23 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                     for (int v = 0; v < NV; ++v) xx[m][v] = tp[v];
 #pragma unroll
                     for (int c = 0; c < NP; ++c) gg[m][c] = gl[c];
+                    ss[m] = 0.0f;
+                    if (seed_win) {
+                        ss[m] = V.seedw[(size_t)wl * N + tid];
+                        if (++rl == P) { rl = 0; ++wl; }
+                    }
                 }
             }
         };
-        float xb[kK][NV], gb[kK][NP];
-        float xn[kPrefetch ? kK : 1][NV], gn[kPrefetch ? kK : 1][NP];
+        float xb[kK][NV], gb[kK][NP], sdb[kK];
+        float xn[kPrefetch ? kK : 1][NV], gn[kPrefetch ? kK : 1][NP], sdn[kPrefetch ? kK : 1];
         bool have = false;   // xb / gb already hold the block (prefetched)
         for (int ttop = t1s - 1; ttop >= tlo; ttop -= kK) {
             const int kk = min(kK, ttop - tlo + 1), tbot = ttop - kk + 1;
-            if (!have) load_block(xb, gb, tbot, kk);
+            if (!have) load_block(xb, gb, sdb, tbot, kk);
             const int ttop2 = ttop - kK, kk2 = min(kK, ttop2 - tlo + 1), tbot2 = ttop2 - kk2 + 1;
             if constexpr (kPrefetch) {
-                if (ttop2 >= tlo) load_block(xn, gn, tbot2, kk2);
+                if (ttop2 >= tlo) load_block(xn, gn, sdn, tbot2, kk2);
             }
             float acc[kK][NP];
 #pragma unroll
@@ -1982,9 +1991,6 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
             }
             float lgb[kK][NP];
             if (own) {
-                // BOLD window of the block's top step and its position, stepped down per step
-                int win = 0, wr = 0;
-                if (seed_win) { win = (ttop - 1) / P; wr = (ttop - 1) - win * P; }
 #pragma unroll
                 for (int m = kK - 1; m >= 0; --m) {
                     if (m < kk) {
@@ -1994,8 +2000,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                         for (int v = 0; v < NV; ++v) zs[v] = S.stoch ? V.noise[((size_t)st * N + tid) * NV + v] * nsc[v] : 0.0f;
                         vjp(st, xb[m], gb[m], zs, lx, lgb[m]);
                         pre_vjp<NC, NP>(xb[m], acc[m], lx);
-                        seed(st, t0s, xb[m], lx, win);
-                        if (--wr < 0) { wr = P - 1; --win; }
+                        seed(st, t0s, xb[m], lx, sdb[m]);
 #pragma unroll
                         for (int v = 0; v < NV; ++v) lam[v] = lx[v];
                     }
@@ -2019,6 +2024,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                         for (int v = 0; v < NV; ++v) xb[m][v] = xn[m][v];
 #pragma unroll
                         for (int c = 0; c < NP; ++c) gb[m][c] = gn[m][c];
+                        sdb[m] = sdn[m];
                     }
                     have = true;
                 }
