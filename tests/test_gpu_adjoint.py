@@ -38,6 +38,9 @@ CASES = {
     # graph in torch with an FP32 neural recursion shows it); at G = 1 the
     # FP32 floor is ~1e-6
     "rww_bold_mse_heun": lambda orc: bold_mse_case(L.RWW, orc, G=1.0, node_param_idx=7),
+    # 30 BOLD windows: the BOLD adjoint's single-chunk path (the cases above split
+    # their 200-400 windows into chunks of the affine scan)
+    "g2do_bold_mse_heun_short": lambda orc: bold_mse_case(L.G2DO, orc, node_param_idx=2, T=150),
 }
 
 
