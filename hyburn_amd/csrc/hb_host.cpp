@@ -127,9 +127,13 @@ struct hb_plan {
     std::vector<LaunchDesc> launches;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    // launch groups run as parallel branches: one side stream and join event each
+    // launch groups run as parallel branches on up to kLanes side streams (one
+    // join event each); lane_of[k] is the lane of launch k, estimated time
+    // balanced (longest group first onto the least loaded lane)
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> join;
+    std::vector<int> lane_of;
+    std::vector<double> launch_cost;
     hipEvent_t fork = nullptr;
     hipGraphExec_t g_noise = nullptr, g_fwd = nullptr;
     hipGraph_t gr_noise = nullptr, gr_fwd = nullptr;
@@ -244,8 +248,8 @@ extern "C" void hb_plan_destroy(hb_plan* p);
 namespace {
 
 // Launch every group: one group on the plan stream, several as parallel
-// branches (fork from the plan stream, one side stream each, join back), so
-// groups of different launch shapes share the chip instead of queueing.
+// branches (fork from the plan stream onto the lanes' side streams, join
+// back), so groups of different launch shapes share the chip.
 thread_local std::string g_launch_err;   // the step of launch_groups that failed
 
 hipError_t launch_groups(hb_plan* P, bool exact) {
@@ -266,17 +270,28 @@ hipError_t launch_groups(hb_plan* P, bool exact) {
     }
     hipError_t e = hipEventRecord(P->fork, P->stream);
     if (e != hipSuccess) g_launch_err = "fork record";
-    for (size_t k = 0; k < P->launches.size() && e == hipSuccess; ++k) {
-        const LaunchDesc& L = P->launches[k];
-        const char* step = "branch wait";
-        e = hipStreamWaitEvent(P->side[k], P->fork, 0);
-        if (e == hipSuccess) { step = "kernel launch"; e = go(L, P->side[k]); }
-        if (e == hipSuccess) { step = "join record"; e = hipEventRecord(P->join[k], P->side[k]); }
-        if (e == hipSuccess) { step = "join wait"; e = hipStreamWaitEvent(P->stream, P->join[k], 0); }
-        if (e != hipSuccess)
-            g_launch_err = std::string(step) + " of group " + std::to_string(k) + " (model " + std::to_string(L.model) +
-                           ", np " + std::to_string(L.np) + ", block " + std::to_string(L.block) + ", grid " +
-                           std::to_string(L.grid) + ", lds " + std::to_string(L.lds_bytes) + ")";
+    for (size_t l = 0; l < P->side.size() && e == hipSuccess; ++l) {
+        e = hipStreamWaitEvent(P->side[l], P->fork, 0);
+        if (e != hipSuccess) g_launch_err = "branch wait of lane " + std::to_string(l);
+        // the lane's groups, longest first
+        std::vector<size_t> mine;
+        for (size_t k = 0; k < P->launches.size(); ++k)
+            if ((size_t)P->lane_of[k] == l) mine.push_back(k);
+        std::stable_sort(mine.begin(), mine.end(), [&](size_t a, size_t b) { return P->launch_cost[a] > P->launch_cost[b]; });
+        for (size_t k : mine) {
+            if (e != hipSuccess) break;
+            const LaunchDesc& L = P->launches[k];
+            e = go(L, P->side[l]);
+            if (e != hipSuccess)
+                g_launch_err = "kernel launch of group " + std::to_string(k) + " (model " + std::to_string(L.model) +
+                               ", np " + std::to_string(L.np) + ", block " + std::to_string(L.block) + ", grid " +
+                               std::to_string(L.grid) + ", lds " + std::to_string(L.lds_bytes) + ")";
+        }
+        if (e == hipSuccess) {
+            e = hipEventRecord(P->join[l], P->side[l]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(P->stream, P->join[l], 0);
+            if (e != hipSuccess) g_launch_err = "join of lane " + std::to_string(l);
+        }
     }
     return e;
 }
@@ -825,6 +840,10 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
             }
         }
         if (opts && opts->grid_waves > 0 && L.counter) L.grid = std::min(L.n_work, opts->grid_waves);
+        // estimated time: the longest instance, or the work spread over the grid
+        double csum = 0.0, cmax = 0.0;
+        for (int i : v) { csum += sh[i].cost; cmax = std::max(cmax, sh[i].cost); }
+        P->launch_cost.push_back(std::max(cmax, csum / std::max(1, L.grid)));
         P->launches.push_back(L);
         ++gi;
         P->block_main = std::max(P->block_main, L.block);
@@ -835,10 +854,24 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
     HB_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     for (auto& e : P->ev) HB_HIP(hipEventCreate(&e));
     if (P->launches.size() > 1) {
+        // as many lanes as the device has hardware queues for a process by
+        // default (more streams would share queues and serialize behind each other)
+        constexpr size_t kLanes = 4;
+        const size_t nl = std::min(kLanes, P->launches.size());
+        std::vector<size_t> byc(P->launches.size());
+        for (size_t k = 0; k < byc.size(); ++k) byc[k] = k;
+        std::stable_sort(byc.begin(), byc.end(), [&](size_t a, size_t b) { return P->launch_cost[a] > P->launch_cost[b]; });
+        std::vector<double> load(nl, 0.0);
+        P->lane_of.assign(P->launches.size(), 0);
+        for (size_t k : byc) {
+            const size_t l = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+            P->lane_of[k] = (int)l;
+            load[l] += P->launch_cost[k];
+        }
         HB_HIP(hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
-        P->side.assign(P->launches.size(), nullptr);
-        P->join.assign(P->launches.size(), nullptr);
-        for (size_t k = 0; k < P->launches.size(); ++k) {
+        P->side.assign(nl, nullptr);
+        P->join.assign(nl, nullptr);
+        for (size_t k = 0; k < nl; ++k) {
             HB_HIP(hipStreamCreateWithFlags(&P->side[k], hipStreamNonBlocking));   // default priority
             HB_HIP(hipEventCreateWithFlags(&P->join[k], hipEventDisableTiming));
         }
