@@ -154,6 +154,9 @@ PROTOTYPES = {
     "hb_adjoint_plan_destroy": (None, [C.c_void_p]),
     "hb_autotune_entries": (C.c_int, [ip, C.c_int32]),
     "hb_autotune_clear": (None, []),
+    "hb_fc_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
+    "hb_fcd_windows": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
+    "hb_fcd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
 }
 
 _lib = None

@@ -271,6 +271,26 @@ void hb_adjoint_plan_destroy(hb_adjoint_plan* p);
 int hb_autotune_entries(int32_t* out, int32_t max_entries);
 void hb_autotune_clear(void);
 
+/* ---------------- functional-connectivity features ---------------- */
+/* Batched FC: series is [n_series][n_steps][n_nodes] row-major (host memory),
+ * fc receives [n_series][n_nodes][n_nodes]. Each matrix follows fc_matrix
+ * (src/sbi/features/fc.rs:11-57): FP64 means and covariances, population
+ * standard deviation (< 1e-12 -> 1), clamp to [-1, 1], f32, symmetric. */
+int hb_fc_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t n_nodes, float* fc,
+                int32_t device);
+/* Number of FCD windows, (n_steps - window_len) / stride + 1, as fcd_matrix
+ * counts them (src/sbi/features/fc.rs:104); 0 for an invalid window. */
+int32_t hb_fcd_windows(int32_t n_steps, int32_t window_len, int32_t stride);
+/* Batched FCD (src/sbi/features/fc.rs:94-179 fcd_matrix): per series, the FC
+ * of each window of window_len steps every stride steps, then the Pearson
+ * correlation of the windows' upper triangles (0 when either is constant),
+ * diagonal 1. fcd receives [n_series][W][W], W = hb_fcd_windows(...); when
+ * W <= 1 nothing is written (the reference returns an empty matrix). Returns
+ * HB_INVALID_CONFIG where fcd_matrix asserts: window_len outside [2, n_steps]
+ * or stride < 1. */
+int hb_fcd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t n_nodes, int32_t window_len,
+                 int32_t stride, float* fcd, int32_t device);
+
 /* ---------------- misc ---------------- */
 const char* hb_last_error(void);   /* thread-local message of the last failure */
 const char* hb_version(void);

@@ -12,7 +12,7 @@
  *   src/engine/coupling.rs:114-444                    (coupling pre/post)
  *   src/engine/sparse.rs:29-52, runtime.rs:121-211    (CSR, per-edge delays)
  *   src/engine/bold.rs:106-166, bold_monitor.rs:72-227 (Balloon-Windkessel)
- *   src/sbi/features/fc.rs:11-57                      (Pearson FC)
+ *   src/sbi/features/fc.rs:11-57,94-179               (Pearson FC, FCD)
  * Transcendentals come from the platform libm (expf/sinf/...), as in the
  * reference (Rust f32::exp etc. call libm).
  *
@@ -316,6 +316,55 @@ int hb_orc_fc(const float* traj, int T, int N, float* out)
     }
     free(mean); free(sd);
     return 0;
+}
+
+/* Pearson correlation of the i < j entries of two N x N matrices
+ * (upper_triangle_correlation, src/sbi/features/fc.rs:143-178). */
+static float orc_upper_corr(const float* a, const float* b, int N)
+{
+    double n = (double)N * (N - 1) / 2, sa = 0.0, sb = 0.0, ma, mb, cov = 0.0, va = 0.0, vb = 0.0, r;
+    int i, j;
+    for (i = 0; i < N; ++i) for (j = i + 1; j < N; ++j) { sa += a[(size_t)i * N + j]; sb += b[(size_t)i * N + j]; }
+    ma = sa / n;
+    mb = sb / n;
+    for (i = 0; i < N; ++i)
+        for (j = i + 1; j < N; ++j) {
+            double da = (double)a[(size_t)i * N + j] - ma, db = (double)b[(size_t)i * N + j] - mb;
+            cov += da * db;
+            va += da * da;
+            vb += db * db;
+        }
+    if (va <= 0.0 || vb <= 0.0) return 0.0f;
+    r = cov / (sqrt(va) * sqrt(vb));
+    if (r < -1.0) r = -1.0;
+    if (r > 1.0) r = 1.0;
+    return (float)r;
+}
+
+/* FCD matrix (fcd_matrix, src/sbi/features/fc.rs:94-141): the FC of each
+ * window of L steps every S steps, then the upper-triangle correlations.
+ * Writes W x W into out and returns W; returns 0 (nothing written) when
+ * W <= 1, -1 where the reference asserts (L outside [2, T], S < 1). */
+int hb_orc_fcd(const float* traj, int T, int N, int L, int S, float* out)
+{
+    int W, w, a, b;
+    float* fcs;
+    if (L < 2 || L > T || S < 1) return -1;
+    W = (T - L) / S + 1;
+    if (W <= 1) return 0;
+    fcs = (float*)malloc(sizeof(float) * (size_t)W * N * N);
+    if (!fcs) return -1;
+    for (w = 0; w < W; ++w) hb_orc_fc(traj + (size_t)w * S * N, L, N, fcs + (size_t)w * N * N);
+    for (a = 0; a < W; ++a) {
+        out[(size_t)a * W + a] = 1.0f;
+        for (b = a + 1; b < W; ++b) {
+            float r = orc_upper_corr(fcs + (size_t)a * N * N, fcs + (size_t)b * N * N, N);
+            out[(size_t)a * W + b] = r;
+            out[(size_t)b * W + a] = r;
+        }
+    }
+    free(fcs);
+    return W;
 }
 
 /* Effective source time read for delay d at step s (-1: the zero/pre(0) value).

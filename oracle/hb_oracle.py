@@ -67,6 +67,7 @@ def lib() -> C.CDLL:
         L.hb_orc_bold_downsample.argtypes = [C.POINTER(C.c_double), fp, C.c_int, C.c_int, C.c_double,
                                              C.c_double, fp, C.c_int]
         L.hb_orc_fc.argtypes = [fp, C.c_int, C.c_int, fp]
+        L.hb_orc_fcd.argtypes = [fp, C.c_int, C.c_int, C.c_int, C.c_int, fp]
         L.hb_orc_source_time.argtypes = [C.c_int, C.c_int64, C.c_int64, C.c_int64]
         L.hb_orc_source_time.restype = C.c_int64
         L.hb_orc_max_threads.restype = C.c_int
@@ -289,6 +290,18 @@ def fc(traj):
     out = np.zeros((N, N), np.float32)
     lib().hb_orc_fc(_p(t), T, N, _p(out))
     return out
+
+
+def fcd(traj, window_len, stride):
+    """FCD matrix of a [T][N] series (fc.rs:94-179); empty (0, 0) when W <= 1."""
+    t = np.ascontiguousarray(traj, np.float32)
+    T, N = t.shape
+    W = (T - window_len) // stride + 1 if 2 <= window_len <= T and stride >= 1 else 0
+    out = np.zeros((max(W, 0), max(W, 0)), np.float32)
+    rc = lib().hb_orc_fcd(_p(t), T, N, window_len, stride, _p(out))
+    if rc < 0:
+        raise ValueError("window_len must be in [2, n_steps] and stride > 0")
+    return out if rc > 1 else np.zeros((0, 0), np.float32)
 
 
 def source_time(mode, s, d, H):

@@ -193,3 +193,24 @@ def test_write_npy_f32_format(tmp_path):
     back = np.load(str(p), allow_pickle=False)
     assert back.shape == (2, 3, 4) and back.dtype == np.dtype("<f4")
     np.testing.assert_array_equal(back.ravel(), data)
+
+
+def test_fc_fcd_host_validation():
+    # fcd_matrix asserts (fc.rs:101-102) -> InvalidConfig before any device call
+    lib = L.lib()
+    x = np.zeros((2, 10, 3), np.float32)
+    out = np.zeros(1000, np.float32)
+    xp, op = x.ctypes.data_as(L.fp), out.ctypes.data_as(L.fp)
+    assert lib.hb_fcd_windows(10, 4, 2) == 4
+    assert lib.hb_fcd_windows(10, 10, 3) == 1
+    assert lib.hb_fcd_windows(10, 1, 1) == 0 and lib.hb_fcd_windows(10, 11, 1) == 0
+    assert lib.hb_fcd_windows(10, 4, 0) == 0
+    for L_, S_ in [(1, 1), (11, 1), (4, 0)]:
+        assert lib.hb_fcd_batch(xp, 2, 10, 3, L_, S_, op, 0) == L.HB_INVALID_CONFIG
+    assert lib.hb_fcd_batch(xp, 0, 10, 3, 4, 2, op, 0) == L.HB_INVALID_CONFIG
+    assert lib.hb_fc_batch(xp, 2, 0, 3, op, 0) == L.HB_INVALID_CONFIG
+    assert lib.hb_fc_batch(None, 2, 10, 3, op, 0) == L.HB_INVALID_CONFIG
+    # a single window: the reference returns an empty matrix, nothing is written
+    out[:] = 7.0
+    assert lib.hb_fcd_batch(xp, 2, 10, 3, 10, 1, op, 0) == L.HB_OK
+    assert np.all(out == 7.0)

@@ -273,6 +273,61 @@ def test_fc_matches_pearson(orc):
     np.testing.assert_array_equal(F, F.T)
 
 
+def test_fc_reference_kats(orc):
+    # fc.rs:225-249: perfectly (anti-)correlated two-node series
+    F = orc.fc(np.array([[1, 1], [2, 2], [3, 3]], np.float32))
+    np.testing.assert_allclose(F, 1.0, atol=1e-5)
+    F = orc.fc(np.array([[1, -1], [2, -2], [3, -3]], np.float32))
+    assert abs(F[0, 1] + 1.0) < 1e-5 and abs(F[1, 0] + 1.0) < 1e-5
+
+
+def test_fcd_reference_kat(orc):
+    # fc.rs:268-275: sin(i) over 10 steps x 3 nodes, window 4, stride 2 -> 4 windows
+    X = np.sin(np.arange(30, dtype=np.float32)).reshape(10, 3)
+    D = orc.fcd(X, 4, 2)
+    assert D.shape == (4, 4)
+    assert abs(D[0, 0] - 1.0) < 1e-5
+
+
+def _upper(F):
+    i, j = np.triu_indices(F.shape[0], 1)
+    return F[i, j].astype(np.float64)
+
+
+def test_fcd_matches_corrcoef_of_window_fcs(orc):
+    # fc.rs:94-179: windows [w*S, w*S+L), FC per window, Pearson of the i<j entries
+    rng = np.random.default_rng(11)
+    T, N, L, S = 230, 9, 40, 15
+    X = np.cumsum(rng.normal(size=(T, N)), axis=0).astype(np.float32)
+    D = orc.fcd(X, L, S)
+    W = (T - L) // S + 1
+    assert D.shape == (W, W)
+    ups = np.stack([_upper(np.corrcoef(X[w * S:w * S + L].astype(np.float64), rowvar=False))
+                    for w in range(W)])
+    np.testing.assert_allclose(D, np.corrcoef(ups), atol=2e-6)
+    assert np.all(np.diag(D) == 1.0)
+    np.testing.assert_array_equal(D, D.T)
+
+
+def test_fcd_closed_forms(orc):
+    rng = np.random.default_rng(12)
+    # a series that repeats every S steps: every window has the same FC -> all ones
+    base = rng.normal(size=(20, 6)).astype(np.float32)
+    X = np.tile(base, (8, 1))
+    D = orc.fcd(X, 40, 20)
+    np.testing.assert_allclose(D, 1.0, atol=1e-6)
+    # two nodes: one upper entry per window -> zero variance -> 0 off-diagonal
+    D2 = orc.fcd(rng.normal(size=(100, 2)).astype(np.float32), 20, 10)
+    assert D2.shape == (9, 9)
+    np.testing.assert_array_equal(D2, np.eye(9, dtype=np.float32))
+    # fewer than two windows -> empty; the asserts of fcd_matrix -> errors
+    assert orc.fcd(X, 160, 1).shape == (0, 0)
+    assert orc.fcd(X, 100, 100).shape == (0, 0)
+    for L, S in [(1, 1), (161, 1), (10, 0)]:
+        with pytest.raises(ValueError):
+            orc.fcd(X, L, S)
+
+
 # ---------------------------------------------------------------- delays ----
 
 def test_slot_tables_ring(orc):
