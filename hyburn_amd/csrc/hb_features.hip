@@ -8,27 +8,42 @@
 //                 Pearson correlation of the windows' upper triangles
 //                 (upper_triangle_correlation, fc.rs:143-178), diagonal 1.
 //
-// The covariance Zc^T Zc is the GEMM-shaped part: for N <= 96 it runs on the
-// FP64 matrix cores (v_mfma_f64_16x16x4f64, lower-triangle 16x16 tiles over
-// sample chunks staged in LDS), one workgroup of four waves per series or
-// window; wider networks take a pair-per-thread FP64 loop.
+// The cross products Zc^T Zc of the centred columns are the GEMM-shaped part
+// and run on the FP64 matrix cores (v_mfma_f64_16x16x4f64 over sample chunks
+// staged in LDS), one workgroup of four waves per 96 x 96 block of one
+// matrix. The FCD stage is the same kernel over the windows' upper triangles,
+// written transposed so that each window is one column.
 #include "hb_internal.h"
 
+#include <algorithm>
 #include <vector>
 
 namespace hb {
 namespace feat {
 
-// One FC to compute: rows x[t * ldx + n], t < T, n < N; out receives the N x N
-// matrix (upper == 0) or the i < j entries in row order (upper == 1).
+// One correlation matrix to compute from the columns of x[t * ldx + n],
+// t < T, n < N (FP64 column sums of nseg sample segments in mean[nseg][N],
+// filled by k_fc_means).
+//   fcd == 0: fc_matrix semantics (fc.rs:11-57): population std, std < 1e-12
+//             -> 1, r = (cov / T) / (std_i std_j).
+//   fcd == 1: upper_triangle_correlation semantics (fc.rs:143-178): r =
+//             cov / (sqrt(var_i) sqrt(var_j)) of the unscaled centred sums, 0
+//             when either var <= 0, diagonal 1.
+// Either way r is clamped to [-1, 1] and rounded to f32 once. The result goes
+// to out[(i * N + j) * ostr] for every (i, j) (upper == 0), or, upper == 1, to
+// out[u * ostr] for the i < j entries numbered u in row order.
 struct FcJob {
     const float* x;
     float* out;
-    int32_t T, N, ldx, upper;
+    const double* mean;
+    int32_t T, N, ldx, upper, ostr, fcd;
 };
 
-constexpr int kTile = 96, kChunk = 32, kCs = kTile + 1;   // padded chunk row stride (doubles)
-constexpr int kTilesPerWave = 6;                          // ceil(21 lower-triangle tiles / 4 waves)
+constexpr int kBlk = 96;                  // columns per block of the correlation matrix
+constexpr int kChunk = 32;                // samples staged per LDS pass
+constexpr int kCs = kBlk + 1;             // padded chunk row stride (doubles)
+constexpr int kTilesPerWave = 9;          // 36 tiles of an off-diagonal 96 x 96 block / 4 waves
+constexpr int kTilesPerWaveOne = 6;       // 21 lower-triangle tiles of a lone diagonal block / 4 waves
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
@@ -37,178 +52,224 @@ __device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
 
 __device__ __forceinline__ void put(const FcJob& J, int i, int j, float r) {
     if (J.upper) {
-        if (j > i) J.out[(size_t)i * J.N - (size_t)i * (i + 1) / 2 + (j - i - 1)] = r;
-        else if (i > j) J.out[(size_t)j * J.N - (size_t)j * (j + 1) / 2 + (i - j - 1)] = r;
+        if (j > i) J.out[((size_t)i * J.N - (size_t)i * (i + 1) / 2 + (j - i - 1)) * J.ostr] = r;
+        else if (i > j) J.out[((size_t)j * J.N - (size_t)j * (j + 1) / 2 + (i - j - 1)) * J.ostr] = r;
     } else {
-        J.out[(size_t)i * J.N + j] = r;
-        J.out[(size_t)j * J.N + i] = r;
+        J.out[((size_t)i * J.N + j) * J.ostr] = r;
+        J.out[((size_t)j * J.N + i) * J.ostr] = r;
     }
 }
 
-__device__ __forceinline__ float pearson(double cov, double si, double sj) {
-    double r = cov / (si * sj);
-    r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
-    return (float)r;
-}
+__device__ __forceinline__ double clamp1(double r) { return r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r); }
 
-// Column means over the rows, FP64 (N <= 96): thread (column c, phase ph)
-// sums rows ph, ph + nph, ... and the phases are added in order.
-__device__ void column_means(const FcJob& J, double* part, double* mean) {
-    const int N = J.N, T = J.T, tid = threadIdx.x;
-    const int nph = min(4, (int)blockDim.x / N);
-    const int c = tid % N, ph = tid / N;
-    if (ph < nph) {
-        double a = 0.0;
-        for (int t = ph; t < T; t += nph) a += (double)J.x[(size_t)t * J.ldx + c];
-        part[ph * kTile + c] = a;
-    }
-    __syncthreads();
-    if (tid < N) {
-        double a = 0.0;
-        for (int q = 0; q < nph; ++q) a += part[q * kTile + tid];
-        mean[tid] = a / (double)T;
-    }
-    __syncthreads();
-}
-
-// N <= 96: covariance tiles on the matrix cores. Fragment layout of the
-// 16x16x4 FP64 product (lane l, item r): A[i][k] = a(lane i + 16 k),
-// B[k][j] = b(lane j + 16 k), D[l / 16 + 4 r][l % 16] = d(lane l)[r].
-__global__ __launch_bounds__(256) void k_fc_tiles(const FcJob* __restrict__ jobs) {
-    __shared__ double chunk[kChunk * kCs];
-    __shared__ double mean[kTile], sd[kTile], part[4 * kTile];
+// Column sums, FP64, over sample segment z of ceil(T / nseg) rows:
+// workgroup (job, 64-column group, segment); thread (column c, phase ph < 4)
+// sums rows ph, ph + 4, ... of the segment and the four phases are added in
+// order into part[z][c] (the job's mean pointer). Consecutive threads read
+// consecutive columns of a row.
+__global__ __launch_bounds__(256) void k_fc_means(const FcJob* __restrict__ jobs, int nseg) {
+    __shared__ double part[4][64];
     const FcJob J = jobs[blockIdx.x];
+    const int tid = threadIdx.x, cl = tid & 63, ph = tid >> 6;
+    const int c = blockIdx.y * 64 + cl, z = blockIdx.z;
+    const int seg = (J.T + nseg - 1) / nseg, t0 = z * seg, t1 = min(J.T, t0 + seg);
+    double a = 0.0;
+    if (c < J.N)
+        for (int t = t0 + ph; t < t1; t += 4) a += (double)J.x[(size_t)t * J.ldx + c];
+    part[ph][cl] = a;
+    __syncthreads();
+    if (tid < 64 && c < J.N)
+        ((double*)J.mean)[(size_t)z * J.N + c] = ((part[0][cl] + part[1][cl]) + part[2][cl]) + part[3][cl];
+}
+
+// Per-column scale: fc_matrix's population std (< 1e-12 -> 1), or the raw
+// sum of squares for upper_triangle_correlation.
+__device__ __forceinline__ double col_scale(const FcJob& J, double var) {
+    if (J.fcd) return var;
+    const double s = sqrt(var / (double)J.T);
+    return s < 1e-12 ? 1.0 : s;
+}
+
+__device__ __forceinline__ double corr_value(const FcJob& J, int i, int j, double cov, double si, double sj) {
+    if (!J.fcd) return clamp1((cov / (double)J.T) / (si * sj));
+    if (i == j) return 1.0;
+    if (si <= 0.0 || sj <= 0.0) return 0.0;
+    return clamp1(cov / (sqrt(si) * sqrt(sj)));
+}
+
+// Lower-triangle block pair bp -> (ib, jb), ib >= jb.
+__device__ __forceinline__ void block_pair(int bp, int& ib, int& jb) {
+    ib = 0;
+    while ((ib + 1) * (ib + 2) / 2 <= bp) ++ib;
+    jb = bp - ib * (ib + 1) / 2;
+}
+
+constexpr int kPart = kBlk * kBlk + 2 * kBlk;   // doubles of one partial: products, then sums of squares
+
+// One 96 x 96 block (ib >= jb) of one job's correlation matrix per
+// workgroup of four waves. Centred samples of the block's row and column
+// ranges are staged in LDS 32 at a time; the cross products run on the FP64
+// matrix cores in 16 x 16 tiles (the lower triangle of a diagonal block), and
+// the per-column sums of squares on the vector units in sample order. With
+// nsplit > 1 the samples are split into nsplit chunk-aligned segments, one
+// workgroup each, which store raw partial sums for k_fc_finish.
+// Fragment layout of the 16x16x4 FP64 product (lane l, item r):
+// A[i][k] = a(lane i + 16 k), B[k][j] = b(lane j + 16 k),
+// D[l / 16 + 4 r][l % 16] = d(lane l)[r]. ONE: the matrix is a single
+// diagonal block (N <= 96), which needs neither the second column-range
+// buffers nor more than 6 tiles per wave.
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_fc_tiles(const FcJob* __restrict__ jobs, int nbp, int nseg, int nsplit,
+                                                  double* __restrict__ partial) {
+    constexpr int kTpw = ONE ? kTilesPerWaveOne : kTilesPerWave;
+    __shared__ double cI[kChunk * kCs], cJ[ONE ? 1 : kChunk * kCs];
+    __shared__ double mI[kBlk], mJ[kBlk], sI[kBlk], sJ[kBlk];
+    const int z = blockIdx.x % nsplit, rest = blockIdx.x / nsplit;
+    const int job = rest / nbp, bp = rest - job * nbp;
+    const FcJob J = jobs[job];
     const int N = J.N, T = J.T, tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
-    constexpr int nwave = 4;
-    const int nt = (N + 15) >> 4, NP = nt * 16;
-    column_means(J, part, mean);
-    const int ntt = nt * (nt + 1) / 2;
-    int tti[kTilesPerWave], ttj[kTilesPerWave];
-    d4 acc[kTilesPerWave];
+    const int cps = ((T + kChunk - 1) / kChunk + nsplit - 1) / nsplit;   // chunks per segment
+    const int kbeg = min(T, z * cps * kChunk), kend = min(T, kbeg + cps * kChunk);
+    int ib, jb;
+    block_pair(bp, ib, jb);
+    const bool diag = ONE || ib == jb;
+    const int I0 = ib * kBlk, J0 = jb * kBlk;
+    const int nI = min(kBlk, N - I0), nJ = min(kBlk, N - J0);
+    const int tI = (nI + 15) >> 4, tJ = (nJ + 15) >> 4, wI = tI * 16, wJ = tJ * 16;
+    const int ntt = diag ? tI * (tI + 1) / 2 : tI * tJ;
+    double* bJ = diag ? cI : cJ;
+    double* sB = diag ? sI : sJ;
+    if (tid < nI || (tid >= 128 && tid - 128 < nJ)) {
+        const int c = tid < 128 ? I0 + tid : J0 + tid - 128;
+        double a = 0.0;
+        for (int z = 0; z < nseg; ++z) a += J.mean[(size_t)z * N + c];
+        (tid < 128 ? mI : mJ)[tid & 127] = a / (double)T;
+    }
+    int tti[kTpw], ttj[kTpw];
+    d4 acc[kTpw];
 #pragma unroll
-    for (int q = 0; q < kTilesPerWave; ++q) {
-        const int t = wave + q * nwave;
+    for (int q = 0; q < kTpw; ++q) {
+        const int t = wave + 4 * q;
         int ti = 0, tj = 0;
         if (t < ntt) {
-            while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-            tj = t - ti * (ti + 1) / 2;
-        }
-        tti[q] = ti; ttj[q] = tj;
-        acc[q] = d4{0.0, 0.0, 0.0, 0.0};
-    }
-    for (int k0 = 0; k0 < T; k0 += kChunk) {
-        const int kc = min(kChunk, T - k0);
-        __syncthreads();
-        for (int q = tid; q < kChunk * NP; q += 256) {
-            const int k = q / NP, c = q - k * NP;
-            chunk[k * kCs + c] = (k < kc && c < N) ? (double)J.x[(size_t)(k0 + k) * J.ldx + c] - mean[c] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < kChunk; kk += 4) {
-            const double* row = chunk + (kk + lq) * kCs + lr;
-#pragma unroll
-            for (int q = 0; q < kTilesPerWave; ++q)
-                if (wave + q * nwave < ntt) acc[q] = mfma_f64(row[tti[q] * 16], row[ttj[q] * 16], acc[q]);
-        }
-    }
-    // population standard deviations from the diagonal tiles
-#pragma unroll
-    for (int q = 0; q < kTilesPerWave; ++q) {
-        if (wave + q * nwave < ntt && tti[q] == ttj[q]) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = tti[q] * 16 + lq + 4 * r, j = ttj[q] * 16 + lr;
-                if (i == j && i < N) {
-                    const double s = sqrt(acc[q][r] / (double)T);
-                    sd[i] = s < 1e-12 ? 1.0 : s;
-                }
+            if (diag) {
+                while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+                tj = t - ti * (ti + 1) / 2;
+            } else {
+                ti = t / tJ;
+                tj = t - ti * tJ;
             }
         }
+        tti[q] = ti;
+        ttj[q] = tj;
+        acc[q] = d4{0.0, 0.0, 0.0, 0.0};
     }
+    // thread tid < nI owns row column I0 + tid; 128 + c < 128 + nJ owns J0 + c
+    const int vc = tid < kBlk ? tid : tid - 128;
+    const bool own_i = tid < nI, own_j = !diag && tid >= 128 && vc < nJ;
+    double var = 0.0;
+    // the next chunk's samples are loaded into registers while the current
+    // chunk's products run: slot m of this thread is q = tid + 256 m
+    constexpr int kSlots = kChunk * kBlk / 256;
+    float pI[kSlots], pJ[ONE ? 1 : kSlots];
+    auto fetch = [&](int k0) {
+        const int kc = min(kChunk, T - k0);
+#pragma unroll
+        for (int m = 0; m < kSlots; ++m) {
+            const int q = tid + 256 * m, k = q / wI, c = q - k * wI;
+            pI[m] = (k < kc && c < nI) ? J.x[(size_t)(k0 + k) * J.ldx + I0 + c] : 0.0f;
+            if (!ONE && !diag) {
+                const int kj = q / wJ, cj = q - kj * wJ;
+                pJ[m] = (kj < kc && cj < nJ) ? J.x[(size_t)(k0 + kj) * J.ldx + J0 + cj] : 0.0f;
+            }
+        }
+    };
+    if (kbeg < kend) fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += kChunk) {
+        const int kc = min(kChunk, T - k0);
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < kSlots; ++m) {
+            const int q = tid + 256 * m, k = q / wI, c = q - k * wI;
+            if (k < kChunk) cI[k * kCs + c] = (k < kc && c < nI) ? (double)pI[m] - mI[c] : 0.0;
+            if (!ONE && !diag) {
+                const int kj = q / wJ, cj = q - kj * wJ;
+                if (kj < kChunk) cJ[kj * kCs + cj] = (kj < kc && cj < nJ) ? (double)pJ[m] - mJ[cj] : 0.0;
+            }
+        }
+        __syncthreads();
+        if (k0 + kChunk < kend) fetch(k0 + kChunk);
+        if (own_i || own_j) {
+            const double* col = (own_i ? cI : cJ) + vc;
+            for (int k = 0; k < kc; ++k) var += col[k * kCs] * col[k * kCs];
+        }
+#pragma unroll
+        for (int kk = 0; kk < kChunk; kk += 4) {
+            const double* rowI = cI + (kk + lq) * kCs + lr;
+            const double* rowJ = bJ + (kk + lq) * kCs + lr;
+#pragma unroll
+            for (int q = 0; q < kTpw; ++q)
+                if (wave + 4 * q < ntt) acc[q] = mfma_f64(rowI[tti[q] * 16], rowJ[ttj[q] * 16], acc[q]);
+        }
+    }
+    if (nsplit > 1) {
+        double* out = partial + ((size_t)rest * nsplit + z) * kPart;
+        if (own_i || own_j) out[kBlk * kBlk + (own_i ? 0 : kBlk) + vc] = var;
+#pragma unroll
+        for (int q = 0; q < kTpw; ++q) {
+            if (wave + 4 * q >= ntt) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int il = tti[q] * 16 + lq + 4 * r, jl = ttj[q] * 16 + lr;
+                if (il < nI && jl < nJ && !(diag && jl > il)) out[il * kBlk + jl] = acc[q][r];
+            }
+        }
+        return;
+    }
+    if (own_i || own_j) (own_i ? sI : sJ)[vc] = col_scale(J, var);
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < kTilesPerWave; ++q) {
-        if (wave + q * nwave >= ntt) continue;
+    for (int q = 0; q < kTpw; ++q) {
+        if (wave + 4 * q >= ntt) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int i = tti[q] * 16 + lq + 4 * r, j = ttj[q] * 16 + lr;
-            if (j <= i && i < N && j < N) put(J, i, j, pearson(acc[q][r] / (double)T, sd[i], sd[j]));
+            const int il = tti[q] * 16 + lq + 4 * r, jl = ttj[q] * 16 + lr;
+            if (il >= nI || jl >= nJ || (diag && jl > il)) continue;
+            put(J, I0 + il, J0 + jl, (float)corr_value(J, I0 + il, J0 + jl, acc[q][r], sI[il], sB[jl]));
         }
     }
 }
 
-// N > 96: FP64 means and deviations per column, then the lower-triangle
-// pairs (i, j <= i) spread over the threads, each an FP64 sum in row order.
-__global__ __launch_bounds__(256) void k_fc_pairs(const FcJob* __restrict__ jobs, double* __restrict__ work) {
-    const FcJob J = jobs[blockIdx.x];
-    const int N = J.N, T = J.T, tid = threadIdx.x;
-    double* mean = work + (size_t)blockIdx.x * 2 * N;
-    double* sd = mean + N;
-    for (int c = tid; c < N; c += blockDim.x) {
-        double m = 0.0;
-        for (int t = 0; t < T; ++t) m += (double)J.x[(size_t)t * J.ldx + c];
-        m /= (double)T;
-        double s = 0.0;
-        for (int t = 0; t < T; ++t) { const double d = (double)J.x[(size_t)t * J.ldx + c] - m; s += d * d; }
-        s = sqrt(s / (double)T);
-        mean[c] = m;
-        sd[c] = s < 1e-12 ? 1.0 : s;
+// Adds the nsplit partials of one block pair in segment order and writes
+// the correlations (workgroup = job x block pair x 256-entry slice).
+__global__ __launch_bounds__(256) void k_fc_finish(const FcJob* __restrict__ jobs, int nbp, int nsplit,
+                                                   const double* __restrict__ partial) {
+    __shared__ double sI[kBlk], sJ[kBlk];
+    const int rest = blockIdx.x, job = rest / nbp, bp = rest - job * nbp;
+    const FcJob J = jobs[job];
+    const int N = J.N, tid = threadIdx.x;
+    int ib, jb;
+    block_pair(bp, ib, jb);
+    const bool diag = ib == jb;
+    const int I0 = ib * kBlk, J0 = jb * kBlk;
+    const int nI = min(kBlk, N - I0), nJ = min(kBlk, N - J0);
+    const double* base = partial + (size_t)rest * nsplit * kPart;
+    if (tid < nI || (!diag && tid >= 128 && tid - 128 < nJ)) {
+        const int off = kBlk * kBlk + (tid < 128 ? tid : kBlk + tid - 128);
+        double v = 0.0;
+        for (int z = 0; z < nsplit; ++z) v += base[(size_t)z * kPart + off];
+        (tid < 128 ? sI : sJ)[tid & 127] = col_scale(J, v);
     }
     __syncthreads();
-    const int64_t npairs = (int64_t)N * (N + 1) / 2;
-    for (int64_t p = tid; p < npairs; p += blockDim.x) {
-        int i = (int)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
-        while ((int64_t)i * (i + 1) / 2 > p) --i;
-        while ((int64_t)(i + 1) * (i + 2) / 2 <= p) ++i;
-        const int j = (int)(p - (int64_t)i * (i + 1) / 2);
+    const double* sB = diag ? sI : sJ;
+    {
+        const int e = blockIdx.y * 256 + tid, il = e / kBlk, jl = e - il * kBlk;
+        if (il >= nI || jl >= nJ || (diag && jl > il)) return;
         double cov = 0.0;
-        for (int t = 0; t < T; ++t)
-            cov += ((double)J.x[(size_t)t * J.ldx + i] - mean[i]) * ((double)J.x[(size_t)t * J.ldx + j] - mean[j]);
-        put(J, i, j, pearson(cov / (double)T, sd[i], sd[j]));
+        for (int z = 0; z < nsplit; ++z) cov += base[(size_t)z * kPart + e];
+        put(J, I0 + il, J0 + jl, (float)corr_value(J, I0 + il, J0 + jl, cov, sI[il], sB[jl]));
     }
-}
-
-// FCD of one series: W window vectors v[w][P] (f32 upper triangles) ->
-// out[W][W]; upper_triangle_correlation (fc.rs:143-178): FP64 means, centred
-// cross and self sums, 0 when either variance is 0, clamp, diagonal 1.
-__global__ __launch_bounds__(256) void k_fcd_corr(const float* __restrict__ vecs, float* __restrict__ out,
-                                                  double* __restrict__ work, int W, int P) {
-    const float* v = vecs + (size_t)blockIdx.x * W * P;
-    float* o = out + (size_t)blockIdx.x * W * W;
-    double* mean = work + (size_t)blockIdx.x * 2 * W;
-    double* ss = mean + W;
-    const int tid = threadIdx.x;
-    for (int w = tid; w < W; w += blockDim.x) {
-        double s = 0.0;
-        for (int p = 0; p < P; ++p) s += (double)v[(size_t)w * P + p];
-        const double m = s / (double)P;
-        double q = 0.0;
-        for (int p = 0; p < P; ++p) { const double d = (double)v[(size_t)w * P + p] - m; q += d * d; }
-        mean[w] = m;
-        ss[w] = q;
-    }
-    __syncthreads();
-    const int64_t npairs = (int64_t)W * (W - 1) / 2;
-    for (int64_t k = tid; k < npairs; k += blockDim.x) {
-        // k -> (a, b), a < b: k = b (b - 1) / 2 + a
-        int b = (int)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
-        while ((int64_t)b * (b - 1) / 2 > k) --b;
-        while ((int64_t)(b + 1) * b / 2 <= k) ++b;
-        const int a = (int)(k - (int64_t)b * (b - 1) / 2);
-        double r = 0.0;
-        if (ss[a] > 0.0 && ss[b] > 0.0) {
-            double cov = 0.0;
-            for (int p = 0; p < P; ++p)
-                cov += ((double)v[(size_t)a * P + p] - mean[a]) * ((double)v[(size_t)b * P + p] - mean[b]);
-            r = cov / (sqrt(ss[a]) * sqrt(ss[b]));
-            r = r < -1.0 ? -1.0 : (r > 1.0 ? 1.0 : r);
-        }
-        o[(size_t)a * W + b] = (float)r;
-        o[(size_t)b * W + a] = (float)r;
-    }
-    for (int w = tid; w < W; w += blockDim.x) o[(size_t)w * W + w] = 1.0f;
 }
 
 // Owns the device buffers of one call; frees them on every exit path.
@@ -227,20 +288,44 @@ struct DevBufs {
     }
 };
 
-hipError_t run_fc_jobs(const std::vector<FcJob>& jobs, int N, int T, DevBufs& B) {
+// Every job of one call has the same N. Fills each job's mean pointer from a
+// work buffer, then launches the means and the block-pair tile kernels.
+hipError_t run_fc_jobs(std::vector<FcJob>& jobs, int N, int T, DevBufs& B) {
+    const size_t nj = jobs.size();
+    // column sums in up to 8 segments of >= 256 samples, added in order
+    const int nseg = std::max(1, std::min(8, T / 256));
+    double* mean = nullptr;
     FcJob* dj = nullptr;
-    hipError_t e = B.alloc(&dj, jobs.size());
-    if (e == hipSuccess) e = hipMemcpy(dj, jobs.data(), sizeof(FcJob) * jobs.size(), hipMemcpyHostToDevice);
+    hipError_t e = B.alloc(&mean, nj * (size_t)N * nseg);
+    if (e == hipSuccess) e = B.alloc(&dj, nj);
     if (e != hipSuccess) return e;
-    if (N <= kTile) {
-        hipLaunchKernelGGL(k_fc_tiles, dim3((unsigned)jobs.size()), dim3(256), 0, nullptr, dj);
-    } else {
-        double* work = nullptr;
-        e = B.alloc(&work, jobs.size() * 2 * (size_t)N);
+    for (size_t q = 0; q < nj; ++q) jobs[q].mean = mean + q * (size_t)N * nseg;
+    e = hipMemcpy(dj, jobs.data(), sizeof(FcJob) * nj, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    const int nb = (N + kBlk - 1) / kBlk, nbp = nb * (nb + 1) / 2;
+    if ((uint64_t)nj * (uint64_t)nbp * 16u > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fc_means, dim3((unsigned)nj, (unsigned)((N + 63) / 64), (unsigned)nseg), dim3(256), 0,
+                       nullptr, dj, nseg);
+    // split the samples so that a launch has about two workgroups per CU,
+    // keeping at least four 32-sample chunks per segment
+    const size_t blocks = nj * (size_t)nbp;
+    const int nch = (T + kChunk - 1) / kChunk;
+    int nsplit = (int)std::min<size_t>(16, (512 + blocks - 1) / blocks);
+    nsplit = std::max(1, std::min(nsplit, nch / 4));
+    double* partial = nullptr;
+    if (nsplit > 1) {
+        e = B.alloc(&partial, blocks * nsplit * kPart);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_fc_pairs, dim3((unsigned)jobs.size()), dim3(256), 0, nullptr, dj, work);
     }
-    (void)T;
+    if (nb == 1)
+        hipLaunchKernelGGL(k_fc_tiles<true>, dim3((unsigned)(blocks * nsplit)), dim3(256), 0, nullptr, dj, nbp, nseg,
+                           nsplit, partial);
+    else
+        hipLaunchKernelGGL(k_fc_tiles<false>, dim3((unsigned)(blocks * nsplit)), dim3(256), 0, nullptr, dj, nbp, nseg,
+                           nsplit, partial);
+    if (nsplit > 1)
+        hipLaunchKernelGGL(k_fc_finish, dim3((unsigned)blocks, (unsigned)((std::min(N, kBlk) * kBlk + 255) / 256)),
+                           dim3(256), 0, nullptr, dj, nbp, nsplit, partial);
     return hipGetLastError();
 }
 
@@ -280,7 +365,7 @@ int hb_fc_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t 
     if (e != hipSuccess) return hip_fail(e, "FC upload");
     std::vector<feat::FcJob> jobs((size_t)n_series);
     for (int32_t b = 0; b < n_series; ++b)
-        jobs[b] = feat::FcJob{dx + per * b, dout + nn * b, n_steps, n_nodes, n_nodes, 0};
+        jobs[b] = feat::FcJob{dx + per * b, dout + nn * b, nullptr, n_steps, n_nodes, n_nodes, 0, 1, 0};
     e = feat::run_fc_jobs(jobs, n_nodes, n_steps, B);
     if (e == hipSuccess) e = hipMemcpy(fc, dout, sizeof(float) * nn * n_series, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "FC kernels");
@@ -305,13 +390,13 @@ int hb_fcd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const size_t per = (size_t)n_steps * n_nodes;
     const size_t P = (size_t)n_nodes * (n_nodes - 1) / 2;
+    // window FCs land transposed, v[p][w] (P upper entries x W windows per
+    // series), so the FCD is the column correlation of a [P][W] series
     feat::DevBufs B;
     float *dx = nullptr, *dv = nullptr, *dout = nullptr;
-    double* work = nullptr;
     e = B.alloc(&dx, per * n_series);
     if (e == hipSuccess) e = B.alloc(&dv, P * W * n_series);
     if (e == hipSuccess) e = B.alloc(&dout, (size_t)W * W * n_series);
-    if (e == hipSuccess) e = B.alloc(&work, (size_t)2 * W * n_series);
     if (e == hipSuccess) e = hipMemcpy(dx, series, sizeof(float) * per * n_series, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "FCD upload");
     if (P > 0) {
@@ -319,12 +404,15 @@ int hb_fcd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t
         for (int32_t b = 0; b < n_series; ++b)
             for (int32_t w = 0; w < W; ++w)
                 jobs[(size_t)b * W + w] = feat::FcJob{dx + per * b + (size_t)w * stride * n_nodes,
-                                                      dv + P * ((size_t)b * W + w), window_len, n_nodes, n_nodes, 1};
+                                                      dv + P * W * b + w, nullptr, window_len, n_nodes, n_nodes,
+                                                      1, W, 0};
         e = feat::run_fc_jobs(jobs, n_nodes, window_len, B);
         if (e != hipSuccess) return hip_fail(e, "FCD window FC");
     }
-    hipLaunchKernelGGL(feat::k_fcd_corr, dim3((unsigned)n_series), dim3(256), 0, nullptr, dv, dout, work, W, (int)P);
-    e = hipGetLastError();
+    std::vector<feat::FcJob> corr((size_t)n_series);
+    for (int32_t b = 0; b < n_series; ++b)
+        corr[b] = feat::FcJob{dv + P * W * b, dout + (size_t)W * W * b, nullptr, (int32_t)P, W, W, 0, 1, 1};
+    e = feat::run_fc_jobs(corr, W, (int)P, B);
     if (e == hipSuccess) e = hipMemcpy(fcd, dout, sizeof(float) * (size_t)W * W * n_series, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "FCD kernels");
     return HB_OK;

@@ -4,7 +4,9 @@ against the CPU oracle's restatement.
 
 Both compute in FP64 and round to f32 once; the GPU sums in another order
 (MFMA tiles over 32-sample chunks), so FC entries agree to atol 2e-6 and FCD
-entries, correlations of those f32 FC values, to atol 2e-5.
+entries, correlations of those f32 FC values, to atol 2e-5. Sizes cover one
+and several 96-column blocks of the correlation matrix (N = 97, 150; FCD with
+161 windows).
 """
 import numpy as np
 import pytest
@@ -54,7 +56,8 @@ def test_fc_special_columns(gpu, orc, N):
 
 
 @pytest.mark.parametrize("N,T,Lw,S", [(1, 50, 10, 5), (2, 100, 20, 10), (9, 230, 40, 15),
-                                      (68, 300, 60, 10), (84, 500, 100, 7), (120, 200, 50, 25)])
+                                      (68, 300, 60, 10), (84, 500, 100, 7), (120, 200, 50, 25),
+                                      (10, 500, 20, 3)])
 def test_fcd_batch_matches_oracle(gpu, orc, N, T, Lw, S):
     rng = np.random.default_rng(N + T)
     X = _series(rng, 2, T, N)
