@@ -125,6 +125,40 @@ def ragged_leg(args):
     return res
 
 
+def features_leg(args):
+    """Batched FC and FCD (fc_matrix / fcd_matrix) of many BOLD-length series:
+    1024 series x 1200 samples x 84 nodes; FCD windows of 60 samples every 6.
+    The C ABI takes host buffers, so the times include the host-device copies."""
+    from hyburn_amd.features import fc_batch, fcd_batch
+    rng = np.random.default_rng(0)
+    B, T, N, Lw, S = args.features_series, 1200, 84, 60, 6
+    X = (np.cumsum(rng.normal(size=(B, T, N)), axis=1) + np.cumsum(rng.normal(size=(B, T, 1)), axis=1)).astype(
+        np.float32)
+    res = {"workload": f"{B} series x {T} samples x {N} nodes, FCD window {Lw} stride {S}"}
+    for name, fn in (("fc", lambda a: fc_batch(a)), ("fcd", lambda a: fcd_batch(a, Lw, S))):
+        fn(X[:8])
+        t0 = time.perf_counter()
+        fn(X)
+        wall = time.perf_counter() - t0
+        res[name] = {"series_per_s": B / wall, "call_ms": wall * 1e3}
+    if not args.no_cpu:
+        from oracle import hb_oracle
+        k = 4
+        t0 = time.perf_counter()
+        for b in range(k):
+            hb_oracle.fc(X[b])
+        fc_s = (time.perf_counter() - t0) / k
+        t0 = time.perf_counter()
+        for b in range(k):
+            hb_oracle.fcd(X[b], Lw, S)
+        fcd_s = (time.perf_counter() - t0) / k
+        res["cpu_baseline"] = {"fc_series_per_s": 1.0 / fc_s, "fcd_series_per_s": 1.0 / fcd_s, "cores": 1,
+                               "kind": "port", "sample": f"{k} of the series, one host thread"}
+        res["fc"]["speedup_vs_cpu"] = res["fc"]["series_per_s"] * fc_s
+        res["fcd"]["speedup_vs_cpu"] = res["fcd"]["series_per_s"] * fcd_s
+    return res
+
+
 def _fc64(B):
     n = B.shape[0]
     A = B - B.mean(0)
@@ -185,6 +219,8 @@ def main() -> int:
     ap.add_argument("--ragged-tscale", type=float, default=1.0)
     ap.add_argument("--ragged-plain-sample", type=int, default=16)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient legs")
+    ap.add_argument("--no-features", dest="features", action="store_false", help="skip the FC / FCD leg")
+    ap.add_argument("--features-series", type=int, default=1024)
     ap.add_argument("--adj4-batch", type=int, default=256)
     ap.add_argument("--adj5-batch", type=int, default=512)
     ap.add_argument("--adj5-steps", type=int, default=10_000)
@@ -285,6 +321,8 @@ def main() -> int:
                                           f"({secs * threads:.1f} core-s)"}
     if args.ragged:
         line["ragged"] = ragged_leg(args)
+    if args.features:
+        line["features"] = features_leg(args)
     if not args.no_adjoint:
         import dataclasses
         from hyburn_amd.adjoint import Loss

@@ -42,7 +42,7 @@ struct FcJob {
 constexpr int kBlk = 96;                  // columns per block of the correlation matrix
 constexpr int kChunk = 32;                // samples staged per LDS pass
 constexpr int kCs = kBlk + 1;             // padded chunk row stride (doubles)
-constexpr int kTilesPerWave = 9;          // 36 tiles of an off-diagonal 96 x 96 block / 4 waves
+constexpr int kTilesPerWave = 5;          // 36 tiles of an off-diagonal 96 x 96 block / 8 waves
 constexpr int kTilesPerWaveOne = 6;       // 21 lower-triangle tiles of a lone diagonal block / 4 waves
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -107,7 +107,7 @@ __device__ __forceinline__ void block_pair(int bp, int& ib, int& jb) {
 constexpr int kPart = kBlk * kBlk + 2 * kBlk;   // doubles of one partial: products, then sums of squares
 
 // One 96 x 96 block (ib >= jb) of one job's correlation matrix per
-// workgroup of four waves. Centred samples of the block's row and column
+// workgroup (four waves for a lone diagonal block, else eight). Centred samples of the block's row and column
 // ranges are staged in LDS 32 at a time; the cross products run on the FP64
 // matrix cores in 16 x 16 tiles (the lower triangle of a diagonal block), and
 // the per-column sums of squares on the vector units in sample order. With
@@ -119,8 +119,9 @@ constexpr int kPart = kBlk * kBlk + 2 * kBlk;   // doubles of one partial: produ
 // diagonal block (N <= 96), which needs neither the second column-range
 // buffers nor more than 6 tiles per wave.
 template <bool ONE>
-__global__ __launch_bounds__(256) void k_fc_tiles(const FcJob* __restrict__ jobs, int nbp, int nseg, int nsplit,
+__global__ __launch_bounds__(ONE ? 256 : 512) void k_fc_tiles(const FcJob* __restrict__ jobs, int nbp, int nseg, int nsplit,
                                                   double* __restrict__ partial) {
+    constexpr int kThreads = ONE ? 256 : 512, kWaves = kThreads / 64;
     constexpr int kTpw = ONE ? kTilesPerWaveOne : kTilesPerWave;
     __shared__ double cI[kChunk * kCs], cJ[ONE ? 1 : kChunk * kCs];
     __shared__ double mI[kBlk], mJ[kBlk], sI[kBlk], sJ[kBlk];
@@ -150,7 +151,7 @@ __global__ __launch_bounds__(256) void k_fc_tiles(const FcJob* __restrict__ jobs
     d4 acc[kTpw];
 #pragma unroll
     for (int q = 0; q < kTpw; ++q) {
-        const int t = wave + 4 * q;
+        const int t = wave + kWaves * q;
         int ti = 0, tj = 0;
         if (t < ntt) {
             if (diag) {
@@ -170,17 +171,28 @@ __global__ __launch_bounds__(256) void k_fc_tiles(const FcJob* __restrict__ jobs
     const bool own_i = tid < nI, own_j = !diag && tid >= 128 && vc < nJ;
     double var = 0.0;
     // the next chunk's samples are loaded into registers while the current
-    // chunk's products run: slot m of this thread is q = tid + 256 m
-    constexpr int kSlots = kChunk * kBlk / 256;
+    // chunk's products run: slot m of this thread is q = tid + kThreads m
+    constexpr int kSlots = kChunk * kBlk / kThreads;
     float pI[kSlots], pJ[ONE ? 1 : kSlots];
+    // (row, column) of each slot, packed as row << 8 | column, computed once
+    int sI_[kSlots], sJ_[ONE ? 1 : kSlots];
+#pragma unroll
+    for (int m = 0; m < kSlots; ++m) {
+        const int q = tid + kThreads * m, k = q / wI;
+        sI_[m] = (k << 8) | (q - k * wI);
+        if (!ONE && !diag) {
+            const int kj = q / wJ;
+            sJ_[m] = (kj << 8) | (q - kj * wJ);
+        }
+    }
     auto fetch = [&](int k0) {
         const int kc = min(kChunk, T - k0);
 #pragma unroll
         for (int m = 0; m < kSlots; ++m) {
-            const int q = tid + 256 * m, k = q / wI, c = q - k * wI;
+            const int k = sI_[m] >> 8, c = sI_[m] & 255;
             pI[m] = (k < kc && c < nI) ? J.x[(size_t)(k0 + k) * J.ldx + I0 + c] : 0.0f;
             if (!ONE && !diag) {
-                const int kj = q / wJ, cj = q - kj * wJ;
+                const int kj = sJ_[m] >> 8, cj = sJ_[m] & 255;
                 pJ[m] = (kj < kc && cj < nJ) ? J.x[(size_t)(k0 + kj) * J.ldx + J0 + cj] : 0.0f;
             }
         }
@@ -191,10 +203,10 @@ __global__ __launch_bounds__(256) void k_fc_tiles(const FcJob* __restrict__ jobs
         __syncthreads();
 #pragma unroll
         for (int m = 0; m < kSlots; ++m) {
-            const int q = tid + 256 * m, k = q / wI, c = q - k * wI;
+            const int k = sI_[m] >> 8, c = sI_[m] & 255;
             if (k < kChunk) cI[k * kCs + c] = (k < kc && c < nI) ? (double)pI[m] - mI[c] : 0.0;
             if (!ONE && !diag) {
-                const int kj = q / wJ, cj = q - kj * wJ;
+                const int kj = sJ_[m] >> 8, cj = sJ_[m] & 255;
                 if (kj < kChunk) cJ[kj * kCs + cj] = (kj < kc && cj < nJ) ? (double)pJ[m] - mJ[cj] : 0.0;
             }
         }
@@ -210,7 +222,7 @@ __global__ __launch_bounds__(256) void k_fc_tiles(const FcJob* __restrict__ jobs
             const double* rowJ = bJ + (kk + lq) * kCs + lr;
 #pragma unroll
             for (int q = 0; q < kTpw; ++q)
-                if (wave + 4 * q < ntt) acc[q] = mfma_f64(rowI[tti[q] * 16], rowJ[ttj[q] * 16], acc[q]);
+                if (wave + kWaves * q < ntt) acc[q] = mfma_f64(rowI[tti[q] * 16], rowJ[ttj[q] * 16], acc[q]);
         }
     }
     if (nsplit > 1) {
@@ -218,7 +230,7 @@ __global__ __launch_bounds__(256) void k_fc_tiles(const FcJob* __restrict__ jobs
         if (own_i || own_j) out[kBlk * kBlk + (own_i ? 0 : kBlk) + vc] = var;
 #pragma unroll
         for (int q = 0; q < kTpw; ++q) {
-            if (wave + 4 * q >= ntt) continue;
+            if (wave + kWaves * q >= ntt) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int il = tti[q] * 16 + lq + 4 * r, jl = ttj[q] * 16 + lr;
@@ -231,7 +243,7 @@ __global__ __launch_bounds__(256) void k_fc_tiles(const FcJob* __restrict__ jobs
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kTpw; ++q) {
-        if (wave + 4 * q >= ntt) continue;
+        if (wave + kWaves * q >= ntt) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int il = tti[q] * 16 + lq + 4 * r, jl = ttj[q] * 16 + lr;
@@ -270,6 +282,21 @@ __global__ __launch_bounds__(256) void k_fc_finish(const FcJob* __restrict__ job
         for (int z = 0; z < nsplit; ++z) cov += base[(size_t)z * kPart + e];
         put(J, I0 + il, J0 + jl, (float)corr_value(J, I0 + il, J0 + jl, cov, sI[il], sB[jl]));
     }
+}
+
+// Per series, v[W][P] -> vt[P][W] through 64 x 64 LDS tiles (grid: P tiles,
+// W tiles, series); reads and writes are row-contiguous.
+__global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ v, float* __restrict__ vt, int W,
+                                                  int P) {
+    __shared__ float tile[64][65];
+    const int p0 = blockIdx.x * 64, w0 = blockIdx.y * 64, tid = threadIdx.x;
+    const size_t base = (size_t)blockIdx.z * W * P;
+    const int cx = tid & 63, ry = tid >> 6;
+    for (int r = ry; r < 64; r += 4)
+        if (w0 + r < W && p0 + cx < P) tile[r][cx] = v[base + (size_t)(w0 + r) * P + p0 + cx];
+    __syncthreads();
+    for (int r = ry; r < 64; r += 4)
+        if (p0 + r < P && w0 + cx < W) vt[base + (size_t)(p0 + r) * W + w0 + cx] = tile[cx][r];
 }
 
 // Owns the device buffers of one call; frees them on every exit path.
@@ -321,7 +348,7 @@ hipError_t run_fc_jobs(std::vector<FcJob>& jobs, int N, int T, DevBufs& B) {
         hipLaunchKernelGGL(k_fc_tiles<true>, dim3((unsigned)(blocks * nsplit)), dim3(256), 0, nullptr, dj, nbp, nseg,
                            nsplit, partial);
     else
-        hipLaunchKernelGGL(k_fc_tiles<false>, dim3((unsigned)(blocks * nsplit)), dim3(256), 0, nullptr, dj, nbp, nseg,
+        hipLaunchKernelGGL(k_fc_tiles<false>, dim3((unsigned)(blocks * nsplit)), dim3(512), 0, nullptr, dj, nbp, nseg,
                            nsplit, partial);
     if (nsplit > 1)
         hipLaunchKernelGGL(k_fc_finish, dim3((unsigned)blocks, (unsigned)((std::min(N, kBlk) * kBlk + 255) / 256)),
@@ -386,16 +413,20 @@ int hb_fcd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t
     if (stride < 1) return set_error(HB_INVALID_CONFIG, "stride must be > 0");
     const int32_t W = hb_fcd_windows(n_steps, window_len, stride);
     if (W <= 1) return HB_OK;   // fcd_matrix returns an empty matrix
-    hipError_t e = hipSetDevice(device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const size_t per = (size_t)n_steps * n_nodes;
     const size_t P = (size_t)n_nodes * (n_nodes - 1) / 2;
-    // window FCs land transposed, v[p][w] (P upper entries x W windows per
-    // series), so the FCD is the column correlation of a [P][W] series
+    if (P > 0x7fffffffu / 64 || n_series > 65535)
+        return set_error(HB_INVALID_CONFIG, "FCD needs n_nodes (n_nodes - 1) / 2 < 2^25 and n_series <= 65535");
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // window FCs are written as v[w][p] (W windows x P upper entries per
+    // series) and transposed to vt[p][w]: the FCD is the column correlation
+    // of that [P][W] series
     feat::DevBufs B;
-    float *dx = nullptr, *dv = nullptr, *dout = nullptr;
+    float *dx = nullptr, *dv = nullptr, *dvt = nullptr, *dout = nullptr;
     e = B.alloc(&dx, per * n_series);
     if (e == hipSuccess) e = B.alloc(&dv, P * W * n_series);
+    if (e == hipSuccess) e = B.alloc(&dvt, P * W * n_series);
     if (e == hipSuccess) e = B.alloc(&dout, (size_t)W * W * n_series);
     if (e == hipSuccess) e = hipMemcpy(dx, series, sizeof(float) * per * n_series, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "FCD upload");
@@ -404,14 +435,18 @@ int hb_fcd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t
         for (int32_t b = 0; b < n_series; ++b)
             for (int32_t w = 0; w < W; ++w)
                 jobs[(size_t)b * W + w] = feat::FcJob{dx + per * b + (size_t)w * stride * n_nodes,
-                                                      dv + P * W * b + w, nullptr, window_len, n_nodes, n_nodes,
-                                                      1, W, 0};
+                                                      dv + P * ((size_t)b * W + w), nullptr, window_len, n_nodes,
+                                                      n_nodes, 1, 1, 0};
         e = feat::run_fc_jobs(jobs, n_nodes, window_len, B);
         if (e != hipSuccess) return hip_fail(e, "FCD window FC");
+        hipLaunchKernelGGL(feat::k_transpose, dim3((unsigned)((P + 63) / 64), (unsigned)((W + 63) / 64),
+                                                   (unsigned)n_series), dim3(256), 0, nullptr, dv, dvt, W, (int)P);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "FCD transpose");
     }
     std::vector<feat::FcJob> corr((size_t)n_series);
     for (int32_t b = 0; b < n_series; ++b)
-        corr[b] = feat::FcJob{dv + P * W * b, dout + (size_t)W * W * b, nullptr, (int32_t)P, W, W, 0, 1, 1};
+        corr[b] = feat::FcJob{dvt + P * W * b, dout + (size_t)W * W * b, nullptr, (int32_t)P, W, W, 0, 1, 1};
     e = feat::run_fc_jobs(corr, W, (int)P, B);
     if (e == hipSuccess) e = hipMemcpy(fcd, dout, sizeof(float) * (size_t)W * W * n_series, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "FCD kernels");
