@@ -126,16 +126,19 @@ def ragged_leg(args):
 
 
 def features_leg(args):
-    """Batched FC and FCD (fc_matrix / fcd_matrix) of many BOLD-length series:
-    1024 series x 1200 samples x 84 nodes; FCD windows of 60 samples every 6.
+    """Batched FC, FCD (fc_matrix / fcd_matrix) and spectral features
+    (spectral_features of every node series) of many BOLD-length series:
+    1024 series x 1200 samples x 84 nodes; FCD windows of 60 samples every 6;
+    Welch PSD with 2048-point transforms of 256-sample windows at fs = 1 kHz.
     The C ABI takes host buffers, so the times include the host-device copies."""
-    from hyburn_amd.features import fc_batch, fcd_batch
+    from hyburn_amd.features import fc_batch, fcd_batch, spectral_batch
     rng = np.random.default_rng(0)
     B, T, N, Lw, S = args.features_series, 1200, 84, 60, 6
     X = (np.cumsum(rng.normal(size=(B, T, N)), axis=1) + np.cumsum(rng.normal(size=(B, T, 1)), axis=1)).astype(
         np.float32)
     res = {"workload": f"{B} series x {T} samples x {N} nodes, FCD window {Lw} stride {S}"}
-    for name, fn in (("fc", lambda a: fc_batch(a)), ("fcd", lambda a: fcd_batch(a, Lw, S))):
+    for name, fn in (("fc", lambda a: fc_batch(a)), ("fcd", lambda a: fcd_batch(a, Lw, S)),
+                     ("spectral", lambda a: spectral_batch(a, 1000.0))):
         fn(X[:8])
         t0 = time.perf_counter()
         fn(X)
@@ -152,10 +155,17 @@ def features_leg(args):
         for b in range(k):
             hb_oracle.fcd(X[b], Lw, S)
         fcd_s = (time.perf_counter() - t0) / k
-        res["cpu_baseline"] = {"fc_series_per_s": 1.0 / fc_s, "fcd_series_per_s": 1.0 / fcd_s, "cores": 1,
-                               "kind": "port", "sample": f"{k} of the series, one host thread"}
+        t0 = time.perf_counter()
+        for n in range(N):
+            hb_oracle.spectral_features(X[0, :, n], 1000.0)
+        sp_s = time.perf_counter() - t0
+        res["cpu_baseline"] = {"fc_series_per_s": 1.0 / fc_s, "fcd_series_per_s": 1.0 / fcd_s,
+                               "spectral_series_per_s": 1.0 / sp_s, "cores": 1, "kind": "port",
+                               "sample": f"{k} of the series (FC, FCD) and the {N} nodes of one series "
+                                         f"(spectral, numpy FFT), one host thread"}
         res["fc"]["speedup_vs_cpu"] = res["fc"]["series_per_s"] * fc_s
         res["fcd"]["speedup_vs_cpu"] = res["fcd"]["series_per_s"] * fcd_s
+        res["spectral"]["speedup_vs_cpu"] = res["spectral"]["series_per_s"] * sp_s
     return res
 
 
@@ -219,7 +229,8 @@ def main() -> int:
     ap.add_argument("--ragged-tscale", type=float, default=1.0)
     ap.add_argument("--ragged-plain-sample", type=int, default=16)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient legs")
-    ap.add_argument("--no-features", dest="features", action="store_false", help="skip the FC / FCD leg")
+    ap.add_argument("--no-features", dest="features", action="store_false",
+                    help="skip the FC / FCD / spectral leg")
     ap.add_argument("--features-series", type=int, default=1024)
     ap.add_argument("--adj4-batch", type=int, default=256)
     ap.add_argument("--adj5-batch", type=int, default=512)

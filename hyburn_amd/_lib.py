@@ -157,6 +157,10 @@ PROTOTYPES = {
     "hb_fc_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
     "hb_fcd_windows": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
     "hb_fcd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
+    "hb_welch_bins": (C.c_int32, [C.c_int32, C.c_int32]),
+    "hb_welch_psd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32,
+                                     C.POINTER(C.c_double), C.c_int32]),
+    "hb_spectral_features_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_double, fp, C.c_int32]),
 }
 
 _lib = None

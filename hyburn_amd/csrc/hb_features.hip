@@ -299,6 +299,130 @@ __global__ __launch_bounds__(256) void k_transpose(const float* __restrict__ v, 
         if (p0 + r < P && w0 + cx < W) vt[base + (size_t)(p0 + r) * W + w0 + cx] = tile[cx][r];
 }
 
+// ---------------------------------------------------------------- Welch PSD
+// welch_psd (catch22/fft.rs:120-170) of every node series of every batch
+// entry: segments of w samples every w / 2, minus the series mean, zero-padded
+// to nf points; |DFT|^2 of bins 0..nf/2 summed over segments, / (nseg w fs).
+// The DFT of a zero-padded segment is a product with a w x K twiddle table,
+// so it runs on the FP64 matrix cores: one wave per 16 nodes x 16 bins,
+// looping over segments and accumulating Re^2 + Im^2 in registers.
+
+// tw[i][k] = (cos, -sin)(2 pi ((i k) mod nf) / nf), i < w, k < K.
+__global__ __launch_bounds__(256) void k_twiddles(double2* __restrict__ tw, int w, int K, int nf) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)w * K) return;
+    const int i = (int)(e / K), k = (int)(e - (int64_t)i * K);
+    const int64_t r = ((int64_t)i * k) % nf;
+    double sn, cs;
+    sincospi(2.0 * (double)r / (double)nf, &sn, &cs);
+    tw[e] = double2{cs, -sn};
+}
+
+struct WelchArgs {
+    const float* y;          // [B][T][N]
+    const double* sums;      // [B][nseg_mean][N] column sums (k_fc_means)
+    const double2* tw;       // [w][K]
+    double* psd;             // [B][N][K]
+    int32_t T, N, w, K, nseg, nseg_mean, hop, pad;
+    double inv_norm;         // 1 / (nseg_total * w * fs)
+};
+
+// grid: (ceil(K / 64), ceil(N / 16), B); wave v of the workgroup owns bins
+// [64 bx + 16 v, +16) of nodes [16 by, +16).
+__global__ __launch_bounds__(256) void k_welch(WelchArgs a) {
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, lr = lane & 15, lq = lane >> 4;
+    const int b = blockIdx.z, n0 = blockIdx.y * 16, k0 = blockIdx.x * 64 + wave * 16;
+    const int n = n0 + lr;
+    double mean = 0.0;
+    if (n < a.N) {
+        for (int z = 0; z < a.nseg_mean; ++z) mean += a.sums[((size_t)b * a.nseg_mean + z) * a.N + n];
+        mean /= (double)a.T;
+    }
+    const float* yb = a.y + (size_t)b * a.T * a.N;
+    const bool kv = k0 + lr < a.K;
+    d4 p = d4{0.0, 0.0, 0.0, 0.0};
+    for (int seg = 0; seg < a.nseg; ++seg) {
+        const int t0 = seg * a.hop;
+        d4 re = d4{0.0, 0.0, 0.0, 0.0}, im = d4{0.0, 0.0, 0.0, 0.0};
+        for (int i0 = 0; i0 < a.w; i0 += 4) {
+            const int i = i0 + lq;
+            const bool iv = i < a.w;
+            const double av = (iv && n < a.N) ? (double)yb[(size_t)(t0 + i) * a.N + n] - mean : 0.0;
+            const double2 t = (iv && kv) ? a.tw[(size_t)i * a.K + k0 + lr] : double2{0.0, 0.0};
+            re = mfma_f64(av, t.x, re);
+            im = mfma_f64(av, t.y, im);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) p[r] += re[r] * re[r] + im[r] * im[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int nn = n0 + lq + 4 * r;
+        if (nn < a.N && kv) a.psd[((size_t)b * a.N + nn) * a.K + k0 + lr] = p[r] * a.inv_norm;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// spectral_features (spectral.rs:20-99) from each PSD row: one wave per
+// (batch entry, node); feats[row][9] = 5 band-power fractions, centroid,
+// spread, skewness, excess kurtosis; NaN when the total power is not > 0 or
+// not finite.
+__global__ __launch_bounds__(256) void k_spectral(const double* __restrict__ psd, float* __restrict__ feats,
+                                                  int64_t rows, int K, double df) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= rows) return;
+    const double* P = psd + row * K;
+    constexpr double lo[5] = {0.5, 4.0, 8.0, 13.0, 30.0}, hi[5] = {4.0, 8.0, 13.0, 30.0, 80.0};
+    double tot = 0.0, pf = 0.0, band[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = lane; k < K; k += 64) {
+        const double p = P[k], f = (double)k * df;
+        tot += p;
+        pf += p * f;
+#pragma unroll
+        for (int q = 0; q < 5; ++q)
+            if (f >= lo[q] && f < hi[q]) band[q] += p;
+    }
+    tot = wave_sum(tot);
+    pf = wave_sum(pf);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) band[q] = wave_sum(band[q]);
+    float* o = feats + row * 9;
+    if (!(tot > 0.0) || !isfinite(tot)) {
+        if (lane < 9) o[lane] = __builtin_nanf("");
+        return;
+    }
+    const double c = pf / tot;
+    double m2 = 0.0, m3 = 0.0, m4 = 0.0;
+    for (int k = lane; k < K; k += 64) {
+        const double p = P[k], d = (double)k * df - c, d2 = d * d;
+        m2 += p * d2;
+        m3 += p * d2 * d;
+        m4 += p * d2 * d2;
+    }
+    m2 = wave_sum(m2);
+    m3 = wave_sum(m3);
+    m4 = wave_sum(m4);
+    const double spread = sqrt(m2 / tot);
+    const double skew = spread > 1e-12 ? m3 / tot / (spread * spread * spread) : 0.0;
+    const double kurt = spread > 1e-12 ? m4 / tot / (spread * spread * spread * spread) - 3.0 : 0.0;
+    if (lane < 5) {
+        double v = band[0];
+#pragma unroll
+        for (int q = 1; q < 5; ++q)
+            if (lane == q) v = band[q];
+        o[lane] = (float)(v / tot);
+    } else if (lane < 9) {
+        const double v = lane == 5 ? c : (lane == 6 ? spread : (lane == 7 ? skew : kurt));
+        o[lane] = (float)v;
+    }
+}
+
 // Owns the device buffers of one call; frees them on every exit path.
 struct DevBufs {
     std::vector<void*> p;
@@ -353,6 +477,43 @@ hipError_t run_fc_jobs(std::vector<FcJob>& jobs, int N, int T, DevBufs& B) {
     if (nsplit > 1)
         hipLaunchKernelGGL(k_fc_finish, dim3((unsigned)blocks, (unsigned)((std::min(N, kBlk) * kBlk + 255) / 256)),
                            dim3(256), 0, nullptr, dj, nbp, nsplit, partial);
+    return hipGetLastError();
+}
+
+static int next_pow2(int n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+// Device Welch PSD of every node series; psd is allocated in B and filled.
+// nf = next_pow2(max(nfft, w)); the caller has validated the sizes.
+hipError_t run_welch(const float* dy, int Bn, int T, int N, int nf, int w, double fs, DevBufs& B, double** psd_out) {
+    const int K = nf / 2 + 1, hop = w / 2;
+    const int nseg_total = T > w ? (T - w) / hop + 1 : 1;
+    const int nseg = T >= w ? nseg_total : 0;   // segments that fit (welch_psd breaks on the rest)
+    const int nsm = std::max(1, std::min(8, T / 256));
+    double *sums = nullptr, *psd = nullptr;
+    double2* tw = nullptr;
+    FcJob* dj = nullptr;
+    hipError_t e = B.alloc(&sums, (size_t)Bn * nsm * N);
+    if (e == hipSuccess) e = B.alloc(&psd, (size_t)Bn * N * K);
+    if (e == hipSuccess) e = B.alloc(&tw, (size_t)w * K);
+    if (e == hipSuccess) e = B.alloc(&dj, (size_t)Bn);
+    if (e != hipSuccess) return e;
+    std::vector<FcJob> jobs((size_t)Bn);
+    for (int b = 0; b < Bn; ++b)
+        jobs[b] = FcJob{dy + (size_t)b * T * N, nullptr, sums + (size_t)b * nsm * N, T, N, N, 0, 1, 0};
+    e = hipMemcpy(dj, jobs.data(), sizeof(FcJob) * jobs.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fc_means, dim3((unsigned)Bn, (unsigned)((N + 63) / 64), (unsigned)nsm), dim3(256), 0, nullptr,
+                       dj, nsm);
+    const int64_t nt = (int64_t)w * K;
+    hipLaunchKernelGGL(k_twiddles, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, nullptr, tw, w, K, nf);
+    WelchArgs a{dy, sums, tw, psd, T, N, w, K, nseg, nsm, hop, 0, 1.0 / ((double)nseg_total * (double)w * fs)};
+    hipLaunchKernelGGL(k_welch, dim3((unsigned)((K + 63) / 64), (unsigned)((N + 15) / 16), (unsigned)Bn), dim3(256), 0,
+                       nullptr, a);
+    *psd_out = psd;
     return hipGetLastError();
 }
 
@@ -450,6 +611,70 @@ int hb_fcd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t
     e = feat::run_fc_jobs(corr, W, (int)P, B);
     if (e == hipSuccess) e = hipMemcpy(fcd, dout, sizeof(float) * (size_t)W * W * n_series, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "FCD kernels");
+    return HB_OK;
+}
+
+int32_t hb_welch_bins(int32_t nfft, int32_t window_width) {
+    if (window_width < 2 || nfft < 0) return 0;
+    const int32_t m = std::max(nfft, window_width);
+    if (m > 65536) return 0;
+    return feat::next_pow2(m) / 2 + 1;
+}
+
+int hb_welch_psd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t n_nodes, int32_t nfft,
+                       double fs, int32_t window_width, double* psd, int32_t device) {
+    const int rc = check_series(series, n_series, n_steps, n_nodes, psd);
+    if (rc) return rc;
+    if (window_width < 2) return set_error(HB_INVALID_CONFIG, "window_width must be >= 2 (its half is the hop)");
+    const int32_t K = hb_welch_bins(nfft, window_width);
+    if (K == 0 || nfft < 0) return set_error(HB_INVALID_CONFIG, "nfft and window_width must be in [0, 65536]");
+    if ((int64_t)window_width * K > (1 << 26) || n_series > 65535)
+        return set_error(HB_INVALID_CONFIG, "window_width * bins must be <= 2^26 and n_series <= 65535");
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const size_t per = (size_t)n_steps * n_nodes;
+    feat::DevBufs B;
+    float* dy = nullptr;
+    double* dpsd = nullptr;
+    e = B.alloc(&dy, per * n_series);
+    if (e == hipSuccess) e = hipMemcpy(dy, series, sizeof(float) * per * n_series, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = feat::run_welch(dy, n_series, n_steps, n_nodes, 2 * (K - 1), window_width, fs, B, &dpsd);
+    if (e == hipSuccess)
+        e = hipMemcpy(psd, dpsd, sizeof(double) * (size_t)n_series * n_nodes * K, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "Welch PSD");
+    return HB_OK;
+}
+
+int hb_spectral_features_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t n_nodes, double fs,
+                               float* feats, int32_t device) {
+    const int rc = check_series(series, n_series, n_steps, n_nodes, feats);
+    if (rc) return rc;
+    if (n_steps > 65536 || n_series > 65535)
+        return set_error(HB_INVALID_CONFIG, "n_steps must be <= 65536 and n_series <= 65535");
+    const size_t rows = (size_t)n_series * n_nodes;
+    if (n_steps < 16) {   // spectral_features: too short for a PSD
+        for (size_t q = 0; q < rows * 9; ++q) feats[q] = __builtin_nanf("");
+        return HB_OK;
+    }
+    const int nf = feat::next_pow2(n_steps);
+    const int w = std::min(std::max(n_steps / 2, 16), 256);
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const size_t per = (size_t)n_steps * n_nodes;
+    feat::DevBufs B;
+    float *dy = nullptr, *dfeat = nullptr;
+    double* dpsd = nullptr;
+    e = B.alloc(&dy, per * n_series);
+    if (e == hipSuccess) e = B.alloc(&dfeat, rows * 9);
+    if (e == hipSuccess) e = hipMemcpy(dy, series, sizeof(float) * per * n_series, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = feat::run_welch(dy, n_series, n_steps, n_nodes, nf, w, fs, B, &dpsd);
+    if (e != hipSuccess) return hip_fail(e, "spectral PSD");
+    hipLaunchKernelGGL(feat::k_spectral, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, nullptr, dpsd, dfeat,
+                       (int64_t)rows, nf / 2 + 1, fs / (double)nf);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(feats, dfeat, sizeof(float) * rows * 9, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "spectral features");
     return HB_OK;
 }
 

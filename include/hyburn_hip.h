@@ -291,6 +291,26 @@ int32_t hb_fcd_windows(int32_t n_steps, int32_t window_len, int32_t stride);
 int hb_fcd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t n_nodes, int32_t window_len,
                  int32_t stride, float* fcd, int32_t device);
 
+/* Bins of welch_psd (src/sbi/features/catch22/fft.rs:120-170):
+ * next_power_of_2(max(nfft, window_width)) / 2 + 1; 0 for window_width < 2 or
+ * a transform longer than 65536 points. */
+int32_t hb_welch_bins(int32_t nfft, int32_t window_width);
+/* Batched Welch PSD (welch_psd, catch22/fft.rs:120-170) of every node series
+ * of series [n_series][n_steps][n_nodes]: segments of window_width samples
+ * every window_width / 2, minus the series mean, zero-padded,
+ * |DFT|^2 summed over segments, / (n_segments * window_width * fs).
+ * psd receives [n_series][n_nodes][hb_welch_bins(nfft, window_width)] (f64);
+ * bin k is at frequency k * fs / (2 * (bins - 1)). */
+int hb_welch_psd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t n_nodes, int32_t nfft,
+                       double fs, int32_t window_width, double* psd, int32_t device);
+/* spectral_features (src/sbi/features/spectral.rs:20-99) of every node series:
+ * feats receives [n_series][n_nodes][9] = delta, theta, alpha, beta, gamma
+ * power fractions, spectral centroid, spread, skewness, excess kurtosis of
+ * the Welch PSD with nfft = next_power_of_2(n_steps) and window
+ * clamp(n_steps / 2, 16, 256); NaN x 9 for n_steps < 16 or zero total power. */
+int hb_spectral_features_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t n_nodes, double fs,
+                               float* feats, int32_t device);
+
 /* ---------------- misc ---------------- */
 const char* hb_last_error(void);   /* thread-local message of the last failure */
 const char* hb_version(void);

@@ -304,5 +304,64 @@ def fcd(traj, window_len, stride):
     return out if rc > 1 else np.zeros((0, 0), np.float32)
 
 
+def next_power_of_2(n):
+    """catch22/fft.rs:6-15."""
+    p = 1
+    while p < n:
+        p <<= 1
+    return p
+
+
+EEG_BANDS = ((0.5, 4.0), (4.0, 8.0), (8.0, 13.0), (13.0, 30.0), (30.0, 80.0))   # spectral.rs:9-15
+
+
+def welch_psd(y, nfft, fs, window_width):
+    """welch_psd (catch22/fft.rs:120-170): segments of window_width samples
+    every window_width // 2, minus the whole series' mean, zero-padded to
+    next_power_of_2(max(nfft, window_width)), |DFT|^2 summed over segments for
+    bins 0..nfft/2, divided by n_segments * window_width * fs. The DFT is
+    numpy's FFT (the reference's radix-2 FFT computes the same transform)."""
+    y = np.asarray(y, np.float64)
+    n = len(y)
+    m = y.sum() / n
+    nf = next_power_of_2(max(nfft, window_width))
+    half = nf // 2 + 1
+    hop = window_width // 2
+    nseg = (n - window_width) // hop + 1 if n > window_width else 1
+    psd = np.zeros(half)
+    for seg in range(nseg):
+        start = seg * hop
+        if start + window_width > n:
+            break
+        d = np.zeros(nf)
+        d[:window_width] = y[start:start + window_width] - m
+        X = np.fft.fft(d)[:half]
+        psd += X.real * X.real + X.imag * X.imag
+    psd /= nseg * window_width * fs
+    return psd, np.arange(half) * (fs / nf)
+
+
+def spectral_features(series, fs):
+    """spectral_features (spectral.rs:20-99): 5 band-power fractions, then
+    centroid, spread, skewness, excess kurtosis of the Welch PSD; NaN x 9 for
+    fewer than 16 samples or a non-positive / non-finite total power."""
+    series = np.asarray(series, np.float32)
+    nan = np.full(9, np.nan, np.float32)
+    if len(series) < 16:
+        return nan
+    nfft = next_power_of_2(len(series))
+    nperseg = min(max(len(series) // 2, 16), 256)
+    psd, freqs = welch_psd(series.astype(np.float64), nfft, fs, nperseg)
+    total = psd.sum()
+    if not (total > 0.0) or not np.isfinite(total):
+        return nan
+    out = [psd[(freqs >= lo) & (freqs < hi)].sum() / total for lo, hi in EEG_BANDS]
+    c = (psd * freqs).sum() / total
+    spread = np.sqrt((psd * (freqs - c) ** 2).sum() / total)
+    skew = (psd * (freqs - c) ** 3).sum() / total / spread ** 3 if spread > 1e-12 else 0.0
+    kurt = (psd * (freqs - c) ** 4).sum() / total / spread ** 4 - 3.0 if spread > 1e-12 else 0.0
+    return np.array(out + [c, spread, skew, kurt], np.float32)
+
+
 def source_time(mode, s, d, H):
     return int(lib().hb_orc_source_time(mode, s, d, H))

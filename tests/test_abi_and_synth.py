@@ -214,3 +214,21 @@ def test_fc_fcd_host_validation():
     out[:] = 7.0
     assert lib.hb_fcd_batch(xp, 2, 10, 3, 10, 1, op, 0) == L.HB_OK
     assert np.all(out == 7.0)
+
+
+def test_welch_host_validation():
+    # welch_psd divides by window_width / 2 (catch22/fft.rs:126-131): width < 2 is rejected
+    lib = L.lib()
+    x = np.zeros((1, 40, 2), np.float32)
+    xp = x.ctypes.data_as(L.fp)
+    psd = np.zeros(4096)
+    pp = psd.ctypes.data_as(C.POINTER(C.c_double))
+    assert lib.hb_welch_bins(0, 16) == 9 and lib.hb_welch_bins(100, 16) == 65
+    assert lib.hb_welch_bins(64, 1) == 0 and lib.hb_welch_bins(70000, 16) == 0
+    assert lib.hb_welch_psd_batch(xp, 1, 40, 2, 64, 100.0, 1, pp, 0) == L.HB_INVALID_CONFIG
+    assert lib.hb_welch_psd_batch(xp, 1, 40, 2, 1 << 20, 100.0, 16, pp, 0) == L.HB_INVALID_CONFIG
+    # fewer than 16 samples: NaN features without any device work (spectral.rs:21-24)
+    f = np.zeros((1, 2, 9), np.float32)
+    y = np.ones((1, 10, 2), np.float32)
+    assert lib.hb_spectral_features_batch(y.ctypes.data_as(L.fp), 1, 10, 2, 100.0, f.ctypes.data_as(L.fp), 0) == 0
+    assert np.all(np.isnan(f))

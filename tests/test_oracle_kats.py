@@ -328,6 +328,60 @@ def test_fcd_closed_forms(orc):
             orc.fcd(X, L, S)
 
 
+# ---------------------------------------------------------------- Welch ---
+
+def _sine(n, f, fs):
+    return np.sin(np.arange(n, dtype=np.float64) * 2.0 * np.pi * f / fs).astype(np.float32)
+
+
+def test_spectral_reference_kats(orc):
+    # spectral.rs:157-264
+    F = orc.spectral_features(_sine(1024, 10.0, 100.0), 100.0)
+    assert F.shape == (9,) and F[2] > 0.5 and abs(F[5] - 10.0) < 3.0
+    F = orc.spectral_features(_sine(1024, 6.0, 100.0), 100.0)
+    assert F[1] > 0.5 and F[0] < 0.3
+    assert orc.spectral_features(_sine(2048, 40.0, 200.0), 200.0)[4] > 0.5
+    assert np.all(np.isnan(orc.spectral_features(np.ones(4, np.float32), 100.0)))
+    assert not np.any(np.isinf(orc.spectral_features(_sine(32, 10.0, 100.0), 100.0)))
+
+
+def test_welch_against_direct_dft(orc):
+    # brute-force DFT of every zero-padded segment, on a tiny input
+    rng = np.random.default_rng(4)
+    y = rng.normal(size=23)
+    nfft, fs, w = 20, 7.0, 6
+    psd, freqs = orc.welch_psd(y, nfft, fs, w)
+    nf, hop, m = 32, 3, y.mean()
+    segs = [(y[s:s + w] - m) for s in range(0, 23 - w + 1, hop)]
+    ref = np.zeros(nf // 2 + 1)
+    for d in segs:
+        for k in range(nf // 2 + 1):
+            X = sum(d[i] * np.exp(-2j * np.pi * k * i / nf) for i in range(w))
+            ref[k] += abs(X) ** 2
+    ref /= len(segs) * w * fs
+    np.testing.assert_allclose(psd, ref, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(freqs, np.arange(17) * fs / nf)
+
+
+def test_welch_closed_forms(orc):
+    # one segment, window = nfft: Parseval over the completed two-sided spectrum
+    rng = np.random.default_rng(5)
+    y = rng.normal(size=64)
+    psd, _ = orc.welch_psd(y, 64, 1.0, 64)
+    two_sided = psd[0] + 2 * psd[1:-1].sum() + psd[-1]
+    d = y - y.mean()
+    np.testing.assert_allclose(two_sided * 64 * 1.0, 64 * (d * d).sum(), rtol=1e-12)
+    # a cosine on bin 5 of a 64-point window: all power there, |X|^2 = (64/2)^2
+    c = np.cos(2 * np.pi * 5 * np.arange(64) / 64)
+    psd, _ = orc.welch_psd(c, 64, 1.0, 64)
+    assert abs(psd[5] * 64 - 32.0 ** 2) < 1e-9 and np.all(np.abs(np.delete(psd, 5)) < 1e-20)
+    # a window longer than the series: no segment fits, the PSD is zero
+    psd, _ = orc.welch_psd(y[:10], 16, 1.0, 12)
+    assert psd.shape == (9,) and np.all(psd == 0.0)
+    # constant series: zero power -> NaN features
+    assert np.all(np.isnan(orc.spectral_features(np.full(100, 3.0, np.float32), 100.0)))
+
+
 # ---------------------------------------------------------------- delays ----
 
 def test_slot_tables_ring(orc):
