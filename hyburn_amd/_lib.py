@@ -126,6 +126,20 @@ class HbAdjointStats(C.Structure):
                 ("device_bytes", C.c_int64)]
 
 
+class HbMultiProjection(C.Structure):
+    _fields_ = [("src", C.c_int32), ("tgt", C.c_int32), ("wkind", C.c_int32), ("scalar_w", C.c_float),
+                ("dense_w", fp), ("cfn", C.c_int32), ("n_cparams", C.c_int32), ("cparams", C.c_float * 8),
+                ("n_map", C.c_int32), ("map_src", C.c_int32 * 4), ("map_tgt", C.c_int32 * 4), ("delay", C.c_int32)]
+
+
+class HbMultiSpec(C.Structure):
+    _fields_ = [("n_subnets", C.c_int32), ("subnets", C.POINTER(HbSubnetwork)), ("n_projections", C.c_int32),
+                ("projections", C.POINTER(HbMultiProjection)), ("dt", C.c_float), ("integrator", C.c_int32),
+                ("hybrid", C.c_int32), ("n_sweep", C.c_int32), ("n_steps", C.c_int32), ("sweep_sub", C.c_int32),
+                ("sweep_param", C.c_int32), ("sweep_values", fp), ("device", C.c_int32), ("tavg", fp),
+                ("final_state", fp), ("kernel_ms", C.c_double)]
+
+
 # Every exported function of include/hyburn_hip.h with its prototype.
 PROTOTYPES = {
     "hb_engine_create": (C.c_int, [C.POINTER(HbSimConfig), C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
@@ -157,6 +171,7 @@ PROTOTYPES = {
     "hb_fc_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
     "hb_fcd_windows": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
     "hb_fcd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
+    "hb_run_multi": (C.c_int, [C.POINTER(HbMultiSpec)]),
     "hb_welch_bins": (C.c_int32, [C.c_int32, C.c_int32]),
     "hb_welch_psd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                      C.POINTER(C.c_double), C.c_int32]),

@@ -1,0 +1,417 @@
+// hb_multi.hip -- the batch engine's multi-subnetwork step on the GPU.
+//
+// BatchHybridEngine::run_sweep with several subnetworks and projections
+// between them (engine.rs:540-773): every step, each projection reads the
+// first ncvar state variables of its source subnetwork, applies the coupling
+// function's pre, weights them (Scalar: w * mean over source nodes,
+// engine.rs:602-608; Dense: W @ pre, 609-620), applies post (with the
+// target's own state where the function needs it), remaps the channels by
+// cvar_map (637-664) and adds the result into the target's coupling
+// (666-673). Then every subnetwork integrates with that frozen coupling,
+// Heun for G2DO and the configured rule for the others when hybrid_integrator
+// is set (698-702, dfun.rs:1871), clamps, and adds to its temporal average.
+//
+// One workgroup per sweep point, one thread per node of every subnetwork;
+// the pre values of each projection's sources live in LDS. Supported: zero
+// projection delays, deterministic Euler / Heun / RK4 (the host rejects the
+// rest with HB_INVALID_CONFIG).
+#include "hb_internal.h"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace hb {
+namespace multi {
+
+#include "hb_model.h"
+
+constexpr int kMaxSub = 4, kMaxProj = 8, kNV = 6, kMaxNodes = 1024;
+constexpr int kMaxLdsFloats = 16384;   // 64 KiB of pre values
+
+struct Sub {
+    int32_t model, n, off, nv, integ, sw;
+    float mc[kMC];
+};
+
+struct Proj {
+    int32_t src, tgt, wkind, cfn, np, nc, classic, pass_all, n_map, lds_off;
+    int32_t map_s[4], map_c[4];
+    float w, cc[8];
+    const float* W;        // [n_tgt][n_src] (HB_W_DENSE) or null
+    const float* rowsum;   // [n_tgt]: Difference over a non-square Dense projection, or null
+};
+
+struct Args {
+    Sub sub[kMaxSub];
+    Proj proj[kMaxProj];
+    int32_t n_sub, n_proj, n_tot, n_steps;
+    float dt, dth, dt6, inv_nsteps;
+    const float* sweep;    // [B] swept values, or null
+    const float* x0;       // [n_tot][kNV]
+    float* tavg;           // [B][n_tot][kNV]
+    float* fin;            // [B][n_tot][kNV]
+};
+
+__device__ __forceinline__ void pre_rt(const Proj& P, const float* x, float* out) {
+    if (P.nc == 1) {
+        if (P.np == 2) pre<1, 2>(P.cfn, P.cc, false, P.classic, x, out);
+        else pre<1, 1>(P.cfn, P.cc, false, P.classic, x, out);
+    } else {
+        if (P.np == 4) pre<2, 4>(P.cfn, P.cc, false, P.classic, x, out);
+        else if (P.np == 1) pre<2, 1>(P.cfn, P.cc, false, P.classic, x, out);
+        else pre<2, 2>(P.cfn, P.cc, false, P.classic, x, out);
+    }
+}
+
+__device__ __forceinline__ void post_rt(const Proj& P, const float* g, const float* own, float* out) {
+    if (P.nc == 1) {
+        if (P.np == 2) post<1, 2>(P.cfn, P.cc, P.classic, g, own, out);
+        else post<1, 1>(P.cfn, P.cc, P.classic, g, own, out);
+    } else {
+        if (P.np == 4) post<2, 4>(P.cfn, P.cc, P.classic, g, own, out);
+        else if (P.np == 1) post<2, 1>(P.cfn, P.cc, P.classic, g, own, out);
+        else post<2, 2>(P.cfn, P.cc, P.classic, g, own, out);
+    }
+}
+
+__device__ __forceinline__ void step_model(int model, const StepCtx& S, const float* mc, const float* cpl,
+                                           const float* z, float* x) {
+    switch (model) {
+    case HB_G2DO: integrate<HB_G2DO>(S, mc, cpl, z, x); break;
+    case HB_MPR: integrate<HB_MPR>(S, mc, cpl, z, x); break;
+    case HB_JANSEN_RIT: integrate<HB_JANSEN_RIT>(S, mc, cpl, z, x); break;
+    case HB_WILSON_COWAN: integrate<HB_WILSON_COWAN>(S, mc, cpl, z, x); break;
+    case HB_KURAMOTO: integrate<HB_KURAMOTO>(S, mc, cpl, z, x); break;
+    default: integrate<HB_RWW>(S, mc, cpl, z, x); break;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
+    extern __shared__ float pv[];   // pre values of every projection's source nodes
+    const int t = threadIdx.x, b = blockIdx.x;
+    int s = -1;
+    for (int q = 0; q < A->n_sub; ++q)
+        if (t >= A->sub[q].off && t < A->sub[q].off + A->sub[q].n) s = q;
+    const bool live = s >= 0;
+    const int i = live ? t - A->sub[s].off : 0;
+    const int model = live ? A->sub[s].model : 0, nv = live ? A->sub[s].nv : 0;
+    float mc[kMC];
+#pragma unroll
+    for (int k = 0; k < kMC; ++k) mc[k] = live ? A->sub[s].mc[k] : 0.0f;
+    float x[kNV], acc[kNV];
+    const float zero[kNV] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int v = 0; v < kNV; ++v) {
+        x[v] = (live && v < nv) ? A->x0[(size_t)t * kNV + v] : 0.0f;
+        acc[v] = 0.0f;
+    }
+    StepCtx S;
+    S.dt = A->dt;
+    S.dth = A->dth;
+    S.dt6 = A->dt6;
+    S.integ = live ? A->sub[s].integ : HB_EULER;
+    S.stoch = false;
+    S.plain = false;
+    S.sw = live && A->sub[s].sw != 0;
+    S.sv = A->sweep ? A->sweep[b] : 0.0f;
+    for (int step = 0; step < A->n_steps; ++step) {
+        for (int p = 0; p < A->n_proj; ++p) {
+            const Proj& P = A->proj[p];
+            if (live && s == P.src) {
+                float o[4];
+                pre_rt(P, x, o);
+                for (int c = 0; c < P.np; ++c) pv[P.lds_off + i * P.np + c] = o[c];
+            }
+        }
+        __syncthreads();
+        if (live) {
+            float cpl[2] = {0.0f, 0.0f};
+            for (int p = 0; p < A->n_proj; ++p) {
+                const Proj& P = A->proj[p];
+                if (s != P.tgt) continue;
+                const int ns = A->sub[P.src].n, np = P.np;
+                const float* src = pv + P.lds_off;
+                float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (P.wkind == HB_W_SCALAR_MEAN) {
+                    for (int c = 0; c < np; ++c) {
+                        float sum = 0.0f;
+                        for (int j = 0; j < ns; ++j) sum += src[j * np + c];
+                        g[c] = (sum / (float)ns) * P.w;
+                    }
+                } else {
+                    const float* Wr = P.W + (size_t)i * ns;
+                    for (int j = 0; j < ns; ++j) {
+                        const float w = Wr[j];
+                        for (int c = 0; c < np; ++c) g[c] += w * src[j * np + c];
+                    }
+                }
+                if (P.rowsum) {
+                    const float rs = P.rowsum[i];
+                    for (int c = 0; c < np; ++c) g[c] = g[c] - x[c] * rs;
+                }
+                float own[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                if (P.cfn == HB_CF_KURAMOTO)
+                    for (int c = 0; c < P.nc; ++c) {
+                        own[c] = sinf(x[c]);
+                        own[P.nc + c] = cosf(x[c]);
+                    }
+                float out[2] = {0.0f, 0.0f};
+                post_rt(P, g, own, out);
+                if (P.pass_all) {
+                    for (int c = 0; c < P.nc; ++c) cpl[c] += out[c];
+                } else {
+                    for (int m = 0; m < P.n_map; ++m) cpl[P.map_c[m]] += out[P.map_s[m]];
+                }
+            }
+            step_model(model, S, mc, cpl, zero, x);
+#pragma unroll
+            for (int v = 0; v < kNV; ++v) acc[v] += x[v];
+        }
+        __syncthreads();
+    }
+    if (live) {
+        const size_t o = ((size_t)b * A->n_tot + t) * kNV;
+        for (int v = 0; v < nv; ++v) {
+            A->tavg[o + v] = acc[v] * A->inv_nsteps;
+            A->fin[o + v] = x[v];
+        }
+    }
+}
+
+// Position of state variable t among the model's coupling variables (the
+// CVAR constants of src/model/*.rs), 0 when absent (engine.rs:391-393).
+int cvar_position(int model, int t) {
+    static const int g2do[] = {0}, mpr[] = {0, 1}, one0[] = {0}, jr[] = {1};
+    const int* cv = one0;
+    int n = 1;
+    if (model == HB_G2DO) cv = g2do;
+    else if (model == HB_MPR) { cv = mpr; n = 2; }
+    else if (model == HB_JANSEN_RIT) cv = jr;
+    for (int k = 0; k < n; ++k)
+        if (cv[k] == t) return k;
+    return 0;
+}
+
+struct DevBufs {
+    std::vector<void*> p;
+    ~DevBufs() {
+        for (void* q : p)
+            if (q) (void)hipFree(q);
+    }
+    template <typename T> hipError_t alloc(T** out, size_t count) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, count ? count * sizeof(T) : 1);
+        if (e == hipSuccess) p.push_back(q);
+        *out = (T*)q;
+        return e;
+    }
+};
+
+}  // namespace multi
+}  // namespace hb
+
+using namespace hb;
+
+namespace {
+
+int bad(const std::string& m) { return set_error(HB_INVALID_CONFIG, m); }
+
+int hip_fail(hipError_t e, const char* what) {
+    return set_error(HB_BACKEND_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" int hb_run_multi(hb_multi_spec* spec) {
+    using namespace hb::multi;
+    if (!spec) return bad("null spec");
+    const hb_multi_spec& Q = *spec;
+    if (Q.n_subnets < 1 || Q.n_subnets > kMaxSub || !Q.subnets) return bad("n_subnets must be in [1, 4]");
+    if (Q.n_projections < 0 || Q.n_projections > kMaxProj || (Q.n_projections > 0 && !Q.projections))
+        return bad("n_projections must be in [0, 8]");
+    if (Q.n_sweep < 1 || Q.n_steps < 0) return bad("n_sweep must be >= 1 and n_steps >= 0");
+    if (Q.integrator != HB_EULER && Q.integrator != HB_HEUN && Q.integrator != HB_RK4)
+        return bad("the multi-subnetwork path integrates deterministically (Euler, Heun, RK4)");
+    if (Q.sweep_sub < -1 || Q.sweep_sub >= Q.n_subnets) return bad("sweep subnetwork index out of range");
+    if (Q.sweep_sub >= 0 && !Q.sweep_values) return bad("sweep values missing");
+    if (!Q.tavg || !Q.final_state) return bad("null output pointer");
+    Args A;
+    std::memset(&A, 0, sizeof(A));
+    int n_tot = 0;
+    for (int q = 0; q < Q.n_subnets; ++q) {
+        const hb_subnetwork& U = Q.subnets[q];
+        if (U.model < HB_G2DO || U.model > HB_RWW) return bad("unknown model in subnetwork " + std::to_string(q));
+        if (U.n_nodes < 1 || !U.params || !U.x0) return bad("subnetwork " + std::to_string(q) + ": nodes, params, x0");
+        Sub& S = A.sub[q];
+        S.model = U.model;
+        S.n = U.n_nodes;
+        S.off = n_tot;
+        S.nv = model_nvar(U.model);
+        // engine.rs:698-702: hybrid -> Heun for G2DO; otherwise the configured
+        // integrator, where a configured Heun falls through to Euler
+        if (Q.hybrid) S.integ = U.model == HB_G2DO ? HB_HEUN : (Q.integrator == HB_HEUN ? HB_EULER : Q.integrator);
+        else S.integ = Q.integrator;
+        S.sw = (q == Q.sweep_sub && Q.sweep_param == model_sweep_index(U.model)) ? 1 : 0;
+        model_consts(U.model, U.params, S.mc);
+        n_tot += U.n_nodes;
+    }
+    if (n_tot > kMaxNodes) return bad("at most 1024 nodes over all subnetworks");
+    DevBufs B;
+    std::vector<std::vector<float>> host_w((size_t)Q.n_projections), host_rs((size_t)Q.n_projections);
+    int lds = 0;
+    for (int p = 0; p < Q.n_projections; ++p) {
+        const hb_multi_projection& R = Q.projections[p];
+        const std::string tag = "projection " + std::to_string(p);
+        if (R.src < 0 || R.src >= Q.n_subnets || R.tgt < 0 || R.tgt >= Q.n_subnets) return bad(tag + ": src/tgt out of range");
+        if (R.delay != 0) return bad(tag + ": projection delays are not supported on the multi-subnetwork path");
+        if (R.wkind != HB_W_SCALAR_MEAN && R.wkind != HB_W_DENSE) return bad(tag + ": weights must be Scalar or Dense");
+        if (R.wkind == HB_W_DENSE && !R.dense_w) return bad(tag + ": dense weights missing");
+        if (R.cfn < HB_CF_LINEAR || R.cfn > HB_CF_PRESIGMOIDAL) return bad(tag + ": unknown coupling function");
+        if (R.n_cparams < 0 || R.n_cparams > 8) return bad(tag + ": at most 8 coupling parameters");
+        const int ms = Q.subnets[R.src].model, mt = Q.subnets[R.tgt].model;
+        const int ncs = model_ncvar(ms), nct = model_ncvar(mt);
+        const int ns = Q.subnets[R.src].n_nodes, nt = Q.subnets[R.tgt].n_nodes;
+        if (R.n_map < 1 || R.n_map > 4) return bad(tag + ": cvar_map needs 1 to 4 pairs");
+        for (int m = 0; m < R.n_map; ++m) {   // engine.rs:371-388
+            if (R.map_src[m] < 0 || R.map_src[m] >= ncs)
+                return bad(tag + " cvar_map src index " + std::to_string(R.map_src[m]) + " >= src_ncvar " + std::to_string(ncs));
+            if (R.map_tgt[m] < 0 || R.map_tgt[m] >= nct)
+                return bad(tag + " cvar_map tgt index " + std::to_string(R.map_tgt[m]) + " >= tgt_ncvar " + std::to_string(nct));
+        }
+        Proj& P = A.proj[p];
+        P.src = R.src;
+        P.tgt = R.tgt;
+        P.wkind = R.wkind;
+        P.cfn = R.cfn;
+        P.nc = ncs;
+        P.np = coupling_npre(R.cfn, ncs);
+        if ((R.cfn == HB_CF_KURAMOTO || R.cfn == HB_CF_DIFFERENCE) && ncs != nct)
+            return bad(tag + ": Kuramoto and Difference coupling need equal source and target ncvar");
+        float cp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int k = 0; k < R.n_cparams; ++k) cp[k] = R.cparams[k];
+        int classic = 0;
+        coupling_consts(R.cfn, cp, R.n_cparams, ns, P.cc, &classic);
+        P.classic = classic;
+        P.w = R.scalar_w;
+        if (R.wkind == HB_W_DENSE) host_w[p].assign(R.dense_w, R.dense_w + (size_t)nt * ns);
+        if (R.cfn == HB_CF_DIFFERENCE) {   // engine.rs:325-366
+            const float a = R.n_cparams > 0 ? cp[0] : 1.0f;
+            if (R.wkind == HB_W_SCALAR_MEAN) {
+                P.w = 0.0f;
+                P.cfn = HB_CF_LINEAR;
+                P.cc[0] = a;
+                P.cc[1] = 0.0f;
+            } else if (nt == ns) {
+                std::vector<float>& W = host_w[p];
+                for (int r = 0; r < nt; ++r) {
+                    float sum = 0.0f;
+                    for (int j = 0; j < ns; ++j) sum += W[(size_t)r * ns + j];
+                    W[(size_t)r * ns + r] -= sum;
+                }
+                P.cfn = HB_CF_LINEAR;
+                P.cc[0] = a;
+                P.cc[1] = 0.0f;
+            } else {
+                host_rs[p].assign((size_t)nt, 0.0f);
+                for (int r = 0; r < nt; ++r) {
+                    float sum = 0.0f;
+                    for (int j = 0; j < ns; ++j) sum += host_w[p][(size_t)r * ns + j];
+                    host_rs[p][r] = sum;
+                }
+                P.cc[0] = a;
+            }
+        }
+        P.pass_all = (nct == ncs && R.n_map == 1 && R.map_src[0] == 0 && R.map_tgt[0] == 0) ? 1 : 0;
+        P.n_map = R.n_map;
+        for (int m = 0; m < R.n_map; ++m) {
+            P.map_s[m] = R.map_src[m];
+            P.map_c[m] = cvar_position(mt, R.map_tgt[m]);
+        }
+        P.lds_off = lds;
+        lds += ns * P.np;
+    }
+    if (lds > kMaxLdsFloats) return bad("projection sources exceed 64 KiB of pre values");
+    hipError_t e = hipSetDevice(Q.device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    for (int p = 0; p < Q.n_projections; ++p) {
+        if (!host_w[p].empty()) {
+            float* d = nullptr;
+            e = B.alloc(&d, host_w[p].size());
+            if (e == hipSuccess) e = hipMemcpy(d, host_w[p].data(), sizeof(float) * host_w[p].size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(e, "dense weights upload");
+            A.proj[p].W = d;
+        }
+        if (!host_rs[p].empty()) {
+            float* d = nullptr;
+            e = B.alloc(&d, host_rs[p].size());
+            if (e == hipSuccess) e = hipMemcpy(d, host_rs[p].data(), sizeof(float) * host_rs[p].size(), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(e, "row sums upload");
+            A.proj[p].rowsum = d;
+        }
+    }
+    A.n_sub = Q.n_subnets;
+    A.n_proj = Q.n_projections;
+    A.n_tot = n_tot;
+    A.n_steps = Q.n_steps;
+    A.dt = Q.dt;
+    A.dth = Q.dt * 0.5f;
+    A.dt6 = Q.dt / 6.0f;
+    A.inv_nsteps = 1.0f / (float)Q.n_steps;
+    std::vector<float> x0((size_t)n_tot * kNV, 0.0f);
+    for (int q = 0; q < Q.n_subnets; ++q)
+        for (int r = 0; r < A.sub[q].n; ++r)
+            for (int v = 0; v < A.sub[q].nv; ++v)
+                x0[(size_t)(A.sub[q].off + r) * kNV + v] = Q.subnets[q].x0[(size_t)r * A.sub[q].nv + v];
+    const size_t nout = (size_t)Q.n_sweep * n_tot * kNV;
+    float *dx0 = nullptr, *dsw = nullptr, *dtavg = nullptr, *dfin = nullptr;
+    Args* dA = nullptr;
+    e = B.alloc(&dx0, x0.size());
+    if (e == hipSuccess) e = B.alloc(&dtavg, nout);
+    if (e == hipSuccess) e = B.alloc(&dfin, nout);
+    if (e == hipSuccess) e = B.alloc(&dA, 1);
+    if (e == hipSuccess && Q.sweep_sub >= 0) e = B.alloc(&dsw, (size_t)Q.n_sweep);
+    if (e == hipSuccess) e = hipMemcpy(dx0, x0.data(), sizeof(float) * x0.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && dsw) e = hipMemcpy(dsw, Q.sweep_values, sizeof(float) * Q.n_sweep, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "multi-subnetwork upload");
+    A.x0 = dx0;
+    A.sweep = dsw;
+    A.tavg = dtavg;
+    A.fin = dfin;
+    e = hipMemcpy(dA, &A, sizeof(A), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "multi-subnetwork arguments");
+    const int threads = std::min(kMaxNodes, (n_tot + 63) / 64 * 64);
+    hipEvent_t ev0, ev1;
+    (void)hipEventCreate(&ev0);
+    (void)hipEventCreate(&ev1);
+    (void)hipEventRecord(ev0, nullptr);
+    hipLaunchKernelGGL(k_multi, dim3((unsigned)Q.n_sweep), dim3((unsigned)threads), (size_t)lds * sizeof(float),
+                       nullptr, dA);
+    e = hipGetLastError();
+    (void)hipEventRecord(ev1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(ev1);
+    float ms = 0.0f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, ev0, ev1);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (e != hipSuccess) return hip_fail(e, "multi-subnetwork kernel");
+    spec->kernel_ms = ms;
+    std::vector<float> ht(nout), hf(nout);
+    e = hipMemcpy(ht.data(), dtavg, sizeof(float) * nout, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hf.data(), dfin, sizeof(float) * nout, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(e, "multi-subnetwork download");
+    // per subnetwork, [n_sweep][n_nodes][nvar] (the BatchSweepResult layout), concatenated
+    size_t o = 0;
+    for (int q = 0; q < Q.n_subnets; ++q) {
+        const Sub& S = A.sub[q];
+        for (int b = 0; b < Q.n_sweep; ++b)
+            for (int r = 0; r < S.n; ++r)
+                for (int v = 0; v < S.nv; ++v, ++o) {
+                    const size_t src = ((size_t)b * n_tot + S.off + r) * kNV + v;
+                    Q.tavg[o] = ht[src];
+                    Q.final_state[o] = hf[src];
+                }
+    }
+    return HB_OK;
+}

@@ -1,0 +1,114 @@
+"""Several subnetworks joined by projections: the multi-subnetwork form of
+BatchHybridEngine::run_sweep (src/engine/batch_engine/engine.rs:540-823),
+backed by hb_run_multi.
+
+  Subnet(model, n_nodes, params, x0)        SubnetworkConfig (config.rs:83-114)
+  Link(src, tgt, weights, coupling_fn, coupling_params, cvar_map, delay)
+                                            ProjectionConfig: weights a float
+                                            (Scalar: w * mean over source
+                                            nodes) or a [n_tgt][n_src] array
+  run_multi(subnets, links, dt, n_steps, integrator, hybrid_integrator,
+            sweep=(sub_idx, param_idx, values))
+      -> MultiResult(tavg, final_states, kernel_ms), one [n_sweep][n_nodes][nvar]
+         array per subnetwork, as BatchSweepResult holds them.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+@dataclass
+class Subnet:
+    model: int
+    n_nodes: int
+    params: Sequence[float]
+    x0: Optional[np.ndarray] = None          # [n_nodes][nvar]; zeros by default
+
+
+@dataclass
+class Link:
+    src: int
+    tgt: int
+    weights: object = 0.0                    # float (Scalar) or [n_tgt][n_src] array
+    coupling_fn: int = L.CF_LINEAR
+    coupling_params: Sequence[float] = (1.0,)
+    cvar_map: Sequence[tuple] = ((0, 0),)
+    delay: int = 0
+
+
+@dataclass
+class MultiResult:
+    tavg: list = field(default_factory=list)
+    final_states: list = field(default_factory=list)
+    kernel_ms: float = 0.0
+
+
+def parse_cvar_map(s: str) -> list:
+    """'0:0,1:1' -> [(0, 0), (1, 1)] (the ProjectionConfig.cvar_map string)."""
+    return [tuple(int(v) for v in part.split(":")) for part in s.split(",") if part.strip()]
+
+
+def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_steps: int,
+              integrator: int = L.HEUN, hybrid_integrator: bool = False, n_sweep: int = 1,
+              sweep: Optional[tuple] = None, device: int = 0) -> MultiResult:
+    keep = []
+    subs = (L.HbSubnetwork * len(subnets))()
+    for k, S in enumerate(subnets):
+        nv = L.MODEL_INFO[S.model][0]
+        p = np.ascontiguousarray(S.params, np.float32)
+        x0 = np.zeros((S.n_nodes, nv), np.float32) if S.x0 is None else np.ascontiguousarray(S.x0, np.float32)
+        if x0.shape != (S.n_nodes, nv):
+            raise ValueError(f"subnetwork {k}: x0 must be [{S.n_nodes}][{nv}]")
+        keep += [p, x0]
+        subs[k] = L.HbSubnetwork(S.model, S.n_nodes, len(p), p.ctypes.data_as(L.fp), x0.ctypes.data_as(L.fp))
+    projs = (L.HbMultiProjection * max(1, len(links)))()
+    for k, R in enumerate(links):
+        P = projs[k]
+        P.src, P.tgt, P.cfn, P.delay = R.src, R.tgt, R.coupling_fn, R.delay
+        if np.isscalar(R.weights):
+            P.wkind, P.scalar_w = L.W_SCALAR_MEAN, float(R.weights)
+        else:
+            W = np.ascontiguousarray(R.weights, np.float32)
+            if W.shape != (subnets[R.tgt].n_nodes, subnets[R.src].n_nodes):
+                raise ValueError(f"projection {k}: dense weights must be [n_tgt][n_src]")
+            keep.append(W)
+            P.wkind, P.dense_w = L.W_DENSE, W.ctypes.data_as(L.fp)
+        cp = list(R.coupling_params)
+        P.n_cparams = len(cp)
+        for j, v in enumerate(cp[:8]):
+            P.cparams[j] = v
+        cm = list(R.cvar_map)
+        P.n_map = len(cm)
+        for j, (a, b) in enumerate(cm[:4]):
+            P.map_src[j], P.map_tgt[j] = a, b
+    spec = L.HbMultiSpec()
+    spec.n_subnets, spec.subnets = len(subnets), subs
+    spec.n_projections, spec.projections = len(links), projs
+    spec.dt, spec.integrator, spec.hybrid = dt, integrator, int(bool(hybrid_integrator))
+    spec.n_sweep, spec.n_steps, spec.device = n_sweep, n_steps, device
+    spec.sweep_sub, spec.sweep_param = -1, 0
+    if sweep is not None:
+        sub_idx, param_idx, values = sweep
+        vals = np.ascontiguousarray(values, np.float32)
+        if len(vals) != n_sweep:
+            raise ValueError("number of sweep values must equal n_sweep")
+        keep.append(vals)
+        spec.sweep_sub, spec.sweep_param, spec.sweep_values = sub_idx, param_idx, vals.ctypes.data_as(L.fp)
+    sizes = [n_sweep * S.n_nodes * L.MODEL_INFO[S.model][0] for S in subnets]
+    tavg = np.zeros(sum(sizes), np.float32)
+    fin = np.zeros(sum(sizes), np.float32)
+    spec.tavg, spec.final_state = tavg.ctypes.data_as(L.fp), fin.ctypes.data_as(L.fp)
+    L.check(L.lib().hb_run_multi(spec))
+    out = MultiResult(kernel_ms=spec.kernel_ms)
+    o = 0
+    for S, n in zip(subnets, sizes):
+        shape = (n_sweep, S.n_nodes, L.MODEL_INFO[S.model][0])
+        out.tavg.append(tavg[o:o + n].reshape(shape).copy())
+        out.final_states.append(fin[o:o + n].reshape(shape).copy())
+        o += n
+    return out

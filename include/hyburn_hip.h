@@ -271,6 +271,45 @@ void hb_adjoint_plan_destroy(hb_adjoint_plan* p);
 int hb_autotune_entries(int32_t* out, int32_t max_entries);
 void hb_autotune_clear(void);
 
+/* ---------------- several subnetworks (BatchHybridEngine with projections) ---------------- */
+typedef struct {
+    int32_t src, tgt;         /* subnetwork indices */
+    int32_t wkind;            /* HB_W_SCALAR_MEAN (w * mean over source nodes) or HB_W_DENSE */
+    float scalar_w;
+    const float* dense_w;     /* [n_tgt][n_src]: row = target node (CSR densified, engine.rs:298-311) */
+    int32_t cfn, n_cparams;   /* HB_CF_*, coupling_params */
+    float cparams[8];
+    int32_t n_map;            /* cvar_map pairs map_src[k]:map_tgt[k] ("0:0" = one pair 0, 0) */
+    int32_t map_src[4], map_tgt[4];
+    int32_t delay;            /* steps; this path supports 0 */
+} hb_multi_projection;
+
+typedef struct {
+    int32_t n_subnets;        /* 1..4, at most 1024 nodes together */
+    const hb_subnetwork* subnets;
+    int32_t n_projections;    /* 0..8 */
+    const hb_multi_projection* projections;
+    float dt;
+    int32_t integrator;       /* HB_EULER, HB_HEUN or HB_RK4 */
+    int32_t hybrid;           /* hybrid_integrator: Heun for G2DO (engine.rs:698-702) */
+    int32_t n_sweep, n_steps;
+    int32_t sweep_sub, sweep_param;   /* SweepParam; sweep_sub = -1: none */
+    const float* sweep_values;        /* [n_sweep] */
+    int32_t device;
+    float* tavg;              /* out: per subnetwork [n_sweep][n_nodes][nvar], concatenated */
+    float* final_state;       /* out: same layout */
+    double kernel_ms;         /* out: the kernel's HIP-event time */
+} hb_multi_spec;
+
+/* BatchHybridEngine::run_sweep over several subnetworks joined by projections
+ * (src/engine/batch_engine/engine.rs:540-823): per step, each projection's
+ * coupling from the current source states (pre, Scalar mean or Dense weights,
+ * post, cvar_map remap), summed per target, then every subnetwork integrates
+ * with that frozen coupling; tavg = sum of states / n_steps. HB_INVALID_CONFIG
+ * for delays, noise, unknown models or cvar_map indices out of range
+ * (engine.rs:371-388). */
+int hb_run_multi(hb_multi_spec* spec);
+
 /* ---------------- functional-connectivity features ---------------- */
 /* Batched FC: series is [n_series][n_steps][n_nodes] row-major (host memory),
  * fc receives [n_series][n_nodes][n_nodes]. Each matrix follows fc_matrix

@@ -304,6 +304,104 @@ def fcd(traj, window_len, stride):
     return out if rc > 1 else np.zeros((0, 0), np.float32)
 
 
+_CVAR = {0: (0,), 1: (0, 1), 2: (1,), 3: (0,), 4: (0,), 5: (0,)}   # CVAR of src/model/*.rs
+_IDENTITY_PRE = (0, 1, 3, 4)   # Linear, Difference, Sigmoidal, ScaledLinear: pre is the identity
+
+
+def _f32sum(v):
+    """Sequential f32 sum (the order the restatement fixes)."""
+    return np.cumsum(np.asarray(v, np.float32), dtype=np.float32)[-1] if len(v) else np.float32(0.0)
+
+
+def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_sweep=1):
+    """The batch engine's multi-subnetwork loop (engine.rs:540-823) in f32, one
+    sweep point at a time: coupling of every projection from the current
+    states (engine.rs:557-673), then each subnetwork's integration with that
+    frozen coupling (676-773), clamp, tavg. subnets: (model, n, params, x0
+    [n][nvar]); links: (src, tgt, weights float|[nt][ns], cfn, cparams,
+    cvar_map, delay=0). Returns per subnetwork tavg and final [n_sweep][n][nvar]."""
+    from hyburn_amd import _lib as HL
+    f32 = np.float32
+    S = len(subnets)
+    info = [HL.MODEL_INFO[m] for (m, _, _, _) in subnets]
+    # Difference preprocessing, engine.rs:325-366
+    prep = []
+    for (src, tgt, w, cfn, cp, cmap, *rest) in links:
+        cp = list(cp)
+        ns, nt = subnets[src][1], subnets[tgt][1]
+        W = None if np.isscalar(w) else np.array(w, np.float32)
+        if cfn == HL.CF_DIFFERENCE:
+            a = cp[0] if cp else 1.0
+            if W is None:
+                w, cfn, cp = 0.0, HL.CF_LINEAR, [a, 0.0]
+            elif nt == ns:
+                for i in range(nt):
+                    W[i, i] = f32(W[i, i] - _f32sum(W[i]))
+                cfn, cp = HL.CF_LINEAR, [a, 0.0]
+        prep.append((src, tgt, w, W, cfn, cp, list(cmap)))
+    tav = [np.zeros((n_sweep, n, info[k][0]), np.float32) for k, (_, n, _, _) in enumerate(subnets)]
+    fin = [np.zeros_like(t) for t in tav]
+    dtf, dth, dt6 = f32(dt), f32(f32(dt) * f32(0.5)), f32(f32(dt) / f32(6.0))
+    for b in range(n_sweep):
+        x = [np.array(x0, np.float32).reshape(n, info[k][0]) for k, (_, n, _, x0) in enumerate(subnets)]
+        acc = [np.zeros_like(v) for v in x]
+        sw = [0] * S
+        sv = 0.0
+        if sweep is not None:
+            sub_idx, param_idx, values = sweep
+            sv = float(values[b])
+            if param_idx == HL.SWEEP_INDEX[subnets[sub_idx][0]]:
+                sw[sub_idx] = 1
+        for _ in range(n_steps):
+            cpl = [None] * S
+            for (src, tgt, w, W, cfn, cp, cmap) in prep:
+                ms, mt = subnets[src][0], subnets[tgt][0]
+                ncs, nct = info[src][1], info[tgt][1]
+                ns, nt = subnets[src][1], subnets[tgt][1]
+                xs = x[src][:, :ncs]
+                xt = x[tgt][:, :ncs] if nct == ncs else np.zeros((nt, ncs), np.float32)
+                if W is None:
+                    if cfn not in _IDENTITY_PRE:
+                        raise NotImplementedError("Scalar weights with a non-identity pre")
+                    g = np.array([[f32(f32(_f32sum(xs[:, c]) / f32(ns)) * f32(w)) for c in range(ncs)]], np.float32)
+                    out = coupling(cfn, cp, g, xt, dense=np.ones((nt, 1), np.float32), n_src=ns)
+                else:
+                    out = coupling(cfn, cp, xs, xt, dense=W, n_src=ns)
+                if nct == ncs and cmap == [(0, 0)]:
+                    rem = out
+                else:
+                    rem = np.zeros((nt, nct), np.float32)
+                    for (sc, tc) in cmap:
+                        ci = _CVAR[mt].index(tc) if tc in _CVAR[mt] else 0
+                        if sc < ncs and ci < nct:
+                            rem[:, ci] = rem[:, ci] + out[:, sc]
+                cpl[tgt] = rem if cpl[tgt] is None else (cpl[tgt] + rem).astype(np.float32)
+            for k, (m, n, params, _) in enumerate(subnets):
+                c = cpl[k] if cpl[k] is not None else np.zeros((n, info[k][1]), np.float32)
+                heun = (m == HL.G2DO) if hybrid else (integrator == HL.HEUN)
+                for i in range(n):
+                    xi, ci = x[k][i].copy(), c[i]
+                    d = dfun(m, params, xi, ci, sw[k], sv)
+                    if heun:
+                        pr = clamp(m, (xi + d * dtf).astype(np.float32))
+                        d2 = dfun(m, params, pr, ci, sw[k], sv)
+                        xn = (xi + (d + d2) * dth).astype(np.float32)
+                    elif integrator == HL.RK4:
+                        k2 = dfun(m, params, clamp(m, xi + d * dth), ci, sw[k], sv)
+                        k3 = dfun(m, params, clamp(m, xi + k2 * dth), ci, sw[k], sv)
+                        k4 = dfun(m, params, clamp(m, xi + k3 * dtf), ci, sw[k], sv)
+                        xn = (xi + (((d + k2 * f32(2.0)) + k3 * f32(2.0)) + k4) * dt6).astype(np.float32)
+                    else:   # Euler (and a configured Heun under hybrid_integrator)
+                        xn = (xi + d * dtf).astype(np.float32)
+                    x[k][i] = clamp(m, xn)
+                acc[k] = (acc[k] + x[k]).astype(np.float32)
+        inv = f32(1.0) / f32(n_steps)
+        for k in range(S):
+            tav[k][b] = acc[k] * inv
+            fin[k][b] = x[k]
+    return tav, fin
+
+
 def next_power_of_2(n):
     """catch22/fft.rs:6-15."""
     p = 1

@@ -1,0 +1,100 @@
+"""GPU parity of the multi-subnetwork path (hb_run_multi, engine.rs:540-823)
+against the oracle's restatement (oracle.hb_oracle.run_multi).
+
+The kernel is built with FMA contraction and the device's expf/sinf, so it
+is compared to the f32 oracle within rtol 2e-4, atol 2e-5 over short runs
+(100-200 steps), and within rtol 2e-3 over the 1000-step ring workload.
+"""
+import numpy as np
+import pytest
+
+from hyburn_amd import _lib as L
+from hyburn_amd.multi import Link, Subnet, run_multi
+from hyburn_amd.synth import G2DO_DEFAULT, JR_DEFAULT, KUR_DEFAULT, MPR_DEFAULT, WC_DEFAULT
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL = 2e-4, 2e-5
+
+
+def _x0(rng, n, nv, scale=0.3):
+    return (rng.normal(size=(n, nv)) * scale).astype(np.float32)
+
+
+def _check(res, tav, fin, rtol=RTOL, atol=ATOL):
+    for k in range(len(tav)):
+        np.testing.assert_allclose(res.tavg[k], tav[k], rtol=rtol, atol=atol)
+        np.testing.assert_allclose(res.final_states[k], fin[k], rtol=rtol, atol=atol)
+
+
+def _ring(n, w=0.01):
+    subs = [(L.G2DO, n, G2DO_DEFAULT, np.zeros((n, 2), np.float32)),
+            (L.JANSEN_RIT, n, JR_DEFAULT, np.zeros((n, 6), np.float32)),
+            (L.WILSON_COWAN, n, WC_DEFAULT, np.zeros((n, 2), np.float32))]
+    links = [(0, 1, w, L.CF_LINEAR, [1.0], [(0, 0)]), (1, 2, w, L.CF_LINEAR, [1.0], [(0, 0)]),
+             (2, 0, w, L.CF_LINEAR, [1.0], [(0, 0)])]
+    return subs, links
+
+
+def _run(subs, links, **kw):
+    return run_multi([Subnet(m, n, p, x0) for (m, n, p, x0) in subs],
+                     [Link(s, t, w, cfn, cp, cm) for (s, t, w, cfn, cp, cm) in links], **kw)
+
+
+def test_three_subnet_ring_hybrid_sweep(gpu, orc):
+    # the G2DO -> JR -> WC ring of the reference's generic CUDA benchmark, at 8 nodes
+    subs, links = _ring(8)
+    vals = np.linspace(-0.5, 0.5, 3).astype(np.float32)
+    res = _run(subs, links, dt=0.1, n_steps=200, integrator=L.HEUN, hybrid_integrator=True, n_sweep=3,
+               sweep=(0, 1, vals))
+    tav, fin = orc.run_multi(subs, links, 0.1, 200, L.HEUN, True, sweep=(0, 1, vals), n_sweep=3)
+    _check(res, tav, fin)
+    assert not np.allclose(res.final_states[0][0], res.final_states[0][2])   # the sweep acts
+
+
+@pytest.mark.parametrize("integ,hybrid", [(L.EULER, False), (L.HEUN, False), (L.RK4, False), (L.RK4, True)])
+def test_dense_links_and_cvar_remap(gpu, orc, integ, hybrid):
+    rng = np.random.default_rng(integ * 2 + hybrid)
+    subs = [(L.G2DO, 7, G2DO_DEFAULT, _x0(rng, 7, 2)), (L.MPR, 5, MPR_DEFAULT, np.abs(_x0(rng, 5, 2)))]
+    W01 = (rng.uniform(size=(5, 7)) * 0.1).astype(np.float32)
+    W10 = (rng.uniform(size=(7, 5)) * 0.1).astype(np.float32)
+    links = [(0, 1, W01, L.CF_LINEAR, [1.0, 0.0], [(0, 1)]),        # G2DO V -> MPR coupling slot 1
+             (1, 0, W10, L.CF_LINEAR, [0.5, 0.0], [(1, 0)]),        # MPR var 1 -> G2DO
+             (1, 1, 0.05, L.CF_SCALED_LINEAR, [0.2, 0.1], [(0, 0)])]
+    res = _run(subs, links, dt=0.05, n_steps=150, integrator=integ, hybrid_integrator=hybrid)
+    tav, fin = orc.run_multi(subs, links, 0.05, 150, integ, hybrid)
+    _check(res, tav, fin)
+
+
+def test_difference_kuramoto_sigmoidal(gpu, orc):
+    rng = np.random.default_rng(7)
+    subs = [(L.G2DO, 5, G2DO_DEFAULT, _x0(rng, 5, 2)), (L.G2DO, 7, G2DO_DEFAULT, _x0(rng, 7, 2)),
+            (L.KURAMOTO, 6, KUR_DEFAULT, (rng.uniform(size=(6, 1)) * 6).astype(np.float32))]
+    Wd = (rng.uniform(size=(7, 5)) * 0.2).astype(np.float32)
+    Wk = (rng.uniform(size=(6, 6)) * 0.5).astype(np.float32)
+    Ws = (rng.uniform(size=(5, 5)) * 0.2).astype(np.float32)
+    links = [(0, 1, Wd, L.CF_DIFFERENCE, [0.3], [(0, 0)]),           # non-square: row sums
+             (0, 0, Ws, L.CF_DIFFERENCE, [0.3], [(0, 0)]),           # square: diagonal rewrite
+             (2, 2, Wk, L.CF_KURAMOTO, [1.5], [(0, 0)]),
+             (1, 0, 0.1, L.CF_SIGMOIDAL, [], [(0, 0)]),
+             (1, 1, 0.2, L.CF_DIFFERENCE, [0.4], [(0, 0)])]          # Scalar Difference: zero coupling
+    res = _run(subs, links, dt=0.1, n_steps=120, integrator=L.HEUN)
+    tav, fin = orc.run_multi(subs, links, 0.1, 120, L.HEUN, False)
+    _check(res, tav, fin)
+
+
+def test_ring_workload_full_length(gpu, orc):
+    # 76 nodes per subnetwork, 1000 steps, 64 sweep points: point 0 and 63 against
+    # the oracle, every point finite, tavg the mean of a bounded trajectory
+    subs, links = _ring(76)
+    vals = np.linspace(-0.5, 0.5, 64).astype(np.float32)
+    res = _run(subs, links, dt=0.1, n_steps=1000, integrator=L.HEUN, hybrid_integrator=True, n_sweep=64,
+               sweep=(0, 1, vals))
+    for k in range(3):
+        assert np.all(np.isfinite(res.tavg[k])) and np.all(np.isfinite(res.final_states[k]))
+    for b in (0, 63):
+        tav, fin = orc.run_multi(subs, links, 0.1, 1000, L.HEUN, True, sweep=(0, 1, vals[b:b + 1]), n_sweep=1)
+        for k in range(3):
+            np.testing.assert_allclose(res.tavg[k][b], tav[k][0], rtol=2e-3, atol=2e-4)
+            np.testing.assert_allclose(res.final_states[k][b], fin[k][0], rtol=2e-3, atol=2e-4)
+    assert res.kernel_ms > 0.0

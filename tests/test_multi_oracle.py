@@ -1,0 +1,88 @@
+"""Pins the oracle's multi-subnetwork restatement (oracle.hb_oracle.run_multi,
+engine.rs:540-823) and checks the host-side validation of hb_run_multi.
+
+* composition: one subnetwork with a self-projection is the single-network
+  problem the oracle's C restatement already runs (pinned by the model and
+  coupling known-answer tests), for Scalar and Dense weights, Heun and RK4;
+* independence: subnetworks without projections step as separate networks
+  with zero coupling;
+* one Euler step of a G2DO -> MPR link with cvar_map "0:1" against the
+  hand-built coupling [0, w * mean(V)] fed to the model's dfun (the remap of
+  engine.rs:637-664 and target_cvar_cpl of 391-393).
+"""
+import numpy as np
+import pytest
+
+from hyburn_amd import _lib as L
+from hyburn_amd.instances import Instance
+from hyburn_amd.synth import G2DO_DEFAULT, JR_DEFAULT, MPR_DEFAULT, WC_DEFAULT
+
+
+def _x0(rng, n, nv, scale=0.3):
+    return (rng.normal(size=(n, nv)) * scale).astype(np.float32)
+
+
+@pytest.mark.parametrize("integ", [L.HEUN, L.RK4])
+@pytest.mark.parametrize("dense", [False, True])
+def test_single_subnet_composition(orc, integ, dense):
+    rng = np.random.default_rng(1)
+    n, T = 6, 60
+    x0 = _x0(rng, n, 2)
+    W = (rng.uniform(size=(n, n)) * 0.2).astype(np.float32)
+    link = (0, 0, W if dense else 0.05, L.CF_LINEAR, [0.5, 0.01], [(0, 0)])
+    tav, fin = orc.run_multi([(L.G2DO, n, G2DO_DEFAULT, x0)], [link], 0.1, T, integ, False)
+    inst = Instance(model=L.G2DO, n_nodes=n, n_steps=T, dt=0.1, params=np.array(G2DO_DEFAULT, np.float32), x0=x0,
+                    integrator=integ, cfn=L.CF_LINEAR, cparams=(0.5, 0.01),
+                    wkind=L.W_DENSE if dense else L.W_SCALAR_MEAN, dense=W if dense else None,
+                    scalar_w=0.0 if dense else 0.05, delay_mode=L.DELAY_SINGLE, single_delay=0)
+    ref = orc.run(inst)
+    np.testing.assert_allclose(tav[0][0], ref["tavg"], rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-6, atol=1e-7)
+
+
+def test_unlinked_subnets_are_independent(orc):
+    rng = np.random.default_rng(2)
+    subs = [(L.G2DO, 4, G2DO_DEFAULT, _x0(rng, 4, 2)), (L.JANSEN_RIT, 3, JR_DEFAULT, _x0(rng, 3, 6, 0.01))]
+    tav, fin = orc.run_multi(subs, [], 0.1, 40, L.HEUN, False)
+    for k, (m, n, p, x0) in enumerate(subs):
+        inst = Instance(model=m, n_nodes=n, n_steps=40, dt=0.1, params=np.array(p, np.float32), x0=x0,
+                        integrator=L.HEUN)
+        ref = orc.run(inst)
+        np.testing.assert_allclose(fin[k][0], ref["final_state"], rtol=1e-6, atol=1e-7)
+
+
+def test_cvar_map_remap_one_step(orc):
+    rng = np.random.default_rng(3)
+    xg, xm = _x0(rng, 5, 2), np.abs(_x0(rng, 4, 2))
+    w = 0.2
+    subs = [(L.G2DO, 5, G2DO_DEFAULT, xg), (L.MPR, 4, MPR_DEFAULT, xm)]
+    link = (0, 1, w, L.CF_LINEAR, [1.0, 0.0], [(0, 1)])
+    _, fin = orc.run_multi(subs, [link], 0.1, 1, L.EULER, False)
+    v = xg[:, 0]
+    g = np.float32(np.float32(np.cumsum(v, dtype=np.float32)[-1] / np.float32(5)) * np.float32(w))
+    for i in range(4):
+        c = np.array([0.0, g], np.float32)      # MPR CVAR = [0, 1]: target var 1 -> coupling slot 1
+        d = orc.dfun(L.MPR, MPR_DEFAULT, xm[i], c)
+        np.testing.assert_array_equal(fin[1][0, i], orc.clamp(L.MPR, (xm[i] + d * np.float32(0.1)).astype(np.float32)))
+
+
+def _spec(subs, links, **kw):
+    from hyburn_amd.multi import Link, Subnet
+    return [Subnet(*s) for s in subs], [Link(*l) for l in links]
+
+
+def test_multi_host_validation():
+    from hyburn_amd.multi import Link, Subnet, run_multi
+    subs = [Subnet(L.G2DO, 4, G2DO_DEFAULT), Subnet(L.MPR, 3, MPR_DEFAULT)]
+    cases = [
+        ([Link(0, 1, 0.1, delay=2)], {}, "delays are not supported"),
+        ([Link(0, 1, 0.1, cvar_map=[(1, 0)])], {}, "cvar_map src index 1 >= src_ncvar 1"),
+        ([Link(1, 0, 0.1, cvar_map=[(0, 1)])], {}, "cvar_map tgt index 1 >= tgt_ncvar 1"),
+        ([Link(0, 5, 0.1)], {}, "src/tgt out of range"),
+        ([Link(0, 1, 0.1)], {"integrator": L.HEUN_STOCH}, "deterministically"),
+    ]
+    for links, kw, msg in cases:
+        with pytest.raises(L.HyburnError, match=msg):
+            run_multi(subs, links, dt=0.1, n_steps=10, **kw)
+    with pytest.raises(L.HyburnError, match="1024 nodes"):
+        run_multi([Subnet(L.G2DO, 600, G2DO_DEFAULT), Subnet(L.G2DO, 600, G2DO_DEFAULT)], [], dt=0.1, n_steps=1)
