@@ -169,6 +169,37 @@ def features_leg(args):
     return res
 
 
+def ring_leg(args):
+    """The reference's generic-engine benchmark network: G2DO -> JR -> WC ring,
+    Scalar all-to-all Linear coupling w = 0.01, zero delays, Heun with
+    hybrid_integrator (Heun for G2DO, Euler for JR and WC), zero initial
+    state, G2DO I swept over linspace(-0.5, 0.5, n_sweep), 1000 steps of
+    dt = 0.1. Published: 360 ms (76 nodes) and 341 ms (164 nodes) per 1024
+    points on an RTX 4090 (6.49e8 and 1.48e9 node-steps/s)."""
+    from hyburn_amd import _lib as L
+    from hyburn_amd.multi import Link, Subnet, run_multi
+    from hyburn_amd.synth import G2DO_DEFAULT, JR_DEFAULT, WC_DEFAULT
+    published = {76: 6.49e8, 164: 1.48e9}
+    res = {}
+    for n in (76, 164):
+        subs = [Subnet(L.G2DO, n, G2DO_DEFAULT), Subnet(L.JANSEN_RIT, n, JR_DEFAULT),
+                Subnet(L.WILSON_COWAN, n, WC_DEFAULT)]
+        links = [Link(0, 1, 0.01), Link(1, 2, 0.01), Link(2, 0, 0.01)]
+        B, T = 1024, 1000
+        vals = (-0.5 + np.arange(B, dtype=np.float32) / np.float32(B - 1)).astype(np.float32)
+        kw = dict(dt=0.1, n_steps=T, integrator=L.HEUN, hybrid_integrator=True, n_sweep=B, sweep=(0, 1, vals))
+        run_multi(subs, links, **kw)
+        t0 = time.perf_counter()
+        r = run_multi(subs, links, **kw)
+        wall = time.perf_counter() - t0
+        ns = B * 3 * n * T
+        res[f"ring_{n}"] = {"node_steps_per_s": ns / (r.kernel_ms * 1e-3), "kernel_ms": r.kernel_ms,
+                            "call_ms": wall * 1e3, "published_rtx4090": published[n],
+                            "vs_published": ns / (r.kernel_ms * 1e-3) / published[n]}
+    res["workload"] = "G2DO->JR->WC ring, 1024 points x 1000 steps, hybrid Heun/Euler, Scalar w=0.01"
+    return res
+
+
 def _fc64(B):
     n = B.shape[0]
     A = B - B.mean(0)
@@ -229,6 +260,8 @@ def main() -> int:
     ap.add_argument("--ragged-tscale", type=float, default=1.0)
     ap.add_argument("--ragged-plain-sample", type=int, default=16)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient legs")
+    ap.add_argument("--no-ring", dest="ring", action="store_false",
+                    help="skip the three-subnetwork ring leg")
     ap.add_argument("--no-features", dest="features", action="store_false",
                     help="skip the FC / FCD / spectral leg")
     ap.add_argument("--features-series", type=int, default=1024)
@@ -334,6 +367,8 @@ def main() -> int:
         line["ragged"] = ragged_leg(args)
     if args.features:
         line["features"] = features_leg(args)
+    if args.ring:
+        line["ring"] = ring_leg(args)
     if not args.no_adjoint:
         import dataclasses
         from hyburn_amd.adjoint import Loss

@@ -12,7 +12,8 @@
 // is set (698-702, dfun.rs:1871), clamps, and adds to its temporal average.
 //
 // One workgroup per sweep point, one thread per node of every subnetwork;
-// the pre values of each projection's sources live in LDS. Supported: zero
+// the pre values of each projection's sources live in LDS, and each Scalar
+// projection's weighted mean is formed once per step. Supported: zero
 // projection delays, deterministic Euler / Heun / RK4 (the host rejects the
 // rest with HB_INVALID_CONFIG).
 #include "hb_internal.h"
@@ -90,6 +91,7 @@ __device__ __forceinline__ void step_model(int model, const StepCtx& S, const fl
 
 __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     extern __shared__ float pv[];   // pre values of every projection's source nodes
+    __shared__ float gmean[kMaxProj * 4];   // w * mean over source nodes, per Scalar projection and channel
     const int t = threadIdx.x, b = blockIdx.x;
     int s = -1;
     for (int q = 0; q < A->n_sub; ++q)
@@ -126,6 +128,20 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
             }
         }
         __syncthreads();
+        // Scalar projections: one thread per (projection, channel) sums the
+        // source pre values in node order, so each mean is formed once
+        if (t < A->n_proj * 4) {
+            const Proj& P = A->proj[t >> 2];
+            const int c = t & 3;
+            if (P.wkind == HB_W_SCALAR_MEAN && c < P.np) {
+                const int ns = A->sub[P.src].n, np = P.np;
+                const float* src = pv + P.lds_off;
+                float sum = 0.0f;
+                for (int j = 0; j < ns; ++j) sum += src[j * np + c];
+                gmean[t] = (sum / (float)ns) * P.w;
+            }
+        }
+        __syncthreads();
         if (live) {
             float cpl[2] = {0.0f, 0.0f};
             for (int p = 0; p < A->n_proj; ++p) {
@@ -135,11 +151,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                 const float* src = pv + P.lds_off;
                 float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
                 if (P.wkind == HB_W_SCALAR_MEAN) {
-                    for (int c = 0; c < np; ++c) {
-                        float sum = 0.0f;
-                        for (int j = 0; j < ns; ++j) sum += src[j * np + c];
-                        g[c] = (sum / (float)ns) * P.w;
-                    }
+                    for (int c = 0; c < np; ++c) g[c] = gmean[p * 4 + c];
                 } else {
                     const float* Wr = P.W + (size_t)i * ns;
                     for (int j = 0; j < ns; ++j) {
