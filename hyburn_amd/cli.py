@@ -8,7 +8,7 @@ Mirrors the reference command line (src/cli.rs:16-60, 785-842, 894-974, 1040-110
   state_final_sub0.npy [nvar, nnodes, nmodes] and state_traj.npy
   [n_steps, nvar, nnodes, nmodes], var-major as the reference stores them;
 * with --sweep: a batched sweep (BatchHybridEngine semantics) over the sweep
-  file's parameter values, writing tavg_sub0.npy with the reference's header
+  file's parameter values, writing tavg_sub{i}.npy with the reference's header
   [n_sweep, nvar, nnodes] over its node-major [n_sweep, nnodes, nvar] data.
 
 n_steps = (sim_length / dt) truncated in f64, as the reference computes it.
@@ -53,11 +53,13 @@ def run(config: str, output: str, backend: str = "hip", sweep: str | None = None
         eng = BatchHybridEngine.from_config(cfg, len(values), device, noise_seed=noise_seed)
         res = eng.run_sweep(SweepParam(sub_idx, param_idx), values, n_steps)
         eng.close()
-        nnodes, nvar = eng.nnodes, eng.nvar
-        path = os.path.join(output, "tavg_sub0.npy")
-        # cli.rs:958-970 labels the [n_sweep, nnodes, nvar] data as [n_sweep, nvar, nnodes]
-        write_npy_f32(path, res.tavg[0], [res.n_sweep, nvar, nnodes])
-        written["tavg_sub0"] = path
+        for i, t in enumerate(res.tavg):   # one file per subnetwork (cli.rs:958-970)
+            sub = cfg.subnetworks[i]
+            nnodes, nvar = sub.nnodes, L.MODEL_INFO[L.MODEL_NAMES[sub.model]][0]
+            path = os.path.join(output, f"tavg_sub{i}.npy")
+            # cli.rs:958-970 labels the [n_sweep, nnodes, nvar] data as [n_sweep, nvar, nnodes]
+            write_npy_f32(path, t, [res.n_sweep, nvar, nnodes])
+            written[f"tavg_sub{i}"] = path
     else:
         # HybridEngine reads an inline initial state var-major [nvar][nnodes]
         # (construction.rs:557-565); the batch engine node-major (engine.rs:161-170)

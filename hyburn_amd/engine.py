@@ -301,20 +301,75 @@ def _load_initial_state(sub: SubnetworkConfig, nv: int, base_dir: str) -> np.nda
     return np.asarray(s, np.float32).reshape(sub.nnodes, nv)
 
 
+def _needs_multi(cfg: SimConfig) -> bool:
+    """Several subnetworks, several projections, a projection between two
+    subnetworks or a cvar_map other than 0:0 take the multi-subnetwork path."""
+    return (len(cfg.subnetworks) != 1 or len(cfg.projections) > 1
+            or any(p.src != p.tgt or p.cvar_map != "0:0" for p in cfg.projections))
+
+
+def _multi_from_config(cfg: SimConfig):
+    """SimConfig -> (Subnet list, Link list) for hyburn_amd.multi (engine.rs:145-415)."""
+    from .multi import Link, Subnet, parse_cvar_map
+    subs = []
+    for i, s in enumerate(cfg.subnetworks):
+        if s.model not in L.MODEL_NAMES:
+            _err(L.HB_INVALID_CONFIG, f"Unknown model '{s.model}' in subnetwork {i}")
+        if s.nmodes != 1:
+            _err(L.HB_INVALID_CONFIG, "nmodes > 1 is not supported by this backend")
+        m = L.MODEL_NAMES[s.model]
+        subs.append(Subnet(m, s.nnodes, list(s.params), _load_initial_state(s, L.MODEL_INFO[m][0], cfg.base_dir)))
+    links = []
+    for k, p in enumerate(cfg.projections):
+        if not (0 <= p.src < len(subs) and 0 <= p.tgt < len(subs)):
+            _err(L.HB_INVALID_CONFIG, f"projection {k}: src/tgt out of range")
+        if p.coupling_fn not in L.COUPLING_NAMES:
+            _err(L.HB_INVALID_CONFIG, f"Unknown coupling function: {p.coupling_fn}")
+        ns, nt = subs[p.src].n_nodes, subs[p.tgt].n_nodes
+        w = p.weights
+        if isinstance(w, dict):   # CSR by target row, densified (engine.rs:298-311)
+            W = np.zeros((nt, ns), np.float32)
+            ip, ix, dat = w["indptr"], w["indices"], w["data"]
+            for r in range(nt):
+                for e in range(ip[r], ip[r + 1]):
+                    W[r, ix[e]] = dat[e]
+            w = W
+        elif isinstance(w, (list, np.ndarray)):
+            w = np.asarray(w, np.float32).reshape(nt, ns)
+        else:
+            w = float(w)
+        links.append(Link(p.src, p.tgt, w, L.COUPLING_NAMES[p.coupling_fn], list(p.coupling_params),
+                          parse_cvar_map(p.cvar_map), int(p.delays[0]) if p.delays else 0))
+    return subs, links
+
+
 class BatchHybridEngine:
-    """BatchHybridEngine<B> with B = the HIP backend."""
+    """BatchHybridEngine<B> with B = the HIP backend. Configurations with
+    several subnetworks (or projections between subnetworks) run through
+    hb_run_multi; set hybrid_integrator = True for Heun on G2DO and the
+    configured rule elsewhere (engine.rs:698-702)."""
 
     def __init__(self):
         self._h = C.c_void_p()
         self._keep: list = []
+        self._multi = None
+        self.hybrid_integrator = False
 
     @classmethod
     def from_config(cls, base_config: SimConfig, n_sweep: int, device: int = 0, *,
                     engine: int = L.ENGINE_BATCH, delay_mode: int = -1, noise_seed: int = 0,
                     noise_shared: bool = True) -> "BatchHybridEngine":
         cfg = base_config
-        if len(cfg.subnetworks) != 1:
-            _err(L.HB_INVALID_CONFIG, "multiple subnetworks are not supported by this backend")
+        if _needs_multi(cfg):
+            self = cls()
+            if cfg.integrator not in L.INTEGRATOR_NAMES:
+                _err(L.HB_INVALID_CONFIG, f"unknown integrator {cfg.integrator}")
+            if any(m.monitor_type.lower() == "bold" for m in cfg.monitors):
+                _err(L.HB_INVALID_CONFIG, "BOLD monitors are not supported on the multi-subnetwork path")
+            subs, links = _multi_from_config(cfg)
+            self._multi = (subs, links, L.INTEGRATOR_NAMES[cfg.integrator], device)
+            self.n_sweep, self.dt = n_sweep, cfg.dt
+            return self
         sub = cfg.subnetworks[0]
         if sub.model not in L.MODEL_NAMES:
             _err(L.HB_INVALID_CONFIG, f"Unknown model '{sub.model}'")
@@ -344,8 +399,6 @@ class BatchHybridEngine:
             if len(cfg.projections) > 1:
                 _err(L.HB_INVALID_CONFIG, "at most one (self) projection is supported by this backend")
             p = cfg.projections[0]
-            if p.cvar_map != "0:0":
-                _err(L.HB_INVALID_CONFIG, "cvar_map other than 0:0 is not supported by this backend")
             if p.coupling_fn not in L.COUPLING_NAMES:
                 _err(L.HB_INVALID_CONFIG, f"Unknown coupling function: {p.coupling_fn}")
             Pj = L.HbProjection()
@@ -380,6 +433,7 @@ class BatchHybridEngine:
             c.bold_period, c.bold_tr_s = int(bp), float(m.tr if m.tr is not None else 2.0)
         c.engine, c.delay_mode, c.noise_seed, c.noise_shared = engine, delay_mode, noise_seed, int(noise_shared)
         self._cfg = c
+        self._pycfg, self._device = cfg, device
         L.check(L.lib().hb_engine_create(C.byref(c), n_sweep, device, C.byref(self._h)))
         return self
 
@@ -406,15 +460,53 @@ class BatchHybridEngine:
                                      int(n_steps), flags, C.byref(r)))
         return self._result(r, v.tolist(), traj)
 
+    def _run_multi(self, param: SweepParam, values, n_steps: int) -> BatchSweepResult:
+        from .multi import run_multi
+        subs, links, integ, device = self._multi
+        v = np.ascontiguousarray(values, np.float32)
+        if len(v) != self.n_sweep:
+            _err(L.HB_INVALID_CONFIG, "number of sweep values must equal n_sweep")
+        if not 0 <= param.sub_idx < len(subs):
+            _err(L.HB_INVALID_CONFIG, "subnetwork index out of range")
+        t0 = time.perf_counter()
+        r = run_multi(subs, links, dt=self.dt, n_steps=int(n_steps), integrator=integ,
+                      hybrid_integrator=self.hybrid_integrator, n_sweep=self.n_sweep,
+                      sweep=(param.sub_idx, param.param_idx, v), device=device)
+        ms = (time.perf_counter() - t0) * 1e3
+        return BatchSweepResult(param_values=v.tolist(), tavg=[t.ravel() for t in r.tavg],
+                                final_states=[f.ravel() for f in r.final_states], trajectories=None, bold=None,
+                                elapsed_ms=ms, n_sweep=self.n_sweep, kernel_ms=r.kernel_ms)
+
+    def _hybrid_changes_rule(self) -> bool:
+        # hybrid_integrator: Heun for G2DO, else the configured integrator with
+        # a configured Heun read as Euler (engine.rs:698-702)
+        if not self.hybrid_integrator or self._multi is not None:
+            return False
+        heun = self._cfg.integrator == L.HEUN
+        return (self.model == L.G2DO) != heun
+
     def run_sweep(self, param: SweepParam, param_values, n_steps: int, exact: bool = False) -> BatchSweepResult:
+        if self._multi is None and self._hybrid_changes_rule():
+            cfg = self._pycfg
+            if any(m.monitor_type.lower() == "bold" for m in cfg.monitors) or self._cfg.integrator not in (
+                    L.EULER, L.HEUN, L.RK4):
+                _err(L.HB_INVALID_CONFIG, "hybrid_integrator needs a deterministic run without BOLD here")
+            subs, links = _multi_from_config(cfg)
+            self._multi = (subs, links, self._cfg.integrator, self._device)
+        if self._multi is not None:
+            return self._run_multi(param, param_values, n_steps)
         return self._run(param, param_values, n_steps, False, exact)
 
     def run_sweep_with_trajectory(self, param: SweepParam, param_values, n_steps: int,
                                   exact: bool = False) -> BatchSweepResult:
+        if self._multi is not None:
+            _err(L.HB_INVALID_CONFIG, "trajectories are not recorded on the multi-subnetwork path")
         return self._run(param, param_values, n_steps, True, exact)
 
     def run_sweep_multi(self, spec: SweepSpec, n_steps: int, exact: bool = False,
                         trajectory: bool = False) -> BatchSweepResult:
+        if self._multi is not None:
+            _err(L.HB_INVALID_CONFIG, "Cartesian sweeps are not supported on the multi-subnetwork path")
         arr = (L.HbSweepPoint * len(spec.points))()
         keep = []
         for i, p in enumerate(spec.points):

@@ -85,3 +85,52 @@ def test_sweep_run_writes_tavg_with_reference_header(gpu, tmp_path):
     r = eng.run_sweep(SweepParam(0, 1), [0.0, 0.1, 0.2], 50)
     eng.close()
     np.testing.assert_array_equal(t.ravel(), r.tavg[0])
+
+
+SIM2 = """
+sim_length = 5.0
+dt = 0.1
+integrator = "heun"
+[network]
+[[network.subnetworks]]
+model = "Generic2dOscillator"
+nnodes = 4
+params = [1, 0, -2, -10, 0, 0.02, 3, 1, 0, 1, 1, 1]
+initial_state = [0, 0, 0, 0, 0, 0, 0, 0]
+[[network.subnetworks]]
+model = "WilsonCowan"
+nnodes = 4
+params = [12, 4, 13, 11, 10, 10, 1.2, 2.8, 1, 0, 1, 4, 1, 0, 1, 1, 1, 1, 0, 0, 1, 1]
+initial_state = [0, 0, 0, 0, 0, 0, 0, 0]
+[[network.projections]]
+src = 0
+tgt = 1
+weights = 0.01
+coupling_fn = "Linear"
+coupling_params = [1.0]
+[[network.projections]]
+src = 1
+tgt = 0
+weights = 0.01
+coupling_fn = "Linear"
+coupling_params = [1.0]
+"""
+
+
+@pytest.mark.gpu
+def test_sweep_run_writes_one_tavg_per_subnetwork(gpu, tmp_path):
+    # a two-subnetwork config runs on the multi-subnetwork path; cli.rs:958-970
+    # writes tavg_sub{i}.npy for every subnetwork (Scalar weights need equal
+    # node counts, as the config validation requires)
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig, SweepParam
+    cfg = tmp_path / "sim2.toml"
+    cfg.write_text(SIM2)
+    sw = tmp_path / "sweep.toml"
+    sw.write_text(SWEEP)
+    out = tmp_path / "sw2"
+    assert main(["run", "-c", str(cfg), "-o", str(out), "--sweep", str(sw)]) == 0
+    assert b"'shape': (3, 2, 4)" in open(out / "tavg_sub0.npy", "rb").read()
+    assert b"'shape': (3, 2, 4)" in open(out / "tavg_sub1.npy", "rb").read()
+    eng = BatchHybridEngine.from_config(SimConfig.from_file(str(cfg)), 3)
+    r = eng.run_sweep(SweepParam(0, 1), [0.0, 0.1, 0.2], 50)
+    np.testing.assert_array_equal(np.load(out / "tavg_sub1.npy").ravel(), r.tavg[1])

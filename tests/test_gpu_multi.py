@@ -98,3 +98,26 @@ def test_ring_workload_full_length(gpu, orc):
             np.testing.assert_allclose(res.tavg[k][b], tav[k][0], rtol=2e-3, atol=2e-4)
             np.testing.assert_allclose(res.final_states[k][b], fin[k][0], rtol=2e-3, atol=2e-4)
     assert res.kernel_ms > 0.0
+
+
+def test_engine_api_multi_config(gpu, orc):
+    # the reference's benchmark config (examples/bench/hyburn_bench_generic_cuda.rs:118-135) at
+    # 6 nodes, through BatchHybridEngine.from_config / run_sweep with hybrid_integrator
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig, SweepParam
+    n = 6
+    d = {"sim_length": 100.0, "dt": 0.1, "integrator": "Heun", "network": {
+        "subnetworks": [
+            {"model": "Generic2dOscillator", "nnodes": n, "params": G2DO_DEFAULT.tolist(), "initial_state": [0.0] * (2 * n)},
+            {"model": "JansenRit", "nnodes": n, "params": JR_DEFAULT.tolist(), "initial_state": [0.0] * (6 * n)},
+            {"model": "WilsonCowan", "nnodes": n, "params": WC_DEFAULT.tolist(), "initial_state": [0.0] * (2 * n)}],
+        "projections": [{"src": a, "tgt": b, "weights": 0.01, "coupling_fn": "Linear", "coupling_params": [1.0],
+                         "cvar_map": "0:0", "delays": [0]} for a, b in ((0, 1), (1, 2), (2, 0))]}}
+    eng = BatchHybridEngine.from_config(SimConfig.from_dict(d), 4)
+    eng.hybrid_integrator = True
+    vals = np.linspace(-0.5, 0.5, 4).astype(np.float32)
+    r = eng.run_sweep(SweepParam(sub_idx=0, param_idx=1), vals, 300)
+    subs, links = _ring(n)
+    tav, fin = orc.run_multi(subs, links, 0.1, 300, L.HEUN, True, sweep=(0, 1, vals), n_sweep=4)
+    for k in range(3):
+        np.testing.assert_allclose(r.tavg[k], tav[k].ravel(), rtol=RTOL, atol=ATOL)
+        np.testing.assert_allclose(r.final_states[k], fin[k].ravel(), rtol=RTOL, atol=ATOL)

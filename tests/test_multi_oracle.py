@@ -86,3 +86,17 @@ def test_multi_host_validation():
             run_multi(subs, links, dt=0.1, n_steps=10, **kw)
     with pytest.raises(L.HyburnError, match="1024 nodes"):
         run_multi([Subnet(L.G2DO, 600, G2DO_DEFAULT), Subnet(L.G2DO, 600, G2DO_DEFAULT)], [], dt=0.1, n_steps=1)
+
+
+def test_engine_routes_multi_configs():
+    # several subnetworks go through hb_run_multi; validation happens before any device work
+    from hyburn_amd.engine import BatchHybridEngine, MonitorConfig, ProjectionConfig, SimConfig, SubnetworkConfig
+    subs = [SubnetworkConfig(model="Generic2dOscillator", nnodes=3, params=list(G2DO_DEFAULT), initial_state=[0.0] * 6),
+            SubnetworkConfig(model="WilsonCowan", nnodes=2, params=list(WC_DEFAULT), initial_state=[0.0] * 4)]
+    projs = [ProjectionConfig(src=0, tgt=1, weights=0.01, delays=[0], coupling_fn="Linear", coupling_params=[1.0])]
+    cfg = SimConfig(sim_length=10.0, dt=0.1, subnetworks=subs, projections=projs, integrator="heun")
+    eng = BatchHybridEngine.from_config(cfg, 4)
+    assert eng._multi is not None and len(eng._multi[0]) == 2
+    cfg.monitors = [MonitorConfig(monitor_type="Bold")]
+    with pytest.raises(L.HyburnError, match="BOLD"):
+        BatchHybridEngine.from_config(cfg, 4)
