@@ -13,9 +13,10 @@
 //
 // One workgroup per sweep point, one thread per node of every subnetwork;
 // the pre values of each projection's sources live in LDS, and each Scalar
-// projection's weighted mean is formed once per step. Supported: zero
-// projection delays, deterministic Euler / Heun / RK4 (the host rejects the
-// rest with HB_INVALID_CONFIG).
+// projection's weighted mean is formed once per step. A delayed projection
+// reads each source node's own history (global memory, written and read by
+// the same thread). Supported: deterministic Euler / Heun / RK4 (the host
+// rejects noise with HB_INVALID_CONFIG).
 #include "hb_internal.h"
 
 #include <algorithm>
@@ -37,7 +38,7 @@ struct Sub {
 };
 
 struct Proj {
-    int32_t src, tgt, wkind, cfn, np, nc, classic, pass_all, n_map, lds_off;
+    int32_t src, tgt, wkind, cfn, np, nc, classic, pass_all, n_map, lds_off, delay;
     int32_t map_s[4], map_c[4];
     float w, cc[8];
     const float* W;        // [n_tgt][n_src] (HB_W_DENSE) or null
@@ -47,8 +48,9 @@ struct Proj {
 struct Args {
     Sub sub[kMaxSub];
     Proj proj[kMaxProj];
-    int32_t n_sub, n_proj, n_tot, n_steps;
+    int32_t n_sub, n_proj, n_tot, n_steps, H;   // H: history horizon (0: no delayed projection)
     float dt, dth, dt6, inv_nsteps;
+    float* hist;           // [B][n_tot][H][2] each node's coupling variables of the last H steps
     const float* sweep;    // [B] swept values, or null
     const float* x0;       // [n_tot][kNV]
     float* tavg;           // [B][n_tot][kNV]
@@ -118,12 +120,35 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     S.plain = false;
     S.sw = live && A->sub[s].sw != 0;
     S.sv = A->sweep ? A->sweep[b] : 0.0f;
+    const int H = A->H;
+    float* hist = (live && H > 0) ? A->hist + ((size_t)b * A->n_tot + t) * H * 2 : nullptr;
     for (int step = 0; step < A->n_steps; ++step) {
+        // the history window gains the current state (engine.rs:541-551); each
+        // node keeps its own first two variables by real step, slot step % H
+        if (hist) {
+            hist[(step % H) * 2] = x[0];
+            hist[(step % H) * 2 + 1] = x[1];
+        }
         for (int p = 0; p < A->n_proj; ++p) {
             const Proj& P = A->proj[p];
             if (live && s == P.src) {
-                float o[4];
-                pre_rt(P, x, o);
+                float o[4], xd[2];
+                const float* xs = x;
+                if (P.delay > 0 && step > 0) {
+                    // engine.rs:565-589: window slot (step - d + H) % H holds the
+                    // state of step step - H + 1 + slot; zeros before step 0 and
+                    // while d > step
+                    xd[0] = xd[1] = 0.0f;
+                    if (P.delay <= step) {
+                        const int ts = step - H + 1 + (step - P.delay + H) % H;
+                        if (ts >= 0) {
+                            xd[0] = hist[(ts % H) * 2];
+                            xd[1] = hist[(ts % H) * 2 + 1];
+                        }
+                    }
+                    xs = xd;
+                }
+                pre_rt(P, xs, o);
                 for (int c = 0; c < P.np; ++c) pv[P.lds_off + i * P.np + c] = o[c];
             }
         }
@@ -277,7 +302,7 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         const hb_multi_projection& R = Q.projections[p];
         const std::string tag = "projection " + std::to_string(p);
         if (R.src < 0 || R.src >= Q.n_subnets || R.tgt < 0 || R.tgt >= Q.n_subnets) return bad(tag + ": src/tgt out of range");
-        if (R.delay != 0) return bad(tag + ": projection delays are not supported on the multi-subnetwork path");
+        if (R.delay < 0) return bad(tag + ": negative delay");
         if (R.wkind != HB_W_SCALAR_MEAN && R.wkind != HB_W_DENSE) return bad(tag + ": weights must be Scalar or Dense");
         if (R.wkind == HB_W_DENSE && !R.dense_w) return bad(tag + ": dense weights missing");
         if (R.cfn < HB_CF_LINEAR || R.cfn > HB_CF_PRESIGMOIDAL) return bad(tag + ": unknown coupling function");
@@ -342,9 +367,14 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
             P.map_c[m] = cvar_position(mt, R.map_tgt[m]);
         }
         P.lds_off = lds;
+        P.delay = R.delay;
         lds += ns * P.np;
     }
     if (lds > kMaxLdsFloats) return bad("projection sources exceed 64 KiB of pre values");
+    int max_delay = std::max(0, Q.max_delay);
+    for (int p = 0; p < Q.n_projections; ++p) max_delay = std::max(max_delay, Q.projections[p].delay);
+    if (max_delay > 1000000) return bad("delays above 10^6 steps");
+    const int H = max_delay > 0 ? max_delay + 1 : 0;   // engine.rs:253-257
     hipError_t e = hipSetDevice(Q.device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     for (int p = 0; p < Q.n_projections; ++p) {
@@ -384,11 +414,15 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     if (e == hipSuccess) e = B.alloc(&dfin, nout);
     if (e == hipSuccess) e = B.alloc(&dA, 1);
     if (e == hipSuccess && Q.sweep_sub >= 0) e = B.alloc(&dsw, (size_t)Q.n_sweep);
+    float* dhist = nullptr;
+    if (e == hipSuccess && H > 0) e = B.alloc(&dhist, (size_t)Q.n_sweep * n_tot * H * 2);
     if (e == hipSuccess) e = hipMemcpy(dx0, x0.data(), sizeof(float) * x0.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && dsw) e = hipMemcpy(dsw, Q.sweep_values, sizeof(float) * Q.n_sweep, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "multi-subnetwork upload");
     A.x0 = dx0;
     A.sweep = dsw;
+    A.hist = dhist;
+    A.H = H;
     A.tavg = dtavg;
     A.fin = dfin;
     e = hipMemcpy(dA, &A, sizeof(A), hipMemcpyHostToDevice);

@@ -353,6 +353,7 @@ class BatchHybridEngine:
         self._h = C.c_void_p()
         self._keep: list = []
         self._multi = None
+        self._max_delay = 0
         self.hybrid_integrator = False
 
     @classmethod
@@ -368,6 +369,7 @@ class BatchHybridEngine:
                 _err(L.HB_INVALID_CONFIG, "BOLD monitors are not supported on the multi-subnetwork path")
             subs, links = _multi_from_config(cfg)
             self._multi = (subs, links, L.INTEGRATOR_NAMES[cfg.integrator], device)
+            self._max_delay = max([max(p.delays) for p in cfg.projections if p.delays] or [0])
             self.n_sweep, self.dt = n_sweep, cfg.dt
             return self
         sub = cfg.subnetworks[0]
@@ -471,7 +473,7 @@ class BatchHybridEngine:
         t0 = time.perf_counter()
         r = run_multi(subs, links, dt=self.dt, n_steps=int(n_steps), integrator=integ,
                       hybrid_integrator=self.hybrid_integrator, n_sweep=self.n_sweep,
-                      sweep=(param.sub_idx, param.param_idx, v), device=device)
+                      sweep=(param.sub_idx, param.param_idx, v), max_delay=int(self._max_delay), device=device)
         ms = (time.perf_counter() - t0) * 1e3
         return BatchSweepResult(param_values=v.tolist(), tavg=[t.ravel() for t in r.tavg],
                                 final_states=[f.ravel() for f in r.final_states], trajectories=None, bold=None,
@@ -493,6 +495,7 @@ class BatchHybridEngine:
                 _err(L.HB_INVALID_CONFIG, "hybrid_integrator needs a deterministic run without BOLD here")
             subs, links = _multi_from_config(cfg)
             self._multi = (subs, links, self._cfg.integrator, self._device)
+            self._max_delay = max([max(p.delays) for p in cfg.projections if p.delays] or [0])
         if self._multi is not None:
             return self._run_multi(param, param_values, n_steps)
         return self._run(param, param_values, n_steps, False, exact)

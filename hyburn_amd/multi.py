@@ -6,7 +6,8 @@ backed by hb_run_multi.
   Link(src, tgt, weights, coupling_fn, coupling_params, cvar_map, delay)
                                             ProjectionConfig: weights a float
                                             (Scalar: w * mean over source
-                                            nodes) or a [n_tgt][n_src] array
+                                            nodes) or a [n_tgt][n_src] array;
+                                            delay = the first of its delays
   run_multi(subnets, links, dt, n_steps, integrator, hybrid_integrator,
             sweep=(sub_idx, param_idx, values))
       -> MultiResult(tavg, final_states, kernel_ms), one [n_sweep][n_nodes][nvar]
@@ -55,7 +56,9 @@ def parse_cvar_map(s: str) -> list:
 
 def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_steps: int,
               integrator: int = L.HEUN, hybrid_integrator: bool = False, n_sweep: int = 1,
-              sweep: Optional[tuple] = None, device: int = 0) -> MultiResult:
+              sweep: Optional[tuple] = None, max_delay: int = 0, device: int = 0) -> MultiResult:
+    """max_delay: the largest entry of any projection's delay list, which sets
+    the history horizon (engine.rs:253-257); each link's own delay counts too."""
     keep = []
     subs = (L.HbSubnetwork * len(subnets))()
     for k, S in enumerate(subnets):
@@ -90,7 +93,7 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
     spec.n_subnets, spec.subnets = len(subnets), subs
     spec.n_projections, spec.projections = len(links), projs
     spec.dt, spec.integrator, spec.hybrid = dt, integrator, int(bool(hybrid_integrator))
-    spec.n_sweep, spec.n_steps, spec.device = n_sweep, n_steps, device
+    spec.n_sweep, spec.n_steps, spec.device, spec.max_delay = n_sweep, n_steps, device, max_delay
     spec.sweep_sub, spec.sweep_param = -1, 0
     if sweep is not None:
         sub_idx, param_idx, values = sweep

@@ -281,7 +281,7 @@ typedef struct {
     float cparams[8];
     int32_t n_map;            /* cvar_map pairs map_src[k]:map_tgt[k] ("0:0" = one pair 0, 0) */
     int32_t map_src[4], map_tgt[4];
-    int32_t delay;            /* steps; this path supports 0 */
+    int32_t delay;            /* steps: the projection's first delay (engine.rs:407) */
 } hb_multi_projection;
 
 typedef struct {
@@ -295,6 +295,8 @@ typedef struct {
     int32_t n_sweep, n_steps;
     int32_t sweep_sub, sweep_param;   /* SweepParam; sweep_sub = -1: none */
     const float* sweep_values;        /* [n_sweep] */
+    int32_t max_delay;        /* largest delay of any projection's delay list (history horizon - 1,
+                                 engine.rs:253-257); the projections' own delays count too */
     int32_t device;
     float* tavg;              /* out: per subnetwork [n_sweep][n_nodes][nvar], concatenated */
     float* final_state;       /* out: same layout */
@@ -305,8 +307,10 @@ typedef struct {
  * (src/engine/batch_engine/engine.rs:540-823): per step, each projection's
  * coupling from the current source states (pre, Scalar mean or Dense weights,
  * post, cvar_map remap), summed per target, then every subnetwork integrates
- * with that frozen coupling; tavg = sum of states / n_steps. HB_INVALID_CONFIG
- * for delays, noise, unknown models or cvar_map indices out of range
+ * with that frozen coupling; tavg = sum of states / n_steps. A projection
+ * delay d > 0 reads the batch engine's history window of max_delay + 1 states
+ * at slot (step - d + H) % H (engine.rs:541-589), zeros while d > step.
+ * HB_INVALID_CONFIG for noise, unknown models or cvar_map indices out of range
  * (engine.rs:371-388). */
 int hb_run_multi(hb_multi_spec* spec);
 

@@ -313,20 +313,26 @@ def _f32sum(v):
     return np.cumsum(np.asarray(v, np.float32), dtype=np.float32)[-1] if len(v) else np.float32(0.0)
 
 
-def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_sweep=1):
+def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_sweep=1, max_delay=0):
     """The batch engine's multi-subnetwork loop (engine.rs:540-823) in f32, one
     sweep point at a time: coupling of every projection from the current
     states (engine.rs:557-673), then each subnetwork's integration with that
     frozen coupling (676-773), clamp, tavg. subnets: (model, n, params, x0
     [n][nvar]); links: (src, tgt, weights float|[nt][ns], cfn, cparams,
-    cvar_map, delay=0). Returns per subnetwork tavg and final [n_sweep][n][nvar]."""
+    cvar_map, delay=0). Delayed reads use the batch engine's history window
+    literally (engine.rs:253-270, 541-589): H = max delay + 1 states,
+    initially [x0, 0, ..., 0], shifted left and appended each step, read at
+    slot (step - d + H) % H. Returns per subnetwork tavg and final [n_sweep][n][nvar]."""
     from hyburn_amd import _lib as HL
     f32 = np.float32
     S = len(subnets)
     info = [HL.MODEL_INFO[m] for (m, _, _, _) in subnets]
     # Difference preprocessing, engine.rs:325-366
     prep = []
+    H = max([max_delay] + [l[6] if len(l) > 6 else 0 for l in links])
+    H = H + 1 if H > 0 else 0
     for (src, tgt, w, cfn, cp, cmap, *rest) in links:
+        delay = rest[0] if rest else 0
         cp = list(cp)
         ns, nt = subnets[src][1], subnets[tgt][1]
         W = None if np.isscalar(w) else np.array(w, np.float32)
@@ -338,7 +344,7 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                 for i in range(nt):
                     W[i, i] = f32(W[i, i] - _f32sum(W[i]))
                 cfn, cp = HL.CF_LINEAR, [a, 0.0]
-        prep.append((src, tgt, w, W, cfn, cp, list(cmap)))
+        prep.append((src, tgt, w, W, cfn, cp, list(cmap), delay))
     tav = [np.zeros((n_sweep, n, info[k][0]), np.float32) for k, (_, n, _, _) in enumerate(subnets)]
     fin = [np.zeros_like(t) for t in tav]
     dtf, dth, dt6 = f32(dt), f32(f32(dt) * f32(0.5)), f32(f32(dt) / f32(6.0))
@@ -352,13 +358,22 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
             sv = float(values[b])
             if param_idx == HL.SWEEP_INDEX[subnets[sub_idx][0]]:
                 sw[sub_idx] = 1
-        for _ in range(n_steps):
+        hist = [[x[k].copy()] + [np.zeros_like(x[k]) for _ in range(H - 1)] for k in range(S)] if H else None
+        for step in range(n_steps):
+            if H:
+                for k in range(S):
+                    hist[k] = hist[k][1:] + [x[k].copy()]
             cpl = [None] * S
-            for (src, tgt, w, W, cfn, cp, cmap) in prep:
+            for (src, tgt, w, W, cfn, cp, cmap, delay) in prep:
                 ms, mt = subnets[src][0], subnets[tgt][0]
                 ncs, nct = info[src][1], info[tgt][1]
                 ns, nt = subnets[src][1], subnets[tgt][1]
-                xs = x[src][:, :ncs]
+                if delay == 0 or step == 0:
+                    xs = x[src][:, :ncs]
+                elif delay <= step:
+                    xs = hist[src][(step - delay + H) % H][:, :ncs]
+                else:
+                    xs = np.zeros((ns, ncs), np.float32)
                 xt = x[tgt][:, :ncs] if nct == ncs else np.zeros((nt, ncs), np.float32)
                 if W is None:
                     if cfn not in _IDENTITY_PRE:

@@ -121,3 +121,21 @@ def test_engine_api_multi_config(gpu, orc):
     for k in range(3):
         np.testing.assert_allclose(r.tavg[k], tav[k].ravel(), rtol=RTOL, atol=ATOL)
         np.testing.assert_allclose(r.final_states[k], fin[k].ravel(), rtol=RTOL, atol=ATOL)
+
+
+def test_delayed_projections(gpu, orc):
+    # the ring with delays 5, 3, 7 steps plus a dense delayed link; the history
+    # horizon comes from the largest delay of any list (here 11 > the links' own)
+    rng = np.random.default_rng(11)
+    n = 6
+    subs = [(L.G2DO, n, G2DO_DEFAULT, _x0(rng, n, 2)), (L.JANSEN_RIT, n, JR_DEFAULT, _x0(rng, n, 6, 0.01)),
+            (L.WILSON_COWAN, n, WC_DEFAULT, np.abs(_x0(rng, n, 2, 0.1)))]
+    W = (rng.uniform(size=(n, n)) * 0.2).astype(np.float32)
+    links = [(0, 1, 0.05, L.CF_LINEAR, [1.0], [(0, 0)], 5), (1, 2, 0.05, L.CF_LINEAR, [1.0], [(0, 0)], 3),
+             (2, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 0)], 7), (0, 0, W, L.CF_LINEAR, [0.3, 0.0], [(0, 0)], 2)]
+    for md in (0, 11):
+        res = run_multi([Subnet(m, k, p, x0) for (m, k, p, x0) in subs],
+                        [Link(s, t, w, cfn, cp, cm, dl) for (s, t, w, cfn, cp, cm, dl) in links],
+                        dt=0.1, n_steps=150, integrator=L.RK4, n_sweep=2, max_delay=md)
+        tav, fin = orc.run_multi(subs, links, 0.1, 150, L.RK4, False, n_sweep=2, max_delay=md)
+        _check(res, tav, fin)

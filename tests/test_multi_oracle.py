@@ -75,7 +75,7 @@ def test_multi_host_validation():
     from hyburn_amd.multi import Link, Subnet, run_multi
     subs = [Subnet(L.G2DO, 4, G2DO_DEFAULT), Subnet(L.MPR, 3, MPR_DEFAULT)]
     cases = [
-        ([Link(0, 1, 0.1, delay=2)], {}, "delays are not supported"),
+        ([Link(0, 1, 0.1, delay=-2)], {}, "negative delay"),
         ([Link(0, 1, 0.1, cvar_map=[(1, 0)])], {}, "cvar_map src index 1 >= src_ncvar 1"),
         ([Link(1, 0, 0.1, cvar_map=[(0, 1)])], {}, "cvar_map tgt index 1 >= tgt_ncvar 1"),
         ([Link(0, 5, 0.1)], {}, "src/tgt out of range"),
@@ -100,3 +100,23 @@ def test_engine_routes_multi_configs():
     cfg.monitors = [MonitorConfig(monitor_type="Bold")]
     with pytest.raises(L.HyburnError, match="BOLD"):
         BatchHybridEngine.from_config(cfg, 4)
+
+
+@pytest.mark.parametrize("d,dense", [(1, False), (4, True), (9, False)])
+def test_delayed_self_projection_matches_batch_compat(orc, d, dense):
+    # one subnetwork with a delayed self-projection is the single-network
+    # problem with the batch engine's history window (DELAY_BATCH_COMPAT in
+    # the C restatement, pinned by the slot-table tests); horizon = d + 1
+    rng = np.random.default_rng(d)
+    n, T = 5, 40
+    x0 = _x0(rng, n, 2)
+    W = (rng.uniform(size=(n, n)) * 0.3).astype(np.float32)
+    link = (0, 0, W if dense else 0.2, L.CF_LINEAR, [0.8, 0.0], [(0, 0)], d)
+    tav, fin = orc.run_multi([(L.G2DO, n, G2DO_DEFAULT, x0)], [link], 0.1, T, L.HEUN, False)
+    inst = Instance(model=L.G2DO, n_nodes=n, n_steps=T, dt=0.1, params=np.array(G2DO_DEFAULT, np.float32), x0=x0,
+                    integrator=L.HEUN, cfn=L.CF_LINEAR, cparams=(0.8, 0.0),
+                    wkind=L.W_DENSE if dense else L.W_SCALAR_MEAN, dense=W if dense else None,
+                    scalar_w=0.0 if dense else 0.2, delay_mode=L.DELAY_BATCH_COMPAT, single_delay=d)
+    ref = orc.run(inst)
+    np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(tav[0][0], ref["tavg"], rtol=1e-6, atol=1e-7)
