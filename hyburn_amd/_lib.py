@@ -136,8 +136,8 @@ class HbMultiSpec(C.Structure):
     _fields_ = [("n_subnets", C.c_int32), ("subnets", C.POINTER(HbSubnetwork)), ("n_projections", C.c_int32),
                 ("projections", C.POINTER(HbMultiProjection)), ("dt", C.c_float), ("integrator", C.c_int32),
                 ("hybrid", C.c_int32), ("n_sweep", C.c_int32), ("n_steps", C.c_int32), ("sweep_sub", C.c_int32),
-                ("sweep_param", C.c_int32), ("sweep_values", fp), ("max_delay", C.c_int32), ("device", C.c_int32),
-                ("tavg", fp),
+                ("sweep_param", C.c_int32), ("sweep_values", fp), ("max_delay", C.c_int32), ("n_nsig", C.c_int32),
+                ("nsig", fp), ("noise", fp), ("device", C.c_int32), ("tavg", fp),
                 ("final_state", fp), ("kernel_ms", C.c_double)]
 
 

@@ -15,8 +15,9 @@
 // the pre values of each projection's sources live in LDS, and each Scalar
 // projection's weighted mean is formed once per step. A delayed projection
 // reads each source node's own history (global memory, written and read by
-// the same thread). Supported: deterministic Euler / Heun / RK4 (the host
-// rejects noise with HB_INVALID_CONFIG).
+// the same thread). Stochastic integrators take one raw N(0,1) draw per
+// node, variable and step, shared by every sweep point as the batch engine
+// broadcasts it.
 #include "hb_internal.h"
 
 #include <algorithm>
@@ -33,8 +34,9 @@ constexpr int kMaxSub = 4, kMaxProj = 8, kNV = 6, kMaxNodes = 1024;
 constexpr int kMaxLdsFloats = 16384;   // 64 KiB of pre values
 
 struct Sub {
-    int32_t model, n, off, nv, integ, sw;
+    int32_t model, n, off, nv, integ, sw, stoch;
     float mc[kMC];
+    float nscale[kNV];     // sqrt(2 nsig_v) * sqrt(dt)
 };
 
 struct Proj {
@@ -51,6 +53,7 @@ struct Args {
     int32_t n_sub, n_proj, n_tot, n_steps, H;   // H: history horizon (0: no delayed projection)
     float dt, dth, dt6, inv_nsteps;
     float* hist;           // [B][n_tot][H][2] each node's coupling variables of the last H steps
+    const float* noise;    // [n_steps][n_tot][kNV] raw N(0,1), shared by the sweep points, or null
     const float* sweep;    // [B] swept values, or null
     const float* x0;       // [n_tot][kNV]
     float* tavg;           // [B][n_tot][kNV]
@@ -105,7 +108,6 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
 #pragma unroll
     for (int k = 0; k < kMC; ++k) mc[k] = live ? A->sub[s].mc[k] : 0.0f;
     float x[kNV], acc[kNV];
-    const float zero[kNV] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int v = 0; v < kNV; ++v) {
         x[v] = (live && v < nv) ? A->x0[(size_t)t * kNV + v] : 0.0f;
@@ -116,8 +118,11 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     S.dth = A->dth;
     S.dt6 = A->dt6;
     S.integ = live ? A->sub[s].integ : HB_EULER;
-    S.stoch = false;
+    S.stoch = live && A->sub[s].stoch != 0;
     S.plain = false;
+    float nscale[kNV];
+#pragma unroll
+    for (int v = 0; v < kNV; ++v) nscale[v] = live ? A->sub[s].nscale[v] : 0.0f;
     S.sw = live && A->sub[s].sw != 0;
     S.sv = A->sweep ? A->sweep[b] : 0.0f;
     const int H = A->H;
@@ -202,7 +207,14 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                     for (int m = 0; m < P.n_map; ++m) cpl[P.map_c[m]] += out[P.map_s[m]];
                 }
             }
-            step_model(model, S, mc, cpl, zero, x);
+            float z[kNV];
+#pragma unroll
+            for (int v = 0; v < kNV; ++v) z[v] = 0.0f;
+            if (S.stoch) {
+                const float* raw = A->noise + ((size_t)step * A->n_tot + t) * kNV;
+                for (int v = 0; v < nv; ++v) z[v] = raw[v] * nscale[v];   // integrator.rs:183-191
+            }
+            step_model(model, S, mc, cpl, z, x);
 #pragma unroll
             for (int v = 0; v < kNV; ++v) acc[v] += x[v];
         }
@@ -269,8 +281,7 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     if (Q.n_projections < 0 || Q.n_projections > kMaxProj || (Q.n_projections > 0 && !Q.projections))
         return bad("n_projections must be in [0, 8]");
     if (Q.n_sweep < 1 || Q.n_steps < 0) return bad("n_sweep must be >= 1 and n_steps >= 0");
-    if (Q.integrator != HB_EULER && Q.integrator != HB_HEUN && Q.integrator != HB_RK4)
-        return bad("the multi-subnetwork path integrates deterministically (Euler, Heun, RK4)");
+    if (Q.integrator < HB_EULER || Q.integrator > HB_RK4_STOCH) return bad("unknown integrator");
     if (Q.sweep_sub < -1 || Q.sweep_sub >= Q.n_subnets) return bad("sweep subnetwork index out of range");
     if (Q.sweep_sub >= 0 && !Q.sweep_values) return bad("sweep values missing");
     if (!Q.tavg || !Q.final_state) return bad("null output pointer");
@@ -286,10 +297,19 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         S.n = U.n_nodes;
         S.off = n_tot;
         S.nv = model_nvar(U.model);
-        // engine.rs:698-702: hybrid -> Heun for G2DO; otherwise the configured
-        // integrator, where a configured Heun falls through to Euler
+        // engine.rs:698-773: hybrid -> deterministic Heun for G2DO; otherwise the
+        // configured integrator, where a configured (deterministic) Heun falls
+        // through to Euler
         if (Q.hybrid) S.integ = U.model == HB_G2DO ? HB_HEUN : (Q.integrator == HB_HEUN ? HB_EULER : Q.integrator);
         else S.integ = Q.integrator;
+        S.stoch = (S.integ == HB_EULER_STOCH || S.integ == HB_HEUN_STOCH || S.integ == HB_RK4_STOCH) ? 1 : 0;
+        if (S.stoch) {
+            if (!Q.nsig || !Q.noise) return bad("stochastic integration needs nsig and the raw noise");
+            if (Q.n_nsig < S.nv)
+                return bad("nsig has " + std::to_string(Q.n_nsig) + " entries, subnetwork " + std::to_string(q) +
+                           " integrates " + std::to_string(S.nv) + " noisy variables");
+            for (int v = 0; v < S.nv; ++v) S.nscale[v] = sqrtf(2.0f * Q.nsig[v]) * sqrtf(Q.dt);
+        }
         S.sw = (q == Q.sweep_sub && Q.sweep_param == model_sweep_index(U.model)) ? 1 : 0;
         model_consts(U.model, U.params, S.mc);
         n_tot += U.n_nodes;
@@ -416,6 +436,22 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     if (e == hipSuccess && Q.sweep_sub >= 0) e = B.alloc(&dsw, (size_t)Q.n_sweep);
     float* dhist = nullptr;
     if (e == hipSuccess && H > 0) e = B.alloc(&dhist, (size_t)Q.n_sweep * n_tot * H * 2);
+    bool any_stoch = false;
+    for (int q = 0; q < Q.n_subnets; ++q) any_stoch = any_stoch || A.sub[q].stoch;
+    float* dnoise = nullptr;
+    if (e == hipSuccess && any_stoch) {
+        // per subnetwork [n_steps][n][nv] -> [n_steps][n_tot][kNV]
+        std::vector<float> zn((size_t)Q.n_steps * n_tot * kNV, 0.0f);
+        size_t o = 0;
+        for (int q = 0; q < Q.n_subnets; ++q) {
+            const Sub& S = A.sub[q];
+            for (int st = 0; st < Q.n_steps; ++st)
+                for (int r = 0; r < S.n; ++r)
+                    for (int v = 0; v < S.nv; ++v, ++o) zn[((size_t)st * n_tot + S.off + r) * kNV + v] = Q.noise[o];
+        }
+        e = B.alloc(&dnoise, zn.size());
+        if (e == hipSuccess) e = hipMemcpy(dnoise, zn.data(), sizeof(float) * zn.size(), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = hipMemcpy(dx0, x0.data(), sizeof(float) * x0.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && dsw) e = hipMemcpy(dsw, Q.sweep_values, sizeof(float) * Q.n_sweep, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "multi-subnetwork upload");
@@ -423,6 +459,7 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     A.sweep = dsw;
     A.hist = dhist;
     A.H = H;
+    A.noise = dnoise;
     A.tavg = dtavg;
     A.fin = dfin;
     e = hipMemcpy(dA, &A, sizeof(A), hipMemcpyHostToDevice);

@@ -354,6 +354,7 @@ class BatchHybridEngine:
         self._keep: list = []
         self._multi = None
         self._max_delay = 0
+        self._nsig, self._noise_seed = 0.0, 0
         self.hybrid_integrator = False
 
     @classmethod
@@ -370,6 +371,7 @@ class BatchHybridEngine:
             subs, links = _multi_from_config(cfg)
             self._multi = (subs, links, L.INTEGRATOR_NAMES[cfg.integrator], device)
             self._max_delay = max([max(p.delays) for p in cfg.projections if p.delays] or [0])
+            self._nsig, self._noise_seed = cfg.nsig, noise_seed
             self.n_sweep, self.dt = n_sweep, cfg.dt
             return self
         sub = cfg.subnetworks[0]
@@ -471,7 +473,8 @@ class BatchHybridEngine:
         if not 0 <= param.sub_idx < len(subs):
             _err(L.HB_INVALID_CONFIG, "subnetwork index out of range")
         t0 = time.perf_counter()
-        r = run_multi(subs, links, dt=self.dt, n_steps=int(n_steps), integrator=integ,
+        r = run_multi(subs, links, dt=self.dt, n_steps=int(n_steps), integrator=integ, nsig=self._nsig,
+                      noise_seed=self._noise_seed,
                       hybrid_integrator=self.hybrid_integrator, n_sweep=self.n_sweep,
                       sweep=(param.sub_idx, param.param_idx, v), max_delay=int(self._max_delay), device=device)
         ms = (time.perf_counter() - t0) * 1e3
@@ -490,12 +493,12 @@ class BatchHybridEngine:
     def run_sweep(self, param: SweepParam, param_values, n_steps: int, exact: bool = False) -> BatchSweepResult:
         if self._multi is None and self._hybrid_changes_rule():
             cfg = self._pycfg
-            if any(m.monitor_type.lower() == "bold" for m in cfg.monitors) or self._cfg.integrator not in (
-                    L.EULER, L.HEUN, L.RK4):
-                _err(L.HB_INVALID_CONFIG, "hybrid_integrator needs a deterministic run without BOLD here")
+            if any(m.monitor_type.lower() == "bold" for m in cfg.monitors):
+                _err(L.HB_INVALID_CONFIG, "hybrid_integrator needs a run without BOLD here")
             subs, links = _multi_from_config(cfg)
             self._multi = (subs, links, self._cfg.integrator, self._device)
             self._max_delay = max([max(p.delays) for p in cfg.projections if p.delays] or [0])
+            self._nsig = cfg.nsig
         if self._multi is not None:
             return self._run_multi(param, param_values, n_steps)
         return self._run(param, param_values, n_steps, False, exact)

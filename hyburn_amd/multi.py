@@ -56,9 +56,15 @@ def parse_cvar_map(s: str) -> list:
 
 def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_steps: int,
               integrator: int = L.HEUN, hybrid_integrator: bool = False, n_sweep: int = 1,
-              sweep: Optional[tuple] = None, max_delay: int = 0, device: int = 0) -> MultiResult:
+              sweep: Optional[tuple] = None, max_delay: int = 0, nsig=None, noise=None,
+              noise_seed: int = 0, device: int = 0) -> MultiResult:
     """max_delay: the largest entry of any projection's delay list, which sets
-    the history horizon (engine.rs:253-257); each link's own delay counts too."""
+    the history horizon (engine.rs:253-257); each link's own delay counts too.
+    Stochastic integrators: nsig per variable (a scalar is repeated over the
+    first subnetwork's nvar, as NsigConfig::to_vec does) and the raw N(0,1)
+    draws, one [n_steps][n_nodes][nvar] array per subnetwork shared by all
+    sweep points (drawn from numpy's PCG64 seeded with noise_seed when not
+    given; the reference draws from an unseeded thread_rng)."""
     keep = []
     subs = (L.HbSubnetwork * len(subnets))()
     for k, S in enumerate(subnets):
@@ -102,6 +108,18 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
             raise ValueError("number of sweep values must equal n_sweep")
         keep.append(vals)
         spec.sweep_sub, spec.sweep_param, spec.sweep_values = sub_idx, param_idx, vals.ctypes.data_as(L.fp)
+    if integrator in (L.EULER_STOCH, L.HEUN_STOCH, L.RK4_STOCH):
+        nv0 = L.MODEL_INFO[subnets[0].model][0]
+        ns = np.atleast_1d(np.asarray(0.0 if nsig is None else nsig, np.float32))
+        if ns.size == 1:
+            ns = np.repeat(ns, nv0)
+        if noise is None:
+            rng = np.random.default_rng(noise_seed)
+            noise = [rng.standard_normal((n_steps, S.n_nodes, L.MODEL_INFO[S.model][0])).astype(np.float32)
+                     for S in subnets]
+        z = np.concatenate([np.ascontiguousarray(a, np.float32).ravel() for a in noise])
+        keep += [ns, z]
+        spec.n_nsig, spec.nsig, spec.noise = len(ns), ns.ctypes.data_as(L.fp), z.ctypes.data_as(L.fp)
     sizes = [n_sweep * S.n_nodes * L.MODEL_INFO[S.model][0] for S in subnets]
     tavg = np.zeros(sum(sizes), np.float32)
     fin = np.zeros(sum(sizes), np.float32)

@@ -290,13 +290,18 @@ typedef struct {
     int32_t n_projections;    /* 0..8 */
     const hb_multi_projection* projections;
     float dt;
-    int32_t integrator;       /* HB_EULER, HB_HEUN or HB_RK4 */
-    int32_t hybrid;           /* hybrid_integrator: Heun for G2DO (engine.rs:698-702) */
+    int32_t integrator;       /* HB_EULER ... HB_RK4_STOCH */
+    int32_t hybrid;           /* hybrid_integrator: deterministic Heun for G2DO (engine.rs:698-702) */
     int32_t n_sweep, n_steps;
     int32_t sweep_sub, sweep_param;   /* SweepParam; sweep_sub = -1: none */
     const float* sweep_values;        /* [n_sweep] */
     int32_t max_delay;        /* largest delay of any projection's delay list (history horizon - 1,
                                  engine.rs:253-257); the projections' own delays count too */
+    int32_t n_nsig;           /* stochastic integrators: nsig per variable (the reference sizes it by
+                                 the first subnetwork's nvar, engine.rs:452) */
+    const float* nsig;
+    const float* noise;       /* raw N(0,1) draws, per subnetwork [n_steps][n_nodes][nvar], concatenated;
+                                 one draw per step shared by every sweep point (engine.rs:726-763) */
     int32_t device;
     float* tavg;              /* out: per subnetwork [n_sweep][n_nodes][nvar], concatenated */
     float* final_state;       /* out: same layout */
@@ -310,8 +315,10 @@ typedef struct {
  * with that frozen coupling; tavg = sum of states / n_steps. A projection
  * delay d > 0 reads the batch engine's history window of max_delay + 1 states
  * at slot (step - d + H) % H (engine.rs:541-589), zeros while d > step.
- * HB_INVALID_CONFIG for noise, unknown models or cvar_map indices out of range
- * (engine.rs:371-388). */
+ * Stochastic integrators add raw * sqrt(2 nsig_v) * sqrt(dt) (integrator.rs:
+ * 179-191) as the batch engine does. HB_INVALID_CONFIG for unknown models,
+ * cvar_map indices out of range (engine.rs:371-388), or an nsig shorter than
+ * a stochastic subnetwork's nvar (the reference indexes past its end). */
 int hb_run_multi(hb_multi_spec* spec);
 
 /* ---------------- functional-connectivity features ---------------- */

@@ -313,7 +313,8 @@ def _f32sum(v):
     return np.cumsum(np.asarray(v, np.float32), dtype=np.float32)[-1] if len(v) else np.float32(0.0)
 
 
-def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_sweep=1, max_delay=0):
+def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_sweep=1, max_delay=0, nsig=None,
+              noise=None):
     """The batch engine's multi-subnetwork loop (engine.rs:540-823) in f32, one
     sweep point at a time: coupling of every projection from the current
     states (engine.rs:557-673), then each subnetwork's integration with that
@@ -322,7 +323,10 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
     cvar_map, delay=0). Delayed reads use the batch engine's history window
     literally (engine.rs:253-270, 541-589): H = max delay + 1 states,
     initially [x0, 0, ..., 0], shifted left and appended each step, read at
-    slot (step - d + H) % H. Returns per subnetwork tavg and final [n_sweep][n][nvar]."""
+    slot (step - d + H) % H. Stochastic forms (engine.rs:726-763): Z = raw *
+    (sqrt(2 nsig_v) * sqrt(dt)) from noise[k][step] (one draw shared by every
+    sweep point); Euler x + d dt + Z; Heun with Z in predictor and corrector;
+    RK4 clamp(det + Z). Returns per subnetwork tavg and final [n_sweep][n][nvar]."""
     from hyburn_amd import _lib as HL
     f32 = np.float32
     S = len(subnets)
@@ -394,9 +398,30 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
             for k, (m, n, params, _) in enumerate(subnets):
                 c = cpl[k] if cpl[k] is not None else np.zeros((n, info[k][1]), np.float32)
                 heun = (m == HL.G2DO) if hybrid else (integrator == HL.HEUN)
+                stoch = (not (hybrid and m == HL.G2DO)) and integrator in (HL.EULER_STOCH, HL.HEUN_STOCH,
+                                                                           HL.RK4_STOCH)
+                if stoch:
+                    scale = np.array([f32(np.sqrt(f32(2.0) * f32(nsig[v]))) * f32(np.sqrt(dtf))
+                                      for v in range(info[k][0])], np.float32)
+                    Zk = (np.asarray(noise[k][step], np.float32) * scale).astype(np.float32)
                 for i in range(n):
                     xi, ci = x[k][i].copy(), c[i]
                     d = dfun(m, params, xi, ci, sw[k], sv)
+                    if stoch:
+                        z = Zk[i]
+                        if integrator == HL.EULER_STOCH:
+                            xn = ((xi + d * dtf) + z).astype(np.float32)
+                        elif integrator == HL.HEUN_STOCH:
+                            pr = clamp(m, ((xi + d * dtf) + z).astype(np.float32))
+                            d2 = dfun(m, params, pr, ci, sw[k], sv)
+                            xn = ((xi + (d + d2) * dth) + z).astype(np.float32)
+                        else:
+                            k2 = dfun(m, params, clamp(m, xi + d * dth), ci, sw[k], sv)
+                            k3 = dfun(m, params, clamp(m, xi + k2 * dth), ci, sw[k], sv)
+                            k4 = dfun(m, params, clamp(m, xi + k3 * dtf), ci, sw[k], sv)
+                            xn = ((xi + (((d + k2 * f32(2.0)) + k3 * f32(2.0)) + k4) * dt6) + z).astype(np.float32)
+                        x[k][i] = clamp(m, xn)
+                        continue
                     if heun:
                         pr = clamp(m, (xi + d * dtf).astype(np.float32))
                         d2 = dfun(m, params, pr, ci, sw[k], sv)

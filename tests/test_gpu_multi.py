@@ -139,3 +139,22 @@ def test_delayed_projections(gpu, orc):
                         dt=0.1, n_steps=150, integrator=L.RK4, n_sweep=2, max_delay=md)
         tav, fin = orc.run_multi(subs, links, 0.1, 150, L.RK4, False, n_sweep=2, max_delay=md)
         _check(res, tav, fin)
+
+
+@pytest.mark.parametrize("integ,hybrid", [(L.EULER_STOCH, False), (L.HEUN_STOCH, True), (L.RK4_STOCH, False)])
+def test_stochastic_multi(gpu, orc, integ, hybrid):
+    # one raw draw per node, variable and step, shared by the sweep points;
+    # under hybrid_integrator G2DO steps with deterministic Heun
+    rng = np.random.default_rng(integ)
+    n, T = 6, 120
+    subs = [(L.G2DO, n, G2DO_DEFAULT, _x0(rng, n, 2)), (L.WILSON_COWAN, n, WC_DEFAULT, np.abs(_x0(rng, n, 2, 0.1)))]
+    links = [(0, 1, 0.05, L.CF_LINEAR, [1.0], [(0, 0)]), (1, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 0)])]
+    raw = [rng.standard_normal((T, n, 2)).astype(np.float32) for _ in subs]
+    nsig = np.array([0.001, 0.002], np.float32)
+    vals = np.array([0.0, 0.3], np.float32)
+    res = run_multi([Subnet(m, k, p, x0) for (m, k, p, x0) in subs],
+                    [Link(s, t, w, cfn, cp, cm) for (s, t, w, cfn, cp, cm) in links],
+                    dt=0.1, n_steps=T, integrator=integ, hybrid_integrator=hybrid, n_sweep=2, sweep=(0, 1, vals),
+                    nsig=nsig, noise=raw)
+    tav, fin = orc.run_multi(subs, links, 0.1, T, integ, hybrid, sweep=(0, 1, vals), n_sweep=2, nsig=nsig, noise=raw)
+    _check(res, tav, fin)

@@ -79,11 +79,16 @@ def test_multi_host_validation():
         ([Link(0, 1, 0.1, cvar_map=[(1, 0)])], {}, "cvar_map src index 1 >= src_ncvar 1"),
         ([Link(1, 0, 0.1, cvar_map=[(0, 1)])], {}, "cvar_map tgt index 1 >= tgt_ncvar 1"),
         ([Link(0, 5, 0.1)], {}, "src/tgt out of range"),
-        ([Link(0, 1, 0.1)], {"integrator": L.HEUN_STOCH}, "deterministically"),
     ]
     for links, kw, msg in cases:
         with pytest.raises(L.HyburnError, match=msg):
             run_multi(subs, links, dt=0.1, n_steps=10, **kw)
+    # nsig is sized by the first subnetwork's nvar (engine.rs:452); a later
+    # stochastic subnetwork with more variables is rejected (the reference
+    # indexes past the end)
+    with pytest.raises(L.HyburnError, match="nsig has 2 entries"):
+        run_multi([Subnet(L.G2DO, 3, G2DO_DEFAULT), Subnet(L.JANSEN_RIT, 3, JR_DEFAULT)], [Link(0, 1, 0.1)],
+                  dt=0.1, n_steps=5, integrator=L.EULER_STOCH, nsig=0.01)
     with pytest.raises(L.HyburnError, match="1024 nodes"):
         run_multi([Subnet(L.G2DO, 600, G2DO_DEFAULT), Subnet(L.G2DO, 600, G2DO_DEFAULT)], [], dt=0.1, n_steps=1)
 
@@ -117,6 +122,25 @@ def test_delayed_self_projection_matches_batch_compat(orc, d, dense):
                     integrator=L.HEUN, cfn=L.CF_LINEAR, cparams=(0.8, 0.0),
                     wkind=L.W_DENSE if dense else L.W_SCALAR_MEAN, dense=W if dense else None,
                     scalar_w=0.0 if dense else 0.2, delay_mode=L.DELAY_BATCH_COMPAT, single_delay=d)
+    ref = orc.run(inst)
+    np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(tav[0][0], ref["tavg"], rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("integ", [L.EULER_STOCH, L.HEUN_STOCH, L.RK4_STOCH])
+def test_stochastic_single_subnet_composition(orc, integ):
+    # the stochastic forms against the C restatement's batch-engine noise path
+    # (the same raw draws, scale sqrt(2 nsig) sqrt(dt))
+    rng = np.random.default_rng(integ)
+    n, T = 5, 40
+    x0 = _x0(rng, n, 2)
+    raw = rng.standard_normal((T, n, 2)).astype(np.float32)
+    nsig = np.array([0.01, 0.02], np.float32)
+    link = (0, 0, 0.1, L.CF_LINEAR, [0.5, 0.0], [(0, 0)])
+    tav, fin = orc.run_multi([(L.G2DO, n, G2DO_DEFAULT, x0)], [link], 0.1, T, integ, False, nsig=nsig, noise=[raw])
+    inst = Instance(model=L.G2DO, n_nodes=n, n_steps=T, dt=0.1, params=np.array(G2DO_DEFAULT, np.float32), x0=x0,
+                    integrator=integ, cfn=L.CF_LINEAR, cparams=(0.5, 0.0), wkind=L.W_SCALAR_MEAN, scalar_w=0.1,
+                    delay_mode=L.DELAY_SINGLE, single_delay=0, nsig=nsig, noise=raw)
     ref = orc.run(inst)
     np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-6, atol=1e-7)
     np.testing.assert_allclose(tav[0][0], ref["tavg"], rtol=1e-6, atol=1e-7)
