@@ -138,7 +138,8 @@ class HbMultiSpec(C.Structure):
                 ("hybrid", C.c_int32), ("n_sweep", C.c_int32), ("n_steps", C.c_int32), ("sweep_sub", C.c_int32),
                 ("sweep_param", C.c_int32), ("sweep_values", fp), ("max_delay", C.c_int32), ("n_nsig", C.c_int32),
                 ("nsig", fp), ("noise", fp), ("device", C.c_int32), ("tavg", fp),
-                ("final_state", fp), ("kernel_ms", C.c_double)]
+                ("final_state", fp), ("kernel_ms", C.c_double), ("step0", C.c_int64), ("state_in", fp),
+                ("hist_in", fp), ("hist_out", fp)]
 
 
 # Every exported function of include/hyburn_hip.h with its prototype.

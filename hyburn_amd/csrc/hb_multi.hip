@@ -51,6 +51,8 @@ struct Args {
     Sub sub[kMaxSub];
     Proj proj[kMaxProj];
     int32_t n_sub, n_proj, n_tot, n_steps, H;   // H: history horizon (0: no delayed projection)
+    int32_t x0_per_point;  // x0 is [B][n_tot][kNV] (resume) instead of [n_tot][kNV]
+    int64_t step0;         // the engine's step counter at the first step
     float dt, dth, dt6, inv_nsteps;
     float* hist;           // [B][n_tot][H][2] each node's coupling variables of the last H steps
     const float* noise;    // [n_steps][n_tot][kNV] raw N(0,1), shared by the sweep points, or null
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     float x[kNV], acc[kNV];
 #pragma unroll
     for (int v = 0; v < kNV; ++v) {
-        x[v] = (live && v < nv) ? A->x0[(size_t)t * kNV + v] : 0.0f;
+        x[v] = (live && v < nv) ? A->x0[((A->x0_per_point ? (size_t)b * A->n_tot : 0) + t) * kNV + v] : 0.0f;
         acc[v] = 0.0f;
     }
     StepCtx S;
@@ -127,7 +129,8 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     S.sv = A->sweep ? A->sweep[b] : 0.0f;
     const int H = A->H;
     float* hist = (live && H > 0) ? A->hist + ((size_t)b * A->n_tot + t) * H * 2 : nullptr;
-    for (int step = 0; step < A->n_steps; ++step) {
+    for (int k = 0; k < A->n_steps; ++k) {
+        const int64_t step = A->step0 + k;   // the engine's step counter (delays read by it)
         // the history window gains the current state (engine.rs:541-551); each
         // node keeps its own first two variables by real step, slot step % H
         if (hist) {
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                     // while d > step
                     xd[0] = xd[1] = 0.0f;
                     if (P.delay <= step) {
-                        const int ts = step - H + 1 + (step - P.delay + H) % H;
+                        const int64_t ts = step - H + 1 + (step - P.delay + H) % H;
                         if (ts >= 0) {
                             xd[0] = hist[(ts % H) * 2];
                             xd[1] = hist[(ts % H) * 2 + 1];
@@ -211,7 +214,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
 #pragma unroll
             for (int v = 0; v < kNV; ++v) z[v] = 0.0f;
             if (S.stoch) {
-                const float* raw = A->noise + ((size_t)step * A->n_tot + t) * kNV;
+                const float* raw = A->noise + ((size_t)k * A->n_tot + t) * kNV;
                 for (int v = 0; v < nv; ++v) z[v] = raw[v] * nscale[v];   // integrator.rs:183-191
             }
             step_model(model, S, mc, cpl, z, x);
@@ -421,11 +424,23 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     A.dth = Q.dt * 0.5f;
     A.dt6 = Q.dt / 6.0f;
     A.inv_nsteps = 1.0f / (float)Q.n_steps;
-    std::vector<float> x0((size_t)n_tot * kNV, 0.0f);
-    for (int q = 0; q < Q.n_subnets; ++q)
-        for (int r = 0; r < A.sub[q].n; ++r)
-            for (int v = 0; v < A.sub[q].nv; ++v)
-                x0[(size_t)(A.sub[q].off + r) * kNV + v] = Q.subnets[q].x0[(size_t)r * A.sub[q].nv + v];
+    if (Q.step0 < 0) return bad("step0 must be >= 0");
+    A.step0 = Q.step0;
+    A.x0_per_point = Q.state_in ? 1 : 0;
+    std::vector<float> x0((size_t)(Q.state_in ? Q.n_sweep : 1) * n_tot * kNV, 0.0f);
+    if (Q.state_in) {   // per subnetwork [n_sweep][n][nv], concatenated
+        size_t o = 0;
+        for (int q = 0; q < Q.n_subnets; ++q)
+            for (int b = 0; b < Q.n_sweep; ++b)
+                for (int r = 0; r < A.sub[q].n; ++r)
+                    for (int v = 0; v < A.sub[q].nv; ++v, ++o)
+                        x0[((size_t)b * n_tot + A.sub[q].off + r) * kNV + v] = Q.state_in[o];
+    } else {
+        for (int q = 0; q < Q.n_subnets; ++q)
+            for (int r = 0; r < A.sub[q].n; ++r)
+                for (int v = 0; v < A.sub[q].nv; ++v)
+                    x0[(size_t)(A.sub[q].off + r) * kNV + v] = Q.subnets[q].x0[(size_t)r * A.sub[q].nv + v];
+    }
     const size_t nout = (size_t)Q.n_sweep * n_tot * kNV;
     float *dx0 = nullptr, *dsw = nullptr, *dtavg = nullptr, *dfin = nullptr;
     Args* dA = nullptr;
@@ -435,7 +450,12 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     if (e == hipSuccess) e = B.alloc(&dA, 1);
     if (e == hipSuccess && Q.sweep_sub >= 0) e = B.alloc(&dsw, (size_t)Q.n_sweep);
     float* dhist = nullptr;
-    if (e == hipSuccess && H > 0) e = B.alloc(&dhist, (size_t)Q.n_sweep * n_tot * H * 2);
+    const size_t nhist = (size_t)Q.n_sweep * n_tot * H * 2;
+    if (e == hipSuccess && H > 0) {
+        e = B.alloc(&dhist, nhist);
+        if (e == hipSuccess && Q.hist_in) e = hipMemcpy(dhist, Q.hist_in, sizeof(float) * nhist, hipMemcpyHostToDevice);
+        else if (e == hipSuccess) e = hipMemset(dhist, 0, sizeof(float) * nhist);
+    }
     bool any_stoch = false;
     for (int q = 0; q < Q.n_subnets; ++q) any_stoch = any_stoch || A.sub[q].stoch;
     float* dnoise = nullptr;
@@ -483,6 +503,8 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     std::vector<float> ht(nout), hf(nout);
     e = hipMemcpy(ht.data(), dtavg, sizeof(float) * nout, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(hf.data(), dfin, sizeof(float) * nout, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && H > 0 && Q.hist_out)
+        e = hipMemcpy(Q.hist_out, dhist, sizeof(float) * nhist, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "multi-subnetwork download");
     // per subnetwork, [n_sweep][n_nodes][nvar] (the BatchSweepResult layout), concatenated
     size_t o = 0;

@@ -355,6 +355,7 @@ class BatchHybridEngine:
         self._multi = None
         self._max_delay = 0
         self._nsig, self._noise_seed = 0.0, 0
+        self._step, self._state, self._hist = 0, None, None   # carried across run_sweep calls
         self.hybrid_integrator = False
 
     @classmethod
@@ -473,11 +474,15 @@ class BatchHybridEngine:
         if not 0 <= param.sub_idx < len(subs):
             _err(L.HB_INVALID_CONFIG, "subnetwork index out of range")
         t0 = time.perf_counter()
+        # like the reference's engine, a run continues from the previous run's
+        # states, step counter and history (engine.rs:496-540)
         r = run_multi(subs, links, dt=self.dt, n_steps=int(n_steps), integrator=integ, nsig=self._nsig,
-                      noise_seed=self._noise_seed,
+                      noise_seed=self._noise_seed + self._step,
                       hybrid_integrator=self.hybrid_integrator, n_sweep=self.n_sweep,
-                      sweep=(param.sub_idx, param.param_idx, v), max_delay=int(self._max_delay), device=device)
+                      sweep=(param.sub_idx, param.param_idx, v), max_delay=int(self._max_delay),
+                      step0=self._step, state=self._state, hist=self._hist, device=device)
         ms = (time.perf_counter() - t0) * 1e3
+        self._step, self._state, self._hist = r.step_end, r.final_states, r.hist
         return BatchSweepResult(param_values=v.tolist(), tavg=[t.ravel() for t in r.tavg],
                                 final_states=[f.ravel() for f in r.final_states], trajectories=None, bold=None,
                                 elapsed_ms=ms, n_sweep=self.n_sweep, kernel_ms=r.kernel_ms)
@@ -523,6 +528,44 @@ class BatchHybridEngine:
         flags = (L.FLAG_TRAJECTORY if trajectory else 0) | (L.FLAG_EXACT if exact else 0)
         L.check(L.lib().hb_run_sweep_multi(self._h, arr, len(spec.points), int(n_steps), flags, C.byref(r)))
         return self._result(r, [pt[0][2] for pt in spec.cartesian_product()], trajectory)
+
+    def checkpoint(self, path: str) -> None:
+        """Save the engine state as a HYBURNCK version 3 (batch) file
+        (runtime.rs:503-561 extended with the sweep dimension and history)."""
+        from . import checkpoint as ck
+        if self._multi is None:
+            _err(L.HB_INVALID_STATE, "checkpoints are kept by the multi-subnetwork path")
+        subs = self._multi[0]
+        shapes = [(L.MODEL_INFO[S.model][0], S.n_nodes) for S in subs]
+        states = self._state or [np.broadcast_to(np.asarray(S.x0, np.float32), (self.n_sweep,) + np.shape(S.x0))
+                                 for S in subs]
+        nsig = np.atleast_1d(np.asarray(self._nsig, np.float32))
+        ck.write_batch(path, step=self._step, dt=self.dt, integrator=self._multi[2], nsig=nsig, shapes=shapes,
+                       states=states, hist=self._hist)
+
+    def resume(self, path: str) -> None:
+        """Restore step, dt, integrator, nsig, states and history from a
+        HYBURNCK file (runtime.rs:570-742); shapes must match."""
+        from . import checkpoint as ck
+        if self._multi is None:
+            _err(L.HB_INVALID_STATE, "checkpoints are kept by the multi-subnetwork path")
+        c = ck.read(path)
+        subs, links, _, device = self._multi
+        if len(c["shapes"]) != len(subs):
+            _err(L.HB_INVALID_STATE, f"Checkpoint has {len(c['shapes'])} subnetworks, expected {len(subs)}")
+        for i, ((nv, nn, nm), S) in enumerate(zip(c["shapes"], subs)):
+            want = (L.MODEL_INFO[S.model][0], S.n_nodes, 1)
+            if (nv, nn, nm) != want:
+                _err(L.HB_INVALID_STATE, f"Checkpoint shape mismatch for subnetwork {i}: expected "
+                                         f"(nvar={want[0]},nnodes={want[1]},nmodes=1), got ({nv},{nn},{nm})")
+        states = c["states"]
+        if c["n_sweep"] != self.n_sweep:
+            if c["n_sweep"] != 1:
+                _err(L.HB_INVALID_STATE, f"Checkpoint holds {c['n_sweep']} sweep points, expected {self.n_sweep}")
+            states = [np.repeat(a, self.n_sweep, axis=0) for a in states]
+        self._step, self.dt, self._nsig = int(c["step"]), float(c["dt"]), c["nsig"]
+        self._multi = (subs, links, c["integrator"], device)
+        self._state, self._hist = states, c["hist"]
 
     def close(self):
         if self._h:

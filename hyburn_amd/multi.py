@@ -47,6 +47,14 @@ class MultiResult:
     tavg: list = field(default_factory=list)
     final_states: list = field(default_factory=list)
     kernel_ms: float = 0.0
+    hist: Optional[np.ndarray] = None     # [n_sweep][total nodes][H][2] by step % H (delayed runs)
+    step_end: int = 0                     # the engine's step counter after the run
+
+
+def horizon(links: Sequence[Link], max_delay: int = 0) -> int:
+    """History length: max delay + 1, 0 without delays (engine.rs:253-257)."""
+    d = max([max_delay] + [L_.delay for L_ in links])
+    return d + 1 if d > 0 else 0
 
 
 def parse_cvar_map(s: str) -> list:
@@ -57,14 +65,18 @@ def parse_cvar_map(s: str) -> list:
 def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_steps: int,
               integrator: int = L.HEUN, hybrid_integrator: bool = False, n_sweep: int = 1,
               sweep: Optional[tuple] = None, max_delay: int = 0, nsig=None, noise=None,
-              noise_seed: int = 0, device: int = 0) -> MultiResult:
+              noise_seed: int = 0, step0: int = 0, state: Optional[list] = None,
+              hist: Optional[np.ndarray] = None, device: int = 0) -> MultiResult:
     """max_delay: the largest entry of any projection's delay list, which sets
     the history horizon (engine.rs:253-257); each link's own delay counts too.
     Stochastic integrators: nsig per variable (a scalar is repeated over the
     first subnetwork's nvar, as NsigConfig::to_vec does) and the raw N(0,1)
     draws, one [n_steps][n_nodes][nvar] array per subnetwork shared by all
     sweep points (drawn from numpy's PCG64 seeded with noise_seed when not
-    given; the reference draws from an unseeded thread_rng)."""
+    given; the reference draws from an unseeded thread_rng).
+    Resume: step0 is the engine's step counter at the start, state one
+    [n_sweep][n_nodes][nvar] array per subnetwork, hist the history returned
+    by the previous run (MultiResult.hist)."""
     keep = []
     subs = (L.HbSubnetwork * len(subnets))()
     for k, S in enumerate(subnets):
@@ -121,11 +133,29 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
         keep += [ns, z]
         spec.n_nsig, spec.nsig, spec.noise = len(ns), ns.ctypes.data_as(L.fp), z.ctypes.data_as(L.fp)
     sizes = [n_sweep * S.n_nodes * L.MODEL_INFO[S.model][0] for S in subnets]
+    spec.step0 = int(step0)
+    if state is not None:
+        st = np.concatenate([np.ascontiguousarray(a, np.float32).ravel() for a in state])
+        if st.size != sum(sizes):
+            raise ValueError("state must hold one [n_sweep][n_nodes][nvar] array per subnetwork")
+        keep.append(st)
+        spec.state_in = st.ctypes.data_as(L.fp)
+    H = horizon(links, max_delay)
+    n_tot = sum(S.n_nodes for S in subnets)
+    hist_out = np.zeros((n_sweep, n_tot, H, 2), np.float32) if H else None
+    if H:
+        spec.hist_out = hist_out.ctypes.data_as(L.fp)
+        if hist is not None:
+            hi = np.ascontiguousarray(hist, np.float32)
+            if hi.shape != hist_out.shape:
+                raise ValueError(f"hist must be {hist_out.shape}")
+            keep.append(hi)
+            spec.hist_in = hi.ctypes.data_as(L.fp)
     tavg = np.zeros(sum(sizes), np.float32)
     fin = np.zeros(sum(sizes), np.float32)
     spec.tavg, spec.final_state = tavg.ctypes.data_as(L.fp), fin.ctypes.data_as(L.fp)
     L.check(L.lib().hb_run_multi(spec))
-    out = MultiResult(kernel_ms=spec.kernel_ms)
+    out = MultiResult(kernel_ms=spec.kernel_ms, hist=hist_out, step_end=int(step0) + int(n_steps))
     o = 0
     for S, n in zip(subnets, sizes):
         shape = (n_sweep, S.n_nodes, L.MODEL_INFO[S.model][0])

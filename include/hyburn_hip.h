@@ -306,6 +306,15 @@ typedef struct {
     float* tavg;              /* out: per subnetwork [n_sweep][n_nodes][nvar], concatenated */
     float* final_state;       /* out: same layout */
     double kernel_ms;         /* out: the kernel's HIP-event time */
+    /* resume (all optional): the engine's step counter at the start, per sweep
+     * point start states (the final_state layout; null: every point starts from
+     * the subnetworks' x0) and the coupling-variable history of the last
+     * max_delay + 1 steps, [n_sweep][total nodes][max_delay + 1][2] by step
+     * modulo max_delay + 1 (null: zeros). hist_out receives it at the end. */
+    int64_t step0;
+    const float* state_in;
+    const float* hist_in;
+    float* hist_out;
 } hb_multi_spec;
 
 /* BatchHybridEngine::run_sweep over several subnetworks joined by projections

@@ -158,3 +158,51 @@ def test_stochastic_multi(gpu, orc, integ, hybrid):
                     nsig=nsig, noise=raw)
     tav, fin = orc.run_multi(subs, links, 0.1, T, integ, hybrid, sweep=(0, 1, vals), n_sweep=2, nsig=nsig, noise=raw)
     _check(res, tav, fin)
+
+
+def test_split_run_resumes_bit_for_bit(gpu):
+    # 200 steps at once == 100 steps, then 100 more from the returned states,
+    # step counter and history (delays cross the split; noise drawn per step)
+    rng = np.random.default_rng(21)
+    n = 5
+    subs = [Subnet(L.G2DO, n, G2DO_DEFAULT, _x0(rng, n, 2)), Subnet(L.WILSON_COWAN, n, WC_DEFAULT,
+                                                                   np.abs(_x0(rng, n, 2, 0.1)))]
+    links = [Link(0, 1, 0.05, delay=7), Link(1, 0, 0.05, delay=3), Link(0, 0, 0.02, delay=12)]
+    raw = [rng.standard_normal((200, n, 2)).astype(np.float32) for _ in subs]
+    vals = np.array([0.0, 0.2, -0.2], np.float32)
+    kw = dict(dt=0.1, integrator=L.HEUN_STOCH, n_sweep=3, sweep=(0, 1, vals), nsig=[0.001, 0.001])
+    full = run_multi(subs, links, n_steps=200, noise=raw, **kw)
+    a = run_multi(subs, links, n_steps=100, noise=[z[:100] for z in raw], **kw)
+    b = run_multi(subs, links, n_steps=100, noise=[z[100:] for z in raw], step0=a.step_end,
+                  state=a.final_states, hist=a.hist, **kw)
+    for k in range(2):
+        np.testing.assert_array_equal(b.final_states[k], full.final_states[k])
+        np.testing.assert_allclose((a.tavg[k] + b.tavg[k]) * 0.5, full.tavg[k], rtol=1e-5, atol=1e-6)
+
+
+def test_engine_checkpoint_resume(gpu, tmp_path):
+    # BatchHybridEngine keeps states, step and history across run_sweep calls
+    # (engine.rs:496-540); checkpoint() / resume() carry them through a file
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig, SweepParam
+    n = 4
+    d = {"sim_length": 10.0, "dt": 0.1, "integrator": "rk4", "network": {
+        "subnetworks": [
+            {"model": "Generic2dOscillator", "nnodes": n, "params": G2DO_DEFAULT.tolist(),
+             "initial_state": (np.arange(2 * n) * 0.05).tolist()},
+            {"model": "JansenRit", "nnodes": n, "params": JR_DEFAULT.tolist(), "initial_state": [0.0] * (6 * n)}],
+        "projections": [{"src": 0, "tgt": 1, "weights": 0.02, "delays": [4]},
+                        {"src": 1, "tgt": 0, "weights": 0.02, "delays": [2, 9]}]}}
+    vals = np.array([0.0, 0.4], np.float32)
+    one = BatchHybridEngine.from_config(SimConfig.from_dict(d), 2)
+    r_full = one.run_sweep(SweepParam(0, 1), vals, 160)
+    two = BatchHybridEngine.from_config(SimConfig.from_dict(d), 2)
+    two.run_sweep(SweepParam(0, 1), vals, 90)
+    path = str(tmp_path / "ck.bin")
+    two.checkpoint(path)
+    three = BatchHybridEngine.from_config(SimConfig.from_dict(d), 2)
+    three.resume(path)
+    r3 = three.run_sweep(SweepParam(0, 1), vals, 70)
+    r2 = two.run_sweep(SweepParam(0, 1), vals, 70)
+    for k in range(2):
+        np.testing.assert_array_equal(r3.final_states[k], r_full.final_states[k])
+        np.testing.assert_array_equal(r2.final_states[k], r_full.final_states[k])

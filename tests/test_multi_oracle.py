@@ -144,3 +144,42 @@ def test_stochastic_single_subnet_composition(orc, integ):
     ref = orc.run(inst)
     np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-6, atol=1e-7)
     np.testing.assert_allclose(tav[0][0], ref["tavg"], rtol=1e-6, atol=1e-7)
+
+
+def test_checkpoint_round_trip_and_errors(tmp_path):
+    # the version 3 (batch) file round-trips; a version 2 file in the
+    # reference's layout ([nvar][nnodes][nmodes]) reads as one sweep point
+    import struct
+    from hyburn_amd import checkpoint as ck
+    rng = np.random.default_rng(0)
+    st = [rng.normal(size=(3, 4, 2)).astype(np.float32), rng.normal(size=(3, 5, 6)).astype(np.float32)]
+    hist = rng.normal(size=(3, 9, 4, 2)).astype(np.float32)
+    p = str(tmp_path / "b.bin")
+    ck.write_batch(p, step=77, dt=0.05, integrator=L.RK4_STOCH, nsig=[0.01, 0.02], shapes=[(2, 4), (6, 5)],
+                   states=st, hist=hist)
+    c = ck.read(p)
+    assert (c["version"], c["step"], c["dt"], c["integrator"], c["n_sweep"]) == (3, 77, 0.05, L.RK4_STOCH, 3)
+    assert c["shapes"] == [(2, 4, 1), (6, 5, 1)]
+    np.testing.assert_array_equal(c["nsig"], np.array([0.01, 0.02], np.float32))
+    for a, b in zip(c["states"], st):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(c["hist"], hist)
+    # a version 2 file written byte by byte as runtime.rs:519-560 does
+    s2 = rng.normal(size=(2, 4, 1)).astype(np.float32)
+    raw = (b"HYBURNCK" + struct.pack("<QQd", 2, 5, 0.1) + bytes([1]) + bytes(7) + struct.pack("<I", 1) + bytes(4)
+           + struct.pack("<f", 0.003) + struct.pack("<Q", 1) + struct.pack("<QQQ", 2, 4, 1) + s2.tobytes())
+    p2 = tmp_path / "p.bin"
+    p2.write_bytes(raw)
+    c2 = ck.read(str(p2))
+    assert (c2["version"], c2["step"], c2["integrator"], c2["n_sweep"]) == (2, 5, L.HEUN, 1)
+    np.testing.assert_array_equal(c2["states"][0][0], s2[:, :, 0].T)
+    # the reference's error cases (runtime.rs:636-700)
+    (tmp_path / "bad.bin").write_bytes(b"NOTACKPT" + raw[8:])
+    with pytest.raises(L.HyburnError, match="bad magic"):
+        ck.read(str(tmp_path / "bad.bin"))
+    (tmp_path / "short.bin").write_bytes(raw[:-8])
+    with pytest.raises(L.HyburnError, match="truncated"):
+        ck.read(str(tmp_path / "short.bin"))
+    (tmp_path / "v9.bin").write_bytes(raw[:8] + struct.pack("<Q", 9) + raw[16:])
+    with pytest.raises(L.HyburnError, match="Unsupported checkpoint version"):
+        ck.read(str(tmp_path / "v9.bin"))
