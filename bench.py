@@ -9,7 +9,12 @@ dt = 0.1 ms, 10,000 integration steps, nsig = 0.01 Philox noise generated on
 device. Inputs are resident in HBM before timing; the timed region covers the
 noise generation and the fused integration kernel.
 
-Prints one JSON line (metric, value, unit, roofline, cpu_baseline, ...).
+Prints one JSON line (metric, value, unit, roofline, cpu_baseline, ...). Extra
+legs, each with its own switch, add objects to the same line: "ragged" (4096
+ragged MPR/Jansen-Rit instances in one plan), "features" (FC, FCD and spectral
+features of 1024 BOLD-length series), "ring" (the G2DO -> JR -> WC network of
+the reference's generic-engine benchmark) and "adjoint" (gradients of the FC
+fit and Kuramoto order-parameter workloads).
 """
 from __future__ import annotations
 
