@@ -132,6 +132,10 @@ class HbMultiProjection(C.Structure):
                 ("n_map", C.c_int32), ("map_src", C.c_int32 * 4), ("map_tgt", C.c_int32 * 4), ("delay", C.c_int32)]
 
 
+class HbStimulus(C.Structure):
+    _fields_ = [("target", C.c_int32), ("pattern", C.c_int32), ("n_params", C.c_int32), ("params", C.c_float * 4)]
+
+
 class HbMultiSpec(C.Structure):
     _fields_ = [("n_subnets", C.c_int32), ("subnets", C.POINTER(HbSubnetwork)), ("n_projections", C.c_int32),
                 ("projections", C.POINTER(HbMultiProjection)), ("dt", C.c_float), ("integrator", C.c_int32),
@@ -139,7 +143,8 @@ class HbMultiSpec(C.Structure):
                 ("sweep_param", C.c_int32), ("sweep_values", fp), ("max_delay", C.c_int32), ("n_nsig", C.c_int32),
                 ("nsig", fp), ("noise", fp), ("device", C.c_int32), ("tavg", fp),
                 ("final_state", fp), ("kernel_ms", C.c_double), ("step0", C.c_int64), ("state_in", fp),
-                ("hist_in", fp), ("hist_out", fp)]
+                ("hist_in", fp), ("hist_out", fp), ("plain", C.c_int32), ("n_stimuli", C.c_int32),
+                ("stimuli", C.POINTER(HbStimulus)), ("dt_ms", C.c_double)]
 
 
 # Every exported function of include/hyburn_hip.h with its prototype.

@@ -47,11 +47,41 @@ struct Proj {
     const float* rowsum;   // [n_tgt]: Difference over a non-square Dense projection, or null
 };
 
+struct Stim {
+    int32_t target, pattern;
+    float p[4];
+};
+
+// stimulus.rs:63-128, in f64 time t = step * dt_ms
+__device__ __attribute__((noinline)) float stim_value(const Stim& S, int64_t step, double dt_ms) {
+    const double t = (double)step * dt_ms;
+    const double onset = (double)S.p[0];
+    switch (S.pattern) {
+    case 0:
+        return (fabs(t - onset) < 1e-6 || (t > onset && t <= onset + 1e-6)) ? S.p[1] : 0.0f;
+    case 1:
+        return (t >= onset && t < (double)S.p[1]) ? S.p[2] : 0.0f;
+    case 2: {
+        if (t < onset) return 0.0f;
+        const double raw = sin(2.0 * 3.141592653589793 * (double)S.p[2] * (t - onset) + (double)S.p[3]);
+        return S.p[1] * (float)raw;
+    }
+    default: {
+        const double period = (double)S.p[2], width = (double)S.p[3];
+        if (t < onset || period <= 0.0 || width <= 0.0) return 0.0f;
+        return fmod(t - onset, period) < width ? S.p[1] : 0.0f;
+    }
+    }
+}
+
 struct Args {
     Sub sub[kMaxSub];
     Proj proj[kMaxProj];
     int32_t n_sub, n_proj, n_tot, n_steps, H;   // H: history horizon (0: no delayed projection)
     int32_t x0_per_point;  // x0 is [B][n_tot][kNV] (resume) instead of [n_tot][kNV]
+    int32_t plain, n_stim; // the plain HybridEngine's semantics; stimuli
+    Stim stim[8];
+    double dt_ms;
     int64_t step0;         // the engine's step counter at the first step
     float dt, dth, dt6, inv_nsteps;
     float* hist;           // [B][n_tot][H][2] each node's coupling variables of the last H steps
@@ -121,7 +151,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     S.dt6 = A->dt6;
     S.integ = live ? A->sub[s].integ : HB_EULER;
     S.stoch = live && A->sub[s].stoch != 0;
-    S.plain = false;
+    S.plain = A->plain != 0;
     float nscale[kNV];
 #pragma unroll
     for (int v = 0; v < kNV; ++v) nscale[v] = live ? A->sub[s].nscale[v] : 0.0f;
@@ -148,7 +178,8 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                     // while d > step
                     xd[0] = xd[1] = 0.0f;
                     if (P.delay <= step) {
-                        const int64_t ts = step - H + 1 + (step - P.delay + H) % H;
+                        // plain: the true ring, state of step - d (runtime.rs:222-233)
+                        const int64_t ts = A->plain ? step - P.delay : step - H + 1 + (step - P.delay + H) % H;
                         if (ts >= 0) {
                             xd[0] = hist[(ts % H) * 2];
                             xd[1] = hist[(ts % H) * 2 + 1];
@@ -210,6 +241,11 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                     for (int m = 0; m < P.n_map; ++m) cpl[P.map_c[m]] += out[P.map_s[m]];
                 }
             }
+            for (int q = 0; q < A->n_stim; ++q)   // runtime.rs:319-357: added to coupling variable 0
+                if (A->stim[q].target == s) {
+                    const float sv = stim_value(A->stim[q], step, A->dt_ms);
+                    if (sv != 0.0f) cpl[0] = cpl[0] + sv;
+                }
             float z[kNV];
 #pragma unroll
             for (int v = 0; v < kNV; ++v) z[v] = 0.0f;
@@ -286,6 +322,9 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     if (Q.n_sweep < 1 || Q.n_steps < 0) return bad("n_sweep must be >= 1 and n_steps >= 0");
     if (Q.integrator < HB_EULER || Q.integrator > HB_RK4_STOCH) return bad("unknown integrator");
     if (Q.sweep_sub < -1 || Q.sweep_sub >= Q.n_subnets) return bad("sweep subnetwork index out of range");
+    if (Q.plain && Q.hybrid) return bad("the plain engine has no hybrid_integrator");
+    if (Q.n_stimuli < 0 || Q.n_stimuli > 8 || (Q.n_stimuli > 0 && !Q.stimuli)) return bad("n_stimuli must be in [0, 8]");
+    if (Q.n_stimuli > 0 && !Q.plain) return bad("stimuli apply to the plain engine only (the batch engine ignores them)");
     if (Q.sweep_sub >= 0 && !Q.sweep_values) return bad("sweep values missing");
     if (!Q.tavg || !Q.final_state) return bad("null output pointer");
     Args A;
@@ -356,9 +395,15 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         P.classic = classic;
         P.w = R.scalar_w;
         if (R.wkind == HB_W_DENSE) host_w[p].assign(R.dense_w, R.dense_w + (size_t)nt * ns);
+        if (Q.plain && R.wkind == HB_W_SCALAR_MEAN) {   // construction.rs:639-654: w on the diagonal
+            if (nt != ns) return bad("Scalar weights require src and tgt nnodes to match");
+            host_w[p].assign((size_t)nt * ns, 0.0f);
+            for (int r = 0; r < nt; ++r) host_w[p][(size_t)r * ns + r] = R.scalar_w;
+            P.wkind = HB_W_DENSE;
+        }
         if (R.cfn == HB_CF_DIFFERENCE) {   // engine.rs:325-366
             const float a = R.n_cparams > 0 ? cp[0] : 1.0f;
-            if (R.wkind == HB_W_SCALAR_MEAN) {
+            if (P.wkind == HB_W_SCALAR_MEAN) {
                 P.w = 0.0f;
                 P.cfn = HB_CF_LINEAR;
                 P.cc[0] = a;
@@ -480,6 +525,21 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     A.hist = dhist;
     A.H = H;
     A.noise = dnoise;
+    A.plain = Q.plain ? 1 : 0;
+    A.dt_ms = Q.dt_ms;
+    A.n_stim = 0;
+    static const int kStimParams[4] = {2, 3, 3, 4};   // stimulus.rs:31-52
+    for (int q = 0; q < Q.n_stimuli; ++q) {
+        const hb_stimulus& T0 = Q.stimuli[q];
+        if (T0.pattern < 0 || T0.pattern > 3) return bad("Unknown temporal pattern");
+        if (T0.n_params < kStimParams[T0.pattern] || T0.n_params > 4)
+            return bad("stimulus " + std::to_string(q) + ": too few parameters for its pattern");
+        if (T0.target < 0 || T0.target >= Q.n_subnets) continue;   // runtime.rs:320-322 skips it
+        Stim& D = A.stim[A.n_stim++];
+        D.target = T0.target;
+        D.pattern = T0.pattern;
+        for (int k = 0; k < 4; ++k) D.p[k] = k < T0.n_params ? T0.params[k] : 0.0f;
+    }
     A.tavg = dtavg;
     A.fin = dfin;
     e = hipMemcpy(dA, &A, sizeof(A), hipMemcpyHostToDevice);

@@ -51,6 +51,9 @@ class MultiResult:
     step_end: int = 0                     # the engine's step counter after the run
 
 
+STIM_PATTERNS = {"impulse": 0, "step": 1, "sinusoid": 2, "pulse_train": 3}
+
+
 def horizon(links: Sequence[Link], max_delay: int = 0) -> int:
     """History length: max delay + 1, 0 without delays (engine.rs:253-257)."""
     d = max([max_delay] + [L_.delay for L_ in links])
@@ -66,7 +69,8 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
               integrator: int = L.HEUN, hybrid_integrator: bool = False, n_sweep: int = 1,
               sweep: Optional[tuple] = None, max_delay: int = 0, nsig=None, noise=None,
               noise_seed: int = 0, step0: int = 0, state: Optional[list] = None,
-              hist: Optional[np.ndarray] = None, device: int = 0) -> MultiResult:
+              hist: Optional[np.ndarray] = None, plain: bool = False, stimuli: Sequence[tuple] = (),
+              dt_ms: Optional[float] = None, device: int = 0) -> MultiResult:
     """max_delay: the largest entry of any projection's delay list, which sets
     the history horizon (engine.rs:253-257); each link's own delay counts too.
     Stochastic integrators: nsig per variable (a scalar is repeated over the
@@ -76,7 +80,11 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
     given; the reference draws from an unseeded thread_rng).
     Resume: step0 is the engine's step counter at the start, state one
     [n_sweep][n_nodes][nvar] array per subnetwork, hist the history returned
-    by the previous run (MultiResult.hist)."""
+    by the previous run (MultiResult.hist).
+    plain: the plain HybridEngine's semantics (true-ring delays, Scalar
+    weights as w * I, stimuli); stimuli: (target, pattern, params) with
+    pattern one of impulse, step, sinusoid, pulse_train (stimulus.rs), at
+    times step * dt_ms (default dt)."""
     keep = []
     subs = (L.HbSubnetwork * len(subnets))()
     for k, S in enumerate(subnets):
@@ -134,6 +142,16 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
         spec.n_nsig, spec.nsig, spec.noise = len(ns), ns.ctypes.data_as(L.fp), z.ctypes.data_as(L.fp)
     sizes = [n_sweep * S.n_nodes * L.MODEL_INFO[S.model][0] for S in subnets]
     spec.step0 = int(step0)
+    spec.plain, spec.dt_ms = int(bool(plain)), float(dt if dt_ms is None else dt_ms)
+    if stimuli:
+        st_arr = (L.HbStimulus * len(stimuli))()
+        for k, (target, pattern, params) in enumerate(stimuli):
+            pid = STIM_PATTERNS[pattern.lower()] if isinstance(pattern, str) else int(pattern)
+            st_arr[k].target, st_arr[k].pattern, st_arr[k].n_params = target, pid, len(params)
+            for j, v in enumerate(list(params)[:4]):
+                st_arr[k].params[j] = v
+        keep.append(st_arr)
+        spec.n_stimuli, spec.stimuli = len(stimuli), st_arr
     if state is not None:
         st = np.concatenate([np.ascontiguousarray(a, np.float32).ravel() for a in state])
         if st.size != sum(sizes):

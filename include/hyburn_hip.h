@@ -284,6 +284,16 @@ typedef struct {
     int32_t delay;            /* steps: the projection's first delay (engine.rs:407) */
 } hb_multi_projection;
 
+/* A stimulus of the plain HybridEngine (src/engine/stimulus.rs:17-128), added to
+ * coupling variable 0 of its target subnetwork after the projections
+ * (runtime.rs:319-357). pattern: 0 impulse [onset_ms, amplitude], 1 step
+ * [onset_ms, offset_ms, amplitude], 2 sinusoid [onset_ms, amplitude, freq_hz,
+ * phase_rad], 3 pulse_train [onset_ms, amplitude, period_ms, width_ms]. */
+typedef struct {
+    int32_t target, pattern, n_params;
+    float params[4];
+} hb_stimulus;
+
 typedef struct {
     int32_t n_subnets;        /* 1..4, at most 1024 nodes together */
     const hb_subnetwork* subnets;
@@ -315,6 +325,14 @@ typedef struct {
     const float* state_in;
     const float* hist_in;
     float* hist_out;
+    /* plain = 1: the plain HybridEngine's semantics (runtime.rs:46-501): delays
+     * read the true ring (step - d), Scalar weights are w * I (construction.rs:
+     * 639-654), stimuli apply, the RK4-stochastic order of integrator.rs:296-315,
+     * no hybrid_integrator */
+    int32_t plain;
+    int32_t n_stimuli;        /* 0..8 */
+    const hb_stimulus* stimuli;
+    double dt_ms;             /* f64 dt for stimulus times (step * dt_ms) */
 } hb_multi_spec;
 
 /* BatchHybridEngine::run_sweep over several subnetworks joined by projections

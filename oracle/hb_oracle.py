@@ -7,6 +7,7 @@ baseline; the product path never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -313,8 +314,30 @@ def _f32sum(v):
     return np.cumsum(np.asarray(v, np.float32), dtype=np.float32)[-1] if len(v) else np.float32(0.0)
 
 
+def stimulus_value(pattern, params, step, dt_ms):
+    """StimulusApplier::apply (stimulus.rs:63-128): f64 time step * dt_ms."""
+    t = float(step) * float(dt_ms)
+    p = [float(np.float32(v)) for v in params]
+    onset = p[0]
+    if pattern == "impulse":
+        return np.float32(p[1]) if (abs(t - onset) < 1e-6 or (onset < t <= onset + 1e-6)) else np.float32(0.0)
+    if pattern == "step":
+        return np.float32(p[2]) if onset <= t < p[1] else np.float32(0.0)
+    if pattern == "sinusoid":
+        if t < onset:
+            return np.float32(0.0)
+        ph = p[3] if len(p) > 3 else 0.0
+        return np.float32(np.float32(p[1]) * np.float32(math.sin(2.0 * math.pi * p[2] * (t - onset) + ph)))
+    if pattern == "pulse_train":
+        period, width = p[2], p[3]
+        if t < onset or period <= 0.0 or width <= 0.0:
+            return np.float32(0.0)
+        return np.float32(p[1]) if math.fmod(t - onset, period) < width else np.float32(0.0)
+    raise ValueError(pattern)
+
+
 def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_sweep=1, max_delay=0, nsig=None,
-              noise=None):
+              noise=None, plain=False, stimuli=(), dt_ms=None):
     """The batch engine's multi-subnetwork loop (engine.rs:540-823) in f32, one
     sweep point at a time: coupling of every projection from the current
     states (engine.rs:557-673), then each subnetwork's integration with that
@@ -326,7 +349,11 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
     slot (step - d + H) % H. Stochastic forms (engine.rs:726-763): Z = raw *
     (sqrt(2 nsig_v) * sqrt(dt)) from noise[k][step] (one draw shared by every
     sweep point); Euler x + d dt + Z; Heun with Z in predictor and corrector;
-    RK4 clamp(det + Z). Returns per subnetwork tavg and final [n_sweep][n][nvar]."""
+    RK4 clamp(det + Z). plain: the plain HybridEngine (runtime.rs:46-501):
+    delays read the state of step - d, Scalar weights are w * I
+    (construction.rs:639-654), stimuli are added to coupling variable 0 of
+    their target (runtime.rs:319-357), RK4-stochastic is clamp(det) + Z
+    (integrator.rs:296-315). Returns per subnetwork tavg and final [n_sweep][n][nvar]."""
     from hyburn_amd import _lib as HL
     f32 = np.float32
     S = len(subnets)
@@ -340,6 +367,8 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
         cp = list(cp)
         ns, nt = subnets[src][1], subnets[tgt][1]
         W = None if np.isscalar(w) else np.array(w, np.float32)
+        if plain and W is None:
+            W = (np.eye(nt, ns) * np.float32(w)).astype(np.float32)
         if cfn == HL.CF_DIFFERENCE:
             a = cp[0] if cp else 1.0
             if W is None:
@@ -363,10 +392,12 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
             if param_idx == HL.SWEEP_INDEX[subnets[sub_idx][0]]:
                 sw[sub_idx] = 1
         hist = [[x[k].copy()] + [np.zeros_like(x[k]) for _ in range(H - 1)] for k in range(S)] if H else None
+        past = [dict() for _ in range(S)]
         for step in range(n_steps):
             if H:
                 for k in range(S):
                     hist[k] = hist[k][1:] + [x[k].copy()]
+                    past[k][step] = x[k].copy()
             cpl = [None] * S
             for (src, tgt, w, W, cfn, cp, cmap, delay) in prep:
                 ms, mt = subnets[src][0], subnets[tgt][0]
@@ -374,6 +405,8 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                 ns, nt = subnets[src][1], subnets[tgt][1]
                 if delay == 0 or step == 0:
                     xs = x[src][:, :ncs]
+                elif delay <= step and plain:
+                    xs = past[src][step - delay][:, :ncs]
                 elif delay <= step:
                     xs = hist[src][(step - delay + H) % H][:, :ncs]
                 else:
@@ -395,6 +428,15 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                         if sc < ncs and ci < nct:
                             rem[:, ci] = rem[:, ci] + out[:, sc]
                 cpl[tgt] = rem if cpl[tgt] is None else (cpl[tgt] + rem).astype(np.float32)
+            for (target, pattern, sp) in stimuli:
+                if 0 <= target < S:
+                    v = stimulus_value(pattern, sp, step, dt if dt_ms is None else dt_ms)
+                    if v != 0.0:
+                        base = cpl[target] if cpl[target] is not None else np.zeros((subnets[target][1], info[target][1]),
+                                                                                     np.float32)
+                        base = base.copy()
+                        base[:, 0] = (base[:, 0] + v).astype(np.float32)
+                        cpl[target] = base
             for k, (m, n, params, _) in enumerate(subnets):
                 c = cpl[k] if cpl[k] is not None else np.zeros((n, info[k][1]), np.float32)
                 heun = (m == HL.G2DO) if hybrid else (integrator == HL.HEUN)
@@ -419,7 +461,11 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                             k2 = dfun(m, params, clamp(m, xi + d * dth), ci, sw[k], sv)
                             k3 = dfun(m, params, clamp(m, xi + k2 * dth), ci, sw[k], sv)
                             k4 = dfun(m, params, clamp(m, xi + k3 * dtf), ci, sw[k], sv)
-                            xn = ((xi + (((d + k2 * f32(2.0)) + k3 * f32(2.0)) + k4) * dt6) + z).astype(np.float32)
+                            det = (xi + (((d + k2 * f32(2.0)) + k3 * f32(2.0)) + k4) * dt6).astype(np.float32)
+                            if plain:   # integrator.rs:296-315: clamp(det) + Z, no clamp after
+                                x[k][i] = (clamp(m, det) + z).astype(np.float32)
+                                continue
+                            xn = (det + z).astype(np.float32)
                         x[k][i] = clamp(m, xn)
                         continue
                     if heun:

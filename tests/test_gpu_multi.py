@@ -206,3 +206,24 @@ def test_engine_checkpoint_resume(gpu, tmp_path):
     for k in range(2):
         np.testing.assert_array_equal(r3.final_states[k], r_full.final_states[k])
         np.testing.assert_array_equal(r2.final_states[k], r_full.final_states[k])
+
+
+@pytest.mark.parametrize("integ", [L.HEUN, L.RK4_STOCH])
+def test_plain_engine_with_stimuli(gpu, orc, integ):
+    # the plain HybridEngine's multi-subnetwork step: true-ring delays, Scalar
+    # weights as w * I, the four stimulus patterns on coupling variable 0
+    rng = np.random.default_rng(integ + 50)
+    n, T = 5, 160
+    subs = [(L.G2DO, n, G2DO_DEFAULT, _x0(rng, n, 2)), (L.JANSEN_RIT, n, JR_DEFAULT, _x0(rng, n, 6, 0.01)),
+            (L.WILSON_COWAN, n, WC_DEFAULT, np.abs(_x0(rng, n, 2, 0.1)))]
+    links = [(0, 1, 0.05, L.CF_LINEAR, [1.0], [(0, 0)], 4), (1, 2, 0.05, L.CF_LINEAR, [1.0], [(0, 0)], 2),
+             (2, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 0)], 6)]
+    stims = [(0, "impulse", [2.0, 1.5]), (0, "sinusoid", [1.0, 0.2, 40.0, 0.1]), (1, "step", [3.0, 9.0, 0.5]),
+             (2, "pulse_train", [0.5, 0.3, 2.0, 0.7]), (7, "step", [0.0, 5.0, 1.0])]   # target 7 is skipped
+    raw = [rng.standard_normal((T, n, L.MODEL_INFO[m][0])).astype(np.float32) for (m, _, _, _) in subs]
+    nsig = np.full(6, 0.0005, np.float32)
+    res = run_multi([Subnet(m, k, p, x0) for (m, k, p, x0) in subs],
+                    [Link(s, t, w, cfn, cp, cm, dl) for (s, t, w, cfn, cp, cm, dl) in links],
+                    dt=0.1, n_steps=T, integrator=integ, plain=True, stimuli=stims, nsig=nsig, noise=raw)
+    tav, fin = orc.run_multi(subs, links, 0.1, T, integ, False, plain=True, stimuli=stims, nsig=nsig, noise=raw)
+    _check(res, tav, fin)

@@ -10,6 +10,8 @@ engine.rs:540-823) and checks the host-side validation of hb_run_multi.
   hand-built coupling [0, w * mean(V)] fed to the model's dfun (the remap of
   engine.rs:637-664 and target_cvar_cpl of 391-393).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -183,3 +185,50 @@ def test_checkpoint_round_trip_and_errors(tmp_path):
     (tmp_path / "v9.bin").write_bytes(raw[:8] + struct.pack("<Q", 9) + raw[16:])
     with pytest.raises(L.HyburnError, match="Unsupported checkpoint version"):
         ck.read(str(tmp_path / "v9.bin"))
+
+
+@pytest.mark.parametrize("integ,dense,d", [(L.HEUN, True, 3), (L.RK4_STOCH, False, 5), (L.EULER, False, 0)])
+def test_plain_single_subnet_composition(orc, integ, dense, d):
+    # plain mode with one subnetwork is the C restatement's HybridEngine run:
+    # true-ring delays (state of step - d), Scalar weights w * I, plain RK4-stochastic
+    rng = np.random.default_rng(40 + d)
+    n, T = 5, 40
+    x0 = _x0(rng, n, 2)
+    W = (rng.uniform(size=(n, n)) * 0.3).astype(np.float32)
+    raw = rng.standard_normal((T, n, 2)).astype(np.float32)
+    nsig = np.array([0.01, 0.01], np.float32)
+    link = (0, 0, W if dense else 0.3, L.CF_LINEAR, [0.8, 0.0], [(0, 0)], d)
+    tav, fin = orc.run_multi([(L.G2DO, n, G2DO_DEFAULT, x0)], [link], 0.1, T, integ, False, nsig=nsig, noise=[raw],
+                             plain=True)
+    inst = Instance(model=L.G2DO, n_nodes=n, n_steps=T, dt=0.1, params=np.array(G2DO_DEFAULT, np.float32), x0=x0,
+                    integrator=integ, engine=L.ENGINE_PLAIN, cfn=L.CF_LINEAR, cparams=(0.8, 0.0),
+                    wkind=L.W_DENSE if dense else L.W_SCALAR_DIAG, dense=W if dense else None,
+                    scalar_w=0.0 if dense else 0.3, delay_mode=L.DELAY_SINGLE, single_delay=d, nsig=nsig, noise=raw)
+    ref = orc.run(inst)
+    np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-6, atol=1e-7)
+
+
+def test_stimulus_values(orc):
+    # stimulus.rs:74-128 closed forms at dt = 0.5 ms
+    v = [orc.stimulus_value("impulse", [2.0, 3.0], s, 0.5) for s in range(8)]
+    assert v == [0, 0, 0, 0, 3.0, 0, 0, 0]
+    v = [orc.stimulus_value("step", [1.0, 2.5, 4.0], s, 0.5) for s in range(8)]
+    assert v == [0, 0, 4.0, 4.0, 4.0, 0, 0, 0]
+    v = [orc.stimulus_value("pulse_train", [1.0, 2.0, 1.5, 0.5], s, 0.5) for s in range(9)]
+    assert v == [0, 0, 2.0, 0, 0, 2.0, 0, 0, 2.0]
+    f = 40.0
+    for s in (0, 3, 7, 20):
+        t = s * 0.5
+        want = 0.0 if t < 1.0 else np.float32(2.0) * np.float32(math.sin(2 * math.pi * f * (t - 1.0) + 0.3))
+        assert orc.stimulus_value("sinusoid", [1.0, 2.0, f, 0.3], s, 0.5) == np.float32(want)
+
+
+def test_stimulus_enters_coupling_one_step(orc):
+    # a step stimulus on a subnetwork without projections: coupling = [stim]
+    rng = np.random.default_rng(6)
+    x0 = _x0(rng, 3, 2)
+    _, fin = orc.run_multi([(L.G2DO, 3, G2DO_DEFAULT, x0)], [], 0.1, 1, L.EULER, False, plain=True,
+                           stimuli=[(0, "step", [0.0, 10.0, 0.7])])
+    for i in range(3):
+        dd = orc.dfun(L.G2DO, G2DO_DEFAULT, x0[i], np.array([0.7], np.float32))
+        np.testing.assert_array_equal(fin[0][0, i], orc.clamp(L.G2DO, (x0[i] + dd * np.float32(0.1)).astype(np.float32)))
