@@ -144,7 +144,7 @@ class HbMultiSpec(C.Structure):
                 ("nsig", fp), ("noise", fp), ("device", C.c_int32), ("tavg", fp),
                 ("final_state", fp), ("kernel_ms", C.c_double), ("step0", C.c_int64), ("state_in", fp),
                 ("hist_in", fp), ("hist_out", fp), ("plain", C.c_int32), ("n_stimuli", C.c_int32),
-                ("stimuli", C.POINTER(HbStimulus)), ("dt_ms", C.c_double)]
+                ("stimuli", C.POINTER(HbStimulus)), ("dt_ms", C.c_double), ("trajectory", fp)]
 
 
 # Every exported function of include/hyburn_hip.h with its prototype.

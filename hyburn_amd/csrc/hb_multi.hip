@@ -90,6 +90,7 @@ struct Args {
     const float* x0;       // [n_tot][kNV]
     float* tavg;           // [B][n_tot][kNV]
     float* fin;            // [B][n_tot][kNV]
+    float* traj;           // [n_steps][B][n_tot][kNV] or null
 };
 
 __device__ __forceinline__ void pre_rt(const Proj& P, const float* x, float* out) {
@@ -256,6 +257,10 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
             step_model(model, S, mc, cpl, z, x);
 #pragma unroll
             for (int v = 0; v < kNV; ++v) acc[v] += x[v];
+            if (A->traj) {
+                float* o = A->traj + (((size_t)k * gridDim.x + b) * A->n_tot + t) * kNV;
+                for (int v = 0; v < nv; ++v) o[v] = x[v];
+            }
         }
         __syncthreads();
     }
@@ -520,6 +525,13 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     if (e == hipSuccess) e = hipMemcpy(dx0, x0.data(), sizeof(float) * x0.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && dsw) e = hipMemcpy(dsw, Q.sweep_values, sizeof(float) * Q.n_sweep, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "multi-subnetwork upload");
+    float* dtraj = nullptr;
+    const size_t ntraj = (size_t)Q.n_steps * nout;
+    if (Q.trajectory) {
+        e = B.alloc(&dtraj, ntraj);
+        if (e != hipSuccess) return hip_fail(e, "trajectory buffer");
+    }
+    A.traj = dtraj;
     A.x0 = dx0;
     A.sweep = dsw;
     A.hist = dhist;
@@ -566,6 +578,20 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     if (e == hipSuccess && H > 0 && Q.hist_out)
         e = hipMemcpy(Q.hist_out, dhist, sizeof(float) * nhist, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "multi-subnetwork download");
+    if (Q.trajectory && ntraj) {   // per subnetwork [n_steps][n_sweep][n][nv], concatenated
+        std::vector<float> tr(ntraj);
+        e = hipMemcpy(tr.data(), dtraj, sizeof(float) * ntraj, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "trajectory download");
+        size_t o = 0;
+        for (int q = 0; q < Q.n_subnets; ++q) {
+            const Sub& S = A.sub[q];
+            for (int st = 0; st < Q.n_steps; ++st)
+                for (int b = 0; b < Q.n_sweep; ++b)
+                    for (int r = 0; r < S.n; ++r)
+                        for (int v = 0; v < S.nv; ++v, ++o)
+                            Q.trajectory[o] = tr[(((size_t)st * Q.n_sweep + b) * n_tot + S.off + r) * kNV + v];
+        }
+    }
     // per subnetwork, [n_sweep][n_nodes][nvar] (the BatchSweepResult layout), concatenated
     size_t o = 0;
     for (int q = 0; q < Q.n_subnets; ++q) {

@@ -465,7 +465,7 @@ class BatchHybridEngine:
                                      int(n_steps), flags, C.byref(r)))
         return self._result(r, v.tolist(), traj)
 
-    def _run_multi(self, param: SweepParam, values, n_steps: int) -> BatchSweepResult:
+    def _run_multi(self, param: SweepParam, values, n_steps: int, traj: bool = False) -> BatchSweepResult:
         from .multi import run_multi
         subs, links, integ, device = self._multi
         v = np.ascontiguousarray(values, np.float32)
@@ -480,11 +480,13 @@ class BatchHybridEngine:
                       noise_seed=self._noise_seed + self._step,
                       hybrid_integrator=self.hybrid_integrator, n_sweep=self.n_sweep,
                       sweep=(param.sub_idx, param.param_idx, v), max_delay=int(self._max_delay),
-                      step0=self._step, state=self._state, hist=self._hist, device=device)
+                      step0=self._step, state=self._state, hist=self._hist, record_trajectory=traj,
+                      device=device)
         ms = (time.perf_counter() - t0) * 1e3
         self._step, self._state, self._hist = r.step_end, r.final_states, r.hist
         return BatchSweepResult(param_values=v.tolist(), tavg=[t.ravel() for t in r.tavg],
-                                final_states=[f.ravel() for f in r.final_states], trajectories=None, bold=None,
+                                final_states=[f.ravel() for f in r.final_states],
+                                trajectories=[t.ravel() for t in r.trajectories] if traj else None, bold=None,
                                 elapsed_ms=ms, n_sweep=self.n_sweep, kernel_ms=r.kernel_ms)
 
     def _hybrid_changes_rule(self) -> bool:
@@ -511,7 +513,7 @@ class BatchHybridEngine:
     def run_sweep_with_trajectory(self, param: SweepParam, param_values, n_steps: int,
                                   exact: bool = False) -> BatchSweepResult:
         if self._multi is not None:
-            _err(L.HB_INVALID_CONFIG, "trajectories are not recorded on the multi-subnetwork path")
+            return self._run_multi(param, param_values, n_steps, traj=True)
         return self._run(param, param_values, n_steps, True, exact)
 
     def run_sweep_multi(self, spec: SweepSpec, n_steps: int, exact: bool = False,

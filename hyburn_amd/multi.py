@@ -48,6 +48,7 @@ class MultiResult:
     final_states: list = field(default_factory=list)
     kernel_ms: float = 0.0
     hist: Optional[np.ndarray] = None     # [n_sweep][total nodes][H][2] by step % H (delayed runs)
+    trajectories: Optional[list] = None   # per subnetwork [n_steps][n_sweep][n_nodes][nvar]
     step_end: int = 0                     # the engine's step counter after the run
 
 
@@ -70,7 +71,7 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
               sweep: Optional[tuple] = None, max_delay: int = 0, nsig=None, noise=None,
               noise_seed: int = 0, step0: int = 0, state: Optional[list] = None,
               hist: Optional[np.ndarray] = None, plain: bool = False, stimuli: Sequence[tuple] = (),
-              dt_ms: Optional[float] = None, device: int = 0) -> MultiResult:
+              dt_ms: Optional[float] = None, record_trajectory: bool = False, device: int = 0) -> MultiResult:
     """max_delay: the largest entry of any projection's delay list, which sets
     the history horizon (engine.rs:253-257); each link's own delay counts too.
     Stochastic integrators: nsig per variable (a scalar is repeated over the
@@ -169,11 +170,20 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
                 raise ValueError(f"hist must be {hist_out.shape}")
             keep.append(hi)
             spec.hist_in = hi.ctypes.data_as(L.fp)
+    traj = np.zeros(sum(sizes) * n_steps, np.float32) if record_trajectory else None
+    if traj is not None:
+        spec.trajectory = traj.ctypes.data_as(L.fp)
     tavg = np.zeros(sum(sizes), np.float32)
     fin = np.zeros(sum(sizes), np.float32)
     spec.tavg, spec.final_state = tavg.ctypes.data_as(L.fp), fin.ctypes.data_as(L.fp)
     L.check(L.lib().hb_run_multi(spec))
     out = MultiResult(kernel_ms=spec.kernel_ms, hist=hist_out, step_end=int(step0) + int(n_steps))
+    if traj is not None:
+        out.trajectories, o = [], 0
+        for S, n in zip(subnets, sizes):
+            out.trajectories.append(traj[o:o + n * n_steps].reshape(n_steps, n_sweep, S.n_nodes,
+                                                                    L.MODEL_INFO[S.model][0]).copy())
+            o += n * n_steps
     o = 0
     for S, n in zip(subnets, sizes):
         shape = (n_sweep, S.n_nodes, L.MODEL_INFO[S.model][0])

@@ -333,6 +333,8 @@ typedef struct {
     int32_t n_stimuli;        /* 0..8 */
     const hb_stimulus* stimuli;
     double dt_ms;             /* f64 dt for stimulus times (step * dt_ms) */
+    float* trajectory;        /* optional out: per subnetwork [n_steps][n_sweep][n_nodes][nvar], concatenated
+                                 (run_sweep_with_trajectory, engine.rs:779-783) */
 } hb_multi_spec;
 
 /* BatchHybridEngine::run_sweep over several subnetworks joined by projections

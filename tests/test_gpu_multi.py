@@ -227,3 +227,24 @@ def test_plain_engine_with_stimuli(gpu, orc, integ):
                     dt=0.1, n_steps=T, integrator=integ, plain=True, stimuli=stims, nsig=nsig, noise=raw)
     tav, fin = orc.run_multi(subs, links, 0.1, T, integ, False, plain=True, stimuli=stims, nsig=nsig, noise=raw)
     _check(res, tav, fin)
+
+
+def test_trajectory_recording(gpu, orc):
+    # run_sweep_with_trajectory's layout, per subnetwork [n_steps][n_sweep][n][nvar]
+    # (engine.rs:779-783): row k is the state after step k
+    subs, links = _ring(5)
+    vals = np.array([-0.3, 0.3], np.float32)
+    kw = dict(dt=0.1, integrator=L.HEUN, hybrid_integrator=True, n_sweep=2, sweep=(0, 1, vals))
+    full = run_multi([Subnet(m, n, p, x0) for (m, n, p, x0) in subs],
+                     [Link(s, t, w, cfn, cp, cm) for (s, t, w, cfn, cp, cm) in links], n_steps=120,
+                     record_trajectory=True, **kw)
+    part = run_multi([Subnet(m, n, p, x0) for (m, n, p, x0) in subs],
+                     [Link(s, t, w, cfn, cp, cm) for (s, t, w, cfn, cp, cm) in links], n_steps=50, **kw)
+    tav, fin = orc.run_multi(subs, links, 0.1, 50, L.HEUN, True, sweep=(0, 1, vals), n_sweep=2)
+    for k in range(3):
+        tr = full.trajectories[k]
+        assert tr.shape == (120, 2, 5, L.MODEL_INFO[subs[k][0]][0])
+        np.testing.assert_array_equal(tr[-1], full.final_states[k])
+        np.testing.assert_array_equal(tr[49], part.final_states[k])
+        np.testing.assert_allclose(tr[49], fin[k], rtol=RTOL, atol=ATOL)
+        np.testing.assert_allclose(tr.mean(axis=0), full.tavg[k], rtol=1e-4, atol=1e-5)
