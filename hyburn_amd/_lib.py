@@ -144,7 +144,8 @@ class HbMultiSpec(C.Structure):
                 ("nsig", fp), ("noise", fp), ("device", C.c_int32), ("tavg", fp),
                 ("final_state", fp), ("kernel_ms", C.c_double), ("step0", C.c_int64), ("state_in", fp),
                 ("hist_in", fp), ("hist_out", fp), ("plain", C.c_int32), ("n_stimuli", C.c_int32),
-                ("stimuli", C.POINTER(HbStimulus)), ("dt_ms", C.c_double), ("trajectory", fp)]
+                ("stimuli", C.POINTER(HbStimulus)), ("dt_ms", C.c_double), ("trajectory", fp),
+                ("bold_period", C.c_int32), ("bold_tr_ms", C.c_double), ("bold", fp)]
 
 
 # Every exported function of include/hyburn_hip.h with its prototype.
@@ -179,6 +180,7 @@ PROTOTYPES = {
     "hb_fcd_windows": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
     "hb_fcd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
     "hb_run_multi": (C.c_int, [C.POINTER(HbMultiSpec)]),
+    "hb_bold_rows": (C.c_int32, [C.c_int32, C.c_int32, C.c_double, C.c_double]),
     "hb_welch_bins": (C.c_int32, [C.c_int32, C.c_int32]),
     "hb_welch_psd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                      C.POINTER(C.c_double), C.c_int32]),

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <string>
 
 #include "../../include/hyburn_hip.h"
@@ -198,6 +199,30 @@ struct LaunchDesc {
     bool lds_ring;             // rings live in LDS
     bool multi;                // nodes per thread > 1 (state table in LDS)
 };
+
+struct BoldShape {
+    int native = 1;
+    int nout = 0;
+    double dtb = 0.0;
+};
+
+// Number of BOLD rows a run produces, bold_monitor.rs:72-227 (window times in
+// f64, TR resampling count floor(max_t / TR)).
+inline BoldShape bold_shape(int T, int P, double dt_ms, double tr_ms) {
+    BoldShape b;
+    if (P <= 0) return b;
+    b.dtb = dt_ms * (double)P;
+    const int n_full = T / P, rem = T % P;
+    const int n_native = n_full + (rem > 0 ? 1 : 0);
+    b.native = (tr_ms <= 0.0 || std::fabs(tr_ms - b.dtb) < 1e-6) ? 1 : 0;
+    if (b.native) { b.nout = n_native; return b; }
+    if (n_native == 0) { b.nout = 0; return b; }
+    double max_t;
+    if (rem > 0) max_t = (double)n_full * b.dtb + dt_ms * (double)rem;
+    else max_t = (double)(n_full - 1) * b.dtb + b.dtb;
+    b.nout = max_t >= tr_ms ? (int)std::floor(max_t / tr_ms) : 0;
+    return b;
+}
 
 // Record the thread-local last-error message (hb_last_error) and return code.
 int set_error(int code, const std::string& msg);

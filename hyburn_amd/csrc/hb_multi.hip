@@ -91,6 +91,9 @@ struct Args {
     float* tavg;           // [B][n_tot][kNV]
     float* fin;            // [B][n_tot][kNV]
     float* traj;           // [n_steps][B][n_tot][kNV] or null
+    float* bold;           // [B][bnout][n of subnetwork 0] or null
+    int32_t bper, bnout, bnative;
+    double tr_ms, dtb_ms;
 };
 
 __device__ __forceinline__ void pre_rt(const Proj& P, const float* x, float* out) {
@@ -127,6 +130,44 @@ __device__ __forceinline__ void step_model(int model, const StepCtx& S, const fl
     }
 }
 
+// BOLD output rows after native sample bk (time tk, value sig), the stream
+// form of bold_monitor.rs:123-203: native samples as they are, or TR rows
+// interpolated linearly between the previous sample and this one.
+struct BoldClock {
+    int k, j;
+    double tprev;
+};
+
+__device__ __attribute__((noinline)) void bold_emit(const Args* A, const BoldClock& c, double tk, float sig, float prev, float* out, int N0) {
+    if (A->bnative) {
+        if (c.k < A->bnout) out[(size_t)c.k * N0] = sig;
+        return;
+    }
+    const double tr = A->tr_ms;
+    int j = c.j;
+    if (c.k == 0) {
+        while (j < A->bnout && (double)(j + 1) * tr <= tk) { out[(size_t)j * N0] = sig; ++j; }
+    } else {
+        while (j < A->bnout && (double)(j + 1) * tr < tk) {
+            const double tt = (double)(j + 1) * tr, d = tk - c.tprev;
+            double w = fabs(d) < 1e-12 ? 0.0 : (tt - c.tprev) / d;
+            w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+            out[(size_t)j * N0] = prev + (sig - prev) * (float)w;
+            ++j;
+        }
+    }
+}
+
+__device__ __attribute__((noinline)) void bold_advance(const Args* A, BoldClock& c, double tk) {
+    if (!A->bnative) {
+        const double tr = A->tr_ms;
+        if (c.k == 0) { while (c.j < A->bnout && (double)(c.j + 1) * tr <= tk) c.j++; }
+        else { while (c.j < A->bnout && (double)(c.j + 1) * tr < tk) c.j++; }
+    }
+    c.k++;
+    c.tprev = tk;
+}
+
 __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     extern __shared__ float pv[];   // pre values of every projection's source nodes
     __shared__ float gmean[kMaxProj * 4];   // w * mean over source nodes, per Scalar projection and channel
@@ -160,6 +201,16 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     S.sv = A->sweep ? A->sweep[b] : 0.0f;
     const int H = A->H;
     float* hist = (live && H > 0) ? A->hist + ((size_t)b * A->n_tot + t) * H * 2 : nullptr;
+    // BOLD of subnetwork 0 (bold_monitor.rs:72-106): window sums of variable 0,
+    // one Balloon-Windkessel step per closed window; the clock is uniform
+    const int bper = A->bold ? A->bper : 0, N0 = A->sub[0].n;
+    const bool bnode = live && s == 0 && bper > 0;
+    float* bout = bnode ? A->bold + (size_t)b * A->bnout * N0 + i : nullptr;
+    Bold bw;
+    bw.s = 0.0f; bw.f = 1.0f; bw.v = 1.0f; bw.q = 1.0f; bw.acc = 0.0f; bw.prev = 0.0f;
+    BoldClock bc;
+    bc.k = 0; bc.j = 0; bc.tprev = 0.0;
+    int bcount = 0;
     for (int k = 0; k < A->n_steps; ++k) {
         const int64_t step = A->step0 + k;   // the engine's step counter (delays read by it)
         // the history window gains the current state (engine.rs:541-551); each
@@ -261,8 +312,37 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                 float* o = A->traj + (((size_t)k * gridDim.x + b) * A->n_tot + t) * kNV;
                 for (int v = 0; v < nv; ++v) o[v] = x[v];
             }
+            if (bnode) bw.acc = bw.acc + x[0];
+        }
+        if (bper > 0 && ++bcount >= bper) {
+            const double dtb = A->dt_ms * (double)bcount;
+            const float dts = (float)(dtb / 1000.0);
+            const double tk = __dadd_rn(__dmul_rn((double)bc.k, dtb), dtb);
+            if (bnode) {
+                const float sig = bold_step(bw, bw.acc / (float)bcount, dts);
+                bold_emit(A, bc, tk, sig, bw.prev, bout, N0);
+                bw.prev = sig;
+                bw.acc = 0.0f;
+            }
+            bold_advance(A, bc, tk);
+            bcount = 0;
         }
         __syncthreads();
+    }
+    if (bper > 0) {   // flush of a partial window, then the last value on the remaining TR rows
+        if (bcount > 0) {
+            const double dtb = A->dt_ms * (double)bcount;
+            const float dts = (float)(dtb / 1000.0);
+            const double tk = __dadd_rn(__dmul_rn((double)bc.k, A->dtb_ms), dtb);
+            if (bnode) {
+                const float sig = bold_step(bw, bw.acc / (float)bcount, dts);
+                bold_emit(A, bc, tk, sig, bw.prev, bout, N0);
+                bw.prev = sig;
+            }
+            bold_advance(A, bc, tk);
+        }
+        if (bnode && !A->bnative && bc.k > 0)
+            for (int jj = bc.j; jj < A->bnout; ++jj) bout[(size_t)jj * N0] = bw.prev;
     }
     if (live) {
         const size_t o = ((size_t)b * A->n_tot + t) * kNV;
@@ -532,6 +612,20 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         if (e != hipSuccess) return hip_fail(e, "trajectory buffer");
     }
     A.traj = dtraj;
+    const BoldShape bs = bold_shape(Q.n_steps, Q.bold_period, Q.dt_ms, Q.bold_tr_ms);
+    float* dbold = nullptr;
+    const size_t nbold = (size_t)Q.n_sweep * bs.nout * Q.subnets[0].n_nodes;
+    if (Q.bold && Q.bold_period > 0) {
+        if (Q.dt_ms <= 0.0) return bad("BOLD needs dt_ms > 0");
+        e = B.alloc(&dbold, nbold);
+        if (e != hipSuccess) return hip_fail(e, "BOLD buffer");
+        A.bold = dbold;
+        A.bper = Q.bold_period;
+        A.bnout = bs.nout;
+        A.bnative = bs.native;
+        A.tr_ms = Q.bold_tr_ms;
+        A.dtb_ms = bs.dtb;
+    }
     A.x0 = dx0;
     A.sweep = dsw;
     A.hist = dhist;
@@ -578,6 +672,10 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     if (e == hipSuccess && H > 0 && Q.hist_out)
         e = hipMemcpy(Q.hist_out, dhist, sizeof(float) * nhist, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "multi-subnetwork download");
+    if (dbold && nbold) {
+        e = hipMemcpy(Q.bold, dbold, sizeof(float) * nbold, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "BOLD download");
+    }
     if (Q.trajectory && ntraj) {   // per subnetwork [n_steps][n_sweep][n][nv], concatenated
         std::vector<float> tr(ntraj);
         e = hipMemcpy(tr.data(), dtraj, sizeof(float) * ntraj, hipMemcpyDeviceToHost);
@@ -605,4 +703,9 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
                 }
     }
     return HB_OK;
+}
+
+extern "C" int32_t hb_bold_rows(int32_t n_steps, int32_t bold_period, double dt_ms, double tr_ms) {
+    if (n_steps < 0 || bold_period <= 0) return 0;
+    return hb::bold_shape(n_steps, bold_period, dt_ms, tr_ms).nout;
 }

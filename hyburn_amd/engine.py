@@ -356,6 +356,7 @@ class BatchHybridEngine:
         self._max_delay = 0
         self._nsig, self._noise_seed = 0.0, 0
         self._step, self._state, self._hist = 0, None, None   # carried across run_sweep calls
+        self._bold = None
         self.hybrid_integrator = False
 
     @classmethod
@@ -367,9 +368,13 @@ class BatchHybridEngine:
             self = cls()
             if cfg.integrator not in L.INTEGRATOR_NAMES:
                 _err(L.HB_INVALID_CONFIG, f"unknown integrator {cfg.integrator}")
-            if any(m.monitor_type.lower() == "bold" for m in cfg.monitors):
-                _err(L.HB_INVALID_CONFIG, "BOLD monitors are not supported on the multi-subnetwork path")
             subs, links = _multi_from_config(cfg)
+            bold = [m for m in cfg.monitors if m.monitor_type.lower() == "bold"]
+            if bold:   # engine.rs:418-447: subnetwork 0, windows of bold_period steps
+                m = bold[0]
+                self._bold = (int(m.bold_period if m.bold_period is not None
+                                  else max(1, round((m.period or 2000.0) / cfg.dt))),
+                              float(m.tr if m.tr is not None else 2.0) * 1000.0)
             self._multi = (subs, links, L.INTEGRATOR_NAMES[cfg.integrator], device)
             self._max_delay = max([max(p.delays) for p in cfg.projections if p.delays] or [0])
             self._nsig, self._noise_seed = cfg.nsig, noise_seed
@@ -481,12 +486,14 @@ class BatchHybridEngine:
                       hybrid_integrator=self.hybrid_integrator, n_sweep=self.n_sweep,
                       sweep=(param.sub_idx, param.param_idx, v), max_delay=int(self._max_delay),
                       step0=self._step, state=self._state, hist=self._hist, record_trajectory=traj,
-                      device=device)
+                      bold_period=self._bold[0] if self._bold else 0,
+                      bold_tr_ms=self._bold[1] if self._bold else 0.0, device=device)
         ms = (time.perf_counter() - t0) * 1e3
         self._step, self._state, self._hist = r.step_end, r.final_states, r.hist
         return BatchSweepResult(param_values=v.tolist(), tavg=[t.ravel() for t in r.tavg],
                                 final_states=[f.ravel() for f in r.final_states],
-                                trajectories=[t.ravel() for t in r.trajectories] if traj else None, bold=None,
+                                trajectories=[t.ravel() for t in r.trajectories] if traj else None,
+                                bold=[r.bold] if r.bold is not None else None,
                                 elapsed_ms=ms, n_sweep=self.n_sweep, kernel_ms=r.kernel_ms)
 
     def _hybrid_changes_rule(self) -> bool:

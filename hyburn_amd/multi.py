@@ -49,6 +49,7 @@ class MultiResult:
     kernel_ms: float = 0.0
     hist: Optional[np.ndarray] = None     # [n_sweep][total nodes][H][2] by step % H (delayed runs)
     trajectories: Optional[list] = None   # per subnetwork [n_steps][n_sweep][n_nodes][nvar]
+    bold: Optional[np.ndarray] = None     # subnetwork 0: [n_sweep][rows][n_nodes]
     step_end: int = 0                     # the engine's step counter after the run
 
 
@@ -71,7 +72,8 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
               sweep: Optional[tuple] = None, max_delay: int = 0, nsig=None, noise=None,
               noise_seed: int = 0, step0: int = 0, state: Optional[list] = None,
               hist: Optional[np.ndarray] = None, plain: bool = False, stimuli: Sequence[tuple] = (),
-              dt_ms: Optional[float] = None, record_trajectory: bool = False, device: int = 0) -> MultiResult:
+              dt_ms: Optional[float] = None, record_trajectory: bool = False, bold_period: int = 0,
+              bold_tr_ms: float = 0.0, device: int = 0) -> MultiResult:
     """max_delay: the largest entry of any projection's delay list, which sets
     the history horizon (engine.rs:253-257); each link's own delay counts too.
     Stochastic integrators: nsig per variable (a scalar is repeated over the
@@ -85,7 +87,10 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
     plain: the plain HybridEngine's semantics (true-ring delays, Scalar
     weights as w * I, stimuli); stimuli: (target, pattern, params) with
     pattern one of impulse, step, sinusoid, pulse_train (stimulus.rs), at
-    times step * dt_ms (default dt)."""
+    times step * dt_ms (default dt).
+    bold_period > 0: a BOLD monitor on subnetwork 0, one series per sweep
+    point, windows of bold_period steps resampled to bold_tr_ms (<= 0: the
+    native window samples), bold_monitor.rs:72-227."""
     keep = []
     subs = (L.HbSubnetwork * len(subnets))()
     for k, S in enumerate(subnets):
@@ -170,6 +175,11 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
                 raise ValueError(f"hist must be {hist_out.shape}")
             keep.append(hi)
             spec.hist_in = hi.ctypes.data_as(L.fp)
+    bold = None
+    if bold_period > 0:
+        rows = int(L.lib().hb_bold_rows(n_steps, bold_period, spec.dt_ms, bold_tr_ms))
+        bold = np.zeros((n_sweep, rows, subnets[0].n_nodes), np.float32)
+        spec.bold_period, spec.bold_tr_ms, spec.bold = bold_period, bold_tr_ms, bold.ctypes.data_as(L.fp)
     traj = np.zeros(sum(sizes) * n_steps, np.float32) if record_trajectory else None
     if traj is not None:
         spec.trajectory = traj.ctypes.data_as(L.fp)
@@ -177,7 +187,7 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
     fin = np.zeros(sum(sizes), np.float32)
     spec.tavg, spec.final_state = tavg.ctypes.data_as(L.fp), fin.ctypes.data_as(L.fp)
     L.check(L.lib().hb_run_multi(spec))
-    out = MultiResult(kernel_ms=spec.kernel_ms, hist=hist_out, step_end=int(step0) + int(n_steps))
+    out = MultiResult(kernel_ms=spec.kernel_ms, hist=hist_out, step_end=int(step0) + int(n_steps), bold=bold)
     if traj is not None:
         out.trajectories, o = [], 0
         for S, n in zip(subnets, sizes):

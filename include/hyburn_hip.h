@@ -335,7 +335,17 @@ typedef struct {
     double dt_ms;             /* f64 dt for stimulus times (step * dt_ms) */
     float* trajectory;        /* optional out: per subnetwork [n_steps][n_sweep][n_nodes][nvar], concatenated
                                  (run_sweep_with_trajectory, engine.rs:779-783) */
+    /* BOLD monitor of subnetwork 0 (the batch engine's monitor target, engine.rs:418-447),
+     * per sweep point: bold_period > 0 steps per window, TR bold_tr_ms (<= 0: native
+     * window samples); bold receives [n_sweep][hb_bold_rows(...)][n_nodes of subnetwork 0] */
+    int32_t bold_period;
+    double bold_tr_ms;
+    float* bold;
 } hb_multi_spec;
+
+/* Rows of BOLD output a run of n_steps produces (bold_monitor.rs:72-227): native
+ * windows when tr_ms <= 0 or equal to the window length, else floor(last time / TR). */
+int32_t hb_bold_rows(int32_t n_steps, int32_t bold_period, double dt_ms, double tr_ms);
 
 /* BatchHybridEngine::run_sweep over several subnetworks joined by projections
  * (src/engine/batch_engine/engine.rs:540-823): per step, each projection's

@@ -336,8 +336,40 @@ def stimulus_value(pattern, params, step, dt_ms):
     raise ValueError(pattern)
 
 
+def bold_series(v0, P, dt_ms, tr_ms):
+    """BOLD of a [T][N] series of variable 0 (bold_monitor.rs:72-227): windows
+    of P steps averaged in f32, one Balloon-Windkessel step per window
+    (dt_s = window ms / 1000), a partial last window, then TR resampling
+    unless TR is <= 0 or the window length."""
+    from types import SimpleNamespace
+    v0 = np.asarray(v0, np.float32)
+    T, N = v0.shape
+    S, F, V, Q = (np.zeros(N, np.float32), np.ones(N, np.float32), np.ones(N, np.float32), np.ones(N, np.float32))
+    acc, cnt, native = np.zeros(N, np.float32), 0, []
+
+    def close(cnt):
+        z = (acc / np.float32(cnt)).astype(np.float32)
+        lib().hb_orc_bold_step_f32(N, _p(z), np.float32((dt_ms * cnt) / 1000.0), _p(S), _p(F), _p(V), _p(Q))
+        sig = np.zeros(N, np.float32)
+        lib().hb_orc_bold_signal_f32(N, _p(V), _p(Q), _p(sig))
+        native.append(sig)
+
+    for t in range(T):
+        acc = (acc + v0[t]).astype(np.float32)
+        cnt += 1
+        if cnt >= P:
+            close(cnt)
+            acc, cnt = np.zeros(N, np.float32), 0
+    if cnt > 0:
+        close(cnt)
+    nat = np.array(native, np.float32).reshape(-1, N)
+    if tr_ms <= 0.0 or abs(tr_ms - dt_ms * P) < 1e-6:
+        return nat
+    return bold_resample(nat, SimpleNamespace(dt=dt_ms, dt_ms=dt_ms, bold_period=P, n_steps=T, bold_tr_ms=tr_ms))
+
+
 def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_sweep=1, max_delay=0, nsig=None,
-              noise=None, plain=False, stimuli=(), dt_ms=None):
+              noise=None, plain=False, stimuli=(), dt_ms=None, bold_period=0, bold_tr_ms=0.0):
     """The batch engine's multi-subnetwork loop (engine.rs:540-823) in f32, one
     sweep point at a time: coupling of every projection from the current
     states (engine.rs:557-673), then each subnetwork's integration with that
@@ -353,7 +385,9 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
     delays read the state of step - d, Scalar weights are w * I
     (construction.rs:639-654), stimuli are added to coupling variable 0 of
     their target (runtime.rs:319-357), RK4-stochastic is clamp(det) + Z
-    (integrator.rs:296-315). Returns per subnetwork tavg and final [n_sweep][n][nvar]."""
+    (integrator.rs:296-315). bold_period > 0: BOLD of subnetwork 0's variable
+    0 per sweep point (bold_series). Returns per subnetwork tavg and final
+    [n_sweep][n][nvar], plus the BOLD [n_sweep][rows][n0] when monitored."""
     from hyburn_amd import _lib as HL
     f32 = np.float32
     S = len(subnets)
@@ -381,7 +415,9 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
     tav = [np.zeros((n_sweep, n, info[k][0]), np.float32) for k, (_, n, _, _) in enumerate(subnets)]
     fin = [np.zeros_like(t) for t in tav]
     dtf, dth, dt6 = f32(dt), f32(f32(dt) * f32(0.5)), f32(f32(dt) / f32(6.0))
+    bold_out = []
     for b in range(n_sweep):
+        v0 = []
         x = [np.array(x0, np.float32).reshape(n, info[k][0]) for k, (_, n, _, x0) in enumerate(subnets)]
         acc = [np.zeros_like(v) for v in x]
         sw = [0] * S
@@ -481,10 +517,16 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                         xn = (xi + d * dtf).astype(np.float32)
                     x[k][i] = clamp(m, xn)
                 acc[k] = (acc[k] + x[k]).astype(np.float32)
+            v0.append(x[0][:, 0].copy())
+        if bold_period > 0:
+            bold_out.append(bold_series(np.array(v0, np.float32).reshape(n_steps, subnets[0][1]), bold_period,
+                                        dt if dt_ms is None else dt_ms, bold_tr_ms))
         inv = f32(1.0) / f32(n_steps)
         for k in range(S):
             tav[k][b] = acc[k] * inv
             fin[k][b] = x[k]
+    if bold_period > 0:
+        return tav, fin, np.array(bold_out, np.float32)
     return tav, fin
 
 

@@ -248,3 +248,23 @@ def test_trajectory_recording(gpu, orc):
         np.testing.assert_array_equal(tr[49], part.final_states[k])
         np.testing.assert_allclose(tr[49], fin[k], rtol=RTOL, atol=ATOL)
         np.testing.assert_allclose(tr.mean(axis=0), full.tavg[k], rtol=1e-4, atol=1e-5)
+
+
+@pytest.mark.parametrize("P,tr", [(10, 0.0), (7, 2.0), (25, 3.3)])
+def test_bold_of_subnetwork_zero(gpu, orc, P, tr):
+    # per sweep point, BOLD of subnetwork 0's variable 0 (the batch engine's
+    # monitor target); windows, flush of a partial window, TR resampling
+    rng = np.random.default_rng(P)
+    subs, links = _ring(6)
+    subs[0] = (L.G2DO, 6, G2DO_DEFAULT, _x0(rng, 6, 2))
+    vals = np.array([-0.2, 0.4], np.float32)
+    T = 233
+    res = run_multi([Subnet(m, n, p, x0) for (m, n, p, x0) in subs],
+                    [Link(s, t, w, cfn, cp, cm) for (s, t, w, cfn, cp, cm) in links], dt=0.1, n_steps=T,
+                    integrator=L.HEUN, hybrid_integrator=True, n_sweep=2, sweep=(0, 1, vals), bold_period=P,
+                    bold_tr_ms=tr)
+    tav, fin, bold = orc.run_multi(subs, links, 0.1, T, L.HEUN, True, sweep=(0, 1, vals), n_sweep=2,
+                                   bold_period=P, bold_tr_ms=tr)
+    assert res.bold.shape == bold.shape and bold.shape[1] > 0
+    np.testing.assert_allclose(res.bold, bold, rtol=RTOL, atol=ATOL)
+    _check(res, tav, fin)

@@ -104,9 +104,10 @@ def test_engine_routes_multi_configs():
     cfg = SimConfig(sim_length=10.0, dt=0.1, subnetworks=subs, projections=projs, integrator="heun")
     eng = BatchHybridEngine.from_config(cfg, 4)
     assert eng._multi is not None and len(eng._multi[0]) == 2
-    cfg.monitors = [MonitorConfig(monitor_type="Bold")]
-    with pytest.raises(L.HyburnError, match="BOLD"):
-        BatchHybridEngine.from_config(cfg, 4)
+    # a BOLD monitor watches subnetwork 0 (engine.rs:418-447); TR in s -> ms
+    cfg.monitors = [MonitorConfig(monitor_type="Bold", bold_period=5, tr=0.002)]
+    eng = BatchHybridEngine.from_config(cfg, 4)
+    assert eng._bold == (5, 2.0)
 
 
 @pytest.mark.parametrize("d,dense", [(1, False), (4, True), (9, False)])
@@ -232,3 +233,21 @@ def test_stimulus_enters_coupling_one_step(orc):
     for i in range(3):
         dd = orc.dfun(L.G2DO, G2DO_DEFAULT, x0[i], np.array([0.7], np.float32))
         np.testing.assert_array_equal(fin[0][0, i], orc.clamp(L.G2DO, (x0[i] + dd * np.float32(0.1)).astype(np.float32)))
+
+
+@pytest.mark.parametrize("P,T,tr", [(10, 100, 0.0), (7, 95, 2.0), (20, 300, 3.5)])
+def test_bold_composition(orc, P, T, tr):
+    # the multi restatement's BOLD of subnetwork 0 equals the C restatement's
+    # per-instance BOLD monitor (windows, flush, TR resampling) for one subnetwork
+    rng = np.random.default_rng(P + T)
+    n = 4
+    x0 = _x0(rng, n, 2)
+    link = (0, 0, 0.1, L.CF_LINEAR, [0.5, 0.0], [(0, 0)])
+    tav, fin, bold = orc.run_multi([(L.G2DO, n, G2DO_DEFAULT, x0)], [link], 0.1, T, L.HEUN, False,
+                                   bold_period=P, bold_tr_ms=tr)
+    inst = Instance(model=L.G2DO, n_nodes=n, n_steps=T, dt=0.1, params=np.array(G2DO_DEFAULT, np.float32), x0=x0,
+                    integrator=L.HEUN, cfn=L.CF_LINEAR, cparams=(0.5, 0.0), wkind=L.W_SCALAR_MEAN, scalar_w=0.1,
+                    delay_mode=L.DELAY_SINGLE, single_delay=0, bold_period=P, bold_tr_ms=tr, record_bold=True)
+    ref = orc.run(inst)
+    assert bold.shape[1] == L.lib().hb_bold_rows(T, P, 0.1, tr)
+    np.testing.assert_allclose(bold[0], ref["bold"], rtol=1e-6, atol=1e-7)
