@@ -60,6 +60,9 @@ def run(config: str, output: str, backend: str = "hip", sweep: str | None = None
             # cli.rs:958-970 labels the [n_sweep, nnodes, nvar] data as [n_sweep, nvar, nnodes]
             write_npy_f32(path, t, [res.n_sweep, nvar, nnodes])
             written[f"tavg_sub{i}"] = path
+    elif len(cfg.subnetworks) > 1 or cfg.stimuli or any(p.src != p.tgt or p.cvar_map != "0:0"
+                                                        for p in cfg.projections) or len(cfg.projections) > 1:
+        written.update(_plain_multi(cfg, n_steps, output, noise_seed, device))
     else:
         # HybridEngine reads an inline initial state var-major [nvar][nnodes]
         # (construction.rs:557-565); the batch engine node-major (engine.rs:161-170)
@@ -81,6 +84,39 @@ def run(config: str, output: str, backend: str = "hip", sweep: str | None = None
         write_npy_f32(p_traj, traj.transpose(0, 2, 1).copy(), [n_steps, nvar, nnodes, 1])
         written["state_final_sub0"] = p_fin
         written["state_traj"] = p_traj
+    return written
+
+
+def _plain_multi(cfg, n_steps: int, output: str, noise_seed: int, device: int) -> dict:
+    """HybridEngine run of several subnetworks (cli.rs:1040-1103) through the
+    plain mode of hb_run_multi: state_final_sub{i}.npy as [nvar, nnodes, 1]
+    and state_traj.npy as the flat per-step concatenation of every
+    subnetwork's [nvar, nnodes, nmodes] state (runtime.rs:430-434)."""
+    from .engine import _multi_from_config
+    from .multi import run_multi
+    for s in cfg.subnetworks:   # inline initial states are var-major in HybridEngine
+        if isinstance(s.initial_state, list):
+            nv = L.MODEL_INFO[L.MODEL_NAMES[s.model]][0]
+            s.initial_state = np.asarray(s.initial_state, np.float32).reshape(nv, s.nnodes).T.ravel().tolist()
+    subs, links = _multi_from_config(cfg)
+    if cfg.integrator not in L.INTEGRATOR_NAMES:
+        raise SystemExit(f"unknown integrator {cfg.integrator}")
+    max_delay = max([max(p.delays) for p in cfg.projections if p.delays] or [0])
+    stims = [(t["target"], t["temporal"].lower(), t["params"]) for t in cfg.stimuli]
+    r = run_multi(subs, links, dt=cfg.dt, n_steps=n_steps, integrator=L.INTEGRATOR_NAMES[cfg.integrator],
+                  plain=True, stimuli=stims, dt_ms=cfg.dt, max_delay=max_delay, nsig=cfg.nsig,
+                  noise_seed=noise_seed, record_trajectory=True, device=device)
+    written = {}
+    for i, (S, f) in enumerate(zip(subs, r.final_states)):
+        nv = L.MODEL_INFO[S.model][0]
+        p = os.path.join(output, f"state_final_sub{i}.npy")
+        write_npy_f32(p, f[0].T.copy(), [nv, S.n_nodes, 1])
+        written[f"state_final_sub{i}"] = p
+    traj = np.concatenate([np.concatenate([t[k, 0].T.ravel() for t in r.trajectories]) for k in range(n_steps)]) \
+        if n_steps else np.zeros(0, np.float32)
+    p = os.path.join(output, "state_traj.npy")
+    write_npy_f32(p, traj, [traj.size])
+    written["state_traj"] = p
     return written
 
 

@@ -83,6 +83,7 @@ class SimConfig:
     backend: str = "ndarray"
     noise_mode: str = "reproducible"
     base_dir: str = "."
+    stimuli: list = field(default_factory=list)   # [{target, temporal, params}] (config.rs:253-258)
 
     @classmethod
     def from_dict(cls, d: dict, base_dir: str = ".") -> "SimConfig":
@@ -102,7 +103,9 @@ class SimConfig:
             return cls(sim_length=float(d["sim_length"]), dt=float(d["dt"]), subnetworks=subs, projections=projs,
                        integrator=str(d.get("integrator", "heun")).lower(), monitors=mons, nsig=d.get("nsig", 0.0),
                        speed=float(d.get("speed", 3.0)), backend=d.get("backend", "ndarray"),
-                       noise_mode=d.get("noise_mode", "reproducible"), base_dir=base_dir)
+                       noise_mode=d.get("noise_mode", "reproducible"), base_dir=base_dir,
+                       stimuli=[dict(target=int(t["target"]), temporal=str(t["temporal"]),
+                                     params=[float(v) for v in t.get("params", [])]) for t in d.get("stimuli", [])])
         except (KeyError, TypeError, ValueError) as e:
             _err(L.HB_INVALID_CONFIG, f"config parse error: {e}")
 

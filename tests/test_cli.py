@@ -134,3 +134,32 @@ def test_sweep_run_writes_one_tavg_per_subnetwork(gpu, tmp_path):
     eng = BatchHybridEngine.from_config(SimConfig.from_file(str(cfg)), 3)
     r = eng.run_sweep(SweepParam(0, 1), [0.0, 0.1, 0.2], 50)
     np.testing.assert_array_equal(np.load(out / "tavg_sub1.npy").ravel(), r.tavg[1])
+
+
+SIM3 = SIM2 + """
+[[stimuli]]
+target = 1
+temporal = "step"
+params = [1.0, 3.0, 0.5]
+"""
+
+
+@pytest.mark.gpu
+def test_plain_run_of_several_subnetworks(gpu, tmp_path):
+    # cli.rs:1040-1103: state_final_sub{i}.npy per subnetwork as [nvar, nnodes, 1];
+    # state_traj.npy is the flat per-step concatenation of every subnetwork's state
+    from hyburn_amd.multi import Link, Subnet, run_multi
+    from hyburn_amd.synth import G2DO_DEFAULT, WC_DEFAULT
+    cfg = tmp_path / "sim3.toml"
+    cfg.write_text(SIM3)
+    out = tmp_path / "plain3"
+    assert main(["run", "-c", str(cfg), "-o", str(out)]) == 0
+    f0, f1 = np.load(out / "state_final_sub0.npy"), np.load(out / "state_final_sub1.npy")
+    assert f0.shape == (2, 4, 1) and f1.shape == (2, 4, 1)
+    tr = np.load(out / "state_traj.npy")
+    assert tr.shape == (50 * (8 + 8),)
+    r = run_multi([Subnet(L.G2DO, 4, G2DO_DEFAULT), Subnet(L.WILSON_COWAN, 4, WC_DEFAULT)],
+                  [Link(0, 1, 0.01), Link(1, 0, 0.01)], dt=0.1, n_steps=50, integrator=L.HEUN, plain=True,
+                  stimuli=[(1, "step", [1.0, 3.0, 0.5])])
+    np.testing.assert_array_equal(f1[:, :, 0], r.final_states[1][0].T)
+    np.testing.assert_array_equal(tr[-16:-8], r.final_states[0][0].T.ravel())
