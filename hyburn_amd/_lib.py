@@ -129,7 +129,8 @@ class HbAdjointStats(C.Structure):
 class HbMultiProjection(C.Structure):
     _fields_ = [("src", C.c_int32), ("tgt", C.c_int32), ("wkind", C.c_int32), ("scalar_w", C.c_float),
                 ("dense_w", fp), ("cfn", C.c_int32), ("n_cparams", C.c_int32), ("cparams", C.c_float * 8),
-                ("n_map", C.c_int32), ("map_src", C.c_int32 * 4), ("map_tgt", C.c_int32 * 4), ("delay", C.c_int32)]
+                ("n_map", C.c_int32), ("map_src", C.c_int32 * 4), ("map_tgt", C.c_int32 * 4), ("delay", C.c_int32),
+                ("indptr", ip), ("indices", ip), ("csr_w", fp), ("edge_delays", ip), ("nnz", C.c_int32)]
 
 
 class HbStimulus(C.Structure):

@@ -98,7 +98,7 @@ def _plain_multi(cfg, n_steps: int, output: str, noise_seed: int, device: int) -
         if isinstance(s.initial_state, list):
             nv = L.MODEL_INFO[L.MODEL_NAMES[s.model]][0]
             s.initial_state = np.asarray(s.initial_state, np.float32).reshape(nv, s.nnodes).T.ravel().tolist()
-    subs, links = _multi_from_config(cfg)
+    subs, links = _multi_from_config(cfg, plain=True)
     if cfg.integrator not in L.INTEGRATOR_NAMES:
         raise SystemExit(f"unknown integrator {cfg.integrator}")
     max_delay = max([max(p.delays) for p in cfg.projections if p.delays] or [0])

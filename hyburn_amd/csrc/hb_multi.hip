@@ -45,6 +45,11 @@ struct Proj {
     float w, cc[8];
     const float* W;        // [n_tgt][n_src] (HB_W_DENSE) or null
     const float* rowsum;   // [n_tgt]: Difference over a non-square Dense projection, or null
+    const int32_t* ip;     // per-edge CSR (plain mode): row pointers, sources, weights, delays
+    const int32_t* ix;
+    const float* cw;
+    const int32_t* cd;
+    int32_t src_off;       // first node of the source subnetwork
 };
 
 struct Stim {
@@ -268,6 +273,29 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                 float g[4] = {0.0f, 0.0f, 0.0f, 0.0f};
                 if (P.wkind == HB_W_SCALAR_MEAN) {
                     for (int c = 0; c < np; ++c) g[c] = gmean[p * 4 + c];
+                } else if (P.wkind == HB_W_CSR) {
+                    // runtime.rs:160-203: every edge reads its source d_e steps
+                    // back from that node's history (written before the barrier
+                    // above); d_e > step contributes a zero pre vector
+                    for (int e = P.ip[i]; e < P.ip[i + 1]; ++e) {
+                        const int j = P.ix[e], d = P.cd[e];
+                        const float w = P.cw[e];
+                        float o[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+                        if (d == 0 || step == 0 || d <= step) {
+                            const int64_t ts = (d == 0 || step == 0) ? step : step - d;
+                            const float* hj = A->hist + (((size_t)b * A->n_tot + P.src_off + j) * H + ts % H) * 2;
+                            float xd[2] = {hj[0], hj[1]};
+                            if (P.nc == 1) {
+                                if (np == 2) pre<1, 2>(P.cfn, P.cc, true, P.classic, xd, o);
+                                else pre<1, 1>(P.cfn, P.cc, true, P.classic, xd, o);
+                            } else {
+                                if (np == 4) pre<2, 4>(P.cfn, P.cc, true, P.classic, xd, o);
+                                else if (np == 1) pre<2, 1>(P.cfn, P.cc, true, P.classic, xd, o);
+                                else pre<2, 2>(P.cfn, P.cc, true, P.classic, xd, o);
+                            }
+                        }
+                        for (int c = 0; c < np; ++c) g[c] += w * o[c];
+                    }
                 } else {
                     const float* Wr = P.W + (size_t)i * ns;
                     for (int j = 0; j < ns; ++j) {
@@ -450,8 +478,22 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         const std::string tag = "projection " + std::to_string(p);
         if (R.src < 0 || R.src >= Q.n_subnets || R.tgt < 0 || R.tgt >= Q.n_subnets) return bad(tag + ": src/tgt out of range");
         if (R.delay < 0) return bad(tag + ": negative delay");
-        if (R.wkind != HB_W_SCALAR_MEAN && R.wkind != HB_W_DENSE) return bad(tag + ": weights must be Scalar or Dense");
+        if (R.wkind != HB_W_SCALAR_MEAN && R.wkind != HB_W_DENSE && R.wkind != HB_W_CSR)
+            return bad(tag + ": weights must be Scalar, Dense or CSR");
         if (R.wkind == HB_W_DENSE && !R.dense_w) return bad(tag + ": dense weights missing");
+        if (R.wkind == HB_W_CSR) {
+            const int ntr = Q.subnets[R.tgt].n_nodes, nsr = Q.subnets[R.src].n_nodes;
+            if (!R.indptr || R.nnz < 0 || (R.nnz > 0 && (!R.indices || !R.csr_w))) return bad(tag + ": CSR arrays missing");
+            if (R.indptr[0] != 0 || R.indptr[ntr] != R.nnz) return bad(tag + ": CSR indptr inconsistent with nnz");
+            for (int r = 0; r < ntr; ++r)
+                if (R.indptr[r + 1] < R.indptr[r]) return bad(tag + ": CSR indptr not monotone");
+            for (int e = 0; e < R.nnz; ++e) {
+                if (R.indices[e] < 0 || R.indices[e] >= nsr) return bad(tag + ": CSR source index out of range");
+                if (R.edge_delays && R.edge_delays[e] < 0) return bad(tag + ": negative edge delay");
+            }
+            if (R.edge_delays && !Q.plain) return bad(tag + ": per-edge delays belong to the plain engine");
+            if (R.edge_delays && R.cfn == HB_CF_DIFFERENCE) return bad(tag + ": Difference with per-edge delays");
+        }
         if (R.cfn < HB_CF_LINEAR || R.cfn > HB_CF_PRESIGMOIDAL) return bad(tag + ": unknown coupling function");
         if (R.n_cparams < 0 || R.n_cparams > 8) return bad(tag + ": at most 8 coupling parameters");
         const int ms = Q.subnets[R.src].model, mt = Q.subnets[R.tgt].model;
@@ -480,6 +522,12 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         P.classic = classic;
         P.w = R.scalar_w;
         if (R.wkind == HB_W_DENSE) host_w[p].assign(R.dense_w, R.dense_w + (size_t)nt * ns);
+        if (R.wkind == HB_W_CSR && !R.edge_delays) {   // densified (engine.rs:298-311)
+            host_w[p].assign((size_t)nt * ns, 0.0f);
+            for (int r = 0; r < nt; ++r)
+                for (int e = R.indptr[r]; e < R.indptr[r + 1]; ++e) host_w[p][(size_t)r * ns + R.indices[e]] = R.csr_w[e];
+            P.wkind = HB_W_DENSE;
+        }
         if (Q.plain && R.wkind == HB_W_SCALAR_MEAN) {   // construction.rs:639-654: w on the diagonal
             if (nt != ns) return bad("Scalar weights require src and tgt nnodes to match");
             host_w[p].assign((size_t)nt * ns, 0.0f);
@@ -525,9 +573,19 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     }
     if (lds > kMaxLdsFloats) return bad("projection sources exceed 64 KiB of pre values");
     int max_delay = std::max(0, Q.max_delay);
-    for (int p = 0; p < Q.n_projections; ++p) max_delay = std::max(max_delay, Q.projections[p].delay);
+    bool per_edge = false;
+    for (int p = 0; p < Q.n_projections; ++p) {
+        const hb_multi_projection& R = Q.projections[p];
+        max_delay = std::max(max_delay, R.delay);
+        if (R.wkind == HB_W_CSR && R.edge_delays) {
+            per_edge = true;
+            for (int e = 0; e < R.nnz; ++e) max_delay = std::max(max_delay, R.edge_delays[e]);
+        }
+    }
     if (max_delay > 1000000) return bad("delays above 10^6 steps");
-    const int H = max_delay > 0 ? max_delay + 1 : 0;   // engine.rs:253-257
+    // engine.rs:253-257 / construction.rs:598-603; per-edge reads always go
+    // through the history, so it exists whenever such a projection does
+    const int H = max_delay > 0 ? max_delay + 1 : (per_edge ? 1 : 0);
     hipError_t e = hipSetDevice(Q.device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     for (int p = 0; p < Q.n_projections; ++p) {
@@ -537,6 +595,26 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
             if (e == hipSuccess) e = hipMemcpy(d, host_w[p].data(), sizeof(float) * host_w[p].size(), hipMemcpyHostToDevice);
             if (e != hipSuccess) return hip_fail(e, "dense weights upload");
             A.proj[p].W = d;
+        }
+        const hb_multi_projection& R = Q.projections[p];
+        A.proj[p].src_off = A.sub[R.src].off;
+        if (R.wkind == HB_W_CSR && R.edge_delays) {
+            const int ntr = Q.subnets[R.tgt].n_nodes;
+            int32_t *dip = nullptr, *dix = nullptr, *dcd = nullptr;
+            float* dcw = nullptr;
+            e = B.alloc(&dip, (size_t)ntr + 1);
+            if (e == hipSuccess) e = B.alloc(&dix, (size_t)R.nnz);
+            if (e == hipSuccess) e = B.alloc(&dcd, (size_t)R.nnz);
+            if (e == hipSuccess) e = B.alloc(&dcw, (size_t)R.nnz);
+            if (e == hipSuccess) e = hipMemcpy(dip, R.indptr, sizeof(int32_t) * (ntr + 1), hipMemcpyHostToDevice);
+            if (e == hipSuccess && R.nnz) e = hipMemcpy(dix, R.indices, sizeof(int32_t) * R.nnz, hipMemcpyHostToDevice);
+            if (e == hipSuccess && R.nnz) e = hipMemcpy(dcd, R.edge_delays, sizeof(int32_t) * R.nnz, hipMemcpyHostToDevice);
+            if (e == hipSuccess && R.nnz) e = hipMemcpy(dcw, R.csr_w, sizeof(float) * R.nnz, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(e, "CSR upload");
+            A.proj[p].ip = dip;
+            A.proj[p].ix = dix;
+            A.proj[p].cd = dcd;
+            A.proj[p].cw = dcw;
         }
         if (!host_rs[p].empty()) {
             float* d = nullptr;

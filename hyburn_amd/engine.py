@@ -311,8 +311,10 @@ def _needs_multi(cfg: SimConfig) -> bool:
             or any(p.src != p.tgt or p.cvar_map != "0:0" for p in cfg.projections))
 
 
-def _multi_from_config(cfg: SimConfig):
-    """SimConfig -> (Subnet list, Link list) for hyburn_amd.multi (engine.rs:145-415)."""
+def _multi_from_config(cfg: SimConfig, plain: bool = False):
+    """SimConfig -> (Subnet list, Link list) for hyburn_amd.multi (engine.rs:145-415).
+    plain: a CSR projection with one delay per edge keeps its per-edge delays
+    (the HybridEngine's path, runtime.rs:125-205); otherwise CSR is densified."""
     from .multi import Link, Subnet, parse_cvar_map
     subs = []
     for i, s in enumerate(cfg.subnetworks):
@@ -330,6 +332,11 @@ def _multi_from_config(cfg: SimConfig):
             _err(L.HB_INVALID_CONFIG, f"Unknown coupling function: {p.coupling_fn}")
         ns, nt = subs[p.src].n_nodes, subs[p.tgt].n_nodes
         w = p.weights
+        if isinstance(w, dict) and plain and len(p.delays) == len(w["data"]) and len(p.delays) > 0:
+            links.append(Link(p.src, p.tgt, (np.asarray(w["indptr"]), np.asarray(w["indices"]), np.asarray(w["data"])),
+                              L.COUPLING_NAMES[p.coupling_fn], list(p.coupling_params), parse_cvar_map(p.cvar_map),
+                              0, edge_delays=list(p.delays)))
+            continue
         if isinstance(w, dict):   # CSR by target row, densified (engine.rs:298-311)
             W = np.zeros((nt, ns), np.float32)
             ip, ix, dat = w["indptr"], w["indices"], w["data"]

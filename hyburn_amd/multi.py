@@ -40,6 +40,7 @@ class Link:
     coupling_params: Sequence[float] = (1.0,)
     cvar_map: Sequence[tuple] = ((0, 0),)
     delay: int = 0
+    edge_delays: Optional[Sequence[int]] = None   # CSR weights (indptr, indices, data): per-edge delays (plain)
 
 
 @dataclass
@@ -57,9 +58,12 @@ STIM_PATTERNS = {"impulse": 0, "step": 1, "sinusoid": 2, "pulse_train": 3}
 
 
 def horizon(links: Sequence[Link], max_delay: int = 0) -> int:
-    """History length: max delay + 1, 0 without delays (engine.rs:253-257)."""
-    d = max([max_delay] + [L_.delay for L_ in links])
-    return d + 1 if d > 0 else 0
+    """History length: max delay + 1, 0 without delays (engine.rs:253-257);
+    per-edge CSR delays count, and such a projection always has a history."""
+    d = max([max_delay] + [L_.delay for L_ in links] +
+            [int(np.max(L_.edge_delays)) for L_ in links if L_.edge_delays is not None and len(L_.edge_delays)])
+    per_edge = any(L_.edge_delays is not None for L_ in links)
+    return d + 1 if d > 0 else (1 if per_edge else 0)
 
 
 def parse_cvar_map(s: str) -> list:
@@ -107,6 +111,16 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
         P.src, P.tgt, P.cfn, P.delay = R.src, R.tgt, R.coupling_fn, R.delay
         if np.isscalar(R.weights):
             P.wkind, P.scalar_w = L.W_SCALAR_MEAN, float(R.weights)
+        elif isinstance(R.weights, tuple) and len(R.weights) == 3:   # CSR by target row
+            ipt, ind, dat = (np.ascontiguousarray(R.weights[0], np.int32), np.ascontiguousarray(R.weights[1], np.int32),
+                             np.ascontiguousarray(R.weights[2], np.float32))
+            keep += [ipt, ind, dat]
+            P.wkind, P.nnz = L.W_CSR, len(dat)
+            P.indptr, P.indices, P.csr_w = ipt.ctypes.data_as(L.ip), ind.ctypes.data_as(L.ip), dat.ctypes.data_as(L.fp)
+            if R.edge_delays is not None:
+                ed = np.ascontiguousarray(R.edge_delays, np.int32)
+                keep.append(ed)
+                P.edge_delays = ed.ctypes.data_as(L.ip)
         else:
             W = np.ascontiguousarray(R.weights, np.float32)
             if W.shape != (subnets[R.tgt].n_nodes, subnets[R.src].n_nodes):

@@ -274,7 +274,7 @@ void hb_autotune_clear(void);
 /* ---------------- several subnetworks (BatchHybridEngine with projections) ---------------- */
 typedef struct {
     int32_t src, tgt;         /* subnetwork indices */
-    int32_t wkind;            /* HB_W_SCALAR_MEAN (w * mean over source nodes) or HB_W_DENSE */
+    int32_t wkind;            /* HB_W_SCALAR_MEAN (w * mean over source nodes), HB_W_DENSE or HB_W_CSR */
     float scalar_w;
     const float* dense_w;     /* [n_tgt][n_src]: row = target node (CSR densified, engine.rs:298-311) */
     int32_t cfn, n_cparams;   /* HB_CF_*, coupling_params */
@@ -282,6 +282,14 @@ typedef struct {
     int32_t n_map;            /* cvar_map pairs map_src[k]:map_tgt[k] ("0:0" = one pair 0, 0) */
     int32_t map_src[4], map_tgt[4];
     int32_t delay;            /* steps: the projection's first delay (engine.rs:407) */
+    /* HB_W_CSR (by target row): densified unless edge_delays is given; with
+     * edge_delays (plain mode only) each edge reads its source d_e steps back
+     * (runtime.rs:125-205) */
+    const int32_t* indptr;
+    const int32_t* indices;
+    const float* csr_w;
+    const int32_t* edge_delays;
+    int32_t nnz;
 } hb_multi_projection;
 
 /* A stimulus of the plain HybridEngine (src/engine/stimulus.rs:17-128), added to

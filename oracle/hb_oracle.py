@@ -394,10 +394,17 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
     info = [HL.MODEL_INFO[m] for (m, _, _, _) in subnets]
     # Difference preprocessing, engine.rs:325-366
     prep = []
-    H = max([max_delay] + [l[6] if len(l) > 6 else 0 for l in links])
-    H = H + 1 if H > 0 else 0
+    H = max([max_delay] + [l[6] if len(l) > 6 else 0 for l in links] +
+            [int(max(l[2][3])) for l in links if isinstance(l[2], tuple) and len(l[2]) == 4 and len(l[2][3])])
+    per_edge = any(isinstance(l[2], tuple) and len(l[2]) == 4 for l in links)
+    H = H + 1 if H > 0 else (1 if per_edge else 0)
     for (src, tgt, w, cfn, cp, cmap, *rest) in links:
         delay = rest[0] if rest else 0
+        if isinstance(w, tuple) and len(w) == 4:   # per-edge CSR (plain): indptr, indices, data, delays
+            if cfn not in _IDENTITY_PRE or cfn == HL.CF_DIFFERENCE:
+                raise NotImplementedError("per-edge links restated for identity-pre coupling functions")
+            prep.append((src, tgt, w, None, cfn, list(cp), list(cmap), 0))
+            continue
         cp = list(cp)
         ns, nt = subnets[src][1], subnets[tgt][1]
         W = None if np.isscalar(w) else np.array(w, np.float32)
@@ -439,7 +446,24 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                 ms, mt = subnets[src][0], subnets[tgt][0]
                 ncs, nct = info[src][1], info[tgt][1]
                 ns, nt = subnets[src][1], subnets[tgt][1]
-                if delay == 0 or step == 0:
+                if isinstance(w, tuple):   # runtime.rs:160-203, identity pre: sum of w_e * x_j(step - d_e)
+                    ipt, ind, dat, edl = w
+                    xt = x[tgt][:, :ncs] if nct == ncs else np.zeros((nt, ncs), np.float32)
+                    out = np.zeros((nt, ncs), np.float32)
+                    for r_ in range(nt):
+                        g = np.zeros(ncs, np.float32)
+                        for e_ in range(ipt[r_], ipt[r_ + 1]):
+                            d_ = int(edl[e_])
+                            if d_ == 0 or step == 0:
+                                xv = x[src][ind[e_], :ncs]
+                            elif d_ <= step:
+                                xv = past[src][step - d_][ind[e_], :ncs]
+                            else:
+                                xv = np.zeros(ncs, np.float32)
+                            g = (g + np.float32(dat[e_]) * xv).astype(np.float32)
+                        out[r_] = coupling(cfn, cp, g[None, :], xt[r_:r_ + 1], dense=np.ones((1, 1), np.float32),
+                                           n_src=ns)[0]
+                elif delay == 0 or step == 0:
                     xs = x[src][:, :ncs]
                 elif delay <= step and plain:
                     xs = past[src][step - delay][:, :ncs]
@@ -448,7 +472,9 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                 else:
                     xs = np.zeros((ns, ncs), np.float32)
                 xt = x[tgt][:, :ncs] if nct == ncs else np.zeros((nt, ncs), np.float32)
-                if W is None:
+                if isinstance(w, tuple):
+                    pass
+                elif W is None:
                     if cfn not in _IDENTITY_PRE:
                         raise NotImplementedError("Scalar weights with a non-identity pre")
                     g = np.array([[f32(f32(_f32sum(xs[:, c]) / f32(ns)) * f32(w)) for c in range(ncs)]], np.float32)

@@ -268,3 +268,34 @@ def test_bold_of_subnetwork_zero(gpu, orc, P, tr):
     assert res.bold.shape == bold.shape and bold.shape[1] > 0
     np.testing.assert_allclose(res.bold, bold, rtol=RTOL, atol=ATOL)
     _check(res, tav, fin)
+
+
+def test_plain_per_edge_csr_delays(gpu, orc):
+    # per-edge CSR delays between and within subnetworks (plain engine,
+    # runtime.rs:125-205): each edge reads its source d_e steps back, zero
+    # while d_e > step
+    rng = np.random.default_rng(77)
+    subs = [(L.G2DO, 6, G2DO_DEFAULT, _x0(rng, 6, 2)), (L.G2DO, 5, G2DO_DEFAULT, _x0(rng, 5, 2)),
+            (L.WILSON_COWAN, 5, WC_DEFAULT, np.abs(_x0(rng, 5, 2, 0.1)))]
+
+    def csr(nt, ns, md):
+        W = (rng.uniform(size=(nt, ns)) * (rng.uniform(size=(nt, ns)) < 0.6)).astype(np.float32)
+        ipt = np.concatenate([[0], np.cumsum((W != 0).sum(1))]).astype(np.int32)
+        ind = np.concatenate([np.nonzero(W[r])[0] for r in range(nt)]).astype(np.int32)
+        dat = np.concatenate([W[r][W[r] != 0] for r in range(nt)]).astype(np.float32)
+        return ipt, ind, dat, rng.integers(0, md + 1, len(dat)).astype(np.int32)
+
+    c00, c10, c21 = csr(6, 6, 9), csr(5, 6, 4), csr(5, 5, 13)
+    links_o = [(0, 0, c00, L.CF_LINEAR, [0.3, 0.0], [(0, 0)]), (0, 1, c10, L.CF_SIGMOIDAL, [], [(0, 0)]),
+               (1, 2, c21, L.CF_LINEAR, [0.2, 0.01], [(0, 0)]), (2, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 0)], 3)]
+    links = [Link(0, 0, c00[:3], L.CF_LINEAR, [0.3, 0.0], edge_delays=c00[3]),
+             Link(0, 1, c10[:3], L.CF_SIGMOIDAL, [], edge_delays=c10[3]),
+             Link(1, 2, c21[:3], L.CF_LINEAR, [0.2, 0.01], edge_delays=c21[3]),
+             Link(2, 0, 0.05, L.CF_LINEAR, [1.0], delay=3)]
+    # Scalar weights between equal-size subnetworks only in the plain engine: 2 -> 0 is 5 -> 6
+    links_o[3] = (2, 0, (rng.uniform(size=(6, 5)) * 0.1).astype(np.float32), L.CF_LINEAR, [1.0], [(0, 0)], 3)
+    links[3] = Link(2, 0, links_o[3][2], L.CF_LINEAR, [1.0], delay=3)
+    res = run_multi([Subnet(m, k, p, x0) for (m, k, p, x0) in subs], links, dt=0.1, n_steps=120,
+                    integrator=L.HEUN, plain=True)
+    tav, fin = orc.run_multi(subs, links_o, 0.1, 120, L.HEUN, False, plain=True)
+    _check(res, tav, fin)

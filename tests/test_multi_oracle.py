@@ -251,3 +251,29 @@ def test_bold_composition(orc, P, T, tr):
     ref = orc.run(inst)
     assert bold.shape[1] == L.lib().hb_bold_rows(T, P, 0.1, tr)
     np.testing.assert_allclose(bold[0], ref["bold"], rtol=1e-6, atol=1e-7)
+
+
+def _csr(rng, nt, ns, dens=0.5, max_delay=6):
+    W = (rng.uniform(size=(nt, ns)) * (rng.uniform(size=(nt, ns)) < dens)).astype(np.float32)
+    ipt = np.concatenate([[0], np.cumsum((W != 0).sum(1))]).astype(np.int32)
+    ind = np.concatenate([np.nonzero(W[r])[0] for r in range(nt)]).astype(np.int32)
+    dat = np.concatenate([W[r][W[r] != 0] for r in range(nt)]).astype(np.float32)
+    dl = rng.integers(0, max_delay + 1, len(dat)).astype(np.int32)
+    return ipt, ind, dat, dl
+
+
+@pytest.mark.parametrize("integ", [L.HEUN, L.RK4])
+def test_per_edge_plain_composition(orc, integ):
+    # plain per-edge CSR delays with one subnetwork equal the C restatement's
+    # HybridEngine per-edge path (runtime.rs:125-205)
+    rng = np.random.default_rng(integ + 90)
+    n, T = 6, 50
+    x0 = _x0(rng, n, 2)
+    ipt, ind, dat, dl = _csr(rng, n, n)
+    link = (0, 0, (ipt, ind, dat, dl), L.CF_LINEAR, [0.4, 0.0], [(0, 0)])
+    tav, fin = orc.run_multi([(L.G2DO, n, G2DO_DEFAULT, x0)], [link], 0.1, T, integ, False, plain=True)
+    inst = Instance(model=L.G2DO, n_nodes=n, n_steps=T, dt=0.1, params=np.array(G2DO_DEFAULT, np.float32), x0=x0,
+                    integrator=integ, engine=L.ENGINE_PLAIN, cfn=L.CF_LINEAR, cparams=(0.4, 0.0), wkind=L.W_CSR,
+                    indptr=ipt, indices=ind, weights=dat, edge_delays=dl, delay_mode=L.DELAY_PER_EDGE)
+    ref = orc.run(inst)
+    np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-6, atol=1e-7)
