@@ -423,22 +423,6 @@ __global__ __launch_bounds__(256) void k_spectral(const double* __restrict__ psd
     }
 }
 
-// Owns the device buffers of one call; frees them on every exit path.
-struct DevBufs {
-    std::vector<void*> p;
-    ~DevBufs() {
-        for (void* q : p)
-            if (q) (void)hipFree(q);
-    }
-    template <typename T> hipError_t alloc(T** out, size_t count) {
-        void* q = nullptr;
-        const hipError_t e = hipMalloc(&q, count ? count * sizeof(T) : 1);
-        if (e == hipSuccess) p.push_back(q);
-        *out = (T*)q;
-        return e;
-    }
-};
-
 // Every job of one call has the same N. Fills each job's mean pointer from a
 // work buffer, then launches the means and the block-pair tile kernels.
 hipError_t run_fc_jobs(std::vector<FcJob>& jobs, int N, int T, DevBufs& B) {
@@ -545,7 +529,7 @@ int hb_fc_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t 
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const size_t per = (size_t)n_steps * n_nodes, nn = (size_t)n_nodes * n_nodes;
-    feat::DevBufs B;
+    DevBufs B;
     float *dx = nullptr, *dout = nullptr;
     e = B.alloc(&dx, per * n_series);
     if (e == hipSuccess) e = B.alloc(&dout, nn * n_series);
@@ -583,7 +567,7 @@ int hb_fcd_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t
     // window FCs are written as v[w][p] (W windows x P upper entries per
     // series) and transposed to vt[p][w]: the FCD is the column correlation
     // of that [P][W] series
-    feat::DevBufs B;
+    DevBufs B;
     float *dx = nullptr, *dv = nullptr, *dvt = nullptr, *dout = nullptr;
     e = B.alloc(&dx, per * n_series);
     if (e == hipSuccess) e = B.alloc(&dv, P * W * n_series);
@@ -633,7 +617,7 @@ int hb_welch_psd_batch(const float* series, int32_t n_series, int32_t n_steps, i
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const size_t per = (size_t)n_steps * n_nodes;
-    feat::DevBufs B;
+    DevBufs B;
     float* dy = nullptr;
     double* dpsd = nullptr;
     e = B.alloc(&dy, per * n_series);
@@ -662,7 +646,7 @@ int hb_spectral_features_batch(const float* series, int32_t n_series, int32_t n_
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const size_t per = (size_t)n_steps * n_nodes;
-    feat::DevBufs B;
+    DevBufs B;
     float *dy = nullptr, *dfeat = nullptr;
     double* dpsd = nullptr;
     e = B.alloc(&dy, per * n_series);

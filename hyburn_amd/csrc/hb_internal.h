@@ -11,6 +11,7 @@
 
 #include <cmath>
 #include <string>
+#include <vector>
 
 #include "../../include/hyburn_hip.h"
 
@@ -223,6 +224,25 @@ inline BoldShape bold_shape(int T, int P, double dt_ms, double tr_ms) {
     b.nout = max_t >= tr_ms ? (int)std::floor(max_t / tr_ms) : 0;
     return b;
 }
+
+// Owns the device buffers of one host call; frees them on every exit path.
+struct DevBufs {
+    std::vector<void*> p;
+    DevBufs() = default;
+    DevBufs(const DevBufs&) = delete;
+    DevBufs& operator=(const DevBufs&) = delete;
+    ~DevBufs() {
+        for (void* q : p)
+            if (q) (void)hipFree(q);
+    }
+    template <typename T> hipError_t alloc(T** out, size_t count) {
+        void* q = nullptr;
+        const hipError_t e = hipMalloc(&q, count ? count * sizeof(T) : 1);
+        if (e == hipSuccess) p.push_back(q);
+        *out = (T*)q;
+        return e;
+    }
+};
 
 // Record the thread-local last-error message (hb_last_error) and return code.
 int set_error(int code, const std::string& msg);

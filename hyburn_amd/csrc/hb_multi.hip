@@ -395,21 +395,6 @@ int cvar_position(int model, int t) {
     return 0;
 }
 
-struct DevBufs {
-    std::vector<void*> p;
-    ~DevBufs() {
-        for (void* q : p)
-            if (q) (void)hipFree(q);
-    }
-    template <typename T> hipError_t alloc(T** out, size_t count) {
-        void* q = nullptr;
-        const hipError_t e = hipMalloc(&q, count ? count * sizeof(T) : 1);
-        if (e == hipSuccess) p.push_back(q);
-        *out = (T*)q;
-        return e;
-    }
-};
-
 }  // namespace multi
 }  // namespace hb
 
