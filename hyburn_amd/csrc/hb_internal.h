@@ -23,19 +23,23 @@ constexpr int kMaxNV = 6;
 constexpr int kMC = 28;   // model constant slots
 constexpr int kCC = 12;   // coupling constant slots
 
-HB_HD int model_nvar(int m) { return m == HB_JANSEN_RIT ? 6 : (m == HB_KURAMOTO || m == HB_RWW) ? 1 : 2; }
-HB_HD int model_ncvar(int m) { return m == HB_MPR ? 2 : 1; }
+HB_HD int model_nvar(int m) {
+    return m == HB_JANSEN_RIT ? 6 : (m == HB_KURAMOTO || m == HB_RWW || m == HB_LINEAR) ? 1 : 2;
+}
+HB_HD int model_ncvar(int m) { return (m == HB_MPR || m == HB_SUP_HOPF) ? 2 : 1; }
 HB_HD int model_nparams(int m) {
     switch (m) {
     case HB_G2DO: return 12; case HB_MPR: return 7; case HB_JANSEN_RIT: return 13;
-    case HB_WILSON_COWAN: return 22; case HB_KURAMOTO: return 1; default: return 8;
+    case HB_WILSON_COWAN: return 22; case HB_KURAMOTO: return 1; case HB_LINEAR: return 1;
+    case HB_SUP_HOPF: return 2; default: return 8;
     }
 }
 // The reference's single batch-sweepable parameter per model (dfun.rs:49-72).
 HB_HD int model_sweep_index(int m) {
     switch (m) {
     case HB_G2DO: return 1; case HB_MPR: return 4; case HB_JANSEN_RIT: return 12;
-    case HB_WILSON_COWAN: return 18; case HB_KURAMOTO: return 0; default: return 7;
+    case HB_WILSON_COWAN: return 18; case HB_KURAMOTO: return 0; case HB_RWW: return 7;
+    default: return -1;   // Linear, SupHopf: the batch dfun ignores a swept value (dfun.rs:522-557)
     }
 }
 // Number of pre() channels the ring stores per node (coupling.rs:204-258).
@@ -109,7 +113,11 @@ HB_HD void model_consts_t(int m, const S* p, S* mc) {
         mc[23] = one / (one + mc_exp(p[10] * p[11]));
         break; }
     case HB_KURAMOTO:
+    case HB_LINEAR:
         mc[0] = p[0];
+        break;
+    case HB_SUP_HOPF:
+        mc[0] = p[0]; mc[1] = p[1];
         break;
     default: // RWW
         mc[0] = p[0]; mc[1] = p[1]; mc[2] = p[2]; mc[3] = p[3];

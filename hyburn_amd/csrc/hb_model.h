@@ -14,6 +14,8 @@ template <> struct Tr<HB_JANSEN_RIT> { static constexpr int NV = 6, NC = 1; };
 template <> struct Tr<HB_WILSON_COWAN> { static constexpr int NV = 2, NC = 1; };
 template <> struct Tr<HB_KURAMOTO> { static constexpr int NV = 1, NC = 1; };
 template <> struct Tr<HB_RWW> { static constexpr int NV = 1, NC = 1; };
+template <> struct Tr<HB_LINEAR> { static constexpr int NV = 1, NC = 1; };
+template <> struct Tr<HB_SUP_HOPF> { static constexpr int NV = 2, NC = 2; };
 
 // ---------------------------------------------------------------- models ---
 
@@ -96,6 +98,22 @@ template <> __device__ __forceinline__ void dfun<HB_RWW>(const float* mc, bool s
     const float den = -et + 1.0f;
     const float h = axb / den;
     dx[0] = (-s) * mc[4] + ((-s + 1.0f) * h) * mc[3];
+}
+
+// dfun.rs:522-532: gamma * x + c0
+template <> __device__ __forceinline__ void dfun<HB_LINEAR>(const float* mc, bool sw, float sv,
+                                                            const float* x, const float* c, float* dx) {
+    dx[0] = x[0] * mc[0] + c[0];
+}
+
+// dfun.rs:534-557: amp = x^2 + y^2; dx = (a - amp) x - omega y + c0,
+// dy = (a - amp) y + omega x + c1
+template <> __device__ __forceinline__ void dfun<HB_SUP_HOPF>(const float* mc, bool sw, float sv,
+                                                              const float* x, const float* c, float* dx) {
+    const float amp = x[0] * x[0] + x[1] * x[1];
+    const float g = -amp + mc[0];
+    dx[0] = (g * x[0] - x[1] * mc[1]) + c[0];
+    dx[1] = (g * x[1] + x[0] * mc[1]) + c[1];
 }
 
 // clamp_batch, dfun.rs:170-239. Ternaries keep NaN like f32::clamp.

@@ -65,6 +65,8 @@ WC_DEFAULT = np.array([12, 4, 13, 11, 10, 10, 1.2, 2.8, 1, 0, 1, 4, 1, 0, 1, 1, 
                       np.float32)                                                  # wilson_cowan.rs:85-92
 KUR_DEFAULT = np.array([1.0], np.float32)                                          # kuramoto_model.rs:53-55
 RWW_DEFAULT = np.array([0.270, 0.108, 154, 0.641, 100, 0.6, 0.2609, 0.33], np.float32)  # rww.rs:61-63
+LINEAR_DEFAULT = np.array([-10.0], np.float32)                                      # linear.rs:33-35
+SUP_HOPF_DEFAULT = np.array([-0.5, 1.0], np.float32)                                # sup_hopf.rs:35-37
 DEFAULTS = {L.G2DO: G2DO_DEFAULT, L.MPR: MPR_DEFAULT, L.JANSEN_RIT: JR_DEFAULT,
             L.WILSON_COWAN: WC_DEFAULT, L.KURAMOTO: KUR_DEFAULT, L.RWW: RWW_DEFAULT}
 

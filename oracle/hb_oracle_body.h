@@ -23,6 +23,8 @@ typedef struct {
          w_ci, w_thi, w_re, w_ri, w_ke, w_ki, w_p, w_q, w_ale, w_ali, w_off_e, w_off_i;
     /* Kuramoto (dfun.rs:480-492) */
     REAL k_omega;
+    /* Linear, SupHopf (dfun.rs:522-557) */
+    REAL l_gamma, h_a, h_omega;
     /* RWW   (dfun.rs:494-516) */
     REAL r_a, r_b, r_d, r_gamma, r_inv_taus, r_wjn, r_jn, r_io;
     int sweep_form;   /* 1: reference's batch-native swept form (dfun.rs:49-72) */
@@ -83,6 +85,12 @@ static void FN(mconst_init)(FN(mconst)* k, int model, const float* pf, int sweep
         k->w_off_e = (REAL)1 / ((REAL)1 + EXPR(k->w_ae * k->w_be));
         k->w_off_i = (REAL)1 / ((REAL)1 + EXPR(k->w_ai * k->w_bi));
         break; }
+    case ORC_LIN:
+        k->l_gamma = p[0];
+        break;
+    case ORC_SH:
+        k->h_a = p[0]; k->h_omega = p[1];
+        break;
     case ORC_KUR:
         k->k_omega = p[0];
         break;
@@ -155,6 +163,15 @@ static void FN(dfun)(const FN(mconst)* k, const REAL* x, const REAL* c, REAL* dx
     case ORC_KUR: /* dfun.rs:486-491 */
         dx[0] = c[0] + (k->sweep_form ? k->sweep_val : k->k_omega);
         break;
+    case ORC_LIN: /* dfun.rs:522-532 */
+        dx[0] = x[0] * k->l_gamma + c[0];
+        break;
+    case ORC_SH: { /* dfun.rs:534-557 */
+        REAL amp = x[0] * x[0] + x[1] * x[1];
+        REAL g = -amp + k->h_a;
+        dx[0] = (g * x[0] - x[1] * k->h_omega) + c[0];
+        dx[1] = (g * x[1] + x[0] * k->h_omega) + c[1];
+        break; }
     case ORC_RWW: { /* dfun.rs:503-515 */
         REAL s = x[0];
         REAL xx = k->sweep_form ? (s * k->r_wjn + k->sweep_val) + c[0] * k->r_jn

@@ -114,6 +114,28 @@ def test_wc_at_zero_and_clamp(orc):
     assert x[0] == 0.0 and x[1] == 1.0
 
 
+def test_linear_and_sup_hopf_kats(orc):
+    # linear.rs:45-55: x = 1, c = 0 -> dx = gamma = -10; sup_hopf.rs:45-55: zero at the origin
+    d = orc.dfun(L.LINEAR, [-10.0], np.array([1.0], np.float32), np.zeros(1, np.float32))
+    assert abs(d[0] + 10.0) < 1e-4
+    d = orc.dfun(L.SUP_HOPF, [-0.5, 1.0], np.zeros(2, np.float32), np.zeros(2, np.float32))
+    assert np.all(np.abs(d) < 1e-5)
+    # closed forms (dfun.rs:522-557): dx = gamma x + c; the Hopf normal form
+    d = orc.dfun(L.LINEAR, [-3.0], np.array([0.5], np.float32), np.array([0.25], np.float32))
+    assert d[0] == np.float32(-1.25)
+    x, c = np.array([0.3, -0.4], np.float32), np.array([0.1, 0.2], np.float32)
+    d = orc.dfun(L.SUP_HOPF, [-0.5, 1.0], x, c)
+    np.testing.assert_allclose(d, [0.275, 0.8], rtol=1e-6)
+    # without coupling the radial part is (a - r^2) r^2 and the rotation drops out
+    d = orc.dfun(L.SUP_HOPF, [0.7, 2.5], x, np.zeros(2, np.float32))
+    r2 = float(x[0]) ** 2 + float(x[1]) ** 2
+    assert abs(float(x[0] * d[0] + x[1] * d[1]) - (0.7 - r2) * r2) < 1e-6
+    for m, p, x in ((L.LINEAR, [-10.0], [0.7]), (L.SUP_HOPF, [-0.5, 1.0], [0.3, -0.2])):
+        x = np.array(x, np.float32)
+        c = np.full(len(x), 0.05, np.float32)
+        np.testing.assert_allclose(orc.dfun(m, p, x, c), orc.dfun(m, p, x, c, dtype=np.float64), rtol=1e-6)
+
+
 def test_models_fp32_vs_fp64_twins_agree(orc):
     rng = np.random.default_rng(0)
     cases = [(L.G2DO, G2DO_DEFAULT, 2, 1), (L.MPR, MPR_DEFAULT, 2, 2), (L.JANSEN_RIT, JR_DEFAULT, 6, 1),
