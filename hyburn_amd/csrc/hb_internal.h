@@ -26,12 +26,12 @@ constexpr int kCC = 12;   // coupling constant slots
 HB_HD int model_nvar(int m) {
     return m == HB_JANSEN_RIT ? 6 : (m == HB_KURAMOTO || m == HB_RWW || m == HB_LINEAR) ? 1 : 2;
 }
-HB_HD int model_ncvar(int m) { return (m == HB_MPR || m == HB_SUP_HOPF) ? 2 : 1; }
+HB_HD int model_ncvar(int m) { return (m == HB_MPR || m == HB_SUP_HOPF || m == HB_HOPFIELD) ? 2 : 1; }
 HB_HD int model_nparams(int m) {
     switch (m) {
     case HB_G2DO: return 12; case HB_MPR: return 7; case HB_JANSEN_RIT: return 13;
     case HB_WILSON_COWAN: return 22; case HB_KURAMOTO: return 1; case HB_LINEAR: return 1;
-    case HB_SUP_HOPF: return 2; default: return 8;
+    case HB_SUP_HOPF: return 2; case HB_HOPFIELD: return 3; default: return 8;
     }
 }
 // The reference's single batch-sweepable parameter per model (dfun.rs:49-72).
@@ -119,6 +119,12 @@ HB_HD void model_consts_t(int m, const S* p, S* mc) {
     case HB_SUP_HOPF:
         mc[0] = p[0]; mc[1] = p[1];
         break;
+    case HB_HOPFIELD: {   // dfun.rs hopfield_dfun_batch: 1 / max(tau, f32::EPSILON), dynamic = p[2] > 0.5
+        const S e32 = S(1.1920929e-07f);
+        mc[0] = one / (p[0] > e32 ? p[0] : e32);
+        mc[1] = one / (p[1] > e32 ? p[1] : e32);
+        mc[2] = p[2] > S(0.5f) ? one : S(0.0f);
+        break; }
     default: // RWW
         mc[0] = p[0]; mc[1] = p[1]; mc[2] = p[2]; mc[3] = p[3];
         mc[4] = one / p[4];

@@ -16,6 +16,7 @@ template <> struct Tr<HB_KURAMOTO> { static constexpr int NV = 1, NC = 1; };
 template <> struct Tr<HB_RWW> { static constexpr int NV = 1, NC = 1; };
 template <> struct Tr<HB_LINEAR> { static constexpr int NV = 1, NC = 1; };
 template <> struct Tr<HB_SUP_HOPF> { static constexpr int NV = 2, NC = 2; };
+template <> struct Tr<HB_HOPFIELD> { static constexpr int NV = 2, NC = 2; };
 
 // ---------------------------------------------------------------- models ---
 
@@ -114,6 +115,14 @@ template <> __device__ __forceinline__ void dfun<HB_SUP_HOPF>(const float* mc, b
     const float g = -amp + mc[0];
     dx[0] = (g * x[0] - x[1] * mc[1]) + c[0];
     dx[1] = (g * x[1] + x[0] * mc[1]) + c[1];
+}
+
+// hopfield_dfun_batch (dfun.rs): dx = (-x + c0) / taux; dtheta = (-theta + c1) / tauT
+// when dynamic, else 0
+template <> __device__ __forceinline__ void dfun<HB_HOPFIELD>(const float* mc, bool sw, float sv,
+                                                              const float* x, const float* c, float* dx) {
+    dx[0] = (-x[0] + c[0]) * mc[0];
+    dx[1] = mc[2] != 0.0f ? (-x[1] + c[1]) * mc[1] : 0.0f;
 }
 
 // clamp_batch, dfun.rs:170-239. Ternaries keep NaN like f32::clamp.

@@ -23,8 +23,9 @@ typedef struct {
          w_ci, w_thi, w_re, w_ri, w_ke, w_ki, w_p, w_q, w_ale, w_ali, w_off_e, w_off_i;
     /* Kuramoto (dfun.rs:480-492) */
     REAL k_omega;
-    /* Linear, SupHopf (dfun.rs:522-557) */
-    REAL l_gamma, h_a, h_omega;
+    /* Linear, SupHopf (dfun.rs:522-557), Hopfield (hopfield_dfun_batch) */
+    REAL l_gamma, h_a, h_omega, p_itx, p_itt;
+    int p_dyn;
     /* RWW   (dfun.rs:494-516) */
     REAL r_a, r_b, r_d, r_gamma, r_inv_taus, r_wjn, r_jn, r_io;
     int sweep_form;   /* 1: reference's batch-native swept form (dfun.rs:49-72) */
@@ -91,6 +92,12 @@ static void FN(mconst_init)(FN(mconst)* k, int model, const float* pf, int sweep
     case ORC_SH:
         k->h_a = p[0]; k->h_omega = p[1];
         break;
+    case ORC_HOP: {
+        REAL e32 = (REAL)1.1920929e-07f;
+        k->p_itx = (REAL)1 / (p[0] > e32 ? p[0] : e32);
+        k->p_itt = (REAL)1 / (p[1] > e32 ? p[1] : e32);
+        k->p_dyn = p[2] > (REAL)0.5;
+        break; }
     case ORC_KUR:
         k->k_omega = p[0];
         break;
@@ -172,6 +179,10 @@ static void FN(dfun)(const FN(mconst)* k, const REAL* x, const REAL* c, REAL* dx
         dx[0] = (g * x[0] - x[1] * k->h_omega) + c[0];
         dx[1] = (g * x[1] + x[0] * k->h_omega) + c[1];
         break; }
+    case ORC_HOP: /* hopfield_dfun_batch */
+        dx[0] = (-x[0] + c[0]) * k->p_itx;
+        dx[1] = k->p_dyn ? (-x[1] + c[1]) * k->p_itt : (REAL)0;
+        break;
     case ORC_RWW: { /* dfun.rs:503-515 */
         REAL s = x[0];
         REAL xx = k->sweep_form ? (s * k->r_wjn + k->sweep_val) + c[0] * k->r_jn

@@ -302,17 +302,20 @@ def test_plain_per_edge_csr_delays(gpu, orc):
 
 
 def test_linear_and_sup_hopf_subnetworks(gpu, orc):
-    # Linear and SupHopf (dfun.rs:522-557) run on the multi-subnetwork path:
-    # a Hopf ring driving a G2DO subnetwork and a Linear relay, with channel remaps
+    # Linear, SupHopf (dfun.rs:522-557) and Hopfield run on the multi-subnetwork
+    # path: a Hopf ring driving G2DO, Linear and Hopfield subnetworks, with remaps
     from hyburn_amd.synth import LINEAR_DEFAULT, SUP_HOPF_DEFAULT
     rng = np.random.default_rng(88)
     subs = [(L.SUP_HOPF, 6, np.array([0.4, 2.0], np.float32), _x0(rng, 6, 2)),
-            (L.G2DO, 6, G2DO_DEFAULT, _x0(rng, 6, 2)), (L.LINEAR, 6, LINEAR_DEFAULT, _x0(rng, 6, 1))]
+            (L.G2DO, 6, G2DO_DEFAULT, _x0(rng, 6, 2)), (L.LINEAR, 6, LINEAR_DEFAULT, _x0(rng, 6, 1)),
+            (L.HOPFIELD, 6, np.array([1.5, 3.0, 1.0], np.float32), _x0(rng, 6, 2))]
     W = (rng.uniform(size=(6, 6)) * 0.2).astype(np.float32)
     links = [(0, 0, W, L.CF_LINEAR, [0.3, 0.0], [(0, 0)]),            # 2 channels pass through
              (0, 1, 0.1, L.CF_LINEAR, [1.0], [(1, 0)]),               # Hopf y -> G2DO
              (1, 2, 0.2, L.CF_LINEAR, [1.0], [(0, 0)]),
-             (2, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 1)])]              # Linear x -> Hopf channel 1
+             (2, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 1)]),              # Linear x -> Hopf channel 1
+             (0, 3, 0.3, L.CF_LINEAR, [1.0], [(0, 0), (1, 1)]),       # Hopf -> Hopfield, both channels
+             (3, 3, W, L.CF_SIGMOIDAL, [], [(0, 0)])]
     for integ in (L.HEUN, L.RK4):
         res = _run(subs, links, dt=0.05, n_steps=150, integrator=integ, n_sweep=2)
         tav, fin = orc.run_multi(subs, links, 0.05, 150, integ, False, n_sweep=2)

@@ -136,6 +136,18 @@ def test_linear_and_sup_hopf_kats(orc):
         np.testing.assert_allclose(orc.dfun(m, p, x, c), orc.dfun(m, p, x, c, dtype=np.float64), rtol=1e-6)
 
 
+def test_hopfield_kats(orc):
+    # hopfield.rs:46-55: static (dynamic = 0), x = 2, c = 1 -> dx = -1
+    d = orc.dfun(L.HOPFIELD, [1.0, 5.0, 0.0], np.array([2.0, 0.0], np.float32), np.array([1.0, 0.0], np.float32))
+    assert abs(d[0] + 1.0) < 1e-4 and d[1] == 0.0
+    # dynamic: dtheta = (-theta + c1) / tauT; static: 0; taux floored at f32 epsilon
+    x, c = np.array([0.5, 0.3], np.float32), np.array([0.1, 0.2], np.float32)
+    np.testing.assert_allclose(orc.dfun(L.HOPFIELD, [2.0, 4.0, 1.0], x, c), [-0.2, -0.025], rtol=1e-6)
+    assert orc.dfun(L.HOPFIELD, [2.0, 4.0, 0.0], x, c)[1] == 0.0
+    d = orc.dfun(L.HOPFIELD, [0.0, 4.0, 1.0], x, c)
+    assert d[0] == np.float32(np.float32(-0.4) * np.float32(1.0 / np.float32(1.1920929e-07)))
+
+
 def test_models_fp32_vs_fp64_twins_agree(orc):
     rng = np.random.default_rng(0)
     cases = [(L.G2DO, G2DO_DEFAULT, 2, 1), (L.MPR, MPR_DEFAULT, 2, 2), (L.JANSEN_RIT, JR_DEFAULT, 6, 1),
