@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libhyburn_hip.so")
 # ---- constants (mirror include/hyburn_hip.h) ----
 HB_OK, HB_INVALID_CONFIG, HB_INVALID_STATE, HB_BACKEND_ERROR, HB_IO_ERROR = 0, 1, 2, 3, 4
 G2DO, MPR, JANSEN_RIT, WILSON_COWAN, KURAMOTO, RWW = 0, 1, 2, 3, 4, 5
-LINEAR, SUP_HOPF, HOPFIELD = 6, 7, 8   # on the multi-subnetwork path only
+LINEAR, SUP_HOPF, HOPFIELD, COOMBES_BYRNE_2D = 6, 7, 8, 9   # on the multi-subnetwork path only
 EULER, HEUN, RK4, EULER_STOCH, HEUN_STOCH, RK4_STOCH = 0, 1, 2, 3, 4, 5
 CF_LINEAR, CF_DIFFERENCE, CF_KURAMOTO, CF_SIGMOIDAL, CF_SCALED_LINEAR, CF_TANH, CF_SJR, CF_PRESIGMOIDAL = range(8)
 W_NONE, W_SCALAR_MEAN, W_SCALAR_DIAG, W_DENSE, W_CSR = 0, 1, 2, 3, 4
@@ -28,14 +28,15 @@ LOSS_FC, LOSS_BOLD_MSE, LOSS_ORDER = 0, 1, 2
 MODEL_NAMES = {
     "Generic2dOscillator": G2DO, "MontbrioPazoRoxin": MPR, "JansenRit": JANSEN_RIT,
     "WilsonCowan": WILSON_COWAN, "Kuramoto": KURAMOTO, "ReducedWongWang": RWW,
-    "Linear": LINEAR, "SupHopf": SUP_HOPF, "Hopfield": HOPFIELD,
+    "Linear": LINEAR, "SupHopf": SUP_HOPF, "Hopfield": HOPFIELD, "CoombesByrne2D": COOMBES_BYRNE_2D,
 }
 # (nvar, ncvar, nparams), config.rs:43-50
 MODEL_INFO = {G2DO: (2, 1, 12), MPR: (2, 2, 7), JANSEN_RIT: (6, 1, 13),
               WILSON_COWAN: (2, 1, 22), KURAMOTO: (1, 1, 1), RWW: (1, 1, 8),
-              LINEAR: (1, 1, 1), SUP_HOPF: (2, 2, 2), HOPFIELD: (2, 2, 3)}
+              LINEAR: (1, 1, 1), SUP_HOPF: (2, 2, 2), HOPFIELD: (2, 2, 3),
+              COOMBES_BYRNE_2D: (2, 2, 4)}
 # the batch-native sweep index per model, dfun.rs:49-72
-SWEEP_INDEX = {G2DO: 1, MPR: 4, JANSEN_RIT: 12, WILSON_COWAN: 18, KURAMOTO: 0, RWW: 7, LINEAR: -1, SUP_HOPF: -1, HOPFIELD: -1}
+SWEEP_INDEX = {G2DO: 1, MPR: 4, JANSEN_RIT: 12, WILSON_COWAN: 18, KURAMOTO: 0, RWW: 7, LINEAR: -1, SUP_HOPF: -1, HOPFIELD: -1, COOMBES_BYRNE_2D: -1}
 COUPLING_NAMES = {
     "Linear": CF_LINEAR, "Difference": CF_DIFFERENCE, "Kuramoto": CF_KURAMOTO,
     "Sigmoidal": CF_SIGMOIDAL, "ScaledLinear": CF_SCALED_LINEAR, "HyperbolicTangent": CF_TANH,

@@ -26,12 +26,15 @@ constexpr int kCC = 12;   // coupling constant slots
 HB_HD int model_nvar(int m) {
     return m == HB_JANSEN_RIT ? 6 : (m == HB_KURAMOTO || m == HB_RWW || m == HB_LINEAR) ? 1 : 2;
 }
-HB_HD int model_ncvar(int m) { return (m == HB_MPR || m == HB_SUP_HOPF || m == HB_HOPFIELD) ? 2 : 1; }
+HB_HD int model_ncvar(int m) {
+    return (m == HB_MPR || m == HB_SUP_HOPF || m == HB_HOPFIELD || m == HB_COOMBES_BYRNE_2D) ? 2 : 1;
+}
 HB_HD int model_nparams(int m) {
     switch (m) {
     case HB_G2DO: return 12; case HB_MPR: return 7; case HB_JANSEN_RIT: return 13;
     case HB_WILSON_COWAN: return 22; case HB_KURAMOTO: return 1; case HB_LINEAR: return 1;
-    case HB_SUP_HOPF: return 2; case HB_HOPFIELD: return 3; default: return 8;
+    case HB_SUP_HOPF: return 2; case HB_HOPFIELD: return 3; case HB_COOMBES_BYRNE_2D: return 4;
+    default: return 8;
     }
 }
 // The reference's single batch-sweepable parameter per model (dfun.rs:49-72).
@@ -125,6 +128,9 @@ HB_HD void model_consts_t(int m, const S* p, S* mc) {
         mc[1] = one / (p[1] > e32 ? p[1] : e32);
         mc[2] = p[2] > S(0.5f) ? one : S(0.0f);
         break; }
+    case HB_COOMBES_BYRNE_2D:   // coombes_byrne2d_dfun_batch: delta / pi, k * pi, pi * pi as f32
+        mc[0] = p[0] / pi; mc[1] = p[2] * pi; mc[2] = p[3]; mc[3] = pi * pi; mc[4] = p[1];
+        break;
     default: // RWW
         mc[0] = p[0]; mc[1] = p[1]; mc[2] = p[2]; mc[3] = p[3];
         mc[4] = one / p[4];

@@ -17,6 +17,7 @@ template <> struct Tr<HB_RWW> { static constexpr int NV = 1, NC = 1; };
 template <> struct Tr<HB_LINEAR> { static constexpr int NV = 1, NC = 1; };
 template <> struct Tr<HB_SUP_HOPF> { static constexpr int NV = 2, NC = 2; };
 template <> struct Tr<HB_HOPFIELD> { static constexpr int NV = 2, NC = 2; };
+template <> struct Tr<HB_COOMBES_BYRNE_2D> { static constexpr int NV = 2, NC = 2; };
 
 // ---------------------------------------------------------------- models ---
 
@@ -125,6 +126,16 @@ template <> __device__ __forceinline__ void dfun<HB_HOPFIELD>(const float* mc, b
     dx[1] = mc[2] != 0.0f ? (-x[1] + c[1]) * mc[1] : 0.0f;
 }
 
+// coombes_byrne2d_dfun_batch: dr = ((v (2 r) + delta/pi) - r^2 k pi) + c_r;
+// dv = ((-v^2 + eta) - r^2 pi^2) + ((-v + v_syn) k pi) r   (c_v unused)
+template <> __device__ __forceinline__ void dfun<HB_COOMBES_BYRNE_2D>(const float* mc, bool sw, float sv,
+                                                                      const float* x, const float* c,
+                                                                      float* dx) {
+    const float r = x[0], v = x[1], r2 = r * r;
+    dx[0] = ((v * (r * 2.0f) + mc[0]) - r2 * mc[1]) + c[0];
+    dx[1] = ((-(v * v) + mc[2]) - r2 * mc[3]) + ((-v + mc[4]) * mc[1]) * r;
+}
+
 // clamp_batch, dfun.rs:170-239. Ternaries keep NaN like f32::clamp.
 template <int M> __device__ __forceinline__ void clampm(float* x) {
     if constexpr (M == HB_WILSON_COWAN) {
@@ -134,6 +145,8 @@ template <int M> __device__ __forceinline__ void clampm(float* x) {
         x[0] = x[0] < 0.0f ? 0.0f : x[0];
     } else if constexpr (M == HB_RWW) {
         x[0] = x[0] < 0.0f ? 0.0f : (x[0] > 1.0f ? 1.0f : x[0]);
+    } else if constexpr (M == HB_COOMBES_BYRNE_2D) {   // dfun.rs:190-194: r clamped at 0
+        x[0] = x[0] < 0.0f ? 0.0f : x[0];
     } else if constexpr (M == HB_KURAMOTO) {
         const float tp = 2.0f * kPi;
         x[0] = x[0] - floorf(x[0] / tp) * tp;

@@ -134,6 +134,7 @@ __device__ __forceinline__ void step_model(int model, const StepCtx& S, const fl
     case HB_LINEAR: integrate<HB_LINEAR>(S, mc, cpl, z, x); break;
     case HB_SUP_HOPF: integrate<HB_SUP_HOPF>(S, mc, cpl, z, x); break;
     case HB_HOPFIELD: integrate<HB_HOPFIELD>(S, mc, cpl, z, x); break;
+    case HB_COOMBES_BYRNE_2D: integrate<HB_COOMBES_BYRNE_2D>(S, mc, cpl, z, x); break;
     default: integrate<HB_RWW>(S, mc, cpl, z, x); break;
     }
 }
@@ -391,7 +392,10 @@ int cvar_position(int model, int t) {
     const int* cv = one0;
     int n = 1;
     if (model == HB_G2DO) cv = g2do;
-    else if (model == HB_MPR || model == HB_SUP_HOPF || model == HB_HOPFIELD) { cv = mpr; n = 2; }
+    else if (model == HB_MPR || model == HB_SUP_HOPF || model == HB_HOPFIELD || model == HB_COOMBES_BYRNE_2D) {
+        cv = mpr;
+        n = 2;
+    }
     else if (model == HB_JANSEN_RIT) cv = jr;
     for (int k = 0; k < n; ++k)
         if (cv[k] == t) return k;
@@ -433,7 +437,7 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     int n_tot = 0;
     for (int q = 0; q < Q.n_subnets; ++q) {
         const hb_subnetwork& U = Q.subnets[q];
-        if (U.model < HB_G2DO || U.model > HB_HOPFIELD) return bad("unknown model in subnetwork " + std::to_string(q));
+        if (U.model < HB_G2DO || U.model > HB_COOMBES_BYRNE_2D) return bad("unknown model in subnetwork " + std::to_string(q));
         if (U.n_nodes < 1 || !U.params || !U.x0) return bad("subnetwork " + std::to_string(q) + ": nodes, params, x0");
         Sub& S = A.sub[q];
         S.model = U.model;

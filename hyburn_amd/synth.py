@@ -68,6 +68,7 @@ RWW_DEFAULT = np.array([0.270, 0.108, 154, 0.641, 100, 0.6, 0.2609, 0.33], np.fl
 LINEAR_DEFAULT = np.array([-10.0], np.float32)                                      # linear.rs:33-35
 SUP_HOPF_DEFAULT = np.array([-0.5, 1.0], np.float32)                                # sup_hopf.rs:35-37
 HOPFIELD_DEFAULT = np.array([1.0, 5.0, 0.0], np.float32)                            # hopfield.rs:36-38
+CB2D_DEFAULT = np.array([1.0, -4.0, 1.0, 2.0], np.float32)                          # coombes_byrne2d.rs:41-43
 DEFAULTS = {L.G2DO: G2DO_DEFAULT, L.MPR: MPR_DEFAULT, L.JANSEN_RIT: JR_DEFAULT,
             L.WILSON_COWAN: WC_DEFAULT, L.KURAMOTO: KUR_DEFAULT, L.RWW: RWW_DEFAULT}
 

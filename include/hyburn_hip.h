@@ -30,7 +30,7 @@ extern "C" {
 /* Models (config.rs:43-50 registry order of the six batch-native models). */
 enum { HB_G2DO = 0, HB_MPR = 1, HB_JANSEN_RIT = 2, HB_WILSON_COWAN = 3, HB_KURAMOTO = 4, HB_RWW = 5,
        /* only on the multi-subnetwork path (hb_run_multi) */
-       HB_LINEAR = 6, HB_SUP_HOPF = 7, HB_HOPFIELD = 8 };
+       HB_LINEAR = 6, HB_SUP_HOPF = 7, HB_HOPFIELD = 8, HB_COOMBES_BYRNE_2D = 9 };
 /* IntegratorKind (integrator.rs:17-30). */
 enum { HB_EULER = 0, HB_HEUN = 1, HB_RK4 = 2, HB_EULER_STOCH = 3, HB_HEUN_STOCH = 4, HB_RK4_STOCH = 5 };
 /* CouplingFnConfig (coupling.rs:53-71). */

@@ -305,7 +305,7 @@ def fcd(traj, window_len, stride):
     return out if rc > 1 else np.zeros((0, 0), np.float32)
 
 
-_CVAR = {0: (0,), 1: (0, 1), 2: (1,), 3: (0,), 4: (0,), 5: (0,), 6: (0,), 7: (0, 1), 8: (0, 1)}   # CVAR of src/model/*.rs
+_CVAR = {0: (0,), 1: (0, 1), 2: (1,), 3: (0,), 4: (0,), 5: (0,), 6: (0,), 7: (0, 1), 8: (0, 1), 9: (0, 1)}   # CVAR of src/model/*.rs
 _IDENTITY_PRE = (0, 1, 3, 4)   # Linear, Difference, Sigmoidal, ScaledLinear: pre is the identity
 
 

@@ -24,7 +24,7 @@ typedef struct {
     /* Kuramoto (dfun.rs:480-492) */
     REAL k_omega;
     /* Linear, SupHopf (dfun.rs:522-557), Hopfield (hopfield_dfun_batch) */
-    REAL l_gamma, h_a, h_omega, p_itx, p_itt;
+    REAL l_gamma, h_a, h_omega, p_itx, p_itt, cb_dpi, cb_kpi, cb_eta, cb_pi2, cb_vsyn;
     int p_dyn;
     /* RWW   (dfun.rs:494-516) */
     REAL r_a, r_b, r_d, r_gamma, r_inv_taus, r_wjn, r_jn, r_io;
@@ -97,6 +97,10 @@ static void FN(mconst_init)(FN(mconst)* k, int model, const float* pf, int sweep
         k->p_itx = (REAL)1 / (p[0] > e32 ? p[0] : e32);
         k->p_itt = (REAL)1 / (p[1] > e32 ? p[1] : e32);
         k->p_dyn = p[2] > (REAL)0.5;
+        break; }
+    case ORC_CB2: { /* coombes_byrne2d_dfun_batch, f32 host products */
+        REAL pi = PIR;
+        k->cb_dpi = p[0] / pi; k->cb_kpi = p[2] * pi; k->cb_eta = p[3]; k->cb_pi2 = pi * pi; k->cb_vsyn = p[1];
         break; }
     case ORC_KUR:
         k->k_omega = p[0];
@@ -183,6 +187,11 @@ static void FN(dfun)(const FN(mconst)* k, const REAL* x, const REAL* c, REAL* dx
         dx[0] = (-x[0] + c[0]) * k->p_itx;
         dx[1] = k->p_dyn ? (-x[1] + c[1]) * k->p_itt : (REAL)0;
         break;
+    case ORC_CB2: { /* coombes_byrne2d_dfun_batch */
+        REAL r = x[0], v = x[1], r2 = r * r;
+        dx[0] = ((v * (r * (REAL)2) + k->cb_dpi) - r2 * k->cb_kpi) + c[0];
+        dx[1] = ((-(v * v) + k->cb_eta) - r2 * k->cb_pi2) + ((-v + k->cb_vsyn) * k->cb_kpi) * r;
+        break; }
     case ORC_RWW: { /* dfun.rs:503-515 */
         REAL s = x[0];
         REAL xx = k->sweep_form ? (s * k->r_wjn + k->sweep_val) + c[0] * k->r_jn
@@ -209,6 +218,9 @@ static void FN(clamp)(int model, REAL* x)
         break;
     case ORC_RWW:
         x[0] = x[0] < 0 ? 0 : (x[0] > 1 ? 1 : x[0]);
+        break;
+    case ORC_CB2: /* dfun.rs:190-194 */
+        x[0] = x[0] < 0 ? 0 : x[0];
         break;
     case ORC_KUR: { /* dfun.rs:188-191 */
         REAL tp = (REAL)2 * PIR;

@@ -302,8 +302,8 @@ def test_plain_per_edge_csr_delays(gpu, orc):
 
 
 def test_linear_and_sup_hopf_subnetworks(gpu, orc):
-    # Linear, SupHopf (dfun.rs:522-557) and Hopfield run on the multi-subnetwork
-    # path: a Hopf ring driving G2DO, Linear and Hopfield subnetworks, with remaps
+    # Linear, SupHopf (dfun.rs:522-557), Hopfield and CoombesByrne2D run on the
+    # multi-subnetwork path: a Hopf ring driving the others, with channel remaps
     from hyburn_amd.synth import LINEAR_DEFAULT, SUP_HOPF_DEFAULT
     rng = np.random.default_rng(88)
     subs = [(L.SUP_HOPF, 6, np.array([0.4, 2.0], np.float32), _x0(rng, 6, 2)),
@@ -321,3 +321,11 @@ def test_linear_and_sup_hopf_subnetworks(gpu, orc):
         tav, fin = orc.run_multi(subs, links, 0.05, 150, integ, False, n_sweep=2)
         _check(res, tav, fin)
     assert SUP_HOPF_DEFAULT.shape == (2,)
+    # CoombesByrne2D with a G2DO driver (r clamped at 0)
+    subs2 = [(L.COOMBES_BYRNE_2D, 6, np.array([1.0, -4.0, 1.0, 2.0], np.float32), np.abs(_x0(rng, 6, 2, 0.2))),
+             (L.G2DO, 6, G2DO_DEFAULT, _x0(rng, 6, 2))]
+    links2 = [(0, 0, 0.1, L.CF_LINEAR, [0.5, 0.0], [(0, 0)]),            # both channels pass through
+              (1, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 1)]), (0, 1, 0.05, L.CF_LINEAR, [1.0], [(0, 0)])]
+    res = _run(subs2, links2, dt=0.02, n_steps=200, integrator=L.HEUN, n_sweep=2)
+    tav, fin = orc.run_multi(subs2, links2, 0.02, 200, L.HEUN, False, n_sweep=2)
+    _check(res, tav, fin)

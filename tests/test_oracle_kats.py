@@ -148,6 +148,22 @@ def test_hopfield_kats(orc):
     assert d[0] == np.float32(np.float32(-0.4) * np.float32(1.0 / np.float32(1.1920929e-07)))
 
 
+def test_coombes_byrne_2d_kats(orc):
+    # coombes_byrne2d.rs:51-64: at zero, dr = Delta / pi, dV = eta
+    p = [1.0, -4.0, 1.0, 2.0]
+    d = orc.dfun(L.COOMBES_BYRNE_2D, p, np.zeros(2, np.float32), np.zeros(2, np.float32))
+    assert abs(d[0] - 1.0 / math.pi) < 1e-4 and abs(d[1] - 2.0) < 1e-4
+    # the written formula at a state, in f64 (c_v does not enter)
+    r, v, cr = 0.3, -0.7, 0.05
+    dl, vs, k, eta = 0.8, -3.0, 1.5, 1.2
+    want = [v * 2 * r + dl / math.pi - r * r * k * math.pi + cr,
+            -v * v + eta - r * r * math.pi ** 2 + (-v + vs) * k * math.pi * r]
+    d = orc.dfun(L.COOMBES_BYRNE_2D, [dl, vs, k, eta], np.array([r, v], np.float32), np.array([cr, 9.0], np.float32))
+    np.testing.assert_allclose(d, want, rtol=1e-5)
+    # clamp_batch (dfun.rs:190-194): r >= 0, V free
+    np.testing.assert_array_equal(orc.clamp(L.COOMBES_BYRNE_2D, [-0.5, -2.0]), [0.0, -2.0])
+
+
 def test_models_fp32_vs_fp64_twins_agree(orc):
     rng = np.random.default_rng(0)
     cases = [(L.G2DO, G2DO_DEFAULT, 2, 1), (L.MPR, MPR_DEFAULT, 2, 2), (L.JANSEN_RIT, JR_DEFAULT, 6, 1),
