@@ -60,13 +60,25 @@ def _compile(src: str, obj: str, extra: list[str], device: bool) -> tuple[str, i
     return obj, p.returncode, err
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> str:
-    """Compile every unit for gfx950 and link libhyburn_hip.so in-tree."""
+def _stale(src: str, obj: str) -> bool:
+    """An object is stale when its source or any header is newer than it."""
+    o = os.path.join(BUILD, obj)
+    if not os.path.exists(o):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(d, f) for d in (CSRC, os.path.join(HERE, "..", "include"))
+                                        for f in os.listdir(d) if f.endswith(".h")]
+    return max(os.path.getmtime(d) for d in deps) > os.path.getmtime(o)
+
+
+def build(verbose: bool = False, jobs: int | None = None, incremental: bool = False) -> str:
+    """Compile every unit for gfx950 and link libhyburn_hip.so in-tree
+    (incremental: only units whose source or headers changed since their object)."""
     os.makedirs(BUILD, exist_ok=True)
     units = [u for u in UNITS if os.path.exists(os.path.join(CSRC, u[0]))]
-    jobs = jobs or min(os.cpu_count() or 8, 16, len(units))
+    todo = [u for u in units if _stale(u[0], u[1])] if incremental else units
+    jobs = jobs or min(os.cpu_count() or 8, 16, max(1, len(todo)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(lambda u: _compile(*u), units))
+        results = list(ex.map(lambda u: _compile(*u), todo))
     failed = [r for r in results if r[1] != 0]
     for obj, rc, err in results:
         if err.strip() and (verbose or rc != 0):
@@ -84,4 +96,4 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="-i" in sys.argv))

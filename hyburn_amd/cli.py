@@ -24,7 +24,7 @@ import sys
 import numpy as np
 
 from . import _lib as L
-from .engine import (BatchHybridEngine, SimConfig, SweepConfig, SweepParam, parse_sweep_param,
+from .engine import (BatchHybridEngine, SimConfig, SweepConfig, SweepParam, _needs_multi, parse_sweep_param,
                      write_npy_f32)
 
 
@@ -60,8 +60,9 @@ def run(config: str, output: str, backend: str = "hip", sweep: str | None = None
             # cli.rs:958-970 labels the [n_sweep, nnodes, nvar] data as [n_sweep, nvar, nnodes]
             write_npy_f32(path, t, [res.n_sweep, nvar, nnodes])
             written[f"tavg_sub{i}"] = path
-    elif len(cfg.subnetworks) > 1 or cfg.stimuli or any(p.src != p.tgt or p.cvar_map != "0:0"
-                                                        for p in cfg.projections) or len(cfg.projections) > 1:
+    elif _needs_multi(cfg) or cfg.stimuli:
+        # several subnetworks, links between them, cvar_map remaps, stimuli or
+        # a model only the multi-subnetwork kernel integrates
         written.update(_plain_multi(cfg, n_steps, output, noise_seed, device))
     else:
         # HybridEngine reads an inline initial state var-major [nvar][nnodes]

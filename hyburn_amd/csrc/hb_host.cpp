@@ -162,6 +162,7 @@ int validate_instance(const hb_instance& I, int idx) {
     if (!I.params || !I.x0) return bad("params and x0 are required");
     if (I.wkind < HB_W_NONE || I.wkind > HB_W_CSR) return bad("unknown weight kind");
     if (I.cfn < HB_CF_LINEAR || I.cfn > HB_CF_PRESIGMOIDAL) return bad("unknown coupling function");
+    if (const char* why = presigmoidal_unsupported(I.cfn, I.n_cparams, I.cparams)) return bad(why);
     const int nc = model_ncvar(I.model);
     const int np = coupling_npre(I.cfn, nc);
     if (np > 2 || (I.cfn == HB_CF_SJR && nc != 1)) return bad("unsupported model/coupling-function combination");

@@ -178,6 +178,22 @@ inline void coupling_consts(int cfn, const float* P, int n, int n_src, float* cc
     }
 }
 
+// PreSigmoidal with a dynamic threshold (coupling_params[5] != 0,
+// coupling.rs:191-198, 238-251) collapses two source cvars into one pre
+// channel. The reference rejects it for one-cvar sources (construction.rs:
+// 731-740), and for the two-cvar model of this path (MPR) its one-channel
+// coupling cannot feed the dfun, which reads coupling channel 1 (dfun.rs:
+// 451-452), nor a cvar_map remap, which indexes it with the source's cvar
+// stride (engine.rs:637-660). No configuration runs it, so it is rejected
+// here rather than ignored. global_t (coupling_params[6]) only acts with a
+// dynamic threshold.
+inline const char* presigmoidal_unsupported(int cfn, int n_cparams, const float* P) {
+    if (cfn == HB_CF_PRESIGMOIDAL && n_cparams > 5 && P[5] != 0.0f)
+        return "PreSigmoidal with a dynamic threshold (dynamic != 0) is not supported: it collapses the "
+               "source's two cvars into one coupling channel that no model of this path can take";
+    return nullptr;
+}
+
 // Per-instance device descriptor (one per simulation in a plan).
 struct alignas(16) DevInst {
     int32_t model, integrator, n_nodes, n_steps;

@@ -488,6 +488,7 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         }
         if (R.cfn < HB_CF_LINEAR || R.cfn > HB_CF_PRESIGMOIDAL) return bad(tag + ": unknown coupling function");
         if (R.n_cparams < 0 || R.n_cparams > 8) return bad(tag + ": at most 8 coupling parameters");
+        if (const char* why = presigmoidal_unsupported(R.cfn, R.n_cparams, R.cparams)) return bad(tag + ": " + why);
         const int ms = Q.subnets[R.src].model, mt = Q.subnets[R.tgt].model;
         const int ncs = model_ncvar(ms), nct = model_ncvar(mt);
         const int ns = Q.subnets[R.src].n_nodes, nt = Q.subnets[R.tgt].n_nodes;

@@ -376,6 +376,16 @@ class BatchHybridEngine:
                     noise_shared: bool = True) -> "BatchHybridEngine":
         cfg = base_config
         if _needs_multi(cfg):
+            # the multi-subnetwork branch runs the batch engine's step; the plain
+            # engine's multi-subnetwork step is run_multi(plain=True), which the
+            # run command calls, and the single-network delay/noise options do
+            # not apply to it
+            if engine != L.ENGINE_BATCH:
+                _err(L.HB_INVALID_CONFIG, "plain-engine runs of this configuration go through "
+                                          "hyburn_amd.multi.run_multi(plain=True), not from_config")
+            if delay_mode != -1 or not noise_shared:
+                _err(L.HB_INVALID_CONFIG, "delay_mode and noise_shared apply to single-network "
+                                          "configurations only")
             self = cls()
             if cfg.integrator not in L.INTEGRATOR_NAMES:
                 _err(L.HB_INVALID_CONFIG, f"unknown integrator {cfg.integrator}")
@@ -578,14 +588,22 @@ class BatchHybridEngine:
             if (nv, nn, nm) != want:
                 _err(L.HB_INVALID_STATE, f"Checkpoint shape mismatch for subnetwork {i}: expected "
                                          f"(nvar={want[0]},nnodes={want[1]},nmodes=1), got ({nv},{nn},{nm})")
-        states = c["states"]
+        from .multi import horizon
+        states, hist = c["states"], c["hist"]
+        H = horizon(links, int(self._max_delay))
+        if hist is not None and (hist.shape[1] != sum(S.n_nodes for S in subs) or hist.shape[2] != H):
+            _err(L.HB_INVALID_STATE, f"Checkpoint history has shape {tuple(hist.shape)}, expected "
+                                     f"(n_sweep, {sum(S.n_nodes for S in subs)}, {H}, 2)")
         if c["n_sweep"] != self.n_sweep:
             if c["n_sweep"] != 1:
                 _err(L.HB_INVALID_STATE, f"Checkpoint holds {c['n_sweep']} sweep points, expected {self.n_sweep}")
+            # one saved point continues every sweep point: states and history alike
             states = [np.repeat(a, self.n_sweep, axis=0) for a in states]
+            if hist is not None:
+                hist = np.repeat(hist, self.n_sweep, axis=0)
         self._step, self.dt, self._nsig = int(c["step"]), float(c["dt"]), c["nsig"]
         self._multi = (subs, links, c["integrator"], device)
-        self._state, self._hist = states, c["hist"]
+        self._state, self._hist = states, hist
 
     def close(self):
         if self._h:

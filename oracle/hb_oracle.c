@@ -220,6 +220,31 @@ int hb_orc_coupling_f32(const hb_orc_coupling* cpl, int n_tgt, int n_src, int nc
     return 0;
 }
 
+/* pre() of n rows of nc cvars -> out [n][npre]; returns npre (coupling.rs:204-321).
+ * cpl->cpu_form selects the per-edge pre_cpu op order. */
+int hb_orc_pre_f32(const hb_orc_coupling* cpl, int n, int nc, const float* x, float* out)
+{
+    f32_cconst k;
+    int i, np;
+    f32_cconst_init(&k, cpl);
+    np = f32_npre(&k, nc);
+    for (i = 0; i < n; ++i) f32_pre(&k, x + (size_t)i * nc, nc, out + (size_t)i * np);
+    return np;
+}
+
+/* post_with_target() of n rows: g [n][npre], x_tgt [n][nc] -> out [n][nc]
+ * (coupling.rs:323-399); rowsum is the Difference row sum, 0 otherwise. */
+void hb_orc_post_f32(const hb_orc_coupling* cpl, int n, int nc, const float* g, const float* x_tgt,
+                     float rowsum, float* out)
+{
+    f32_cconst k;
+    int i, np;
+    f32_cconst_init(&k, cpl);
+    np = f32_npre(&k, nc);
+    for (i = 0; i < n; ++i)
+        f32_post(&k, g + (size_t)i * np, x_tgt + (size_t)i * nc, rowsum, nc, out + (size_t)i * nc);
+}
+
 /* CSR matvec, sparse.rs:29-52 */
 void hb_orc_csr_matvec_f32(const float* data, const int32_t* indices, const int32_t* indptr,
                            int n_tgt, const float* x, float* y)

@@ -62,6 +62,9 @@ def lib() -> C.CDLL:
         L.hb_orc_clamp_f32.argtypes = [C.c_int, fp]
         L.hb_orc_coupling_f32.argtypes = [C.POINTER(OrcCoupling), C.c_int, C.c_int, C.c_int, fp, fp, fp]
         L.hb_orc_csr_matvec_f32.argtypes = [fp, ip, ip, C.c_int, fp, fp]
+        L.hb_orc_pre_f32.argtypes = [C.POINTER(OrcCoupling), C.c_int, C.c_int, fp, fp]
+        L.hb_orc_pre_f32.restype = C.c_int
+        L.hb_orc_post_f32.argtypes = [C.POINTER(OrcCoupling), C.c_int, C.c_int, fp, fp, C.c_float, fp]
         L.hb_orc_bold_step_f32.argtypes = [C.c_int, fp, C.c_float, fp, fp, fp, fp]
         L.hb_orc_bold_step_f64.argtypes = [C.c_int] + [C.POINTER(C.c_double)] + [C.c_double] + [C.POINTER(C.c_double)] * 4
         L.hb_orc_bold_signal_f32.argtypes = [C.c_int, fp, fp, fp]
@@ -255,6 +258,37 @@ def clamp(model, x):
     x = np.array(x, np.float32, copy=True)
     lib().hb_orc_clamp_f32(model, _p(x))
     return x
+
+
+def _cfg(cfn, cparams, cpu_form=False, n_src=1):
+    cpl = OrcCoupling()
+    cpl.cfn = cfn
+    cpl.n_cparams = len(cparams)
+    for i, v in enumerate(cparams):
+        cpl.cparams[i] = v
+    cpl.n_src = n_src
+    cpl.cpu_form = int(cpu_form)
+    return cpl
+
+
+def pre(cfn, cparams, x, cpu_form=False):
+    """pre() of each row of x [n][nc] (coupling.rs:204-321) -> [n][npre]."""
+    x = np.ascontiguousarray(x, np.float32)
+    n, nc = x.shape
+    out = np.zeros(n * 2 * nc, np.float32)
+    np_ = lib().hb_orc_pre_f32(C.byref(_cfg(cfn, cparams, cpu_form)), n, nc, _p(x), _p(out))
+    return out[:n * np_].reshape(n, np_).copy()
+
+
+def post(cfn, cparams, g, x_tgt=None, rowsum=0.0, n_src=1):
+    """post_with_target() of each row of g [n][npre] (coupling.rs:323-399) -> [n][nc]."""
+    g = np.ascontiguousarray(g, np.float32)
+    n = g.shape[0]
+    nc = g.shape[1] // 2 if cfn == 2 else (x_tgt.shape[1] if x_tgt is not None else g.shape[1])
+    x_tgt = np.zeros((n, nc), np.float32) if x_tgt is None else np.ascontiguousarray(x_tgt, np.float32)
+    out = np.zeros((n, nc), np.float32)
+    lib().hb_orc_post_f32(C.byref(_cfg(cfn, cparams, n_src=n_src)), n, nc, _p(g), _p(x_tgt), rowsum, _p(out))
+    return out
 
 
 def coupling(cfn, cparams, x_src, x_tgt=None, dense=None, csr=None, n_src=None):

@@ -74,6 +74,13 @@ def _base(**kw):
     (dict(integrator=L.HEUN_STOCH), "nsig"),
     (dict(cfn=L.CF_KURAMOTO, wkind=L.W_DENSE, dense=np.zeros((4, 4), np.float32)), "Kuramoto coupling"),
     (dict(model=L.MPR, params=np.ones(7, np.float32), cfn=L.CF_SJR), "combination"),
+    # PreSigmoidal's dynamic threshold (coupling_params[5]) has no valid
+    # configuration in the reference (construction.rs:731-740, dfun.rs:451-452)
+    (dict(model=L.MPR, params=np.ones(7, np.float32), cfn=L.CF_PRESIGMOIDAL,
+          cparams=(1.0, 1.0, 1.0, 1.0, 0.5, 1.0), wkind=L.W_DENSE, dense=np.zeros((4, 4), np.float32)),
+     "dynamic threshold"),
+    (dict(cfn=L.CF_PRESIGMOIDAL, cparams=(1.0, 1.0, 1.0, 1.0, 0.5, 1.0, 1.0), wkind=L.W_DENSE,
+          dense=np.zeros((4, 4), np.float32)), "dynamic threshold"),
 ])
 def test_plan_validation_errors(kw, needle):
     rc, msg = _plan_error(_base(**kw))

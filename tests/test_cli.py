@@ -163,3 +163,56 @@ def test_plain_run_of_several_subnetworks(gpu, tmp_path):
                   stimuli=[(1, "step", [1.0, 3.0, 0.5])])
     np.testing.assert_array_equal(f1[:, :, 0], r.final_states[1][0].T)
     np.testing.assert_array_equal(tr[-16:-8], r.final_states[0][0].T.ravel())
+
+
+LINEAR_SIM = """
+sim_length = 1.0
+dt = 0.1
+integrator = "heun"
+[network]
+[[network.subnetworks]]
+model = "Linear"
+nnodes = 3
+params = [-10.0]
+initial_state = [0.1, 0.2, 0.3]
+[[network.projections]]
+src = 0
+tgt = 0
+weights = 0.5
+coupling_fn = "Linear"
+coupling_params = [0.01]
+"""
+
+
+def test_plain_run_of_multi_only_model_takes_plain_multi(tmp_path, monkeypatch):
+    """A plain run (no --sweep) of one subnetwork whose model only the
+    multi-subnetwork kernel integrates goes through run_multi(plain=True):
+    w * I Scalar weights and the true delay ring of the plain engine. The
+    kernel call is stubbed, so this runs on the CPU."""
+    import types
+
+    from hyburn_amd import multi
+    seen = {}
+
+    def fake_run_multi(subs, links, **kw):
+        seen.update(kw)
+        n = kw["n_steps"]
+        fin = [np.zeros((1, S.n_nodes, L.MODEL_INFO[S.model][0]), np.float32) for S in subs]
+        trj = [np.zeros((n, 1, S.n_nodes, L.MODEL_INFO[S.model][0]), np.float32) for S in subs]
+        return types.SimpleNamespace(final_states=fin, trajectories=trj)
+
+    monkeypatch.setattr(multi, "run_multi", fake_run_multi)
+    cfg = tmp_path / "lin.toml"
+    cfg.write_text(LINEAR_SIM)
+    out = run(str(cfg), str(tmp_path / "out"))
+    assert seen["plain"] is True and seen["n_steps"] == 10
+    assert np.load(out["state_final_sub0"]).shape == (1, 3, 1)
+
+
+def test_from_config_multi_branch_rejects_single_network_options():
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig
+    cfg = SimConfig.from_toml_str(LINEAR_SIM)
+    for kw in (dict(engine=L.ENGINE_PLAIN), dict(delay_mode=L.DELAY_SINGLE), dict(noise_shared=False)):
+        with pytest.raises(L.HyburnError) as e:
+            BatchHybridEngine.from_config(cfg, 2, **kw)
+        assert e.value.code == L.HB_INVALID_CONFIG

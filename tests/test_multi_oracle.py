@@ -277,3 +277,50 @@ def test_per_edge_plain_composition(orc, integ):
                     indptr=ipt, indices=ind, weights=dat, edge_delays=dl, delay_mode=L.DELAY_PER_EDGE)
     ref = orc.run(inst)
     np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-6, atol=1e-7)
+
+
+def test_resume_one_point_checkpoint_with_history(tmp_path):
+    """A one-point version 3 file with a history continues every sweep point:
+    states and history are both repeated over the batch; a history of the
+    wrong horizon is HB_INVALID_STATE (no GPU call: resume only restores)."""
+    from hyburn_amd import checkpoint as ck
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig
+    toml = """
+sim_length = 1.0
+dt = 0.1
+integrator = "euler"
+[network]
+[[network.subnetworks]]
+model = "Generic2dOscillator"
+nnodes = 3
+params = [1, 0, -2, -10, 0, 0.02, 3, 1, 0, 1, 1, 1]
+initial_state = [0.1, 0.2, 0.3, 0.4, 0.5, 0.6]
+[[network.subnetworks]]
+model = "Kuramoto"
+nnodes = 2
+params = [1.0]
+initial_state = [0.5, 1.0]
+[[network.projections]]
+src = 0
+tgt = 1
+weights = 0.1
+coupling_fn = "Linear"
+coupling_params = [1.0]
+delays = [4]
+"""
+    eng = BatchHybridEngine.from_config(SimConfig.from_toml_str(toml), 3)
+    rng = np.random.default_rng(1)
+    st = [rng.normal(size=(1, 3, 2)).astype(np.float32), rng.normal(size=(1, 2, 1)).astype(np.float32)]
+    hist = rng.normal(size=(1, 5, 5, 2)).astype(np.float32)
+    p = str(tmp_path / "one.bin")
+    ck.write_batch(p, step=12, dt=0.1, integrator=L.EULER, nsig=[0.0], shapes=[(2, 3), (1, 2)], states=st, hist=hist)
+    eng.resume(p)
+    assert eng._step == 12 and eng._hist.shape == (3, 5, 5, 2)
+    for k in range(3):
+        np.testing.assert_array_equal(eng._hist[k], hist[0])
+        np.testing.assert_array_equal(eng._state[0][k], st[0][0])
+    ck.write_batch(p, step=12, dt=0.1, integrator=L.EULER, nsig=[0.0], shapes=[(2, 3), (1, 2)], states=st,
+                   hist=hist[:, :, :3])
+    with pytest.raises(L.HyburnError) as e:
+        eng.resume(p)
+    assert e.value.code == L.HB_INVALID_STATE
