@@ -598,6 +598,10 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     const int bper = OUTS ? V.bold_period : 0;
 
     for (int i = tid; i < NP * NHd; i += BS) ring[i] = 0.0f;   // per-edge: exact-zero fill
+    // every zero is stored before any lane writes a pre value into the ring:
+    // without this wait a fill store of another wave can land after (and
+    // erase) a node's step-0 entry
+    __syncthreads();
 
     // lanes per node: lane `sub` of node n gathers row edges sub, sub+lpn, ...
     const int lpn = V.lpn, sub = tid & (lpn - 1);

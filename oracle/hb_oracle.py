@@ -230,7 +230,14 @@ def run_many_f32(insts, n_threads: int = 0) -> float:
         outs.append((t, f))
         P.out_tavg = t.ctypes.data
         P.out_final = f.ctypes.data
-        P.bold_period = 0
+        if P.bold_period > 0:   # the BOLD monitor runs as it does on the device
+            cap = I.n_steps // I.bold_period + 1
+            b = np.zeros((cap, N), np.float32)
+            nb = np.zeros(1, np.int32)
+            keep += [b, nb]
+            P.out_bold, P.bold_cap, P.out_bold_n = b.ctypes.data, cap, nb.ctypes.data_as(ip)
+        else:
+            P.bold_period = 0
         arr[i] = P
     t0 = time.perf_counter()
     rc = lib().hb_orc_run_many_f32(arr, len(insts), n_threads)

@@ -380,3 +380,42 @@ def test_autotuned_plan_matches_default(gpu, orc):
         assert e["choice"] in ((1, 2, 4) if e["blocked"] else (256, 512)), e
     for I, x, y in zip(insts, a, b):
         np.testing.assert_allclose(y.final_state, x.final_state, rtol=1e-4, atol=1e-5)
+
+
+# HyperbolicTangent and PreSigmoidal pre() run through tanhf: exact mode within
+# 1e-5 of the FP32 oracle (device vs host libm), FMA mode within the FP64 bound
+TANH_CASES = {
+    "tanh_g2do_csr": dict(model=L.G2DO, cfn=L.CF_TANH, cparams=(0.02, 1.5, 0.1, 2.0)),
+    "tanh_kuramoto_csr_rk4": dict(model=L.KURAMOTO, cfn=L.CF_TANH, cparams=(0.05, 0.5, 1.0, 1.0), integ=L.RK4),
+    "presig_mpr_csr": dict(model=L.MPR, cfn=L.CF_PRESIGMOIDAL, cparams=(0.05, 0.5, 2.0, 0.8, 0.1)),
+    "presig_g2do_csr_heun_stoch": dict(model=L.G2DO, cfn=L.CF_PRESIGMOIDAL, cparams=(0.03,), integ=L.HEUN_STOCH),
+    "presig_mpr_cv_csr": dict(model=L.MPR, cfn=L.CF_PRESIGMOIDAL, cparams=(0.05, 0.5, 2.0, 0.8, 0.1),
+                              params=np.array([1, 1, -5, 15, 0, 1, 0.3], np.float32)),
+}
+
+
+@pytest.mark.parametrize("name", list(TANH_CASES))
+def test_tanh_and_presigmoidal_coupling(gpu, orc, name):
+    kw = dict(TANH_CASES[name])
+    model, integ = kw.pop("model"), kw.pop("integ", L.HEUN)
+    I = net(model, integ=integ, T=250, seed=900 + len(name), **kw)
+    (g,), _ = run_instances([I], exact=True)
+    o32, o64 = orc.run(I), orc.run(I, dtype=np.float64)
+    np.testing.assert_allclose(g.traj, o32["traj"], rtol=1e-5, atol=1e-5 if model == L.KURAMOTO else 1e-6)
+    (gf,), _ = run_instances([I], exact=False)
+    _err_bound(gf.traj, o32["traj"], o64["traj"], atol=1e-5 if model == L.KURAMOTO else 1e-6)
+
+
+@pytest.mark.parametrize("wkind", [L.W_DENSE, L.W_SCALAR_MEAN, L.W_SCALAR_DIAG])
+@pytest.mark.parametrize("cfn,cp", [(L.CF_TANH, (0.02, 1.5, 0.1, 2.0)), (L.CF_PRESIGMOIDAL, (0.05, 0.5, 2.0, 0.8, 0.1))])
+def test_tanh_and_presigmoidal_weight_kinds(gpu, orc, wkind, cfn, cp):
+    """Dense, mean-field and diagonal weights with one delay per projection,
+    including the pre(0) fill of not-yet-simulated time (Q11)."""
+    rng = np.random.default_rng(31 + wkind)
+    N = 10
+    W = lognormal_connectome(rng, N, 0.5)
+    I = Instance(model=L.MPR, n_nodes=N, n_steps=200, dt=0.05, params=DEFAULTS[L.MPR].copy(),
+                 x0=_x0(L.MPR, N, rng), integrator=L.HEUN, wkind=wkind, dense=W if wkind == L.W_DENSE else None,
+                 scalar_w=0.3, cfn=cfn, cparams=cp, delay_mode=L.DELAY_SINGLE, single_delay=7, record_traj=True)
+    (g,), _ = run_instances([I], exact=True)
+    np.testing.assert_allclose(g.traj, orc.run(I)["traj"], rtol=1e-5, atol=1e-6)
