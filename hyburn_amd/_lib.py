@@ -23,6 +23,7 @@ DELAY_PER_EDGE, DELAY_SINGLE, DELAY_BATCH_COMPAT = 0, 1, 2
 ENGINE_BATCH, ENGINE_PLAIN = 0, 1
 FLAG_TRAJECTORY, FLAG_EXACT, FLAG_NO_GRAPH, FLAG_NO_BLOCKING, FLAG_AUTOTUNE = 0x1, 0x2, 0x4, 0x8, 0x10
 FLAG_SEPARATE_BOLD = 0x20
+FLAG_KEEP_NOISE, FLAG_RESTART, FLAG_BOLD_BATCH_MEAN = 0x40, 0x80, 0x100
 LOSS_FC, LOSS_BOLD_MSE, LOSS_ORDER = 0, 1, 2
 
 MODEL_NAMES = {
@@ -68,7 +69,7 @@ class HbInstance(C.Structure):
         ("bold_period", C.c_int32), ("bold_tr_ms", C.c_double),
         ("tavg", fp), ("final_state", fp), ("traj", fp), ("bold", fp),
         ("bold_cap", C.c_int32), ("bold_n", ip),
-    ]
+        ("step0", C.c_int64), ("history", fp), ("bold_state", fp)]
 
 
 class HbRunOpts(C.Structure):
@@ -112,7 +113,8 @@ class HbResult(C.Structure):
     _fields_ = [("n_sweep", C.c_int32), ("n_nodes", C.c_int32), ("nvar", C.c_int32),
                 ("n_steps", C.c_int32), ("tavg", fp), ("final_states", fp),
                 ("trajectories", fp), ("bold", fp), ("n_bold", C.c_int32),
-                ("elapsed_ms", C.c_double), ("kernel_ms", C.c_double)]
+                ("elapsed_ms", C.c_double), ("kernel_ms", C.c_double), ("bold_series", C.c_int32),
+                ("step", C.c_int64)]
 
 
 class HbAdjointSpec(C.Structure):
@@ -168,6 +170,7 @@ PROTOTYPES = {
     "hb_plan_destroy": (None, [C.c_void_p]),
     "hb_run_instances": (C.c_int, [C.POINTER(HbInstance), C.c_int32, C.POINTER(HbRunOpts), C.POINTER(HbRunStats)]),
     "hb_philox_normal": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, fp, C.c_int32]),
+    "hb_history_floats": (C.c_int64, [C.POINTER(HbInstance)]),
     "hb_last_error": (C.c_char_p, []),
     "hb_version": (C.c_char_p, []),
     "hb_device_count": (C.c_int, []),

@@ -40,6 +40,23 @@ struct hb_engine {
     int delay_mode = -1;
     uint64_t noise_seed = 0;
     int noise_shared = 1;
+    // Carried across runs, as the reference engine mutates self.states,
+    // self.step, self.histories and self.bold_monitors (engine.rs:496-540,
+    // 1328-1331). n_state == 0: the configured initial state at step 0.
+    int64_t step = 0;
+    int n_state = 0;
+    std::vector<float> state;        // [n_state][N][V]
+    std::vector<std::vector<float>> hist;   // per point [max_delay][N][channels]: pre() of the last steps
+    // BOLD monitors: one per sweep point, or one fed the batch mean
+    // (engine.rs:785-801). Each keeps its Balloon-Windkessel state and every
+    // native sample so far; a run returns them resampled to TR over the whole
+    // series, as BoldMonitor::flush does (bold_monitor.rs:123-227).
+    int n_mon = 0;
+    std::vector<float> bw;                    // [n_mon][N][4]: s, f, v, q
+    std::vector<std::vector<float>> native;   // per monitor: [rows][N]
+    std::vector<double> times;                // native sample times (ms), shared
+    bool monitors_dropped = false;            // run_sweep_multi leaves the engine without monitors
+    std::vector<float> x0_multi;              // start state of the current run_sweep_multi
 };
 
 namespace {
@@ -113,6 +130,54 @@ hb_instance base_instance(const hb_engine* E, int64_t n_steps) {
     return I;
 }
 
+void reset_state(hb_engine* E) {
+    E->monitors_dropped = false;
+    E->step = 0;
+    E->n_state = 0;
+    E->state.clear();
+    E->hist.clear();
+    E->n_mon = 0;
+    E->bw.clear();
+    E->native.clear();
+    E->times.clear();
+}
+
+// BoldMonitor::downsample_to_tr (bold_monitor.rs:123-203) over the whole
+// native series of one monitor: samples at j * TR, linear between the
+// bracketing native samples, nearest before the first and after the last.
+// Returns the rows written to out ([rows][N], or the count only when out is null).
+int downsample_to_tr(const std::vector<double>& times, const std::vector<float>& data, int N, double dtb,
+                     double tr_ms, float* out) {
+    const int n = (int)times.size();
+    if (n == 0) return 0;
+    if (tr_ms <= 0.0 || std::fabs(tr_ms - dtb) < 1e-6) {
+        if (out) memcpy(out, data.data(), sizeof(float) * (size_t)n * N);
+        return n;
+    }
+    const double max_t = times[n - 1];
+    const int n_out = max_t >= tr_ms ? (int)std::floor(max_t / tr_ms) : 0;
+    if (!out) return n_out;
+    for (int j = 0; j < n_out; ++j) {
+        const double tt = (double)(j + 1) * tr_ms;
+        float* o = out + (size_t)j * N;
+        if (tt <= times[0]) { memcpy(o, data.data(), sizeof(float) * N); continue; }
+        int idx = 0;
+        for (int k = 1; k < n; ++k) {
+            if (times[k] > tt) { idx = k - 1; break; }
+            idx = k;
+        }
+        if (idx + 1 >= n) { memcpy(o, data.data() + (size_t)idx * N, sizeof(float) * N); continue; }
+        const double t0 = times[idx], t1 = times[idx + 1];
+        double w = std::fabs(t1 - t0) < 1e-12 ? 0.0 : (tt - t0) / (t1 - t0);
+        w = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
+        const float wf = (float)w;
+        const float* v0 = data.data() + (size_t)idx * N;
+        const float* v1 = v0 + N;
+        for (int q = 0; q < N; ++q) o[q] = v0[q] + (v1[q] - v0[q]) * wf;
+    }
+    return n_out;
+}
+
 void fill_result(const hb_engine* E, int n_sweep, int64_t n_steps, bool traj, hb_result* out) {
     memset(out, 0, sizeof(*out));
     out->n_sweep = n_sweep;
@@ -125,40 +190,71 @@ void fill_result(const hb_engine* E, int n_sweep, int64_t n_steps, bool traj, hb
     if (traj) out->trajectories = (float*)calloc(st * (size_t)n_steps + 1, 4);
 }
 
-int run_built(hb_engine* E, Built& B, int64_t n_steps, uint32_t flags, bool with_bold, hb_result* out) {
+// Run the instances of one call and fill the batched result. carry: the
+// instances continue the engine's carried state (x0, step, history, BOLD
+// monitors), which is updated afterwards; monitors: BOLD is recorded.
+int run_built(hb_engine* E, Built& B, int64_t n_steps, uint32_t flags, bool monitors, hb_result* out) {
     const auto t0 = std::chrono::steady_clock::now();
     const int n = (int)B.inst.size();
     const bool traj = (flags & HB_FLAG_TRAJECTORY) != 0;
+    const bool mean_bold = (flags & HB_FLAG_BOLD_BATCH_MEAN) != 0;
+    const bool with_bold = monitors && E->bold_period > 0;
     fill_result(E, n, n_steps, traj, out);
     const size_t per = (size_t)E->n_nodes * E->nv;
+    const int N = E->n_nodes;
     std::vector<float> traj_tmp;
     if (traj) traj_tmp.resize(per * (size_t)n_steps * n + 1);
-    int bold_rows = 0;
-    std::vector<int32_t> bold_n(n, 0);
-    if (with_bold && E->bold_period > 0) {
-        // rows per instance, same count for every instance of a sweep
-        const double dtb = E->dt * (double)E->bold_period;
-        const double tr_ms = E->bold_tr_s * 1000.0;
-        const int P = E->bold_period;
-        const int n_full = (int)(n_steps / P), rem = (int)(n_steps % P);
-        const int n_native = n_full + (rem > 0);
-        if (tr_ms <= 0.0 || std::fabs(tr_ms - dtb) < 1e-6) bold_rows = n_native;
-        else if (n_native > 0) {
-            const double max_t = rem > 0 ? (double)n_full * dtb + E->dt * (double)rem : (double)(n_full - 1) * dtb + dtb;
-            bold_rows = max_t >= tr_ms ? (int)std::floor(max_t / tr_ms) : 0;
+
+    // carried delay history: one buffer per sweep point, [L][N][channels]
+    std::vector<std::vector<float>> hist(n);
+    for (int i = 0; i < n; ++i) {
+        hb_instance& I = B.inst[i];
+        const int64_t hf = hb_history_floats(&I);
+        if (hf <= 0) continue;
+        if (I.step0 > 0) {
+            if ((int)E->hist.size() != n || (int64_t)E->hist[i].size() != hf) {
+                hb_result_free(out);
+                return efail(HB_INVALID_STATE, "the carried delay history does not match this run");
+            }
+            hist[i] = E->hist[i];
+        } else {
+            hist[i].assign((size_t)hf, 0.0f);
         }
-        out->n_bold = bold_rows;
-        out->bold = (float*)calloc((size_t)n * bold_rows * E->n_nodes + 1, 4);
+        I.history = hist[i].data();
     }
+
+    // BOLD: native samples of this run per monitor, Balloon-Windkessel state carried
+    const int P = E->bold_period;
+    const int n_full = P > 0 ? (int)(n_steps / P) : 0, rem = P > 0 ? (int)(n_steps % P) : 0;
+    const int rows = n_full + (rem > 0 ? 1 : 0);
+    const int n_mon = with_bold ? (mean_bold ? 1 : n) : 0;
+    if (with_bold) {
+        if (E->n_mon == 0) {
+            E->n_mon = n_mon;
+            E->bw.assign((size_t)n_mon * N * 4, 0.0f);
+            for (size_t k = 0; k < (size_t)n_mon * N; ++k) E->bw[k * 4 + 1] = E->bw[k * 4 + 2] = E->bw[k * 4 + 3] = 1.0f;
+            E->native.assign(n_mon, std::vector<float>());
+            E->times.clear();
+        } else if (E->n_mon != n_mon) {
+            hb_result_free(out);
+            return efail(HB_INVALID_STATE, "the BOLD monitors of this engine were set up for " +
+                                               std::to_string(E->n_mon) + " series, this run asks for " +
+                                               std::to_string(n_mon) + " (restart the engine)");
+        }
+    }
+    std::vector<float> nat((size_t)std::max(n_mon, 1) * std::max(rows, 1) * N, 0.0f);
+    std::vector<int32_t> bold_n(n, 0);
     for (int i = 0; i < n; ++i) {
         hb_instance& I = B.inst[i];
         I.tavg = out->tavg + (size_t)i * per;
         I.final_state = out->final_states + (size_t)i * per;
         I.traj = traj ? traj_tmp.data() + (size_t)i * per * (size_t)n_steps : nullptr;
-        if (out->bold) {
-            I.bold = out->bold + (size_t)i * bold_rows * E->n_nodes;
-            I.bold_cap = bold_rows;
+        if (with_bold && !mean_bold) {
+            I.bold_tr_ms = 0.0;   // native samples; resampled over the whole series below
+            I.bold = nat.data() + (size_t)i * rows * N;
+            I.bold_cap = rows;
             I.bold_n = &bold_n[i];
+            I.bold_state = E->bw.data() + (size_t)i * N * 4;
         } else {
             I.bold_period = 0;
         }
@@ -166,14 +262,17 @@ int run_built(hb_engine* E, Built& B, int64_t n_steps, uint32_t flags, bool with
     hb_run_opts o;
     memset(&o, 0, sizeof(o));
     o.device = E->device;
-    o.flags = flags;
+    o.flags = flags & ~(HB_FLAG_BOLD_BATCH_MEAN | HB_FLAG_RESTART);
+    if (with_bold && mean_bold) o.flags |= kFlagDeviceTrajectory;
     hb_run_stats st;
     memset(&st, 0, sizeof(st));
-    hb_plan* P = nullptr;
-    int rc = hb_plan_create(B.inst.data(), n, &o, &P);
-    if (!rc) rc = hb_plan_run(P, &st);
-    if (!rc) rc = hb_plan_download(P);
-    hb_plan_destroy(P);
+    hb_plan* Pn = nullptr;
+    int rc = hb_plan_create(B.inst.data(), n, &o, &Pn);
+    if (!rc) rc = hb_plan_run(Pn, &st);
+    if (!rc && with_bold && mean_bold && rows > 0)
+        rc = plan_bold_batch_mean(Pn, P, E->dt, E->bw.data(), nat.data(), o.flags & HB_FLAG_EXACT);
+    if (!rc) rc = hb_plan_download(Pn);
+    hb_plan_destroy(Pn);
     if (rc) {
         hb_result_free(out);
         return rc;
@@ -183,6 +282,32 @@ int run_built(hb_engine* E, Built& B, int64_t n_steps, uint32_t flags, bool with
             for (int64_t t = 0; t < n_steps; ++t)
                 memcpy(out->trajectories + ((size_t)t * n + i) * per, traj_tmp.data() + ((size_t)i * n_steps + t) * per, per * 4);
     }
+    // the engine continues from here on the next run
+    E->step += n_steps;
+    E->n_state = n;
+    E->state.assign(out->final_states, out->final_states + per * n);
+    E->hist = std::move(hist);
+    if (with_bold) {
+        // sample times as BoldMonitor forms them: full windows at
+        // len * dtb + dtb, the flushed partial window at len * dtb + dt * rem
+        // (bold_monitor.rs:100, 214)
+        const double dtb = E->dt * (double)P;
+        for (int k = 0; k < rows; ++k) {
+            const double len = (double)E->times.size();
+            E->times.push_back(k < n_full ? len * dtb + dtb : len * dtb + E->dt * (double)rem);
+        }
+        for (int m = 0; m < n_mon; ++m)
+            E->native[m].insert(E->native[m].end(), nat.begin() + (size_t)m * rows * N,
+                                nat.begin() + (size_t)(m + 1) * rows * N);
+        const double tr_ms = E->bold_tr_s * 1000.0;   // bold_monitor.rs:131 (tr * 1000)
+        const int n_out = downsample_to_tr(E->times, E->native[0], N, dtb, tr_ms, nullptr);
+        out->n_bold = n_out;
+        out->bold_series = n_mon;
+        out->bold = (float*)calloc((size_t)n_mon * n_out * N + 1, 4);
+        for (int m = 0; m < n_mon; ++m)
+            downsample_to_tr(E->times, E->native[m], N, dtb, tr_ms, out->bold + (size_t)m * n_out * N);
+    }
+    out->step = E->step;
     out->kernel_ms = st.kernel_ms;
     out->elapsed_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return HB_OK;
@@ -284,14 +409,25 @@ int hb_run_sweep(hb_engine* E, int32_t sub_idx, int32_t param_idx, const float* 
     B.inst.reserve(n_values);
     // Only the model's batch-native index is applied; any other index is
     // ignored, as in dfun_batch (dfun.rs:49-72).
+    if (flags & HB_FLAG_RESTART) reset_state(E);
+    if (E->n_state > 0 && E->n_state != n_values)
+        return efail(HB_INVALID_STATE, "the engine holds " + std::to_string(E->n_state) +
+                                           " sweep points (left by run_sweep_multi); this run has " +
+                                           std::to_string(n_values));
     const bool native = param_idx == model_sweep_index(E->model);
+    const size_t per = (size_t)E->n_nodes * E->nv;
     for (int i = 0; i < n_values; ++i) {
         hb_instance I = base_instance(E, n_steps);
         if (native) { I.sweep_form = 1; I.sweep_val = values[i]; }
         I.noise_stream = E->noise_shared ? 0u : (uint64_t)i;
+        // continue where the previous run stopped (engine.rs:496-540); fresh
+        // noise for the new steps (the reference draws it per step)
+        I.step0 = E->step;
+        I.noise_seed = E->noise_seed + (uint64_t)E->step;
+        if (E->n_state > 0) I.x0 = E->state.data() + (size_t)i * per;
         B.inst.push_back(I);
     }
-    return run_built(E, B, n_steps, flags, true, out);
+    return run_built(E, B, n_steps, flags, !E->monitors_dropped, out);
 }
 
 int hb_run_sweep_multi(hb_engine* E, const hb_sweep_point* pts, int32_t n_points, int64_t n_steps,
@@ -316,12 +452,23 @@ int hb_run_sweep_multi(hb_engine* E, const hb_sweep_point* pts, int32_t n_points
         total *= pts[d].n_values;
         if (total > (int64_t)1 << 24) return efail(HB_INVALID_CONFIG, "sweep too large");
     }
+    if (flags & HB_FLAG_RESTART) reset_state(E);
+    // run_sweep_multi builds a fresh engine whose points all start from sweep
+    // point 0 of the current state, at step 0 with empty histories and no
+    // monitors, then takes over its states, step and histories
+    // (engine.rs:886-909, 1326-1331)
+    std::vector<float> x0 = E->n_state > 0 ? std::vector<float>(E->state.begin(), E->state.begin() + E->n_nodes * E->nv)
+                                           : E->x0;
+    reset_state(E);
+    E->monitors_dropped = true;
+    E->x0_multi = std::move(x0);
     Built B;
     B.inst.reserve((size_t)total);
     B.params.resize((size_t)total);
     std::vector<int> idx(n_points, 0);
     for (int64_t s = 0; s < total; ++s) {
         hb_instance I = base_instance(E, n_steps);
+        I.x0 = E->x0_multi.data();
         std::vector<float>& p = B.params[(size_t)s];
         p = E->params;
         for (int d = 0; d < n_points; ++d) {
@@ -350,8 +497,7 @@ int hb_run_sweep_multi(hb_engine* E, const hb_sweep_point* pts, int32_t n_points
         }
     }
     for (int64_t s = 0; s < total; ++s) B.inst[(size_t)s].params = B.params[(size_t)s].data();
-    // run_sweep_multi rebuilds the engine without monitors (engine.rs:897-909)
-    return run_built(E, B, n_steps, flags, false, out);
+    return run_built(E, B, n_steps, flags & ~HB_FLAG_BOLD_BATCH_MEAN, false, out);
 }
 
 void hb_result_free(hb_result* r) {

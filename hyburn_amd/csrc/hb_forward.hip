@@ -72,7 +72,7 @@ __device__ __forceinline__ uint2 edge_at(gp<const uint32_t> e, int i) {
 struct View {
     int model, integrator, n_nodes, n_steps, np, wkind, cfn, delay_mode, single_delay, ring_depth;
     int sweep_form, node_param_idx, bold_period, bold_nout, bold_native, cpu_form, sjr_classic, engine;
-    int max_row, blocked_k, lpn, lpn_shift;
+    int max_row, blocked_k, lpn, lpn_shift, step0, step0_modh, hist_len, bold_carry;
     float dt, dt_half, dt_sixth, inv_nsteps, sweep_val, scalar_w, fill0, fill1;
     double dt_ms, tr_ms, dtb_ms;
     int64_t traj_tstride;
@@ -80,6 +80,8 @@ struct View {
     gp<const int32_t> indptr;
     gp<const uint32_t> edges;   // 2 words per edge: src | delay << 16, weight bits
     gp<float> ring, tavg, fin, traj, bold;
+    gp<const float> hist_in;
+    gp<float> hist_out, bold_io;
 };
 
 __device__ __forceinline__ View make_view(const DevInst* I) {
@@ -92,6 +94,8 @@ __device__ __forceinline__ View make_view(const DevInst* I) {
     V.cpu_form = u32(I->cpu_form); V.sjr_classic = u32(I->sjr_classic); V.engine = u32(I->engine);
     V.max_row = u32(I->max_row); V.blocked_k = u32(I->blocked_k);
     V.lpn = u32(I->lpn); V.lpn_shift = u32(I->lpn_shift);
+    V.step0 = u32(I->step0); V.step0_modh = u32(I->step0_modh);
+    V.hist_len = u32(I->hist_len); V.bold_carry = u32(I->bold_carry);
     V.dt = uf32(I->dt); V.dt_half = uf32(I->dt_half); V.dt_sixth = uf32(I->dt_sixth); V.inv_nsteps = uf32(I->inv_nsteps);
     V.sweep_val = uf32(I->sweep_val); V.scalar_w = uf32(I->scalar_w); V.fill0 = uf32(I->fill0); V.fill1 = uf32(I->fill1);
     V.dt_ms = uf64(I->dt_ms); V.tr_ms = uf64(I->tr_ms); V.dtb_ms = uf64(I->dtb_ms);
@@ -99,7 +103,14 @@ __device__ __forceinline__ View make_view(const DevInst* I) {
     V.x0 = ug(I->x0); V.node_mc = ug(I->node_mc); V.dense_t = ug(I->dense_t); V.noise = ug(I->noise);
     V.indptr = ug(I->indptr); V.edges = ug((const uint32_t*)I->edges);
     V.ring = ug(I->ring); V.tavg = ug(I->tavg); V.fin = ug(I->fin); V.traj = ug(I->traj); V.bold = ug(I->bold);
+    V.hist_in = ug(I->hist_in); V.hist_out = ug(I->hist_out); V.bold_io = ug(I->bold_io);
     return V;
+}
+
+// Global step of local step t of a continued run, saturated at 2^30 (it only
+// enters comparisons with delays <= 65535 and with 0).
+__device__ __forceinline__ int global_step(const View& V, int t) {
+    return t >= (1 << 30) - V.step0 ? (1 << 30) : t + V.step0;
 }
 
 // Sum over the lanes that share a node (1, 2 or 4 adjacent lanes of a quad).
@@ -197,6 +208,13 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
         const int total = Hd * NNP;
         for (int i = tid; i < total; i += BS) ring[i] = (NP == 2 && (i & 1)) ? fill[1] : fill[0];
     }
+    // a continued run: the carried pre() of times -L..-1 go to slots Hd-L..Hd-1
+    // (after the fill, which another thread may be writing to the same slots)
+    if (V.hist_in) {
+        __syncthreads();
+        const int L = V.hist_len, total = L * NNP;
+        for (int i = tid; i < total; i += BS) ring[(size_t)(Hd - L) * NNP + i] = V.hist_in[i];
+    }
 
     const int nper = MULTI ? (N + BS - 1) / BS : 1;
     // model constants (uniform, or per node when a parameter is given per node)
@@ -210,6 +228,10 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
     Bold bw;
     bw.s = 0.0f; bw.f = 1.0f; bw.v = 1.0f; bw.q = 1.0f; bw.acc = 0.0f; bw.prev = 0.0f;
     const bool own = tid < N;
+    if (!MULTI && V.bold_carry && own) {
+        bw.s = V.bold_io[tid * 4 + 0]; bw.f = V.bold_io[tid * 4 + 1];
+        bw.v = V.bold_io[tid * 4 + 2]; bw.q = V.bold_io[tid * 4 + 3];
+    }
     // ---- CSR row of this thread's node, folded into ring offsets and kept in
     // registers: offset = delay * N*np - src * np (read address = ws*N*np - offset)
     const bool csr_reg = !MULTI && wkind == HB_W_CSR;
@@ -250,6 +272,7 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
             for (int v = 0; v < NV; ++v) { tab_x[n * NV + v] = V.x0[n * NV + v]; tab_t[n * NV + v] = 0.0f; }
             float* b = tab_b + n * 6;
             b[0] = 0.0f; b[1] = 1.0f; b[2] = 1.0f; b[3] = 1.0f; b[4] = 0.0f; b[5] = 0.0f;
+            if (V.bold_carry) { b[0] = V.bold_io[n * 4]; b[1] = V.bold_io[n * 4 + 1]; b[2] = V.bold_io[n * 4 + 2]; b[3] = V.bold_io[n * 4 + 3]; }
         }
     }
     float mc_n[kMC];
@@ -264,7 +287,10 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
     __syncthreads();
 
     int ws = 0;
+    // batch-window position of the global step: (step0 + t) mod (max_delay + 1)
+    int tm = V.step0_modh;
     for (int t = 0; t < T; ++t) {
+        const int tg = global_step(V, t);
         // ---- phase 1: ring[ws][n] = pre(x_t[n])
         for (int kk = 0; kk < nper; ++kk) {
             const int n = MULTI ? tid + kk * BS : tid;
@@ -282,20 +308,22 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
 
         // uniform read slot for single-delay projections
         int slot_u = ws;
-        if (dmode != HB_DELAY_PER_EDGE && t > 0) {
+        if (dmode != HB_DELAY_PER_EDGE && tg > 0) {
             const int d = V.single_delay;
             int teff_back = 0;   // how many steps back
             if (d == 0) teff_back = 0;
-            else if (d > t) teff_back = -1;
+            else if (d > tg) teff_back = -1;
             else if (dmode == HB_DELAY_BATCH_COMPAT) {
                 const int H = Hd - 1;   // reference horizon = max_delay + 1
-                const int back = H - 1 - ((t - d + H) % H);
-                teff_back = back > t ? -1 : back;
+                int r = tm - d + H;     // (t - d + H) mod H, tm = t mod H
+                r -= r >= H ? H : 0;
+                const int back = H - 1 - r;
+                teff_back = back > tg ? -1 : back;
             } else teff_back = d;
             if (teff_back < 0) slot_u = -1;
             else { slot_u = ws - teff_back; if (slot_u < 0) slot_u += Hd; }
         }
-        const int dmask = t == 0 ? 0 : 0xFFFF;
+        const int dmask = tg == 0 ? 0 : 0xFFFF;
         // mean-field sum (sequential over sources, identical in every thread)
         float mf[NP];
         if (wkind == HB_W_SCALAR_MEAN) {
@@ -342,7 +370,7 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
                 if (wkind == HB_W_CSR) {
                     int e_mem0, e_mem1;
                     // register-resident rows (steady state): same summation order as the row
-                    const bool reg_ok = csr_reg && (per_edge ? t > 0 : slot_u >= 0);
+                    const bool reg_ok = csr_reg && (per_edge ? tg > 0 : slot_u >= 0);
                     if (reg_ok) {
                         const int base = (per_edge ? ws : slot_u) * NNP;
                         const int wrap = Hd * NNP;
@@ -502,6 +530,19 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
             }
         }
         ws = (ws + 1 == Hd) ? 0 : ws + 1;
+        tm = (tm + 1 == Hd - 1) ? 0 : tm + 1;
+    }
+
+    // ---- the carried history: pre() of times T-L..T-1, oldest first
+    if (V.hist_out) {
+        __syncthreads();
+        const int L = V.hist_len, total = L * NNP;
+        for (int i = tid; i < total; i += BS) {
+            const int j = i / NNP;
+            int sl = ws - L + j;
+            sl += sl < 0 ? Hd : 0;
+            V.hist_out[i] = ring[(size_t)sl * NNP + (i - j * NNP)];
+        }
     }
 
     // ---- BOLD flush of a partial window, then nearest-sample tail rows
@@ -535,6 +576,20 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
                     const float last = MULTI ? tab_b[n * 6 + 5] : bw.prev;
                     for (int jj = bc.j; jj < V.bold_nout; ++jj) V.bold[(size_t)jj * N + n] = last;
                 }
+            }
+        }
+    }
+
+    // ---- the monitor's Balloon-Windkessel state for a continued run
+    if (bper > 0 && V.bold_io) {
+        for (int kk = 0; kk < nper; ++kk) {
+            const int n = MULTI ? tid + kk * BS : tid;
+            if (n < N) {
+                float q4[4];
+                if (!MULTI) { q4[0] = bw.s; q4[1] = bw.f; q4[2] = bw.v; q4[3] = bw.q; }
+                else { const float* q = tab_b + n * 6; q4[0] = q[0]; q4[1] = q[1]; q4[2] = q[2]; q4[3] = q[3]; }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) V.bold_io[n * 4 + k] = q4[k];
             }
         }
     }
@@ -602,6 +657,15 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     // without this wait a fill store of another wave can land after (and
     // erase) a node's step-0 entry
     __syncthreads();
+    // a continued run: the carried pre() of times -L..-1 go to slots
+    // Hd-L..Hd-1 (>= K + 1, so no mirrored copy)
+    if (V.hist_in) {
+        const int L = V.hist_len, total = L * N * NP;
+        for (int i = tid; i < total; i += BS) {
+            const int j = i / (N * NP), r = i - j * (N * NP), nn = r / NP, c = r - nn * NP;
+            ring[c * NHd + nn * Hs + (Hd - L + j)] = V.hist_in[i];
+        }
+    }
 
     // lanes per node: lane `sub` of node n gathers row edges sub, sub+lpn, ...
     const int lpn = V.lpn, sub = tid & (lpn - 1);
@@ -619,6 +683,10 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     for (int v = 0; v < NV; ++v) { x[v] = own ? V.x0[n * NV + v] : 0.0f; tv[v] = 0.0f; }
     Bold bw;
     bw.s = 0.0f; bw.f = 1.0f; bw.v = 1.0f; bw.q = 1.0f; bw.acc = 0.0f; bw.prev = 0.0f;
+    if (V.bold_carry && own) {
+        bw.s = V.bold_io[n * 4 + 0]; bw.f = V.bold_io[n * 4 + 1];
+        bw.v = V.bold_io[n * 4 + 2]; bw.q = V.bold_io[n * 4 + 3];
+    }
     BoldClock bc;
     bc.k = 0; bc.j = 0; bc.tprev = 0.0;
     int bcount = 0;
@@ -687,10 +755,12 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         }
     };
 
-    // ---- step 0: every edge reads the current state (runtime.rs:162-164)
+    // ---- step 0 of a fresh run: every edge reads the current state
+    // (runtime.rs:162-164); a continued run has no such step
+    const bool fresh = V.step0 == 0;
     write_pre(0);
     __syncthreads();
-    if (T > 0) {
+    if (fresh && T > 0) {
         float g[NP];
 #pragma unroll
         for (int c = 0; c < NP; ++c) g[c] = 0.0f;
@@ -710,9 +780,10 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         write_pre(1 % Hd);
     }
 
-    // ---- blocks of K steps starting at t0 = 1
-    int ws0 = 1 % Hd;   // ring slot of time t0
-    for (int t0 = 1; t0 < T; t0 += K) {
+    // ---- blocks of K steps starting at t0 = 1 (a continued run: t0 = 0)
+    const int tb = fresh ? 1 : 0;
+    int ws0 = tb % Hd;   // ring slot of time t0
+    for (int t0 = tb; t0 < T; t0 += K) {
         __syncthreads();
         const int kb = min(K, T - t0);
         // the block's noise, issued before the gather so its latency overlaps it
@@ -827,6 +898,19 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         ws0 -= ws0 >= Hd ? Hd : 0;
     }
 
+    // ---- the carried history: pre() of times T-L..T-1, oldest first
+    if (V.hist_out) {
+        __syncthreads();
+        const int L = V.hist_len, total = L * N * NP;
+        const int wsT = T % Hd;
+        for (int i = tid; i < total; i += BS) {
+            const int j = i / (N * NP), r = i - j * (N * NP), nn = r / NP, c = r - nn * NP;
+            int sl = wsT - L + j;
+            sl += sl < 0 ? Hd : 0;
+            V.hist_out[i] = ring[c * NHd + nn * Hs + sl];
+        }
+    }
+
     // ---- BOLD flush and TR tail rows
     if (bper > 0) {
         if (bcount > 0) {
@@ -840,6 +924,10 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         }
         if (!V.bold_native && bc.k > 0 && lead)
             for (int jj = bc.j; jj < V.bold_nout; ++jj) V.bold[(size_t)jj * N + n] = bw.prev;
+        if (V.bold_io && lead) {
+            V.bold_io[n * 4 + 0] = bw.s; V.bold_io[n * 4 + 1] = bw.f;
+            V.bold_io[n * 4 + 2] = bw.v; V.bold_io[n * 4 + 3] = bw.q;
+        }
     }
     if (lead) {
         const float inv = V.inv_nsteps;
@@ -960,6 +1048,61 @@ static const void* unit_kernel(bool lring, bool multi, int np) {
 #endif
 }
 
+#if HB_UNIT == 0
+// ------------------------------------------- batch-averaged BOLD monitor -
+//
+// The batch engine's BOLD monitor (engine.rs:785-801) is fed, every step,
+// var0 averaged over the modes and over the sweep points, and it runs one
+// Balloon-Windkessel model per node (bold_monitor.rs:72-107, 207-227). From
+// the plan's device trajectories: the mean over the instances per (step,
+// node), summed in instance order and divided by their count (mean_dim), then
+// one thread per node accumulating windows of `period` steps.
+__global__ void k_bold_mean_input(const float* const* __restrict__ trajs, int B, int T, int N, int nv,
+                                  float* __restrict__ zm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)T * N) return;
+    float s = 0.0f;
+    for (int b = 0; b < B; ++b) s += trajs[b][i * nv];
+    zm[i] = s / (float)B;
+}
+
+__global__ void k_bold_mean_monitor(const float* __restrict__ zm, int T, int N, int period, double dt_ms,
+                                    float* __restrict__ bw, float* __restrict__ native) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    Bold b;
+    b.s = bw[n * 4 + 0]; b.f = bw[n * 4 + 1]; b.v = bw[n * 4 + 2]; b.q = bw[n * 4 + 3];
+    b.acc = 0.0f; b.prev = 0.0f;
+    int cnt = 0, k = 0;
+    for (int t = 0; t < T; ++t) {
+        b.acc = b.acc + zm[(size_t)t * N + n];
+        if (++cnt >= period) {
+            const float dts = (float)(dt_ms * (double)cnt / 1000.0);
+            native[(size_t)k * N + n] = bold_step(b, b.acc / (float)cnt, dts);
+            ++k;
+            b.acc = 0.0f;
+            cnt = 0;
+        }
+    }
+    if (cnt > 0) {   // BoldMonitor::flush of the partial window
+        const float dts = (float)(dt_ms * (double)cnt / 1000.0);
+        native[(size_t)k * N + n] = bold_step(b, b.acc / (float)cnt, dts);
+    }
+    bw[n * 4 + 0] = b.s; bw[n * 4 + 1] = b.f; bw[n * 4 + 2] = b.v; bw[n * 4 + 3] = b.q;
+}
+
+static hipError_t bold_mean(const float* const* trajs, int n_inst, int T, int N, int nv, float* zm, int period,
+                            double dt_ms, float* bw, float* native, hipStream_t s) {
+    const int64_t tn = (int64_t)T * N;
+    if (tn > 0)
+        hipLaunchKernelGGL(k_bold_mean_input, dim3((unsigned)((tn + 255) / 256)), dim3(256), 0, s, trajs, n_inst, T,
+                           N, nv, zm);
+    hipLaunchKernelGGL(k_bold_mean_monitor, dim3((unsigned)((N + 63) / 64)), dim3(64), 0, s, zm, T, N, period, dt_ms,
+                       bw, native);
+    return hipGetLastError();
+}
+#endif
+
 // op 0: launch; 1: occupancy (blocks per CU into *out); 2: set the maximum
 // dynamic LDS (arg bytes) on every kernel of the unit.
 static hipError_t unit_op(int op, const LaunchDesc* d, hipStream_t s, int arg, int* out) {
@@ -986,6 +1129,19 @@ static hipError_t unit_op(int op, const LaunchDesc* d, hipStream_t s, int arg, i
 
 #define HB_CAT3(a, b, c) a##b##c
 #define HB_XCAT3(a, b, c) HB_CAT3(a, b, c)
+#if HB_UNIT == 0
+#if HB_EXACT
+hipError_t launch_bold_mean_exact(const float* const* trajs, int n_inst, int T, int N, int nv, float* zm, int period,
+                                  double dt_ms, float* bw, float* native, hipStream_t s) {
+    return exact::bold_mean(trajs, n_inst, T, N, nv, zm, period, dt_ms, bw, native, s);
+}
+#else
+hipError_t launch_bold_mean_fast(const float* const* trajs, int n_inst, int T, int N, int nv, float* zm, int period,
+                                 double dt_ms, float* bw, float* native, hipStream_t s) {
+    return fast::bold_mean(trajs, n_inst, T, N, nv, zm, period, dt_ms, bw, native, s);
+}
+#endif
+#endif
 #if HB_EXACT
 hipError_t HB_XCAT3(fwd_unit_exact_, HB_UNIT, )(int op, const LaunchDesc* d, hipStream_t s, int arg, int* out) {
     return exact::unit_op(op, d, s, arg, out);

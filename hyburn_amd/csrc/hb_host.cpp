@@ -121,6 +121,8 @@ struct hb_plan {
     std::vector<Out> outs;
     double node_steps = 0.0;
     int block_main = 0, lds_main = 0;
+    bool noise_done = false;   // HB_FLAG_KEEP_NOISE: the noise was generated by an earlier run
+    std::vector<size_t> hist_off, bold_off;   // carried history / BOLD state regions (or SIZE_MAX)
 };
 
 namespace {
@@ -188,6 +190,7 @@ int validate_instance(const hb_instance& I, int idx) {
     if (I.node_param_idx >= 0 && !I.node_param) return bad("node parameter values missing");
     if (is_stoch(I.integrator) && !I.nsig) return bad("stochastic integrator needs nsig");
     if (I.bold_period < 0) return bad("bold_period must be >= 0");
+    if (I.step0 < 0) return bad("step0 must be >= 0");
     return HB_OK;
 }
 
@@ -209,6 +212,11 @@ int max_delay_of(const hb_instance& I) {
         return m;
     }
     return I.single_delay;
+}
+
+// Floats of the carried delay history: max_delay steps x nodes x ring channels.
+int64_t history_floats(const hb_instance& I, int np) {
+    return (int64_t)max_delay_of(I) * I.n_nodes * np;
 }
 
 TuneKey tune_key(const hb_instance& I, int np, int blocked, int lring) {
@@ -563,8 +571,8 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         uploads.push_back(std::move(u));
     };
     struct Ptrs {
-        size_t x0, node_mc, indptr, edges, dense_t, noise, ring, tavg, fin, traj, bold;
-        bool has_node_mc, has_noise, has_ring, has_traj, has_bold, has_csr, has_dense;
+        size_t x0, node_mc, indptr, edges, dense_t, noise, ring, tavg, fin, traj, bold, hist, bold_io;
+        bool has_node_mc, has_noise, has_ring, has_traj, has_bold, has_csr, has_dense, has_hist, has_bold_io;
         int64_t noise_n;
         uint64_t seed, stream;
         bool gen_noise;
@@ -664,12 +672,26 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         q.fin = A.take(N * NV * 4);
         P->outs[i].tavg = q.tavg;
         P->outs[i].fin = q.fin;
-        if (I.traj) {
+        if (I.traj || (P->flags & kFlagDeviceTrajectory)) {
             q.has_traj = true;
             P->outs[i].n_traj = (size_t)I.n_steps * N * NV;
-            q.traj = A.take(P->outs[i].n_traj * 4);
+            q.traj = A.take(std::max(P->outs[i].n_traj, (size_t)1) * 4);
             P->outs[i].traj = q.traj;
         }
+        // a continued run's delay history and BOLD monitor state (in and out)
+        const int64_t hf = history_floats(I, s.np);
+        if (I.history && hf > 0) {
+            q.has_hist = true;
+            q.hist = A.take((size_t)hf * 4);
+            if (I.step0 > 0) stage(q.hist, I.history, (size_t)hf * 4);
+        }
+        if (I.bold_state && I.bold_period > 0 && I.bold) {
+            q.has_bold_io = true;
+            q.bold_io = A.take(N * 4 * 4);
+            stage(q.bold_io, I.bold_state, N * 4 * 4);
+        }
+        P->hist_off.push_back(q.has_hist ? q.hist : SIZE_MAX);
+        P->bold_off.push_back(q.has_bold_io ? q.bold_io : SIZE_MAX);
         if (I.bold && I.bold_period > 0) {
             BoldShape b = bold_shape(I.n_steps, I.bold_period, I.dt_ms, I.bold_tr_ms);
             q.has_bold = true;
@@ -767,6 +789,17 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         D.fin = (float*)(base + q.fin);
         D.traj = q.has_traj ? (float*)(base + q.traj) : nullptr;
         D.traj_tstride = (int64_t)I.n_nodes * s.nv;
+        D.step0 = (int32_t)std::min<int64_t>(I.step0, (int64_t)1 << 30);
+        D.step0_modh = (int32_t)(I.step0 % ((int64_t)max_delay_of(I) + 1));
+        if (q.has_hist) {
+            D.hist_len = max_delay_of(I);
+            D.hist_in = I.step0 > 0 ? (const float*)(base + q.hist) : nullptr;
+            D.hist_out = (float*)(base + q.hist);
+        }
+        if (q.has_bold_io) {
+            D.bold_io = (float*)(base + q.bold_io);
+            D.bold_carry = 1;
+        }
     }
     P->d_inst = (DevInst*)(base + off_inst);
     P->d_order = (int32_t*)(base + off_order);
@@ -884,8 +917,11 @@ int hb_plan_run(hb_plan* P, hb_run_stats* stats) {
     HB_HIP(hipSetDevice(P->device));
     const bool exact = (P->flags & HB_FLAG_EXACT) != 0;
     HB_HIP(hipEventRecord(P->ev[0], P->stream));
-    if (P->g_noise) HB_HIP(hipGraphLaunch(P->g_noise, P->stream));
-    else HB_HIP(launch_noise(P->d_jobs, P->n_jobs, P->max_noise, P->stream));
+    if (!(P->flags & HB_FLAG_KEEP_NOISE) || !P->noise_done) {
+        if (P->g_noise) HB_HIP(hipGraphLaunch(P->g_noise, P->stream));
+        else HB_HIP(launch_noise(P->d_jobs, P->n_jobs, P->max_noise, P->stream));
+        P->noise_done = true;
+    }
     HB_HIP(hipEventRecord(P->ev[1], P->stream));
     if (P->g_fwd) HB_HIP(hipGraphLaunch(P->g_fwd, P->stream));
     else {
@@ -919,6 +955,11 @@ int hb_plan_download(hb_plan* P) {
         if (I.tavg) HB_HIP(hipMemcpy(I.tavg, P->arena + o.tavg, o.n_state * 4, hipMemcpyDeviceToHost));
         if (I.final_state) HB_HIP(hipMemcpy(I.final_state, P->arena + o.fin, o.n_state * 4, hipMemcpyDeviceToHost));
         if (I.traj && o.n_traj) HB_HIP(hipMemcpy(I.traj, P->arena + o.traj, o.n_traj * 4, hipMemcpyDeviceToHost));
+        if (P->hist_off[i] != SIZE_MAX)
+            HB_HIP(hipMemcpy(I.history, P->arena + P->hist_off[i],
+                             (size_t)history_floats(I, P->dev[i].np) * 4, hipMemcpyDeviceToHost));
+        if (P->bold_off[i] != SIZE_MAX)
+            HB_HIP(hipMemcpy(I.bold_state, P->arena + P->bold_off[i], (size_t)I.n_nodes * 16, hipMemcpyDeviceToHost));
         if (I.bold && I.bold_period > 0) {
             const int N = I.n_nodes;
             const size_t rows = N ? o.n_bold / N : 0;
@@ -960,4 +1001,46 @@ int hb_philox_normal(uint64_t seed, uint64_t stream, int64_t n, float* out_host,
     return HB_OK;
 }
 
+int64_t hb_history_floats(const hb_instance* inst) {
+    if (!inst || !inst->params || inst->model < 0 || inst->model > HB_RWW || inst->n_nodes < 1) return 0;
+    const hb_instance& I = *inst;
+    return history_floats(I, ring_channels(I.model, I.cfn, I.params, I.node_param_idx, -1));
+}
+
 }  // extern "C"
+
+namespace hb {
+
+int plan_bold_batch_mean(hb_plan* P, int period, double dt_ms, float* bw, float* native, bool exact) {
+    if (!P || P->inst.empty() || period < 1) return fail(HB_INVALID_STATE, "batch-mean BOLD: bad plan or period");
+    const int B = (int)P->inst.size();
+    const int T = P->inst[0].n_steps, N = P->inst[0].n_nodes, nv = model_nvar(P->inst[0].model);
+    std::vector<const float*> trajs(B);
+    for (int i = 0; i < B; ++i) {
+        const hb_instance& I = P->inst[i];
+        if (I.n_steps != T || I.n_nodes != N || !P->outs[i].n_traj)
+            return fail(HB_INVALID_STATE, "batch-mean BOLD needs one shape and device trajectories");
+        trajs[i] = (const float*)(P->arena + P->outs[i].traj);
+    }
+    const int rows = T / period + (T % period ? 1 : 0);
+    HB_HIP(hipSetDevice(P->device));
+    DevBufs D;
+    const float** d_trajs = nullptr;
+    float *zm = nullptr, *d_bw = nullptr, *d_nat = nullptr;
+    hipError_t e = D.alloc(&d_trajs, (size_t)B);
+    if (e == hipSuccess) e = D.alloc(&zm, (size_t)T * N);
+    if (e == hipSuccess) e = D.alloc(&d_bw, (size_t)N * 4);
+    if (e == hipSuccess) e = D.alloc(&d_nat, (size_t)std::max(rows, 1) * N);
+    if (e == hipSuccess) e = hipMemcpy(d_trajs, trajs.data(), sizeof(float*) * B, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_bw, bw, sizeof(float) * N * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = exact ? launch_bold_mean_exact(d_trajs, B, T, N, nv, zm, period, dt_ms, d_bw, d_nat, P->stream)
+                  : launch_bold_mean_fast(d_trajs, B, T, N, nv, zm, period, dt_ms, d_bw, d_nat, P->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(P->stream);
+    if (e == hipSuccess && rows > 0) e = hipMemcpy(native, d_nat, sizeof(float) * rows * N, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(bw, d_bw, sizeof(float) * N * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(HB_BACKEND_ERROR, std::string("batch-mean BOLD: ") + hipGetErrorString(e));
+    return HB_OK;
+}
+
+}  // namespace hb

@@ -201,7 +201,10 @@ struct alignas(16) DevInst {
     int32_t single_delay, ring_depth, sweep_form, node_param_idx;
     int32_t bold_period, bold_nout, bold_native, cpu_form;
     int32_t sjr_classic, engine, max_row, blocked_k;   // longest CSR row; delay-block length (0: per step)
-    int32_t lpn, lpn_shift, pad2, pad3;                // lanes per node in the blocked path (1, 2, 4)
+    int32_t lpn, lpn_shift, step0, step0_modh;         // lanes per node in the blocked path (1, 2, 4);
+                                                       // steps before this run (saturated at 2^30) and
+                                                       // step0 % (max_delay + 1) for the batch window
+    int32_t hist_len, bold_carry, pad4, pad5;          // carried history steps; BOLD state read from bold_io
     float dt, dt_half, dt_sixth, inv_nsteps;
     float sweep_val, scalar_w, fill0, fill1;
     float mc[kMC];
@@ -220,6 +223,9 @@ struct alignas(16) DevInst {
     float* traj;
     int64_t traj_tstride;      // floats between consecutive steps of the trajectory
     float* bold;               // [bold rows][N]
+    const float* hist_in;      // [hist_len][N][np] pre() of the steps before step0, oldest first, or null
+    float* hist_out;           // same layout for the steps before the run's end, or null
+    float* bold_io;            // [N][4] Balloon-Windkessel s, f, v, q in and out, or null
 };
 
 // Launch description handed from the planner to a kernel translation unit.
@@ -283,6 +289,17 @@ struct DevBufs {
 // Record the thread-local last-error message (hb_last_error) and return code.
 int set_error(int code, const std::string& msg);
 
+// Internal run flag (never set by callers): every instance's trajectory is
+// kept on the device (not downloaded) for a follow-up kernel of the plan.
+constexpr uint32_t kFlagDeviceTrajectory = 0x40000000u;
+
+// The reference's batch-averaged BOLD monitor (engine.rs:785-801) over a run
+// plan whose trajectories are on the device (kFlagDeviceTrajectory): per
+// step the mean over the plan's instances of var0, accumulated into windows
+// of `period` steps, one Balloon-Windkessel step per window (bw [N][4] host
+// in/out), native samples written to native [rows][N] (host).
+int plan_bold_batch_mean(hb_plan* P, int period, double dt_ms, float* bw, float* native, bool exact);
+
 // Forward launchers (hb_forward.hip compiled twice: contracted and exact).
 hipError_t launch_forward_fast(const LaunchDesc& d, hipStream_t s);
 hipError_t launch_forward_exact(const LaunchDesc& d, hipStream_t s);
@@ -298,5 +315,12 @@ struct NoiseJob {
     uint64_t stream;
 };
 hipError_t launch_noise(const NoiseJob* jobs_dev, int n_jobs, int64_t max_n, hipStream_t s);
+// Batch-mean BOLD kernels (hb_forward.hip, unit 0): the mean over n_inst
+// trajectories (per-instance bases in trajs, [T][N][nv]) of var0 per step
+// into zm [T][N], then the windowed monitor per node.
+hipError_t launch_bold_mean_fast(const float* const* trajs, int n_inst, int T, int N, int nv, float* zm, int period,
+                                 double dt_ms, float* bw, float* native, hipStream_t s);
+hipError_t launch_bold_mean_exact(const float* const* trajs, int n_inst, int T, int N, int nv, float* zm, int period,
+                                  double dt_ms, float* bw, float* native, hipStream_t s);
 
 }  // namespace hb

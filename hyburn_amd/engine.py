@@ -272,7 +272,11 @@ class SweepSpec:
 
 @dataclass
 class BatchSweepResult:
-    """engine.rs:15-31 (tavg/final per subnetwork, flat [n_sweep, nnodes, nvar])."""
+    """engine.rs:15-31 (tavg/final per subnetwork, flat [n_sweep, nnodes, nvar]).
+    bold: one [series][rows][nnodes] array, the whole series since the engine
+    started resampled to TR; series = n_sweep (a monitor per sweep point) or
+    1 (bold_batch_mean: the reference's monitor of the sweep mean,
+    engine.rs:785-801). The multi-subnetwork path has per-point series only."""
     param_values: list
     tavg: list
     final_states: list
@@ -369,11 +373,24 @@ class BatchHybridEngine:
         self._step, self._state, self._hist = 0, None, None   # carried across run_sweep calls
         self._bold = None
         self.hybrid_integrator = False
+        # one BOLD monitor fed the sweep mean of var0 (the reference's batch
+        # monitor, engine.rs:785-801) instead of one per sweep point
+        self.bold_batch_mean = False
+        self.step = 0
 
     @classmethod
     def from_config(cls, base_config: SimConfig, n_sweep: int, device: int = 0, *,
                     engine: int = L.ENGINE_BATCH, delay_mode: int = -1, noise_seed: int = 0,
-                    noise_shared: bool = True) -> "BatchHybridEngine":
+                    noise_shared: bool = True, bold_batch_mean: bool = False) -> "BatchHybridEngine":
+        self = cls._build(base_config, n_sweep, device, engine, delay_mode, noise_seed, noise_shared)
+        if bold_batch_mean and self._multi is not None:
+            _err(L.HB_INVALID_CONFIG, "the batch-mean BOLD monitor is built for single-network configurations; "
+                                      "the multi-subnetwork path records one BOLD series per sweep point")
+        self.bold_batch_mean = bool(bold_batch_mean)
+        return self
+
+    @classmethod
+    def _build(cls, base_config, n_sweep, device, engine, delay_mode, noise_seed, noise_shared):
         cfg = base_config
         if _needs_multi(cfg):
             # the multi-subnetwork branch runs the batch engine's step; the plain
@@ -475,18 +492,22 @@ class BatchHybridEngine:
         trj = np.ctypeslib.as_array(r.trajectories, (n * r.n_steps,)).copy() if traj and r.n_steps else None
         bold = None
         if r.bold and r.n_bold > 0:
-            bold = [np.ctypeslib.as_array(r.bold, (r.n_sweep * r.n_bold * r.n_nodes,)).copy()
-                    .reshape(r.n_sweep, r.n_bold, r.n_nodes)]
+            # one series per sweep point, or one fed the batch mean (bold_batch_mean)
+            bold = [np.ctypeslib.as_array(r.bold, (r.bold_series * r.n_bold * r.n_nodes,)).copy()
+                    .reshape(r.bold_series, r.n_bold, r.n_nodes)]
+        self.step = int(r.step)
         out = BatchSweepResult(param_values=list(values), tavg=[tavg], final_states=[fin],
                                trajectories=[trj] if trj is not None else None, bold=bold,
                                elapsed_ms=r.elapsed_ms, n_sweep=r.n_sweep, kernel_ms=r.kernel_ms)
         L.lib().hb_result_free(C.byref(r))
         return out
 
-    def _run(self, param: SweepParam, values, n_steps: int, traj: bool, exact: bool) -> BatchSweepResult:
+    def _run(self, param: SweepParam, values, n_steps: int, traj: bool, exact: bool,
+             restart: bool = False) -> BatchSweepResult:
         v = np.ascontiguousarray(values, np.float32)
         r = L.HbResult()
-        flags = (L.FLAG_TRAJECTORY if traj else 0) | (L.FLAG_EXACT if exact else 0)
+        flags = ((L.FLAG_TRAJECTORY if traj else 0) | (L.FLAG_EXACT if exact else 0)
+                 | (L.FLAG_RESTART if restart else 0) | (L.FLAG_BOLD_BATCH_MEAN if self.bold_batch_mean else 0))
         L.check(L.lib().hb_run_sweep(self._h, param.sub_idx, param.param_idx, v.ctypes.data_as(L.fp), len(v),
                                      int(n_steps), flags, C.byref(r)))
         return self._result(r, v.tolist(), traj)
@@ -525,7 +546,10 @@ class BatchHybridEngine:
         heun = self._cfg.integrator == L.HEUN
         return (self.model == L.G2DO) != heun
 
-    def run_sweep(self, param: SweepParam, param_values, n_steps: int, exact: bool = False) -> BatchSweepResult:
+    def run_sweep(self, param: SweepParam, param_values, n_steps: int, exact: bool = False,
+                  restart: bool = False) -> BatchSweepResult:
+        """Run n_steps from where the previous run stopped (engine.rs:496-540);
+        restart=True starts from the configured initial state at step 0."""
         if self._multi is None and self._hybrid_changes_rule():
             cfg = self._pycfg
             if any(m.monitor_type.lower() == "bold" for m in cfg.monitors):
@@ -536,13 +560,13 @@ class BatchHybridEngine:
             self._nsig = cfg.nsig
         if self._multi is not None:
             return self._run_multi(param, param_values, n_steps)
-        return self._run(param, param_values, n_steps, False, exact)
+        return self._run(param, param_values, n_steps, False, exact, restart)
 
     def run_sweep_with_trajectory(self, param: SweepParam, param_values, n_steps: int,
-                                  exact: bool = False) -> BatchSweepResult:
+                                  exact: bool = False, restart: bool = False) -> BatchSweepResult:
         if self._multi is not None:
             return self._run_multi(param, param_values, n_steps, traj=True)
-        return self._run(param, param_values, n_steps, True, exact)
+        return self._run(param, param_values, n_steps, True, exact, restart)
 
     def run_sweep_multi(self, spec: SweepSpec, n_steps: int, exact: bool = False,
                         trajectory: bool = False) -> BatchSweepResult:

@@ -66,6 +66,13 @@ class Instance:
     bold_tr_ms: float = 0.0
     record_traj: bool = False
     record_bold: bool = False
+    # continuation of an earlier run (hb_instance.step0 / history / bold_state):
+    # steps already integrated, the carried pre() history (float32 array of
+    # hb_history_floats() values, read when step0 > 0 and overwritten with the
+    # history at the run's end) and the BOLD monitor state [n_nodes][4] (in/out)
+    step0: int = 0
+    history: Optional[np.ndarray] = None
+    bold_state: Optional[np.ndarray] = None
 
     @property
     def nvar(self) -> int:
@@ -166,6 +173,13 @@ class _Marshal:
         s.max_delay_hint = int(I.max_delay_hint)
         s.bold_period = int(I.bold_period) if I.record_bold else 0
         s.bold_tr_ms = float(I.bold_tr_ms)
+        s.step0 = int(I.step0)
+        for name in ("history", "bold_state"):   # in/out buffers: written in place
+            a = getattr(I, name)
+            if a is not None:
+                if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous):
+                    raise ValueError(f"Instance.{name} must be a C-contiguous float32 array (written in place)")
+                setattr(s, name, _fptr(self._k(a)))
         out = Output(tavg=np.zeros((I.n_nodes, nv), np.float32),
                      final_state=np.zeros((I.n_nodes, nv), np.float32))
         s.tavg = _fptr(out.tavg)

@@ -94,7 +94,11 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
     times step * dt_ms (default dt).
     bold_period > 0: a BOLD monitor on subnetwork 0, one series per sweep
     point, windows of bold_period steps resampled to bold_tr_ms (<= 0: the
-    native window samples), bold_monitor.rs:72-227."""
+    native window samples), bold_monitor.rs:72-227. The reference's batch
+    engine instead feeds one monitor the mean of var0 over the sweep points
+    (engine.rs:785-801); the Balloon-Windkessel model is nonlinear, so with
+    n_sweep > 1 these per-point series are not the reference's single series
+    (the single-network engine has that monitor: bold_batch_mean)."""
     keep = []
     subs = (L.HbSubnetwork * len(subnets))()
     for k, S in enumerate(subnets):

@@ -53,6 +53,12 @@ enum { HB_ENGINE_BATCH = 0, HB_ENGINE_PLAIN = 1 };
 #define HB_FLAG_NO_BLOCKING 0x8u  /* disable the delay-blocked schedule (one barrier per step) */
 #define HB_FLAG_AUTOTUNE 0x10u    /* time candidate launch shapes per shape class before planning */
 #define HB_FLAG_SEPARATE_BOLD 0x20u /* adjoint: BOLD monitor forward in its own kernel, not in helper waves */
+#define HB_FLAG_KEEP_NOISE 0x40u  /* plans: generate the noise at the first hb_plan_run only; later runs
+                                     reuse it (pre-generated noise resident in HBM, as an input) */
+#define HB_FLAG_RESTART 0x80u     /* hb_run_sweep*: start from the configured initial state at step 0
+                                     instead of continuing the engine's previous run */
+#define HB_FLAG_BOLD_BATCH_MEAN 0x100u /* hb_run_sweep: one BOLD monitor fed var0 averaged over the sweep
+                                     points (engine.rs:785-801) instead of one monitor per point */
 
 /* ---------------- config-level API (BatchHybridEngine mirror) ---------------- */
 
@@ -117,6 +123,8 @@ typedef struct {
     int32_t n_bold;
     double elapsed_ms;       /* wall time of the run call (engine.rs:503,825) */
     double kernel_ms;        /* integration kernel time measured with HIP events */
+    int32_t bold_series;     /* BOLD series in bold: n_sweep (one monitor per point) or 1 (batch mean) */
+    int64_t step;            /* the engine's step counter after the run (self.step) */
 } hb_result;
 
 typedef struct hb_engine hb_engine;
@@ -125,11 +133,20 @@ typedef struct hb_engine hb_engine;
 int hb_engine_create(const hb_sim_config* cfg, int32_t n_sweep, int32_t device, hb_engine** out);
 void hb_engine_destroy(hb_engine* e);
 /* BatchHybridEngine::run_sweep / run_sweep_with_trajectory (engine.rs:477-494).
- * n_values must equal n_sweep (checked; the reference broadcasts unchecked). */
+ * n_values must equal n_sweep (checked; the reference broadcasts unchecked).
+ * Like the reference engine (engine.rs:496-540), a run continues from where
+ * the previous one stopped: states, step counter, delay histories and BOLD
+ * monitors carry over (HB_FLAG_RESTART starts afresh); tavg averages this
+ * run's steps; bold holds every BOLD sample since the engine started,
+ * resampled to TR (BoldMonitor::flush, bold_monitor.rs:210-227), as
+ * [bold_series][n_bold][n_nodes]. */
 int hb_run_sweep(hb_engine* e, int32_t sub_idx, int32_t param_idx, const float* values,
                  int32_t n_values, int64_t n_steps, uint32_t flags, hb_result* out);
 /* BatchHybridEngine::run_sweep_multi (engine.rs:845-860): Cartesian product,
- * last dimension fastest. */
+ * last dimension fastest. As the reference does (engine.rs:886-909,
+ * 1326-1331), every point starts from sweep point 0 of the engine's current
+ * state, at step 0, without BOLD monitors; the engine then holds the run's
+ * states, step and histories (and no monitors until HB_FLAG_RESTART). */
 int hb_run_sweep_multi(hb_engine* e, const hb_sweep_point* points, int32_t n_points,
                        int64_t n_steps, uint32_t flags, hb_result* out);
 void hb_result_free(hb_result* r);
@@ -172,6 +189,16 @@ typedef struct {
     float* bold;              /* [bold_cap][n_nodes] or NULL */
     int32_t bold_cap;
     int32_t* bold_n;
+    /* continuation of an earlier run (all optional; zero for a fresh run) */
+    int64_t step0;            /* steps integrated before this run: delays, the step-0 rule and the
+                                 batch window read global steps step0 + t */
+    float* history;           /* in/out pre() history, hb_history_floats(inst) floats, [max_delay][n_nodes]
+                                 [channels] oldest first: read at plan time when step0 > 0 (the last
+                                 max_delay steps before step0), written by hb_plan_download (those
+                                 before the run's end) */
+    float* bold_state;        /* in/out Balloon-Windkessel s, f, v, q per node [n_nodes][4]: read at
+                                 plan time (the monitor's state at the run's start), written by
+                                 hb_plan_download */
 } hb_instance;
 
 typedef struct {
@@ -202,6 +229,12 @@ int hb_plan_download(hb_plan* p);
 void hb_plan_destroy(hb_plan* p);
 /* One-call convenience: create, run, download, destroy. */
 int hb_run_instances(const hb_instance* inst, int32_t n, const hb_run_opts* opts, hb_run_stats* stats);
+
+/* Floats of an instance's carried delay history (hb_instance.history): max_delay
+ * steps x n_nodes x the pre() channels its ring keeps (Kuramoto 2 per cvar,
+ * SigmoidalJansenRit 1, MPR 1 when its cv coefficient is 0, else ncvar);
+ * 0 without delays. */
+int64_t hb_history_floats(const hb_instance* inst);
 
 /* Philox4x32-10 standard normals (Box-Muller in f64), written as f32 to a host
  * buffer: element k of stream s under key seed. Device-generated. */

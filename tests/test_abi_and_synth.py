@@ -239,3 +239,25 @@ def test_welch_host_validation():
     y = np.ones((1, 10, 2), np.float32)
     assert lib.hb_spectral_features_batch(y.ctypes.data_as(L.fp), 1, 10, 2, 100.0, f.ctypes.data_as(L.fp), 0) == 0
     assert np.all(np.isnan(f))
+
+
+def test_history_floats():
+    """hb_history_floats: max_delay steps x nodes x ring channels (Kuramoto 2
+    per cvar; MPR 1 while its cv coefficient is 0, else 2); 0 without delays."""
+    from hyburn_amd.instances import _Marshal
+    from hyburn_amd.synth import MPR_DEFAULT, KUR_DEFAULT
+    ip = np.array([0, 1, 2, 3, 4, 5], np.int32)
+    ix = np.array([1, 2, 3, 4, 0], np.int32)
+    w = np.ones(5, np.float32)
+    d = np.array([3, 12, 1, 7, 9], np.int32)
+    csr = dict(wkind=L.W_CSR, indptr=ip, indices=ix, weights=w, edge_delays=d, n_nodes=5,
+               x0=np.zeros((5, 2), np.float32))
+    cases = [(_base(**csr), 12 * 5),
+             (_base(**{**csr, "model": L.MPR, "params": MPR_DEFAULT}), 12 * 5),
+             (_base(**{**csr, "model": L.MPR, "params": np.array([1, 1, -5, 15, 0, 1, 0.5], np.float32)}), 12 * 5 * 2),
+             (_base(**{**csr, "model": L.KURAMOTO, "params": KUR_DEFAULT, "x0": np.zeros((5, 1), np.float32),
+                       "cfn": L.CF_KURAMOTO}), 12 * 5 * 2),
+             (_base(), 0)]
+    for I, want in cases:
+        m = _Marshal([I])
+        assert L.lib().hb_history_floats(m.arr) == want
