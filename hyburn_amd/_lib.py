@@ -185,6 +185,8 @@ PROTOTYPES = {
     "hb_autotune_entries": (C.c_int, [ip, C.c_int32]),
     "hb_autotune_clear": (None, []),
     "hb_fc_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
+    "hb_fc_batch_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+                                      C.c_void_p]),
     "hb_fcd_windows": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
     "hb_fcd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
     "hb_run_multi": (C.c_int, [C.POINTER(HbMultiSpec)]),

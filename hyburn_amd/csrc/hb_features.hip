@@ -16,6 +16,8 @@
 #include "hb_internal.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 namespace hb {
@@ -423,45 +425,88 @@ __global__ __launch_bounds__(256) void k_spectral(const double* __restrict__ psd
     }
 }
 
+// Launch shape of one FC call: column sums in up to 8 segments of >= 256
+// samples (added in order), and the sample split of the tile kernel, chosen
+// so that a launch has about two workgroups per CU while each segment keeps
+// at least four 32-sample chunks.
+struct FcShape {
+    int nseg, nb, nbp, nsplit;
+    size_t blocks;
+};
+
+FcShape fc_shape(size_t nj, int N, int T) {
+    FcShape s;
+    s.nseg = std::max(1, std::min(8, T / 256));
+    s.nb = (N + kBlk - 1) / kBlk;
+    s.nbp = s.nb * (s.nb + 1) / 2;
+    s.blocks = nj * (size_t)s.nbp;
+    const int nch = (T + kChunk - 1) / kChunk;
+    int nsplit = (int)std::min<size_t>(16, (512 + s.blocks - 1) / s.blocks);
+    s.nsplit = std::max(1, std::min(nsplit, nch / 4));
+    return s;
+}
+
+// Doubles of the column sums and of the split partials a call needs.
+size_t fc_mean_doubles(size_t nj, int N, const FcShape& s) { return nj * (size_t)N * s.nseg; }
+size_t fc_partial_doubles(const FcShape& s) { return s.nsplit > 1 ? s.blocks * s.nsplit * kPart : 0; }
+
+// The kernels of one call on stream st: means, block-pair tiles, and the
+// ordered sum of the split partials. dj holds nj jobs with their mean
+// pointers set; partial holds fc_partial_doubles(s) doubles (or null).
+hipError_t launch_fc(const FcJob* dj, size_t nj, int N, const FcShape& s, double* partial, hipStream_t st) {
+    if ((uint64_t)nj * (uint64_t)s.nbp * 16u > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fc_means, dim3((unsigned)nj, (unsigned)((N + 63) / 64), (unsigned)s.nseg), dim3(256), 0, st,
+                       dj, s.nseg);
+    if (s.nb == 1)
+        hipLaunchKernelGGL(k_fc_tiles<true>, dim3((unsigned)(s.blocks * s.nsplit)), dim3(256), 0, st, dj, s.nbp,
+                           s.nseg, s.nsplit, partial);
+    else
+        hipLaunchKernelGGL(k_fc_tiles<false>, dim3((unsigned)(s.blocks * s.nsplit)), dim3(512), 0, st, dj, s.nbp,
+                           s.nseg, s.nsplit, partial);
+    if (s.nsplit > 1)
+        hipLaunchKernelGGL(k_fc_finish, dim3((unsigned)s.blocks, (unsigned)((std::min(N, kBlk) * kBlk + 255) / 256)),
+                           dim3(256), 0, st, dj, s.nbp, s.nsplit, partial);
+    return hipGetLastError();
+}
+
 // Every job of one call has the same N. Fills each job's mean pointer from a
-// work buffer, then launches the means and the block-pair tile kernels.
+// work buffer, uploads the jobs and launches them on the null stream.
 hipError_t run_fc_jobs(std::vector<FcJob>& jobs, int N, int T, DevBufs& B) {
     const size_t nj = jobs.size();
-    // column sums in up to 8 segments of >= 256 samples, added in order
-    const int nseg = std::max(1, std::min(8, T / 256));
+    const FcShape s = fc_shape(nj, N, T);
     double* mean = nullptr;
     FcJob* dj = nullptr;
-    hipError_t e = B.alloc(&mean, nj * (size_t)N * nseg);
+    hipError_t e = B.alloc(&mean, fc_mean_doubles(nj, N, s));
     if (e == hipSuccess) e = B.alloc(&dj, nj);
     if (e != hipSuccess) return e;
-    for (size_t q = 0; q < nj; ++q) jobs[q].mean = mean + q * (size_t)N * nseg;
+    for (size_t q = 0; q < nj; ++q) jobs[q].mean = mean + q * (size_t)N * s.nseg;
     e = hipMemcpy(dj, jobs.data(), sizeof(FcJob) * nj, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
-    const int nb = (N + kBlk - 1) / kBlk, nbp = nb * (nb + 1) / 2;
-    if ((uint64_t)nj * (uint64_t)nbp * 16u > 0x7fffffffu) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_fc_means, dim3((unsigned)nj, (unsigned)((N + 63) / 64), (unsigned)nseg), dim3(256), 0,
-                       nullptr, dj, nseg);
-    // split the samples so that a launch has about two workgroups per CU,
-    // keeping at least four 32-sample chunks per segment
-    const size_t blocks = nj * (size_t)nbp;
-    const int nch = (T + kChunk - 1) / kChunk;
-    int nsplit = (int)std::min<size_t>(16, (512 + blocks - 1) / blocks);
-    nsplit = std::max(1, std::min(nsplit, nch / 4));
     double* partial = nullptr;
-    if (nsplit > 1) {
-        e = B.alloc(&partial, blocks * nsplit * kPart);
+    if (fc_partial_doubles(s)) {
+        e = B.alloc(&partial, fc_partial_doubles(s));
         if (e != hipSuccess) return e;
     }
-    if (nb == 1)
-        hipLaunchKernelGGL(k_fc_tiles<true>, dim3((unsigned)(blocks * nsplit)), dim3(256), 0, nullptr, dj, nbp, nseg,
-                           nsplit, partial);
-    else
-        hipLaunchKernelGGL(k_fc_tiles<false>, dim3((unsigned)(blocks * nsplit)), dim3(512), 0, nullptr, dj, nbp, nseg,
-                           nsplit, partial);
-    if (nsplit > 1)
-        hipLaunchKernelGGL(k_fc_finish, dim3((unsigned)blocks, (unsigned)((std::min(N, kBlk) * kBlk + 255) / 256)),
-                           dim3(256), 0, nullptr, dj, nbp, nsplit, partial);
-    return hipGetLastError();
+    return launch_fc(dj, nj, N, s, partial, nullptr);
+}
+
+// The jobs of a full-matrix FC call over B contiguous [T][N] series, written
+// on the device (the no-allocation entry point has no host staging).
+__global__ void k_fc_make_jobs(FcJob* jobs, const float* x, float* out, double* mean, int B, int T, int N,
+                               int nseg) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    FcJob J;
+    J.x = x + (size_t)T * N * b;
+    J.out = out + (size_t)N * N * b;
+    J.mean = mean + (size_t)b * N * nseg;
+    J.T = T;
+    J.N = N;
+    J.ldx = N;
+    J.upper = 0;
+    J.ostr = 1;
+    J.fcd = 0;
+    jobs[b] = J;
 }
 
 static int next_pow2(int n) {
@@ -501,6 +546,14 @@ hipError_t run_welch(const float* dy, int Bn, int T, int N, int nf, int w, doubl
     return hipGetLastError();
 }
 
+// Scratch of the no-allocation FC entry point (hb_fc_batch_noalloc): jobs,
+// column sums and split partials of one call. Each GPU holds its own copy
+// from the first call on it until the process ends; its address comes from
+// hipGetSymbolAddress with that device current (the symbol's name is only a
+// host-side stand-in). Calls that overlap on one GPU share it and race.
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
 }  // namespace feat
 }  // namespace hb
 
@@ -517,6 +570,35 @@ int check_series(const float* series, int32_t B, int32_t T, int32_t N, const voi
 
 int hip_fail(hipError_t e, const char* what) {
     return set_error(HB_BACKEND_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Device address of `scratch` on each device, looked up on the first
+// no-allocation call there (with that device current) and kept.
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch;
+
+hipError_t scratch_on(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it != g_scratch.end()) { *out = (unsigned char*)it->second; return hipSuccess; }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(feat::scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = p;
+    *out = (unsigned char*)p;
+    return hipSuccess;
+}
+
+// True when p is device memory of `device` (or managed memory) that the
+// kernels may read or write.
+bool device_memory_on(const void* p, int device) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (a.type == hipMemoryTypeManaged) return true;
+    return a.type == hipMemoryTypeDevice && a.device == device;
 }
 
 }  // namespace
@@ -541,6 +623,42 @@ int hb_fc_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t 
     e = feat::run_fc_jobs(jobs, n_nodes, n_steps, B);
     if (e == hipSuccess) e = hipMemcpy(fc, dout, sizeof(float) * nn * n_series, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "FC kernels");
+    return HB_OK;
+}
+
+int hb_fc_batch_noalloc(const float* d_series, int32_t n_series, int32_t n_steps, int32_t n_nodes, float* d_fc,
+                        int32_t device, void* stream) {
+    const int rc = check_series(d_series, n_series, n_steps, n_nodes, d_fc);
+    if (rc) return rc;
+    const size_t nj = (size_t)n_series;
+    const feat::FcShape s = feat::fc_shape(nj, n_nodes, n_steps);
+    const size_t jobs_b = (sizeof(feat::FcJob) * nj + 255) & ~(size_t)255;
+    const size_t mean_b = (sizeof(double) * feat::fc_mean_doubles(nj, n_nodes, s) + 255) & ~(size_t)255;
+    const size_t part_b = sizeof(double) * feat::fc_partial_doubles(s);
+    if ((uint64_t)nj * (uint64_t)s.nbp * 16u > 0x7fffffffu || jobs_b + mean_b + part_b > feat::kNoallocScratchBytes)
+        return set_error(HB_INVALID_CONFIG, "hb_fc_batch_noalloc: this call needs " +
+                                                std::to_string(jobs_b + mean_b + part_b) + " bytes of scratch; the "
+                                                "no-allocation entry point holds " +
+                                                std::to_string(feat::kNoallocScratchBytes) + " per GPU");
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const size_t nx = (size_t)n_steps * n_nodes * nj;
+    if (!device_memory_on(d_series, device) || !device_memory_on(d_series + nx - 1, device) ||
+        !device_memory_on(d_fc, device) || !device_memory_on(d_fc + (size_t)n_nodes * n_nodes * nj - 1, device))
+        return set_error(HB_INVALID_CONFIG, "hb_fc_batch_noalloc: series and fc must be device memory of the "
+                                            "given device");
+    unsigned char* base = nullptr;
+    e = scratch_on(device, &base);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    feat::FcJob* dj = (feat::FcJob*)base;
+    double* mean = (double*)(base + jobs_b);
+    double* partial = part_b ? (double*)(base + jobs_b + mean_b) : nullptr;
+    const hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(feat::k_fc_make_jobs, dim3((unsigned)((nj + 63) / 64)), dim3(64), 0, st, dj, d_series, d_fc,
+                       mean, n_series, n_steps, n_nodes, s.nseg);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = feat::launch_fc(dj, nj, n_nodes, s, partial, st);
+    if (e != hipSuccess) return hip_fail(e, "FC kernels (no-allocation)");
     return HB_OK;
 }
 

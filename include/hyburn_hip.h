@@ -410,6 +410,18 @@ int hb_run_multi(hb_multi_spec* spec);
  * standard deviation (< 1e-12 -> 1), clamp to [-1, 1], f32, symmetric. */
 int hb_fc_batch(const float* series, int32_t n_series, int32_t n_steps, int32_t n_nodes, float* fc,
                 int32_t device);
+/* No counterpart in the reference: the FC entry point for callers that cannot
+ * allocate (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). d_series [n_series][n_steps][n_nodes] and d_fc
+ * [n_series][n_nodes][n_nodes] are device memory on `device`; the kernels of
+ * hb_fc_batch run on `stream` (a hipStream_t, NULL: the null stream) and are
+ * not waited for. Jobs, column sums and split partials live in a fixed 64 MiB
+ * device array per GPU (looked up on the first call on a device: make that
+ * call outside graph capture); calls whose scratch exceeds it return
+ * HB_INVALID_CONFIG before any GPU work. Every call on a GPU shares its array:
+ * calls that overlap on one GPU race. Nothing in it needs clearing. */
+int hb_fc_batch_noalloc(const float* d_series, int32_t n_series, int32_t n_steps, int32_t n_nodes, float* d_fc,
+                        int32_t device, void* stream);
 /* Number of FCD windows, (n_steps - window_len) / stride + 1, as fcd_matrix
  * counts them (src/sbi/features/fc.rs:104); 0 for an invalid window. */
 int32_t hb_fcd_windows(int32_t n_steps, int32_t window_len, int32_t stride);
