@@ -202,10 +202,12 @@ class _Marshal:
         return self.outputs
 
 
-def _opts(device: int, exact: bool, graph: bool, block_size: int, grid: int, autotune: bool = False) -> L.HbRunOpts:
+def _opts(device: int, exact: bool, graph: bool, block_size: int, grid: int, autotune: bool = False,
+          keep_noise: bool = False) -> L.HbRunOpts:
     o = L.HbRunOpts()
     o.device = device
-    o.flags = (L.FLAG_EXACT if exact else 0) | (0 if graph else L.FLAG_NO_GRAPH) | (L.FLAG_AUTOTUNE if autotune else 0)
+    o.flags = ((L.FLAG_EXACT if exact else 0) | (0 if graph else L.FLAG_NO_GRAPH) | (L.FLAG_AUTOTUNE if autotune else 0)
+               | (L.FLAG_KEEP_NOISE if keep_noise else 0))
     o.block_size = block_size
     o.grid_waves = grid
     return o
@@ -216,10 +218,13 @@ class Plan:
     in HBM until download()."""
 
     def __init__(self, insts: Sequence[Instance], device: int = 0, exact: bool = False,
-                 graph: bool = True, block_size: int = 0, grid: int = 0, autotune: bool = False):
+                 graph: bool = True, block_size: int = 0, grid: int = 0, autotune: bool = False,
+                 keep_noise: bool = False):
+        """keep_noise: the first run() generates the noise, later runs reuse it
+        (pre-generated noise resident in HBM, as an input)."""
         self._m = _Marshal(insts)
         self._h = C.c_void_p()
-        o = _opts(device, exact, graph, block_size, grid, autotune)
+        o = _opts(device, exact, graph, block_size, grid, autotune, keep_noise)
         L.check(L.lib().hb_plan_create(self._m.arr, len(insts), C.byref(o), C.byref(self._h)))
         self.node_steps = float(sum(I.n_nodes * I.n_steps for I in insts))
 
