@@ -14,8 +14,9 @@ Prints one JSON line (metric, value, unit, roofline, cpu_baseline, ...). Extra
 legs, each with its own switch, add objects to the same line: "ragged" (4096
 ragged MPR/Jansen-Rit instances in one plan), "features" (FC, FCD and spectral
 features of 1024 BOLD-length series), "ring" (the G2DO -> JR -> WC network of
-the reference's generic-engine benchmark) and "adjoint" (gradients of the FC
-fit and Kuramoto order-parameter workloads).
+the reference's generic-engine benchmark), "dense" (dense all-to-all coupling
+of 4096 points on the matrix cores against per-lane row sums) and "adjoint"
+(gradients of the FC fit and Kuramoto order-parameter workloads).
 """
 from __future__ import annotations
 
@@ -270,6 +271,44 @@ def ring_leg(args):
     return res
 
 
+def dense_leg(args):
+    """Dense all-to-all coupling at large batch: Generic2dOscillator, 74 nodes,
+    the batched sweep's 20% lognormal connectome as a dense matrix, no delay,
+    Heun, 4096 sweep points of I x 2000 steps. The matrix-core product over
+    16-instance tiles against the per-lane row sums (HB_FLAG_NO_MFMA), both
+    FMA mode, inputs resident, kernel time from HIP events."""
+    import ctypes as C
+
+    from hyburn_amd import _lib as L
+    from hyburn_amd.instances import Instance, _Marshal, _opts
+    from hyburn_amd.synth import G2DO_DEFAULT, lognormal_connectome
+    N, B, T = 74, args.dense_batch, 2000
+    W = lognormal_connectome(np.random.default_rng(3), N, 0.20)
+    x0 = np.random.default_rng(5).normal(0.0, 0.1, (N, 2)).astype(np.float32)
+    insts = [Instance(model=L.G2DO, n_nodes=N, n_steps=T, dt=0.1, params=G2DO_DEFAULT.copy(), x0=x0,
+                      integrator=L.HEUN, wkind=L.W_DENSE, dense=W, cfn=L.CF_LINEAR, cparams=(0.05, 0.0),
+                      sweep_form=1, sweep_val=float(v)) for v in np.linspace(-0.5, 0.5, B)]
+    res = {"workload": f"G2DO {N} nodes, dense 20% lognormal, no delay, Heun, {B} points x {T} steps"}
+    ns = float(B * N * T)
+    for name, extra in (("matrix_cores", 0), ("lane_row_sums", L.FLAG_NO_MFMA)):
+        m = _Marshal(insts)
+        o = _opts(0, False, True, 0, 0)
+        o.flags |= extra
+        h = C.c_void_p()
+        L.check(L.lib().hb_plan_create(m.arr, len(insts), C.byref(o), C.byref(h)))
+        st = L.HbRunStats()
+        L.check(L.lib().hb_plan_run(h, C.byref(st)))
+        ks = []
+        for _ in range(3):
+            L.check(L.lib().hb_plan_run(h, C.byref(st)))
+            ks.append(st.kernel_ms)
+        L.lib().hb_plan_destroy(h)
+        k = sum(ks) / len(ks)
+        res[name] = {"node_steps_per_s": ns / (k * 1e-3), "kernel_ms": k, "block": st.block_size}
+    res["speedup"] = res["matrix_cores"]["node_steps_per_s"] / res["lane_row_sums"]["node_steps_per_s"]
+    return res
+
+
 def _fc64(B):
     n = B.shape[0]
     A = B - B.mean(0)
@@ -341,6 +380,9 @@ def main() -> int:
     ap.add_argument("--no-features", dest="features", action="store_false",
                     help="skip the FC / FCD / spectral leg")
     ap.add_argument("--features-series", type=int, default=1024)
+    ap.add_argument("--no-dense", dest="dense", action="store_false",
+                    help="skip the dense all-to-all (matrix cores vs row sums) leg")
+    ap.add_argument("--dense-batch", type=int, default=4096)
     ap.add_argument("--adj4-batch", type=int, default=256)
     ap.add_argument("--adj5-batch", type=int, default=512)
     ap.add_argument("--adj5-steps", type=int, default=10_000)
@@ -453,6 +495,8 @@ def main() -> int:
         line["features"] = features_leg(args)
     if args.ring:
         line["ring"] = ring_leg(args)
+    if args.dense:
+        line["dense"] = dense_leg(args)
     if not args.no_adjoint:
         import dataclasses
         from hyburn_amd.adjoint import Loss

@@ -23,7 +23,7 @@ DELAY_PER_EDGE, DELAY_SINGLE, DELAY_BATCH_COMPAT = 0, 1, 2
 ENGINE_BATCH, ENGINE_PLAIN = 0, 1
 FLAG_TRAJECTORY, FLAG_EXACT, FLAG_NO_GRAPH, FLAG_NO_BLOCKING, FLAG_AUTOTUNE = 0x1, 0x2, 0x4, 0x8, 0x10
 FLAG_SEPARATE_BOLD = 0x20
-FLAG_KEEP_NOISE, FLAG_RESTART, FLAG_BOLD_BATCH_MEAN = 0x40, 0x80, 0x100
+FLAG_KEEP_NOISE, FLAG_RESTART, FLAG_BOLD_BATCH_MEAN, FLAG_NO_MFMA = 0x40, 0x80, 0x100, 0x200
 LOSS_FC, LOSS_BOLD_MSE, LOSS_ORDER = 0, 1, 2
 
 MODEL_NAMES = {

@@ -44,6 +44,7 @@ UNITS = [
     ("hb_noise.hip", "hb_noise.o", [], True),
     ("hb_features.hip", "hb_features.o", [], True),
     ("hb_multi.hip", "hb_multi.o", [], True),
+    ("hb_dense.hip", "hb_dense.o", [], True),
     *[("hb_adjoint.hip", f"hb_adjoint_u{u}.o", [f"-DHB_ADJ_UNIT={u}"], True) for u in range(ADJOINT_UNITS)],
     ("hb_host.cpp", "hb_host.o", [], False),
     ("hb_engine.cpp", "hb_engine.o", [], False),

@@ -240,6 +240,7 @@ thread_local std::string g_launch_err;   // the step of launch_groups that faile
 
 hipError_t launch_groups(hb_plan* P, bool exact) {
     auto go = [&](const LaunchDesc& L, hipStream_t st) {
+        if (L.mfma) return launch_dense(L, st);
         return exact ? launch_forward_exact(L, st) : launch_forward_fast(L, st);
     };
     g_launch_err.clear();
@@ -297,6 +298,11 @@ int autotune_plan(const hb_instance* inst, int n, const hb_run_opts* opts, int l
         if (I.n_nodes > 512) {
             key = tune_key(I, np, 0, 0);
             cand = {256, 512};
+        } else if (I.wkind == HB_W_DENSE && max_delay_of(I) == 0 && !(opts->flags & HB_FLAG_EXACT) &&
+                   !(opts->flags & HB_FLAG_NO_MFMA) && I.node_param_idx < 0 && !(I.bold && I.bold_period > 0) &&
+                   I.n_nodes <= kDenseMaxNodes) {
+            key = tune_key(I, np, 2, 0);
+            cand = {kChoiceLanes, kChoiceMfma};
         } else if (per_edge && !(opts->flags & HB_FLAG_EXACT) && !(opts->flags & HB_FLAG_NO_BLOCKING)) {
             int dmin = 1 << 30;
             for (int e = 0; e < I.nnz; ++e) dmin = std::min(dmin, (int)I.edge_delays[e]);
@@ -443,6 +449,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         if (device < 64 && !attrs_done[device]) {
             HB_HIP(forward_set_attributes_fast(lds_budget));
             HB_HIP(forward_set_attributes_exact(lds_budget));
+            HB_HIP(dense_set_attributes(lds_budget));
             attrs_done[device] = true;
         }
     }
@@ -464,7 +471,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
     // ---- per-instance shapes, LDS-vs-HBM ring, block size
     struct Shape {
         int nv, nc, np, depth, block, blocked, lpn;
-        bool multi, lds_ring;
+        bool multi, lds_ring, mfma;
         int lds_bytes;
         double cost;
     };
@@ -537,6 +544,22 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         // ring floats: depth slots per node and channel (+ K-1 mirrored slots when blocked)
         const int ring_b = (s.depth + (s.blocked ? s.blocked - 1 : 0)) * I.n_nodes * s.np * 4;
         s.lds_ring = (int64_t)ring_b + tab_b <= lds_budget;
+        // undelayed dense coupling of a batch sharing one matrix: the
+        // matrix-core product over 16-instance tiles (the group size is
+        // checked when the launch groups are formed)
+        s.mfma = false;
+        if (!exact && !(P->flags & HB_FLAG_NO_MFMA) && I.wkind == HB_W_DENSE && max_delay_of(I) == 0 &&
+            I.node_param_idx < 0 && !(I.bold && I.bold_period > 0) && I.n_nodes <= kDenseMaxNodes && !s.multi &&
+            s.np <= 2 && req_block <= 0) {
+            int want = 0;
+            if (g_force_choice > 0) want = g_force_choice;
+            else if (autotune) {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                const auto it = g_tuned.find(tune_key(I, s.np, 2, 0));
+                if (it != g_tuned.end()) want = it->second.choice;
+            }
+            s.mfma = want == 0 || want == kChoiceMfma;
+        }
         s.lds_bytes = (s.lds_ring ? ring_b : 0) + tab_b;
         if (s.lds_bytes > lds_budget) {
             hb_plan_destroy(P);
@@ -702,7 +725,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
     }
     // instance descriptors, order, counters, noise jobs
     const size_t off_inst = A.take(sizeof(DevInst) * n);
-    const size_t off_order = A.take(sizeof(int32_t) * n);
+    const size_t off_order = A.take(sizeof(int32_t) * (size_t)n * 17);   // + padding of 16-instance tiles
     const size_t off_cnt = A.take(sizeof(int32_t) * 64);
     const size_t off_jobs = A.take(sizeof(NoiseJob) * std::max<size_t>(jobs_host.size(), 1));
 
@@ -814,13 +837,55 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
     // ---- launch groups: (model, pre channels, block, lds_ring, multi), each a
     // single-model kernel; cost-descending order inside. Groups run as
     // parallel graph branches, so groups of different models share the chip.
+    // Dense batches on the matrix cores: instances of one model, size, weight
+    // matrix, coupling function, integrator, length and sweep form share a
+    // kernel's uniform controls; a batch smaller than kDenseMinBatch keeps
+    // the lane loop.
+    std::map<std::tuple<int, int, int, const void*, int, int, int, int, int, int>, std::vector<int>> dense_groups;
+    for (int i = 0; i < n; ++i)
+        if (sh[i].mfma) {
+            const hb_instance& I = inst[i];
+            dense_groups[std::make_tuple(I.model, sh[i].np, I.n_nodes, (const void*)I.dense, I.cfn, I.integrator,
+                                         I.n_steps, I.sweep_form, I.engine, (int)(I.n_cparams >= 5))].push_back(i);
+        }
+    for (auto& kv : dense_groups)
+        if ((int)kv.second.size() < kDenseMinBatch)
+            for (int i : kv.second) sh[i].mfma = false;
     std::map<std::tuple<int, int, int, int, int>, std::vector<int>> groups;
     for (int i = 0; i < n; ++i)
-        groups[std::make_tuple(inst[i].model, sh[i].np, sh[i].block, (int)sh[i].lds_ring, (int)sh[i].multi)].push_back(i);
+        if (!sh[i].mfma)
+            groups[std::make_tuple(inst[i].model, sh[i].np, sh[i].block, (int)sh[i].lds_ring, (int)sh[i].multi)].push_back(i);
     std::vector<int32_t> order;
-    order.reserve(n);
+    order.reserve((size_t)n * 17);
     int gi = 0;
     int max_lds_all = 0;
+    for (auto& kv : dense_groups) {
+        auto& v = kv.second;
+        if ((int)v.size() < kDenseMinBatch) continue;
+        LaunchDesc L;
+        memset(&L, 0, sizeof(L));
+        L.insts = P->d_inst;
+        L.order = P->d_order + order.size();
+        const int N = std::get<2>(kv.first);
+        const int tiles = ((int)v.size() + 15) / 16;
+        for (int k = 0; k < tiles * 16; ++k) order.push_back(k < (int)v.size() ? v[k] : -1);
+        L.mfma = 1;
+        L.n_nodes = N;
+        L.nk = (N + 3) / 4 * 4;
+        L.n_work = tiles;
+        L.grid = tiles;
+        L.block = (N + 15) / 16 * 64;
+        L.model = std::get<0>(kv.first);
+        L.np = std::get<1>(kv.first);
+        L.lds_bytes = dense_lds_bytes(N, L.np);
+        max_lds_all = std::max(max_lds_all, L.lds_bytes);
+        double cmax = 0.0;
+        for (int i : v) cmax = std::max(cmax, sh[i].cost);
+        P->launch_cost.push_back(cmax * 0.25);
+        P->launches.push_back(L);
+        P->block_main = std::max(P->block_main, L.block);
+        P->lds_main = std::max(P->lds_main, L.lds_bytes);
+    }
     for (auto& kv : groups) {
         auto& v = kv.second;
         std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return sh[a].cost > sh[b].cost; });
@@ -860,7 +925,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         P->block_main = std::max(P->block_main, L.block);
         P->lds_main = std::max(P->lds_main, L.lds_bytes);
     }
-    HB_HIP(hipMemcpy(P->d_order, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    HB_HIP(hipMemcpy(P->d_order, order.data(), sizeof(int32_t) * order.size(), hipMemcpyHostToDevice));
 
     HB_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
     for (auto& e : P->ev) HB_HIP(hipEventCreate(&e));

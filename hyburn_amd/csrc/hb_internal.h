@@ -241,7 +241,20 @@ struct LaunchDesc {
     int32_t np;                // pre channels
     bool lds_ring;             // rings live in LDS
     bool multi;                // nodes per thread > 1 (state table in LDS)
+    int32_t mfma;              // 1: dense coupling on the matrix cores, 16 instances per workgroup
+                               // (order: tiles of 16 instance indices, -1 padding; n_work tiles)
+    int32_t n_nodes, nk;       // mfma: nodes, and nodes padded to a multiple of 4
 };
+
+// Dense all-to-all coupling on the matrix cores (hb_dense.hip): instances
+// sharing one weight matrix, no delay, FMA mode; at most kDenseMaxNodes nodes,
+// and a batch of at least kDenseMinBatch such instances.
+constexpr int kDenseMaxNodes = 128, kDenseMinBatch = 8;
+// Autotuner choice values of a dense class: the lane loop or the matrix cores.
+constexpr int kChoiceLanes = 1, kChoiceMfma = 16;
+int dense_lds_bytes(int n_nodes, int np);
+hipError_t launch_dense(const LaunchDesc& d, hipStream_t s);
+hipError_t dense_set_attributes(int max_lds);
 
 struct BoldShape {
     int native = 1;

@@ -59,6 +59,8 @@ enum { HB_ENGINE_BATCH = 0, HB_ENGINE_PLAIN = 1 };
                                      instead of continuing the engine's previous run */
 #define HB_FLAG_BOLD_BATCH_MEAN 0x100u /* hb_run_sweep: one BOLD monitor fed var0 averaged over the sweep
                                      points (engine.rs:785-801) instead of one monitor per point */
+#define HB_FLAG_NO_MFMA 0x200u    /* dense coupling: keep the per-lane row sums instead of the matrix-core
+                                     product over 16-instance tiles */
 
 /* ---------------- config-level API (BatchHybridEngine mirror) ---------------- */
 
@@ -297,7 +299,9 @@ void hb_adjoint_plan_destroy(hb_adjoint_plan* p);
 /* Launch-shape autotuner (HB_FLAG_AUTOTUNE on hb_plan_create). A shape class is
  * (model, ring channels, nodes, delay-blocked, ring in LDS, integrator class,
  * stochastic); its choice is the lanes per node of the delay-blocked kernel
- * (1, 2, 4) or the block size of a network larger than one block (256, 512),
+ * (1, 2, 4), the block size of a network larger than one block (256, 512), or,
+ * for undelayed dense coupling (blocked field 2), per-lane row sums (1) or the
+ * matrix-core product over 16-instance tiles (16),
  * timed on a short sub-plan of up to 32 sampled instances and cached for the
  * process. Instances per workgroup stay 1: every class fills at least one wave.
  * hb_autotune_entries writes up to max_entries rows of 8 int32 (model, np,

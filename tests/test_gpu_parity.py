@@ -377,7 +377,7 @@ def test_autotuned_plan_matches_default(gpu, orc):
     ent = autotune_entries()
     assert len(ent) >= 1
     for e in ent:
-        assert e["choice"] in ((1, 2, 4) if e["blocked"] else (256, 512)), e
+        assert e["choice"] in {0: (256, 512), 1: (1, 2, 4), 2: (1, 16)}[e["blocked"]], e
     for I, x, y in zip(insts, a, b):
         np.testing.assert_allclose(y.final_state, x.final_state, rtol=1e-4, atol=1e-5)
 
