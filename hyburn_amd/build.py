@@ -71,10 +71,27 @@ def _stale(src: str, obj: str) -> bool:
     return max(os.path.getmtime(d) for d in deps) > os.path.getmtime(o)
 
 
+def _source_digest() -> str:
+    """Digest of every source and header the units read."""
+    import hashlib
+    h = hashlib.sha256()
+    for d in (CSRC, os.path.join(HERE, "..", "include")):
+        for f in sorted(os.listdir(d)):
+            if f.endswith((".h", ".hip", ".cpp")):
+                h.update(f.encode())
+                with open(os.path.join(d, f), "rb") as fh:
+                    h.update(fh.read())
+    return h.hexdigest()
+
+
 def build(verbose: bool = False, jobs: int | None = None, incremental: bool = False) -> str:
     """Compile every unit for gfx950 and link libhyburn_hip.so in-tree
-    (incremental: only units whose source or headers changed since their object)."""
+    (incremental: only units whose source or headers changed since their object).
+    A source edited while the units compile would leave objects whose host and
+    device passes saw different code (kernel stubs with no device kernel), so
+    such a build is refused."""
     os.makedirs(BUILD, exist_ok=True)
+    digest0 = _source_digest()
     units = [u for u in UNITS if os.path.exists(os.path.join(CSRC, u[0]))]
     todo = [u for u in units if _stale(u[0], u[1])] if incremental else units
     jobs = jobs or min(os.cpu_count() or 8, 16, max(1, len(todo)))
@@ -86,6 +103,13 @@ def build(verbose: bool = False, jobs: int | None = None, incremental: bool = Fa
             print(f"[build] {obj}:\n{err}", file=sys.stderr)
     if failed:
         raise RuntimeError("hipcc failed for: " + ", ".join(r[0] for r in failed))
+    if _source_digest() != digest0:
+        for u in todo:   # the objects of this build are not trustworthy
+            try:
+                os.remove(os.path.join(BUILD, u[1]))
+            except OSError:
+                pass
+        raise RuntimeError("sources changed while they compiled; build again")
     objs = [os.path.join(BUILD, u[1]) for u in units]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs

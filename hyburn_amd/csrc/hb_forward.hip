@@ -623,7 +623,7 @@ __device__ __forceinline__ void run_instance(const DevInst* __restrict__ I, floa
 // CF >= 0 fixes the coupling function at compile time (-1: read per
 // instance); OUTS = false drops the trajectory and BOLD outputs. Both remove
 // per-step uniform branches from the step loop of the common sweep case.
-template <int M, int NP, bool LRING, int K, int IC, int CF, bool OUTS>
+template <int M, int NP, bool LRING, int K, int IC, int CF, bool OUTS, bool WIDE = false>
 __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float* lds) {
     const DevInst* __restrict__ Ia = I;
     const View V = make_view(I);
@@ -694,7 +694,8 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     // CSR row in registers: source base j*Hd, delay, weight
     // register-resident row edges: fewer with an HBM ring, whose deeper gather
     // pipeline needs the registers (the rest stream through the batched tail)
-    constexpr int RE = LRING ? kMaxRegEdges : kMaxRegEdges / 2;
+    // (1024-thread launches have 128 registers per lane: 8 register edges)
+    constexpr int RE = WIDE ? 8 : (LRING ? kMaxRegEdges : kMaxRegEdges / 2);
     const int kreg = min((V.max_row + lpn - 1) >> V.lpn_shift, RE);
     int e0 = 0, e1 = 0;
     if (own) { e0 = V.indptr[n]; e1 = V.indptr[n + 1]; }
@@ -810,7 +811,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         };
         // D edge windows in flight: 2 from LDS; 4 from an HBM ring when they fit
         // the register budget (HBM latency is what the gather waits on there)
-        constexpr int D = (!LRING && K * NP <= 16) ? 4 : (K * NP * NV <= 64 ? 2 : 1);
+        constexpr int D = WIDE ? (K * NP <= 16 ? 2 : 1) : ((!LRING && K * NP <= 16) ? 4 : (K * NP * NV <= 64 ? 2 : 1));
         float vbuf[D][K][NP];
         auto load_win = [&](float (&vb)[K][NP], int base) {
 #pragma unroll
@@ -940,24 +941,24 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     __syncthreads();
 }
 
-template <int M, int NP, bool LRING, int K, int CF, bool OUTS>
+template <int M, int NP, bool LRING, int K, int CF, bool OUTS, bool WIDE>
 __device__ __forceinline__ void run_blocked_ic(const DevInst* __restrict__ I, float* lds, int ic) {
-    if (ic == 0) run_blocked<M, NP, LRING, K, 0, CF, OUTS>(I, lds);
-    else if (ic == 1) run_blocked<M, NP, LRING, K, 1, CF, OUTS>(I, lds);
-    else run_blocked<M, NP, LRING, K, 2, CF, OUTS>(I, lds);
+    if (ic == 0) run_blocked<M, NP, LRING, K, 0, CF, OUTS, WIDE>(I, lds);
+    else if (ic == 1) run_blocked<M, NP, LRING, K, 1, CF, OUTS, WIDE>(I, lds);
+    else run_blocked<M, NP, LRING, K, 2, CF, OUTS, WIDE>(I, lds);
 }
 
 // compile-time variants: Linear with and without outputs, legacy sigmoidal
 // (Jansen-Rit) without outputs; anything else generic
-template <int M, int NP, bool LRING, int K>
+template <int M, int NP, bool LRING, int K, bool WIDE = false>
 __device__ __forceinline__ void run_blocked_any(const DevInst* __restrict__ I, float* lds) {
     const int ic = __builtin_amdgcn_readfirstlane(integ_class(I->integrator));
     const int cfn = __builtin_amdgcn_readfirstlane(I->cfn);
     const bool outs = I->traj != nullptr || __builtin_amdgcn_readfirstlane(I->bold_period) > 0;
-    if (cfn == HB_CF_LINEAR && !outs) run_blocked_ic<M, NP, LRING, K, HB_CF_LINEAR, false>(I, lds, ic);
-    else if (cfn == HB_CF_LINEAR) run_blocked_ic<M, NP, LRING, K, HB_CF_LINEAR, true>(I, lds, ic);
-    else if (cfn == HB_CF_SJR && !outs) run_blocked_ic<M, NP, LRING, K, HB_CF_SJR, false>(I, lds, ic);
-    else run_blocked_ic<M, NP, LRING, K, -1, true>(I, lds, ic);
+    if (cfn == HB_CF_LINEAR && !outs) run_blocked_ic<M, NP, LRING, K, HB_CF_LINEAR, false, WIDE>(I, lds, ic);
+    else if (cfn == HB_CF_LINEAR) run_blocked_ic<M, NP, LRING, K, HB_CF_LINEAR, true, WIDE>(I, lds, ic);
+    else if (cfn == HB_CF_SJR && !outs) run_blocked_ic<M, NP, LRING, K, HB_CF_SJR, false, WIDE>(I, lds, ic);
+    else run_blocked_ic<M, NP, LRING, K, -1, true, WIDE>(I, lds, ic);
 }
 
 template <int M, int NP, bool LRING, bool MULTI>
@@ -1016,8 +1017,31 @@ __global__ __launch_bounds__(512) void k_single(LaunchDesc a) {
     }
 }
 
+// One node (times lanes per node) per thread for networks of up to 1024
+// threads: the delay-blocked schedule of networks wider than 512 nodes (and
+// 2 lanes per node at 400 nodes) needs every node of the instance in one
+// workgroup. The launch bound caps registers at 128 per lane, so the gather
+// keeps 8 row edges in registers and 1-2 edge windows in flight.
+// The planner sends only delay-blocked instances with 8- or 16-step blocks
+// here (it checks the block length of every wide launch).
+template <int M, int NP, bool LRING>
+__global__ __launch_bounds__(1024) void k_wide(LaunchDesc a) {
+    extern __shared__ float lds[];
+    for (int it = 0;; ++it) {
+        const int w = next_work(a, it);
+        if (w >= a.n_work) break;
+        const DevInst* I = uniform_inst(a.insts + a.order[w]);
+        const int bk = __builtin_amdgcn_readfirstlane(I->blocked_k);
+        if (bk == kBlockSteps) run_blocked_any<M, NP, LRING, kBlockSteps, true>(I, lds);
+        else if constexpr (!LRING) {
+            if (bk == kLongBlockSteps) run_blocked_any<M, NP, LRING, kLongBlockSteps, true>(I, lds);
+        }
+    }
+}
+
 template <int M, int NP>
-static const void* kptr_single(bool lring, bool multi) {
+static const void* kptr_single(bool lring, bool multi, bool wide) {
+    if (wide) return lring ? (const void*)k_wide<M, NP, true> : (const void*)k_wide<M, NP, false>;
     if (lring) return multi ? (const void*)k_single<M, NP, true, true> : (const void*)k_single<M, NP, true, false>;
     return multi ? (const void*)k_single<M, NP, false, true> : (const void*)k_single<M, NP, false, false>;
 }
@@ -1029,22 +1053,23 @@ static const void* kptr_single(bool lring, bool multi) {
 #endif
 // MPR has a second instantiation with one pre channel: when its second
 // coupling coefficient (cv) is zero the V channel only ever adds c1 * 0.
-static const void* unit_kernel(bool lring, bool multi, int np) {
+static const void* unit_kernel(bool lring, bool multi, int np, bool wide) {
     (void)np;
+    if (wide && multi) return nullptr;   // wide launches hold one node (group) per thread
 #if HB_UNIT == 0
-    return kptr_single<HB_G2DO, 1>(lring, multi);
+    return kptr_single<HB_G2DO, 1>(lring, multi, wide);
 #elif HB_UNIT == 1
-    return np == 1 ? kptr_single<HB_MPR, 1>(lring, multi) : kptr_single<HB_MPR, 2>(lring, multi);
+    return np == 1 ? kptr_single<HB_MPR, 1>(lring, multi, wide) : kptr_single<HB_MPR, 2>(lring, multi, wide);
 #elif HB_UNIT == 2
-    return kptr_single<HB_JANSEN_RIT, 1>(lring, multi);
+    return kptr_single<HB_JANSEN_RIT, 1>(lring, multi, wide);
 #elif HB_UNIT == 3
-    return kptr_single<HB_WILSON_COWAN, 1>(lring, multi);
+    return kptr_single<HB_WILSON_COWAN, 1>(lring, multi, wide);
 #elif HB_UNIT == 4
-    return kptr_single<HB_KURAMOTO, 1>(lring, multi);
+    return kptr_single<HB_KURAMOTO, 1>(lring, multi, wide);
 #elif HB_UNIT == 5
-    return kptr_single<HB_KURAMOTO, 2>(lring, multi);
+    return kptr_single<HB_KURAMOTO, 2>(lring, multi, wide);
 #else
-    return kptr_single<HB_RWW, 1>(lring, multi);
+    return kptr_single<HB_RWW, 1>(lring, multi, wide);
 #endif
 }
 
@@ -1109,15 +1134,17 @@ static hipError_t unit_op(int op, const LaunchDesc* d, hipStream_t s, int arg, i
     if (op == 2) {
         for (int lr = 0; lr < 2; ++lr)
             for (int mu = 0; mu < 2; ++mu)
-                for (int np = 1; np <= 2; ++np) {
-                    const void* k = unit_kernel(lr != 0, mu != 0, np);
-                    if (!k) continue;
-                    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, arg);
-                    if (e != hipSuccess) return e;
-                }
+                for (int np = 1; np <= 2; ++np)
+                    for (int wd = 0; wd < 2; ++wd) {
+                        const void* k = unit_kernel(lr != 0, mu != 0, np, wd != 0);
+                        if (!k) continue;
+                        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, arg);
+                        if (e != hipSuccess) return e;
+                    }
         return hipSuccess;
     }
-    const void* k = unit_kernel(d->lds_ring, d->multi, d->np);
+    if (d->block > kWideBlockMax || (d->multi && d->block > kBlockMax)) return hipErrorInvalidValue;
+    const void* k = unit_kernel(d->lds_ring, d->multi, d->np, d->block > kBlockMax);
     if (!k) return hipErrorInvalidValue;
     if (op == 1) return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k, d->block, (size_t)d->lds_bytes);
     LaunchDesc a = *d;

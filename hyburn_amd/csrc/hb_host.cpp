@@ -295,7 +295,12 @@ int autotune_plan(const hb_instance* inst, int n, const hb_run_opts* opts, int l
         const bool per_edge = I.wkind == HB_W_CSR && I.delay_mode == HB_DELAY_PER_EDGE && I.edge_delays;
         TuneKey key;
         std::vector<int> cand;
-        if (I.n_nodes > 512) {
+        int dmin = 1 << 30;
+        if (per_edge)
+            for (int e = 0; e < I.nnz; ++e) dmin = std::min(dmin, (int)I.edge_delays[e]);
+        const bool blockable = per_edge && dmin >= 8 && !(opts->flags & HB_FLAG_NO_BLOCKING) &&
+                               (int)align_up((size_t)I.n_nodes, 64) <= kWideBlockMax;
+        if (I.n_nodes > kBlockMax && !blockable) {
             key = tune_key(I, np, 0, 0);
             cand = {256, 512};
         } else if (I.wkind == HB_W_DENSE && max_delay_of(I) == 0 && !(opts->flags & HB_FLAG_EXACT) &&
@@ -303,14 +308,11 @@ int autotune_plan(const hb_instance* inst, int n, const hb_run_opts* opts, int l
                    I.n_nodes <= kDenseMaxNodes) {
             key = tune_key(I, np, 2, 0);
             cand = {kChoiceLanes, kChoiceMfma};
-        } else if (per_edge && !(opts->flags & HB_FLAG_EXACT) && !(opts->flags & HB_FLAG_NO_BLOCKING)) {
-            int dmin = 1 << 30;
-            for (int e = 0; e < I.nnz; ++e) dmin = std::min(dmin, (int)I.edge_delays[e]);
-            if (dmin < 8) continue;
+        } else if (blockable && !(opts->flags & HB_FLAG_EXACT)) {
             const int64_t ring_k = (int64_t)(max_delay_of(I) + 16) * I.n_nodes * np * 4;
             key = tune_key(I, np, 1, ring_k <= lds_budget);
             for (int l : {1, 2, 4})
-                if ((int)align_up((size_t)I.n_nodes * l, 64) <= 512) cand.push_back(l);
+                if ((int)align_up((size_t)I.n_nodes * l, 64) <= kWideBlockMax) cand.push_back(l);
         } else {
             continue;
         }
@@ -484,14 +486,16 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
         s.np = ring_channels(I.model, I.cfn, I.params, I.node_param_idx, -1);
         s.depth = max_delay_of(I) + 2;
         int b = req_block > 0 ? req_block : (int)align_up((size_t)I.n_nodes, 64);
-        b = std::max(64, std::min(512, b));
+        b = std::max(64, std::min(kBlockMax, b));
         s.block = b;
         s.multi = I.n_nodes > b;
-        const int tab_b = s.multi ? I.n_nodes * (2 * s.nv + 6) * 4 : 0;
-        // delay-blocked schedule: per-edge delays with min delay >= K, one node per thread
+        // delay-blocked schedule: per-edge delays with min delay >= K, one node
+        // (or lpn lanes) per thread; networks wider than 512 nodes take the
+        // 1024-thread kernel
         s.blocked = 0;
         s.lpn = 1;
-        if (!s.multi && I.wkind == HB_W_CSR && I.delay_mode == HB_DELAY_PER_EDGE && I.edge_delays &&
+        const bool fits_wide = req_block <= 0 && (int)align_up((size_t)I.n_nodes, 64) <= kWideBlockMax;
+        if ((!s.multi || fits_wide) && I.wkind == HB_W_CSR && I.delay_mode == HB_DELAY_PER_EDGE && I.edge_delays &&
             !(opts && (opts->flags & HB_FLAG_NO_BLOCKING))) {
             int dmin = 1 << 30;
             for (int e = 0; e < I.nnz; ++e) dmin = std::min(dmin, (int)I.edge_delays[e]);
@@ -511,7 +515,8 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
                     s.blocked = K;
                     s.depth = dmax + K + 1;
                     // lanes per node (FMA mode only: exact mode keeps the row's sequential sum):
-                    // forced by the tuner's trials, tuned, or the widest that fits 512 threads
+                    // forced by the tuner's trials or tuned (up to 1024 threads), else the
+                    // widest that fits 512 threads
                     if (!exact && req_block <= 0) {
                         int want = 0;
                         if (g_force_choice > 0) want = g_force_choice;
@@ -520,10 +525,17 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
                             const auto it = g_tuned.find(tune_key(I, s.np, 1, ring_k <= lds_budget));
                             if (it != g_tuned.end()) want = it->second.choice;
                         }
+                        const int cap = want > 0 ? kWideBlockMax : kBlockMax;
                         for (int l : {4, 2})
-                            if ((want == 0 || l <= want) && (int)align_up((size_t)I.n_nodes * l, 64) <= 512) { s.lpn = l; break; }
+                            if ((want == 0 || l <= want) && (int)align_up((size_t)I.n_nodes * l, 64) <= cap) { s.lpn = l; break; }
                     }
                     s.block = std::max(64, (int)align_up((size_t)I.n_nodes * s.lpn, 64));
+                    s.multi = false;
+                    // the 1024-thread kernel has 8- and 16-step blocks only
+                    if (s.block > kBlockMax && K == 32) {
+                        s.blocked = 16;
+                        s.depth = dmax + 16 + 1;
+                    }
                     break;
                 }
             }
@@ -541,6 +553,7 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
                 s.multi = I.n_nodes > s.block;
             }
         }
+        const int tab_b = s.multi ? I.n_nodes * (2 * s.nv + 6) * 4 : 0;
         // ring floats: depth slots per node and channel (+ K-1 mirrored slots when blocked)
         const int ring_b = (s.depth + (s.blocked ? s.blocked - 1 : 0)) * I.n_nodes * s.np * 4;
         s.lds_ring = (int64_t)ring_b + tab_b <= lds_budget;
@@ -851,6 +864,16 @@ int hb_plan_create(const hb_instance* inst, int32_t n, const hb_run_opts* opts, 
     for (auto& kv : dense_groups)
         if ((int)kv.second.size() < kDenseMinBatch)
             for (int i : kv.second) sh[i].mfma = false;
+    // a launch wider than 512 threads runs the 1024-thread kernel, which has
+    // only the delay-blocked schedule with 8-step (any ring) or 16-step (HBM
+    // ring) blocks
+    for (int i = 0; i < n; ++i)
+        if (!sh[i].mfma && sh[i].block > kBlockMax &&
+            !(sh[i].blocked == 8 || (sh[i].blocked == 16 && !sh[i].lds_ring))) {
+            hb_plan_destroy(P);
+            return fail(HB_BACKEND_ERROR, "instance " + std::to_string(i) + ": " + std::to_string(sh[i].block) +
+                                              "-thread launch without a delay-blocked schedule");
+        }
     std::map<std::tuple<int, int, int, int, int>, std::vector<int>> groups;
     for (int i = 0; i < n; ++i)
         if (!sh[i].mfma)

@@ -22,6 +22,9 @@ namespace hb {
 constexpr int kMaxNV = 6;
 constexpr int kMC = 28;   // model constant slots
 constexpr int kCC = 12;   // coupling constant slots
+// Threads per forward workgroup: k_single takes up to 512; k_wide (the
+// one-node-per-thread schedules, delay-blocked ones among them) up to 1024.
+constexpr int kBlockMax = 512, kWideBlockMax = 1024;
 
 HB_HD int model_nvar(int m) {
     return m == HB_JANSEN_RIT ? 6 : (m == HB_KURAMOTO || m == HB_RWW || m == HB_LINEAR) ? 1 : 2;

@@ -220,6 +220,43 @@ int hb_orc_coupling_f32(const hb_orc_coupling* cpl, int n_tgt, int n_src, int nc
     return 0;
 }
 
+/* The FP64 twin of hb_orc_coupling_f32: the same pre/post order in double,
+ * weights widened from f32. */
+int hb_orc_coupling_f64(const hb_orc_coupling* cpl, int n_tgt, int n_src, int nc,
+                        const double* x_src, const double* x_tgt, double* out)
+{
+    f64_cconst k;
+    double pre[8];
+    double* P;
+    double g[16];
+    int i, j, c, e, np;
+    f64_cconst_init(&k, cpl);
+    np = f64_npre(&k, nc);
+    P = (double*)calloc((size_t)n_src * np, sizeof(double));
+    if (!P) return -1;
+    for (j = 0; j < n_src; ++j) { f64_pre(&k, x_src + (size_t)j * nc, nc, pre); memcpy(P + (size_t)j * np, pre, sizeof(double) * np); }
+    for (i = 0; i < n_tgt; ++i) {
+        double rowsum = 0;
+        for (c = 0; c < np; ++c) g[c] = 0;
+        if (cpl->wkind == ORC_W_DENSE) {
+            for (j = 0; j < n_src; ++j) {
+                double w = cpl->dense_w[(size_t)i * n_src + j];
+                for (c = 0; c < np; ++c) g[c] += w * P[(size_t)j * np + c];
+                rowsum += w;
+            }
+        } else if (cpl->wkind == ORC_W_CSR) {
+            for (e = cpl->indptr[i]; e < cpl->indptr[i + 1]; ++e) {
+                double w = cpl->weights[e];
+                for (c = 0; c < np; ++c) g[c] += w * P[(size_t)cpl->indices[e] * np + c];
+                rowsum += w;
+            }
+        }
+        f64_post(&k, g, x_tgt + (size_t)i * nc, rowsum, nc, out + (size_t)i * nc);
+    }
+    free(P);
+    return 0;
+}
+
 /* pre() of n rows of nc cvars -> out [n][npre]; returns npre (coupling.rs:204-321).
  * cpl->cpu_form selects the per-edge pre_cpu op order. */
 int hb_orc_pre_f32(const hb_orc_coupling* cpl, int n, int nc, const float* x, float* out)
@@ -271,6 +308,12 @@ void hb_orc_bold_signal_f32(int n, const float* v, const float* q, float* out)
 {
     int i;
     for (i = 0; i < n; ++i) out[i] = f32_bold_signal1(v[i], q[i]);
+}
+
+void hb_orc_bold_signal_f64(int n, const double* v, const double* q, double* out)
+{
+    int i;
+    for (i = 0; i < n; ++i) out[i] = f64_bold_signal1(v[i], q[i]);
 }
 
 /* Downsample native BOLD samples to TR, bold_monitor.rs:123-203.

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libhb_oracle.so")
 
 fp = C.POINTER(C.c_float)
 ip = C.POINTER(C.c_int32)
+dp = C.POINTER(C.c_double)
 
 
 class OrcCoupling(C.Structure):
@@ -60,7 +61,9 @@ def lib() -> C.CDLL:
         L.hb_orc_dfun_f64.argtypes = [C.c_int, fp, C.c_int, C.c_float, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.hb_orc_clamp_f32.argtypes = [C.c_int, fp]
+        L.hb_orc_clamp_f64.argtypes = [C.c_int, dp]
         L.hb_orc_coupling_f32.argtypes = [C.POINTER(OrcCoupling), C.c_int, C.c_int, C.c_int, fp, fp, fp]
+        L.hb_orc_coupling_f64.argtypes = [C.POINTER(OrcCoupling), C.c_int, C.c_int, C.c_int, dp, dp, dp]
         L.hb_orc_csr_matvec_f32.argtypes = [fp, ip, ip, C.c_int, fp, fp]
         L.hb_orc_pre_f32.argtypes = [C.POINTER(OrcCoupling), C.c_int, C.c_int, fp, fp]
         L.hb_orc_pre_f32.restype = C.c_int
@@ -68,6 +71,7 @@ def lib() -> C.CDLL:
         L.hb_orc_bold_step_f32.argtypes = [C.c_int, fp, C.c_float, fp, fp, fp, fp]
         L.hb_orc_bold_step_f64.argtypes = [C.c_int] + [C.POINTER(C.c_double)] + [C.c_double] + [C.POINTER(C.c_double)] * 4
         L.hb_orc_bold_signal_f32.argtypes = [C.c_int, fp, fp, fp]
+        L.hb_orc_bold_signal_f64.argtypes = [C.c_int, dp, dp, dp]
         L.hb_orc_bold_downsample.argtypes = [C.POINTER(C.c_double), fp, C.c_int, C.c_int, C.c_double,
                                              C.c_double, fp, C.c_int]
         L.hb_orc_fc.argtypes = [fp, C.c_int, C.c_int, fp]
@@ -261,9 +265,12 @@ def dfun(model, params, x, c, sweep_form=0, sweep_val=0.0, dtype=np.float32):
     return dx
 
 
-def clamp(model, x):
-    x = np.array(x, np.float32, copy=True)
-    lib().hb_orc_clamp_f32(model, _p(x))
+def clamp(model, x, dtype=np.float32):
+    x = np.array(x, dtype, copy=True)
+    if dtype == np.float32:
+        lib().hb_orc_clamp_f32(model, _p(x))
+    else:
+        lib().hb_orc_clamp_f64(model, _p(x, dp))
     return x
 
 
@@ -298,13 +305,14 @@ def post(cfn, cparams, g, x_tgt=None, rowsum=0.0, n_src=1):
     return out
 
 
-def coupling(cfn, cparams, x_src, x_tgt=None, dense=None, csr=None, n_src=None):
-    """One delay-free coupling evaluation post(W @ pre(x_src), x_tgt)."""
+def coupling(cfn, cparams, x_src, x_tgt=None, dense=None, csr=None, n_src=None, dtype=np.float32):
+    """One delay-free coupling evaluation post(W @ pre(x_src), x_tgt) (FP32, or
+    the FP64 twin with dtype float64; weights stay f32 values)."""
     from hyburn_amd import _lib as HL
-    x_src = np.ascontiguousarray(x_src, np.float32)
+    x_src = np.ascontiguousarray(x_src, dtype)
     ns, nc = x_src.shape
     nt = dense.shape[0] if dense is not None else len(csr[0]) - 1
-    x_tgt = np.zeros((nt, nc), np.float32) if x_tgt is None else np.ascontiguousarray(x_tgt, np.float32)
+    x_tgt = np.zeros((nt, nc), dtype) if x_tgt is None else np.ascontiguousarray(x_tgt, dtype)
     cpl = OrcCoupling()
     cpl.cfn = cfn
     cpl.n_cparams = len(cparams)
@@ -321,8 +329,11 @@ def coupling(cfn, cparams, x_src, x_tgt=None, dense=None, csr=None, n_src=None):
                          np.ascontiguousarray(csr[2], np.float32))
         keep += [ipt, ind, dat]
         cpl.wkind, cpl.indptr, cpl.indices, cpl.weights = HL.W_CSR, _p(ipt, ip), _p(ind, ip), _p(dat)
-    out = np.zeros((nt, nc), np.float32)
-    lib().hb_orc_coupling_f32(C.byref(cpl), nt, ns, nc, _p(x_src), _p(x_tgt), _p(out))
+    out = np.zeros((nt, nc), dtype)
+    if dtype == np.float32:
+        lib().hb_orc_coupling_f32(C.byref(cpl), nt, ns, nc, _p(x_src), _p(x_tgt), _p(out))
+    else:
+        lib().hb_orc_coupling_f64(C.byref(cpl), nt, ns, nc, _p(x_src, dp), _p(x_tgt, dp), _p(out, dp))
     return out
 
 
@@ -377,40 +388,49 @@ def stimulus_value(pattern, params, step, dt_ms):
     raise ValueError(pattern)
 
 
-def bold_series(v0, P, dt_ms, tr_ms):
+def bold_series(v0, P, dt_ms, tr_ms, dtype=np.float32):
     """BOLD of a [T][N] series of variable 0 (bold_monitor.rs:72-227): windows
     of P steps averaged in f32, one Balloon-Windkessel step per window
     (dt_s = window ms / 1000), a partial last window, then TR resampling
-    unless TR is <= 0 or the window length."""
+    unless TR is <= 0 or the window length. dtype float64: the FP64 twin of
+    the windows and the Balloon-Windkessel steps (the TR resampling of its
+    native rows stays the f32 interpolation)."""
     from types import SimpleNamespace
-    v0 = np.asarray(v0, np.float32)
+    fl = dtype
+    v0 = np.asarray(v0, fl)
     T, N = v0.shape
-    S, F, V, Q = (np.zeros(N, np.float32), np.ones(N, np.float32), np.ones(N, np.float32), np.ones(N, np.float32))
-    acc, cnt, native = np.zeros(N, np.float32), 0, []
+    S, F, V, Q = (np.zeros(N, fl), np.ones(N, fl), np.ones(N, fl), np.ones(N, fl))
+    acc, cnt, native = np.zeros(N, fl), 0, []
 
     def close(cnt):
-        z = (acc / np.float32(cnt)).astype(np.float32)
-        lib().hb_orc_bold_step_f32(N, _p(z), np.float32((dt_ms * cnt) / 1000.0), _p(S), _p(F), _p(V), _p(Q))
-        sig = np.zeros(N, np.float32)
-        lib().hb_orc_bold_signal_f32(N, _p(V), _p(Q), _p(sig))
+        z = (acc / fl(cnt)).astype(fl)
+        sig = np.zeros(N, fl)
+        if fl is np.float32:
+            lib().hb_orc_bold_step_f32(N, _p(z), np.float32((dt_ms * cnt) / 1000.0), _p(S), _p(F), _p(V), _p(Q))
+            lib().hb_orc_bold_signal_f32(N, _p(V), _p(Q), _p(sig))
+        else:
+            lib().hb_orc_bold_step_f64(N, _p(z, dp), (dt_ms * cnt) / 1000.0, _p(S, dp), _p(F, dp),
+                                       _p(V, dp), _p(Q, dp))
+            lib().hb_orc_bold_signal_f64(N, _p(V, dp), _p(Q, dp), _p(sig, dp))
         native.append(sig)
 
     for t in range(T):
-        acc = (acc + v0[t]).astype(np.float32)
+        acc = (acc + v0[t]).astype(fl)
         cnt += 1
         if cnt >= P:
             close(cnt)
-            acc, cnt = np.zeros(N, np.float32), 0
+            acc, cnt = np.zeros(N, fl), 0
     if cnt > 0:
         close(cnt)
-    nat = np.array(native, np.float32).reshape(-1, N)
+    nat = np.array(native, fl).reshape(-1, N)
     if tr_ms <= 0.0 or abs(tr_ms - dt_ms * P) < 1e-6:
         return nat
-    return bold_resample(nat, SimpleNamespace(dt=dt_ms, dt_ms=dt_ms, bold_period=P, n_steps=T, bold_tr_ms=tr_ms))
+    return bold_resample(nat, SimpleNamespace(dt=dt_ms, dt_ms=dt_ms, bold_period=P, n_steps=T,
+                                              bold_tr_ms=tr_ms)).astype(fl)
 
 
 def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_sweep=1, max_delay=0, nsig=None,
-              noise=None, plain=False, stimuli=(), dt_ms=None, bold_period=0, bold_tr_ms=0.0):
+              noise=None, plain=False, stimuli=(), dt_ms=None, bold_period=0, bold_tr_ms=0.0, dtype=np.float32):
     """The batch engine's multi-subnetwork loop (engine.rs:540-823) in f32, one
     sweep point at a time: coupling of every projection from the current
     states (engine.rs:557-673), then each subnetwork's integration with that
@@ -431,6 +451,10 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
     [n_sweep][n][nvar], plus the BOLD [n_sweep][rows][n0] when monitored."""
     from hyburn_amd import _lib as HL
     f32 = np.float32
+    fl = dtype   # float64: the FP64 twin (same op order; weights, params and raw noise stay f32 values)
+
+    def _sum(v):
+        return _f32sum(v) if fl is np.float32 else np.float64(np.sum(np.asarray(v, np.float64)))
     S = len(subnets)
     info = [HL.MODEL_INFO[m] for (m, _, _, _) in subnets]
     # Difference preprocessing, engine.rs:325-366
@@ -460,13 +484,14 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                     W[i, i] = f32(W[i, i] - _f32sum(W[i]))
                 cfn, cp = HL.CF_LINEAR, [a, 0.0]
         prep.append((src, tgt, w, W, cfn, cp, list(cmap), delay))
-    tav = [np.zeros((n_sweep, n, info[k][0]), np.float32) for k, (_, n, _, _) in enumerate(subnets)]
+    tav = [np.zeros((n_sweep, n, info[k][0]), fl) for k, (_, n, _, _) in enumerate(subnets)]
     fin = [np.zeros_like(t) for t in tav]
-    dtf, dth, dt6 = f32(dt), f32(f32(dt) * f32(0.5)), f32(f32(dt) / f32(6.0))
+    dtf = fl(np.float32(dt))   # the engine's f32 dt (engine.rs:456), widened in the FP64 twin
+    dth, dt6 = fl(dtf * fl(0.5)), fl(dtf / fl(6.0))
     bold_out = []
     for b in range(n_sweep):
         v0 = []
-        x = [np.array(x0, np.float32).reshape(n, info[k][0]) for k, (_, n, _, x0) in enumerate(subnets)]
+        x = [np.array(x0, fl).reshape(n, info[k][0]) for k, (_, n, _, x0) in enumerate(subnets)]
         acc = [np.zeros_like(v) for v in x]
         sw = [0] * S
         sv = 0.0
@@ -489,10 +514,10 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                 ns, nt = subnets[src][1], subnets[tgt][1]
                 if isinstance(w, tuple):   # runtime.rs:160-203, identity pre: sum of w_e * x_j(step - d_e)
                     ipt, ind, dat, edl = w
-                    xt = x[tgt][:, :ncs] if nct == ncs else np.zeros((nt, ncs), np.float32)
-                    out = np.zeros((nt, ncs), np.float32)
+                    xt = x[tgt][:, :ncs] if nct == ncs else np.zeros((nt, ncs), fl)
+                    out = np.zeros((nt, ncs), fl)
                     for r_ in range(nt):
-                        g = np.zeros(ncs, np.float32)
+                        g = np.zeros(ncs, fl)
                         for e_ in range(ipt[r_], ipt[r_ + 1]):
                             d_ = int(edl[e_])
                             if d_ == 0 or step == 0:
@@ -500,10 +525,10 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                             elif d_ <= step:
                                 xv = past[src][step - d_][ind[e_], :ncs]
                             else:
-                                xv = np.zeros(ncs, np.float32)
-                            g = (g + np.float32(dat[e_]) * xv).astype(np.float32)
-                        out[r_] = coupling(cfn, cp, g[None, :], xt[r_:r_ + 1], dense=np.ones((1, 1), np.float32),
-                                           n_src=ns)[0]
+                                xv = np.zeros(ncs, fl)
+                            g = (g + fl(dat[e_]) * xv).astype(fl)
+                        out[r_] = coupling(cfn, cp, g[None, :], xt[r_:r_ + 1], dense=np.ones((1, 1), fl),
+                                           n_src=ns, dtype=fl)[0]
                 elif delay == 0 or step == 0:
                     xs = x[src][:, :ncs]
                 elif delay <= step and plain:
@@ -511,89 +536,89 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                 elif delay <= step:
                     xs = hist[src][(step - delay + H) % H][:, :ncs]
                 else:
-                    xs = np.zeros((ns, ncs), np.float32)
-                xt = x[tgt][:, :ncs] if nct == ncs else np.zeros((nt, ncs), np.float32)
+                    xs = np.zeros((ns, ncs), fl)
+                xt = x[tgt][:, :ncs] if nct == ncs else np.zeros((nt, ncs), fl)
                 if isinstance(w, tuple):
                     pass
                 elif W is None:
                     if cfn not in _IDENTITY_PRE:
                         raise NotImplementedError("Scalar weights with a non-identity pre")
-                    g = np.array([[f32(f32(_f32sum(xs[:, c]) / f32(ns)) * f32(w)) for c in range(ncs)]], np.float32)
-                    out = coupling(cfn, cp, g, xt, dense=np.ones((nt, 1), np.float32), n_src=ns)
+                    g = np.array([[fl(fl(_sum(xs[:, c]) / fl(ns)) * fl(np.float32(w))) for c in range(ncs)]], fl)
+                    out = coupling(cfn, cp, g, xt, dense=np.ones((nt, 1), fl), n_src=ns, dtype=fl)
                 else:
-                    out = coupling(cfn, cp, xs, xt, dense=W, n_src=ns)
+                    out = coupling(cfn, cp, xs, xt, dense=W, n_src=ns, dtype=fl)
                 if nct == ncs and cmap == [(0, 0)]:
                     rem = out
                 else:
-                    rem = np.zeros((nt, nct), np.float32)
+                    rem = np.zeros((nt, nct), fl)
                     for (sc, tc) in cmap:
                         ci = _CVAR[mt].index(tc) if tc in _CVAR[mt] else 0
                         if sc < ncs and ci < nct:
                             rem[:, ci] = rem[:, ci] + out[:, sc]
-                cpl[tgt] = rem if cpl[tgt] is None else (cpl[tgt] + rem).astype(np.float32)
+                cpl[tgt] = rem if cpl[tgt] is None else (cpl[tgt] + rem).astype(fl)
             for (target, pattern, sp) in stimuli:
                 if 0 <= target < S:
                     v = stimulus_value(pattern, sp, step, dt if dt_ms is None else dt_ms)
                     if v != 0.0:
                         base = cpl[target] if cpl[target] is not None else np.zeros((subnets[target][1], info[target][1]),
-                                                                                     np.float32)
+                                                                                     fl)
                         base = base.copy()
-                        base[:, 0] = (base[:, 0] + v).astype(np.float32)
+                        base[:, 0] = (base[:, 0] + v).astype(fl)
                         cpl[target] = base
             for k, (m, n, params, _) in enumerate(subnets):
-                c = cpl[k] if cpl[k] is not None else np.zeros((n, info[k][1]), np.float32)
+                c = cpl[k] if cpl[k] is not None else np.zeros((n, info[k][1]), fl)
                 heun = (m == HL.G2DO) if hybrid else (integrator == HL.HEUN)
                 stoch = (not (hybrid and m == HL.G2DO)) and integrator in (HL.EULER_STOCH, HL.HEUN_STOCH,
                                                                            HL.RK4_STOCH)
                 if stoch:
-                    scale = np.array([f32(np.sqrt(f32(2.0) * f32(nsig[v]))) * f32(np.sqrt(dtf))
-                                      for v in range(info[k][0])], np.float32)
-                    Zk = (np.asarray(noise[k][step], np.float32) * scale).astype(np.float32)
+                    scale = np.array([fl(np.sqrt(fl(2.0) * fl(nsig[v]))) * fl(np.sqrt(dtf))
+                                      for v in range(info[k][0])], fl)
+                    Zk = (np.asarray(noise[k][step], fl) * scale).astype(fl)
                 for i in range(n):
                     xi, ci = x[k][i].copy(), c[i]
-                    d = dfun(m, params, xi, ci, sw[k], sv)
+                    d = dfun(m, params, xi, ci, sw[k], sv, dtype=fl)
                     if stoch:
                         z = Zk[i]
                         if integrator == HL.EULER_STOCH:
-                            xn = ((xi + d * dtf) + z).astype(np.float32)
+                            xn = ((xi + d * dtf) + z).astype(fl)
                         elif integrator == HL.HEUN_STOCH:
-                            pr = clamp(m, ((xi + d * dtf) + z).astype(np.float32))
-                            d2 = dfun(m, params, pr, ci, sw[k], sv)
-                            xn = ((xi + (d + d2) * dth) + z).astype(np.float32)
+                            pr = clamp(m, ((xi + d * dtf) + z).astype(fl), fl)
+                            d2 = dfun(m, params, pr, ci, sw[k], sv, dtype=fl)
+                            xn = ((xi + (d + d2) * dth) + z).astype(fl)
                         else:
-                            k2 = dfun(m, params, clamp(m, xi + d * dth), ci, sw[k], sv)
-                            k3 = dfun(m, params, clamp(m, xi + k2 * dth), ci, sw[k], sv)
-                            k4 = dfun(m, params, clamp(m, xi + k3 * dtf), ci, sw[k], sv)
-                            det = (xi + (((d + k2 * f32(2.0)) + k3 * f32(2.0)) + k4) * dt6).astype(np.float32)
+                            k2 = dfun(m, params, clamp(m, xi + d * dth, fl), ci, sw[k], sv, dtype=fl)
+                            k3 = dfun(m, params, clamp(m, xi + k2 * dth, fl), ci, sw[k], sv, dtype=fl)
+                            k4 = dfun(m, params, clamp(m, xi + k3 * dtf, fl), ci, sw[k], sv, dtype=fl)
+                            det = (xi + (((d + k2 * fl(2.0)) + k3 * fl(2.0)) + k4) * dt6).astype(fl)
                             if plain:   # integrator.rs:296-315: clamp(det) + Z, no clamp after
-                                x[k][i] = (clamp(m, det) + z).astype(np.float32)
+                                x[k][i] = (clamp(m, det, fl) + z).astype(fl)
                                 continue
-                            xn = (det + z).astype(np.float32)
-                        x[k][i] = clamp(m, xn)
+                            xn = (det + z).astype(fl)
+                        x[k][i] = clamp(m, xn, fl)
                         continue
                     if heun:
-                        pr = clamp(m, (xi + d * dtf).astype(np.float32))
-                        d2 = dfun(m, params, pr, ci, sw[k], sv)
-                        xn = (xi + (d + d2) * dth).astype(np.float32)
+                        pr = clamp(m, (xi + d * dtf).astype(fl), fl)
+                        d2 = dfun(m, params, pr, ci, sw[k], sv, dtype=fl)
+                        xn = (xi + (d + d2) * dth).astype(fl)
                     elif integrator == HL.RK4:
-                        k2 = dfun(m, params, clamp(m, xi + d * dth), ci, sw[k], sv)
-                        k3 = dfun(m, params, clamp(m, xi + k2 * dth), ci, sw[k], sv)
-                        k4 = dfun(m, params, clamp(m, xi + k3 * dtf), ci, sw[k], sv)
-                        xn = (xi + (((d + k2 * f32(2.0)) + k3 * f32(2.0)) + k4) * dt6).astype(np.float32)
+                        k2 = dfun(m, params, clamp(m, xi + d * dth, fl), ci, sw[k], sv, dtype=fl)
+                        k3 = dfun(m, params, clamp(m, xi + k2 * dth, fl), ci, sw[k], sv, dtype=fl)
+                        k4 = dfun(m, params, clamp(m, xi + k3 * dtf, fl), ci, sw[k], sv, dtype=fl)
+                        xn = (xi + (((d + k2 * fl(2.0)) + k3 * fl(2.0)) + k4) * dt6).astype(fl)
                     else:   # Euler (and a configured Heun under hybrid_integrator)
-                        xn = (xi + d * dtf).astype(np.float32)
-                    x[k][i] = clamp(m, xn)
-                acc[k] = (acc[k] + x[k]).astype(np.float32)
+                        xn = (xi + d * dtf).astype(fl)
+                    x[k][i] = clamp(m, xn, fl)
+                acc[k] = (acc[k] + x[k]).astype(fl)
             v0.append(x[0][:, 0].copy())
         if bold_period > 0:
-            bold_out.append(bold_series(np.array(v0, np.float32).reshape(n_steps, subnets[0][1]), bold_period,
-                                        dt if dt_ms is None else dt_ms, bold_tr_ms))
-        inv = f32(1.0) / f32(n_steps)
+            bold_out.append(bold_series(np.array(v0, fl).reshape(n_steps, subnets[0][1]), bold_period,
+                                        dt if dt_ms is None else dt_ms, bold_tr_ms, dtype=fl))
+        inv = fl(1.0) / fl(n_steps)
         for k in range(S):
             tav[k][b] = acc[k] * inv
             fin[k][b] = x[k]
     if bold_period > 0:
-        return tav, fin, np.array(bold_out, np.float32)
+        return tav, fin, np.array(bold_out, fl)
     return tav, fin
 
 

@@ -4,6 +4,10 @@ against the oracle's restatement (oracle.hb_oracle.run_multi).
 The kernel is built with FMA contraction and the device's expf/sinf, so it
 is compared to the f32 oracle within rtol 2e-4, atol 2e-5 over short runs
 (100-200 steps), and within rtol 2e-3 over the 1000-step ring workload.
+
+FP64 bar: the oracle's FP64 twin (run_multi(..., dtype=float64), the same op
+order in double) gives each result's error; the GPU's max abs error against
+it must be at most 4x the f32 oracle's own, + 1e-6 (tavg, final states, BOLD).
 """
 import numpy as np
 import pytest
@@ -21,10 +25,25 @@ def _x0(rng, n, nv, scale=0.3):
     return (rng.normal(size=(n, nv)) * scale).astype(np.float32)
 
 
-def _check(res, tav, fin, rtol=RTOL, atol=ATOL):
+def _bar(g, o32, o64, atol=1e-6):
+    """GPU error against the FP64 twin <= 4x the f32 oracle's own + atol."""
+    e_gpu = float(np.max(np.abs(np.asarray(g, np.float64) - o64))) if np.size(o64) else 0.0
+    e_orc = float(np.max(np.abs(np.asarray(o32, np.float64) - o64))) if np.size(o64) else 0.0
+    assert e_gpu <= 4 * e_orc + atol, (e_gpu, e_orc)
+    return e_gpu, e_orc
+
+
+def _check(res, tav, fin, rtol=RTOL, atol=ATOL, o64=None, bold=None):
     for k in range(len(tav)):
+        assert np.all(np.isfinite(fin[k])) and np.all(np.isfinite(tav[k])), f"subnetwork {k} leaves f32 range"
         np.testing.assert_allclose(res.tavg[k], tav[k], rtol=rtol, atol=atol)
         np.testing.assert_allclose(res.final_states[k], fin[k], rtol=rtol, atol=atol)
+    if o64 is not None:
+        for k in range(len(tav)):
+            _bar(res.tavg[k], tav[k], o64[0][k])
+            _bar(res.final_states[k], fin[k], o64[1][k])
+        if bold is not None:
+            _bar(res.bold, bold, o64[2])
 
 
 def _ring(n, w=0.01):
@@ -48,7 +67,8 @@ def test_three_subnet_ring_hybrid_sweep(gpu, orc):
     res = _run(subs, links, dt=0.1, n_steps=200, integrator=L.HEUN, hybrid_integrator=True, n_sweep=3,
                sweep=(0, 1, vals))
     tav, fin = orc.run_multi(subs, links, 0.1, 200, L.HEUN, True, sweep=(0, 1, vals), n_sweep=3)
-    _check(res, tav, fin)
+    o64 = orc.run_multi(subs, links, 0.1, 200, L.HEUN, True, sweep=(0, 1, vals), n_sweep=3, dtype=np.float64)
+    _check(res, tav, fin, o64=o64)
     assert not np.allclose(res.final_states[0][0], res.final_states[0][2])   # the sweep acts
 
 
@@ -63,7 +83,7 @@ def test_dense_links_and_cvar_remap(gpu, orc, integ, hybrid):
              (1, 1, 0.05, L.CF_SCALED_LINEAR, [0.2, 0.1], [(0, 0)])]
     res = _run(subs, links, dt=0.05, n_steps=150, integrator=integ, hybrid_integrator=hybrid)
     tav, fin = orc.run_multi(subs, links, 0.05, 150, integ, hybrid)
-    _check(res, tav, fin)
+    _check(res, tav, fin, o64=orc.run_multi(subs, links, 0.05, 150, integ, hybrid, dtype=np.float64))
 
 
 def test_difference_kuramoto_sigmoidal(gpu, orc):
@@ -80,7 +100,7 @@ def test_difference_kuramoto_sigmoidal(gpu, orc):
              (1, 1, 0.2, L.CF_DIFFERENCE, [0.4], [(0, 0)])]          # Scalar Difference: zero coupling
     res = _run(subs, links, dt=0.1, n_steps=120, integrator=L.HEUN)
     tav, fin = orc.run_multi(subs, links, 0.1, 120, L.HEUN, False)
-    _check(res, tav, fin)
+    _check(res, tav, fin, o64=orc.run_multi(subs, links, 0.1, 120, L.HEUN, False, dtype=np.float64))
 
 
 def test_ring_workload_full_length(gpu, orc):
@@ -97,6 +117,12 @@ def test_ring_workload_full_length(gpu, orc):
         for k in range(3):
             np.testing.assert_allclose(res.tavg[k][b], tav[k][0], rtol=2e-3, atol=2e-4)
             np.testing.assert_allclose(res.final_states[k][b], fin[k][0], rtol=2e-3, atol=2e-4)
+        if b == 0:
+            t64, f64 = orc.run_multi(subs, links, 0.1, 1000, L.HEUN, True, sweep=(0, 1, vals[:1]), n_sweep=1,
+                                     dtype=np.float64)
+            for k in range(3):
+                _bar(res.tavg[k][0], tav[k][0], t64[k][0])
+                _bar(res.final_states[k][0], fin[k][0], f64[k][0])
     assert res.kernel_ms > 0.0
 
 
@@ -138,7 +164,8 @@ def test_delayed_projections(gpu, orc):
                         [Link(s, t, w, cfn, cp, cm, dl) for (s, t, w, cfn, cp, cm, dl) in links],
                         dt=0.1, n_steps=150, integrator=L.RK4, n_sweep=2, max_delay=md)
         tav, fin = orc.run_multi(subs, links, 0.1, 150, L.RK4, False, n_sweep=2, max_delay=md)
-        _check(res, tav, fin)
+        _check(res, tav, fin, o64=orc.run_multi(subs, links, 0.1, 150, L.RK4, False, n_sweep=2, max_delay=md,
+                                                dtype=np.float64))
 
 
 @pytest.mark.parametrize("integ,hybrid", [(L.EULER_STOCH, False), (L.HEUN_STOCH, True), (L.RK4_STOCH, False)])
@@ -157,7 +184,8 @@ def test_stochastic_multi(gpu, orc, integ, hybrid):
                     dt=0.1, n_steps=T, integrator=integ, hybrid_integrator=hybrid, n_sweep=2, sweep=(0, 1, vals),
                     nsig=nsig, noise=raw)
     tav, fin = orc.run_multi(subs, links, 0.1, T, integ, hybrid, sweep=(0, 1, vals), n_sweep=2, nsig=nsig, noise=raw)
-    _check(res, tav, fin)
+    _check(res, tav, fin, o64=orc.run_multi(subs, links, 0.1, T, integ, hybrid, sweep=(0, 1, vals), n_sweep=2,
+                                            nsig=nsig, noise=raw, dtype=np.float64))
 
 
 def test_split_run_resumes_bit_for_bit(gpu):
@@ -226,7 +254,8 @@ def test_plain_engine_with_stimuli(gpu, orc, integ):
                     [Link(s, t, w, cfn, cp, cm, dl) for (s, t, w, cfn, cp, cm, dl) in links],
                     dt=0.1, n_steps=T, integrator=integ, plain=True, stimuli=stims, nsig=nsig, noise=raw)
     tav, fin = orc.run_multi(subs, links, 0.1, T, integ, False, plain=True, stimuli=stims, nsig=nsig, noise=raw)
-    _check(res, tav, fin)
+    _check(res, tav, fin, o64=orc.run_multi(subs, links, 0.1, T, integ, False, plain=True, stimuli=stims, nsig=nsig,
+                                            noise=raw, dtype=np.float64))
 
 
 def test_trajectory_recording(gpu, orc):
@@ -267,7 +296,9 @@ def test_bold_of_subnetwork_zero(gpu, orc, P, tr):
                                    bold_period=P, bold_tr_ms=tr)
     assert res.bold.shape == bold.shape and bold.shape[1] > 0
     np.testing.assert_allclose(res.bold, bold, rtol=RTOL, atol=ATOL)
-    _check(res, tav, fin)
+    o64 = orc.run_multi(subs, links, 0.1, T, L.HEUN, True, sweep=(0, 1, vals), n_sweep=2, bold_period=P,
+                        bold_tr_ms=tr, dtype=np.float64)
+    _check(res, tav, fin, o64=o64, bold=bold)
 
 
 def test_plain_per_edge_csr_delays(gpu, orc):
@@ -298,7 +329,7 @@ def test_plain_per_edge_csr_delays(gpu, orc):
     res = run_multi([Subnet(m, k, p, x0) for (m, k, p, x0) in subs], links, dt=0.1, n_steps=120,
                     integrator=L.HEUN, plain=True)
     tav, fin = orc.run_multi(subs, links_o, 0.1, 120, L.HEUN, False, plain=True)
-    _check(res, tav, fin)
+    _check(res, tav, fin, o64=orc.run_multi(subs, links_o, 0.1, 120, L.HEUN, False, plain=True, dtype=np.float64))
 
 
 def test_linear_and_sup_hopf_subnetworks(gpu, orc):
@@ -319,13 +350,16 @@ def test_linear_and_sup_hopf_subnetworks(gpu, orc):
     for integ in (L.HEUN, L.RK4):
         res = _run(subs, links, dt=0.05, n_steps=150, integrator=integ, n_sweep=2)
         tav, fin = orc.run_multi(subs, links, 0.05, 150, integ, False, n_sweep=2)
-        _check(res, tav, fin)
+        _check(res, tav, fin, o64=orc.run_multi(subs, links, 0.05, 150, integ, False, n_sweep=2, dtype=np.float64))
     assert SUP_HOPF_DEFAULT.shape == (2,)
-    # CoombesByrne2D with a G2DO driver (r clamped at 0)
-    subs2 = [(L.COOMBES_BYRNE_2D, 6, np.array([1.0, -4.0, 1.0, 2.0], np.float32), np.abs(_x0(rng, 6, 2, 0.2))),
+    # CoombesByrne2D with a G2DO driver (r clamped at 0); v_syn = 0 keeps this
+    # coupled network bounded (at the default v_syn = -4 it leaves f32 range
+    # near step 100, in the oracle too)
+    subs2 = [(L.COOMBES_BYRNE_2D, 6, np.array([1.0, 0.0, 1.0, 2.0], np.float32), np.abs(_x0(rng, 6, 2, 0.2))),
              (L.G2DO, 6, G2DO_DEFAULT, _x0(rng, 6, 2))]
     links2 = [(0, 0, 0.1, L.CF_LINEAR, [0.5, 0.0], [(0, 0)]),            # both channels pass through
               (1, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 1)]), (0, 1, 0.05, L.CF_LINEAR, [1.0], [(0, 0)])]
     res = _run(subs2, links2, dt=0.02, n_steps=200, integrator=L.HEUN, n_sweep=2)
     tav, fin = orc.run_multi(subs2, links2, 0.02, 200, L.HEUN, False, n_sweep=2)
-    _check(res, tav, fin)
+    assert all(np.all(np.isfinite(f)) for f in fin)
+    _check(res, tav, fin, o64=orc.run_multi(subs2, links2, 0.02, 200, L.HEUN, False, n_sweep=2, dtype=np.float64))

@@ -362,6 +362,45 @@ def test_blocked_lanes_per_node_fast_mode(gpu, orc):
         _err_bound(x.traj, orc.run(I)["traj"], orc.run(I, dtype=np.float64)["traj"])
 
 
+WIDE_CASES = {
+    # (model, N, integrator, dmin, dmax, density): networks wider than 512
+    # nodes on the 1024-thread delay-blocked kernel
+    "kuramoto_linear_600_rk4_k8": (L.KURAMOTO, 600, L.RK4, 8, 40, 0.05),
+    "g2do_700_heun_stoch_k8": (L.G2DO, 700, L.HEUN_STOCH, 9, 60, 0.04),
+    "mpr_520_rk4_k16": (L.MPR, 520, L.RK4, 16, 200, 0.05),
+    "mpr_530_heun_k32_as_k16": (L.MPR, 530, L.HEUN, 32, 300, 0.05),
+}
+
+
+@pytest.mark.parametrize("name", list(WIDE_CASES))
+def test_wide_blocked_exact_bitwise(gpu, orc, name):
+    """One node per thread over up to 1024 threads: bit-identical to the
+    oracle in exact mode (LDS ring for the 8-step block, HBM ring for 16-step
+    blocks; a 32-step choice becomes 16 steps on the wide kernel)."""
+    model, N, integ, dmin, dmax, dens = WIDE_CASES[name]
+    I = _blocked_net(model, N=N, T=90, integ=integ, dmin=dmin, dmax=dmax, dens=dens, seed=800 + N)
+    (g,), st = run_instances([I], exact=True)
+    assert st.block_size > 512, st.block_size
+    _assert_exact(g, orc.run(I), name)
+
+
+def test_wide_blocked_kuramoto_1000_fast_mode(gpu, orc):
+    """The order-parameter workload's shape (Kuramoto coupling, 1000 nodes, 5%
+    density, RK4, delays 9-400 steps) on the wide kernel in FMA mode: within
+    the FP64 error bound, reproducible run to run."""
+    rng = np.random.default_rng(811)
+    N = 1000
+    I = _blocked_net(L.KURAMOTO, N=N, T=60, integ=L.RK4, dmin=9, dmax=400, dens=0.05, seed=811,
+                     cfn=L.CF_KURAMOTO, cparams=(1.5,), dt=0.1,
+                     params=np.array([2 * np.pi * 10 / 1000], np.float32))
+    I.x0 = rng.uniform(0, 2 * np.pi, (N, 1)).astype(np.float32)
+    (a,), st = run_instances([I], exact=False)
+    (b,), _ = run_instances([I], exact=False)
+    assert st.block_size == 1024
+    np.testing.assert_array_equal(a.traj, b.traj)
+    _err_bound(a.traj, orc.run(I)["traj"], orc.run(I, dtype=np.float64)["traj"], atol=1e-5)
+
+
 def test_autotuned_plan_matches_default(gpu, orc):
     """The launch-shape autotuner only changes lanes per node / block size:
     results stay within the FP32 bound of the default shapes, and every tuned

@@ -9,6 +9,9 @@ engine.rs:540-823) and checks the host-side validation of hb_run_multi.
 * one Euler step of a G2DO -> MPR link with cvar_map "0:1" against the
   hand-built coupling [0, w * mean(V)] fed to the model's dfun (the remap of
   engine.rs:637-664 and target_cvar_cpl of 391-393).
+* the FP64 twin (dtype float64) by the same three routes: the C oracle's
+  FP64 single-network run, independent subnetworks, and the one-step remap
+  computed in double.
 """
 import math
 
@@ -40,6 +43,51 @@ def test_single_subnet_composition(orc, integ, dense):
     ref = orc.run(inst)
     np.testing.assert_allclose(tav[0][0], ref["tavg"], rtol=1e-6, atol=1e-7)
     np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("integ", [L.HEUN, L.RK4])
+@pytest.mark.parametrize("dense", [False, True])
+def test_fp64_twin_single_subnet_composition(orc, integ, dense):
+    rng = np.random.default_rng(11)
+    n, T = 5, 50
+    x0 = _x0(rng, n, 2)
+    W = (rng.uniform(size=(n, n)) * 0.2).astype(np.float32)
+    link = (0, 0, W if dense else 0.05, L.CF_LINEAR, [0.5, 0.01], [(0, 0)])
+    tav, fin = orc.run_multi([(L.G2DO, n, G2DO_DEFAULT, x0)], [link], 0.1, T, integ, False, dtype=np.float64)
+    assert tav[0].dtype == np.float64
+    inst = Instance(model=L.G2DO, n_nodes=n, n_steps=T, dt=0.1, params=np.array(G2DO_DEFAULT, np.float32), x0=x0,
+                    integrator=integ, cfn=L.CF_LINEAR, cparams=(0.5, 0.01),
+                    wkind=L.W_DENSE if dense else L.W_SCALAR_MEAN, dense=W if dense else None,
+                    scalar_w=0.0 if dense else 0.05, delay_mode=L.DELAY_SINGLE, single_delay=0)
+    ref = orc.run(inst, dtype=np.float64)
+    np.testing.assert_allclose(tav[0][0], ref["tavg"], rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(fin[0][0], ref["final_state"], rtol=1e-12, atol=1e-14)
+    # and it is not the f32 run: the twins differ by f32 rounding only
+    t32, f32 = orc.run_multi([(L.G2DO, n, G2DO_DEFAULT, x0)], [link], 0.1, T, integ, False)
+    err = np.max(np.abs(f32[0][0].astype(np.float64) - fin[0][0]))
+    assert 0.0 < err < 1e-4
+
+
+def test_fp64_twin_unlinked_and_remap(orc):
+    rng = np.random.default_rng(12)
+    subs = [(L.G2DO, 4, G2DO_DEFAULT, _x0(rng, 4, 2)), (L.JANSEN_RIT, 3, JR_DEFAULT, _x0(rng, 3, 6, 0.01))]
+    tav, fin = orc.run_multi(subs, [], 0.1, 30, L.RK4, False, dtype=np.float64)
+    for k, (m, n, p, x0) in enumerate(subs):
+        inst = Instance(model=m, n_nodes=n, n_steps=30, dt=0.1, params=np.array(p, np.float32), x0=x0,
+                        integrator=L.RK4)
+        ref = orc.run(inst, dtype=np.float64)
+        np.testing.assert_allclose(fin[k][0], ref["final_state"], rtol=1e-12, atol=1e-14)
+    # the G2DO -> MPR remap step of test_cvar_map_remap_one_step, in double
+    xg, xm = _x0(rng, 5, 2), np.abs(_x0(rng, 4, 2))
+    subs = [(L.G2DO, 5, G2DO_DEFAULT, xg), (L.MPR, 4, MPR_DEFAULT, xm)]
+    _, fin = orc.run_multi(subs, [(0, 1, 0.2, L.CF_LINEAR, [1.0, 0.0], [(0, 1)])], 0.1, 1, L.EULER, False,
+                           dtype=np.float64)
+    g = float(np.sum(xg[:, 0].astype(np.float64))) / 5.0 * float(np.float32(0.2))
+    for i in range(4):
+        d = orc.dfun(L.MPR, MPR_DEFAULT, xm[i].astype(np.float64), np.array([0.0, g]), dtype=np.float64)
+        want = xm[i].astype(np.float64) + d * float(np.float32(0.1))
+        want[0] = max(want[0], 0.0)   # MPR r >= 0
+        np.testing.assert_allclose(fin[1][0, i], want, rtol=1e-13, atol=1e-15)
 
 
 def test_unlinked_subnets_are_independent(orc):
