@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "libhyburn_hip.so")
 HB_OK, HB_INVALID_CONFIG, HB_INVALID_STATE, HB_BACKEND_ERROR, HB_IO_ERROR = 0, 1, 2, 3, 4
 G2DO, MPR, JANSEN_RIT, WILSON_COWAN, KURAMOTO, RWW = 0, 1, 2, 3, 4, 5
 LINEAR, SUP_HOPF, HOPFIELD, COOMBES_BYRNE_2D = 6, 7, 8, 9   # on the multi-subnetwork path only
+# the models the reference pins with trace fixtures (tests/model_reference_test.rs); multi-subnetwork path only
+EPILEPTOR_CODIM3, EPILEPTOR_CODIM3_SLOWMOD, EPILEPTOR_RS, ZERLAUT_FIRST, ZERLAUT_SECOND, KIONEX = 10, 11, 12, 13, 14, 15
 EULER, HEUN, RK4, EULER_STOCH, HEUN_STOCH, RK4_STOCH = 0, 1, 2, 3, 4, 5
 CF_LINEAR, CF_DIFFERENCE, CF_KURAMOTO, CF_SIGMOIDAL, CF_SCALED_LINEAR, CF_TANH, CF_SJR, CF_PRESIGMOIDAL = range(8)
 W_NONE, W_SCALAR_MEAN, W_SCALAR_DIAG, W_DENSE, W_CSR = 0, 1, 2, 3, 4
@@ -30,14 +32,24 @@ MODEL_NAMES = {
     "Generic2dOscillator": G2DO, "MontbrioPazoRoxin": MPR, "JansenRit": JANSEN_RIT,
     "WilsonCowan": WILSON_COWAN, "Kuramoto": KURAMOTO, "ReducedWongWang": RWW,
     "Linear": LINEAR, "SupHopf": SUP_HOPF, "Hopfield": HOPFIELD, "CoombesByrne2D": COOMBES_BYRNE_2D,
+    "EpileptorCodim3": EPILEPTOR_CODIM3, "EpileptorCodim3SlowMod": EPILEPTOR_CODIM3_SLOWMOD,
+    "EpileptorRestingState": EPILEPTOR_RS, "ZerlautAdaptationFirstOrder": ZERLAUT_FIRST,
+    "ZerlautAdaptationSecondOrder": ZERLAUT_SECOND, "KIonEx": KIONEX,
 }
 # (nvar, ncvar, nparams), config.rs:43-50
 MODEL_INFO = {G2DO: (2, 1, 12), MPR: (2, 2, 7), JANSEN_RIT: (6, 1, 13),
               WILSON_COWAN: (2, 1, 22), KURAMOTO: (1, 1, 1), RWW: (1, 1, 8),
               LINEAR: (1, 1, 1), SUP_HOPF: (2, 2, 2), HOPFIELD: (2, 2, 3),
-              COOMBES_BYRNE_2D: (2, 2, 4)}
+              COOMBES_BYRNE_2D: (2, 2, 4), EPILEPTOR_CODIM3: (3, 1, 13), EPILEPTOR_CODIM3_SLOWMOD: (5, 1, 27),
+              EPILEPTOR_RS: (8, 3, 28), ZERLAUT_FIRST: (5, 1, 50), ZERLAUT_SECOND: (8, 1, 50), KIONEX: (5, 1, 14)}
+MULTI_ONLY = (LINEAR, SUP_HOPF, HOPFIELD, COOMBES_BYRNE_2D, EPILEPTOR_CODIM3, EPILEPTOR_CODIM3_SLOWMOD, EPILEPTOR_RS,
+              ZERLAUT_FIRST, ZERLAUT_SECOND, KIONEX)
+# hybrid_integrator: deterministic Heun for these models (model_prefers_heun, dfun.rs:1871-1873;
+# Epileptor and Epileptor2D are not built)
+PREFERS_HEUN = (G2DO, EPILEPTOR_RS, EPILEPTOR_CODIM3, EPILEPTOR_CODIM3_SLOWMOD, KIONEX)
 # the batch-native sweep index per model, dfun.rs:49-72
-SWEEP_INDEX = {G2DO: 1, MPR: 4, JANSEN_RIT: 12, WILSON_COWAN: 18, KURAMOTO: 0, RWW: 7, LINEAR: -1, SUP_HOPF: -1, HOPFIELD: -1, COOMBES_BYRNE_2D: -1}
+SWEEP_INDEX = {G2DO: 1, MPR: 4, JANSEN_RIT: 12, WILSON_COWAN: 18, KURAMOTO: 0, RWW: 7}
+SWEEP_INDEX.update({m: -1 for m in MULTI_ONLY})
 COUPLING_NAMES = {
     "Linear": CF_LINEAR, "Difference": CF_DIFFERENCE, "Kuramoto": CF_KURAMOTO,
     "Sigmoidal": CF_SIGMOIDAL, "ScaledLinear": CF_SCALED_LINEAR, "HyperbolicTangent": CF_TANH,
@@ -152,7 +164,11 @@ class HbMultiSpec(C.Structure):
                 ("final_state", fp), ("kernel_ms", C.c_double), ("step0", C.c_int64), ("state_in", fp),
                 ("hist_in", fp), ("hist_out", fp), ("plain", C.c_int32), ("n_stimuli", C.c_int32),
                 ("stimuli", C.POINTER(HbStimulus)), ("dt_ms", C.c_double), ("trajectory", fp),
-                ("bold_period", C.c_int32), ("bold_tr_ms", C.c_double), ("bold", fp)]
+                ("bold_period", C.c_int32), ("bold_tr_ms", C.c_double), ("bold", fp),
+                ("point_mode", C.c_int32 * 4), ("point_values", fp), ("point_params", fp)]
+
+
+MAX_PARAMS = 64   # HB_MAX_PARAMS
 
 
 # Every exported function of include/hyburn_hip.h with its prototype.

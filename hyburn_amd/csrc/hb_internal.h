@@ -26,10 +26,19 @@ constexpr int kCC = 12;   // coupling constant slots
 // one-node-per-thread schedules, delay-blocked ones among them) up to 1024.
 constexpr int kBlockMax = 512, kWideBlockMax = 1024;
 
+// Model registry (config.rs:43-72): nvar, ncvar, nparams.
 HB_HD int model_nvar(int m) {
-    return m == HB_JANSEN_RIT ? 6 : (m == HB_KURAMOTO || m == HB_RWW || m == HB_LINEAR) ? 1 : 2;
+    switch (m) {
+    case HB_JANSEN_RIT: return 6;
+    case HB_KURAMOTO: case HB_RWW: case HB_LINEAR: return 1;
+    case HB_EPILEPTOR_CODIM3: return 3;
+    case HB_EPILEPTOR_CODIM3_SLOWMOD: case HB_ZERLAUT_FIRST: case HB_KIONEX: return 5;
+    case HB_EPILEPTOR_RS: case HB_ZERLAUT_SECOND: return 8;
+    default: return 2;
+    }
 }
 HB_HD int model_ncvar(int m) {
+    if (m == HB_EPILEPTOR_RS) return 3;
     return (m == HB_MPR || m == HB_SUP_HOPF || m == HB_HOPFIELD || m == HB_COOMBES_BYRNE_2D) ? 2 : 1;
 }
 HB_HD int model_nparams(int m) {
@@ -37,8 +46,19 @@ HB_HD int model_nparams(int m) {
     case HB_G2DO: return 12; case HB_MPR: return 7; case HB_JANSEN_RIT: return 13;
     case HB_WILSON_COWAN: return 22; case HB_KURAMOTO: return 1; case HB_LINEAR: return 1;
     case HB_SUP_HOPF: return 2; case HB_HOPFIELD: return 3; case HB_COOMBES_BYRNE_2D: return 4;
+    case HB_EPILEPTOR_CODIM3: return 13; case HB_EPILEPTOR_CODIM3_SLOWMOD: return 27;
+    case HB_EPILEPTOR_RS: return 28; case HB_ZERLAUT_FIRST: case HB_ZERLAUT_SECOND: return 50;
+    case HB_KIONEX: return 14;
     default: return 8;
     }
+}
+// Models 10-15 keep their constants in a 64-slot table (model_consts_ext).
+HB_HD bool model_ext(int m) { return m >= HB_EPILEPTOR_CODIM3 && m <= HB_KIONEX; }
+constexpr int kMCX = 64;
+// hybrid_integrator's deterministic-Heun models (model_prefers_heun, dfun.rs:1871-1873)
+HB_HD bool model_prefers_heun(int m) {
+    return m == HB_G2DO || m == HB_EPILEPTOR_RS || m == HB_EPILEPTOR_CODIM3 || m == HB_EPILEPTOR_CODIM3_SLOWMOD ||
+           m == HB_KIONEX;
 }
 // The reference's single batch-sweepable parameter per model (dfun.rs:49-72).
 HB_HD int model_sweep_index(int m) {
@@ -143,6 +163,79 @@ HB_HD void model_consts_t(int m, const S* p, S* mc) {
 }
 
 HB_HD void model_consts(int m, const float* p, float* mc) { model_consts_t<float>(m, p, mc); }
+
+// Constants of models 10-15 (kMCX slots): the parameters, then the products
+// the reference forms on the host once per call, in its f32 order.
+inline void model_consts_ext(int m, const float* p, float* c) {
+    const float eps = kEps;
+    for (int i = 0; i < kMCX; ++i) c[i] = 0.0f;
+    for (int i = 0; i < model_nparams(m); ++i) c[i] = p[i];
+    switch (m) {
+    case HB_EPILEPTOR_CODIM3: {   // dfun.rs:958-983: unit start/stop points, e and f of the great circle
+        const float st[3] = {p[1], -p[0], p[2]}, sp[3] = {p[4], -p[3], p[5]};
+        const float sn = sqrtf(((0.0f + st[0] * st[0]) + st[1] * st[1]) + st[2] * st[2]);
+        const float pn = sqrtf(((0.0f + sp[0] * sp[0]) + sp[1] * sp[1]) + sp[2] * sp[2]);
+        float a[3], b[3], cr[3];
+        for (int i = 0; i < 3; ++i) { a[i] = st[i] / (sn > eps ? sn : eps); b[i] = sp[i] / (pn > eps ? pn : eps); }
+        cr[0] = a[1] * b[2] - a[2] * b[1];
+        cr[1] = a[2] * b[0] - a[0] * b[2];
+        cr[2] = a[0] * b[1] - a[1] * b[0];
+        const float cn = sqrtf(((0.0f + cr[0] * cr[0]) + cr[1] * cr[1]) + cr[2] * cr[2]);
+        for (int i = 0; i < 3; ++i) c[13 + i] = a[i];
+        if (cn > eps) {
+            float cca[3];
+            for (int i = 0; i < 3; ++i) cca[i] = cr[(i + 1) % 3] * a[(i + 2) % 3] - cr[(i + 2) % 3] * a[(i + 1) % 3];
+            float cc2 = sqrtf(((0.0f + cca[0] * cca[0]) + cca[1] * cca[1]) + cca[2] * cca[2]);
+            cc2 = cc2 > eps ? cc2 : eps;
+            for (int i = 0; i < 3; ++i) c[16 + i] = cca[i] / cc2;
+        } else {
+            c[16] = 0.0f; c[17] = 1.0f; c[18] = 0.0f;
+        }
+        break; }
+    case HB_EPILEPTOR_CODIM3_SLOWMOD:   // dfun.rs:1063-1074: g*R, h*R, l*R, m*R
+        for (int i = 0; i < 3; ++i) {
+            c[27 + i] = p[13 + i] * p[7]; c[30 + i] = p[19 + i] * p[7];
+            c[33 + i] = p[16 + i] * p[7]; c[36 + i] = p[22 + i] * p[7];
+        }
+        break;
+    case HB_EPILEPTOR_RS:   // dfun.rs:1143-1208
+        c[17] = p[17] > eps ? p[17] : eps;
+        c[28] = -4.0f * p[2];
+        c[29] = p[15] * p[7];
+        c[30] = -0.01f * p[15];
+        c[31] = p[21] * c[17];
+        c[32] = p[21] / c[17];
+        c[33] = p[26] * p[18];
+        c[34] = p[26] * p[27];
+        break;
+    case HB_ZERLAUT_FIRST: case HB_ZERLAUT_SECOND:   // dfun.rs:1449-1620
+        c[50] = ((1.0f - p[19]) * p[17]) * p[16];
+        c[51] = (p[19] * p[18]) * p[16];
+        c[52] = p[12] * p[14];
+        c[53] = p[13] * p[15];
+        c[54] = p[0] * p[1];
+        c[55] = p[0] * p[2];
+        c[56] = p[0] / p[3];
+        c[57] = 1.0f / p[22];
+        c[58] = p[16] * (1.0f - p[19]);
+        c[59] = p[16] * p[19];
+        c[60] = 1.0f / ((2.0f * 1e-4f) * 1e3f);
+        c[61] = 1.0f / (((1e-4f * 1e3f) * 1e-4f) * 1e3f);
+        break;
+    case HB_KIONEX:   // dfun.rs:1788-1849
+        c[10] = p[10] > eps ? p[10] : eps;
+        c[11] = p[11] > eps ? p[11] : eps;
+        c[14] = -1.0f / c[10];
+        c[15] = p[6] / kPi;
+        c[16] = 2.0f * p[6];
+        c[17] = 2.0f * p[8];
+        c[18] = -(p[12] / 2160.0f);
+        c[19] = 26.64f * logf(112.0f / 5.0f);
+        break;
+    default:
+        break;
+    }
+}
 
 // Coupling constants (coupling.rs:114-202 parsing, host-side products as the
 // reference forms them). Slot meaning depends on cfn; see pre()/post().

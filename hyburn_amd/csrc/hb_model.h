@@ -18,6 +18,12 @@ template <> struct Tr<HB_LINEAR> { static constexpr int NV = 1, NC = 1; };
 template <> struct Tr<HB_SUP_HOPF> { static constexpr int NV = 2, NC = 2; };
 template <> struct Tr<HB_HOPFIELD> { static constexpr int NV = 2, NC = 2; };
 template <> struct Tr<HB_COOMBES_BYRNE_2D> { static constexpr int NV = 2, NC = 2; };
+template <> struct Tr<HB_EPILEPTOR_CODIM3> { static constexpr int NV = 3, NC = 1; };
+template <> struct Tr<HB_EPILEPTOR_CODIM3_SLOWMOD> { static constexpr int NV = 5, NC = 1; };
+template <> struct Tr<HB_EPILEPTOR_RS> { static constexpr int NV = 8, NC = 3; };
+template <> struct Tr<HB_ZERLAUT_FIRST> { static constexpr int NV = 5, NC = 1; };
+template <> struct Tr<HB_ZERLAUT_SECOND> { static constexpr int NV = 8, NC = 1; };
+template <> struct Tr<HB_KIONEX> { static constexpr int NV = 5, NC = 1; };
 
 // ---------------------------------------------------------------- models ---
 
@@ -136,6 +142,279 @@ template <> __device__ __forceinline__ void dfun<HB_COOMBES_BYRNE_2D>(const floa
     dx[1] = ((-(v * v) + mc[2]) - r2 * mc[3]) + ((-v + mc[4]) * mc[1]) * r;
 }
 
+// ------------------------------------------- models 10-15 (multi path) ---
+//
+// EpileptorCodim3 (+SlowMod), EpileptorRestingState, Zerlaut first/second
+// order and KIonEx, dfun.rs:952-1870. c is the model's kMCX-slot constant
+// table (model_consts_ext), read from global memory: slots 0..nparams-1 hold
+// the parameters, the host-side products follow.
+
+__device__ __forceinline__ float sgnf(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+// atan_approx, dfun.rs:1403-1416: |x| > 1 reduced to 1/x, a rational start,
+// 12 Newton steps on sin(t) - x cos(t), pi/2 - t for the reduced inputs.
+static __device__ __attribute__((noinline)) float atan_ap(float x) {
+    const float ax = x < 0.0f ? -x : x;
+    const bool red = ax > 1.0f;
+    const float xe = red ? 1.0f / x : x;
+    const float x2 = xe * xe;
+    float th = xe / (x2 * 0.28f + 1.0f);
+    for (int it = 0; it < 12; ++it) {
+        const float sn = sinf(th), cs = cosf(th);
+        th = th - (sn - xe * cs) * cs;
+    }
+    return red ? -th + 1.57079632679489661923f : th;
+}
+
+// acos_approx, dfun.rs:1418-1422
+__device__ __forceinline__ float acos_ap(float x) {
+    const float xc = x < -0.9999f ? -0.9999f : (x > 0.9999f ? 0.9999f : x);
+    return atan_ap(sqrtf((-xc + 1.0f) / (xc + 1.0f))) * 2.0f;
+}
+
+// erf_approx, dfun.rs:1424-1446 (Abramowitz-Stegun 7.1.26)
+__device__ __forceinline__ float erf_ap(float x) {
+    const float ax = x < 0.0f ? -x : x;
+    const float t = 1.0f / (ax * 0.3275911f + 1.0f);
+    const float t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+    const float y = (((t5 * 1.061405429f + t4 * -1.453152027f) + t3 * 1.421413741f) + t2 * -0.284496736f) +
+                    t * 0.254829592f;
+    return sgnf(x) * (-(y * expf((ax * -1.0f) * ax)) + 1.0f);
+}
+
+// xs on the slow manifold (dfun.rs:996-1019, 1100-1118): trigonometric root
+// for a negative discriminant, Cardano otherwise.
+static __device__ __attribute__((noinline)) float ec3_xs(float mu1, float mu2, int branch) {
+    const float disc = (mu1 * 0.25f) * mu1 - (mu2 * mu2) * (mu2 * (1.0f / 27.0f));
+    const float m3 = mu2 / 3.0f;
+    const float sr = sqrtf(m3 > kEps ? m3 : kEps);
+    const float r32 = m3 * sr;
+    float c3 = (mu1 / 2.0f) / (r32 > kEps ? r32 : kEps);
+    c3 = c3 < -1.0f ? -1.0f : (c3 > 1.0f ? 1.0f : c3);
+    const float th = acos_ap(c3) / 3.0f;
+    const float tp3 = 2.0943951023931954923f;   // 2 pi / 3 in f32
+    const float xt = (sr * 2.0f) * cosf(branch == 2 ? th - tp3 : (branch == 3 ? th + tp3 : th));
+    const float sd = sqrtf(disc > 0.0f ? disc : 0.0f);
+    const float h = mu1 / 2.0f, pa = h + sd, na = h - sd, third = 1.0f / 3.0f;
+    const float cp = sgnf(pa) * powf(pa < 0.0f ? -pa : pa, third);
+    const float cn = sgnf(na) * powf(na < 0.0f ? -na : na, third);
+    return disc < 0.0f ? xt : cp + cn;
+}
+
+__device__ __forceinline__ float ec3_mod(float z) {   // 0.1 (z - 0.5)^7
+    const float zs = z + -0.5f;
+    return ((((((zs * zs) * zs) * zs) * zs) * zs) * zs) * 0.1f;
+}
+
+// get_fluct_regime_vars, dfun.rs:1449-1488
+__device__ __forceinline__ void zl_fluct(const float* __restrict__ c, float fe, float fi, float fe_ext, float fi_ext,
+                                         float w, float gle, float& mu_v, float& sigma_v, float& t_v) {
+    const float fe_tot = (fe + 1e-6f) * c[50] + fe_ext * c[20];
+    const float fi_tot = (fi + 1e-6f) * c[51] + fi_ext * c[21];
+    const float ge = fe_tot * c[52], gi = fi_tot * c[53];
+    const float mu_g = (ge + gi) + c[0];
+    const float t_m = (1.0f / mu_g) * c[3];
+    mu_v = (((ge * c[10] + gi * c[11]) + (-w)) + gle) / mu_g;
+    const float ue = ((-mu_v + c[10]) * c[12]) / mu_g;
+    const float ui = ((-mu_v + c[11]) * c[13]) / mu_g;
+    const float uet = ue * c[14], uit = ui * c[15];
+    const float ue2 = uet * uet, ui2 = uit * uit;
+    const float sv2 = (fe_tot * ue2) / ((t_m + c[14]) * 2.0f) + (fi_tot * ui2) / ((t_m + c[15]) * 2.0f);
+    sigma_v = sqrtf(sv2);
+    if (!(sigma_v >= 1e-10f)) sigma_v = sigma_v != sigma_v ? sigma_v : 1e-10f;   // clamp keeps NaN
+    const float num = fe_tot * ue2 + fi_tot * ui2;
+    float den = (fe_tot * ue2) / (t_m + c[14]) + (fi_tot * ui2) / (t_m + c[15]);
+    if (!(den >= 1e-10f)) den = den != den ? den : 1e-10f;
+    t_v = num / den;
+}
+
+// TF: threshold polynomial (dfun.rs:1490-1513), firing rate (1515-1523), 1525-1544
+static __device__ __attribute__((noinline)) float zl_tf(const float* __restrict__ c, float fe, float fi, float fe_ext,
+                                                 float fi_ext, float w, float gle, int poff) {
+    float mu_v, sigma_v, t_v;
+    zl_fluct(c, fe, fi, fe_ext, fi_ext, w, gle, mu_v, sigma_v, t_v);
+    const float* __restrict__ P = c + poff;
+    const float tvn = t_v * c[56];
+    const float v = (mu_v + 60.0f) / 10.0f, sg = (sigma_v + -4.0f) / 6.0f, t = tvn + -0.5f;
+    float thr = (v * v) * P[4] + (sg * sg) * P[5];
+    thr = thr + (t * t) * P[6];
+    thr = thr + (v * sg) * P[7];
+    thr = thr + (v * t) * P[8];
+    thr = thr + (sg * t) * P[9];
+    thr = thr + v * P[1];
+    thr = thr + sg * P[2];
+    thr = thr + t * P[3];
+    thr = (thr + P[0]) * 1000.0f;
+    const float arg = (thr - mu_v) / (sigma_v * 1.41421356237309504880f);
+    float den = t_v * 2.0f;
+    den = den > kEps ? den : (den != den ? den : kEps);
+    return (-erf_ap(arg) + 1.0f) / den;
+}
+
+__device__ __forceinline__ float kx_sig(float u) { return 1.0f / (expf(-u) + 1.0f); }
+__device__ __forceinline__ float kx_ln(float u) { const float l = logf(u); return l > 1e-10f ? l : (l != l ? l : 1e-10f); }
+
+template <> __device__ __forceinline__ void dfun<HB_EPILEPTOR_CODIM3>(const float* c, bool, float, const float* x,
+                                                                     const float* cpl, float* dx) {
+    // dfun.rs:985-1032
+    const float cz = cosf(x[2]), sz = sinf(x[2]);
+    const float mu2 = (cz * c[13] + sz * c[16]) * c[7];
+    const float mu1 = (cz * c[14] + sz * c[17]) * (-c[7]);
+    const float nu = (cz * c[15] + sz * c[18]) * c[7];
+    const float xs = ec3_xs(mu1, mu2, (int)c[11]);
+    const float dd = sqrtf((x[0] - xs) * (x[0] - xs) + x[1] * x[1]);
+    float t = -(-dd + c[9]);
+    dx[0] = -x[1];
+    dx[1] = (((x[0] * x[0]) * x[0] - mu2 * x[0]) - mu1) - x[1] * ((nu + x[0] * c[6]) + x[0] * x[0]);
+    if (c[12] > 0.5f) t = t + ec3_mod(x[2]);
+    dx[2] = -((t + cpl[0] * c[10]) * c[8]);
+}
+
+template <> __device__ __forceinline__ void dfun<HB_EPILEPTOR_CODIM3_SLOWMOD>(const float* c, bool, float,
+                                                                             const float* x, const float* cpl,
+                                                                             float* dx) {
+    // dfun.rs:1035-1134
+    const float cua = cosf(x[3]), sua = sinf(x[3]), cub = cosf(x[4]), sub = sinf(x[4]);
+    const float a0 = cua * c[27] + sua * c[30], a1 = cua * c[28] + sua * c[31], a2 = cua * c[29] + sua * c[32];
+    const float b0 = cub * c[33] + sub * c[36], b1 = cub * c[34] + sub * c[37], b2 = cub * c[35] + sub * c[38];
+    float an = sqrtf((a0 * a0 + a1 * a1) + a2 * a2);
+    an = an > kEps ? an : (an != an ? an : kEps);
+    const float e0 = a0 / an, e1 = a1 / an, e2 = a2 / an;
+    const float c0 = a1 * b2 - a2 * b1, c1 = a2 * b0 - a0 * b2, c2 = a0 * b1 - a1 * b0;
+    float f0 = c1 * a2 - c2 * a1, f1 = c2 * a0 - c0 * a2, f2 = c0 * a1 - c1 * a0;
+    float fn = sqrtf((f0 * f0 + f1 * f1) + f2 * f2);
+    fn = fn > kEps ? fn : (fn != fn ? fn : kEps);
+    f0 = f0 / fn; f1 = f1 / fn; f2 = f2 / fn;
+    const float cz = cosf(x[2]), sz = sinf(x[2]);
+    const float mu2 = (cz * e0 + sz * f0) * c[7];
+    const float mu1 = -((cz * e1 + sz * f1) * c[7]);
+    const float nu = (cz * e2 + sz * f2) * c[7];
+    const float xs = ec3_xs(mu1, mu2, (int)c[11]);
+    dx[0] = -x[1];
+    dx[1] = (((x[0] * x[0]) * x[0] - mu2 * x[0]) - mu1) - x[1] * ((nu + x[0] * c[6]) + x[0] * x[0]);
+    const float dd = sqrtf((x[0] - xs) * (x[0] - xs) + x[1] * x[1]);
+    const float t = -(-dd + c[9]) + (c[12] > 0.5f ? ec3_mod(x[2]) : 0.0f);
+    dx[2] = -((t + cpl[0] * c[10]) * c[8]);
+    dx[3] = 0.0f + c[25];
+    dx[4] = 0.0f + c[26];
+}
+
+template <> __device__ __forceinline__ void dfun<HB_EPILEPTOR_RS>(const float* c, bool, float, const float* x,
+                                                                 const float* cpl, float* dx) {
+    // dfun.rs:1137-1209
+    const float x1 = x[0], y1 = x[1], z = x[2], x2 = x[3], y2 = x[4], g = x[5], xr = x[6], yr = x[7];
+    float f1;
+    if (x1 < 0.0f) f1 = (-(x1 * x1)) * c[3] + x1 * c[4];
+    else { const float z4 = z + -4.0f; f1 = ((-x2) + c[8]) + (z4 * z4) * 0.6f; }
+    dx[0] = ((((y1 - z) + cpl[0] * c[12]) + f1 * x1) + c[0]) * c[15];
+    dx[1] = (((-y1) + c[5]) - (x1 * x1) * c[6]) * c[15];
+    const float zm = z * -0.1f;
+    const float yz = z < 0.0f ? -((((((zm * zm) * zm) * zm) * zm) * zm) * zm) : 0.0f;
+    const float h = c[16] > 0.5f ? (x1 + 3.0f) / (expf(-((x1 + 0.5f) * 10.0f)) + 1.0f) + yz : (x1 * 4.0f + c[28]) + yz;
+    dx[2] = ((h - z) + cpl[0] * c[14]) * c[29];
+    dx[3] = ((((((x2 - (x2 * x2) * x2) - y2) + c[1]) + g * c[11]) - (z + -3.5f) * 0.3f) + cpl[1] * c[13]) * c[15];
+    const float xp = x2 + 0.25f;
+    const float f2 = xp < 0.0f ? 0.0f : xp * c[10];
+    dx[4] = ((f2 - y2) * c[15]) / c[9];
+    dx[5] = ((-x1) * 0.1f + g) * c[30];
+    dx[6] = ((((yr * c[24] - ((xr * xr) * xr) * c[23]) + (xr * xr) * c[22]) + c[33]) + cpl[2] * c[34]) * c[31];
+    dx[7] = ((xr * c[20] - yr * c[25]) + c[19]) * c[32];
+}
+
+template <> __device__ __forceinline__ void dfun<HB_ZERLAUT_FIRST>(const float* c, bool, float, const float* x,
+                                                                  const float* cpl, float* dx) {
+    // dfun.rs:1546-1608
+    const float e = x[0], i = x[1], we = x[2], wi = x[3], ou = x[4];
+    float fx = cpl[0] + ou * c[28];
+    if (fx * c[20] < 0.0f) fx = 0.0f;
+    const float ee = fx + c[23], ei = 0.0f + c[24], ie = fx + c[25], ii = 0.0f + c[26];
+    const float tfe = zl_tf(c, e, i, ee, ei, we, c[54], 30);
+    const float tfi = zl_tf(c, e, i, ie, ii, wi, c[55], 40);
+    dx[0] = (tfe - e) / c[22];
+    dx[1] = (tfi - i) / c[22];
+    float mv, s_, t_;
+    zl_fluct(c, e, i, ee, ei, we, c[54], mv, s_, t_);
+    dx[2] = ((-we) / c[8] + e * c[4]) + ((mv + (-c[1])) * c[5]) / c[8];
+    zl_fluct(c, e, i, ie, ii, wi, c[55], mv, s_, t_);
+    dx[3] = ((-wi) / c[9] + i * c[6]) + ((mv + (-c[2])) * c[7]) / c[9];
+    dx[4] = (-ou) / c[27];
+}
+
+template <> __device__ __forceinline__ void dfun<HB_ZERLAUT_SECOND>(const float* c, bool, float, const float* x,
+                                                                   const float* cpl, float* dx) {
+    // dfun.rs:1610-1780
+    const float e = x[0], i = x[1], cee = x[2], cei = x[3], cii = x[4], we = x[5], wi = x[6], ou = x[7];
+    float ex = cpl[0] + (ou * c[28] + c[23]);
+    if (ex < 0.0f) ex = 0.0f;
+    float ix = cpl[0] * c[29] + (ou * c[28] + c[25]);
+    if (ix < 0.0f) ix = 0.0f;
+    const float iex = 0.0f + c[24], iix = 0.0f + c[26];
+    const float df = 1e-4f, ep = e + df, em = e + (-df), ip = i + df, im = i + (-df);
+    const float a2 = c[60], q2 = c[61];
+    const float tfe = zl_tf(c, e, i, ex, iex, we, c[54], 30);
+    const float tfi = zl_tf(c, e, i, ix, iix, wi, c[55], 40);
+    const float e_p0 = zl_tf(c, ep, i, ex, iex, we, c[54], 30), e_n0 = zl_tf(c, em, i, ex, iex, we, c[54], 30);
+    const float e_0p = zl_tf(c, e, ip, ex, iex, we, c[54], 30), e_0n = zl_tf(c, e, im, ex, iex, we, c[54], 30);
+    const float e_pp = zl_tf(c, ep, ip, ex, iex, we, c[54], 30), e_pn = zl_tf(c, ep, im, ex, iex, we, c[54], 30);
+    const float e_np = zl_tf(c, em, ip, ex, iex, we, c[54], 30), e_nn = zl_tf(c, em, im, ex, iex, we, c[54], 30);
+    const float i_p0 = zl_tf(c, ep, i, ix, iix, wi, c[55], 40), i_n0 = zl_tf(c, em, i, ix, iix, wi, c[55], 40);
+    const float i_0p = zl_tf(c, e, ip, ix, iix, wi, c[55], 40), i_0n = zl_tf(c, e, im, ix, iix, wi, c[55], 40);
+    const float i_pp = zl_tf(c, ep, ip, ix, iix, wi, c[55], 40), i_pn = zl_tf(c, ep, im, ix, iix, wi, c[55], 40);
+    const float i_np = zl_tf(c, em, ip, ix, iix, wi, c[55], 40), i_nn = zl_tf(c, em, im, ix, iix, wi, c[55], 40);
+    const float dfe_e = (e_p0 - e_n0) * a2, dfi_e = (e_0p - e_0n) * a2;
+    const float dfe_i = (i_p0 - i_n0) * a2, dfi_i = (i_0p - i_0n) * a2;
+    const float fefe_e = ((e_p0 - tfe * 2.0f) + e_n0) * q2, fifi_e = ((e_0p - tfe * 2.0f) + e_0n) * q2;
+    const float fefi_e = ((e_pp - e_np) * a2 - (e_pn - e_nn) * a2) * a2;
+    const float fife_e = ((e_pp - e_pn) * a2 - (e_np - e_nn) * a2) * a2;
+    const float fefe_i = ((i_p0 - tfi * 2.0f) + i_n0) * q2, fifi_i = ((i_0p - tfi * 2.0f) + i_0n) * q2;
+    const float fefi_i = ((i_pp - i_np) * a2 - (i_pn - i_nn) * a2) * a2;
+    const float fife_i = ((i_pp - i_pn) * a2 - (i_np - i_nn) * a2) * a2;
+    dx[0] = (((((tfe - e) + cee * (fefe_e * 0.5f)) + cei * (fefi_e * 0.5f)) + cei * (fife_e * 0.5f)) +
+             cii * (fifi_e * 0.5f)) / c[22];
+    dx[1] = (((((tfi - i) + cee * (fefe_i * 0.5f)) + cei * (fefi_i * 0.5f)) + cei * (fife_i * 0.5f)) +
+             cii * (fifi_i * 0.5f)) / c[22];
+    dx[2] = ((((tfe * ((c[57] - tfe) / c[58]) + (tfe - e) * (tfe - e)) + cee * (dfe_e * 2.0f)) + cei * (dfi_e * 2.0f)) -
+             cee * 2.0f) / c[22];
+    dx[3] = ((((((tfe - e) * (tfi - i) + cee * dfe_e) + cei * dfe_i) + cei * dfi_e) + cii * dfi_i) - cei * 2.0f) / c[22];
+    dx[4] = ((((tfi * ((c[57] - tfi) / c[59]) + (tfi - i) * (tfi - i)) + cii * (dfi_i * 2.0f)) + cei * (dfe_i * 2.0f)) -
+             cii * 2.0f) / c[22];
+    float mv, s_, t_;
+    zl_fluct(c, e, i, ex, iex, we, c[54], mv, s_, t_);
+    dx[5] = ((-we) / c[8] + e * c[4]) + ((mv + (-c[1])) * c[5]) / c[8];
+    zl_fluct(c, e, i, ix, iix, wi, c[55], mv, s_, t_);
+    dx[6] = ((-wi) / c[9] + i * c[6]) + ((mv + (-c[2])) * c[7]) / c[9];
+    dx[7] = (-ou) / c[27];
+}
+
+template <> __device__ __forceinline__ void dfun<HB_KIONEX>(const float* c, bool, float, const float* x,
+                                                           const float* cpl, float* dx) {
+    // dfun.rs:1782-1869
+    const float xx = x[0], v = x[1], n = x[2], dki = x[3], kg = x[4];
+    const float m_inf = kx_sig((v + 24.0f) * (1.0f / 12.0f));
+    const float n_inf = kx_sig((v + 19.0f) * (1.0f / 18.0f));
+    const float h_na = -(1.0f / (expf((n + -0.4f) * -8.0f) + 1.0f)) + 1.1f;
+    const float bv = 2160.0f / 720.0f;
+    const float dna_i = -dki, dna_o = dna_i * (-bv), dk_o = dki * (-bv);
+    const float ki = (-dki) + 130.0f, nai = dna_i + 16.0f, nao = dna_o + 138.0f, ko = (dk_o + 4.8f) + kg;
+    const float i_k = (n * 22.0f + 0.12f) * (v - (kx_ln(ko) - kx_ln(ki)) * 26.64f);
+    const float i_na = ((m_inf * h_na) * 40.0f + 0.02f) * (v - (kx_ln(nao) - kx_ln(nai)) * 26.64f);
+    const float i_cl = (v + c[19]) * 7.5f;
+    const float i_pump = (kx_sig(nai + -21.0f) * kx_sig(ko + -5.5f)) * 250.0f;
+    const float v_dot = (((i_na + i_k) + i_cl) + i_pump) * c[14];
+    const float r_x = (xx * c[6]) / kPi;
+    const float syn = (cpl[0] * c[15]) * ((-v) + c[0]);
+    if (v < c[9]) {
+        dx[0] = ((((v + (-c[5])) * c[16]) * xx) + c[4]) - r_x * (xx * c[2]);
+        dx[1] = v_dot - xx * (((xx * c[6]) + c[3]) + syn);
+    } else {
+        dx[0] = ((((v + (-c[7])) * c[17]) * xx) + c[4]) - r_x * (xx * c[2]);
+        dx[1] = v_dot - xx * (((xx * c[8]) + c[3]) + syn);
+    }
+    dx[2] = (n_inf - n) / c[11];
+    dx[3] = (i_k - i_pump * 2.0f) * c[18];
+    dx[4] = ((-ko) + c[1]) * c[13];
+}
+
 // clamp_batch, dfun.rs:170-239. Ternaries keep NaN like f32::clamp.
 template <int M> __device__ __forceinline__ void clampm(float* x) {
     if constexpr (M == HB_WILSON_COWAN) {
@@ -145,8 +424,11 @@ template <int M> __device__ __forceinline__ void clampm(float* x) {
         x[0] = x[0] < 0.0f ? 0.0f : x[0];
     } else if constexpr (M == HB_RWW) {
         x[0] = x[0] < 0.0f ? 0.0f : (x[0] > 1.0f ? 1.0f : x[0]);
-    } else if constexpr (M == HB_COOMBES_BYRNE_2D) {   // dfun.rs:190-194: r clamped at 0
+    } else if constexpr (M == HB_COOMBES_BYRNE_2D || M == HB_KIONEX) {   // dfun.rs:190-194, 232-236: r, x >= 0
         x[0] = x[0] < 0.0f ? 0.0f : x[0];
+    } else if constexpr (M == HB_ZERLAUT_FIRST || M == HB_ZERLAUT_SECOND) {   // dfun.rs:219-230: E, I in [0, 1]
+        x[0] = x[0] < 0.0f ? 0.0f : (x[0] > 1.0f ? 1.0f : x[0]);
+        x[1] = x[1] < 0.0f ? 0.0f : (x[1] > 1.0f ? 1.0f : x[1]);
     } else if constexpr (M == HB_KURAMOTO) {
         const float tp = 2.0f * kPi;
         x[0] = x[0] - floorf(x[0] / tp) * tp;

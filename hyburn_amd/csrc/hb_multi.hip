@@ -30,12 +30,12 @@ namespace multi {
 
 #include "hb_model.h"
 
-constexpr int kMaxSub = 4, kMaxProj = 8, kNV = 6, kMaxNodes = 1024;
+constexpr int kMaxSub = 4, kMaxProj = 8, kNV = 8, kMaxNodes = 1024;
 constexpr int kMaxLdsFloats = 16384;   // 64 KiB of pre values
 
 struct Sub {
     int32_t model, n, off, nv, integ, sw, stoch;
-    float mc[kMC];
+    float mc[kMCX];        // model_consts (first kMC slots) or model_consts_ext
     float nscale[kNV];     // sqrt(2 nsig_v) * sqrt(dt)
 };
 
@@ -92,6 +92,8 @@ struct Args {
     float* hist;           // [B][n_tot][H][2] each node's coupling variables of the last H steps
     const float* noise;    // [n_steps][n_tot][kNV] raw N(0,1), shared by the sweep points, or null
     const float* sweep;    // [B] swept values, or null
+    const float* pt_mc;    // [B][n_sub][kMCX] per-point model constants (run_sweep_multi), or null
+    const float* pt_sv;    // [B][n_sub] per-point swept-form values, or null
     const float* x0;       // [n_tot][kNV]
     float* tavg;           // [B][n_tot][kNV]
     float* fin;            // [B][n_tot][kNV]
@@ -102,7 +104,10 @@ struct Args {
 };
 
 __device__ __forceinline__ void pre_rt(const Proj& P, const float* x, float* out) {
-    if (P.nc == 1) {
+    if (P.nc == 3) {   // EpileptorRestingState's three cvars (no Kuramoto: the host refuses it)
+        if (P.np == 1) pre<3, 1>(P.cfn, P.cc, false, P.classic, x, out);
+        else pre<3, 3>(P.cfn, P.cc, false, P.classic, x, out);
+    } else if (P.nc == 1) {
         if (P.np == 2) pre<1, 2>(P.cfn, P.cc, false, P.classic, x, out);
         else pre<1, 1>(P.cfn, P.cc, false, P.classic, x, out);
     } else {
@@ -113,7 +118,10 @@ __device__ __forceinline__ void pre_rt(const Proj& P, const float* x, float* out
 }
 
 __device__ __forceinline__ void post_rt(const Proj& P, const float* g, const float* own, float* out) {
-    if (P.nc == 1) {
+    if (P.nc == 3) {
+        if (P.np == 1) post<3, 1>(P.cfn, P.cc, P.classic, g, own, out);
+        else post<3, 3>(P.cfn, P.cc, P.classic, g, own, out);
+    } else if (P.nc == 1) {
         if (P.np == 2) post<1, 2>(P.cfn, P.cc, P.classic, g, own, out);
         else post<1, 1>(P.cfn, P.cc, P.classic, g, own, out);
     } else {
@@ -123,9 +131,17 @@ __device__ __forceinline__ void post_rt(const Proj& P, const float* g, const flo
     }
 }
 
-__device__ __forceinline__ void step_model(int model, const StepCtx& S, const float* mc, const float* cpl,
-                                           const float* z, float* x) {
+// mc: the register copy of the first kMC constants; mcg: the full table in
+// global memory, which models 10-15 read directly.
+__device__ __forceinline__ void step_model(int model, const StepCtx& S, const float* mc, const float* mcg,
+                                           const float* cpl, const float* z, float* x) {
     switch (model) {
+    case HB_EPILEPTOR_CODIM3: integrate<HB_EPILEPTOR_CODIM3>(S, mcg, cpl, z, x); break;
+    case HB_EPILEPTOR_CODIM3_SLOWMOD: integrate<HB_EPILEPTOR_CODIM3_SLOWMOD>(S, mcg, cpl, z, x); break;
+    case HB_EPILEPTOR_RS: integrate<HB_EPILEPTOR_RS>(S, mcg, cpl, z, x); break;
+    case HB_ZERLAUT_FIRST: integrate<HB_ZERLAUT_FIRST>(S, mcg, cpl, z, x); break;
+    case HB_ZERLAUT_SECOND: integrate<HB_ZERLAUT_SECOND>(S, mcg, cpl, z, x); break;
+    case HB_KIONEX: integrate<HB_KIONEX>(S, mcg, cpl, z, x); break;
     case HB_G2DO: integrate<HB_G2DO>(S, mc, cpl, z, x); break;
     case HB_MPR: integrate<HB_MPR>(S, mc, cpl, z, x); break;
     case HB_JANSEN_RIT: integrate<HB_JANSEN_RIT>(S, mc, cpl, z, x); break;
@@ -187,9 +203,11 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     const bool live = s >= 0;
     const int i = live ? t - A->sub[s].off : 0;
     const int model = live ? A->sub[s].model : 0, nv = live ? A->sub[s].nv : 0;
+    // per-point constants of a run_sweep_multi, else the subnetwork's own
+    const float* mcg = live ? (A->pt_mc ? A->pt_mc + ((size_t)b * A->n_sub + s) * kMCX : A->sub[s].mc) : A->sub[0].mc;
     float mc[kMC];
 #pragma unroll
-    for (int k = 0; k < kMC; ++k) mc[k] = live ? A->sub[s].mc[k] : 0.0f;
+    for (int k = 0; k < kMC; ++k) mc[k] = live ? mcg[k] : 0.0f;
     float x[kNV], acc[kNV];
 #pragma unroll
     for (int v = 0; v < kNV; ++v) {
@@ -207,7 +225,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
 #pragma unroll
     for (int v = 0; v < kNV; ++v) nscale[v] = live ? A->sub[s].nscale[v] : 0.0f;
     S.sw = live && A->sub[s].sw != 0;
-    S.sv = A->sweep ? A->sweep[b] : 0.0f;
+    S.sv = A->pt_sv ? (live ? A->pt_sv[(size_t)b * A->n_sub + s] : 0.0f) : (A->sweep ? A->sweep[b] : 0.0f);
     const int H = A->H;
     float* hist = (live && H > 0) ? A->hist + ((size_t)b * A->n_tot + t) * H * 2 : nullptr;
     // BOLD of subnetwork 0 (bold_monitor.rs:72-106): window sums of variable 0,
@@ -268,7 +286,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
         }
         __syncthreads();
         if (live) {
-            float cpl[2] = {0.0f, 0.0f};
+            float cpl[3] = {0.0f, 0.0f, 0.0f};
             for (int p = 0; p < A->n_proj; ++p) {
                 const Proj& P = A->proj[p];
                 if (s != P.tgt) continue;
@@ -317,7 +335,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                         own[c] = sinf(x[c]);
                         own[P.nc + c] = cosf(x[c]);
                     }
-                float out[2] = {0.0f, 0.0f};
+                float out[3] = {0.0f, 0.0f, 0.0f};
                 post_rt(P, g, own, out);
                 if (P.pass_all) {
                     for (int c = 0; c < P.nc; ++c) cpl[c] += out[c];
@@ -337,7 +355,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                 const float* raw = A->noise + ((size_t)k * A->n_tot + t) * kNV;
                 for (int v = 0; v < nv; ++v) z[v] = raw[v] * nscale[v];   // integrator.rs:183-191
             }
-            step_model(model, S, mc, cpl, z, x);
+            step_model(model, S, mc, mcg, cpl, z, x);
 #pragma unroll
             for (int v = 0; v < kNV; ++v) acc[v] += x[v];
             if (A->traj) {
@@ -388,10 +406,14 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
 // Position of state variable t among the model's coupling variables (the
 // CVAR constants of src/model/*.rs), 0 when absent (engine.rs:391-393).
 int cvar_position(int model, int t) {
-    static const int g2do[] = {0}, mpr[] = {0, 1}, one0[] = {0}, jr[] = {1};
+    static const int g2do[] = {0}, mpr[] = {0, 1}, one0[] = {0}, jr[] = {1}, ers[] = {0, 3, 6};
     const int* cv = one0;
     int n = 1;
     if (model == HB_G2DO) cv = g2do;
+    else if (model == HB_EPILEPTOR_RS) {
+        cv = ers;
+        n = 3;
+    }
     else if (model == HB_MPR || model == HB_SUP_HOPF || model == HB_HOPFIELD || model == HB_COOMBES_BYRNE_2D) {
         cv = mpr;
         n = 2;
@@ -432,13 +454,25 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     if (Q.n_stimuli > 0 && !Q.plain) return bad("stimuli apply to the plain engine only (the batch engine ignores them)");
     if (Q.sweep_sub >= 0 && !Q.sweep_values) return bad("sweep values missing");
     if (!Q.tavg || !Q.final_state) return bad("null output pointer");
+    bool any_point_mode = false;
+    for (int q = 0; q < Q.n_subnets; ++q) {
+        if (Q.point_mode[q] < 0 || Q.point_mode[q] > 2) return bad("point_mode must be 0, 1 or 2");
+        if (Q.point_mode[q] == 1 && !Q.point_values) return bad("point_mode 1 needs point_values");
+        if (Q.point_mode[q] == 1 && model_sweep_index(Q.subnets[q].model) < 0)
+            return bad("point_mode 1: subnetwork " + std::to_string(q) + "'s model has no batch-native swept parameter");
+        if (Q.point_mode[q] == 2 && !Q.point_params) return bad("point_mode 2 needs point_params");
+        any_point_mode = any_point_mode || Q.point_mode[q] != 0;
+    }
     Args A;
     std::memset(&A, 0, sizeof(A));
     int n_tot = 0;
     for (int q = 0; q < Q.n_subnets; ++q) {
         const hb_subnetwork& U = Q.subnets[q];
-        if (U.model < HB_G2DO || U.model > HB_COOMBES_BYRNE_2D) return bad("unknown model in subnetwork " + std::to_string(q));
+        if (U.model < HB_G2DO || U.model > HB_KIONEX) return bad("unknown model in subnetwork " + std::to_string(q));
         if (U.n_nodes < 1 || !U.params || !U.x0) return bad("subnetwork " + std::to_string(q) + ": nodes, params, x0");
+        if (model_ext(U.model) && U.n_params != model_nparams(U.model))
+            return bad("subnetwork " + std::to_string(q) + ": expected " + std::to_string(model_nparams(U.model)) +
+                       " parameters");
         Sub& S = A.sub[q];
         S.model = U.model;
         S.n = U.n_nodes;
@@ -447,7 +481,7 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         // engine.rs:698-773: hybrid -> deterministic Heun for G2DO; otherwise the
         // configured integrator, where a configured (deterministic) Heun falls
         // through to Euler
-        if (Q.hybrid) S.integ = U.model == HB_G2DO ? HB_HEUN : (Q.integrator == HB_HEUN ? HB_EULER : Q.integrator);
+        if (Q.hybrid) S.integ = model_prefers_heun(U.model) ? HB_HEUN : (Q.integrator == HB_HEUN ? HB_EULER : Q.integrator);
         else S.integ = Q.integrator;
         S.stoch = (S.integ == HB_EULER_STOCH || S.integ == HB_HEUN_STOCH || S.integ == HB_RK4_STOCH) ? 1 : 0;
         if (S.stoch) {
@@ -458,7 +492,10 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
             for (int v = 0; v < S.nv; ++v) S.nscale[v] = sqrtf(2.0f * Q.nsig[v]) * sqrtf(Q.dt);
         }
         S.sw = (q == Q.sweep_sub && Q.sweep_param == model_sweep_index(U.model)) ? 1 : 0;
-        model_consts(U.model, U.params, S.mc);
+        if (any_point_mode) S.sw = Q.point_mode[q] == 1 ? 1 : 0;
+        for (int k = 0; k < kMCX; ++k) S.mc[k] = 0.0f;
+        if (model_ext(U.model)) model_consts_ext(U.model, U.params, S.mc);
+        else model_consts(U.model, U.params, S.mc);
         n_tot += U.n_nodes;
     }
     if (n_tot > kMaxNodes) return bad("at most 1024 nodes over all subnetworks");
@@ -508,6 +545,12 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         P.np = coupling_npre(R.cfn, ncs);
         if ((R.cfn == HB_CF_KURAMOTO || R.cfn == HB_CF_DIFFERENCE) && ncs != nct)
             return bad(tag + ": Kuramoto and Difference coupling need equal source and target ncvar");
+        // three coupling variables (EpileptorRestingState): the history keeps two
+        // per node, and Kuramoto's six pre channels exceed the kernel's four
+        if (ncs > 2 && (R.delay > 0 || R.edge_delays))
+            return bad(tag + ": delayed projections from a three-cvar subnetwork are not supported");
+        if (ncs > 2 && R.cfn == HB_CF_KURAMOTO)
+            return bad(tag + ": Kuramoto coupling from a three-cvar subnetwork is not supported");
         float cp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int k = 0; k < R.n_cparams; ++k) cp[k] = R.cparams[k];
         int classic = 0;
@@ -675,6 +718,31 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     }
     if (e == hipSuccess) e = hipMemcpy(dx0, x0.data(), sizeof(float) * x0.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && dsw) e = hipMemcpy(dsw, Q.sweep_values, sizeof(float) * Q.n_sweep, hipMemcpyHostToDevice);
+    // run_sweep_multi: every point's constants and swept-form values per subnetwork
+    float *dpt_mc = nullptr, *dpt_sv = nullptr;
+    if (e == hipSuccess && any_point_mode) {
+        const size_t nq = (size_t)Q.n_sweep * Q.n_subnets;
+        std::vector<float> pmc(nq * kMCX, 0.0f), psv(nq, 0.0f);
+        for (int b = 0; b < Q.n_sweep; ++b)
+            for (int q = 0; q < Q.n_subnets; ++q) {
+                float* row = pmc.data() + ((size_t)b * Q.n_subnets + q) * kMCX;
+                const int m = Q.subnets[q].model;
+                if (Q.point_mode[q] == 2) {
+                    const float* pp = Q.point_params + ((size_t)b * Q.n_subnets + q) * HB_MAX_PARAMS;
+                    if (model_ext(m)) model_consts_ext(m, pp, row);
+                    else model_consts(m, pp, row);
+                } else {
+                    std::memcpy(row, A.sub[q].mc, sizeof(float) * kMCX);
+                }
+                if (Q.point_mode[q] == 1) psv[(size_t)b * Q.n_subnets + q] = Q.point_values[(size_t)b * Q.n_subnets + q];
+            }
+        e = B.alloc(&dpt_mc, pmc.size());
+        if (e == hipSuccess) e = B.alloc(&dpt_sv, psv.size());
+        if (e == hipSuccess) e = hipMemcpy(dpt_mc, pmc.data(), sizeof(float) * pmc.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dpt_sv, psv.data(), sizeof(float) * psv.size(), hipMemcpyHostToDevice);
+    }
+    A.pt_mc = dpt_mc;
+    A.pt_sv = dpt_sv;
     if (e != hipSuccess) return hip_fail(e, "multi-subnetwork upload");
     float* dtraj = nullptr;
     const size_t ntraj = (size_t)Q.n_steps * nout;

@@ -520,6 +520,9 @@ class BatchHybridEngine:
             _err(L.HB_INVALID_CONFIG, "number of sweep values must equal n_sweep")
         if not 0 <= param.sub_idx < len(subs):
             _err(L.HB_INVALID_CONFIG, "subnetwork index out of range")
+        if self._state is not None and len(self._state[0]) != self.n_sweep:
+            _err(L.HB_INVALID_STATE, f"the engine holds {len(self._state[0])} sweep points (left by run_sweep_multi); "
+                                     f"this run has {self.n_sweep}")
         t0 = time.perf_counter()
         # like the reference's engine, a run continues from the previous run's
         # states, step counter and history (engine.rs:496-540)
@@ -570,8 +573,13 @@ class BatchHybridEngine:
 
     def run_sweep_multi(self, spec: SweepSpec, n_steps: int, exact: bool = False,
                         trajectory: bool = False) -> BatchSweepResult:
+        """The Cartesian product of spec's dimensions (engine.rs:845-851, 862-1348):
+        a fresh run of n_sweep() points from sweep point 0 of the current
+        state at step 0, with empty histories and no monitors, whose states,
+        step and histories the engine then keeps. param_values lists every
+        point's values, dimension by dimension."""
         if self._multi is not None:
-            _err(L.HB_INVALID_CONFIG, "Cartesian sweeps are not supported on the multi-subnetwork path")
+            return self._run_sweep_multi_subnets(spec, n_steps, trajectory)
         arr = (L.HbSweepPoint * len(spec.points))()
         keep = []
         for i, p in enumerate(spec.points):
@@ -581,7 +589,32 @@ class BatchHybridEngine:
         r = L.HbResult()
         flags = (L.FLAG_TRAJECTORY if trajectory else 0) | (L.FLAG_EXACT if exact else 0)
         L.check(L.lib().hb_run_sweep_multi(self._h, arr, len(spec.points), int(n_steps), flags, C.byref(r)))
-        return self._result(r, [pt[0][2] for pt in spec.cartesian_product()], trajectory)
+        return self._result(r, [v for pt in spec.cartesian_product() for (_, _, v) in pt], trajectory)
+
+    def _run_sweep_multi_subnets(self, spec: SweepSpec, n_steps: int, traj: bool) -> BatchSweepResult:
+        from .multi import Subnet, run_multi
+        subs, links, integ, device = self._multi
+        if any(p.kind != 0 for p in spec.points):
+            _err(L.HB_INVALID_CONFIG, "coupling-parameter dimensions are swept on single-network configurations only")
+        n = spec.n_sweep()
+        if n < 1:
+            _err(L.HB_INVALID_CONFIG, "empty sweep spec")
+        t0 = time.perf_counter()
+        # the fresh engine starts from sweep point 0 of the current states (engine.rs:886-897)
+        start = [Subnet(S.model, S.n_nodes, S.params,
+                        np.asarray(self._state[k][0], np.float32) if self._state is not None else S.x0)
+                 for k, S in enumerate(subs)]
+        dims = [(p.sub_idx, p.param_idx, list(p.values)) for p in spec.points]
+        r = run_multi(start, links, dt=self.dt, n_steps=int(n_steps), integrator=integ, nsig=self._nsig,
+                      noise_seed=self._noise_seed, hybrid_integrator=self.hybrid_integrator, n_sweep=n,
+                      sweep_spec=dims, max_delay=int(self._max_delay), record_trajectory=traj, device=device)
+        ms = (time.perf_counter() - t0) * 1e3
+        # engine.rs:1326-1331: the caller's engine takes over states, step and histories
+        self._state, self._hist, self._step = r.final_states, r.hist, int(n_steps)
+        return BatchSweepResult(param_values=[v for pt in spec.cartesian_product() for (_, _, v) in pt],
+                                tavg=[t.ravel() for t in r.tavg], final_states=[f.ravel() for f in r.final_states],
+                                trajectories=[t.ravel() for t in r.trajectories] if traj else None, bold=None,
+                                elapsed_ms=ms, n_sweep=n, kernel_ms=r.kernel_ms)
 
     def checkpoint(self, path: str) -> None:
         """Save the engine state as a HYBURNCK version 3 (batch) file

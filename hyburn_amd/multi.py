@@ -71,13 +71,54 @@ def parse_cvar_map(s: str) -> list:
     return [tuple(int(v) for v in part.split(":")) for part in s.split(",") if part.strip()]
 
 
+def cartesian_points(dims: Sequence[tuple]) -> list:
+    """SweepSpec::cartesian_product (engine.rs:69-90): one [(sub, param, value), ...]
+    per point, the last dimension varying fastest."""
+    import itertools
+    return [[(d[0], d[1], float(v)) for d, v in zip(dims, combo)]
+            for combo in itertools.product(*[list(d[2]) for d in dims])]
+
+
+def cartesian_tables(subnets: Sequence[Subnet], dims: Sequence[tuple]):
+    """hb_multi_spec point_mode / point_values / point_params of a Cartesian
+    sweep: per subnetwork, 1 for a single dimension on its native index, 0
+    for a single other index (the batch dfun ignores it) or none, 2 for two
+    or more dimensions (per-point parameter vectors, out-of-range indices
+    dropped as set_model_params drops them)."""
+    pts = cartesian_points(dims)
+    S = len(subnets)
+    per_sub = [[d for d in dims if d[0] == q] for q in range(S)]
+    modes = []
+    for q, ds in enumerate(per_sub):
+        if len(ds) == 1 and ds[0][1] == L.SWEEP_INDEX[subnets[q].model]:
+            modes.append(1)
+        elif len(ds) >= 2:
+            modes.append(2)
+        else:
+            modes.append(0)
+    pvals = np.zeros((len(pts), S), np.float32)
+    pparams = np.zeros((len(pts), S, L.MAX_PARAMS), np.float32)
+    for b, pt in enumerate(pts):
+        for q in range(S):
+            base = np.asarray(subnets[q].params, np.float32)
+            pparams[b, q, :len(base)] = base
+        for (sub, pidx, v) in pt:
+            if not 0 <= sub < S:
+                raise L.HyburnError(L.HB_INVALID_CONFIG, "sweep subnetwork index out of range")
+            if modes[sub] == 1:
+                pvals[b, sub] = v
+            elif modes[sub] == 2 and pidx < len(subnets[sub].params):
+                pparams[b, sub, pidx] = v
+    return modes, pvals, pparams
+
+
 def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_steps: int,
               integrator: int = L.HEUN, hybrid_integrator: bool = False, n_sweep: int = 1,
               sweep: Optional[tuple] = None, max_delay: int = 0, nsig=None, noise=None,
               noise_seed: int = 0, step0: int = 0, state: Optional[list] = None,
               hist: Optional[np.ndarray] = None, plain: bool = False, stimuli: Sequence[tuple] = (),
               dt_ms: Optional[float] = None, record_trajectory: bool = False, bold_period: int = 0,
-              bold_tr_ms: float = 0.0, device: int = 0) -> MultiResult:
+              bold_tr_ms: float = 0.0, device: int = 0, sweep_spec: Optional[Sequence[tuple]] = None) -> MultiResult:
     """max_delay: the largest entry of any projection's delay list, which sets
     the history horizon (engine.rs:253-257); each link's own delay counts too.
     Stochastic integrators: nsig per variable (a scalar is repeated over the
@@ -98,7 +139,13 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
     engine instead feeds one monitor the mean of var0 over the sweep points
     (engine.rs:785-801); the Balloon-Windkessel model is nonlinear, so with
     n_sweep > 1 these per-point series are not the reference's single series
-    (the single-network engine has that monitor: bold_batch_mean)."""
+    (the single-network engine has that monitor: bold_batch_mean).
+    sweep_spec: run_sweep_multi's Cartesian product (engine.rs:56-91,
+    862-1348), dimensions (sub_idx, param_idx, values), the last varying
+    fastest; n_sweep is the product of their lengths. Per subnetwork, one
+    dimension on the model's batch-native index is the swept form, one on
+    any other index is ignored, and two or more set every point's parameter
+    vector (dfun.rs:38-46, 99-133). It replaces sweep."""
     keep = []
     subs = (L.HbSubnetwork * len(subnets))()
     for k, S in enumerate(subnets):
@@ -152,6 +199,15 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
             raise ValueError("number of sweep values must equal n_sweep")
         keep.append(vals)
         spec.sweep_sub, spec.sweep_param, spec.sweep_values = sub_idx, param_idx, vals.ctypes.data_as(L.fp)
+    if sweep_spec is not None:
+        modes, pvals, pparams = cartesian_tables(subnets, sweep_spec)
+        if pvals.shape[0] != n_sweep:
+            raise ValueError("n_sweep must equal the product of the sweep dimensions' lengths")
+        keep += [pvals, pparams]
+        spec.sweep_sub = -1
+        for q, m in enumerate(modes):
+            spec.point_mode[q] = m
+        spec.point_values, spec.point_params = pvals.ctypes.data_as(L.fp), pparams.ctypes.data_as(L.fp)
     if integrator in (L.EULER_STOCH, L.HEUN_STOCH, L.RK4_STOCH):
         nv0 = L.MODEL_INFO[subnets[0].model][0]
         ns = np.atleast_1d(np.asarray(0.0 if nsig is None else nsig, np.float32))

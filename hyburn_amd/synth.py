@@ -69,6 +69,26 @@ LINEAR_DEFAULT = np.array([-10.0], np.float32)                                  
 SUP_HOPF_DEFAULT = np.array([-0.5, 1.0], np.float32)                                # sup_hopf.rs:35-37
 HOPFIELD_DEFAULT = np.array([1.0, 5.0, 0.0], np.float32)                            # hopfield.rs:36-38
 CB2D_DEFAULT = np.array([1.0, -4.0, 1.0, 2.0], np.float32)                          # coombes_byrne2d.rs:41-43
+EPI_CODIM3_DEFAULT = np.array([-0.02285, 0.3448, 0.2014, -0.07465, 0.3351, 0.2053,
+                               1.0, 0.4, 0.001, 0.3, 0.0, 1.0, 1.0], np.float32)    # epileptor_codim3.rs:51-57
+EPI_CODIM3_SLOWMOD_DEFAULT = np.array([
+    -0.02285, 0.3448, 0.2014, -0.07465, 0.3351, 0.2053, 1.0, 0.4, 0.002, 0.3, 0.0, 1.0, 1.0,
+    0.68281186, -0.13736244, 0.717565, 0.60745907, 0.11524223, 0.785947,
+    0.6668986, -0.2839006, -0.6889461, 0.79066, 0.00755125, -0.6122089, 0.0001, 0.00012],
+    np.float32)                                                                    # epileptor_codim3_slowmod.rs:72-83
+EPI_RS_DEFAULT = np.array([3.1, 0.45, -1.6, 1.0, 3.0, 1.0, 5.0, 0.00035, 0.0, 10.0, 6.0, 2.0, 0.0, 0.0, 0.0, 1.0,
+                           0.0, 1.0, 0.0, -2.0, -10.0, 0.02, 3.0, 1.0, 1.0, 1.0, 1.0, 1.0],
+                          np.float32)                                              # epileptor_rs.rs:72-79
+ZERLAUT_DEFAULT = np.array([
+    10.0, -65.0, -65.0, 200.0, 60.0, 4.0, 0.0, 0.0, 500.0, 1.0, 0.0, -80.0, 1.5, 5.0, 5.0, 5.0, 10000.0,
+    0.05, 0.05, 0.2, 400.0, 0.0, 20.0, 0.0, 0.0, 0.0, 0.0, 5.0, 10.5, 1.0,
+    -0.04983106, 0.005063551, -0.023470122, 0.0022951514, -0.00041053028,
+    0.010547051, -0.03659253, 0.0074374876, 0.0012650647, -0.040721614,
+    -0.05149122, 0.004003689, -0.008352013, 0.0002414238, -0.0005070645,
+    0.0014345394, -0.01468669, 0.004502706, 0.0028472191, -0.0153578045],
+    np.float32)                                        # zerlaut_first.rs:108-128 (zerlaut_second.rs: the same)
+KIONEX_DEFAULT = np.array([0.0, 5.5, 0.1, 0.0, 1.0, -40.0, 0.5, -20.0, -0.5, -31.0, 1.0, 4.0, 0.04, 0.001],
+                          np.float32)                                              # kionex.rs:58-64
 DEFAULTS = {L.G2DO: G2DO_DEFAULT, L.MPR: MPR_DEFAULT, L.JANSEN_RIT: JR_DEFAULT,
             L.WILSON_COWAN: WC_DEFAULT, L.KURAMOTO: KUR_DEFAULT, L.RWW: RWW_DEFAULT}
 

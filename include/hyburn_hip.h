@@ -30,7 +30,13 @@ extern "C" {
 /* Models (config.rs:43-50 registry order of the six batch-native models). */
 enum { HB_G2DO = 0, HB_MPR = 1, HB_JANSEN_RIT = 2, HB_WILSON_COWAN = 3, HB_KURAMOTO = 4, HB_RWW = 5,
        /* only on the multi-subnetwork path (hb_run_multi) */
-       HB_LINEAR = 6, HB_SUP_HOPF = 7, HB_HOPFIELD = 8, HB_COOMBES_BYRNE_2D = 9 };
+       HB_LINEAR = 6, HB_SUP_HOPF = 7, HB_HOPFIELD = 8, HB_COOMBES_BYRNE_2D = 9,
+       /* the models the reference pins with trace fixtures (tests/model_reference_test.rs:90-156),
+        * multi-subnetwork path only: dfun.rs:952-1870 */
+       HB_EPILEPTOR_CODIM3 = 10, HB_EPILEPTOR_CODIM3_SLOWMOD = 11, HB_EPILEPTOR_RS = 12,
+       HB_ZERLAUT_FIRST = 13, HB_ZERLAUT_SECOND = 14, HB_KIONEX = 15 };
+/* Largest parameter vector of any model (the Zerlaut models: 50). */
+#define HB_MAX_PARAMS 64
 /* IntegratorKind (integrator.rs:17-30). */
 enum { HB_EULER = 0, HB_HEUN = 1, HB_RK4 = 2, HB_EULER_STOCH = 3, HB_HEUN_STOCH = 4, HB_RK4_STOCH = 5 };
 /* CouplingFnConfig (coupling.rs:53-71). */
@@ -388,6 +394,17 @@ typedef struct {
     int32_t bold_period;
     double bold_tr_ms;
     float* bold;
+    /* run_sweep_multi over several subnetworks (engine.rs:862-1348; all optional):
+     * per subnetwork, point_mode 0 keeps its parameters for every sweep point;
+     * 1 sweeps its batch-native parameter in the swept form with the value
+     * point_values[point * n_subnets + subnetwork] (one swept dimension on the
+     * model's native index, dfun.rs:38-72); 2 gives every point its own parameter
+     * vector point_params[(point * n_subnets + subnetwork) * HB_MAX_PARAMS + k]
+     * (two or more swept dimensions, dfun_batch_multi_fallback, dfun.rs:99-133).
+     * A nonzero point_mode replaces sweep_sub / sweep_param for that run. */
+    int32_t point_mode[4];
+    const float* point_values;
+    const float* point_params;
 } hb_multi_spec;
 
 /* Rows of BOLD output a run of n_steps produces (bold_monitor.rs:72-227): native

@@ -27,7 +27,9 @@
 #include <omp.h>
 #endif
 
-enum { ORC_G2DO = 0, ORC_MPR = 1, ORC_JR = 2, ORC_WC = 3, ORC_KUR = 4, ORC_RWW = 5, ORC_LIN = 6, ORC_SH = 7, ORC_HOP = 8, ORC_CB2 = 9 };
+enum { ORC_G2DO = 0, ORC_MPR = 1, ORC_JR = 2, ORC_WC = 3, ORC_KUR = 4, ORC_RWW = 5, ORC_LIN = 6, ORC_SH = 7, ORC_HOP = 8, ORC_CB2 = 9,
+       ORC_EC3 = 10, ORC_EC3S = 11, ORC_ERS = 12, ORC_ZL1 = 13, ORC_ZL2 = 14, ORC_KIX = 15 };
+enum { ORC_NMODELS = 16, ORC_MAXP = 64 };
 enum { ORC_EULER = 0, ORC_HEUN = 1, ORC_RK4 = 2, ORC_EULER_STOCH = 3, ORC_HEUN_STOCH = 4, ORC_RK4_STOCH = 5 };
 enum { ORC_CF_LINEAR = 0, ORC_CF_DIFFERENCE = 1, ORC_CF_KURAMOTO = 2, ORC_CF_SIGMOIDAL = 3,
        ORC_CF_SCALED_LINEAR = 4, ORC_CF_TANH = 5, ORC_CF_SJR = 6, ORC_CF_PRESIGMOIDAL = 7 };
@@ -76,13 +78,14 @@ typedef struct {
 } hb_orc_problem;
 
 /* model registry, config.rs:43-50 */
-static int hb_orc_nvar(int m) { static const int v[10] = {2, 2, 6, 2, 1, 1, 1, 2, 2, 2}; return v[m]; }
-static int hb_orc_ncvar(int m) { static const int v[10] = {1, 2, 1, 1, 1, 1, 1, 2, 2, 2}; return v[m]; }
-static int hb_orc_nparams(int m) { static const int v[10] = {12, 7, 13, 22, 1, 8, 1, 2, 3, 4}; return v[m]; }
+static int hb_orc_nvar(int m) { static const int v[ORC_NMODELS] = {2, 2, 6, 2, 1, 1, 1, 2, 2, 2, 3, 5, 8, 5, 8, 5}; return v[m]; }
+static int hb_orc_ncvar(int m) { static const int v[ORC_NMODELS] = {1, 2, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1, 3, 1, 1, 1}; return v[m]; }
+/* registry rows of config.rs:43-72 (EpileptorCodim3 ... KIonEx: 10..15) */
+static int hb_orc_nparams(int m) { static const int v[ORC_NMODELS] = {12, 7, 13, 22, 1, 8, 1, 2, 3, 4, 13, 27, 28, 50, 50, 14}; return v[m]; }
 
 int hb_orc_model_info(int m, int32_t* nvar, int32_t* ncvar, int32_t* nparams)
 {
-    if (m < 0 || m > 9) return -1;
+    if (m < 0 || m >= ORC_NMODELS) return -1;
     *nvar = hb_orc_nvar(m); *ncvar = hb_orc_ncvar(m); *nparams = hb_orc_nparams(m);
     return 0;
 }
@@ -99,6 +102,7 @@ int hb_orc_model_info(int m, int32_t* nvar, int32_t* ncvar, int32_t* nparams)
 #define POWR powf
 #define SQRTR sqrtf
 #define FLOORR floorf
+#define LOGR logf
 #include "hb_oracle_body.h"
 #undef REAL
 #undef FN
@@ -111,6 +115,7 @@ int hb_orc_model_info(int m, int32_t* nvar, int32_t* ncvar, int32_t* nparams)
 #undef POWR
 #undef SQRTR
 #undef FLOORR
+#undef LOGR
 
 /* ---------------- FP64 twin ---------------- */
 #define REAL double
@@ -124,6 +129,7 @@ int hb_orc_model_info(int m, int32_t* nvar, int32_t* ncvar, int32_t* nparams)
 #define POWR pow
 #define SQRTR sqrt
 #define FLOORR floor
+#define LOGR log
 #include "hb_oracle_body.h"
 
 /* ---------------- exported API ---------------- */

@@ -7,6 +7,7 @@ baseline; the product path never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import math
 import os
 import subprocess
@@ -357,7 +358,8 @@ def fcd(traj, window_len, stride):
     return out if rc > 1 else np.zeros((0, 0), np.float32)
 
 
-_CVAR = {0: (0,), 1: (0, 1), 2: (1,), 3: (0,), 4: (0,), 5: (0,), 6: (0,), 7: (0, 1), 8: (0, 1), 9: (0, 1)}   # CVAR of src/model/*.rs
+_CVAR = {0: (0,), 1: (0, 1), 2: (1,), 3: (0,), 4: (0,), 5: (0,), 6: (0,), 7: (0, 1), 8: (0, 1), 9: (0, 1),
+         10: (0,), 11: (0,), 12: (0, 3, 6), 13: (0,), 14: (0,), 15: (0,)}   # CVAR of src/model/*.rs
 _IDENTITY_PRE = (0, 1, 3, 4)   # Linear, Difference, Sigmoidal, ScaledLinear: pre is the identity
 
 
@@ -430,7 +432,8 @@ def bold_series(v0, P, dt_ms, tr_ms, dtype=np.float32):
 
 
 def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_sweep=1, max_delay=0, nsig=None,
-              noise=None, plain=False, stimuli=(), dt_ms=None, bold_period=0, bold_tr_ms=0.0, dtype=np.float32):
+              noise=None, plain=False, stimuli=(), dt_ms=None, bold_period=0, bold_tr_ms=0.0, dtype=np.float32,
+              sweep_spec=None):
     """The batch engine's multi-subnetwork loop (engine.rs:540-823) in f32, one
     sweep point at a time: coupling of every projection from the current
     states (engine.rs:557-673), then each subnetwork's integration with that
@@ -448,7 +451,13 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
     their target (runtime.rs:319-357), RK4-stochastic is clamp(det) + Z
     (integrator.rs:296-315). bold_period > 0: BOLD of subnetwork 0's variable
     0 per sweep point (bold_series). Returns per subnetwork tavg and final
-    [n_sweep][n][nvar], plus the BOLD [n_sweep][rows][n0] when monitored."""
+    [n_sweep][n][nvar], plus the BOLD [n_sweep][rows][n0] when monitored.
+    sweep_spec: run_sweep_multi's dimensions (sub_idx, param_idx, values),
+    expanded as SweepSpec::cartesian_product (engine.rs:69-90, last fastest);
+    per subnetwork the dispatch of dfun.rs:38-46: one dimension is the
+    swept form on the model's native index and ignored elsewhere, two or
+    more override that point's parameters (dfun_batch_multi_fallback,
+    dfun.rs:99-133)."""
     from hyburn_amd import _lib as HL
     f32 = np.float32
     fl = dtype   # float64: the FP64 twin (same op order; weights, params and raw noise stay f32 values)
@@ -494,10 +503,22 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
         x = [np.array(x0, fl).reshape(n, info[k][0]) for k, (_, n, _, x0) in enumerate(subnets)]
         acc = [np.zeros_like(v) for v in x]
         sw = [0] * S
-        sv = 0.0
-        if sweep is not None:
+        svs = [0.0] * S
+        prm = [np.asarray(sp[2], np.float32) for sp in subnets]
+        if sweep_spec is not None:
+            combo = list(itertools.product(*[list(d[2]) for d in sweep_spec]))[b]
+            for k in range(S):
+                mine = [(d[1], float(v)) for d, v in zip(sweep_spec, combo) if d[0] == k]
+                if len(mine) == 1 and mine[0][0] == HL.SWEEP_INDEX[subnets[k][0]]:
+                    sw[k], svs[k] = 1, mine[0][1]
+                elif len(mine) >= 2:
+                    prm[k] = prm[k].copy()
+                    for (pi, v) in mine:
+                        if pi < len(prm[k]):
+                            prm[k][pi] = v
+        elif sweep is not None:
             sub_idx, param_idx, values = sweep
-            sv = float(values[b])
+            svs = [float(values[b])] * S
             if param_idx == HL.SWEEP_INDEX[subnets[sub_idx][0]]:
                 sw[sub_idx] = 1
         hist = [[x[k].copy()] + [np.zeros_like(x[k]) for _ in range(H - 1)] for k in range(S)] if H else None
@@ -565,11 +586,13 @@ def run_multi(subnets, links, dt, n_steps, integrator, hybrid, sweep=None, n_swe
                         base = base.copy()
                         base[:, 0] = (base[:, 0] + v).astype(fl)
                         cpl[target] = base
-            for k, (m, n, params, _) in enumerate(subnets):
+            for k, (m, n, _, _) in enumerate(subnets):
+                params, sv = prm[k], svs[k]
                 c = cpl[k] if cpl[k] is not None else np.zeros((n, info[k][1]), fl)
-                heun = (m == HL.G2DO) if hybrid else (integrator == HL.HEUN)
-                stoch = (not (hybrid and m == HL.G2DO)) and integrator in (HL.EULER_STOCH, HL.HEUN_STOCH,
-                                                                           HL.RK4_STOCH)
+                # hybrid_integrator: deterministic Heun for model_prefers_heun (dfun.rs:1871-1873)
+                heun = (m in HL.PREFERS_HEUN) if hybrid else (integrator == HL.HEUN)
+                stoch = (not (hybrid and m in HL.PREFERS_HEUN)) and integrator in (HL.EULER_STOCH, HL.HEUN_STOCH,
+                                                                                   HL.RK4_STOCH)
                 if stoch:
                     scale = np.array([fl(np.sqrt(fl(2.0) * fl(nsig[v]))) * fl(np.sqrt(dtf))
                                       for v in range(info[k][0])], fl)

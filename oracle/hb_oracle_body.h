@@ -28,18 +28,96 @@ typedef struct {
     int p_dyn;
     /* RWW   (dfun.rs:494-516) */
     REAL r_a, r_b, r_d, r_gamma, r_inv_taus, r_wjn, r_jn, r_io;
+    /* EpileptorCodim3 ... KIonEx (models 10-15): raw parameters and the
+     * host-side products the reference forms from them, FN(ext_consts) */
+    REAL xc[64];
     int sweep_form;   /* 1: reference's batch-native swept form (dfun.rs:49-72) */
     REAL sweep_val;
 } FN(mconst);
+
+/* Constants of models 10-15 (slot layout shared with the GPU's constant
+ * table): slots 0..nparams-1 are the parameters, the products the reference
+ * evaluates once per call on the host follow. */
+static void FN(ext_consts)(REAL* c, int model, const REAL* p)
+{
+    int i;
+    const REAL eps = EPSR;
+    for (i = 0; i < 64; ++i) c[i] = 0;
+    for (i = 0; i < hb_orc_nparams(model); ++i) c[i] = p[i];
+    switch (model) {
+    case ORC_EC3: { /* dfun.rs:958-983: unit start/stop points, the great circle's e and f */
+        REAL st[3], sp[3], a[3], b[3], cr[3], cca[3], sn, pn, cn, cc2;
+        st[0] = p[1]; st[1] = -p[0]; st[2] = p[2];
+        sp[0] = p[4]; sp[1] = -p[3]; sp[2] = p[5];
+        sn = SQRTR(((REAL)0 + st[0] * st[0] + st[1] * st[1]) + st[2] * st[2]);
+        pn = SQRTR(((REAL)0 + sp[0] * sp[0] + sp[1] * sp[1]) + sp[2] * sp[2]);
+        for (i = 0; i < 3; ++i) { a[i] = st[i] / (sn > eps ? sn : eps); b[i] = sp[i] / (pn > eps ? pn : eps); }
+        cr[0] = a[1] * b[2] - a[2] * b[1];
+        cr[1] = a[2] * b[0] - a[0] * b[2];
+        cr[2] = a[0] * b[1] - a[1] * b[0];
+        cn = SQRTR(((REAL)0 + cr[0] * cr[0] + cr[1] * cr[1]) + cr[2] * cr[2]);
+        for (i = 0; i < 3; ++i) c[13 + i] = a[i];
+        if (cn > eps) {
+            for (i = 0; i < 3; ++i) cca[i] = cr[(i + 1) % 3] * a[(i + 2) % 3] - cr[(i + 2) % 3] * a[(i + 1) % 3];
+            cc2 = SQRTR(((REAL)0 + cca[0] * cca[0] + cca[1] * cca[1]) + cca[2] * cca[2]);
+            cc2 = cc2 > eps ? cc2 : eps;
+            for (i = 0; i < 3; ++i) c[16 + i] = cca[i] / cc2;
+        } else {
+            c[16] = 0; c[17] = 1; c[18] = 0;
+        }
+        break; }
+    case ORC_EC3S: /* dfun.rs:1063-1074: g*R, h*R, l*R, m*R */
+        for (i = 0; i < 3; ++i) {
+            c[27 + i] = p[13 + i] * p[7]; c[30 + i] = p[19 + i] * p[7];
+            c[33 + i] = p[16 + i] * p[7]; c[36 + i] = p[22 + i] * p[7];
+        }
+        break;
+    case ORC_ERS: /* dfun.rs:1143-1208 */
+        c[17] = p[17] > eps ? p[17] : eps;
+        c[28] = (REAL)-4.0 * p[2];
+        c[29] = p[15] * p[7];
+        c[30] = (REAL)-0.01 * p[15];
+        c[31] = p[21] * c[17];
+        c[32] = p[21] / c[17];
+        c[33] = p[26] * p[18];
+        c[34] = p[26] * p[27];
+        break;
+    case ORC_ZL1: case ORC_ZL2: /* dfun.rs:1449-1620 host products */
+        c[50] = (((REAL)1 - p[19]) * p[17]) * p[16];
+        c[51] = (p[19] * p[18]) * p[16];
+        c[52] = p[12] * p[14];
+        c[53] = p[13] * p[15];
+        c[54] = p[0] * p[1];
+        c[55] = p[0] * p[2];
+        c[56] = p[0] / p[3];
+        c[57] = (REAL)1 / p[22];
+        c[58] = p[16] * ((REAL)1 - p[19]);
+        c[59] = p[16] * p[19];
+        c[60] = (REAL)1 / (((REAL)2 * (REAL)1e-4f) * (REAL)1e3f);
+        c[61] = (REAL)1 / ((((REAL)1e-4f * (REAL)1e3f) * (REAL)1e-4f) * (REAL)1e3f);
+        break;
+    case ORC_KIX: { /* dfun.rs:1788-1849 */
+        REAL pi = PIR;
+        c[10] = p[10] > eps ? p[10] : eps;
+        c[11] = p[11] > eps ? p[11] : eps;
+        c[14] = (REAL)-1 / c[10];
+        c[15] = p[6] / pi;
+        c[16] = (REAL)2 * p[6];
+        c[17] = (REAL)2 * p[8];
+        c[18] = -(p[12] / (REAL)2160);
+        c[19] = (REAL)26.64f * (REAL)LOGR((REAL)112 / (REAL)5);
+        break; }
+    }
+}
 
 /* Build the constants from a parameter vector (f32 values, as in the config).
  * The FP32 twin evaluates each host-side constant in f32 with the reference's
  * operation order; the FP64 twin uses the same expressions in double. */
 static void FN(mconst_init)(FN(mconst)* k, int model, const float* pf, int sweep_form, float sweep_val)
 {
-    REAL p[32];
+    REAL p[ORC_MAXP];
     int i;
-    for (i = 0; i < 32; ++i) p[i] = 0;
+    for (i = 0; i < ORC_MAXP; ++i) p[i] = 0;
     for (i = 0; i < hb_orc_nparams(model); ++i) p[i] = (REAL)pf[i];
     memset(k, 0, sizeof(*k));
     k->model = model;
@@ -110,6 +188,9 @@ static void FN(mconst_init)(FN(mconst)* k, int model, const float* pf, int sweep
         k->r_inv_taus = (REAL)1 / p[4];
         k->r_wjn = p[5] * p[6]; k->r_jn = p[6]; k->r_io = p[7];
         break;
+    default:
+        FN(ext_consts)(k->xc, model, p);
+        break;
     }
 }
 
@@ -119,6 +200,292 @@ static REAL FN(jr_sigm)(const FN(mconst)* k, REAL v)
     REAL arg = (-v + k->j_v0) * k->j_r;
     REAL den = EXPR(arg) + (REAL)1;
     return k->j_2nu / den;
+}
+
+/* ---- models 10-15 (EpileptorCodim3 ... KIonEx), dfun.rs:952-1870 ---- */
+
+static REAL FN(sgn)(REAL x) { return x > 0 ? (REAL)1 : (x < 0 ? (REAL)-1 : (REAL)0); }
+
+/* atan_approx, dfun.rs:1403-1416: reduce |x| > 1 to 1/x, a rational start,
+ * 12 Newton steps on sin(t) - x cos(t), pi/2 - t for the reduced inputs. */
+static REAL FN(atan_ap)(REAL x)
+{
+    REAL ax = x < 0 ? -x : x, xe, x2, th;
+    int red = ax > (REAL)1, it;
+    xe = red ? (REAL)1 / x : x;
+    x2 = xe * xe;
+    th = xe / (x2 * (REAL)0.28f + (REAL)1);
+    for (it = 0; it < 12; ++it) {
+        REAL sn = SINR(th), cs = COSR(th);
+        th = th - (sn - xe * cs) * cs;
+    }
+    return red ? -th + (REAL)1.57079632679489661923f : th;
+}
+
+/* acos_approx, dfun.rs:1418-1422 */
+static REAL FN(acos_ap)(REAL x)
+{
+    REAL xc = x < (REAL)-0.9999f ? (REAL)-0.9999f : (x > (REAL)0.9999f ? (REAL)0.9999f : x);
+    REAL ratio = (-xc + (REAL)1) / (xc + (REAL)1);
+    return FN(atan_ap)(SQRTR(ratio)) * (REAL)2;
+}
+
+/* erf_approx, dfun.rs:1424-1446 (Abramowitz-Stegun 7.1.26) */
+static REAL FN(erf_ap)(REAL x)
+{
+    REAL ax = x < 0 ? -x : x;
+    REAL t = (REAL)1 / (ax * (REAL)0.3275911f + (REAL)1);
+    REAL t2 = t * t, t3 = t2 * t, t4 = t3 * t, t5 = t4 * t;
+    REAL y = (((t5 * (REAL)1.061405429f + t4 * (REAL)-1.453152027f) + t3 * (REAL)1.421413741f) +
+              t2 * (REAL)-0.284496736f) + t * (REAL)0.254829592f;
+    REAL ev = EXPR((ax * (REAL)-1) * ax);
+    return FN(sgn)(x) * (-(y * ev) + (REAL)1);
+}
+
+/* The fixed point xs on the slow manifold (dfun.rs:996-1019, 1100-1118):
+ * trigonometric root when the discriminant is negative, Cardano otherwise. */
+static REAL FN(ec3_xs)(REAL mu1, REAL mu2, int branch)
+{
+    REAL disc = (mu1 * (REAL)0.25f) * mu1 - (mu2 * mu2) * (mu2 * ((REAL)1.0f / (REAL)27.0f));
+    REAL m3 = mu2 / (REAL)3, sr, r32, c3, th, xt, sd, h, pa, na, third, cp, cn;
+    const REAL tp3 = (REAL)2.0943951023931954923f;   /* 2 pi / 3 in f32 */
+    sr = SQRTR(m3 > EPSR ? m3 : EPSR);
+    r32 = m3 * sr;
+    c3 = (mu1 / (REAL)2) / (r32 > EPSR ? r32 : EPSR);
+    c3 = c3 < (REAL)-1 ? (REAL)-1 : (c3 > (REAL)1 ? (REAL)1 : c3);
+    th = FN(acos_ap)(c3) / (REAL)3;
+    if (branch == 2) xt = (sr * (REAL)2) * COSR(th - tp3);
+    else if (branch == 3) xt = (sr * (REAL)2) * COSR(th + tp3);
+    else xt = (sr * (REAL)2) * COSR(th);
+    sd = SQRTR(disc > 0 ? disc : (REAL)0);
+    h = mu1 / (REAL)2;
+    pa = h + sd;
+    na = h - sd;
+    third = (REAL)1.0f / (REAL)3.0f;
+    cp = FN(sgn)(pa) * POWR(pa < 0 ? -pa : pa, third);
+    cn = FN(sgn)(na) * POWR(na < 0 ? -na : na, third);
+    return disc < 0 ? xt : cp + cn;
+}
+
+/* z - 0.5 to the seventh, times 0.1 (the modification term, dfun.rs:1024, 1124) */
+static REAL FN(ec3_mod)(REAL z)
+{
+    REAL zs = z + (REAL)-0.5f;
+    return ((((((zs * zs) * zs) * zs) * zs) * zs) * zs) * (REAL)0.1f;
+}
+
+/* get_fluct_regime_vars, dfun.rs:1449-1488 */
+static void FN(zl_fluct)(const REAL* c, REAL fe, REAL fi, REAL fe_ext, REAL fi_ext, REAL w, REAL gle,
+                         REAL* mu_v, REAL* sigma_v, REAL* t_v)
+{
+    REAL fe_tot = (fe + (REAL)1e-6f) * c[50] + fe_ext * c[20];
+    REAL fi_tot = (fi + (REAL)1e-6f) * c[51] + fi_ext * c[21];
+    REAL ge = fe_tot * c[52], gi = fi_tot * c[53];
+    REAL mu_g = (ge + gi) + c[0];
+    REAL t_m = ((REAL)1 / mu_g) * c[3];
+    REAL ue, ui, uet, uit, ue2, ui2, sv2, num, den;
+    *mu_v = (((ge * c[10] + gi * c[11]) + (-w)) + gle) / mu_g;
+    ue = ((-*mu_v + c[10]) * c[12]) / mu_g;
+    ui = ((-*mu_v + c[11]) * c[13]) / mu_g;
+    uet = ue * c[14]; uit = ui * c[15];
+    ue2 = uet * uet; ui2 = uit * uit;
+    sv2 = (fe_tot * ue2) / ((t_m + c[14]) * (REAL)2) + (fi_tot * ui2) / ((t_m + c[15]) * (REAL)2);
+    *sigma_v = SQRTR(sv2);
+    if (!(*sigma_v >= (REAL)1e-10f)) *sigma_v = *sigma_v != *sigma_v ? *sigma_v : (REAL)1e-10f;
+    num = fe_tot * ue2 + fi_tot * ui2;
+    den = (fe_tot * ue2) / (t_m + c[14]) + (fi_tot * ui2) / (t_m + c[15]);
+    if (!(den >= (REAL)1e-10f)) den = den != den ? den : (REAL)1e-10f;
+    *t_v = num / den;
+}
+
+/* TF: threshold polynomial (1490-1513), firing rate (1515-1523), dfun.rs:1525-1544 */
+static REAL FN(zl_tf)(const REAL* c, REAL fe, REAL fi, REAL fe_ext, REAL fi_ext, REAL w, REAL gle, const REAL* P)
+{
+    REAL mu_v, sigma_v, t_v, tvn, v, sg, t, thr, arg, den;
+    FN(zl_fluct)(c, fe, fi, fe_ext, fi_ext, w, gle, &mu_v, &sigma_v, &t_v);
+    tvn = t_v * c[56];
+    v = (mu_v + (REAL)60) / (REAL)10;
+    sg = (sigma_v + (REAL)-4) / (REAL)6;
+    t = tvn + (REAL)-0.5f;
+    thr = (v * v) * P[4] + (sg * sg) * P[5];
+    thr = thr + (t * t) * P[6];
+    thr = thr + (v * sg) * P[7];
+    thr = thr + (v * t) * P[8];
+    thr = thr + (sg * t) * P[9];
+    thr = thr + v * P[1];
+    thr = thr + sg * P[2];
+    thr = thr + t * P[3];
+    thr = thr + P[0];
+    thr = thr * (REAL)1000;
+    arg = (thr - mu_v) / (sigma_v * (REAL)1.41421356237309504880f);
+    den = t_v * (REAL)2;
+    den = den > EPSR ? den : (den != den ? den : EPSR);
+    return (-FN(erf_ap)(arg) + (REAL)1) / den;
+}
+
+/* KIonEx logistic helpers: 1 / (exp(-u) + 1) */
+static REAL FN(kx_sig)(REAL u) { return (REAL)1 / (EXPR(-u) + (REAL)1); }
+static REAL FN(kx_ln)(REAL u) { REAL l = LOGR(u); return l > (REAL)1e-10f ? l : (l != l ? l : (REAL)1e-10f); }
+
+static void FN(dfun_ext)(const FN(mconst)* k, const REAL* x, const REAL* cpl, REAL* dx)
+{
+    const REAL* c = k->xc;
+    switch (k->model) {
+    case ORC_EC3: { /* dfun.rs:985-1032 */
+        REAL cz = COSR(x[2]), sz = SINR(x[2]);
+        REAL mu2 = (cz * c[13] + sz * c[16]) * c[7];
+        REAL mu1 = (cz * c[14] + sz * c[17]) * (-c[7]);
+        REAL nu = (cz * c[15] + sz * c[18]) * c[7];
+        REAL xs = FN(ec3_xs)(mu1, mu2, (int)c[11]);
+        REAL dd = SQRTR((x[0] - xs) * (x[0] - xs) + x[1] * x[1]);
+        REAL t = -(-dd + c[9]);
+        dx[0] = -x[1];
+        dx[1] = ((((x[0] * x[0]) * x[0] - mu2 * x[0]) - mu1) - x[1] * ((nu + x[0] * c[6]) + x[0] * x[0]));
+        if (c[12] > (REAL)0.5f) t = t + FN(ec3_mod)(x[2]);
+        dx[2] = -((t + cpl[0] * c[10]) * c[8]);
+        break; }
+    case ORC_EC3S: { /* dfun.rs:1035-1134 */
+        REAL cua = COSR(x[3]), sua = SINR(x[3]), cub = COSR(x[4]), sub = SINR(x[4]);
+        REAL a0 = cua * c[27] + sua * c[30], a1 = cua * c[28] + sua * c[31], a2 = cua * c[29] + sua * c[32];
+        REAL b0 = cub * c[33] + sub * c[36], b1 = cub * c[34] + sub * c[37], b2 = cub * c[35] + sub * c[38];
+        REAL an = SQRTR((a0 * a0 + a1 * a1) + a2 * a2), e0, e1, e2, c0, c1, c2, f0, f1, f2, fn_;
+        REAL cz, sz, mu2, mu1, nu, xs, dd, t;
+        an = an > EPSR ? an : (an != an ? an : EPSR);
+        e0 = a0 / an; e1 = a1 / an; e2 = a2 / an;
+        c0 = a1 * b2 - a2 * b1; c1 = a2 * b0 - a0 * b2; c2 = a0 * b1 - a1 * b0;
+        f0 = c1 * a2 - c2 * a1; f1 = c2 * a0 - c0 * a2; f2 = c0 * a1 - c1 * a0;
+        fn_ = SQRTR((f0 * f0 + f1 * f1) + f2 * f2);
+        fn_ = fn_ > EPSR ? fn_ : (fn_ != fn_ ? fn_ : EPSR);
+        f0 = f0 / fn_; f1 = f1 / fn_; f2 = f2 / fn_;
+        cz = COSR(x[2]); sz = SINR(x[2]);
+        mu2 = (cz * e0 + sz * f0) * c[7];
+        mu1 = -((cz * e1 + sz * f1) * c[7]);
+        nu = (cz * e2 + sz * f2) * c[7];
+        xs = FN(ec3_xs)(mu1, mu2, (int)c[11]);
+        dx[0] = -x[1];
+        dx[1] = ((((x[0] * x[0]) * x[0] - mu2 * x[0]) - mu1) - x[1] * ((nu + x[0] * c[6]) + x[0] * x[0]));
+        dd = SQRTR((x[0] - xs) * (x[0] - xs) + x[1] * x[1]);
+        t = -(-dd + c[9]) + (c[12] > (REAL)0.5f ? FN(ec3_mod)(x[2]) : (REAL)0);
+        dx[2] = -((t + cpl[0] * c[10]) * c[8]);
+        dx[3] = (REAL)0 + c[25];
+        dx[4] = (REAL)0 + c[26];
+        break; }
+    case ORC_ERS: { /* dfun.rs:1137-1209 */
+        REAL x1 = x[0], y1 = x[1], z = x[2], x2 = x[3], y2 = x[4], g = x[5], xr = x[6], yr = x[7];
+        REAL f1, z4, zm, yz, h, f2, xp;
+        if (x1 < 0) f1 = (-(x1 * x1)) * c[3] + x1 * c[4];
+        else { z4 = z + (REAL)-4; f1 = ((-x2) + c[8]) + (z4 * z4) * (REAL)0.6f; }
+        dx[0] = ((((y1 - z) + cpl[0] * c[12]) + f1 * x1) + c[0]) * c[15];
+        dx[1] = (((-y1) + c[5]) - (x1 * x1) * c[6]) * c[15];
+        zm = z * (REAL)-0.1f;
+        yz = z < 0 ? -((((((zm * zm) * zm) * zm) * zm) * zm) * zm) : (REAL)0;
+        if (c[16] > (REAL)0.5f) h = (x1 + (REAL)3) / (EXPR(-((x1 + (REAL)0.5f) * (REAL)10)) + (REAL)1) + yz;
+        else h = (x1 * (REAL)4 + c[28]) + yz;
+        dx[2] = ((h - z) + cpl[0] * c[14]) * c[29];
+        dx[3] = ((((((x2 - (x2 * x2) * x2) - y2) + c[1]) + g * c[11]) - (z + (REAL)-3.5f) * (REAL)0.3f) +
+                 cpl[1] * c[13]) * c[15];
+        xp = x2 + (REAL)0.25f;
+        f2 = xp < 0 ? (REAL)0 : xp * c[10];
+        dx[4] = ((f2 - y2) * c[15]) / c[9];
+        dx[5] = ((-x1) * (REAL)0.1f + g) * c[30];
+        dx[6] = ((((yr * c[24] - ((xr * xr) * xr) * c[23]) + (xr * xr) * c[22]) + c[33]) + cpl[2] * c[34]) * c[31];
+        dx[7] = ((xr * c[20] - yr * c[25]) + c[19]) * c[32];
+        break; }
+    case ORC_ZL1: { /* dfun.rs:1546-1608 */
+        REAL e = x[0], i = x[1], we = x[2], wi = x[3], ou = x[4];
+        REAL fx = cpl[0] + ou * c[28], tfe, tfi, mve, mvi, s_, t_;
+        if (fx * c[20] < 0) fx = 0;
+        tfe = FN(zl_tf)(c, e, i, fx + c[23], (REAL)0 + c[24], we, c[54], c + 30);
+        tfi = FN(zl_tf)(c, e, i, fx + c[25], (REAL)0 + c[26], wi, c[55], c + 40);
+        dx[0] = (tfe - e) / c[22];
+        dx[1] = (tfi - i) / c[22];
+        FN(zl_fluct)(c, e, i, fx + c[23], (REAL)0 + c[24], we, c[54], &mve, &s_, &t_);
+        dx[2] = ((-we) / c[8] + e * c[4]) + ((mve + (-c[1])) * c[5]) / c[8];
+        FN(zl_fluct)(c, e, i, fx + c[25], (REAL)0 + c[26], wi, c[55], &mvi, &s_, &t_);
+        dx[3] = ((-wi) / c[9] + i * c[6]) + ((mvi + (-c[2])) * c[7]) / c[9];
+        dx[4] = (-ou) / c[27];
+        break; }
+    case ORC_ZL2: { /* dfun.rs:1610-1780 */
+        REAL e = x[0], i = x[1], cee = x[2], cei = x[3], cii = x[4], we = x[5], wi = x[6], ou = x[7];
+        REAL ex = cpl[0] + (ou * c[28] + c[23]), ix, iex = (REAL)0 + c[24], iix = (REAL)0 + c[26];
+        REAL df = (REAL)1e-4f, ep = e + df, em = e + (-df), ip = i + df, im = i + (-df);
+        REAL tfe, tfi, e_p0, e_n0, e_0p, e_0n, e_pp, e_pn, e_np, e_nn, i_p0, i_n0, i_0p, i_0n, i_pp, i_pn, i_np, i_nn;
+        REAL a2 = c[60], q2 = c[61], dfe_e, dfi_e, dfe_i, dfi_i, fefe_e, fifi_e, fefi_e, fife_e, fefe_i, fifi_i,
+             fefi_i, fife_i, mve, mvi, s_, t_;
+        if (ex < 0) ex = 0;
+        ix = cpl[0] * c[29] + (ou * c[28] + c[25]);
+        if (ix < 0) ix = 0;
+        tfe = FN(zl_tf)(c, e, i, ex, iex, we, c[54], c + 30);
+        tfi = FN(zl_tf)(c, e, i, ix, iix, wi, c[55], c + 40);
+        e_p0 = FN(zl_tf)(c, ep, i, ex, iex, we, c[54], c + 30);
+        e_n0 = FN(zl_tf)(c, em, i, ex, iex, we, c[54], c + 30);
+        e_0p = FN(zl_tf)(c, e, ip, ex, iex, we, c[54], c + 30);
+        e_0n = FN(zl_tf)(c, e, im, ex, iex, we, c[54], c + 30);
+        e_pp = FN(zl_tf)(c, ep, ip, ex, iex, we, c[54], c + 30);
+        e_pn = FN(zl_tf)(c, ep, im, ex, iex, we, c[54], c + 30);
+        e_np = FN(zl_tf)(c, em, ip, ex, iex, we, c[54], c + 30);
+        e_nn = FN(zl_tf)(c, em, im, ex, iex, we, c[54], c + 30);
+        i_p0 = FN(zl_tf)(c, ep, i, ix, iix, wi, c[55], c + 40);
+        i_n0 = FN(zl_tf)(c, em, i, ix, iix, wi, c[55], c + 40);
+        i_0p = FN(zl_tf)(c, e, ip, ix, iix, wi, c[55], c + 40);
+        i_0n = FN(zl_tf)(c, e, im, ix, iix, wi, c[55], c + 40);
+        i_pp = FN(zl_tf)(c, ep, ip, ix, iix, wi, c[55], c + 40);
+        i_pn = FN(zl_tf)(c, ep, im, ix, iix, wi, c[55], c + 40);
+        i_np = FN(zl_tf)(c, em, ip, ix, iix, wi, c[55], c + 40);
+        i_nn = FN(zl_tf)(c, em, im, ix, iix, wi, c[55], c + 40);
+        dfe_e = (e_p0 - e_n0) * a2; dfi_e = (e_0p - e_0n) * a2;
+        dfe_i = (i_p0 - i_n0) * a2; dfi_i = (i_0p - i_0n) * a2;
+        fefe_e = ((e_p0 - tfe * (REAL)2) + e_n0) * q2;
+        fifi_e = ((e_0p - tfe * (REAL)2) + e_0n) * q2;
+        fefi_e = ((e_pp - e_np) * a2 - (e_pn - e_nn) * a2) * a2;
+        fife_e = ((e_pp - e_pn) * a2 - (e_np - e_nn) * a2) * a2;
+        fefe_i = ((i_p0 - tfi * (REAL)2) + i_n0) * q2;
+        fifi_i = ((i_0p - tfi * (REAL)2) + i_0n) * q2;
+        fefi_i = ((i_pp - i_np) * a2 - (i_pn - i_nn) * a2) * a2;
+        fife_i = ((i_pp - i_pn) * a2 - (i_np - i_nn) * a2) * a2;
+        dx[0] = (((((tfe - e) + cee * (fefe_e * (REAL)0.5f)) + cei * (fefi_e * (REAL)0.5f)) + cei * (fife_e * (REAL)0.5f)) +
+                 cii * (fifi_e * (REAL)0.5f)) / c[22];
+        dx[1] = (((((tfi - i) + cee * (fefe_i * (REAL)0.5f)) + cei * (fefi_i * (REAL)0.5f)) + cei * (fife_i * (REAL)0.5f)) +
+                 cii * (fifi_i * (REAL)0.5f)) / c[22];
+        dx[2] = ((((tfe * ((c[57] - tfe) / c[58]) + (tfe - e) * (tfe - e)) + cee * (dfe_e * (REAL)2)) +
+                  cei * (dfi_e * (REAL)2)) - cee * (REAL)2) / c[22];
+        dx[3] = (((((((tfe - e) * (tfi - i) + cee * dfe_e) + cei * dfe_i) + cei * dfi_e) + cii * dfi_i)) -
+                 cei * (REAL)2) / c[22];
+        dx[4] = ((((tfi * ((c[57] - tfi) / c[59]) + (tfi - i) * (tfi - i)) + cii * (dfi_i * (REAL)2)) +
+                  cei * (dfe_i * (REAL)2)) - cii * (REAL)2) / c[22];
+        FN(zl_fluct)(c, e, i, ex, iex, we, c[54], &mve, &s_, &t_);
+        dx[5] = ((-we) / c[8] + e * c[4]) + ((mve + (-c[1])) * c[5]) / c[8];
+        FN(zl_fluct)(c, e, i, ix, iix, wi, c[55], &mvi, &s_, &t_);
+        dx[6] = ((-wi) / c[9] + i * c[6]) + ((mvi + (-c[2])) * c[7]) / c[9];
+        dx[7] = (-ou) / c[27];
+        break; }
+    case ORC_KIX: { /* dfun.rs:1782-1869 */
+        REAL xx = x[0], v = x[1], n = x[2], dki = x[3], kg = x[4];
+        REAL m_inf = FN(kx_sig)((v + (REAL)24) * ((REAL)1 / (REAL)12));
+        REAL n_inf = FN(kx_sig)((v + (REAL)19) * ((REAL)1 / (REAL)18));
+        REAL h_na = -((REAL)1 / (EXPR((n + (REAL)-0.4f) * (REAL)-8) + (REAL)1)) + (REAL)1.1f;
+        REAL bv = (REAL)2160 / (REAL)720;
+        REAL dna_i = -dki, dna_o = dna_i * (-bv), dk_o = dki * (-bv);
+        REAL ki = (-dki) + (REAL)130, nai = dna_i + (REAL)16, nao = dna_o + (REAL)138, ko = (dk_o + (REAL)4.8f) + kg;
+        REAL i_k = (n * (REAL)22 + (REAL)0.12f) * (v - (FN(kx_ln)(ko) - FN(kx_ln)(ki)) * (REAL)26.64f);
+        REAL i_na = ((m_inf * h_na) * (REAL)40 + (REAL)0.02f) * (v - (FN(kx_ln)(nao) - FN(kx_ln)(nai)) * (REAL)26.64f);
+        REAL i_cl = (v + c[19]) * (REAL)7.5f;
+        REAL i_pump = (FN(kx_sig)(nai + (REAL)-21) * FN(kx_sig)(ko + (REAL)-5.5f)) * (REAL)250;
+        REAL v_dot = (((i_na + i_k) + i_cl) + i_pump) * c[14];
+        REAL r_x = (xx * c[6]) / PIR;
+        REAL syn = (cpl[0] * c[15]) * ((-v) + c[0]);
+        if (v < c[9]) {
+            dx[0] = ((((v + (-c[5])) * c[16]) * xx) + c[4]) - r_x * (xx * c[2]);
+            dx[1] = v_dot - xx * (((xx * c[6]) + c[3]) + syn);
+        } else {
+            dx[0] = ((((v + (-c[7])) * c[17]) * xx) + c[4]) - r_x * (xx * c[2]);
+            dx[1] = v_dot - xx * (((xx * c[8]) + c[3]) + syn);
+        }
+        dx[2] = (n_inf - n) / c[11];
+        dx[3] = (i_k - i_pump * (REAL)2) * c[18];
+        dx[4] = ((-ko) + c[1]) * c[13];
+        break; }
+    }
 }
 
 /* Derivative of one node: x[NV], c[NC] -> dx[NV].
@@ -192,6 +559,9 @@ static void FN(dfun)(const FN(mconst)* k, const REAL* x, const REAL* c, REAL* dx
         dx[0] = ((v * (r * (REAL)2) + k->cb_dpi) - r2 * k->cb_kpi) + c[0];
         dx[1] = ((-(v * v) + k->cb_eta) - r2 * k->cb_pi2) + ((-v + k->cb_vsyn) * k->cb_kpi) * r;
         break; }
+    case ORC_EC3: case ORC_EC3S: case ORC_ERS: case ORC_ZL1: case ORC_ZL2: case ORC_KIX:
+        FN(dfun_ext)(k, x, c, dx);
+        break;
     case ORC_RWW: { /* dfun.rs:503-515 */
         REAL s = x[0];
         REAL xx = k->sweep_form ? (s * k->r_wjn + k->sweep_val) + c[0] * k->r_jn
@@ -220,6 +590,13 @@ static void FN(clamp)(int model, REAL* x)
         x[0] = x[0] < 0 ? 0 : (x[0] > 1 ? 1 : x[0]);
         break;
     case ORC_CB2: /* dfun.rs:190-194 */
+        x[0] = x[0] < 0 ? 0 : x[0];
+        break;
+    case ORC_ZL1: case ORC_ZL2: /* dfun.rs:219-230: E, I in [0, 1] */
+        x[0] = x[0] < 0 ? 0 : (x[0] > 1 ? 1 : x[0]);
+        x[1] = x[1] < 0 ? 0 : (x[1] > 1 ? 1 : x[1]);
+        break;
+    case ORC_KIX: /* dfun.rs:232-236: x >= 0 */
         x[0] = x[0] < 0 ? 0 : x[0];
         break;
     case ORC_KUR: { /* dfun.rs:188-191 */

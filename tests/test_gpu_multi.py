@@ -363,3 +363,122 @@ def test_linear_and_sup_hopf_subnetworks(gpu, orc):
     tav, fin = orc.run_multi(subs2, links2, 0.02, 200, L.HEUN, False, n_sweep=2)
     assert all(np.all(np.isfinite(f)) for f in fin)
     _check(res, tav, fin, o64=orc.run_multi(subs2, links2, 0.02, 200, L.HEUN, False, n_sweep=2, dtype=np.float64))
+
+
+# ---- the six fixture-pinned models (dfun.rs:952-1870) on the multi path ----
+
+def _ext_cases():
+    from hyburn_amd.synth import (EPI_CODIM3_DEFAULT, EPI_CODIM3_SLOWMOD_DEFAULT, EPI_RS_DEFAULT, KIONEX_DEFAULT,
+                                  ZERLAUT_DEFAULT)
+    return {
+        "codim3": (L.EPILEPTOR_CODIM3, EPI_CODIM3_DEFAULT, [0.5, 0.0, 0.1], 0.1, L.HEUN, 300),
+        "codim3_slowmod": (L.EPILEPTOR_CODIM3_SLOWMOD, EPI_CODIM3_SLOWMOD_DEFAULT, [0.5, 0.0, 0.05, 0.0, 0.0], 0.1,
+                           L.RK4, 300),
+        "epileptor_rs": (L.EPILEPTOR_RS, EPI_RS_DEFAULT, [-1.6, -12.5, 3.8, -1.0, 0.005, 0.0, 1.0, 0.0], 0.1, L.HEUN,
+                         300),
+        "zerlaut_first": (L.ZERLAUT_FIRST, ZERLAUT_DEFAULT, [0.01, 0.01, 0.0, 0.0, 0.0], 0.1, L.EULER, 200),
+        "zerlaut_second": (L.ZERLAUT_SECOND, ZERLAUT_DEFAULT, [0.01, 0.01, 1e-4, 0.0, 1e-4, 0.0, 0.0, 0.0], 0.1,
+                           L.EULER, 150),
+        "kionex": (L.KIONEX, KIONEX_DEFAULT, [0.1, -65.0, 0.5, -0.5, -2.0], 0.01, L.HEUN, 600),
+    }
+
+
+@pytest.mark.parametrize("name", ["codim3", "codim3_slowmod", "epileptor_rs", "zerlaut_first", "zerlaut_second",
+                                  "kionex"])
+def test_fixture_models_on_the_gpu(gpu, orc, name):
+    """Each model as a subnetwork of 6 nodes (spread initial states) driven
+    by a G2DO subnetwork through a Linear link into coupling channel 0 and
+    feeding back into it: GPU against the oracle (rtol 2e-4) and within 4x
+    the f32 oracle's error against its FP64 twin."""
+    m, p, base, dt, integ, T = _ext_cases()[name]
+    rng = np.random.default_rng(len(name))
+    nv = L.MODEL_INFO[m][0]
+    x0 = (np.array(base, np.float32)[None, :] * (1.0 + 0.05 * rng.standard_normal((6, nv)))).astype(np.float32)
+    subs = [(m, 6, p, x0), (L.G2DO, 5, G2DO_DEFAULT, _x0(rng, 5, 2))]
+    links = [(1, 0, 0.02, L.CF_LINEAR, [1.0], [(0, 0)]), (0, 1, 0.01, L.CF_LINEAR, [1.0], [(0, 0)])]
+    res = _run(subs, links, dt=dt, n_steps=T, integrator=integ, n_sweep=2)
+    tav, fin = orc.run_multi(subs, links, dt, T, integ, False, n_sweep=2)
+    _check(res, tav, fin, rtol=2e-4, atol=2e-5,
+           o64=orc.run_multi(subs, links, dt, T, integ, False, n_sweep=2, dtype=np.float64))
+    # the reference's trace test shape (two uncoupled nodes, plain engine): against the oracle, which the
+    # CPU suite pins to the reference's TVB fixture rows
+    two = [(m, 2, p, np.repeat(np.array(base, np.float32)[None, :], 2, 0))]
+    r2 = run_multi([Subnet(*two[0])], [], dt=dt, n_steps=T, integrator=integ, plain=True)
+    t2, f2 = orc.run_multi(two, [], dt, T, integ, False, plain=True)
+    np.testing.assert_allclose(r2.final_states[0], f2[0], rtol=2e-4, atol=2e-5)
+
+
+def test_fixture_models_hybrid_and_stochastic(gpu, orc):
+    """hybrid_integrator: deterministic Heun for EpileptorRestingState, the
+    Codim3 models and KIonEx (model_prefers_heun, dfun.rs:1871-1873), the
+    configured stochastic Euler for Zerlaut; EpileptorRestingState takes its
+    three coupling channels from a self-projection (channels pass through
+    unmapped) and channel 0 from Codim3."""
+    ext = _ext_cases()
+    rng = np.random.default_rng(5)
+    subs = []
+    for name in ("epileptor_rs", "zerlaut_first", "codim3"):
+        m, p, base, _, _, _ = ext[name]
+        nv = L.MODEL_INFO[m][0]
+        subs.append((m, 4, p, (np.array(base, np.float32)[None, :] *
+                               (1.0 + 0.02 * rng.standard_normal((4, nv)))).astype(np.float32)))
+    links = [(2, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 0)]),              # codim3 x -> EpiRS channel 0
+             (0, 0, 0.01, L.CF_LINEAR, [0.5, 0.0], [(0, 0)]),         # EpiRS x1, y1, z -> its 3 channels
+             (0, 1, 0.001, L.CF_LINEAR, [1.0], [(0, 0)]), (1, 2, 0.1, L.CF_LINEAR, [1.0], [(0, 0)])]
+    raw = [rng.standard_normal((120, 4, L.MODEL_INFO[s[0]][0])).astype(np.float32) for s in subs]
+    nsig = np.full(8, 1e-6, np.float32)
+    res = _run(subs, links, dt=0.05, n_steps=120, integrator=L.EULER_STOCH, hybrid_integrator=True, nsig=nsig,
+               noise=raw)
+    tav, fin = orc.run_multi(subs, links, 0.05, 120, L.EULER_STOCH, True, nsig=nsig, noise=raw)
+    _check(res, tav, fin, o64=orc.run_multi(subs, links, 0.05, 120, L.EULER_STOCH, True, nsig=nsig, noise=raw,
+                                            dtype=np.float64))
+
+
+# ---- run_sweep_multi over several subnetworks (engine.rs:862-1348) ----
+
+def test_cartesian_sweep_over_subnetworks(gpu, orc):
+    subs, links = _ring(6)
+    rng = np.random.default_rng(9)
+    subs[0] = (L.G2DO, 6, G2DO_DEFAULT, _x0(rng, 6, 2))
+    # G2DO I (native: swept form) x WC P (native) -> 6 points; then JR mu and J (two dims: per-point params)
+    for spec, n, moved in (([(0, 1, [0.0, 0.25, -0.25]), (2, 18, [0.0, 0.5])], 6, 0),
+                           ([(1, 12, [0.1, 0.3]), (1, 7, [120.0, 150.0]), (0, 2, [-1.0, -3.0])], 8, 1)):
+        res = _run(subs, links, dt=0.1, n_steps=150, integrator=L.HEUN, hybrid_integrator=True, n_sweep=n,
+                   sweep_spec=spec)
+        tav, fin = orc.run_multi(subs, links, 0.1, 150, L.HEUN, True, n_sweep=n, sweep_spec=spec)
+        _check(res, tav, fin, o64=orc.run_multi(subs, links, 0.1, 150, L.HEUN, True, n_sweep=n, sweep_spec=spec,
+                                                dtype=np.float64))
+        assert not np.allclose(res.final_states[moved][0], res.final_states[moved][-1])   # the sweep acts
+
+
+def test_engine_run_sweep_multi_several_subnetworks(gpu, orc):
+    """BatchHybridEngine.run_sweep_multi on a three-subnetwork config: every
+    point's values in param_values, points from sweep point 0 of the current
+    state, and the engine keeps the new states (a later run with the old
+    n_sweep is an InvalidState)."""
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig, SweepParam, SweepPoint, SweepSpec
+    n = 5
+    d = {"sim_length": 10.0, "dt": 0.1, "integrator": "heun", "network": {
+        "subnetworks": [
+            {"model": "Generic2dOscillator", "nnodes": n, "params": G2DO_DEFAULT.tolist(),
+             "initial_state": (np.arange(2 * n) * 0.03).tolist()},
+            {"model": "JansenRit", "nnodes": n, "params": JR_DEFAULT.tolist(), "initial_state": [0.0] * (6 * n)},
+            {"model": "WilsonCowan", "nnodes": n, "params": WC_DEFAULT.tolist(), "initial_state": [0.1] * (2 * n)}],
+        "projections": [{"src": a, "tgt": b, "weights": 0.01, "coupling_fn": "Linear", "coupling_params": [1.0],
+                         "cvar_map": "0:0", "delays": [0]} for a, b in ((0, 1), (1, 2), (2, 0))]}}
+    eng = BatchHybridEngine.from_config(SimConfig.from_dict(d), 2)
+    eng.run_sweep(SweepParam(0, 1), np.array([0.0, 0.3], np.float32), 40)
+    start = [np.asarray(s).reshape(2, n, -1)[0] for s in eng._state]
+    spec = SweepSpec([SweepPoint(0, 1, [0.1, 0.2, 0.3]), SweepPoint(2, 18, [0.0, 1.0])])
+    r = eng.run_sweep_multi(spec, 60)
+    assert r.n_sweep == 6 and r.param_values == [0.1, 0.0, 0.1, 1.0, 0.2, 0.0, 0.2, 1.0, 0.3, 0.0, 0.3, 1.0]
+    subs = [(L.G2DO, n, G2DO_DEFAULT, start[0]), (L.JANSEN_RIT, n, JR_DEFAULT, start[1]),
+            (L.WILSON_COWAN, n, WC_DEFAULT, start[2])]
+    links = [(a, b, 0.01, L.CF_LINEAR, [1.0], [(0, 0)]) for a, b in ((0, 1), (1, 2), (2, 0))]
+    tav, fin = orc.run_multi(subs, links, 0.1, 60, L.HEUN, False, n_sweep=6,
+                             sweep_spec=[(0, 1, [0.1, 0.2, 0.3]), (2, 18, [0.0, 1.0])])
+    for k in range(3):
+        np.testing.assert_allclose(r.final_states[k], fin[k].ravel(), rtol=RTOL, atol=ATOL)
+    assert eng._step == 60 and len(eng._state[0]) == 6
+    with pytest.raises(L.HyburnError, match="sweep points"):
+        eng.run_sweep(SweepParam(0, 1), np.array([0.0, 0.3], np.float32), 10)

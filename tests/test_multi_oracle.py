@@ -372,3 +372,37 @@ delays = [4]
     with pytest.raises(L.HyburnError) as e:
         eng.resume(p)
     assert e.value.code == L.HB_INVALID_STATE
+
+
+def test_cartesian_spec_expansion_and_dispatch(orc):
+    """run_sweep_multi over subnetworks (engine.rs:69-90, dfun.rs:38-46, 99-133):
+    points in last-fastest order; per subnetwork one native dimension is the
+    swept form (= the SweepParam run), one non-native dimension is ignored,
+    two dimensions override that point's parameters (= a plain run with those
+    parameters)."""
+    rng = np.random.default_rng(21)
+    subs = [(L.G2DO, 4, G2DO_DEFAULT, _x0(rng, 4, 2)), (L.JANSEN_RIT, 3, JR_DEFAULT, _x0(rng, 3, 6, 0.01))]
+    links = [(0, 1, 0.05, L.CF_LINEAR, [1.0], [(0, 0)]), (1, 0, 0.05, L.CF_LINEAR, [1.0], [(0, 0)])]
+    I_vals, mu_vals, J_vals = [0.0, 0.3, -0.2], [0.1, 0.3], [120.0, 150.0]
+    # G2DO I (native) x JR mu (native): both swept forms
+    spec = [(0, 1, I_vals), (1, 12, mu_vals)]
+    tav, fin = orc.run_multi(subs, links, 0.1, 30, L.HEUN, False, n_sweep=6, sweep_spec=spec)
+    for b, (I, mu) in enumerate([(i, m) for i in I_vals for m in mu_vals]):
+        # point b = the SweepParam run on I with JR's mu set (JR's swept form
+        # adds mu exactly where its parameter form does)
+        p = JR_DEFAULT.copy()
+        p[12] = mu
+        t1, f1 = orc.run_multi([subs[0], (L.JANSEN_RIT, 3, p, subs[1][3])], links, 0.1, 30, L.HEUN, False,
+                               n_sweep=1, sweep=(0, 1, [I]))
+        for k in range(2):
+            np.testing.assert_array_equal(fin[k][b], f1[k][0])
+            np.testing.assert_array_equal(tav[k][b], t1[k][0])
+    # two dimensions on JR (mu, J): per-point parameter vectors
+    spec = [(1, 12, mu_vals), (1, 7, J_vals), (0, 5, [0.5])]          # G2DO idx 5: a single non-native dim, ignored
+    tav, fin = orc.run_multi(subs, links, 0.1, 30, L.RK4, False, n_sweep=4, sweep_spec=spec)
+    for b, (mu, J) in enumerate([(m, j) for m in mu_vals for j in J_vals]):
+        p = JR_DEFAULT.copy()
+        p[12], p[7] = mu, J
+        t1, f1 = orc.run_multi([subs[0], (L.JANSEN_RIT, 3, p, subs[1][3])], links, 0.1, 30, L.RK4, False)
+        np.testing.assert_array_equal(fin[1][b], f1[1][0])
+        np.testing.assert_array_equal(fin[0][b], f1[0][0])
