@@ -1514,11 +1514,13 @@ __device__ __forceinline__ void bold_helper_forward(const AdjInst& I, int n) {
     advance(nw);
 }
 
-// Forward save, delay-blocked. One node per thread (N <= blockDim).
-template <int M, int NP, bool LRING, int IC, int CF>
-__global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict__ insts) {
+// Forward save, delay-blocked. One node per thread (N <= blockDim). MAXT:
+// the launch bound, 512 or 1024 (networks of 513-1024 nodes; 128 registers
+// per lane there, so fewer register-resident row edges).
+template <int M, int NP, bool LRING, int IC, int CF, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_adj_forward_b(const AdjInst* __restrict__ insts) {
     extern __shared__ float lds[];
-    __shared__ float ordp[2][kK][8][2];   // per-wave order-parameter partials, by block parity
+    __shared__ float ordp[2][kK][MAXT / 64][2];   // per-wave order-parameter partials, by block parity
     const AdjInst* Ip = insts + blockIdx.x;
     constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
     const BView V = bview(Ip, false);
@@ -1556,7 +1558,7 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
         }
     }
     // CSR row of this node in registers: source row base, delay, weight
-    constexpr int kReg = 32;
+    constexpr int kReg = MAXT > 512 ? 8 : 32;
     int en[kReg], ed[kReg];
     float ew[kReg];
     int e0 = 0, e1 = 0;
@@ -1579,15 +1581,19 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
     int ck_left = V.Sg > 0 ? V.Sg : -1, ck_idx = 0;   // steps to the next checkpoint, its index
     float bacc = 0.0f;
     const size_t chan = (size_t)N * Hs;
+    // an HBM ring with several channels interleaves them per slot: one
+    // contiguous window per edge (LDS rings keep channel planes)
+    constexpr bool ILV = !LRING && NP > 1;
+    auto ridx = [&](int c, size_t off) { return ILV ? off * NP + c : (size_t)c * chan + off; };
 
     auto ring_put = [&](int sl, const float* xs) {   // sl: ring slot of the state's time
         float pr[NP];
         pre<NC, NP>(cfn, cc, false, false, xs, pr);
 #pragma unroll
         for (int c = 0; c < NP; ++c) {
-            const size_t b = (size_t)c * chan + (size_t)tid * Hs;
-            ring[b + sl] = pr[c];
-            if (sl < kK - 1) ring[b + H + sl] = pr[c];
+            const size_t b = (size_t)tid * Hs;
+            ring[ridx(c, b + sl)] = pr[c];
+            if (sl < kK - 1) ring[ridx(c, b + H + sl)] = pr[c];
         }
     };
     auto step = [&](int t, const float* g) {
@@ -1656,7 +1662,7 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
                 const uint32_t ex = V.edges[2 * e], ey = V.edges[2 * e + 1];
                 const float w = __uint_as_float(ey);
 #pragma unroll
-                for (int c = 0; c < NP; ++c) g[c] += w * ring[(size_t)c * chan + (size_t)(ex & 0xFFFF) * Hs];
+                for (int c = 0; c < NP; ++c) g[c] += w * ring[ridx(c, (size_t)(ex & 0xFFFF) * Hs)];
             }
         }
         step(0, g);
@@ -1682,9 +1688,9 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
                     sl += sl < 0 ? H : 0;
 #pragma unroll
                     for (int c = 0; c < NP; ++c) {
-                        const size_t base = (size_t)c * chan + (size_t)en[e] + sl;
+                        const size_t base = (size_t)en[e] + sl;
 #pragma unroll
-                        for (int k = 0; k < kK; ++k) acc[k][c] += ew[e] * ring[base + k];
+                        for (int k = 0; k < kK; ++k) acc[k][c] += ew[e] * ring[ridx(c, base + k)];
                     }
                 }
             }
@@ -1695,9 +1701,9 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
                 sl += sl < 0 ? H : 0;
 #pragma unroll
                 for (int c = 0; c < NP; ++c) {
-                    const size_t base = (size_t)c * chan + (size_t)(ex & 0xFFFF) * Hs + sl;
+                    const size_t base = (size_t)(ex & 0xFFFF) * Hs + sl;
 #pragma unroll
-                    for (int k = 0; k < kK; ++k) acc[k][c] += w * ring[base + k];
+                    for (int k = 0; k < kK; ++k) acc[k][c] += w * ring[ridx(c, base + k)];
                 }
             }
         }
@@ -1738,10 +1744,10 @@ __global__ __launch_bounds__(512) void k_adj_forward_b(const AdjInst* __restrict
 // holds dL/dg (coupling adjoints) by step: step s reads, over out-edge
 // (j -> i, delay d >= kK), the adjoint of step s + d -- always in a later
 // block. Step 0 (every edge read the current state) runs last, per step.
-template <int M, int NP, bool LRING, int IC, int CF>
-__global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict__ insts) {
+template <int M, int NP, bool LRING, int IC, int CF, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restrict__ insts) {
     extern __shared__ float lds[];
-    __shared__ double redd[8];
+    __shared__ double redd[MAXT / 64];
     const AdjInst* Ip = insts + blockIdx.x;
     constexpr int NV = Tr<M>::NV, NC = Tr<M>::NC;
     const BView V = bview(Ip, true);
@@ -1750,6 +1756,9 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
     const int tid = threadIdx.x, BS = blockDim.x;
     const bool own = tid < N;
     const size_t chan = (size_t)N * Hs;
+    // channels interleaved per slot in an HBM ring, as in the forward save
+    constexpr bool ILV = !LRING && NP > 1;
+    auto ridx = [&](int c, size_t off) { return ILV ? off * NP + c : (size_t)c * chan + off; };
     using RingPtr = std::conditional_t<LRING, float*, gp<float>>;
     RingPtr ring;
     if constexpr (LRING) ring = lds; else ring = V.ring;
@@ -1771,7 +1780,7 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
 #pragma unroll
     for (int v = 0; v < NV; ++v) lam[v] = 0.0f;
     double gpar = 0.0, gcc = 0.0;
-    constexpr int kReg = 16;   // register-resident out-edges (the rest read from memory)
+    constexpr int kReg = MAXT > 512 ? 8 : 16;   // register-resident out-edges (the rest read from memory)
     int en[kReg], ed[kReg];
     float ew[kReg];
     int e0 = 0, e1 = 0;
@@ -1825,9 +1834,9 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
     auto ring_put = [&](int sl, const float* lg) {   // sl: ring slot of the step
 #pragma unroll
         for (int c = 0; c < NP; ++c) {
-            const size_t b = (size_t)c * chan + (size_t)tid * Hs;
-            ring[b + sl] = lg[c];
-            if (sl < kK - 1) ring[b + H + sl] = lg[c];
+            const size_t b = (size_t)tid * Hs;
+            ring[ridx(c, b + sl)] = lg[c];
+            if (sl < kK - 1) ring[ridx(c, b + H + sl)] = lg[c];
         }
     };
     __syncthreads();
@@ -1970,9 +1979,9 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                         sl -= sl >= H ? H : 0;
 #pragma unroll
                         for (int c = 0; c < NP; ++c) {
-                            const size_t base = (size_t)c * chan + (size_t)en[e] + sl;
+                            const size_t base = (size_t)en[e] + sl;
 #pragma unroll
-                            for (int m = 0; m < kK; ++m) acc[m][c] += ew[e] * ring[base + m];
+                            for (int m = 0; m < kK; ++m) acc[m][c] += ew[e] * ring[ridx(c, base + m)];
                         }
                     }
                 }
@@ -1983,9 +1992,9 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                     sl -= sl >= H ? H : 0;
 #pragma unroll
                     for (int c = 0; c < NP; ++c) {
-                        const size_t base = (size_t)c * chan + (size_t)(ex & 0xFFFF) * Hs + sl;
+                        const size_t base = (size_t)(ex & 0xFFFF) * Hs + sl;
 #pragma unroll
-                        for (int m = 0; m < kK; ++m) acc[m][c] += w * ring[base + m];
+                        for (int m = 0; m < kK; ++m) acc[m][c] += w * ring[ridx(c, base + m)];
                     }
                 }
             }
@@ -2058,9 +2067,9 @@ __global__ __launch_bounds__(512) void k_adj_reverse_b(const AdjInst* __restrict
                     const int d = (int)(ex >> 16), node = (int)(ex & 0xFFFF);
 #pragma unroll
                     for (int c = 0; c < NP; ++c) {
-                        const size_t b = (size_t)c * chan + (size_t)node * Hs;
-                        acc[c] += w * ring[b + wrap_slot(d, H)];
-                        if (d > 0) acc[c] += w * ring[b];
+                        const size_t b = (size_t)node * Hs;
+                        acc[c] += w * ring[ridx(c, b + wrap_slot(d, H))];
+                        if (d > 0) acc[c] += w * ring[ridx(c, b)];
                     }
                 }
             }
@@ -2093,20 +2102,31 @@ static const void* ptr(bool rev, int npt) {
     return npt == 1 ? (const void*)k_adj_forward<M, NP, 1, IC, -1> : (const void*)k_adj_forward<M, NP, 2, IC, -1>;
 }
 
+template <int M, int NP, int IC, int CF, int MAXT>
+static const void* ptr_bt(bool rev, bool lring) {
+    if (rev)
+        return lring ? (const void*)k_adj_reverse_b<M, NP, true, IC, CF, MAXT>
+                     : (const void*)k_adj_reverse_b<M, NP, false, IC, CF, MAXT>;
+    return lring ? (const void*)k_adj_forward_b<M, NP, true, IC, CF, MAXT>
+                 : (const void*)k_adj_forward_b<M, NP, false, IC, CF, MAXT>;
+}
+
+// blocks wider than 512 threads take the 1024-thread instantiation
 template <int M, int NP, int IC, int CF>
-static const void* ptr_b(bool rev, bool lring) {
-    if (rev) return lring ? (const void*)k_adj_reverse_b<M, NP, true, IC, CF> : (const void*)k_adj_reverse_b<M, NP, false, IC, CF>;
-    return lring ? (const void*)k_adj_forward_b<M, NP, true, IC, CF> : (const void*)k_adj_forward_b<M, NP, false, IC, CF>;
+static const void* ptr_b(bool rev, bool lring, bool wide) {
+    return wide ? ptr_bt<M, NP, IC, CF, 1024>(rev, lring) : ptr_bt<M, NP, IC, CF, 512>(rev, lring);
 }
 
 template <int M, int NP>
 static const void* pick(const AdjLaunch& L, bool rev) {
     if (L.blocked) {
         const bool lin = L.cfk == HB_CF_LINEAR;
+        // the launch width: the forward save of a fused group doubles the block
+        const bool wide = (rev ? L.block : (L.bold_fused ? 2 * L.block : L.block)) > 512;
         switch (L.ic) {
-        case 0: return lin ? ptr_b<M, NP, 0, HB_CF_LINEAR>(rev, L.lring) : ptr_b<M, NP, 0, -1>(rev, L.lring);
-        case 1: return lin ? ptr_b<M, NP, 1, HB_CF_LINEAR>(rev, L.lring) : ptr_b<M, NP, 1, -1>(rev, L.lring);
-        default: return lin ? ptr_b<M, NP, 2, HB_CF_LINEAR>(rev, L.lring) : ptr_b<M, NP, 2, -1>(rev, L.lring);
+        case 0: return lin ? ptr_b<M, NP, 0, HB_CF_LINEAR>(rev, L.lring, wide) : ptr_b<M, NP, 0, -1>(rev, L.lring, wide);
+        case 1: return lin ? ptr_b<M, NP, 1, HB_CF_LINEAR>(rev, L.lring, wide) : ptr_b<M, NP, 1, -1>(rev, L.lring, wide);
+        default: return lin ? ptr_b<M, NP, 2, HB_CF_LINEAR>(rev, L.lring, wide) : ptr_b<M, NP, 2, -1>(rev, L.lring, wide);
         }
     }
     switch (L.ic) {
@@ -2141,12 +2161,14 @@ hipError_t HB_ADJ_XCAT(adj_unit_, HB_ADJ_UNIT)(int op, const AdjLaunch* L, hipSt
         q.model = HB_ADJ_UNIT;
         for (int np = 1; np <= 2; ++np)
             for (int ic = 0; ic < 3; ++ic)
-                for (int v = 0; v < 6; ++v) {
+                for (int v = 0; v < 12; ++v) {
                     q.np = np; q.ic = ic;
-                    q.blocked = v >= 2;
-                    q.npt = v == 1 ? 2 : 1;
-                    q.lring = v >= 4;
-                    q.cfk = (v & 1) && v >= 2 ? HB_CF_LINEAR : -1;
+                    q.blocked = (v % 6) >= 2;
+                    q.npt = (v % 6) == 1 ? 2 : 1;
+                    q.lring = (v % 6) >= 4;
+                    q.cfk = (v & 1) && (v % 6) >= 2 ? HB_CF_LINEAR : -1;
+                    q.block = v >= 6 ? 1024 : 64;   // both launch widths of the blocked kernels
+                    q.bold_fused = 0;
                     for (int rev = 0; rev < 2; ++rev) {
                         const void* k = adj::kernel_of(q, rev != 0);
                         if (!k) continue;

@@ -187,8 +187,6 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         s.kur = I.cfn == HB_CF_KURAMOTO && I.wkind == HB_W_CSR;
         s.np = I.wkind == HB_W_CSR ? ring_channels(I.model, I.cfn, I.params, I.node_param_idx, spec[i].grad_param)
                                    : s.nc;
-        s.npt = I.n_nodes <= 512 ? 1 : 2;
-        s.block = s.npt == 1 ? std::max(64, (I.n_nodes + 63) / 64 * 64) : 512;
         int dmax = 0, dmin = 1 << 30;
         s.nnz = I.wkind == HB_W_CSR ? I.nnz : 0;
         for (int e = 0; e < s.nnz; ++e) {
@@ -196,8 +194,11 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
             dmax = std::max(dmax, d);
             dmin = std::min(dmin, d);
         }
-        // delay-blocked kernels: one barrier per kBlock steps when every delay >= kBlock
-        s.blocked = s.nnz > 0 && dmin >= kBlock && I.n_nodes <= 512 && !(opts && (opts->flags & HB_FLAG_NO_BLOCKING));
+        // delay-blocked kernels: one barrier per kBlock steps when every delay >= kBlock,
+        // one node per thread up to 1024 (the 1024-thread instantiation past 512)
+        s.blocked = s.nnz > 0 && dmin >= kBlock && I.n_nodes <= 1024 && !(opts && (opts->flags & HB_FLAG_NO_BLOCKING));
+        s.npt = (s.blocked || I.n_nodes <= 512) ? 1 : 2;
+        s.block = s.npt == 1 ? std::max(64, (I.n_nodes + 63) / 64 * 64) : 512;
         s.Hd = s.blocked ? dmax + kBlock + 1 : dmax + 2;
         s.S = ce > 0 ? ce : keep_all ? I.n_steps : std::max(1, (int)std::ceil(std::sqrt((double)I.n_steps)));
         s.S = std::min(s.S, I.n_steps);

@@ -632,6 +632,11 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     const int tid = threadIdx.x, BS = blockDim.x;
     const int Hs = Hd + K - 1;      // row stride: ring slots + mirrored copy of slots 0..K-2
     const int NHd = N * Hs;
+    // An HBM ring with several channels interleaves them per slot, so each
+    // edge's K-step window of every channel is one contiguous run (one or two
+    // cache lines instead of one per channel); LDS rings keep channel planes.
+    constexpr bool ILV = !LRING && NP > 1;
+    auto ridx = [&](int c, int off) { return ILV ? off * NP + c : c * NHd + off; };
     using RingPtr = std::conditional_t<LRING, float*, gp<float>>;
     RingPtr ring;
     if constexpr (LRING) ring = lds; else ring = V.ring;
@@ -663,7 +668,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         const int L = V.hist_len, total = L * N * NP;
         for (int i = tid; i < total; i += BS) {
             const int j = i / (N * NP), r = i - j * (N * NP), nn = r / NP, c = r - nn * NP;
-            ring[c * NHd + nn * Hs + (Hd - L + j)] = V.hist_in[i];
+            ring[ridx(c, nn * Hs + (Hd - L + j))] = V.hist_in[i];
         }
     }
 
@@ -722,8 +727,8 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
         if (lead) {
 #pragma unroll
             for (int c = 0; c < NP; ++c) {
-                ring[c * NHd + n * Hs + slot] = own_pre[c];
-                if (slot < K - 1) ring[c * NHd + n * Hs + Hd + slot] = own_pre[c];   // mirror
+                ring[ridx(c, n * Hs + slot)] = own_pre[c];
+                if (slot < K - 1) ring[ridx(c, n * Hs + Hd + slot)] = own_pre[c];   // mirror
             }
         }
     };
@@ -770,7 +775,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
             const int j = ed.x & 0xFFFF;
             const float w = __uint_as_float(ed.y);
 #pragma unroll
-            for (int c = 0; c < NP; ++c) g[c] += w * ring[c * NHd + j * Hs];
+            for (int c = 0; c < NP; ++c) g[c] += w * ring[ridx(c, j * Hs)];
         }
 #pragma unroll
         for (int c = 0; c < NP; ++c) g[c] = lane_group_sum(g[c], lpn);
@@ -817,7 +822,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
 #pragma unroll
             for (int k = 0; k < K; ++k)
 #pragma unroll
-                for (int c = 0; c < NP; ++c) vb[k][c] = ring[c * NHd + base + k];
+                for (int c = 0; c < NP; ++c) vb[k][c] = ring[ridx(c, base + k)];
         };
 #pragma unroll
         for (int p = 0; p + 1 < D; ++p)
@@ -908,7 +913,7 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
             const int j = i / (N * NP), r = i - j * (N * NP), nn = r / NP, c = r - nn * NP;
             int sl = wsT - L + j;
             sl += sl < 0 ? Hd : 0;
-            V.hist_out[i] = ring[c * NHd + nn * Hs + sl];
+            V.hist_out[i] = ring[ridx(c, nn * Hs + sl)];
         }
     }
 

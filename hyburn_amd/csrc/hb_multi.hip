@@ -132,16 +132,23 @@ __device__ __forceinline__ void post_rt(const Proj& P, const float* g, const flo
 }
 
 // mc: the register copy of the first kMC constants; mcg: the full table in
-// global memory, which models 10-15 read directly.
+// global memory, which models 10-15 read directly. EXT: the kernel holds
+// models 10-15 (their code stays out of the kernel of the other ten).
+template <bool EXT>
 __device__ __forceinline__ void step_model(int model, const StepCtx& S, const float* mc, const float* mcg,
                                            const float* cpl, const float* z, float* x) {
+    if constexpr (EXT) {
+        switch (model) {
+        case HB_EPILEPTOR_CODIM3: integrate<HB_EPILEPTOR_CODIM3>(S, mcg, cpl, z, x); return;
+        case HB_EPILEPTOR_CODIM3_SLOWMOD: integrate<HB_EPILEPTOR_CODIM3_SLOWMOD>(S, mcg, cpl, z, x); return;
+        case HB_EPILEPTOR_RS: integrate<HB_EPILEPTOR_RS>(S, mcg, cpl, z, x); return;
+        case HB_ZERLAUT_FIRST: integrate<HB_ZERLAUT_FIRST>(S, mcg, cpl, z, x); return;
+        case HB_ZERLAUT_SECOND: integrate<HB_ZERLAUT_SECOND>(S, mcg, cpl, z, x); return;
+        case HB_KIONEX: integrate<HB_KIONEX>(S, mcg, cpl, z, x); return;
+        default: break;
+        }
+    }
     switch (model) {
-    case HB_EPILEPTOR_CODIM3: integrate<HB_EPILEPTOR_CODIM3>(S, mcg, cpl, z, x); break;
-    case HB_EPILEPTOR_CODIM3_SLOWMOD: integrate<HB_EPILEPTOR_CODIM3_SLOWMOD>(S, mcg, cpl, z, x); break;
-    case HB_EPILEPTOR_RS: integrate<HB_EPILEPTOR_RS>(S, mcg, cpl, z, x); break;
-    case HB_ZERLAUT_FIRST: integrate<HB_ZERLAUT_FIRST>(S, mcg, cpl, z, x); break;
-    case HB_ZERLAUT_SECOND: integrate<HB_ZERLAUT_SECOND>(S, mcg, cpl, z, x); break;
-    case HB_KIONEX: integrate<HB_KIONEX>(S, mcg, cpl, z, x); break;
     case HB_G2DO: integrate<HB_G2DO>(S, mc, cpl, z, x); break;
     case HB_MPR: integrate<HB_MPR>(S, mc, cpl, z, x); break;
     case HB_JANSEN_RIT: integrate<HB_JANSEN_RIT>(S, mc, cpl, z, x); break;
@@ -193,7 +200,11 @@ __device__ __attribute__((noinline)) void bold_advance(const Args* A, BoldClock&
     c.tprev = tk;
 }
 
+// NVR: state variables held in registers (6 for the ten models of the base
+// kernel, 8 with models 10-15); memory layouts keep the kNV stride.
+template <bool EXT>
 __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
+    constexpr int NVR = EXT ? kNV : 6;
     extern __shared__ float pv[];   // pre values of every projection's source nodes
     __shared__ float gmean[kMaxProj * 4];   // w * mean over source nodes, per Scalar projection and channel
     const int t = threadIdx.x, b = blockIdx.x;
@@ -208,9 +219,9 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     float mc[kMC];
 #pragma unroll
     for (int k = 0; k < kMC; ++k) mc[k] = live ? mcg[k] : 0.0f;
-    float x[kNV], acc[kNV];
+    float x[NVR], acc[NVR];
 #pragma unroll
-    for (int v = 0; v < kNV; ++v) {
+    for (int v = 0; v < NVR; ++v) {
         x[v] = (live && v < nv) ? A->x0[((A->x0_per_point ? (size_t)b * A->n_tot : 0) + t) * kNV + v] : 0.0f;
         acc[v] = 0.0f;
     }
@@ -221,9 +232,9 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
     S.integ = live ? A->sub[s].integ : HB_EULER;
     S.stoch = live && A->sub[s].stoch != 0;
     S.plain = A->plain != 0;
-    float nscale[kNV];
+    float nscale[NVR];
 #pragma unroll
-    for (int v = 0; v < kNV; ++v) nscale[v] = live ? A->sub[s].nscale[v] : 0.0f;
+    for (int v = 0; v < NVR; ++v) nscale[v] = live ? A->sub[s].nscale[v] : 0.0f;
     S.sw = live && A->sub[s].sw != 0;
     S.sv = A->pt_sv ? (live ? A->pt_sv[(size_t)b * A->n_sub + s] : 0.0f) : (A->sweep ? A->sweep[b] : 0.0f);
     const int H = A->H;
@@ -348,16 +359,16 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                     const float sv = stim_value(A->stim[q], step, A->dt_ms);
                     if (sv != 0.0f) cpl[0] = cpl[0] + sv;
                 }
-            float z[kNV];
+            float z[NVR];
 #pragma unroll
-            for (int v = 0; v < kNV; ++v) z[v] = 0.0f;
+            for (int v = 0; v < NVR; ++v) z[v] = 0.0f;
             if (S.stoch) {
                 const float* raw = A->noise + ((size_t)k * A->n_tot + t) * kNV;
                 for (int v = 0; v < nv; ++v) z[v] = raw[v] * nscale[v];   // integrator.rs:183-191
             }
-            step_model(model, S, mc, mcg, cpl, z, x);
+            step_model<EXT>(model, S, mc, mcg, cpl, z, x);
 #pragma unroll
-            for (int v = 0; v < kNV; ++v) acc[v] += x[v];
+            for (int v = 0; v < NVR; ++v) acc[v] += x[v];
             if (A->traj) {
                 float* o = A->traj + (((size_t)k * gridDim.x + b) * A->n_tot + t) * kNV;
                 for (int v = 0; v < nv; ++v) o[v] = x[v];
@@ -794,8 +805,14 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     (void)hipEventCreate(&ev0);
     (void)hipEventCreate(&ev1);
     (void)hipEventRecord(ev0, nullptr);
-    hipLaunchKernelGGL(k_multi, dim3((unsigned)Q.n_sweep), dim3((unsigned)threads), (size_t)lds * sizeof(float),
-                       nullptr, dA);
+    bool ext = false;
+    for (int q = 0; q < Q.n_subnets; ++q) ext = ext || model_ext(Q.subnets[q].model);
+    if (ext)
+        hipLaunchKernelGGL(k_multi<true>, dim3((unsigned)Q.n_sweep), dim3((unsigned)threads),
+                           (size_t)lds * sizeof(float), nullptr, dA);
+    else
+        hipLaunchKernelGGL(k_multi<false>, dim3((unsigned)Q.n_sweep), dim3((unsigned)threads),
+                           (size_t)lds * sizeof(float), nullptr, dA);
     e = hipGetLastError();
     (void)hipEventRecord(ev1, nullptr);
     if (e == hipSuccess) e = hipEventSynchronize(ev1);

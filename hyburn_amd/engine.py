@@ -313,7 +313,7 @@ def _needs_multi(cfg: SimConfig) -> bool:
     subnetworks or a cvar_map other than 0:0 take the multi-subnetwork path."""
     return (len(cfg.subnetworks) != 1 or len(cfg.projections) > 1
             or any(p.src != p.tgt or p.cvar_map != "0:0" for p in cfg.projections)
-            or L.MODEL_NAMES.get(cfg.subnetworks[0].model, 0) in (L.LINEAR, L.SUP_HOPF, L.HOPFIELD, L.COOMBES_BYRNE_2D))
+            or L.MODEL_NAMES.get(cfg.subnetworks[0].model, 0) in L.MULTI_ONLY)
 
 
 def _multi_from_config(cfg: SimConfig, plain: bool = False):

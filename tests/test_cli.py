@@ -216,3 +216,38 @@ def test_from_config_multi_branch_rejects_single_network_options():
         with pytest.raises(L.HyburnError) as e:
             BatchHybridEngine.from_config(cfg, 2, **kw)
         assert e.value.code == L.HB_INVALID_CONFIG
+
+
+EXT_SIM = """
+sim_length = {T}
+dt = {dt}
+integrator = "{integ}"
+[network]
+[[network.subnetworks]]
+model = "{model}"
+nnodes = 2
+params = [{params}]
+initial_state = [{ic}]
+"""
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["EpileptorCodim3", "ZerlautAdaptationSecondOrder", "KIonEx"])
+def test_plain_run_of_a_fixture_model(gpu, orc, tmp_path, name):
+    """`hyburn run` of the reference's trace-test configuration
+    (tests/model_reference_test.rs:90-156: two uncoupled nodes, the plain
+    engine): state_final_sub0.npy [nvar, 2, 1] against the oracle, which the
+    CPU suite pins to the reference's fixture rows."""
+    from test_oracle_models_ext import TRACE_CASES
+    m, p, ic, dt, integ, T, _, _ = TRACE_CASES[name]
+    cfg = tmp_path / "sim.toml"
+    cfg.write_text(EXT_SIM.format(T=dt * T, dt=dt, integ={L.HEUN: "heun", L.EULER: "euler"}[integ], model=name,
+                                  params=", ".join(repr(float(v)) for v in p), ic=", ".join(str(v) for v in ic)))
+    out = tmp_path / "out"
+    assert main(["run", "-c", str(cfg), "-o", str(out), "-b", "hip"]) == 0
+    fin = np.load(out / "state_final_sub0.npy")
+    nv = L.MODEL_INFO[m][0]
+    assert fin.shape == (nv, 2, 1)
+    x0 = np.array(ic, np.float32).reshape(nv, 2).T.copy()
+    _, f = orc.run_multi([(m, 2, p, x0)], [], dt, T, integ, False, plain=True)
+    np.testing.assert_allclose(fin[:, :, 0], f[0][0].T, rtol=2e-4, atol=2e-5)

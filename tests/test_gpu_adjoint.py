@@ -52,6 +52,10 @@ BLOCKED = {
                                                          dmin=8, dmax=16),
     "g2do_bold_mse_heun_stoch_blocked": lambda orc: bold_mse_case(L.G2DO, orc, integ=L.HEUN_STOCH, noise=True,
                                                                   node_param_idx=1, dmin=9, dmax=30),
+    # wider than 512 nodes: the 1024-thread instantiations (the Kuramoto ring in HBM, channels interleaved)
+    "kuramoto_order_rk4_wide": lambda orc: kuramoto_case(N=600, T=160, dmin=8, dmax=60, seed=7, dens=0.03),
+    "mpr_bold_mse_heun_wide": lambda orc: bold_mse_case(L.MPR, orc, N=530, G=0.2, node_param_idx=2, dmin=8,
+                                                        dmax=20, T=400),
 }
 
 

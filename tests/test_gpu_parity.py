@@ -362,6 +362,19 @@ def test_blocked_lanes_per_node_fast_mode(gpu, orc):
         _err_bound(x.traj, orc.run(I)["traj"], orc.run(I, dtype=np.float64)["traj"])
 
 
+@pytest.mark.parametrize("dmin", [9, 20])
+def test_blocked_two_channel_hbm_ring_bitwise(gpu, orc, dmin):
+    """MPR with cv != 0 keeps both coupling channels; with the ring in HBM the
+    channels are interleaved per slot (one contiguous window per edge). 8-step
+    (dmin 9) and 16-step (dmin 20) blocks, bit-identical to the oracle."""
+    p = DEFAULTS[L.MPR].copy()
+    p[6] = 0.3
+    I = _blocked_net(L.MPR, N=300, T=200, integ=L.RK4, dens=0.1, dmin=dmin, dmax=400, seed=650 + dmin, params=p)
+    (g,), st = run_instances([I], exact=True)
+    assert st.lds_bytes == 0
+    _assert_exact(g, orc.run(I), f"two-channel HBM ring dmin={dmin}")
+
+
 WIDE_CASES = {
     # (model, N, integrator, dmin, dmax, density): networks wider than 512
     # nodes on the 1024-thread delay-blocked kernel
