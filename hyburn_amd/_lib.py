@@ -129,6 +129,11 @@ class HbResult(C.Structure):
                 ("step", C.c_int64)]
 
 
+class HbEngineState(C.Structure):
+    _fields_ = [("step", C.c_int64), ("dt", C.c_double), ("integrator", C.c_int32), ("n_state", C.c_int32),
+                ("n_nodes", C.c_int32), ("nvar", C.c_int32), ("hist_floats", C.c_int64)]
+
+
 class HbAdjointSpec(C.Structure):
     _fields_ = [("loss_kind", C.c_int32), ("target", fp), ("grad_param", C.c_int32),
                 ("loss", fp), ("grad_coupling", fp), ("grad_param_out", fp), ("grad_x0", fp),
@@ -180,6 +185,8 @@ PROTOTYPES = {
     "hb_run_sweep_multi": (C.c_int, [C.c_void_p, C.POINTER(HbSweepPoint), C.c_int32, C.c_int64, C.c_uint32,
                                      C.POINTER(HbResult)]),
     "hb_result_free": (None, [C.POINTER(HbResult)]),
+    "hb_engine_get_state": (C.c_int, [C.c_void_p, C.POINTER(HbEngineState), fp, fp, fp]),
+    "hb_engine_set_state": (C.c_int, [C.c_void_p, C.POINTER(HbEngineState), fp, C.c_int32, fp, fp]),
     "hb_plan_create": (C.c_int, [C.POINTER(HbInstance), C.c_int32, C.POINTER(HbRunOpts), C.POINTER(C.c_void_p)]),
     "hb_plan_run": (C.c_int, [C.c_void_p, C.POINTER(HbRunStats)]),
     "hb_plan_download": (C.c_int, [C.c_void_p]),

@@ -61,13 +61,31 @@ def _compile(src: str, obj: str, extra: list[str], device: bool) -> tuple[str, i
     return obj, p.returncode, err
 
 
+def _includes(path: str, seen: set) -> set:
+    """Headers of this build that path includes, transitively ("..." includes
+    found in csrc/ or include/)."""
+    import re
+    try:
+        text = open(path).read()
+    except OSError:
+        return seen
+    for name in re.findall(r'^\s*#\s*include\s+"([^"]+)"', text, flags=re.M):
+        for d in (CSRC, os.path.join(HERE, "..", "include")):
+            h = os.path.normpath(os.path.join(d, name))
+            if os.path.exists(h) and h not in seen:
+                seen.add(h)
+                _includes(h, seen)
+                break
+    return seen
+
+
 def _stale(src: str, obj: str) -> bool:
-    """An object is stale when its source or any header is newer than it."""
+    """An object is stale when its source or a header it includes is newer than it."""
     o = os.path.join(BUILD, obj)
     if not os.path.exists(o):
         return True
-    deps = [os.path.join(CSRC, src)] + [os.path.join(d, f) for d in (CSRC, os.path.join(HERE, "..", "include"))
-                                        for f in os.listdir(d) if f.endswith(".h")]
+    s = os.path.join(CSRC, src)
+    deps = [s] + sorted(_includes(s, set()))
     return max(os.path.getmtime(d) for d in deps) > os.path.getmtime(o)
 
 

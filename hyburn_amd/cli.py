@@ -2,7 +2,8 @@
 
 Mirrors the reference command line (src/cli.rs:16-60, 785-842, 894-974, 1040-1103):
 
-  run -c sim.toml -o output [-b hip] [--sweep sweep.toml] [--noise-seed S]
+  run -c sim.toml -o output [-b hip] [--checkpoint FILE] [--resume FILE]
+      [--sweep sweep.toml] [--noise-seed S]
 
 * without --sweep: one plain-engine run (HybridEngine semantics), writing
   state_final_sub0.npy [nvar, nnodes, nmodes] and state_traj.npy
@@ -10,6 +11,12 @@ Mirrors the reference command line (src/cli.rs:16-60, 785-842, 894-974, 1040-110
 * with --sweep: a batched sweep (BatchHybridEngine semantics) over the sweep
   file's parameter values, writing tavg_sub{i}.npy with the reference's header
   [n_sweep, nvar, nnodes] over its node-major [n_sweep, nnodes, nvar] data.
+
+--resume restores a HYBURNCK checkpoint before the run and --checkpoint
+saves one after it (runtime.rs:519-742, cli.rs:1057-1071): the reference's
+version 2 file for a single network without a delay history to carry, the
+batch files (versions 3 and 4) otherwise. Sweep mode ignores both, with a
+warning, as the reference does (cli.rs:885-890).
 
 n_steps = (sim_length / dt) truncated in f64, as the reference computes it.
 Only the hip backend is provided: any other --backend value is an error
@@ -36,7 +43,7 @@ def _backend(name: str) -> None:
 
 
 def run(config: str, output: str, backend: str = "hip", sweep: str | None = None, noise_seed: int = 0,
-        device: int = 0) -> dict:
+        device: int = 0, checkpoint: str | None = None, resume: str | None = None) -> dict:
     """Run the command; returns the written file paths."""
     _backend(backend)
     cfg = SimConfig.from_file(config)
@@ -45,6 +52,10 @@ def run(config: str, output: str, backend: str = "hip", sweep: str | None = None
     os.makedirs(output, exist_ok=True)
     written = {}
     if sweep:
+        if checkpoint:
+            print("warning: Checkpoint ignored in sweep mode", file=sys.stderr)
+        if resume:
+            print("warning: Resume ignored in sweep mode", file=sys.stderr)
         sc = SweepConfig.from_file(sweep)
         values = sc.generate_values()
         if not values:
@@ -63,7 +74,7 @@ def run(config: str, output: str, backend: str = "hip", sweep: str | None = None
     elif _needs_multi(cfg) or cfg.stimuli:
         # several subnetworks, links between them, cvar_map remaps, stimuli or
         # a model only the multi-subnetwork kernel integrates
-        written.update(_plain_multi(cfg, n_steps, output, noise_seed, device))
+        written.update(_plain_multi(cfg, n_steps, output, noise_seed, device, checkpoint, resume))
     else:
         # HybridEngine reads an inline initial state var-major [nvar][nnodes]
         # (construction.rs:557-565); the batch engine node-major (engine.rs:161-170)
@@ -72,8 +83,13 @@ def run(config: str, output: str, backend: str = "hip", sweep: str | None = None
             nv0 = L.MODEL_INFO[L.MODEL_NAMES[s0.model]][0]
             s0.initial_state = np.asarray(s0.initial_state, np.float32).reshape(nv0, s0.nnodes).T.ravel().tolist()
         eng = BatchHybridEngine.from_config(cfg, 1, device, engine=L.ENGINE_PLAIN, noise_seed=noise_seed)
+        if resume:
+            eng.resume(resume)
         # a plain run: no parameter is swept (an index the dfun ignores)
         res = eng.run_sweep_with_trajectory(SweepParam(0, -1), [0.0], n_steps)
+        if checkpoint:
+            eng.checkpoint(checkpoint)
+            written["checkpoint"] = checkpoint
         eng.close()
         nnodes, nvar = eng.nnodes, eng.nvar
         fin = np.asarray(res.final_states[0], np.float32).reshape(nnodes, nvar)
@@ -88,12 +104,14 @@ def run(config: str, output: str, backend: str = "hip", sweep: str | None = None
     return written
 
 
-def _plain_multi(cfg, n_steps: int, output: str, noise_seed: int, device: int) -> dict:
+def _plain_multi(cfg, n_steps: int, output: str, noise_seed: int, device: int, checkpoint: str | None = None,
+                 resume: str | None = None) -> dict:
     """HybridEngine run of several subnetworks (cli.rs:1040-1103) through the
     plain mode of hb_run_multi: state_final_sub{i}.npy as [nvar, nnodes, 1]
     and state_traj.npy as the flat per-step concatenation of every
     subnetwork's [nvar, nnodes, nmodes] state (runtime.rs:430-434)."""
-    from .engine import _multi_from_config
+    from . import checkpoint as ck
+    from .engine import _multi_from_config, multi_checkpoint_states
     from .multi import run_multi
     for s in cfg.subnetworks:   # inline initial states are var-major in HybridEngine
         if isinstance(s.initial_state, list):
@@ -104,10 +122,20 @@ def _plain_multi(cfg, n_steps: int, output: str, noise_seed: int, device: int) -
         raise SystemExit(f"unknown integrator {cfg.integrator}")
     max_delay = max([max(p.delays) for p in cfg.projections if p.delays] or [0])
     stims = [(t["target"], t["temporal"].lower(), t["params"]) for t in cfg.stimuli]
-    r = run_multi(subs, links, dt=cfg.dt, n_steps=n_steps, integrator=L.INTEGRATOR_NAMES[cfg.integrator],
-                  plain=True, stimuli=stims, dt_ms=cfg.dt, max_delay=max_delay, nsig=cfg.nsig,
-                  noise_seed=noise_seed, record_trajectory=True, device=device)
+    step0, state, hist, dt, integ, nsig = 0, None, None, cfg.dt, L.INTEGRATOR_NAMES[cfg.integrator], cfg.nsig
+    if resume:   # runtime.rs:576-742: step, dt, integrator, nsig and states from the file
+        c = ck.read(resume)
+        state, hist = multi_checkpoint_states(c, subs, links, max_delay, 1)
+        step0, dt, integ, nsig = int(c["step"]), float(c["dt"]), c["integrator"], c["nsig"]
+    r = run_multi(subs, links, dt=dt, n_steps=n_steps, integrator=integ, plain=True, stimuli=stims, dt_ms=dt,
+                  max_delay=max_delay, nsig=nsig, noise_seed=noise_seed, step0=step0, state=state, hist=hist,
+                  record_trajectory=True, device=device)
     written = {}
+    if checkpoint:
+        ck.write_batch(checkpoint, step=step0 + n_steps, dt=dt, integrator=integ, nsig=nsig,
+                       shapes=[(L.MODEL_INFO[S.model][0], S.n_nodes) for S in subs], states=r.final_states,
+                       hist=r.hist)
+        written["checkpoint"] = checkpoint
     for i, (S, f) in enumerate(zip(subs, r.final_states)):
         nv = L.MODEL_INFO[S.model][0]
         p = os.path.join(output, f"state_final_sub{i}.npy")
@@ -128,12 +156,14 @@ def main(argv=None) -> int:
     r.add_argument("-c", "--config", default="sim.toml")
     r.add_argument("-o", "--output", default="output")
     r.add_argument("-b", "--backend", default="hip")
+    r.add_argument("--checkpoint", default=None, help="save a HYBURNCK checkpoint after the run")
+    r.add_argument("--resume", default=None, help="resume from a HYBURNCK checkpoint")
     r.add_argument("--sweep", default=None)
     r.add_argument("--noise-seed", type=int, default=0)
     r.add_argument("--device", type=int, default=0)
     a = ap.parse_args(argv)
     try:
-        out = run(a.config, a.output, a.backend, a.sweep, a.noise_seed, a.device)
+        out = run(a.config, a.output, a.backend, a.sweep, a.noise_seed, a.device, a.checkpoint, a.resume)
     except L.HyburnError as e:
         print(f"error: {e}", file=sys.stderr)
         return 1

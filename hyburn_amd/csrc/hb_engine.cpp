@@ -8,6 +8,7 @@
 //   run_sweep_multi, Cartesian order  engine.rs:56-91, 845-1348
 //   SimConfig::validate               src/config.rs:286-428
 //   HybridEngine weights/delays       src/engine/construction.rs:543-790
+//   checkpoint / resume state         src/engine/runtime.rs:519-561, 576-742
 #include "hb_internal.h"
 
 #include <algorithm>
@@ -498,6 +499,82 @@ int hb_run_sweep_multi(hb_engine* E, const hb_sweep_point* pts, int32_t n_points
     }
     for (int64_t s = 0; s < total; ++s) B.inst[(size_t)s].params = B.params[(size_t)s].data();
     return run_built(E, B, n_steps, flags & ~HB_FLAG_BOLD_BATCH_MEAN, false, out);
+}
+
+int hb_engine_get_state(const hb_engine* E, hb_engine_state* st, float* nsig, float* states, float* hist) {
+    if (!E || !st) return efail(HB_INVALID_STATE, "null engine or state");
+    memset(st, 0, sizeof(*st));
+    st->step = E->step;
+    st->dt = E->dt;
+    st->integrator = E->integrator;
+    st->n_state = E->n_state;
+    st->n_nodes = E->n_nodes;
+    st->nvar = E->nv;
+    const hb_instance I = base_instance(E, 0);
+    st->hist_floats = hb_history_floats(&I);
+    const size_t per = (size_t)E->n_nodes * E->nv;
+    if (nsig) memcpy(nsig, E->nsig.data(), (size_t)E->nv * 4);
+    if (states) {
+        if (E->n_state > 0) memcpy(states, E->state.data(), per * E->n_state * 4);
+        else memcpy(states, E->x0.data(), per * 4);
+    }
+    if (hist && E->n_state > 0 && st->hist_floats > 0) {
+        for (int i = 0; i < E->n_state; ++i) {
+            float* dst = hist + (size_t)i * st->hist_floats;
+            // a run that never used the history (step 0, or a run_sweep_multi
+            // left nothing) holds none: that is a zero history
+            if (i < (int)E->hist.size() && (int64_t)E->hist[i].size() == st->hist_floats)
+                memcpy(dst, E->hist[i].data(), (size_t)st->hist_floats * 4);
+            else
+                memset(dst, 0, (size_t)st->hist_floats * 4);
+        }
+    }
+    return HB_OK;
+}
+
+int hb_engine_set_state(hb_engine* E, const hb_engine_state* st, const float* nsig, int32_t n_nsig,
+                        const float* states, const float* hist) {
+    if (!E || !st) return efail(HB_INVALID_STATE, "null engine or state");
+    if (st->step < 0) return efail(HB_INVALID_STATE, "negative step");
+    if (!(st->dt > 0.0)) return efail(HB_INVALID_STATE, "dt must be positive");
+    if (st->integrator < HB_EULER || st->integrator > HB_RK4_STOCH)
+        return efail(HB_INVALID_STATE, "Unknown integrator kind in checkpoint: " + std::to_string(st->integrator));
+    if (st->n_state < 1 || st->n_state > (1 << 24) || !states)
+        return efail(HB_INVALID_STATE, "a checkpoint needs at least one state");
+    if (st->n_nodes != E->n_nodes || st->nvar != E->nv)
+        return efail(HB_INVALID_STATE, "Checkpoint shape mismatch for subnetwork 0: expected (nvar=" +
+                                           std::to_string(E->nv) + ",nnodes=" + std::to_string(E->n_nodes) +
+                                           ",nmodes=1), got (" + std::to_string(st->nvar) + "," +
+                                           std::to_string(st->n_nodes) + ",1)");
+    if (n_nsig != 0 && n_nsig != 1 && n_nsig != E->nv)
+        return efail(HB_INVALID_STATE, "nsig per-variable length does not match nvar");
+    if (n_nsig > 0 && !nsig) return efail(HB_INVALID_STATE, "nsig missing");
+    const size_t per = (size_t)E->n_nodes * E->nv;
+    // the history layout follows the engine's configuration; dt and the
+    // integrator do not enter it
+    const hb_instance I = base_instance(E, 0);
+    const int64_t hf = hb_history_floats(&I);
+    if (hist && st->hist_floats != hf)
+        return efail(HB_INVALID_STATE, "checkpoint history holds " + std::to_string(st->hist_floats) +
+                                           " floats per point, this engine keeps " + std::to_string(hf));
+    const int n = (st->n_state == 1 && E->n_sweep > 1) ? E->n_sweep : st->n_state;
+    E->state.resize(per * n);
+    E->hist.assign(hf > 0 ? n : 0, std::vector<float>());
+    for (int i = 0; i < n; ++i) {
+        const int src = st->n_state == 1 ? 0 : i;
+        memcpy(E->state.data() + per * i, states + per * src, per * 4);
+        if (hf > 0) {
+            if (hist) E->hist[i].assign(hist + (size_t)src * hf, hist + (size_t)(src + 1) * hf);
+            else E->hist[i].assign((size_t)hf, 0.0f);
+        }
+    }
+    E->n_state = n;
+    E->step = st->step;
+    E->dt = st->dt;
+    E->integrator = st->integrator;
+    if (n_nsig == 1) E->nsig.assign(E->nv, nsig[0]);
+    else if (n_nsig == E->nv) E->nsig.assign(nsig, nsig + E->nv);
+    return HB_OK;
 }
 
 void hb_result_free(hb_result* r) {

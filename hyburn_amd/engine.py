@@ -358,6 +358,37 @@ def _multi_from_config(cfg: SimConfig, plain: bool = False):
     return subs, links
 
 
+def multi_checkpoint_states(c: dict, subs, links, max_delay: int, n_sweep: int):
+    """Check a read checkpoint against a multi-subnetwork configuration
+    (HybridEngine::resume's shape checks, runtime.rs:693-725) and return its
+    states and history for n_sweep points; a file of one point continues
+    every point."""
+    from .multi import horizon
+    if len(c["shapes"]) != len(subs):
+        _err(L.HB_INVALID_STATE, f"Checkpoint has {len(c['shapes'])} subnetworks, expected {len(subs)}")
+    for i, ((nv, nn, nm), S) in enumerate(zip(c["shapes"], subs)):
+        want = (L.MODEL_INFO[S.model][0], S.n_nodes, 1)
+        if (nv, nn, nm) != want:
+            _err(L.HB_INVALID_STATE, f"Checkpoint shape mismatch for subnetwork {i}: expected "
+                                     f"(nvar={want[0]},nnodes={want[1]},nmodes=1), got ({nv},{nn},{nm})")
+    if c.get("hist_single") is not None:
+        _err(L.HB_INVALID_STATE, "this checkpoint carries a single-network engine's history (version 4)")
+    states, hist = c["states"], c["hist"]
+    H = horizon(links, max_delay)
+    n_tot = sum(S.n_nodes for S in subs)
+    if hist is not None and (hist.shape[1] != n_tot or hist.shape[2] != H):
+        _err(L.HB_INVALID_STATE, f"Checkpoint history has shape {tuple(hist.shape)}, expected "
+                                 f"(n_sweep, {n_tot}, {H}, 2)")
+    if c["n_sweep"] != n_sweep:
+        if c["n_sweep"] != 1:
+            _err(L.HB_INVALID_STATE, f"Checkpoint holds {c['n_sweep']} sweep points, expected {n_sweep}")
+        # one saved point continues every sweep point: states and history alike
+        states = [np.repeat(a, n_sweep, axis=0) for a in states]
+        if hist is not None:
+            hist = np.repeat(hist, n_sweep, axis=0)
+    return states, hist
+
+
 class BatchHybridEngine:
     """BatchHybridEngine<B> with B = the HIP backend. Configurations with
     several subnetworks (or projections between subnetworks) run through
@@ -616,48 +647,90 @@ class BatchHybridEngine:
                                 trajectories=[t.ravel() for t in r.trajectories] if traj else None, bold=None,
                                 elapsed_ms=ms, n_sweep=n, kernel_ms=r.kernel_ms)
 
-    def checkpoint(self, path: str) -> None:
-        """Save the engine state as a HYBURNCK version 3 (batch) file
-        (runtime.rs:503-561 extended with the sweep dimension and history)."""
+    def checkpoint(self, path: str, reference_format: Optional[bool] = None) -> None:
+        """Save the engine state (HybridEngine::checkpoint, runtime.rs:519-561).
+
+        Single-network engines write the reference's version 2 file when that
+        loses nothing (one sweep point, no delay history to carry) and the
+        version 4 batch file otherwise; reference_format True forces version
+        2 (one sweep point; a delay history is dropped, as the reference
+        keeps none), False forces version 4. Multi-subnetwork engines write
+        version 3 (states and history of every sweep point)."""
         from . import checkpoint as ck
-        if self._multi is None:
-            _err(L.HB_INVALID_STATE, "checkpoints are kept by the multi-subnetwork path")
-        subs = self._multi[0]
-        shapes = [(L.MODEL_INFO[S.model][0], S.n_nodes) for S in subs]
-        states = self._state or [np.broadcast_to(np.asarray(S.x0, np.float32), (self.n_sweep,) + np.shape(S.x0))
-                                 for S in subs]
-        nsig = np.atleast_1d(np.asarray(self._nsig, np.float32))
-        ck.write_batch(path, step=self._step, dt=self.dt, integrator=self._multi[2], nsig=nsig, shapes=shapes,
-                       states=states, hist=self._hist)
+        if self._multi is not None:
+            if reference_format:
+                _err(L.HB_INVALID_STATE, "a multi-subnetwork engine writes the batch (version 3) checkpoint")
+            subs = self._multi[0]
+            shapes = [(L.MODEL_INFO[S.model][0], S.n_nodes) for S in subs]
+            states = self._state or [np.broadcast_to(np.asarray(S.x0, np.float32), (self.n_sweep,) + np.shape(S.x0))
+                                     for S in subs]
+            nsig = np.atleast_1d(np.asarray(self._nsig, np.float32))
+            ck.write_batch(path, step=self._step, dt=self.dt, integrator=self._multi[2], nsig=nsig, shapes=shapes,
+                           states=states, hist=self._hist)
+            return
+        st, nsig, states, hist = self._get_state()
+        carried = st.n_state > 0 and st.hist_floats > 0
+        v2 = states.shape[0] == 1 and not carried if reference_format is None else bool(reference_format)
+        if v2:
+            if states.shape[0] != 1:
+                _err(L.HB_INVALID_STATE, f"the reference's checkpoint holds one simulation; this engine holds "
+                                         f"{states.shape[0]} sweep points")
+            ck.write_reference(path, step=st.step, dt=st.dt, integrator=st.integrator, nsig=nsig, nvar=self.nvar,
+                               nnodes=self.nnodes, state=states[0])
+        else:
+            ck.write_single(path, step=st.step, dt=st.dt, integrator=st.integrator, nsig=nsig, nvar=self.nvar,
+                            nnodes=self.nnodes, states=states, hist=hist if carried else None)
+
+    def _get_state(self):
+        st = L.HbEngineState()
+        L.check(L.lib().hb_engine_get_state(self._h, C.byref(st), None, None, None))
+        states = np.zeros((max(st.n_state, 1), self.nnodes, self.nvar), np.float32)
+        nsig = np.zeros(self.nvar, np.float32)
+        hist = np.zeros((st.n_state, st.hist_floats), np.float32) if st.n_state and st.hist_floats else None
+        L.check(L.lib().hb_engine_get_state(self._h, C.byref(st), nsig.ctypes.data_as(L.fp),
+                                            states.ctypes.data_as(L.fp),
+                                            hist.ctypes.data_as(L.fp) if hist is not None else None))
+        return st, nsig, states, hist
 
     def resume(self, path: str) -> None:
-        """Restore step, dt, integrator, nsig, states and history from a
-        HYBURNCK file (runtime.rs:570-742); shapes must match."""
+        """Restore step, dt, integrator, nsig and the states from a HYBURNCK
+        file (HybridEngine::resume, runtime.rs:576-742); the shapes must
+        match. A file of one sweep point continues every sweep point; one
+        without history (the reference's versions 1 and 2) continues from a
+        zero delay history, which is what a freshly built reference engine
+        resumes into."""
         from . import checkpoint as ck
-        if self._multi is None:
-            _err(L.HB_INVALID_STATE, "checkpoints are kept by the multi-subnetwork path")
         c = ck.read(path)
+        if self._multi is not None:
+            self._resume_multi(c)
+            return
+        if len(c["shapes"]) != 1:
+            _err(L.HB_INVALID_STATE, f"Checkpoint has {len(c['shapes'])} subnetworks, expected 1")
+        nv, nn, nm = c["shapes"][0]
+        if (nv, nn, nm) != (self.nvar, self.nnodes, 1):
+            _err(L.HB_INVALID_STATE, f"Checkpoint shape mismatch for subnetwork 0: expected "
+                                     f"(nvar={self.nvar},nnodes={self.nnodes},nmodes=1), got ({nv},{nn},{nm})")
+        if c["hist"] is not None:
+            _err(L.HB_INVALID_STATE, "this checkpoint carries a multi-subnetwork history; "
+                                     "a single-network engine keeps its own layout (version 4)")
+        states = np.ascontiguousarray(c["states"][0], np.float32)
+        hist = c["hist_single"]
+        n = states.shape[0]
+        if n != 1 and n != self.n_sweep:
+            _err(L.HB_INVALID_STATE, f"Checkpoint holds {n} sweep points, expected 1 or {self.n_sweep}")
+        st = L.HbEngineState(int(c["step"]), float(c["dt"]), int(c["integrator"]), n, self.nnodes, self.nvar,
+                             0 if hist is None else hist.shape[1])
+        nsig = np.ascontiguousarray(c["nsig"], np.float32)
+        if hist is not None:
+            hist = np.ascontiguousarray(hist, np.float32)
+        L.check(L.lib().hb_engine_set_state(self._h, C.byref(st), nsig.ctypes.data_as(L.fp), len(nsig),
+                                            states.ctypes.data_as(L.fp),
+                                            hist.ctypes.data_as(L.fp) if hist is not None else None))
+        self.step, self.dt = int(c["step"]), float(c["dt"])
+
+    def _resume_multi(self, c: dict) -> None:
         subs, links, _, device = self._multi
-        if len(c["shapes"]) != len(subs):
-            _err(L.HB_INVALID_STATE, f"Checkpoint has {len(c['shapes'])} subnetworks, expected {len(subs)}")
-        for i, ((nv, nn, nm), S) in enumerate(zip(c["shapes"], subs)):
-            want = (L.MODEL_INFO[S.model][0], S.n_nodes, 1)
-            if (nv, nn, nm) != want:
-                _err(L.HB_INVALID_STATE, f"Checkpoint shape mismatch for subnetwork {i}: expected "
-                                         f"(nvar={want[0]},nnodes={want[1]},nmodes=1), got ({nv},{nn},{nm})")
-        from .multi import horizon
-        states, hist = c["states"], c["hist"]
-        H = horizon(links, int(self._max_delay))
-        if hist is not None and (hist.shape[1] != sum(S.n_nodes for S in subs) or hist.shape[2] != H):
-            _err(L.HB_INVALID_STATE, f"Checkpoint history has shape {tuple(hist.shape)}, expected "
-                                     f"(n_sweep, {sum(S.n_nodes for S in subs)}, {H}, 2)")
-        if c["n_sweep"] != self.n_sweep:
-            if c["n_sweep"] != 1:
-                _err(L.HB_INVALID_STATE, f"Checkpoint holds {c['n_sweep']} sweep points, expected {self.n_sweep}")
-            # one saved point continues every sweep point: states and history alike
-            states = [np.repeat(a, self.n_sweep, axis=0) for a in states]
-            if hist is not None:
-                hist = np.repeat(hist, self.n_sweep, axis=0)
+        states, hist = multi_checkpoint_states(c, subs, links, int(self._max_delay), self.n_sweep)
         self._step, self.dt, self._nsig = int(c["step"]), float(c["dt"]), c["nsig"]
         self._multi = (subs, links, c["integrator"], device)
         self._state, self._hist = states, hist

@@ -159,6 +159,34 @@ int hb_run_sweep_multi(hb_engine* e, const hb_sweep_point* points, int32_t n_poi
                        int64_t n_steps, uint32_t flags, hb_result* out);
 void hb_result_free(hb_result* r);
 
+/* What an engine carries from one run to the next, for checkpoints. */
+typedef struct {
+    int64_t step;            /* steps integrated so far (self.step) */
+    double dt;
+    int32_t integrator;      /* HB_EULER .. HB_RK4_STOCH */
+    int32_t n_state;         /* sweep points held; 0: the configured initial state (one point), step 0 */
+    int32_t n_nodes, nvar;   /* read-only: the engine's shape */
+    int64_t hist_floats;     /* delay history per point ([max_delay][n_nodes][channels], oldest
+                                first, hb_history_floats); 0 without delays */
+} hb_engine_state;
+
+/* HybridEngine::checkpoint (runtime.rs:519-561) saves step, dt, integrator,
+ * nsig and the states; this also returns each point's delay history, so that
+ * a resumed run continues bit for bit. Call with null buffers for the sizes,
+ * then again with nsig (nvar floats), states (max(n_state, 1) x n_nodes x
+ * nvar, node-major) and hist (n_state x hist_floats; ignored when either
+ * is 0). */
+int hb_engine_get_state(const hb_engine* e, hb_engine_state* st, float* nsig, float* states, float* hist);
+/* HybridEngine::resume (runtime.rs:576-742): restores step, dt, integrator,
+ * nsig (n_nsig 1 or nvar; 0 keeps the engine's) and the states. n_state 1 on
+ * an engine of several sweep points continues every point from that one
+ * state. hist null: a zero history, which is what a reference checkpoint
+ * (states only) resumes into; else n_state x st->hist_floats floats, and
+ * st->hist_floats must match the engine's. BOLD monitors are left as they
+ * are, as the reference's resume leaves its monitors. */
+int hb_engine_set_state(hb_engine* e, const hb_engine_state* st, const float* nsig, int32_t n_nsig,
+                        const float* states, const float* hist);
+
 /* ---------------- instance-level API (ragged batches) ---------------- */
 
 /* One fully specified simulation. Arrays are host memory, read at plan time. */

@@ -71,6 +71,51 @@ def test_plain_run_writes_reference_layout(gpu, orc, tmp_path):
 
 
 @pytest.mark.gpu
+def test_plain_run_checkpoint_then_resume(gpu, tmp_path):
+    """cli.rs:1057-1071: --resume before the run, --checkpoint after it. 50
+    steps, a checkpoint (the reference's version 2 file: one network, no
+    delays), then 50 more from it equal the last 50 of a 100-step run."""
+    from hyburn_amd import checkpoint as ck
+    cfg, _, ic = _files(tmp_path)
+    long_cfg = tmp_path / "sim100.toml"
+    long_cfg.write_text(open(cfg).read().replace("sim_length = 5.0", "sim_length = 10.0"))
+    ckpt = str(tmp_path / "a.ckpt")
+    assert main(["run", "-c", cfg, "-o", str(tmp_path / "a"), "--checkpoint", ckpt]) == 0
+    c = ck.read(ckpt)
+    assert c["version"] == 2 and c["step"] == 50
+    np.testing.assert_array_equal(c["states"][0][0].T, np.load(tmp_path / "a" / "state_final_sub0.npy")[:, :, 0])
+    assert main(["run", "-c", cfg, "-o", str(tmp_path / "b"), "--resume", ckpt]) == 0
+    assert main(["run", "-c", str(long_cfg), "-o", str(tmp_path / "w")]) == 0
+    tb, tw = np.load(tmp_path / "b" / "state_traj.npy"), np.load(tmp_path / "w" / "state_traj.npy")
+    assert tb.shape == (50, 2, 6, 1) and tw.shape == (100, 2, 6, 1)
+    np.testing.assert_allclose(tb, tw[50:], rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(np.load(tmp_path / "b" / "state_final_sub0.npy"),
+                               np.load(tmp_path / "w" / "state_final_sub0.npy"), rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.gpu
+def test_plain_run_of_several_subnetworks_checkpoint_then_resume(gpu, tmp_path):
+    """The multi-subnetwork plain run saves the batch file (version 3: states
+    and history) and resumes from it; the stimulus clock runs on the global
+    step, so 50 + 50 steps equal the last 50 of 100."""
+    from hyburn_amd import checkpoint as ck
+    cfg = tmp_path / "sim3.toml"
+    cfg.write_text(SIM3)
+    long_cfg = tmp_path / "sim3_100.toml"
+    long_cfg.write_text(SIM3.replace("sim_length = 5.0", "sim_length = 10.0"))
+    ckpt = str(tmp_path / "m.ckpt")
+    assert main(["run", "-c", str(cfg), "-o", str(tmp_path / "a"), "--checkpoint", ckpt]) == 0
+    assert ck.read(ckpt)["version"] == 3
+    assert main(["run", "-c", str(cfg), "-o", str(tmp_path / "b"), "--resume", ckpt]) == 0
+    assert main(["run", "-c", str(long_cfg), "-o", str(tmp_path / "w")]) == 0
+    tb, tw = np.load(tmp_path / "b" / "state_traj.npy"), np.load(tmp_path / "w" / "state_traj.npy")
+    np.testing.assert_allclose(tb, tw[tw.size // 2:], rtol=1e-6, atol=1e-7)
+    for i in (0, 1):
+        np.testing.assert_allclose(np.load(tmp_path / "b" / f"state_final_sub{i}.npy"),
+                                   np.load(tmp_path / "w" / f"state_final_sub{i}.npy"), rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.gpu
 def test_sweep_run_writes_tavg_with_reference_header(gpu, tmp_path):
     """tavg_sub0.npy carries the header [n_sweep, nvar, nnodes] over the
     node-major [n_sweep, nnodes, nvar] data, as cli.rs:958-970 writes it."""

@@ -74,6 +74,60 @@ def test_split_run_equals_one_run(gpu, case):
     np.testing.assert_array_equal(b.tavg[0].reshape(3, N, 2), (acc * np.float32(1.0 / T2)).astype(np.float32))
 
 
+@pytest.mark.parametrize("case", ["batch_window", "plain_no_delay", "per_edge_blocked_hbm", "stochastic"])
+def test_checkpoint_file_continues_bit_for_bit(gpu, case, tmp_path):
+    """HybridEngine::checkpoint / resume (runtime.rs:519-742) through a file
+    into a freshly built engine: after 90 steps and a checkpoint, 70 steps in
+    the new engine equal the 70 the checkpointing engine goes on to run, bit
+    for bit, and (without noise, which each run draws afresh from its global
+    step) the last 70 of one 160-step run. A single point without delays is
+    the reference's own version 2 file; the others carry their delay
+    histories (version 4)."""
+    from hyburn_amd import checkpoint as ck
+    N, T1, T2 = 16, 90, 70
+    kw, n_sweep, integ, nsig = {}, 3, "heun", 0.0
+    if case == "batch_window":
+        ip, ix, w, d = _csr_net(N, 51, max_delay=12)
+    elif case == "plain_no_delay":
+        ip, ix, w, _ = _csr_net(N, 52)
+        d = np.zeros(0, np.int32)
+        kw, n_sweep = dict(engine=L.ENGINE_PLAIN), 1
+    elif case == "per_edge_blocked_hbm":
+        rng = np.random.default_rng(53)
+        ip, ix, w, _ = _csr_net(N, 53)
+        d = rng.integers(8, 3000, len(ix)).astype(np.int32)
+        kw = dict(engine=L.ENGINE_PLAIN)
+    else:
+        ip, ix, w, d = _csr_net(N, 54, max_delay=6)
+        integ, nsig, kw = "heun_stochastic", 0.01, dict(noise_seed=99)
+    x0 = np.random.default_rng(55).normal(0, 0.3, (N, 2)).astype(np.float32)
+    cfg = _engine_cfg("Generic2dOscillator", N, x0, ip, ix, w, d, T=T1 + T2, cp=(0.2, 0.0), integ=integ, nsig=nsig)
+    vals = np.array([0.0, 0.3, -0.2][:n_sweep], np.float32)
+    param = SweepParam(0, 1)
+    two = BatchHybridEngine.from_config(cfg, n_sweep, **kw)
+    two.run_sweep(param, vals, T1, exact=True)
+    path = str(tmp_path / "split.ckpt")
+    two.checkpoint(path)
+    cont = two.run_sweep_with_trajectory(param, vals, T2, exact=True)
+    two.close()
+    assert ck.read(path)["version"] == (2 if case == "plain_no_delay" else 4)
+    three = BatchHybridEngine.from_config(cfg, n_sweep, **kw)
+    three.resume(path)
+    assert three.step == T1
+    b = three.run_sweep_with_trajectory(param, vals, T2, exact=True)
+    assert three.step == T1 + T2
+    three.close()
+    np.testing.assert_array_equal(b.trajectories[0], cont.trajectories[0])
+    np.testing.assert_array_equal(b.final_states[0], cont.final_states[0])
+    np.testing.assert_array_equal(b.tavg[0], cont.tavg[0])
+    if nsig == 0.0:
+        one = BatchHybridEngine.from_config(cfg, n_sweep, **kw)
+        whole = one.run_sweep_with_trajectory(param, vals, T1 + T2, exact=True)
+        one.close()
+        tw = whole.trajectories[0].reshape(T1 + T2, n_sweep, N, 2)
+        np.testing.assert_array_equal(b.trajectories[0].reshape(T2, n_sweep, N, 2), tw[T1:])
+
+
 def test_restart_flag_and_step_counter(gpu):
     N = 10
     ip, ix, w, d = _csr_net(N, 45)
