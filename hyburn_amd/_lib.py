@@ -18,6 +18,9 @@ G2DO, MPR, JANSEN_RIT, WILSON_COWAN, KURAMOTO, RWW = 0, 1, 2, 3, 4, 5
 LINEAR, SUP_HOPF, HOPFIELD, COOMBES_BYRNE_2D = 6, 7, 8, 9   # on the multi-subnetwork path only
 # the models the reference pins with trace fixtures (tests/model_reference_test.rs); multi-subnetwork path only
 EPILEPTOR_CODIM3, EPILEPTOR_CODIM3_SLOWMOD, EPILEPTOR_RS, ZERLAUT_FIRST, ZERLAUT_SECOND, KIONEX = 10, 11, 12, 13, 14, 15
+# the remaining models of the registry (config.rs:43-72); multi-subnetwork path only
+(COOMBES_BYRNE, GAST_SD, GAST_SF, LARTER_BREAKSPEAR, EPILEPTOR_2D, EPILEPTOR, RWW_EXC_INH, DECO_BALANCED,
+ ZETTERBERG_JANSEN, REDUCED_FHN, REDUCED_HR, DUMONT_GUTKIN) = range(16, 28)
 EULER, HEUN, RK4, EULER_STOCH, HEUN_STOCH, RK4_STOCH = 0, 1, 2, 3, 4, 5
 CF_LINEAR, CF_DIFFERENCE, CF_KURAMOTO, CF_SIGMOIDAL, CF_SCALED_LINEAR, CF_TANH, CF_SJR, CF_PRESIGMOIDAL = range(8)
 W_NONE, W_SCALAR_MEAN, W_SCALAR_DIAG, W_DENSE, W_CSR = 0, 1, 2, 3, 4
@@ -35,18 +38,26 @@ MODEL_NAMES = {
     "EpileptorCodim3": EPILEPTOR_CODIM3, "EpileptorCodim3SlowMod": EPILEPTOR_CODIM3_SLOWMOD,
     "EpileptorRestingState": EPILEPTOR_RS, "ZerlautAdaptationFirstOrder": ZERLAUT_FIRST,
     "ZerlautAdaptationSecondOrder": ZERLAUT_SECOND, "KIonEx": KIONEX,
+    "CoombesByrne": COOMBES_BYRNE, "GastSchmidtKnoscheSD": GAST_SD, "GastSchmidtKnoscheSF": GAST_SF,
+    "LarterBreakspear": LARTER_BREAKSPEAR, "Epileptor2D": EPILEPTOR_2D, "Epileptor": EPILEPTOR,
+    "ReducedWongWangExcInh": RWW_EXC_INH, "DecoBalancedExcInh": DECO_BALANCED,
+    "ZetterbergJansen": ZETTERBERG_JANSEN, "ReducedSetFitzHughNagumo": REDUCED_FHN,
+    "ReducedSetHindmarshRose": REDUCED_HR, "DumontGutkin": DUMONT_GUTKIN,
 }
 # (nvar, ncvar, nparams), config.rs:43-50
 MODEL_INFO = {G2DO: (2, 1, 12), MPR: (2, 2, 7), JANSEN_RIT: (6, 1, 13),
               WILSON_COWAN: (2, 1, 22), KURAMOTO: (1, 1, 1), RWW: (1, 1, 8),
               LINEAR: (1, 1, 1), SUP_HOPF: (2, 2, 2), HOPFIELD: (2, 2, 3),
               COOMBES_BYRNE_2D: (2, 2, 4), EPILEPTOR_CODIM3: (3, 1, 13), EPILEPTOR_CODIM3_SLOWMOD: (5, 1, 27),
-              EPILEPTOR_RS: (8, 3, 28), ZERLAUT_FIRST: (5, 1, 50), ZERLAUT_SECOND: (8, 1, 50), KIONEX: (5, 1, 14)}
+              EPILEPTOR_RS: (8, 3, 28), ZERLAUT_FIRST: (5, 1, 50), ZERLAUT_SECOND: (8, 1, 50), KIONEX: (5, 1, 14),
+              COOMBES_BYRNE: (4, 4, 5), GAST_SD: (4, 4, 9), GAST_SF: (4, 4, 9), LARTER_BREAKSPEAR: (3, 1, 32),
+              EPILEPTOR_2D: (2, 1, 12), EPILEPTOR: (6, 2, 17), RWW_EXC_INH: (2, 1, 19), DECO_BALANCED: (2, 1, 20),
+              ZETTERBERG_JANSEN: (12, 1, 18), REDUCED_FHN: (4, 2, 8), REDUCED_HR: (6, 2, 12),
+              DUMONT_GUTKIN: (8, 4, 14)}
 MULTI_ONLY = (LINEAR, SUP_HOPF, HOPFIELD, COOMBES_BYRNE_2D, EPILEPTOR_CODIM3, EPILEPTOR_CODIM3_SLOWMOD, EPILEPTOR_RS,
-              ZERLAUT_FIRST, ZERLAUT_SECOND, KIONEX)
-# hybrid_integrator: deterministic Heun for these models (model_prefers_heun, dfun.rs:1871-1873;
-# Epileptor and Epileptor2D are not built)
-PREFERS_HEUN = (G2DO, EPILEPTOR_RS, EPILEPTOR_CODIM3, EPILEPTOR_CODIM3_SLOWMOD, KIONEX)
+              ZERLAUT_FIRST, ZERLAUT_SECOND, KIONEX) + tuple(range(16, 28))
+# hybrid_integrator: deterministic Heun for these models (model_prefers_heun, dfun.rs:1871-1873)
+PREFERS_HEUN = (G2DO, EPILEPTOR_RS, EPILEPTOR_CODIM3, EPILEPTOR_CODIM3_SLOWMOD, KIONEX, EPILEPTOR, EPILEPTOR_2D)
 # the batch-native sweep index per model, dfun.rs:49-72
 SWEEP_INDEX = {G2DO: 1, MPR: 4, JANSEN_RIT: 12, WILSON_COWAN: 18, KURAMOTO: 0, RWW: 7}
 SWEEP_INDEX.update({m: -1 for m in MULTI_ONLY})

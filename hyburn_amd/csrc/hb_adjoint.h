@@ -11,7 +11,9 @@ struct alignas(16) AdjInst {
     int32_t np, cfn, engine, has_coupling;
     int32_t seg_len, n_seg, ring_depth, aring_depth;   // S, ceil(T/S), forward and adjoint ring depths
     int32_t ring_lds, aring_lds, bold_period, n_win;
-    int32_t loss_kind, bold_fused, pad1, pad2;   // bold_fused: BOLD forward in helper waves of the blocked forward
+    int32_t loss_kind, bold_fused, rc_fused, pad2;   // bold_fused: BOLD forward in helper waves of the blocked
+                                                     // forward; rc_fused: segment recompute in helper waves of
+                                                     // the blocked reverse (double tape)
     float dt, dt_half, dt_sixth, pad3;
     float mc[kMC];
     float cc[kCC];
@@ -28,7 +30,7 @@ struct alignas(16) AdjInst {
     float* ring;                // HBM ring (forward, then adjoint) when not in LDS
     float* glog;                // [T][N][np] gathered coupling sums
     float* ckpt;                // [n_seg][N][NV]
-    float* tape;                // [S + 1][N][NV]
+    float* tape;                // [S + 1][N][NV], two of them with rc_fused (segments by parity)
     float* zlog;                // [n_win][N] BOLD window inputs
     double* bstate;             // [n_win][N][4] Balloon-Windkessel states (FP64)
     double* bold;               // [n_win][N] native BOLD samples (FP64)
@@ -55,6 +57,7 @@ struct AdjLaunch {
     bool blocked;               // delay-blocked kernels (min delay >= 8, npt 1)
     bool lring;                 // blocked kernels: rings in LDS
     bool bold_fused;            // blocked forward carries the BOLD monitor in helper waves (launched 2 x block)
+    bool rc_fused;              // blocked reverse recomputes segments in helper waves (launched 2 x block)
     int32_t ic;                 // integrator class of every instance (0 Euler, 1 Heun, 2 RK4)
     int32_t cfk;                // HB_CF_LINEAR when every instance couples linearly (blocked kernels), else -1
     int32_t block;

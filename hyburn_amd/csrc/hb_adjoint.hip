@@ -1754,7 +1754,17 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
     const int cfn = CF >= 0 ? CF : V.cfn;   // coupling function, fixed at compile time when CF >= 0
     const int N = V.N, T = V.T, H = V.H, Hs = V.Hs, Sg = V.Sg, P = V.P;
     const int tid = threadIdx.x, BS = blockDim.x;
-    const bool own = tid < N;
+    // Recompute helper waves (AdjInst::rc_fused, the workgroup launched twice
+    // as wide): thread NB + n recomputes node n's states of the segment below
+    // the one being reversed, into the other half of a double tape, and takes
+    // part in exactly the barriers the reverse threads execute.
+    // (the 1024-thread instantiation never takes them: the doubled workgroup must fit 512)
+    const bool rcf = MAXT <= 512 && u32(Ip->rc_fused) != 0;
+    const int NB = rcf ? (BS >> 1) : BS;
+    const bool helper = tid >= NB;
+    const int me = helper ? tid - NB : tid;   // the node this thread works on
+    const bool own = !helper && tid < N;      // reverse role
+    const bool hown = helper && me < N;       // recompute role
     const size_t chan = (size_t)N * Hs;
     // channels interleaved per slot in an HBM ring, as in the forward save
     constexpr bool ILV = !LRING && NP > 1;
@@ -1774,7 +1784,7 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
     float mc[kMC], jac[kMC], lam[NV];
 #pragma unroll
     for (int i = 0; i < kMC; ++i) {
-        mc[i] = (V.node_mc && own) ? V.node_mc[(size_t)tid * kMC + i] : uf32(Ip->mc[i]);
+        mc[i] = (V.node_mc && (own || hown)) ? V.node_mc[(size_t)me * kMC + i] : uf32(Ip->mc[i]);
         jac[i] = (V.node_jac && own) ? V.node_jac[(size_t)tid * kMC + i] : 0.0f;
     }
 #pragma unroll
@@ -1841,16 +1851,62 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
     };
     __syncthreads();
 
+    // tape rows of segment sg: one buffer, or with helper waves two halves (the
+    // segment being reversed and the one being recomputed below it)
+    const size_t tape_half = (size_t)(Sg + 1) * N * NV;
+    auto tape_of = [&](int sg) { return V.tape + (rcf ? (size_t)(sg & 1) * tape_half : (size_t)0); };
+    // helper role: node me's steps [ta, tb) of the segment starting at t0 (at
+    // most kK), from x (the state at ta), into tape rows ta - t0 + 1 ...
+    auto rc_chunk = [&](gp<float> tq, int t0, int ta, int tb, float (&x)[NV]) {
+        float gq[kK][NP];
+#pragma unroll
+        for (int m = 0; m < kK; ++m) {
+            if (ta + m < tb) {
+                const gp<float> gl = V.glog + ((size_t)(ta + m) * N + me) * NP;
+#pragma unroll
+                for (int c = 0; c < NP; ++c) gq[m][c] = gl[c];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < kK; ++m) {
+            const int t = ta + m;
+            if (t < tb) {
+                float own_pre[NP], cpl[NC], z[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) z[v] = S.stoch ? V.noise[((size_t)t * N + me) * NV + v] * nsc[v] : 0.0f;
+                pre<NC, NP>(cfn, cc, false, false, x, own_pre);
+                if (V.has_c) post<NC, NP>(cfn, cc, false, gq[m], own_pre, cpl);
+                else {
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
+                }
+                integ_step<M, IC>(S, mc, cpl, z, x);
+                gp<float> tp = tq + ((size_t)(t - t0 + 1) * N + me) * NV;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) tp[v] = x[v];
+            }
+        }
+    };
+
     for (int seg = V.nseg - 1; seg >= 0; --seg) {
         const int t0s = seg * Sg, t1s = min(T, t0s + Sg);
+        const gp<float> tape = tape_of(seg);
         // ---- recompute states t0s..t1s from the checkpoint and the coupling log
         // (with one segment the forward wrote the whole trajectory); the log
         // and noise rows of the next kK steps are loaded while the current kK
-        // steps integrate (double buffer)
-        if (own && V.nseg > 1) {
+        // steps integrate (double buffer). With helper waves, only the last
+        // segment is recomputed here (by the helpers); each other segment was
+        // recomputed while the segment above it was reversed.
+        if (rcf && hown && seg == V.nseg - 1) {
             float x[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) { x[v] = V.ckpt[((size_t)seg * N + tid) * NV + v]; V.tape[(size_t)tid * NV + v] = x[v]; }
+            for (int v = 0; v < NV; ++v) { x[v] = V.ckpt[((size_t)seg * N + me) * NV + v]; tape[(size_t)me * NV + v] = x[v]; }
+            for (int ta = t0s; ta < t1s; ta += kK) rc_chunk(tape, t0s, ta, min(t1s, ta + kK), x);
+        }
+        if (!rcf && own && V.nseg > 1) {
+            float x[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { x[v] = V.ckpt[((size_t)seg * N + tid) * NV + v]; tape[(size_t)tid * NV + v] = x[v]; }
             float gq0[kK][NP], gq1[kK][NP];
             auto fetch = [&](float (&gq)[kK][NP], int tg) {
 #pragma unroll
@@ -1878,7 +1934,7 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
                             for (int c = 0; c < NC; ++c) cpl[c] = 0.0f;
                         }
                         integ_step<M, IC>(S, mc, cpl, z, x);
-                        gp<float> tp = V.tape + ((size_t)(t - t0s + 1) * N + tid) * NV;
+                        gp<float> tp = tape + ((size_t)(t - t0s + 1) * N + tid) * NV;
 #pragma unroll
                         for (int v = 0; v < NV; ++v) tp[v] = x[v];
                     }
@@ -1899,7 +1955,7 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
             for (int r = w; r <= t1s - t0s; r += nwv) {
                 float cs = 0.0f, sn = 0.0f;
                 for (int n = l; n < N; n += 64) {
-                    const float th = V.tape[((size_t)r * N + n) * NV];
+                    const float th = tape[((size_t)r * N + n) * NV];
                     cs += cosf(th);
                     sn += sinf(th);
                 }
@@ -1923,7 +1979,7 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
         if (seg == V.nseg - 1 && own) {
             float xT[NV];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) xT[v] = V.tape[((size_t)(t1s - t0s) * N + tid) * NV + v];
+            for (int v = 0; v < NV; ++v) xT[v] = tape[((size_t)(t1s - t0s) * N + tid) * NV + v];
             seed(T, t0s, xT, lam, seed_win ? V.seedw[(size_t)((T - 1) / P) * N + tid] : 0.0f);
         }
         // ---- blocks of reverse steps over [max(t0s, 1), t1s - 1], highest first
@@ -1941,7 +1997,7 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
             for (int m = 0; m < kK; ++m) {
                 if (m < nk) {
                     const int st = tb + m;
-                    const gp<float> tp = V.tape + ((size_t)(st - t0s) * N + tid) * NV;
+                    const gp<float> tp = tape + ((size_t)(st - t0s) * N + tid) * NV;
                     const gp<float> gl = V.glog + ((size_t)st * N + tid) * NP;
 #pragma unroll
                     for (int v = 0; v < NV; ++v) xx[m][v] = tp[v];
@@ -1955,10 +2011,30 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
                 }
             }
         };
+        // helper role: segment seg - 1, one chunk of kK steps per reverse block
+        float xr[NV];
+        int ra = 0, rb = 0;
+        const int r0 = (seg - 1) * Sg;
+        const gp<float> tq = tape_of(seg - 1);
+        if (rcf && hown && seg > 0) {
+            ra = r0;
+            rb = seg * Sg;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { xr[v] = V.ckpt[((size_t)(seg - 1) * N + me) * NV + v]; tq[(size_t)me * NV + v] = xr[v]; }
+        }
         float xb[kK][NV], gb[kK][NP], sdb[kK];
         float xn[kPrefetch ? kK : 1][NV], gn[kPrefetch ? kK : 1][NP], sdn[kPrefetch ? kK : 1];
         bool have = false;   // xb / gb already hold the block (prefetched)
         for (int ttop = t1s - 1; ttop >= tlo; ttop -= kK) {
+            if (helper) {   // one barrier per block, as the reverse threads
+                if (ra < rb) {
+                    const int e = min(rb, ra + kK);
+                    rc_chunk(tq, r0, ra, e, xr);
+                    ra = e;
+                }
+                __syncthreads();
+                continue;
+            }
             const int kk = min(kK, ttop - tlo + 1), tbot = ttop - kk + 1;
             if (!have) load_block(xb, gb, sdb, tbot, kk);
             const int ttop2 = ttop - kK, kk2 = min(kK, ttop2 - tlo + 1), tbot2 = ttop2 - kk2 + 1;
@@ -2039,6 +2115,11 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
                 }
             }
         }
+        while (ra < rb) {   // a short last segment leaves steps of the one below
+            const int e = min(rb, ra + kK);
+            rc_chunk(tq, r0, ra, e, xr);
+            ra = e;
+        }
         if (seg > 0) __syncthreads();
     }
     // ---- step 0: the delayed terms (steps d) plus every edge's current-state read
@@ -2048,7 +2129,7 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
         if (own) {
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                x0v[v] = V.tape[(size_t)tid * NV + v];
+                x0v[v] = V.tape[(size_t)tid * NV + v];   // segment 0: the first tape half
                 z0[v] = S.stoch ? V.noise[(size_t)tid * NV + v] * nsc[v] : 0.0f;
             }
 #pragma unroll
@@ -2121,8 +2202,9 @@ template <int M, int NP>
 static const void* pick(const AdjLaunch& L, bool rev) {
     if (L.blocked) {
         const bool lin = L.cfk == HB_CF_LINEAR;
-        // the launch width: the forward save of a fused group doubles the block
-        const bool wide = (rev ? L.block : (L.bold_fused ? 2 * L.block : L.block)) > 512;
+        // the launch width: the forward save of a BOLD-fused group and the
+        // reverse of a recompute-fused group double the block
+        const bool wide = (rev ? (L.rc_fused ? 2 * L.block : L.block) : (L.bold_fused ? 2 * L.block : L.block)) > 512;
         switch (L.ic) {
         case 0: return lin ? ptr_b<M, NP, 0, HB_CF_LINEAR>(rev, L.lring, wide) : ptr_b<M, NP, 0, -1>(rev, L.lring, wide);
         case 1: return lin ? ptr_b<M, NP, 1, HB_CF_LINEAR>(rev, L.lring, wide) : ptr_b<M, NP, 1, -1>(rev, L.lring, wide);
@@ -2169,6 +2251,7 @@ hipError_t HB_ADJ_XCAT(adj_unit_, HB_ADJ_UNIT)(int op, const AdjLaunch* L, hipSt
                     q.cfk = (v & 1) && (v % 6) >= 2 ? HB_CF_LINEAR : -1;
                     q.block = v >= 6 ? 1024 : 64;   // both launch widths of the blocked kernels
                     q.bold_fused = 0;
+                    q.rc_fused = 0;
                     for (int rev = 0; rev < 2; ++rev) {
                         const void* k = adj::kernel_of(q, rev != 0);
                         if (!k) continue;
@@ -2182,8 +2265,9 @@ hipError_t HB_ADJ_XCAT(adj_unit_, HB_ADJ_UNIT)(int op, const AdjLaunch* L, hipSt
     if (!k) return hipErrorInvalidValue;
     const AdjInst* d = L->insts;
     void* args[] = {(void*)&d};
-    // the forward save of a fused group carries the BOLD helper waves
-    const int block = (op == 0 && L->bold_fused) ? 2 * L->block : L->block;
+    // the forward save of a fused group carries the BOLD helper waves, the
+    // reverse of a recompute-fused group its recompute helper waves
+    const int block = ((op == 0 && L->bold_fused) || (op == 1 && L->rc_fused)) ? 2 * L->block : L->block;
     return hipLaunchKernel(k, dim3(L->n), dim3(block), args, op == 0 ? L->lds_fwd : L->lds_rev, s);
 }
 #else

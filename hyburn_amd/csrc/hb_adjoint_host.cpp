@@ -229,6 +229,21 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         if (f && 2 * blk <= 512 && !(opts && (opts->flags & HB_FLAG_SEPARATE_BOLD)))
             for (int i : kv.second) bold_fused[i] = 1;
     }
+    // With checkpoint segments, the blocked reverse recomputes the segment
+    // below the one it reverses in helper waves (one helper thread per node,
+    // workgroup doubled, two tapes) when the doubled workgroup fits 512
+    // threads; only the last segment's recompute is left on the critical path.
+    std::vector<int> rc_fused(n, 0);
+    for (auto& kv : groups) {
+        bool f = std::get<3>(kv.first) != 0;
+        int blk = 64;
+        for (int i : kv.second) {
+            f = f && sh[i].nseg > 1;
+            blk = std::max(blk, sh[i].block);
+        }
+        if (f && 2 * blk <= 512)
+            for (int i : kv.second) rc_fused[i] = 1;
+    }
 
     // ---- arena layout
     struct Off {
@@ -260,7 +275,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         o.ring = o.has_ring ? A.take(s.ring_b) : 0;
         o.glog = A.take(T * N * s.np * 4);
         o.ckpt = A.take((size_t)s.nseg * N * s.nv * 4);
-        o.tape = A.take((size_t)(s.S + 1) * N * s.nv * 4);
+        o.tape = A.take((size_t)(rc_fused[i] ? 2 : 1) * (s.S + 1) * N * s.nv * 4);
         const size_t nw = s.nwin;
         o.zlog = A.take(nw * N * 4);
         o.bstate = A.take(nw * N * 32);
@@ -333,6 +348,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
         D.n_win = s.nwin;
         D.loss_kind = Sp.loss_kind;
         D.bold_fused = bold_fused[i];
+        D.rc_fused = rc_fused[i];
         D.dt = I.dt; D.dt_half = I.dt * 0.5f; D.dt_sixth = I.dt / 6.0f;
         model_consts(I.model, I.params, D.mc);
         int classic = 0;
@@ -468,6 +484,7 @@ int hb_adjoint_plan_create(const hb_instance* inst, const hb_adjoint_spec* spec,
             if (sh[i].ring_lds) lds = std::max(lds, (int)sh[i].ring_b);
         }
         L.bold_fused = bold_fused[kv.second[0]] != 0;
+        L.rc_fused = rc_fused[kv.second[0]] != 0;
         L.block = blk;
         L.lds_fwd = lds;
         L.lds_rev = lds;

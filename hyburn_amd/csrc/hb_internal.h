@@ -31,15 +31,20 @@ HB_HD int model_nvar(int m) {
     switch (m) {
     case HB_JANSEN_RIT: return 6;
     case HB_KURAMOTO: case HB_RWW: case HB_LINEAR: return 1;
-    case HB_EPILEPTOR_CODIM3: return 3;
+    case HB_EPILEPTOR_CODIM3: case HB_LARTER_BREAKSPEAR: return 3;
+    case HB_COOMBES_BYRNE: case HB_GAST_SD: case HB_GAST_SF: case HB_REDUCED_FHN: return 4;
     case HB_EPILEPTOR_CODIM3_SLOWMOD: case HB_ZERLAUT_FIRST: case HB_KIONEX: return 5;
-    case HB_EPILEPTOR_RS: case HB_ZERLAUT_SECOND: return 8;
+    case HB_EPILEPTOR: case HB_REDUCED_HR: return 6;
+    case HB_EPILEPTOR_RS: case HB_ZERLAUT_SECOND: case HB_DUMONT_GUTKIN: return 8;
+    case HB_ZETTERBERG_JANSEN: return 12;
     default: return 2;
     }
 }
 HB_HD int model_ncvar(int m) {
     if (m == HB_EPILEPTOR_RS) return 3;
-    return (m == HB_MPR || m == HB_SUP_HOPF || m == HB_HOPFIELD || m == HB_COOMBES_BYRNE_2D) ? 2 : 1;
+    if (m == HB_COOMBES_BYRNE || m == HB_GAST_SD || m == HB_GAST_SF || m == HB_DUMONT_GUTKIN) return 4;
+    return (m == HB_MPR || m == HB_SUP_HOPF || m == HB_HOPFIELD || m == HB_COOMBES_BYRNE_2D || m == HB_EPILEPTOR ||
+            m == HB_REDUCED_FHN || m == HB_REDUCED_HR) ? 2 : 1;
 }
 HB_HD int model_nparams(int m) {
     switch (m) {
@@ -49,16 +54,20 @@ HB_HD int model_nparams(int m) {
     case HB_EPILEPTOR_CODIM3: return 13; case HB_EPILEPTOR_CODIM3_SLOWMOD: return 27;
     case HB_EPILEPTOR_RS: return 28; case HB_ZERLAUT_FIRST: case HB_ZERLAUT_SECOND: return 50;
     case HB_KIONEX: return 14;
+    case HB_COOMBES_BYRNE: return 5; case HB_GAST_SD: case HB_GAST_SF: return 9;
+    case HB_LARTER_BREAKSPEAR: return 32; case HB_EPILEPTOR_2D: case HB_REDUCED_HR: return 12;
+    case HB_EPILEPTOR: return 17; case HB_RWW_EXC_INH: return 19; case HB_DECO_BALANCED: return 20;
+    case HB_ZETTERBERG_JANSEN: return 18; case HB_DUMONT_GUTKIN: return 14;
     default: return 8;
     }
 }
-// Models 10-15 keep their constants in a 64-slot table (model_consts_ext).
-HB_HD bool model_ext(int m) { return m >= HB_EPILEPTOR_CODIM3 && m <= HB_KIONEX; }
+// Models 10-27 keep their constants in a 64-slot table (model_consts_ext).
+HB_HD bool model_ext(int m) { return m >= HB_EPILEPTOR_CODIM3 && m <= HB_DUMONT_GUTKIN; }
 constexpr int kMCX = 64;
 // hybrid_integrator's deterministic-Heun models (model_prefers_heun, dfun.rs:1871-1873)
 HB_HD bool model_prefers_heun(int m) {
     return m == HB_G2DO || m == HB_EPILEPTOR_RS || m == HB_EPILEPTOR_CODIM3 || m == HB_EPILEPTOR_CODIM3_SLOWMOD ||
-           m == HB_KIONEX;
+           m == HB_KIONEX || m == HB_EPILEPTOR || m == HB_EPILEPTOR_2D;
 }
 // The reference's single batch-sweepable parameter per model (dfun.rs:49-72).
 HB_HD int model_sweep_index(int m) {
@@ -164,7 +173,7 @@ HB_HD void model_consts_t(int m, const S* p, S* mc) {
 
 HB_HD void model_consts(int m, const float* p, float* mc) { model_consts_t<float>(m, p, mc); }
 
-// Constants of models 10-15 (kMCX slots): the parameters, then the products
+// Constants of models 10-27 (kMCX slots): the parameters, then the products
 // the reference forms on the host once per call, in its f32 order.
 inline void model_consts_ext(int m, const float* p, float* c) {
     const float eps = kEps;
@@ -231,6 +240,78 @@ inline void model_consts_ext(int m, const float* p, float* c) {
         c[17] = 2.0f * p[8];
         c[18] = -(p[12] / 2160.0f);
         c[19] = 26.64f * logf(112.0f / 5.0f);
+        break;
+    case HB_COOMBES_BYRNE:   // dfun.rs:609-637: delta/pi, pi*pi, k*pi
+        c[5] = p[0] / kPi; c[6] = kPi * kPi; c[7] = p[3] * kPi;
+        break;
+    case HB_GAST_SD: case HB_GAST_SF:   // dfun.rs:639-716
+        c[9] = p[0] > eps ? p[0] : eps;
+        c[10] = p[1] > eps ? p[1] : eps;
+        c[11] = 1.0f / c[9];
+        c[12] = p[4] / (kPi * c[9]);
+        c[13] = ((kPi * kPi) * c[9]) * c[9];
+        c[14] = p[6] + p[3];
+        c[15] = p[5] * c[9];
+        c[16] = 1.0f / c[10];
+        break;
+    case HB_LARTER_BREAKSPEAR:   // dfun.rs:718-766
+        c[31] = p[31] > eps ? p[31] : eps;
+        c[32] = 1.0f / p[12]; c[33] = 1.0f / p[13]; c[34] = 1.0f / p[14];
+        c[35] = 0.5f * p[28]; c[36] = 1.0f / p[15];
+        c[37] = 0.5f * p[29]; c[38] = 1.0f / p[16];
+        c[39] = (p[23] * p[25]) * p[19];
+        c[40] = ((1.0f - p[23]) * p[25]) * p[19];
+        c[41] = (1.0f - p[23]) * p[19];
+        c[42] = p[23] * p[19];
+        c[43] = p[20] * p[24];
+        c[44] = ((p[30] * p[4]) * p[22]) / c[31];
+        c[45] = p[21] * p[24];
+        c[46] = p[30] * p[22];
+        break;
+    case HB_EPILEPTOR_2D:   // dfun.rs:776-818
+        c[12] = p[6] - p[3];
+        c[13] = p[5] + p[0];
+        c[14] = -4.0f * p[1];
+        c[15] = p[10] * p[7];
+        break;
+    case HB_EPILEPTOR:   // dfun.rs:820-874
+        c[17] = -4.0f * p[2];
+        c[18] = p[15] * p[7];
+        c[19] = -0.01f * p[15];
+        break;
+    case HB_RWW_EXC_INH: case HB_DECO_BALANCED: {   // dfun.rs:884-950
+        const float te = p[4] > eps ? p[4] : eps, ti = p[12] > eps ? p[12] : eps;
+        c[19] = p[17] * p[6];
+        c[20] = p[7] * p[15] + p[16];
+        c[21] = 1.0f / te;
+        c[22] = p[14] * p[15];
+        c[23] = 1.0f / ti;
+        if (m == HB_DECO_BALANCED) {
+            c[24] = p[0] * p[19]; c[25] = (-p[1]) * p[19];
+            c[26] = p[8] * p[19]; c[27] = (-p[9]) * p[19];
+        }
+        break; }
+    case HB_ZETTERBERG_JANSEN:   // dfun.rs:1212-1268
+        c[18] = 2.0f * p[4];
+        c[19] = p[0] * p[2]; c[20] = p[1] * p[3];
+        c[21] = p[2] * p[2]; c[22] = p[3] * p[3];
+        c[23] = c[19] * p[7]; c[24] = p[13] * p[8]; c[25] = p[14] * p[9];
+        c[26] = c[19] * p[10]; c[27] = c[20] * p[11];
+        c[28] = 2.0f * p[2]; c[29] = 2.0f * p[3];
+        break;
+    case HB_REDUCED_HR:   // dfun.rs:1302-1348
+        c[12] = p[0] * p[5];
+        c[13] = (-p[0]) * p[6];
+        break;
+    case HB_DUMONT_GUTKIN:   // dfun.rs:1350-1400
+        c[14] = p[3] > eps ? p[3] : eps;
+        c[15] = p[7] > eps ? p[7] : eps;
+        c[16] = p[8] > eps ? p[8] : eps;
+        c[17] = 1.0f / c[14]; c[18] = 1.0f / c[15]; c[19] = 1.0f / c[16];
+        c[20] = p[1] / (kPi * c[14]);
+        c[21] = ((kPi * kPi) * c[14]) * c[14];
+        c[22] = p[5] / (kPi * c[15]);
+        c[23] = ((kPi * kPi) * c[15]) * c[15];
         break;
     default:
         break;

@@ -24,6 +24,18 @@ template <> struct Tr<HB_EPILEPTOR_RS> { static constexpr int NV = 8, NC = 3; };
 template <> struct Tr<HB_ZERLAUT_FIRST> { static constexpr int NV = 5, NC = 1; };
 template <> struct Tr<HB_ZERLAUT_SECOND> { static constexpr int NV = 8, NC = 1; };
 template <> struct Tr<HB_KIONEX> { static constexpr int NV = 5, NC = 1; };
+template <> struct Tr<HB_COOMBES_BYRNE> { static constexpr int NV = 4, NC = 4; };
+template <> struct Tr<HB_GAST_SD> { static constexpr int NV = 4, NC = 4; };
+template <> struct Tr<HB_GAST_SF> { static constexpr int NV = 4, NC = 4; };
+template <> struct Tr<HB_LARTER_BREAKSPEAR> { static constexpr int NV = 3, NC = 1; };
+template <> struct Tr<HB_EPILEPTOR_2D> { static constexpr int NV = 2, NC = 1; };
+template <> struct Tr<HB_EPILEPTOR> { static constexpr int NV = 6, NC = 2; };
+template <> struct Tr<HB_RWW_EXC_INH> { static constexpr int NV = 2, NC = 1; };
+template <> struct Tr<HB_DECO_BALANCED> { static constexpr int NV = 2, NC = 1; };
+template <> struct Tr<HB_ZETTERBERG_JANSEN> { static constexpr int NV = 12, NC = 1; };
+template <> struct Tr<HB_REDUCED_FHN> { static constexpr int NV = 4, NC = 2; };
+template <> struct Tr<HB_REDUCED_HR> { static constexpr int NV = 6, NC = 2; };
+template <> struct Tr<HB_DUMONT_GUTKIN> { static constexpr int NV = 8, NC = 4; };
 
 // ---------------------------------------------------------------- models ---
 
@@ -415,6 +427,174 @@ template <> __device__ __forceinline__ void dfun<HB_KIONEX>(const float* c, bool
     dx[4] = ((-ko) + c[1]) * c[13];
 }
 
+// CoombesByrne, dfun.rs:609-637 (c[5] = delta/pi, c[6] = pi^2, c[7] = k pi)
+template <> __device__ __forceinline__ void dfun<HB_COOMBES_BYRNE>(const float* c, bool, float, const float* x,
+                                                                  const float* cpl, float* dx) {
+    const float r = x[0], v = x[1], g = x[2], q = x[3];
+    dx[0] = ((((v * r) * 2.0f) + c[5]) - g * r) + cpl[0];
+    dx[1] = (((v * v) + c[4]) - r * (r * c[6])) + ((-v) + c[2]) * g;
+    dx[2] = q * c[1];
+    dx[3] = (((r * c[7]) - g) - q * 2.0f) * c[1];
+}
+
+// GastSchmidtKnoscheSD / SF, dfun.rs:639-716. The reference multiplies v by the
+// whole bracket (method calls bind before the * operator there); so does this.
+template <int M> __device__ __forceinline__ void gast_dfun(const float* c, const float* x, const float* cpl,
+                                                           float* dx) {
+    const float r = x[0], v = x[1], a = x[2], b = x[3];
+    dx[0] = (((r * v) * 2.0f) + c[12]) * c[11];
+    float br = (v - r * (r * c[13])) + c[14];
+    if constexpr (M == HB_GAST_SD) br = br + (r * c[15]) * ((-a) + 1.0f);
+    else br = (br + r * c[15]) - a;
+    dx[1] = (v * ((br + cpl[0] * c[7]) + cpl[1] * c[8])) * c[11];
+    dx[2] = b * c[16];
+    dx[3] = (((b * -2.0f) - a) + r * c[2]) * c[16];
+}
+template <> __device__ __forceinline__ void dfun<HB_GAST_SD>(const float* c, bool, float, const float* x,
+                                                            const float* cpl, float* dx) {
+    gast_dfun<HB_GAST_SD>(c, x, cpl, dx);
+}
+template <> __device__ __forceinline__ void dfun<HB_GAST_SF>(const float* c, bool, float, const float* x,
+                                                            const float* cpl, float* dx) {
+    gast_dfun<HB_GAST_SF>(c, x, cpl, dx);
+}
+
+// LarterBreakspear, dfun.rs:718-766
+template <> __device__ __forceinline__ void dfun<HB_LARTER_BREAKSPEAR>(const float* c, bool, float, const float* x,
+                                                                      const float* cpl, float* dx) {
+    const float V = x[0], W = x[1], Z = x[2];
+    const float m_ca = 0.5f * (tanhf((V + (-c[9])) * c[32]) + 1.0f);
+    const float m_na = 0.5f * (tanhf((V + (-c[10])) * c[33]) + 1.0f);
+    const float m_k = 0.5f * (tanhf((V + (-c[11])) * c[34]) + 1.0f);
+    const float qv = c[35] * (tanhf((V + (-c[26])) * c[36]) + 1.0f);
+    const float qz = c[37] * (tanhf((Z + (-c[27])) * c[38]) + 1.0f);
+    const float tca = (c[0] + qv * c[40]) + cpl[0] * c[39];
+    const float tna = ((m_na * c[3]) + qv * c[41]) + cpl[0] * c[42];
+    dx[0] = ((((((-tca) * m_ca) * (V + (-c[5])) - (W * c[1]) * (V + (-c[6]))) - (V + (-c[7])) * c[2]) -
+              tna * (V + (-c[8]))) - (Z * c[18]) * qz) * c[30] + c[43];
+    dx[1] = (m_k - W) * c[44];
+    dx[2] = ((V * c[17]) * qv + c[45]) * c[46];
+}
+
+// the Epileptor slow-variable term: -(-0.1 z)^7 for z < 0, else 0
+__device__ __forceinline__ float epi_ydot(float z) {
+    const float zm = z * -0.1f;
+    return z < 0.0f ? -((((((zm * zm) * zm) * zm) * zm) * zm) * zm) : 0.0f;
+}
+
+// Epileptor2D, dfun.rs:776-818
+template <> __device__ __forceinline__ void dfun<HB_EPILEPTOR_2D>(const float* c, bool, float, const float* x,
+                                                                 const float* cpl, float* dx) {
+    const float x1 = x[0], z = x[1];
+    float f1;
+    if (x1 < 0.0f) f1 = (x1 * x1) * c[2] + x1 * c[12];
+    else { const float z4 = z + -4.0f; f1 = ((z4 * z4) * -0.6f + (-c[4])) + x1 * c[6]; }
+    dx[0] = ((((-z) + c[13]) + cpl[0] * c[8]) - f1 * x1) * c[10];
+    const float yz = epi_ydot(z);
+    const float h = c[11] > 0.5f ? (x1 + 3.0f) / (expf(-((x1 + 0.5f) * 10.0f)) + 1.0f) + yz : (x1 * 4.0f + c[14]) + yz;
+    dx[1] = ((h - z) + cpl[0] * c[9]) * c[15];
+}
+
+// Epileptor, dfun.rs:820-874
+template <> __device__ __forceinline__ void dfun<HB_EPILEPTOR>(const float* c, bool, float, const float* x,
+                                                              const float* cpl, float* dx) {
+    const float x1 = x[0], y1 = x[1], z = x[2], x2 = x[3], y2 = x[4], g = x[5];
+    float f1;
+    if (x1 < 0.0f) f1 = (-(x1 * x1)) * c[3] + x1 * c[4];
+    else { const float z4 = z + -4.0f; f1 = ((-x2) + c[8]) + (z4 * z4) * 0.6f; }
+    dx[0] = ((((y1 - z) + cpl[0] * c[12]) + f1 * x1) + c[0]) * c[15];
+    dx[1] = (((-((x1 * x1) * c[6])) + c[5]) - y1) * c[15];
+    const float yz = epi_ydot(z);
+    const float h = c[16] > 0.5f ? (x1 + 3.0f) / (expf(-((x1 + 0.5f) * 10.0f)) + 1.0f) + yz : (x1 * 4.0f + c[17]) + yz;
+    dx[2] = ((h - z) + cpl[0] * c[14]) * c[18];
+    dx[3] = ((((((x2 - (x2 * x2) * x2) - y2) + c[1]) + g * c[11]) - (z + -3.5f) * 0.3f) + cpl[1] * c[13]) * c[15];
+    const float xp = x2 + 0.25f;
+    const float f2 = xp < 0.0f ? 0.0f : xp * c[10];
+    dx[4] = ((f2 - y2) * c[15]) / c[9];
+    dx[5] = ((-x1) * 0.1f + g) * c[19];
+}
+
+// ReducedWongWangExcInh / DecoBalancedExcInh, dfun.rs:876-950
+template <int M> __device__ __forceinline__ void rwei_dfun(const float* c, const float* x, const float* cpl,
+                                                           float* dx) {
+    const float se = x[0], si = x[1], cc = cpl[0] * c[19], jse = se * c[6];
+    const float xe = (((jse * c[5]) - si * c[13]) + c[20]) + cc;
+    const float xi = ((jse - si) + c[22]) + cc * c[18];
+    float ae, ai;
+    if constexpr (M == HB_DECO_BALANCED) { ae = xe * c[24] + c[25]; ai = xi * c[26] + c[27]; }
+    else { ae = xe * c[0] + (-c[1]); ai = xi * c[8] + (-c[9]); }
+    const float he = ae / ((-expf(ae * (-c[2]))) + 1.0f);
+    const float hi = ai / ((-expf(ai * (-c[10]))) + 1.0f);
+    dx[0] = (-se) * c[21] + (((-se) + 1.0f) * he) * c[3];
+    dx[1] = (-si) * c[23] + hi * c[11];
+}
+template <> __device__ __forceinline__ void dfun<HB_RWW_EXC_INH>(const float* c, bool, float, const float* x,
+                                                                const float* cpl, float* dx) {
+    rwei_dfun<HB_RWW_EXC_INH>(c, x, cpl, dx);
+}
+template <> __device__ __forceinline__ void dfun<HB_DECO_BALANCED>(const float* c, bool, float, const float* x,
+                                                                  const float* cpl, float* dx) {
+    rwei_dfun<HB_DECO_BALANCED>(c, x, cpl, dx);
+}
+
+// ZetterbergJansen, dfun.rs:1212-1268; sigmoid 2 e0 / (exp((rho_2 - v) rho_1) + 1)
+template <> __device__ __forceinline__ void dfun<HB_ZETTERBERG_JANSEN>(const float* c, bool, float, const float* x,
+                                                                      const float* cpl, float* dx) {
+    const float s0 = c[18] / (expf(((-x[0]) + c[5]) * c[6]) + 1.0f);
+    const float s23 = c[18] / (expf(((-(x[2] - x[4])) + c[5]) * c[6]) + 1.0f);
+    dx[0] = x[1];
+    dx[1] = ((s23 * c[23]) - x[1] * c[28]) - x[0] * c[21];
+    dx[2] = x[3];
+    dx[3] = ((((s0 * c[24]) + c[15]) + cpl[0]) * c[19] - x[3] * c[28]) - x[2] * c[21];
+    dx[4] = x[5];
+    dx[5] = (((s0 * c[25]) + c[17]) * c[20] - x[5] * c[29]) - x[4] * c[22];
+    dx[6] = x[7];
+    dx[7] = ((s23 * c[26]) - x[7] * c[28]) - x[6] * c[21];
+    dx[8] = x[9];
+    dx[9] = ((s0 * c[27]) - x[9] * c[29]) - x[8] * c[22];
+    dx[10] = x[3] - x[5];
+    dx[11] = x[7] - x[9];
+}
+
+// ReducedSetFitzHughNagumo, dfun.rs:1270-1300
+template <> __device__ __forceinline__ void dfun<HB_REDUCED_FHN>(const float* c, bool, float, const float* x,
+                                                                const float* cpl, float* dx) {
+    const float xi = x[0], eta = x[1], al = x[2], be = x[3];
+    const float xi3 = (xi * xi) * xi, al3 = (al * al) * al;
+    dx[0] = ((((xi - xi3 / 3.0f) - eta) * c[0] + (-xi) * c[3]) - (al - xi) * c[4]) + cpl[0] * c[0];
+    dx[1] = ((xi - eta * c[2]) + c[1]) / c[0];
+    dx[2] = (((al - al3 / 3.0f) - be) * c[0] + (-xi) * c[5]) + cpl[1] * c[0];
+    dx[3] = ((al - be * c[2]) + c[1]) / c[0];
+}
+
+// ReducedSetHindmarshRose, dfun.rs:1302-1348
+template <> __device__ __forceinline__ void dfun<HB_REDUCED_HR>(const float* c, bool, float, const float* x,
+                                                               const float* cpl, float* dx) {
+    const float xi = x[0], eta = x[1], ts = x[2], al = x[3], be = x[4], ga = x[5];
+    const float xi2 = xi * xi, xi3 = (xi * xi) * xi, al2 = al * al, al3 = (al * al) * al;
+    dx[0] = (((((eta - xi3 * c[1]) + xi2 * c[2]) - ts) + (-xi) * c[7]) - (al - xi) * c[8]) + cpl[0];
+    dx[1] = ((-xi2) * c[4] - eta) + c[3];
+    dx[2] = (xi * c[12] - ts * c[0]) + c[13];
+    dx[3] = ((((be - al3 * c[1]) + al2 * c[2]) - ga) + (-xi) * c[9]) + cpl[1];
+    dx[4] = ((-al2) * c[4] - be) + c[3];
+    dx[5] = (al * c[12] - ga * c[0]) + c[13];
+}
+
+// DumontGutkin, dfun.rs:1350-1400 (v times the bracket, as the reference writes it)
+template <> __device__ __forceinline__ void dfun<HB_DUMONT_GUTKIN>(const float* c, bool, float, const float* x,
+                                                                  const float* cpl, float* dx) {
+    const float re = x[0], ve = x[1], see = x[2], sei = x[3], ri = x[4], vi = x[5], sie = x[6], sii = x[7];
+    dx[0] = (((ve * 2.0f) * re) + c[20]) * c[17];
+    dx[1] = (ve * (((((((ve - (re * re) * c[21]) + c[2]) + see * c[14]) - sei * c[14]) + c[0]) + cpl[0]) +
+                   cpl[1] * 0.0f)) * c[17];
+    dx[2] = (((-see) + re * c[9]) + cpl[2]) * c[19];
+    dx[3] = ((-sei) + ri * c[10]) * c[19];
+    dx[4] = (((vi * 2.0f) * ri) + c[22]) * c[18];
+    dx[5] = (vi * ((((((vi - (ri * ri) * c[23]) + c[6]) + sie * c[15]) - sii * c[15]) + c[4]) + cpl[2] * c[13])) * c[18];
+    dx[6] = (((-sie) + re * c[11]) + cpl[2] * c[13]) * c[19];
+    dx[7] = ((-sii) + ri * c[12]) * c[19];
+}
+
 // clamp_batch, dfun.rs:170-239. Ternaries keep NaN like f32::clamp.
 template <int M> __device__ __forceinline__ void clampm(float* x) {
     if constexpr (M == HB_WILSON_COWAN) {
@@ -424,8 +604,15 @@ template <int M> __device__ __forceinline__ void clampm(float* x) {
         x[0] = x[0] < 0.0f ? 0.0f : x[0];
     } else if constexpr (M == HB_RWW) {
         x[0] = x[0] < 0.0f ? 0.0f : (x[0] > 1.0f ? 1.0f : x[0]);
-    } else if constexpr (M == HB_COOMBES_BYRNE_2D || M == HB_KIONEX) {   // dfun.rs:190-194, 232-236: r, x >= 0
+    } else if constexpr (M == HB_COOMBES_BYRNE_2D || M == HB_KIONEX || M == HB_COOMBES_BYRNE || M == HB_GAST_SD ||
+                         M == HB_GAST_SF) {   // dfun.rs:190-209, 232-236: r, x >= 0
         x[0] = x[0] < 0.0f ? 0.0f : x[0];
+    } else if constexpr (M == HB_RWW_EXC_INH || M == HB_DECO_BALANCED) {   // dfun.rs:172-176: both in [0, 1]
+        x[0] = x[0] < 0.0f ? 0.0f : (x[0] > 1.0f ? 1.0f : x[0]);
+        x[1] = x[1] < 0.0f ? 0.0f : (x[1] > 1.0f ? 1.0f : x[1]);
+    } else if constexpr (M == HB_DUMONT_GUTKIN) {   // dfun.rs:210-218: r_e, r_i >= 0
+        x[0] = x[0] < 0.0f ? 0.0f : x[0];
+        x[4] = x[4] < 0.0f ? 0.0f : x[4];
     } else if constexpr (M == HB_ZERLAUT_FIRST || M == HB_ZERLAUT_SECOND) {   // dfun.rs:219-230: E, I in [0, 1]
         x[0] = x[0] < 0.0f ? 0.0f : (x[0] > 1.0f ? 1.0f : x[0]);
         x[1] = x[1] < 0.0f ? 0.0f : (x[1] > 1.0f ? 1.0f : x[1]);

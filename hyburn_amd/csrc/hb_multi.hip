@@ -30,7 +30,7 @@ namespace multi {
 
 #include "hb_model.h"
 
-constexpr int kMaxSub = 4, kMaxProj = 8, kNV = 8, kMaxNodes = 1024;
+constexpr int kMaxSub = 4, kMaxProj = 8, kNV = 12, kMaxNodes = 1024;
 constexpr int kMaxLdsFloats = 16384;   // 64 KiB of pre values
 
 struct Sub {
@@ -104,7 +104,10 @@ struct Args {
 };
 
 __device__ __forceinline__ void pre_rt(const Proj& P, const float* x, float* out) {
-    if (P.nc == 3) {   // EpileptorRestingState's three cvars (no Kuramoto: the host refuses it)
+    if (P.nc == 4) {   // four cvars (CoombesByrne, the Gast models, DumontGutkin; no Kuramoto)
+        if (P.np == 1) pre<4, 1>(P.cfn, P.cc, false, P.classic, x, out);
+        else pre<4, 4>(P.cfn, P.cc, false, P.classic, x, out);
+    } else if (P.nc == 3) {   // EpileptorRestingState's three cvars (no Kuramoto: the host refuses it)
         if (P.np == 1) pre<3, 1>(P.cfn, P.cc, false, P.classic, x, out);
         else pre<3, 3>(P.cfn, P.cc, false, P.classic, x, out);
     } else if (P.nc == 1) {
@@ -118,7 +121,10 @@ __device__ __forceinline__ void pre_rt(const Proj& P, const float* x, float* out
 }
 
 __device__ __forceinline__ void post_rt(const Proj& P, const float* g, const float* own, float* out) {
-    if (P.nc == 3) {
+    if (P.nc == 4) {
+        if (P.np == 1) post<4, 1>(P.cfn, P.cc, P.classic, g, own, out);
+        else post<4, 4>(P.cfn, P.cc, P.classic, g, own, out);
+    } else if (P.nc == 3) {
         if (P.np == 1) post<3, 1>(P.cfn, P.cc, P.classic, g, own, out);
         else post<3, 3>(P.cfn, P.cc, P.classic, g, own, out);
     } else if (P.nc == 1) {
@@ -132,12 +138,30 @@ __device__ __forceinline__ void post_rt(const Proj& P, const float* g, const flo
 }
 
 // mc: the register copy of the first kMC constants; mcg: the full table in
-// global memory, which models 10-15 read directly. EXT: the kernel holds
-// models 10-15 (their code stays out of the kernel of the other ten).
-template <bool EXT>
+// global memory, which models 10-27 read directly. EXT 1 holds models 10-27
+// but ZetterbergJansen, EXT 2 all of them with twelve state registers; the
+// ten models of EXT 0 keep a kernel free of the others' code.
+template <int EXT>
 __device__ __forceinline__ void step_model(int model, const StepCtx& S, const float* mc, const float* mcg,
                                            const float* cpl, const float* z, float* x) {
-    if constexpr (EXT) {
+    if constexpr (EXT == 2) {
+        if (model == HB_ZETTERBERG_JANSEN) { integrate<HB_ZETTERBERG_JANSEN>(S, mcg, cpl, z, x); return; }
+    }
+    if constexpr (EXT >= 1) {
+        switch (model) {
+        case HB_COOMBES_BYRNE: integrate<HB_COOMBES_BYRNE>(S, mcg, cpl, z, x); return;
+        case HB_GAST_SD: integrate<HB_GAST_SD>(S, mcg, cpl, z, x); return;
+        case HB_GAST_SF: integrate<HB_GAST_SF>(S, mcg, cpl, z, x); return;
+        case HB_LARTER_BREAKSPEAR: integrate<HB_LARTER_BREAKSPEAR>(S, mcg, cpl, z, x); return;
+        case HB_EPILEPTOR_2D: integrate<HB_EPILEPTOR_2D>(S, mcg, cpl, z, x); return;
+        case HB_EPILEPTOR: integrate<HB_EPILEPTOR>(S, mcg, cpl, z, x); return;
+        case HB_RWW_EXC_INH: integrate<HB_RWW_EXC_INH>(S, mcg, cpl, z, x); return;
+        case HB_DECO_BALANCED: integrate<HB_DECO_BALANCED>(S, mcg, cpl, z, x); return;
+        case HB_REDUCED_FHN: integrate<HB_REDUCED_FHN>(S, mcg, cpl, z, x); return;
+        case HB_REDUCED_HR: integrate<HB_REDUCED_HR>(S, mcg, cpl, z, x); return;
+        case HB_DUMONT_GUTKIN: integrate<HB_DUMONT_GUTKIN>(S, mcg, cpl, z, x); return;
+        default: break;
+        }
         switch (model) {
         case HB_EPILEPTOR_CODIM3: integrate<HB_EPILEPTOR_CODIM3>(S, mcg, cpl, z, x); return;
         case HB_EPILEPTOR_CODIM3_SLOWMOD: integrate<HB_EPILEPTOR_CODIM3_SLOWMOD>(S, mcg, cpl, z, x); return;
@@ -201,10 +225,11 @@ __device__ __attribute__((noinline)) void bold_advance(const Args* A, BoldClock&
 }
 
 // NVR: state variables held in registers (6 for the ten models of the base
-// kernel, 8 with models 10-15); memory layouts keep the kNV stride.
-template <bool EXT>
+// kernel, 8 with models 10-27 but ZetterbergJansen, 12 with it); memory
+// layouts keep the kNV stride.
+template <int EXT>
 __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
-    constexpr int NVR = EXT ? kNV : 6;
+    constexpr int NVR = EXT == 2 ? 12 : (EXT == 1 ? 8 : 6);
     extern __shared__ float pv[];   // pre values of every projection's source nodes
     __shared__ float gmean[kMaxProj * 4];   // w * mean over source nodes, per Scalar projection and channel
     const int t = threadIdx.x, b = blockIdx.x;
@@ -297,7 +322,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
         }
         __syncthreads();
         if (live) {
-            float cpl[3] = {0.0f, 0.0f, 0.0f};
+            float cpl[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             for (int p = 0; p < A->n_proj; ++p) {
                 const Proj& P = A->proj[p];
                 if (s != P.tgt) continue;
@@ -346,7 +371,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                         own[c] = sinf(x[c]);
                         own[P.nc + c] = cosf(x[c]);
                     }
-                float out[3] = {0.0f, 0.0f, 0.0f};
+                float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
                 post_rt(P, g, own, out);
                 if (P.pass_all) {
                     for (int c = 0; c < P.nc; ++c) cpl[c] += out[c];
@@ -417,7 +442,8 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
 // Position of state variable t among the model's coupling variables (the
 // CVAR constants of src/model/*.rs), 0 when absent (engine.rs:391-393).
 int cvar_position(int model, int t) {
-    static const int g2do[] = {0}, mpr[] = {0, 1}, one0[] = {0}, jr[] = {1}, ers[] = {0, 3, 6};
+    static const int g2do[] = {0}, mpr[] = {0, 1}, one0[] = {0}, jr[] = {1}, ers[] = {0, 3, 6},
+                     four[] = {0, 1, 2, 3}, x03[] = {0, 3}, x02[] = {0, 2}, zj[] = {3}, dg[] = {0, 1, 4, 5};
     const int* cv = one0;
     int n = 1;
     if (model == HB_G2DO) cv = g2do;
@@ -430,6 +456,11 @@ int cvar_position(int model, int t) {
         n = 2;
     }
     else if (model == HB_JANSEN_RIT) cv = jr;
+    else if (model == HB_COOMBES_BYRNE || model == HB_GAST_SD || model == HB_GAST_SF) { cv = four; n = 4; }
+    else if (model == HB_EPILEPTOR || model == HB_REDUCED_HR) { cv = x03; n = 2; }
+    else if (model == HB_REDUCED_FHN) { cv = x02; n = 2; }
+    else if (model == HB_ZETTERBERG_JANSEN) cv = zj;
+    else if (model == HB_DUMONT_GUTKIN) { cv = dg; n = 4; }
     for (int k = 0; k < n; ++k)
         if (cv[k] == t) return k;
     return 0;
@@ -479,7 +510,7 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     int n_tot = 0;
     for (int q = 0; q < Q.n_subnets; ++q) {
         const hb_subnetwork& U = Q.subnets[q];
-        if (U.model < HB_G2DO || U.model > HB_KIONEX) return bad("unknown model in subnetwork " + std::to_string(q));
+        if (U.model < HB_G2DO || U.model > HB_DUMONT_GUTKIN) return bad("unknown model in subnetwork " + std::to_string(q));
         if (U.n_nodes < 1 || !U.params || !U.x0) return bad("subnetwork " + std::to_string(q) + ": nodes, params, x0");
         if (model_ext(U.model) && U.n_params != model_nparams(U.model))
             return bad("subnetwork " + std::to_string(q) + ": expected " + std::to_string(model_nparams(U.model)) +
@@ -556,12 +587,12 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
         P.np = coupling_npre(R.cfn, ncs);
         if ((R.cfn == HB_CF_KURAMOTO || R.cfn == HB_CF_DIFFERENCE) && ncs != nct)
             return bad(tag + ": Kuramoto and Difference coupling need equal source and target ncvar");
-        // three coupling variables (EpileptorRestingState): the history keeps two
-        // per node, and Kuramoto's six pre channels exceed the kernel's four
+        // three or four coupling variables: the history keeps two per node, and
+        // Kuramoto's six or eight pre channels exceed the kernel's four
         if (ncs > 2 && (R.delay > 0 || R.edge_delays))
-            return bad(tag + ": delayed projections from a three-cvar subnetwork are not supported");
+            return bad(tag + ": delayed projections from a subnetwork of more than two cvars are not supported");
         if (ncs > 2 && R.cfn == HB_CF_KURAMOTO)
-            return bad(tag + ": Kuramoto coupling from a three-cvar subnetwork is not supported");
+            return bad(tag + ": Kuramoto coupling from a subnetwork of more than two cvars is not supported");
         float cp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int k = 0; k < R.n_cparams; ++k) cp[k] = R.cparams[k];
         int classic = 0;
@@ -805,13 +836,19 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     (void)hipEventCreate(&ev0);
     (void)hipEventCreate(&ev1);
     (void)hipEventRecord(ev0, nullptr);
-    bool ext = false;
-    for (int q = 0; q < Q.n_subnets; ++q) ext = ext || model_ext(Q.subnets[q].model);
-    if (ext)
-        hipLaunchKernelGGL(k_multi<true>, dim3((unsigned)Q.n_sweep), dim3((unsigned)threads),
+    int ext = 0;   // kernel variant: 0 the ten base models, 1 with models 10-27, 2 with twelve state variables
+    for (int q = 0; q < Q.n_subnets; ++q) {
+        const int m = Q.subnets[q].model;
+        if (model_ext(m)) ext = std::max(ext, model_nvar(m) > 8 ? 2 : 1);
+    }
+    if (ext == 2)
+        hipLaunchKernelGGL(k_multi<2>, dim3((unsigned)Q.n_sweep), dim3((unsigned)threads),
+                           (size_t)lds * sizeof(float), nullptr, dA);
+    else if (ext == 1)
+        hipLaunchKernelGGL(k_multi<1>, dim3((unsigned)Q.n_sweep), dim3((unsigned)threads),
                            (size_t)lds * sizeof(float), nullptr, dA);
     else
-        hipLaunchKernelGGL(k_multi<false>, dim3((unsigned)Q.n_sweep), dim3((unsigned)threads),
+        hipLaunchKernelGGL(k_multi<0>, dim3((unsigned)Q.n_sweep), dim3((unsigned)threads),
                            (size_t)lds * sizeof(float), nullptr, dA);
     e = hipGetLastError();
     (void)hipEventRecord(ev1, nullptr);

@@ -89,8 +89,39 @@ ZERLAUT_DEFAULT = np.array([
     np.float32)                                        # zerlaut_first.rs:108-128 (zerlaut_second.rs: the same)
 KIONEX_DEFAULT = np.array([0.0, 5.5, 0.1, 0.0, 1.0, -40.0, 0.5, -20.0, -0.5, -31.0, 1.0, 4.0, 0.04, 0.001],
                           np.float32)                                              # kionex.rs:58-64
+COOMBES_BYRNE_DEFAULT = np.array([0.5, 0.95, -10.0, 1.0, 20.0], np.float32)               # coombes_byrne.rs:46-48
+GAST_SD_DEFAULT = np.array([1.0, 10.0, 0.5, 0.0, 2.0, 21.2132, -6.0, 1.0, 0.0], np.float32)  # gast_schmidt_knosche_sd.rs:52-54
+GAST_SF_DEFAULT = np.array([1.0, 10.0, 10.0, 0.0, 2.0, 21.2132, 1.0, 1.0, 0.0], np.float32)  # gast_schmidt_knosche_sf.rs:52-54
+LARTER_BREAKSPEAR_DEFAULT = np.array([1.1, 2.0, 0.5, 6.7, 0.7, 1.0, -0.7, -0.5, 0.53, -0.01, 0.3, 0.0,
+                                      0.15, 0.15, 0.3, 0.65, 0.7, 2.0, 2.0, 0.4, 1.0, 0.4, 0.1, 0.1, 0.3,
+                                      0.25, 0.0, 0.0, 1.0, 1.0, 1.0, 1.0], np.float32)  # larter_breakspear.rs:74-84
+EPILEPTOR_2D_DEFAULT = np.array([3.1, -1.6, 1.0, 3.0, 0.0, 1.0, 5.0, 0.00035, 0.0, 0.0, 1.0, 0.0],
+                                np.float32)                                       # epileptor2d.rs:47-49
+EPILEPTOR_DEFAULT = np.array([3.1, 0.45, -1.6, 1.0, 3.0, 1.0, 5.0, 0.00035, 0.0, 10.0, 6.0, 2.0, 0.0, 0.0,
+                              0.0, 1.0, 0.0], np.float32)                         # epileptor.rs:57-62
+RWW_EXC_INH_DEFAULT = np.array([310.0, 125.0, 0.160, 0.000641, 100.0, 1.4, 0.15, 1.0, 615.0, 177.0, 0.087,
+                                0.001, 10.0, 1.0, 0.7, 0.382, 0.0, 2.0, 0.0], np.float32)  # rww_exc_inh.rs:61-68
+DECO_BALANCED_DEFAULT = np.append(RWW_EXC_INH_DEFAULT, np.float32(1.0))          # deco_balanced_exc_inh.rs:62-69
+ZETTERBERG_JANSEN_DEFAULT = np.array([3.25, 22.0, 0.1, 0.05, 0.0025, 6.0, 0.56, 135.0, 108.0, 33.75, 33.75,
+                                      15.0, 1.0, 1.0, 1.0, 0.12, 0.12, 0.12], np.float32)  # zetterberg_jansen.rs:67-75
+REDUCED_FHN_DEFAULT = np.array([3.0, 0.45, 0.9, 0.5, 0.15, 0.15, 0.35, 0.0], np.float32)  # reduced_fhn.rs:45-47
+REDUCED_HR_DEFAULT = np.array([0.006, 1.0, 3.0, 1.0, 5.0, 4.0, -1.6, 0.5, 0.1, 0.15, 0.3, 3.3],
+                              np.float32)                                         # reduced_hr.rs:52-54
+DUMONT_GUTKIN_DEFAULT = np.array([0.0, 1.0, -5.0, 10.0, 0.0, 1.0, -5.0, 10.0, 1.0, 0.0, 10.0, 0.0, 15.0, 5.0],
+                                 np.float32)                                      # dumont_gutkin.rs:67-73
 DEFAULTS = {L.G2DO: G2DO_DEFAULT, L.MPR: MPR_DEFAULT, L.JANSEN_RIT: JR_DEFAULT,
             L.WILSON_COWAN: WC_DEFAULT, L.KURAMOTO: KUR_DEFAULT, L.RWW: RWW_DEFAULT}
+# defaults of the models that run on the multi-subnetwork path only
+MULTI_DEFAULTS = {L.LINEAR: LINEAR_DEFAULT, L.SUP_HOPF: SUP_HOPF_DEFAULT, L.HOPFIELD: HOPFIELD_DEFAULT,
+                  L.COOMBES_BYRNE_2D: CB2D_DEFAULT, L.EPILEPTOR_CODIM3: EPI_CODIM3_DEFAULT,
+                  L.EPILEPTOR_CODIM3_SLOWMOD: EPI_CODIM3_SLOWMOD_DEFAULT, L.EPILEPTOR_RS: EPI_RS_DEFAULT,
+                  L.ZERLAUT_FIRST: ZERLAUT_DEFAULT, L.ZERLAUT_SECOND: ZERLAUT_DEFAULT, L.KIONEX: KIONEX_DEFAULT,
+                  L.COOMBES_BYRNE: COOMBES_BYRNE_DEFAULT, L.GAST_SD: GAST_SD_DEFAULT, L.GAST_SF: GAST_SF_DEFAULT,
+                  L.LARTER_BREAKSPEAR: LARTER_BREAKSPEAR_DEFAULT, L.EPILEPTOR_2D: EPILEPTOR_2D_DEFAULT,
+                  L.EPILEPTOR: EPILEPTOR_DEFAULT, L.RWW_EXC_INH: RWW_EXC_INH_DEFAULT,
+                  L.DECO_BALANCED: DECO_BALANCED_DEFAULT, L.ZETTERBERG_JANSEN: ZETTERBERG_JANSEN_DEFAULT,
+                  L.REDUCED_FHN: REDUCED_FHN_DEFAULT, L.REDUCED_HR: REDUCED_HR_DEFAULT,
+                  L.DUMONT_GUTKIN: DUMONT_GUTKIN_DEFAULT}
 
 
 def dense_to_csr(W: np.ndarray):

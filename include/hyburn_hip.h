@@ -34,7 +34,12 @@ enum { HB_G2DO = 0, HB_MPR = 1, HB_JANSEN_RIT = 2, HB_WILSON_COWAN = 3, HB_KURAM
        /* the models the reference pins with trace fixtures (tests/model_reference_test.rs:90-156),
         * multi-subnetwork path only: dfun.rs:952-1870 */
        HB_EPILEPTOR_CODIM3 = 10, HB_EPILEPTOR_CODIM3_SLOWMOD = 11, HB_EPILEPTOR_RS = 12,
-       HB_ZERLAUT_FIRST = 13, HB_ZERLAUT_SECOND = 14, HB_KIONEX = 15 };
+       HB_ZERLAUT_FIRST = 13, HB_ZERLAUT_SECOND = 14, HB_KIONEX = 15,
+       /* the remaining models of the registry (config.rs:56-75), multi-subnetwork path only:
+        * dfun.rs:609-950, 1212-1400 */
+       HB_COOMBES_BYRNE = 16, HB_GAST_SD = 17, HB_GAST_SF = 18, HB_LARTER_BREAKSPEAR = 19,
+       HB_EPILEPTOR_2D = 20, HB_EPILEPTOR = 21, HB_RWW_EXC_INH = 22, HB_DECO_BALANCED = 23,
+       HB_ZETTERBERG_JANSEN = 24, HB_REDUCED_FHN = 25, HB_REDUCED_HR = 26, HB_DUMONT_GUTKIN = 27 };
 /* Largest parameter vector of any model (the Zerlaut models: 50). */
 #define HB_MAX_PARAMS 64
 /* IntegratorKind (integrator.rs:17-30). */

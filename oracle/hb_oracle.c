@@ -6,7 +6,7 @@
  * checker (or as the timed CPU baseline). The product path never calls it.
  *
  * It restates, in plain C, the reference's NdArray<f32> arithmetic:
- *   src/engine/batch_engine/dfun.rs:170-239,271-516   (six dfuns + clamps)
+ *   src/engine/batch_engine/dfun.rs:170-239,271-1870  (the 28 dfuns + clamps)
  *   src/engine/batch_engine/engine.rs:540-823         (time loop, integrators, tavg)
  *   src/engine/integrator.rs:89-315                   (Heun/RK4/stochastic forms)
  *   src/engine/coupling.rs:114-444                    (coupling pre/post)
@@ -28,8 +28,10 @@
 #endif
 
 enum { ORC_G2DO = 0, ORC_MPR = 1, ORC_JR = 2, ORC_WC = 3, ORC_KUR = 4, ORC_RWW = 5, ORC_LIN = 6, ORC_SH = 7, ORC_HOP = 8, ORC_CB2 = 9,
-       ORC_EC3 = 10, ORC_EC3S = 11, ORC_ERS = 12, ORC_ZL1 = 13, ORC_ZL2 = 14, ORC_KIX = 15 };
-enum { ORC_NMODELS = 16, ORC_MAXP = 64 };
+       ORC_EC3 = 10, ORC_EC3S = 11, ORC_ERS = 12, ORC_ZL1 = 13, ORC_ZL2 = 14, ORC_KIX = 15,
+       ORC_CB = 16, ORC_GSD = 17, ORC_GSF = 18, ORC_LB = 19, ORC_E2D = 20, ORC_EPI = 21, ORC_RWEI = 22,
+       ORC_DECO = 23, ORC_ZJ = 24, ORC_RFHN = 25, ORC_RHR = 26, ORC_DG = 27 };
+enum { ORC_NMODELS = 28, ORC_MAXP = 64 };
 enum { ORC_EULER = 0, ORC_HEUN = 1, ORC_RK4 = 2, ORC_EULER_STOCH = 3, ORC_HEUN_STOCH = 4, ORC_RK4_STOCH = 5 };
 enum { ORC_CF_LINEAR = 0, ORC_CF_DIFFERENCE = 1, ORC_CF_KURAMOTO = 2, ORC_CF_SIGMOIDAL = 3,
        ORC_CF_SCALED_LINEAR = 4, ORC_CF_TANH = 5, ORC_CF_SJR = 6, ORC_CF_PRESIGMOIDAL = 7 };
@@ -78,10 +80,21 @@ typedef struct {
 } hb_orc_problem;
 
 /* model registry, config.rs:43-50 */
-static int hb_orc_nvar(int m) { static const int v[ORC_NMODELS] = {2, 2, 6, 2, 1, 1, 1, 2, 2, 2, 3, 5, 8, 5, 8, 5}; return v[m]; }
-static int hb_orc_ncvar(int m) { static const int v[ORC_NMODELS] = {1, 2, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1, 3, 1, 1, 1}; return v[m]; }
-/* registry rows of config.rs:43-72 (EpileptorCodim3 ... KIonEx: 10..15) */
-static int hb_orc_nparams(int m) { static const int v[ORC_NMODELS] = {12, 7, 13, 22, 1, 8, 1, 2, 3, 4, 13, 27, 28, 50, 50, 14}; return v[m]; }
+static int hb_orc_nvar(int m) {
+    static const int v[ORC_NMODELS] = {2, 2, 6, 2, 1, 1, 1, 2, 2, 2, 3, 5, 8, 5, 8, 5, 4, 4, 4, 3, 2, 6, 2, 2, 12, 4, 6, 8};
+    return v[m];
+}
+static int hb_orc_ncvar(int m) {
+    static const int v[ORC_NMODELS] = {1, 2, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1, 3, 1, 1, 1, 4, 4, 4, 1, 1, 2, 1, 1, 1, 2, 2, 4};
+    return v[m];
+}
+/* registry rows of config.rs:43-72 (EpileptorCodim3 ... KIonEx: 10..15,
+ * CoombesByrne ... DumontGutkin: 16..27) */
+static int hb_orc_nparams(int m) {
+    static const int v[ORC_NMODELS] = {12, 7, 13, 22, 1, 8, 1, 2, 3, 4, 13, 27, 28, 50, 50, 14,
+                                       5, 9, 9, 32, 12, 17, 19, 20, 18, 8, 12, 14};
+    return v[m];
+}
 
 int hb_orc_model_info(int m, int32_t* nvar, int32_t* ncvar, int32_t* nparams)
 {

@@ -359,7 +359,9 @@ def fcd(traj, window_len, stride):
 
 
 _CVAR = {0: (0,), 1: (0, 1), 2: (1,), 3: (0,), 4: (0,), 5: (0,), 6: (0,), 7: (0, 1), 8: (0, 1), 9: (0, 1),
-         10: (0,), 11: (0,), 12: (0, 3, 6), 13: (0,), 14: (0,), 15: (0,)}   # CVAR of src/model/*.rs
+         10: (0,), 11: (0,), 12: (0, 3, 6), 13: (0,), 14: (0,), 15: (0,), 16: (0, 1, 2, 3), 17: (0, 1, 2, 3),
+         18: (0, 1, 2, 3), 19: (0,), 20: (0,), 21: (0, 3), 22: (0,), 23: (0,), 24: (3,), 25: (0, 2), 26: (0, 3),
+         27: (0, 1, 4, 5)}   # CVAR of src/model/*.rs
 _IDENTITY_PRE = (0, 1, 3, 4)   # Linear, Difference, Sigmoidal, ScaledLinear: pre is the identity
 
 

@@ -107,6 +107,81 @@ static void FN(ext_consts)(REAL* c, int model, const REAL* p)
         c[18] = -(p[12] / (REAL)2160);
         c[19] = (REAL)26.64f * (REAL)LOGR((REAL)112 / (REAL)5);
         break; }
+    case ORC_CB: { /* dfun.rs:609-637: delta/pi, pi*pi, k*pi */
+        REAL pi = PIR;
+        c[5] = p[0] / pi; c[6] = pi * pi; c[7] = p[3] * pi;
+        break; }
+    case ORC_GSD: case ORC_GSF: { /* dfun.rs:639-716: tau, tau_a (eps), 1/tau, delta/(pi tau), pi^2 tau^2, eta+I, J tau, 1/tau_a */
+        REAL pi = PIR;
+        c[9] = p[0] > eps ? p[0] : eps;
+        c[10] = p[1] > eps ? p[1] : eps;
+        c[11] = (REAL)1 / c[9];
+        c[12] = p[4] / (pi * c[9]);
+        c[13] = ((pi * pi) * c[9]) * c[9];
+        c[14] = p[6] + p[3];
+        c[15] = p[5] * c[9];
+        c[16] = (REAL)1 / c[10];
+        break; }
+    case ORC_LB: /* dfun.rs:718-766 */
+        c[31] = p[31] > eps ? p[31] : eps;
+        c[32] = (REAL)1 / p[12]; c[33] = (REAL)1 / p[13]; c[34] = (REAL)1 / p[14];
+        c[35] = (REAL)0.5f * p[28]; c[36] = (REAL)1 / p[15];
+        c[37] = (REAL)0.5f * p[29]; c[38] = (REAL)1 / p[16];
+        c[39] = (p[23] * p[25]) * p[19];
+        c[40] = (((REAL)1 - p[23]) * p[25]) * p[19];
+        c[41] = ((REAL)1 - p[23]) * p[19];
+        c[42] = p[23] * p[19];
+        c[43] = p[20] * p[24];
+        c[44] = ((p[30] * p[4]) * p[22]) / c[31];
+        c[45] = p[21] * p[24];
+        c[46] = p[30] * p[22];
+        break;
+    case ORC_E2D: /* dfun.rs:776-818 */
+        c[12] = p[6] - p[3];
+        c[13] = p[5] + p[0];
+        c[14] = (REAL)-4 * p[1];
+        c[15] = p[10] * p[7];
+        break;
+    case ORC_EPI: /* dfun.rs:820-874 */
+        c[17] = (REAL)-4 * p[2];
+        c[18] = p[15] * p[7];
+        c[19] = (REAL)-0.01f * p[15];
+        break;
+    case ORC_RWEI: case ORC_DECO: { /* dfun.rs:884-950 */
+        REAL te = p[4] > eps ? p[4] : eps, ti = p[12] > eps ? p[12] : eps;
+        c[19] = p[17] * p[6];
+        c[20] = p[7] * p[15] + p[16];
+        c[21] = (REAL)1 / te;
+        c[22] = p[14] * p[15];
+        c[23] = (REAL)1 / ti;
+        if (model == ORC_DECO) {
+            c[24] = p[0] * p[19]; c[25] = (-p[1]) * p[19];
+            c[26] = p[8] * p[19]; c[27] = (-p[9]) * p[19];
+        }
+        break; }
+    case ORC_ZJ: /* dfun.rs:1212-1268 */
+        c[18] = (REAL)2 * p[4];
+        c[19] = p[0] * p[2]; c[20] = p[1] * p[3];
+        c[21] = p[2] * p[2]; c[22] = p[3] * p[3];
+        c[23] = c[19] * p[7]; c[24] = p[13] * p[8]; c[25] = p[14] * p[9];
+        c[26] = c[19] * p[10]; c[27] = c[20] * p[11];
+        c[28] = (REAL)2 * p[2]; c[29] = (REAL)2 * p[3];
+        break;
+    case ORC_RHR: /* dfun.rs:1302-1348 */
+        c[12] = p[0] * p[5];
+        c[13] = (-p[0]) * p[6];
+        break;
+    case ORC_DG: { /* dfun.rs:1350-1400 */
+        REAL pi = PIR;
+        c[14] = p[3] > eps ? p[3] : eps;
+        c[15] = p[7] > eps ? p[7] : eps;
+        c[16] = p[8] > eps ? p[8] : eps;
+        c[17] = (REAL)1 / c[14]; c[18] = (REAL)1 / c[15]; c[19] = (REAL)1 / c[16];
+        c[20] = p[1] / (pi * c[14]);
+        c[21] = ((pi * pi) * c[14]) * c[14];
+        c[22] = p[5] / (pi * c[15]);
+        c[23] = ((pi * pi) * c[15]) * c[15];
+        break; }
     }
 }
 
@@ -485,6 +560,124 @@ static void FN(dfun_ext)(const FN(mconst)* k, const REAL* x, const REAL* cpl, RE
         dx[3] = (i_k - i_pump * (REAL)2) * c[18];
         dx[4] = ((-ko) + c[1]) * c[13];
         break; }
+    case ORC_CB: { /* dfun.rs:609-637 */
+        REAL r = x[0], v = x[1], g = x[2], q = x[3];
+        dx[0] = ((((v * r) * (REAL)2) + c[5]) - g * r) + cpl[0];
+        dx[1] = (((v * v) + c[4]) - r * (r * c[6])) + ((-v) + c[2]) * g;
+        dx[2] = q * c[1];
+        dx[3] = (((r * c[7]) - g) - q * (REAL)2) * c[1];
+        break; }
+    case ORC_GSD: case ORC_GSF: { /* dfun.rs:639-716 (v times the bracket, as the reference writes it) */
+        REAL r = x[0], v = x[1], a = x[2], b = x[3], br;
+        dx[0] = (((r * v) * (REAL)2) + c[12]) * c[11];
+        br = (v - r * (r * c[13])) + c[14];
+        if (k->model == ORC_GSD) br = br + (r * c[15]) * ((-a) + (REAL)1);
+        else br = (br + r * c[15]) - a;
+        dx[1] = (v * ((br + cpl[0] * c[7]) + cpl[1] * c[8])) * c[11];
+        dx[2] = b * c[16];
+        dx[3] = (((b * (REAL)-2) - a) + r * c[2]) * c[16];
+        break; }
+    case ORC_LB: { /* dfun.rs:718-766 */
+        REAL V = x[0], W = x[1], Z = x[2];
+        REAL m_ca = (REAL)0.5f * (TANHR((V + (-c[9])) * c[32]) + (REAL)1);
+        REAL m_na = (REAL)0.5f * (TANHR((V + (-c[10])) * c[33]) + (REAL)1);
+        REAL m_k = (REAL)0.5f * (TANHR((V + (-c[11])) * c[34]) + (REAL)1);
+        REAL qv = c[35] * (TANHR((V + (-c[26])) * c[36]) + (REAL)1);
+        REAL qz = c[37] * (TANHR((Z + (-c[27])) * c[38]) + (REAL)1);
+        REAL tca = (c[0] + qv * c[40]) + cpl[0] * c[39];
+        REAL tna = ((m_na * c[3]) + qv * c[41]) + cpl[0] * c[42];
+        dx[0] = ((((((-tca) * m_ca) * (V + (-c[5])) - (W * c[1]) * (V + (-c[6]))) - (V + (-c[7])) * c[2]) -
+                  tna * (V + (-c[8]))) - (Z * c[18]) * qz) * c[30] + c[43];
+        dx[1] = (m_k - W) * c[44];
+        dx[2] = ((V * c[17]) * qv + c[45]) * c[46];
+        break; }
+    case ORC_E2D: { /* dfun.rs:776-818 */
+        REAL x1 = x[0], z = x[1], f1, zm, yz, h, z4;
+        if (x1 < 0) f1 = (x1 * x1) * c[2] + x1 * c[12];
+        else { z4 = z + (REAL)-4; f1 = ((z4 * z4) * (REAL)-0.6f + (-c[4])) + x1 * c[6]; }
+        dx[0] = ((((-z) + c[13]) + cpl[0] * c[8]) - f1 * x1) * c[10];
+        zm = z * (REAL)-0.1f;
+        yz = z < 0 ? -((((((zm * zm) * zm) * zm) * zm) * zm) * zm) : (REAL)0;
+        if (c[11] > (REAL)0.5f) h = (x1 + (REAL)3) / (EXPR(-((x1 + (REAL)0.5f) * (REAL)10)) + (REAL)1) + yz;
+        else h = (x1 * (REAL)4 + c[14]) + yz;
+        dx[1] = ((h - z) + cpl[0] * c[9]) * c[15];
+        break; }
+    case ORC_EPI: { /* dfun.rs:820-874 */
+        REAL x1 = x[0], y1 = x[1], z = x[2], x2 = x[3], y2 = x[4], g = x[5], f1, z4, zm, yz, h, xp, f2;
+        if (x1 < 0) f1 = (-(x1 * x1)) * c[3] + x1 * c[4];
+        else { z4 = z + (REAL)-4; f1 = ((-x2) + c[8]) + (z4 * z4) * (REAL)0.6f; }
+        dx[0] = ((((y1 - z) + cpl[0] * c[12]) + f1 * x1) + c[0]) * c[15];
+        dx[1] = (((-((x1 * x1) * c[6])) + c[5]) - y1) * c[15];
+        zm = z * (REAL)-0.1f;
+        yz = z < 0 ? -((((((zm * zm) * zm) * zm) * zm) * zm) * zm) : (REAL)0;
+        if (c[16] > (REAL)0.5f) h = (x1 + (REAL)3) / (EXPR(-((x1 + (REAL)0.5f) * (REAL)10)) + (REAL)1) + yz;
+        else h = (x1 * (REAL)4 + c[17]) + yz;
+        dx[2] = ((h - z) + cpl[0] * c[14]) * c[18];
+        dx[3] = ((((((x2 - (x2 * x2) * x2) - y2) + c[1]) + g * c[11]) - (z + (REAL)-3.5f) * (REAL)0.3f) +
+                 cpl[1] * c[13]) * c[15];
+        xp = x2 + (REAL)0.25f;
+        f2 = xp < 0 ? (REAL)0 : xp * c[10];
+        dx[4] = ((f2 - y2) * c[15]) / c[9];
+        dx[5] = ((-x1) * (REAL)0.1f + g) * c[19];
+        break; }
+    case ORC_RWEI: case ORC_DECO: { /* dfun.rs:876-950 */
+        REAL se = x[0], si = x[1], cc = cpl[0] * c[19], jse = se * c[6], xe, xi, ae, ai, he, hi;
+        xe = (((jse * c[5]) - si * c[13]) + c[20]) + cc;
+        xi = ((jse - si) + c[22]) + cc * c[18];
+        if (k->model == ORC_DECO) { ae = xe * c[24] + c[25]; ai = xi * c[26] + c[27]; }
+        else { ae = xe * c[0] + (-c[1]); ai = xi * c[8] + (-c[9]); }
+        he = ae / ((-EXPR(ae * (-c[2]))) + (REAL)1);
+        hi = ai / ((-EXPR(ai * (-c[10]))) + (REAL)1);
+        dx[0] = (-se) * c[21] + (((-se) + (REAL)1) * he) * c[3];
+        dx[1] = (-si) * c[23] + hi * c[11];
+        break; }
+    case ORC_ZJ: { /* dfun.rs:1212-1268; sigmoid 2 e0 / (exp((rho_2 - v) rho_1) + 1) */
+        REAL s0 = c[18] / (EXPR(((-x[0]) + c[5]) * c[6]) + (REAL)1);
+        REAL s23 = c[18] / (EXPR(((-(x[2] - x[4])) + c[5]) * c[6]) + (REAL)1);
+        dx[0] = x[1];
+        dx[1] = ((s23 * c[23]) - x[1] * c[28]) - x[0] * c[21];
+        dx[2] = x[3];
+        dx[3] = ((((s0 * c[24]) + c[15]) + cpl[0]) * c[19] - x[3] * c[28]) - x[2] * c[21];
+        dx[4] = x[5];
+        dx[5] = (((s0 * c[25]) + c[17]) * c[20] - x[5] * c[29]) - x[4] * c[22];
+        dx[6] = x[7];
+        dx[7] = ((s23 * c[26]) - x[7] * c[28]) - x[6] * c[21];
+        dx[8] = x[9];
+        dx[9] = ((s0 * c[27]) - x[9] * c[29]) - x[8] * c[22];
+        dx[10] = x[3] - x[5];
+        dx[11] = x[7] - x[9];
+        break; }
+    case ORC_RFHN: { /* dfun.rs:1270-1300 */
+        REAL xi = x[0], eta = x[1], al = x[2], be = x[3];
+        REAL xi3 = (xi * xi) * xi, al3 = (al * al) * al;
+        dx[0] = ((((xi - xi3 / (REAL)3) - eta) * c[0] + (-xi) * c[3]) - (al - xi) * c[4]) + cpl[0] * c[0];
+        dx[1] = ((xi - eta * c[2]) + c[1]) / c[0];
+        dx[2] = (((al - al3 / (REAL)3) - be) * c[0] + (-xi) * c[5]) + cpl[1] * c[0];
+        dx[3] = ((al - be * c[2]) + c[1]) / c[0];
+        break; }
+    case ORC_RHR: { /* dfun.rs:1302-1348 */
+        REAL xi = x[0], eta = x[1], ts = x[2], al = x[3], be = x[4], ga = x[5];
+        REAL xi2 = xi * xi, xi3 = (xi * xi) * xi, al2 = al * al, al3 = (al * al) * al;
+        dx[0] = (((((eta - xi3 * c[1]) + xi2 * c[2]) - ts) + (-xi) * c[7]) - (al - xi) * c[8]) + cpl[0];
+        dx[1] = ((-xi2) * c[4] - eta) + c[3];
+        dx[2] = (xi * c[12] - ts * c[0]) + c[13];
+        dx[3] = ((((be - al3 * c[1]) + al2 * c[2]) - ga) + (-xi) * c[9]) + cpl[1];
+        dx[4] = ((-al2) * c[4] - be) + c[3];
+        dx[5] = (al * c[12] - ga * c[0]) + c[13];
+        break; }
+    case ORC_DG: { /* dfun.rs:1350-1400 (v times the bracket, as the reference writes it) */
+        REAL re = x[0], ve = x[1], see = x[2], sei = x[3], ri = x[4], vi = x[5], sie = x[6], sii = x[7];
+        dx[0] = (((ve * (REAL)2) * re) + c[20]) * c[17];
+        dx[1] = (ve * (((((((ve - (re * re) * c[21]) + c[2]) + see * c[14]) - sei * c[14]) + c[0]) + cpl[0]) +
+                       cpl[1] * (REAL)0)) * c[17];
+        dx[2] = (((-see) + re * c[9]) + cpl[2]) * c[19];
+        dx[3] = ((-sei) + ri * c[10]) * c[19];
+        dx[4] = (((vi * (REAL)2) * ri) + c[22]) * c[18];
+        dx[5] = (vi * ((((((vi - (ri * ri) * c[23]) + c[6]) + sie * c[15]) - sii * c[15]) + c[4]) + cpl[2] * c[13])) *
+                c[18];
+        dx[6] = (((-sie) + re * c[11]) + cpl[2] * c[13]) * c[19];
+        dx[7] = ((-sii) + ri * c[12]) * c[19];
+        break; }
     }
 }
 
@@ -560,6 +753,8 @@ static void FN(dfun)(const FN(mconst)* k, const REAL* x, const REAL* c, REAL* dx
         dx[1] = ((-(v * v) + k->cb_eta) - r2 * k->cb_pi2) + ((-v + k->cb_vsyn) * k->cb_kpi) * r;
         break; }
     case ORC_EC3: case ORC_EC3S: case ORC_ERS: case ORC_ZL1: case ORC_ZL2: case ORC_KIX:
+    case ORC_CB: case ORC_GSD: case ORC_GSF: case ORC_LB: case ORC_E2D: case ORC_EPI: case ORC_RWEI:
+    case ORC_DECO: case ORC_ZJ: case ORC_RFHN: case ORC_RHR: case ORC_DG:
         FN(dfun_ext)(k, x, c, dx);
         break;
     case ORC_RWW: { /* dfun.rs:503-515 */
@@ -597,7 +792,16 @@ static void FN(clamp)(int model, REAL* x)
         x[1] = x[1] < 0 ? 0 : (x[1] > 1 ? 1 : x[1]);
         break;
     case ORC_KIX: /* dfun.rs:232-236: x >= 0 */
+    case ORC_CB: case ORC_GSD: case ORC_GSF: /* dfun.rs:195-209: r >= 0 */
         x[0] = x[0] < 0 ? 0 : x[0];
+        break;
+    case ORC_RWEI: case ORC_DECO: /* dfun.rs:172-176: both in [0, 1] */
+        x[0] = x[0] < 0 ? 0 : (x[0] > 1 ? 1 : x[0]);
+        x[1] = x[1] < 0 ? 0 : (x[1] > 1 ? 1 : x[1]);
+        break;
+    case ORC_DG: /* dfun.rs:210-218: r_e, r_i >= 0 */
+        x[0] = x[0] < 0 ? 0 : x[0];
+        x[4] = x[4] < 0 ? 0 : x[4];
         break;
     case ORC_KUR: { /* dfun.rs:188-191 */
         REAL tp = (REAL)2 * PIR;

@@ -434,6 +434,102 @@ def test_fixture_models_hybrid_and_stochastic(gpu, orc):
                                             dtype=np.float64))
 
 
+# ---- the remaining registry models (dfun.rs:609-950, 1212-1400) ----
+
+def _golden_row0(name):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "model_fixture_traces.npz")
+    return np.load(path, allow_pickle=False)[name]
+
+
+# model, the fixture whose first row is the base state, dt, steps, integrator
+REMAINING = {
+    "coombes_byrne": (L.COOMBES_BYRNE, "coombes_byrne_ref", 0.01, 100, L.RK4),
+    "gast_sd": (L.GAST_SD, "gast_sd_ref", 0.1, 200, L.HEUN),
+    "gast_sf": (L.GAST_SF, "gast_sf_ref", 0.01, 100, L.RK4),
+    "larter_breakspear": (L.LARTER_BREAKSPEAR, "larter_breakspear_ref", 0.1, 200, L.EULER),
+    "epileptor2d": (L.EPILEPTOR_2D, "epileptor2d_ref", 0.1, 200, L.HEUN),
+    "epileptor": (L.EPILEPTOR, "epileptor_ref", 0.05, 200, L.RK4),
+    "rww_exc_inh": (L.RWW_EXC_INH, "rww_exc_inh_ref", 0.1, 200, L.EULER),
+    "deco_balanced": (L.DECO_BALANCED, "deco_balanced_ref", 0.1, 200, L.HEUN),
+    "zetterberg_jansen": (L.ZETTERBERG_JANSEN, "zetterberg_jansen_ref", 0.1, 200, L.RK4),
+    "reduced_fhn": (L.REDUCED_FHN, "reduced_fhn_ref", 0.1, 200, L.EULER),
+    "reduced_hr": (L.REDUCED_HR, "reduced_hr_ref", 0.1, 200, L.HEUN),
+    "dumont_gutkin": (L.DUMONT_GUTKIN, "dumont_gutkin_ref", 0.1, 200, L.RK4),
+}
+
+
+@pytest.mark.parametrize("name", list(REMAINING))
+def test_remaining_models_on_the_gpu(gpu, orc, name):
+    """Each model as a subnetwork of 6 nodes (the fixture's first row, spread)
+    coupled both ways with a G2DO subnetwork through channel 0, 2 sweep
+    points: GPU against the f32 oracle (rtol 2e-4) and within 4x the oracle's
+    own error against its FP64 twin."""
+    from hyburn_amd.synth import MULTI_DEFAULTS
+    m, fx, dt, T, integ = REMAINING[name]
+    nv = L.MODEL_INFO[m][0]
+    rng = np.random.default_rng(len(name))
+    base = _golden_row0(fx)[0]
+    x0 = (base[None, :] * (1.0 + 0.05 * rng.standard_normal((6, nv)))).astype(np.float32)
+    subs = [(m, 6, MULTI_DEFAULTS[m], x0), (L.G2DO, 5, G2DO_DEFAULT, _x0(rng, 5, 2))]
+    links = [(1, 0, 0.02, L.CF_LINEAR, [1.0], [(0, 0)]), (0, 1, 0.01, L.CF_LINEAR, [1.0], [(0, 0)])]
+    res = _run(subs, links, dt=dt, n_steps=T, integrator=integ, n_sweep=2)
+    tav, fin = orc.run_multi(subs, links, dt, T, integ, False, n_sweep=2)
+    _check(res, tav, fin, o64=orc.run_multi(subs, links, dt, T, integ, False, n_sweep=2, dtype=np.float64))
+    assert not np.allclose(res.final_states[0][0], x0)   # the model moves
+
+
+@pytest.mark.parametrize("name", ["coombes_byrne", "larter_breakspear", "epileptor2d", "epileptor", "rww_exc_inh",
+                                  "deco_balanced", "reduced_fhn", "reduced_hr"])
+def test_remaining_models_replay_their_fixture(gpu, name):
+    """One uncoupled node from the fixture's first row, the plain engine's
+    Euler at dt 0.1 with default parameters: the GPU follows the TVB trace the
+    reference keeps (relative 5e-4 where |expected| >= 1e-4, absolute below),
+    as the CPU suite checks for the oracle."""
+    from hyburn_amd.synth import MULTI_DEFAULTS
+    m, fx = REMAINING[name][0], REMAINING[name][1]
+    ref = _golden_row0(fx)
+    ok = np.isfinite(ref).all(1) & (np.abs(ref).max(1) < 1e4)
+    n = int(np.argmin(ok)) if not ok.all() else ref.shape[0]
+    r = run_multi([Subnet(m, 1, MULTI_DEFAULTS[m], ref[0][None, :])], [], dt=0.1, n_steps=n - 1,
+                  integrator=L.EULER, plain=True, record_trajectory=True)
+    got = r.trajectories[0][:, 0, 0, :].astype(np.float64)
+    e = ref[1:n].astype(np.float64)
+    big = np.abs(e) >= 1e-4
+    assert np.all(np.isfinite(got))
+    assert np.all(np.abs(got - e)[~big] < 1e-4)
+    assert np.max(np.abs(got - e)[big] / np.abs(e)[big]) <= 5e-4
+
+
+def test_four_cvar_and_twelve_variable_models_hybrid_stochastic(gpu, orc):
+    """DumontGutkin's four coupling channels from a Linear self-projection
+    (all four pass through), ZetterbergJansen's twelve variables (the
+    twelve-register kernel), Epileptor's two channels from ReducedSetHindmarshRose
+    (cvar_map 0:0 and 1:1); hybrid_integrator: deterministic Heun for
+    Epileptor (model_prefers_heun), stochastic Euler elsewhere."""
+    from hyburn_amd.synth import MULTI_DEFAULTS
+    rng = np.random.default_rng(12)
+    subs = []
+    for name in ("dumont_gutkin", "zetterberg_jansen", "epileptor", "reduced_hr"):
+        m, fx = REMAINING[name][0], REMAINING[name][1]
+        nv = L.MODEL_INFO[m][0]
+        base = _golden_row0(fx)[0]
+        subs.append((m, 4, MULTI_DEFAULTS[m], (base[None, :] * (1.0 + 0.02 * rng.standard_normal((4, nv))))
+                     .astype(np.float32)))
+    links = [(0, 0, 0.02, L.CF_LINEAR, [0.5, 0.0], [(0, 0)]),   # DG: r_e, v_e, s_ee, s_ei -> its 4 channels
+             (1, 0, 0.01, L.CF_LINEAR, [1.0], [(0, 0)]),        # ZJ v1 -> DG channel 0
+             (3, 2, 0.01, L.CF_LINEAR, [1.0], [(0, 0), (1, 1)]),  # RHR xi, eta -> Epileptor channels 0, 1
+             (2, 1, 0.001, L.CF_LINEAR, [1.0], [(0, 0)])]
+    T = 120
+    raw = [rng.standard_normal((T, 4, L.MODEL_INFO[s[0]][0])).astype(np.float32) for s in subs]
+    nsig = np.full(12, 1e-6, np.float32)
+    res = _run(subs, links, dt=0.05, n_steps=T, integrator=L.EULER_STOCH, hybrid_integrator=True, nsig=nsig,
+               noise=raw, n_sweep=2)
+    tav, fin = orc.run_multi(subs, links, 0.05, T, L.EULER_STOCH, True, nsig=nsig, noise=raw, n_sweep=2)
+    _check(res, tav, fin, o64=orc.run_multi(subs, links, 0.05, T, L.EULER_STOCH, True, nsig=nsig, noise=raw,
+                                            n_sweep=2, dtype=np.float64))
+
+
 # ---- run_sweep_multi over several subnetworks (engine.rs:862-1348) ----
 
 def test_cartesian_sweep_over_subnetworks(gpu, orc):
