@@ -789,18 +789,40 @@ __device__ __forceinline__ void run_blocked(const DevInst* __restrict__ I, float
     // ---- blocks of K steps starting at t0 = 1 (a continued run: t0 = 0)
     const int tb = fresh ? 1 : 0;
     int ws0 = tb % Hd;   // ring slot of time t0
+    // The block's noise (the long block serves deterministic runs only: no
+    // noise registers). With up to two state variables the next block's
+    // noise is loaded while the current block steps (one block of HBM
+    // latency hidden); otherwise it is issued at the block's start, before
+    // the gather, so that its latency overlaps the gather only.
+    constexpr int KZ = K == kBlockSteps ? K : 1;
+    constexpr bool ZAHEAD = KZ == K && NV <= 2;
+    float zn[ZAHEAD ? KZ : 1][NV];
+    auto load_noise = [&](float (&z)[ZAHEAD ? KZ : 1][NV], int ta) {
+        const int ka = min(K, T - ta);
+#pragma unroll
+        for (int k = 0; k < (ZAHEAD ? KZ : 1); ++k)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                z[k][v] = (S.stoch && own && k < ka) ? noise[((size_t)(ta + k) * N + n) * NV + v] : 0.0f;
+    };
+    if constexpr (ZAHEAD) load_noise(zn, tb);
     for (int t0 = tb; t0 < T; t0 += K) {
         __syncthreads();
         const int kb = min(K, T - t0);
-        // the block's noise, issued before the gather so its latency overlaps it
-        // (the long block serves deterministic runs only: no noise registers)
-        constexpr int KZ = K == kBlockSteps ? K : 1;
         float zb[KZ][NV];
+        if constexpr (ZAHEAD) {
 #pragma unroll
-        for (int k = 0; k < KZ; ++k)
+            for (int k = 0; k < KZ; ++k)
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
-                zb[k][v] = (KZ == K && S.stoch && own && k < kb) ? noise[((size_t)(t0 + k) * N + n) * NV + v] : 0.0f;
+                for (int v = 0; v < NV; ++v) zb[k][v] = zn[k][v];
+            if (t0 + K < T) load_noise(zn, t0 + K);
+        } else {
+#pragma unroll
+            for (int k = 0; k < KZ; ++k)
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+                    zb[k][v] = (KZ == K && S.stoch && own && k < kb) ? noise[((size_t)(t0 + k) * N + n) * NV + v] : 0.0f;
+        }
         float acc[K][NP];
 #pragma unroll
         for (int k = 0; k < K; ++k)
