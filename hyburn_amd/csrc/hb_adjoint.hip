@@ -1557,8 +1557,11 @@ __global__ __launch_bounds__(MAXT) void k_adj_forward_b(const AdjInst* __restric
             if (full_tape) V.tape[tid * NV + v] = x[v];
         }
     }
-    // CSR row of this node in registers: source row base, delay, weight
+    // CSR row of this node in registers: source row base, delay, weight;
+    // kG register edges per gather group (one at 1024 threads, whose register
+    // budget is spent)
     constexpr int kReg = MAXT > 512 ? 8 : 32;
+    constexpr int kG = MAXT > 512 ? 1 : 4;
     int en[kReg], ed[kReg];
     float ew[kReg];
     int e0 = 0, e1 = 0;
@@ -1681,17 +1684,29 @@ __global__ __launch_bounds__(MAXT) void k_adj_forward_b(const AdjInst* __restric
             for (int c = 0; c < NP; ++c) acc[k][c] = 0.0f;
         if (own) {
             const int s0 = s0b;
+            // register edges in groups of kG whose windows load together (one
+            // round trip per group); a group's padding entries read the own
+            // row's base with weight 0. Sums stay in row order.
 #pragma unroll
-            for (int e = 0; e < kReg; ++e) {
-                if (e < nreg) {
-                    int sl = s0 - ed[e];
-                    sl += sl < 0 ? H : 0;
+            for (int g = 0; g < kReg; g += kG) {
+                if (g < nreg) {
+                    float wv[kG][kK][NP];
 #pragma unroll
-                    for (int c = 0; c < NP; ++c) {
-                        const size_t base = (size_t)en[e] + sl;
+                    for (int j = 0; j < kG; ++j) {
+                        int sl = s0 - ed[g + j];
+                        sl += sl < 0 ? H : 0;
+                        const size_t base = (size_t)en[g + j] + sl;
 #pragma unroll
-                        for (int k = 0; k < kK; ++k) acc[k][c] += ew[e] * ring[ridx(c, base + k)];
+                        for (int c = 0; c < NP; ++c)
+#pragma unroll
+                            for (int k = 0; k < kK; ++k) wv[j][k][c] = ring[ridx(c, base + k)];
                     }
+#pragma unroll
+                    for (int j = 0; j < kG; ++j)
+#pragma unroll
+                        for (int c = 0; c < NP; ++c)
+#pragma unroll
+                            for (int k = 0; k < kK; ++k) acc[k][c] += ew[g + j] * wv[j][k][c];
                 }
             }
             for (int e = e0 + nreg; e < e1; ++e) {
@@ -1791,6 +1806,7 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
     for (int v = 0; v < NV; ++v) lam[v] = 0.0f;
     double gpar = 0.0, gcc = 0.0;
     constexpr int kReg = MAXT > 512 ? 8 : 16;   // register-resident out-edges (the rest read from memory)
+    constexpr int kG = MAXT > 512 ? 1 : 4;      // register out-edges per gather group
     int en[kReg], ed[kReg];
     float ew[kReg];
     int e0 = 0, e1 = 0;
@@ -2048,17 +2064,27 @@ __global__ __launch_bounds__(MAXT) void k_adj_reverse_b(const AdjInst* __restric
                 for (int c = 0; c < NP; ++c) acc[m][c] = 0.0f;
             const int sb = wrap_slot(tbot, H);   // ring slot of tbot (one division per block)
             if (own && V.has_c) {
+                // register out-edges in groups of kG loaded together, as in the forward
 #pragma unroll
-                for (int e = 0; e < kReg; ++e) {
-                    if (e < nreg) {
-                        int sl = sb + ed[e];
-                        sl -= sl >= H ? H : 0;
+                for (int g = 0; g < kReg; g += kG) {
+                    if (g < nreg) {
+                        float wv[kG][kK][NP];
 #pragma unroll
-                        for (int c = 0; c < NP; ++c) {
-                            const size_t base = (size_t)en[e] + sl;
+                        for (int j = 0; j < kG; ++j) {
+                            int sl = sb + ed[g + j];
+                            sl -= sl >= H ? H : 0;
+                            const size_t base = (size_t)en[g + j] + sl;
 #pragma unroll
-                            for (int m = 0; m < kK; ++m) acc[m][c] += ew[e] * ring[ridx(c, base + m)];
+                            for (int c = 0; c < NP; ++c)
+#pragma unroll
+                                for (int m = 0; m < kK; ++m) wv[j][m][c] = ring[ridx(c, base + m)];
                         }
+#pragma unroll
+                        for (int j = 0; j < kG; ++j)
+#pragma unroll
+                            for (int c = 0; c < NP; ++c)
+#pragma unroll
+                                for (int m = 0; m < kK; ++m) acc[m][c] += ew[g + j] * wv[j][m][c];
                     }
                 }
                 for (int e = e0 + nreg; e < e1; ++e) {
