@@ -181,7 +181,8 @@ class HbMultiSpec(C.Structure):
                 ("hist_in", fp), ("hist_out", fp), ("plain", C.c_int32), ("n_stimuli", C.c_int32),
                 ("stimuli", C.POINTER(HbStimulus)), ("dt_ms", C.c_double), ("trajectory", fp),
                 ("bold_period", C.c_int32), ("bold_tr_ms", C.c_double), ("bold", fp),
-                ("point_mode", C.c_int32 * 4), ("point_values", fp), ("point_params", fp)]
+                ("point_mode", C.c_int32 * 4), ("point_values", fp), ("point_params", fp),
+                ("bold_mean", C.c_int32), ("bold_state", fp)]
 
 
 MAX_PARAMS = 64   # HB_MAX_PARAMS
@@ -225,6 +226,8 @@ PROTOTYPES = {
     "hb_fcd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, fp, C.c_int32]),
     "hb_run_multi": (C.c_int, [C.POINTER(HbMultiSpec)]),
     "hb_bold_rows": (C.c_int32, [C.c_int32, C.c_int32, C.c_double, C.c_double]),
+    "hb_bold_downsample": (C.c_int32, [C.POINTER(C.c_double), C.c_int32, fp, C.c_int32, C.c_double, C.c_double, fp,
+                                       C.c_int32]),
     "hb_welch_bins": (C.c_int32, [C.c_int32, C.c_int32]),
     "hb_welch_psd_batch": (C.c_int, [fp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32,
                                      C.POINTER(C.c_double), C.c_int32]),

@@ -586,4 +586,15 @@ void hb_result_free(hb_result* r) {
     r->tavg = r->final_states = r->trajectories = r->bold = nullptr;
 }
 
+int32_t hb_bold_downsample(const double* times, int32_t n, const float* native, int32_t n_nodes, double dtb_ms,
+                           double tr_ms, float* out, int32_t cap) {
+    if (n <= 0 || n_nodes < 1 || !times || !native) return 0;
+    const std::vector<double> tm(times, times + n);
+    const std::vector<float> data(native, native + (size_t)n * n_nodes);
+    const int rows = downsample_to_tr(tm, data, n_nodes, dtb_ms, tr_ms, nullptr);
+    if (!out) return rows;
+    if (rows > cap) return -1;
+    return downsample_to_tr(tm, data, n_nodes, dtb_ms, tr_ms, out);
+}
+
 }  // extern "C"

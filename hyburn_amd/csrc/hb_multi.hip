@@ -101,6 +101,7 @@ struct Args {
     float* bold;           // [B][bnout][n of subnetwork 0] or null
     int32_t bper, bnout, bnative;
     double tr_ms, dtb_ms;
+    float* v0rec;          // [B][n_steps][n of subnetwork 0] var0 per step (the batch-mean monitor's input) or null
 };
 
 __device__ __forceinline__ void pre_rt(const Proj& P, const float* x, float* out) {
@@ -399,6 +400,7 @@ __global__ __launch_bounds__(1024) void k_multi(const Args* __restrict__ A) {
                 for (int v = 0; v < nv; ++v) o[v] = x[v];
             }
             if (bnode) bw.acc = bw.acc + x[0];
+            if (A->v0rec && s == 0) A->v0rec[((size_t)b * A->n_steps + k) * N0 + i] = x[0];
         }
         if (bper > 0 && ++bcount >= bper) {
             const double dtb = A->dt_ms * (double)bcount;
@@ -796,7 +798,31 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     const BoldShape bs = bold_shape(Q.n_steps, Q.bold_period, Q.dt_ms, Q.bold_tr_ms);
     float* dbold = nullptr;
     const size_t nbold = (size_t)Q.n_sweep * bs.nout * Q.subnets[0].n_nodes;
-    if (Q.bold && Q.bold_period > 0) {
+    // the batch engine's monitor of the sweep mean: var0 of subnetwork 0 is
+    // recorded per point and step, then averaged and fed to one monitor below
+    const bool bmean = Q.bold_mean != 0 && Q.bold && Q.bold_period > 0;
+    const int N0 = Q.subnets[0].n_nodes;
+    const BoldShape bsn = bold_shape(Q.n_steps, Q.bold_period, Q.dt_ms, 0.0);
+    float *dv0 = nullptr, *dzm = nullptr, *dbw = nullptr, *dnat = nullptr;
+    const float** dptr = nullptr;
+    if (bmean) {
+        if (Q.dt_ms <= 0.0) return bad("BOLD needs dt_ms > 0");
+        std::vector<float> bw((size_t)N0 * 4);
+        for (int n = 0; n < N0; ++n) {
+            for (int c = 0; c < 4; ++c) bw[(size_t)n * 4 + c] = Q.bold_state ? Q.bold_state[(size_t)n * 4 + c] : (c ? 1.0f : 0.0f);
+        }
+        e = B.alloc(&dv0, (size_t)Q.n_sweep * Q.n_steps * N0);
+        if (e == hipSuccess) e = B.alloc(&dzm, (size_t)Q.n_steps * N0);
+        if (e == hipSuccess) e = B.alloc(&dbw, (size_t)N0 * 4);
+        if (e == hipSuccess) e = B.alloc(&dnat, (size_t)std::max(bsn.nout, 1) * N0);
+        if (e == hipSuccess) e = B.alloc(&dptr, (size_t)Q.n_sweep);
+        std::vector<const float*> ptrs((size_t)Q.n_sweep);
+        for (int b = 0; b < Q.n_sweep; ++b) ptrs[b] = dv0 + (size_t)b * Q.n_steps * N0;
+        if (e == hipSuccess) e = hipMemcpy(dptr, ptrs.data(), sizeof(float*) * ptrs.size(), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dbw, bw.data(), sizeof(float) * bw.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "batch-mean BOLD buffers");
+        A.v0rec = dv0;
+    } else if (Q.bold && Q.bold_period > 0) {
         if (Q.dt_ms <= 0.0) return bad("BOLD needs dt_ms > 0");
         e = B.alloc(&dbold, nbold);
         if (e != hipSuccess) return hip_fail(e, "BOLD buffer");
@@ -859,6 +885,15 @@ extern "C" int hb_run_multi(hb_multi_spec* spec) {
     (void)hipEventDestroy(ev1);
     if (e != hipSuccess) return hip_fail(e, "multi-subnetwork kernel");
     spec->kernel_ms = ms;
+    if (bmean) {   // mean over the points in order, then the monitor's windows (engine.rs:785-801)
+        e = launch_bold_mean_fast(dptr, Q.n_sweep, Q.n_steps, N0, 1, dzm, Q.bold_period, Q.dt_ms, dbw, dnat, nullptr);
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+        if (e == hipSuccess && bsn.nout > 0)
+            e = hipMemcpy(Q.bold, dnat, sizeof(float) * (size_t)bsn.nout * N0, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && Q.bold_state)
+            e = hipMemcpy(Q.bold_state, dbw, sizeof(float) * (size_t)N0 * 4, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "batch-mean BOLD");
+    }
     std::vector<float> ht(nout), hf(nout);
     e = hipMemcpy(ht.data(), dtavg, sizeof(float) * nout, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(hf.data(), dfin, sizeof(float) * nout, hipMemcpyDeviceToHost);

@@ -276,7 +276,9 @@ class BatchSweepResult:
     bold: one [series][rows][nnodes] array, the whole series since the engine
     started resampled to TR; series = n_sweep (a monitor per sweep point) or
     1 (bold_batch_mean: the reference's monitor of the sweep mean,
-    engine.rs:785-801). The multi-subnetwork path has per-point series only."""
+    engine.rs:785-801). On the multi-subnetwork path the per-point series
+    cover the run only; the batch-mean monitor carries its series across runs
+    as the reference's does."""
     param_values: list
     tavg: list
     final_states: list
@@ -407,6 +409,9 @@ class BatchHybridEngine:
         # one BOLD monitor fed the sweep mean of var0 (the reference's batch
         # monitor, engine.rs:785-801) instead of one per sweep point
         self.bold_batch_mean = False
+        # that monitor on the multi-subnetwork path, carried across runs:
+        # Balloon-Windkessel state, native samples and their times
+        self._bmon_state, self._bmon_native, self._bmon_times = None, [], []
         self.step = 0
 
     @classmethod
@@ -414,9 +419,6 @@ class BatchHybridEngine:
                     engine: int = L.ENGINE_BATCH, delay_mode: int = -1, noise_seed: int = 0,
                     noise_shared: bool = True, bold_batch_mean: bool = False) -> "BatchHybridEngine":
         self = cls._build(base_config, n_sweep, device, engine, delay_mode, noise_seed, noise_shared)
-        if bold_batch_mean and self._multi is not None:
-            _err(L.HB_INVALID_CONFIG, "the batch-mean BOLD monitor is built for single-network configurations; "
-                                      "the multi-subnetwork path records one BOLD series per sweep point")
         self.bold_batch_mean = bool(bold_batch_mean)
         return self
 
@@ -563,14 +565,45 @@ class BatchHybridEngine:
                       sweep=(param.sub_idx, param.param_idx, v), max_delay=int(self._max_delay),
                       step0=self._step, state=self._state, hist=self._hist, record_trajectory=traj,
                       bold_period=self._bold[0] if self._bold else 0,
-                      bold_tr_ms=self._bold[1] if self._bold else 0.0, device=device)
+                      bold_tr_ms=self._bold[1] if self._bold else 0.0, device=device,
+                      bold_mean=self.bold_batch_mean, bold_state=self._bmon_state)
         ms = (time.perf_counter() - t0) * 1e3
         self._step, self._state, self._hist = r.step_end, r.final_states, r.hist
+        bold = r.bold
+        if r.bold is not None and self.bold_batch_mean:
+            bold = self._bold_mean_series(r, int(n_steps))
         return BatchSweepResult(param_values=v.tolist(), tavg=[t.ravel() for t in r.tavg],
                                 final_states=[f.ravel() for f in r.final_states],
                                 trajectories=[t.ravel() for t in r.trajectories] if traj else None,
-                                bold=[r.bold] if r.bold is not None else None,
+                                bold=[bold] if bold is not None else None,
                                 elapsed_ms=ms, n_sweep=self.n_sweep, kernel_ms=r.kernel_ms)
+
+    def _bold_mean_series(self, r, n_steps: int) -> np.ndarray:
+        """The batch monitor's whole series since the engine started, [1][rows][nnodes]:
+        this run's native samples appended at the times BoldMonitor gives them
+        (full windows end at len * dtb + dtb, the flushed partial window at
+        len * dtb + dt * rem; bold_monitor.rs:100, 214), resampled to TR by
+        hb_bold_downsample (bold_monitor.rs:123-203)."""
+        P, tr_ms = self._bold
+        dtb = float(self.dt) * P
+        n_full, rem = divmod(n_steps, P)
+        for k in range(r.bold.shape[0]):
+            ln = len(self._bmon_times)
+            self._bmon_times.append(ln * dtb + dtb if k < n_full else ln * dtb + float(self.dt) * rem)
+        self._bmon_native.append(r.bold)
+        self._bmon_state = r.bold_state
+        nat = np.ascontiguousarray(np.concatenate(self._bmon_native), np.float32)
+        tm = np.ascontiguousarray(self._bmon_times, np.float64)
+        dp = C.POINTER(C.c_double)
+        N = nat.shape[1]
+        rows = int(L.lib().hb_bold_downsample(tm.ctypes.data_as(dp), len(tm), nat.ctypes.data_as(L.fp), N, dtb,
+                                              tr_ms, None, 0))
+        out = np.zeros((max(rows, 1), N), np.float32)
+        got = int(L.lib().hb_bold_downsample(tm.ctypes.data_as(dp), len(tm), nat.ctypes.data_as(L.fp), N, dtb,
+                                             tr_ms, out.ctypes.data_as(L.fp), out.shape[0]))
+        if got != rows:
+            _err(L.HB_BACKEND_ERROR, "BOLD resampling returned an unexpected row count")
+        return out[:rows][None]
 
     def _hybrid_changes_rule(self) -> bool:
         # hybrid_integrator: Heun for G2DO, else the configured integrator with
@@ -640,8 +673,11 @@ class BatchHybridEngine:
                       noise_seed=self._noise_seed, hybrid_integrator=self.hybrid_integrator, n_sweep=n,
                       sweep_spec=dims, max_delay=int(self._max_delay), record_trajectory=traj, device=device)
         ms = (time.perf_counter() - t0) * 1e3
-        # engine.rs:1326-1331: the caller's engine takes over states, step and histories
+        # engine.rs:1326-1331: the caller's engine takes over states, step and
+        # histories, and the local engine's monitors, of which it has none
+        # (engine.rs:899-905): later runs record no BOLD
         self._state, self._hist, self._step = r.final_states, r.hist, int(n_steps)
+        self._bold, self._bmon_state, self._bmon_native, self._bmon_times = None, None, [], []
         return BatchSweepResult(param_values=[v for pt in spec.cartesian_product() for (_, _, v) in pt],
                                 tavg=[t.ravel() for t in r.tavg], final_states=[f.ravel() for f in r.final_states],
                                 trajectories=[t.ravel() for t in r.trajectories] if traj else None, bold=None,

@@ -50,7 +50,8 @@ class MultiResult:
     kernel_ms: float = 0.0
     hist: Optional[np.ndarray] = None     # [n_sweep][total nodes][H][2] by step % H (delayed runs)
     trajectories: Optional[list] = None   # per subnetwork [n_steps][n_sweep][n_nodes][nvar]
-    bold: Optional[np.ndarray] = None     # subnetwork 0: [n_sweep][rows][n_nodes]
+    bold: Optional[np.ndarray] = None     # subnetwork 0: [n_sweep][rows][n_nodes] ([rows][n_nodes] native: bold_mean)
+    bold_state: Optional[np.ndarray] = None   # bold_mean: the monitor's s, f, v, q per node after the run
     step_end: int = 0                     # the engine's step counter after the run
 
 
@@ -118,7 +119,8 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
               noise_seed: int = 0, step0: int = 0, state: Optional[list] = None,
               hist: Optional[np.ndarray] = None, plain: bool = False, stimuli: Sequence[tuple] = (),
               dt_ms: Optional[float] = None, record_trajectory: bool = False, bold_period: int = 0,
-              bold_tr_ms: float = 0.0, device: int = 0, sweep_spec: Optional[Sequence[tuple]] = None) -> MultiResult:
+              bold_tr_ms: float = 0.0, device: int = 0, sweep_spec: Optional[Sequence[tuple]] = None,
+              bold_mean: bool = False, bold_state: Optional[np.ndarray] = None) -> MultiResult:
     """max_delay: the largest entry of any projection's delay list, which sets
     the history horizon (engine.rs:253-257); each link's own delay counts too.
     Stochastic integrators: nsig per variable (a scalar is repeated over the
@@ -139,7 +141,12 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
     engine instead feeds one monitor the mean of var0 over the sweep points
     (engine.rs:785-801); the Balloon-Windkessel model is nonlinear, so with
     n_sweep > 1 these per-point series are not the reference's single series
-    (the single-network engine has that monitor: bold_batch_mean).
+    (bold_mean=True gives that monitor).
+    bold_mean: one monitor fed var0 of subnetwork 0 averaged over the sweep
+    points in point order (engine.rs:785-801); bold is then this run's native
+    window samples [rows][n_nodes], the partial window flushed, and
+    MultiResult.bold_state its Balloon-Windkessel state [n_nodes][4] (s, f, v,
+    q), which bold_state= carries into the next run (None: from rest).
     sweep_spec: run_sweep_multi's Cartesian product (engine.rs:56-91,
     862-1348), dimensions (sub_idx, param_idx, values), the last varying
     fastest; n_sweep is the product of their lengths. Per subnetwork, one
@@ -249,8 +256,20 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
                 raise ValueError(f"hist must be {hist_out.shape}")
             keep.append(hi)
             spec.hist_in = hi.ctypes.data_as(L.fp)
-    bold = None
-    if bold_period > 0:
+    bold = bstate = None
+    if bold_period > 0 and bold_mean:
+        rows = int(L.lib().hb_bold_rows(n_steps, bold_period, spec.dt_ms, 0.0))
+        bold = np.zeros((rows, subnets[0].n_nodes), np.float32)
+        bstate = np.zeros((subnets[0].n_nodes, 4), np.float32)
+        bstate[:, 1:] = 1.0
+        if bold_state is not None:
+            bs = np.asarray(bold_state, np.float32)
+            if bs.shape != bstate.shape:
+                raise ValueError(f"bold_state must be {bstate.shape}")
+            bstate[:] = bs
+        spec.bold_period, spec.bold, spec.bold_mean = bold_period, bold.ctypes.data_as(L.fp), 1
+        spec.bold_state = bstate.ctypes.data_as(L.fp)
+    elif bold_period > 0:
         rows = int(L.lib().hb_bold_rows(n_steps, bold_period, spec.dt_ms, bold_tr_ms))
         bold = np.zeros((n_sweep, rows, subnets[0].n_nodes), np.float32)
         spec.bold_period, spec.bold_tr_ms, spec.bold = bold_period, bold_tr_ms, bold.ctypes.data_as(L.fp)
@@ -261,7 +280,8 @@ def run_multi(subnets: Sequence[Subnet], links: Sequence[Link], *, dt: float, n_
     fin = np.zeros(sum(sizes), np.float32)
     spec.tavg, spec.final_state = tavg.ctypes.data_as(L.fp), fin.ctypes.data_as(L.fp)
     L.check(L.lib().hb_run_multi(spec))
-    out = MultiResult(kernel_ms=spec.kernel_ms, hist=hist_out, step_end=int(step0) + int(n_steps), bold=bold)
+    out = MultiResult(kernel_ms=spec.kernel_ms, hist=hist_out, step_end=int(step0) + int(n_steps), bold=bold,
+                      bold_state=bstate)
     if traj is not None:
         out.trajectories, o = [], 0
         for S, n in zip(subnets, sizes):

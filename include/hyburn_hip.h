@@ -438,11 +438,31 @@ typedef struct {
     int32_t point_mode[4];
     const float* point_values;
     const float* point_params;
+    /* bold_mean = 1 (with bold_period > 0): the batch engine's own monitor
+     * (engine.rs:785-801), one Balloon-Windkessel model per node of subnetwork 0
+     * fed var0 averaged over the sweep points. bold_state is its s, f, v, q per
+     * node, [n_nodes][4], read and written back (null: from rest, 0 1 1 1), so a
+     * caller carries it between runs as the engine's BoldMonitor does; bold
+     * receives this run's native window samples, [hb_bold_rows(n_steps,
+     * bold_period, dt_ms, 0)][n_nodes], the partial window flushed at the end
+     * (bold_monitor.rs:72-107, 207-227). hb_bold_downsample resamples the
+     * series of several runs to TR; bold_tr_ms is not used. */
+    int32_t bold_mean;
+    float* bold_state;
 } hb_multi_spec;
 
 /* Rows of BOLD output a run of n_steps produces (bold_monitor.rs:72-227): native
  * windows when tr_ms <= 0 or equal to the window length, else floor(last time / TR). */
 int32_t hb_bold_rows(int32_t n_steps, int32_t bold_period, double dt_ms, double tr_ms);
+
+/* BoldMonitor::downsample_to_tr (bold_monitor.rs:123-203): n native samples
+ * [n][n_nodes] taken at times[n] (ms, as the monitor forms them) with window
+ * length dtb_ms, resampled at multiples of tr_ms (linear between the
+ * bracketing samples, nearest outside them; tr_ms <= 0 or equal to dtb_ms:
+ * the native samples). Writes the rows into out when out is not null and cap
+ * holds them; returns the row count, or -1 when out is too small. */
+int32_t hb_bold_downsample(const double* times, int32_t n, const float* native, int32_t n_nodes, double dtb_ms,
+                           double tr_ms, float* out, int32_t cap);
 
 /* BatchHybridEngine::run_sweep over several subnetworks joined by projections
  * (src/engine/batch_engine/engine.rs:540-823): per step, each projection's

@@ -261,3 +261,37 @@ def test_history_floats():
     for I, want in cases:
         m = _Marshal([I])
         assert L.lib().hb_history_floats(m.arr) == want
+
+
+@pytest.mark.parametrize("tr_ms", [0.0, 1.0, 50.0, 0.7, 333.3])
+def test_bold_downsample_matches_the_monitor(orc, tr_ms):
+    """hb_bold_downsample (host code, no device) resamples a monitor's native
+    series as BoldMonitor::downsample_to_tr does (bold_monitor.rs:123-203):
+    against the oracle's restatement, bit for bit, over the times of two runs
+    with partial windows (window 1 ms); tr 0 or equal to the window returns the
+    native samples; a short output buffer is refused."""
+    rng = np.random.default_rng(5)
+    N, dtb = 7, 1.0
+    times = [k * dtb + dtb for k in range(200)] + [200 * dtb + 0.5]
+    times += [len(times) + k + 1.0 * dtb for k in range(199)] + [400 * dtb + 0.5]
+    tm = np.ascontiguousarray(times, np.float64)
+    nat = rng.standard_normal((len(tm), N)).astype(np.float32)
+    dp = C.POINTER(C.c_double)
+    lib = L.lib()
+    rows = lib.hb_bold_downsample(tm.ctypes.data_as(dp), len(tm), nat.ctypes.data_as(L.fp), N, dtb, tr_ms, None, 0)
+    got = np.zeros((rows, N), np.float32)
+    assert lib.hb_bold_downsample(tm.ctypes.data_as(dp), len(tm), nat.ctypes.data_as(L.fp), N, dtb, tr_ms,
+                                  got.ctypes.data_as(L.fp), rows) == rows
+    cap = len(tm) * 2
+    want = np.zeros((cap, N), np.float32)
+    m = orc.lib().hb_orc_bold_downsample(tm.ctypes.data_as(dp), orc._p(nat), len(tm), N, dtb, tr_ms,
+                                         orc._p(want), cap)
+    assert m == rows
+    np.testing.assert_array_equal(got, want[:m])
+    if tr_ms in (0.0, dtb):
+        np.testing.assert_array_equal(got, nat)
+    else:
+        assert rows == int(np.floor(tm[-1] / tr_ms))
+    if rows > 1:
+        assert lib.hb_bold_downsample(tm.ctypes.data_as(dp), len(tm), nat.ctypes.data_as(L.fp), N, dtb, tr_ms,
+                                      got.ctypes.data_as(L.fp), rows - 1) == -1

@@ -578,3 +578,48 @@ def test_engine_run_sweep_multi_several_subnetworks(gpu, orc):
     assert eng._step == 60 and len(eng._state[0]) == 6
     with pytest.raises(L.HyburnError, match="sweep points"):
         eng.run_sweep(SweepParam(0, 1), np.array([0.0, 0.3], np.float32), 10)
+
+
+def test_engine_batch_mean_bold_across_runs(gpu, orc):
+    """The batch engine's BOLD monitor on the multi-subnetwork path
+    (engine.rs:785-801): var0 of subnetwork 0 averaged over the sweep points in
+    point order feeds one Balloon-Windkessel model per node, carried across two
+    runs of 2005 and 1995 steps (partial windows flushed at each end), the
+    second returning the whole series resampled to TR. A run_sweep_multi then
+    leaves the engine without monitors (its local engine has none,
+    engine.rs:899-905, 1331)."""
+    from test_gpu_engine_state import BOLD_ULP, _monitor
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig, SweepParam, SweepPoint, SweepSpec
+    n, P, dt = 6, 10, 0.1
+    d = {"sim_length": 400.0, "dt": dt, "integrator": "Heun", "network": {
+        "subnetworks": [
+            {"model": "Generic2dOscillator", "nnodes": n, "params": G2DO_DEFAULT.tolist(),
+             "initial_state": [0.1 * k for k in range(2 * n)]},
+            {"model": "JansenRit", "nnodes": n, "params": JR_DEFAULT.tolist(), "initial_state": [0.0] * (6 * n)},
+            {"model": "WilsonCowan", "nnodes": n, "params": WC_DEFAULT.tolist(), "initial_state": [0.0] * (2 * n)}],
+        "projections": [{"src": a, "tgt": b, "weights": 0.01, "coupling_fn": "Linear", "coupling_params": [1.0],
+                         "cvar_map": "0:0", "delays": [0]} for a, b in ((0, 1), (1, 2), (2, 0))]},
+        "monitors": [{"monitor_type": "bold", "tr": 0.05, "bold_period": P}]}
+    vals = np.array([-0.5, 0.1, 0.6], np.float32)
+    eng = BatchHybridEngine.from_config(SimConfig.from_dict(d), 3, bold_batch_mean=True)
+    a = eng.run_sweep_with_trajectory(SweepParam(sub_idx=0, param_idx=1), vals, 2005)
+    b = eng.run_sweep_with_trajectory(SweepParam(sub_idx=0, param_idx=1), vals, 1995)
+
+    def pmean(tr, T):   # var0 summed over the points in order, divided by their count (mean_dim)
+        t = tr.reshape(T, 3, n, 2)[..., 0]
+        s = t[:, 0].copy()
+        for k in (1, 2):
+            s = (s + t[:, k]).astype(np.float32)
+        return (s / np.float32(3)).astype(np.float32)
+
+    want = _monitor(orc, [pmean(a.trajectories[0], 2005), pmean(b.trajectories[0], 1995)], n, P, dt, 50.0)
+    got = b.bold[0]
+    # 201 + 200 native samples up to 400.5 ms: 8 rows at TR = 50 ms
+    assert got.shape == (1, 8, n), got.shape
+    np.testing.assert_allclose(got[0], want, rtol=0, atol=4 * BOLD_ULP)
+    assert a.bold[0].shape == (1, 4, n)
+    np.testing.assert_array_equal(a.bold[0], got[:, :4])
+    eng.run_sweep_multi(SweepSpec([SweepPoint(0, 1, [-0.5, 0.0, 0.5])]), 50)
+    c = eng.run_sweep(SweepParam(sub_idx=0, param_idx=1), vals, 50)
+    assert c.bold is None
+    eng.close()
