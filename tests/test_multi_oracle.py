@@ -406,3 +406,68 @@ def test_cartesian_spec_expansion_and_dispatch(orc):
         t1, f1 = orc.run_multi([subs[0], (L.JANSEN_RIT, 3, p, subs[1][3])], links, 0.1, 30, L.RK4, False)
         np.testing.assert_array_equal(fin[1][b], f1[1][0])
         np.testing.assert_array_equal(fin[0][b], f1[0][0])
+
+
+def test_engine_carries_the_batch_mean_monitor_across_runs(orc, monkeypatch):
+    """The multi-subnetwork engine's batch-mean BOLD bookkeeping, with the
+    kernel call stubbed (CPU): each run's native samples are appended at the
+    times BoldMonitor gives them (full windows at len * dtb + dtb, a flushed
+    partial window at len * dtb + dt * rem; bold_monitor.rs:100, 214), the
+    Balloon-Windkessel state returned by one run goes into the next, the whole
+    series is resampled as downsample_to_tr does, and after run_sweep_multi the
+    engine records no BOLD (engine.rs:899-905, 1331)."""
+    import ctypes as C
+    import types
+
+    from hyburn_amd import _lib as L
+    from hyburn_amd import multi
+    from hyburn_amd.engine import BatchHybridEngine, SimConfig, SweepParam, SweepPoint, SweepSpec
+    from hyburn_amd.synth import G2DO_DEFAULT, JR_DEFAULT
+    rng = np.random.default_rng(3)
+    n, P, dt, tr = 4, 10, 0.1, 0.0025   # window 1 ms, TR 2.5 ms
+    seen, natives, seen_state = [], [], []
+
+    def fake_run_multi(subs, links, **kw):
+        seen.append(kw)
+        T, ns = kw["n_steps"], kw["n_sweep"]
+        out = types.SimpleNamespace(
+            final_states=[np.zeros((ns, S.n_nodes, L.MODEL_INFO[S.model][0]), np.float32) for S in subs],
+            tavg=[np.zeros((ns, S.n_nodes, L.MODEL_INFO[S.model][0]), np.float32) for S in subs],
+            hist=None, step_end=kw.get("step0", 0) + T, kernel_ms=0.0, trajectories=None, bold=None,
+            bold_state=None)
+        if kw.get("bold_period", 0) > 0:
+            assert kw["bold_mean"] is True
+            out.bold = rng.standard_normal((-(-T // P), n)).astype(np.float32)
+            natives.append(out.bold.copy())
+            out.bold_state = rng.standard_normal((n, 4)).astype(np.float32)
+            seen_state.append(out.bold_state.copy())
+        return out
+
+    monkeypatch.setattr(multi, "run_multi", fake_run_multi)
+    d = {"sim_length": 10.0, "dt": dt, "integrator": "Heun", "network": {
+        "subnetworks": [
+            {"model": "Generic2dOscillator", "nnodes": n, "params": G2DO_DEFAULT.tolist(), "initial_state": [0.0] * (2 * n)},
+            {"model": "JansenRit", "nnodes": n, "params": JR_DEFAULT.tolist(), "initial_state": [0.0] * (6 * n)}],
+        "projections": [{"src": 0, "tgt": 1, "weights": 0.01, "coupling_fn": "Linear", "coupling_params": [1.0],
+                         "cvar_map": "0:0", "delays": [0]}]},
+        "monitors": [{"monitor_type": "bold", "tr": tr, "bold_period": P}]}
+    eng = BatchHybridEngine.from_config(SimConfig.from_dict(d), 2, bold_batch_mean=True)
+    vals = np.array([0.0, 1.0], np.float32)
+    a = eng.run_sweep(SweepParam(0, 1), vals, 25)
+    b = eng.run_sweep(SweepParam(0, 1), vals, 20)
+    assert seen[0]["bold_state"] is None
+    np.testing.assert_array_equal(seen[1]["bold_state"], seen_state[0])
+    dtb = dt * P
+    times = np.array([dtb, 2 * dtb, 2 * dtb + dt * 5, 3 * dtb + dtb, 4 * dtb + dtb], np.float64)
+    nat = np.ascontiguousarray(np.concatenate(natives), np.float32)
+    cap = 64
+    want = np.zeros((cap, n), np.float32)
+    m = orc.lib().hb_orc_bold_downsample(times.ctypes.data_as(C.POINTER(C.c_double)), orc._p(nat), len(times), n,
+                                         dtb, tr * 1000.0, orc._p(want), cap)
+    np.testing.assert_array_equal(np.asarray(eng._bmon_times), times)
+    assert b.bold[0].shape == (1, m, n) and m == int(np.floor(times[-1] / (tr * 1000.0)))
+    np.testing.assert_array_equal(b.bold[0][0], want[:m])
+    assert a.bold[0].shape == (1, 1, n)   # 2.5 ms of samples: one TR row
+    eng.run_sweep_multi(SweepSpec([SweepPoint(0, 1, [0.0, 1.0])]), 10)
+    c = eng.run_sweep(SweepParam(0, 1), vals, 10)
+    assert seen[-1]["bold_period"] == 0 and c.bold is None
