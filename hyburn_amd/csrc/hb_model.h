@@ -68,8 +68,14 @@ template <> __device__ __forceinline__ void dfun<HB_MPR>(const float* mc, bool s
 }
 
 __device__ __forceinline__ float jr_sigm(const float* mc, float v) {
-    // dfun.rs:350-354: 2*nu_max / (exp((v0 - v) * r) + 1)
+    // dfun.rs:350-354: 2*nu_max / (exp((v0 - v) * r) + 1). Exact builds keep
+    // the correctly rounded quotient; FMA builds multiply by the hardware
+    // reciprocal (1 ulp; the denominator is >= 1, and an overflowed exp gives 0)
+#if HB_EXACT
     return mc[2] / (expf((-v + mc[0]) * mc[1]) + 1.0f);
+#else
+    return mc[2] * __builtin_amdgcn_rcpf(expf((-v + mc[0]) * mc[1]) + 1.0f);
+#endif
 }
 
 template <> __device__ __forceinline__ void dfun<HB_JANSEN_RIT>(const float* mc, bool sw, float sv,
@@ -641,8 +647,13 @@ __device__ __forceinline__ void pre(int cfn, const float* cc, bool cpu, bool cla
         if constexpr (NC >= 2) diff = x[0] - x[1];
         else diff = x[0] - 0.0f;
         const float den = expf((diff + (-cc[4])) * (-cc[3])) + 1.0f;
-        if (classic) out[0] = cpu ? cc[1] + cc[2] / den : (1.0f / den) * cc[2] + cc[1];
-        else out[0] = cpu ? cc[1] / den : (1.0f / den) * cc[1];
+#if HB_EXACT
+        const float inv = 1.0f / den;
+#else
+        const float inv = __builtin_amdgcn_rcpf(den);   // FMA builds: the 1-ulp reciprocal (den >= 1)
+#endif
+        if (classic) out[0] = cpu ? cc[1] + cc[2] / den : inv * cc[2] + cc[1];
+        else out[0] = cpu ? cc[1] / den : inv * cc[1];
     } else if (cfn == HB_CF_PRESIGMOIDAL) {
 #pragma unroll
         for (int c = 0; c < NP; ++c) out[c] = (tanhf((x[c] * cc[3] + (-cc[4])) * cc[2]) + cc[1]) * cc[0];
